@@ -1,0 +1,470 @@
+"""Boussinesq Chebyshev CNAB2 dynamo step (oracle restatement, FP64 numpy).
+
+One :class:`OracleRun` holds one configuration's state (the reference keeps
+it in module globals: ``fields.py:14-67``, ``dt_fields.py:55-123``).
+
+* ``initialize_fields`` — ``init_fields.py:470-532`` with ``initS``
+  (:379-406) and insulating ``initB`` (:409-467); optional seeded noise
+  (``add_noise``) is this build's extension, applied between
+  ``initialize_fields`` and ``setup_initial_state``.
+* ``setup_initial_state`` — ``step_time.py:398-617`` (Boussinesq branch).
+* ``radial_loop`` — ``step_time.py:931-1212``: inverse SHT, ``get_nl``
+  (``get_nl.py:30-87``), forward SHT, ``get_td`` (``get_td.py:58-308``),
+  ``finish_exp_entropy`` (``update_s.py:191-210``), ``finish_exp_mag``
+  (``update_b.py:514-527``) and ``courant_check`` (``courant.py:80-127``).
+* ``lm_loop`` — ``_fused_lm_loop_fast`` (``step_time.py:222-394``) and
+  ``_updateB_insulating`` (``update_b.py:557-631``).
+* ``one_step`` — ``_one_step_cnab2`` (``step_time.py:1602-1637``) with
+  ``dt_courant`` (``courant.py:130-155``) and ``CNAB2.set_weights``
+  (``time_scheme.py:56-73``).
+* energies — ``get_e_kin_full``/``get_e_mag_oc_full`` (``output.py:111-193``).
+
+Pinned by ``tests/test_oracle_step.py`` against the committed fixtures in
+``tests/golden/`` written by ``tests/golden/make_golden.py`` from the
+reference itself at l_max=16 and l_max=32 (the composed trajectory is the
+part only the reference can pin), plus the self-contained invariants listed
+there (conduction state is a steady solution of the s/p equations, m=0
+realness, boundary conditions).
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from .params import Params
+from . import horizontal, radial, matrices
+from .sht import SHT
+
+
+class OracleRun:
+    def __init__(self, cfg: dict):
+        p = self.prm = Params(dict(cfg))
+        if p.l_anel or p.l_finite_diff or p.l_cond_ic or p.raxi != 0.0 or p.nRotIC != 0:
+            raise NotImplementedError("oracle step: Boussinesq Chebyshev insulating-IC only")
+        if p.time_scheme != "CNAB2":
+            raise NotImplementedError("oracle step: CNAB2 only")
+        self.maps = horizontal.lm_maps(p)
+        self.grid = horizontal.grid_theta(p)
+        self.lmarr = horizontal.lm_arrays(p, self.maps)
+        self.sht = SHT(p, self.grid, maps=self.maps)
+        self.rad = radial.radial_functions(p)
+        self.mats = radial.cheb_mats(p, self.rad["x"], self.rad["r_cmb"], self.rad["r_icb"])
+        self.D1, self.D2, self.D3 = radial.diff_mats(p, self.mats)
+        self.q = radial.rint_weights(p, self.rad["r_cmb"], self.rad["r_icb"])
+        L, N = p.lm_max, p.n_r_max
+        z = lambda: np.zeros((L, N), np.complex128)
+        for name in ("w", "dw", "ddw", "z", "dz", "p", "dp", "s", "ds",
+                     "b", "db", "ddb", "aj", "dj", "ddj"):
+            setattr(self, name, z())
+        self.td = {k: dict(old=z(), impl=z(), expl=[z(), z()])
+                   for k in ("s", "z", "w", "p", "b", "j")}
+        self.dt_hist = np.zeros(2)
+        self.dtMax = 0.0
+        self.inv = None
+        self.m0 = self.maps["lm2m"] == 0
+        self.lm_l0 = int(self.maps["lm2"][0, 0])
+
+    # ------------------------------------------------------------ init
+    def initialize_fields(self):
+        """``init_fields.py:470-532`` (heat + insulating magnetic field)."""
+        p, rad = self.prm, self.rad
+        for name in ("w", "dw", "ddw", "z", "dz", "p", "dp", "s", "ds",
+                     "b", "db", "ddb", "aj", "dj", "ddj"):
+            getattr(self, name)[:] = 0.0
+        if p.l_heat:
+            s0, p0 = matrices.ps_cond(p, rad, self.mats)
+            self.s[0] = s0
+            self.p[0] = p0
+            r, r_cmb, r_icb = rad["r"], rad["r_cmb"], rad["r_icb"]
+            x = 2.0 * r - r_cmb - r_icb
+            s1 = 1.0 - 3.0 * x ** 2 + 3.0 * x ** 4 - x ** 6
+            if p.init_s1 >= 100:
+                l, m = p.init_s1 // 100, p.init_s1 % 100
+                self.s[self.maps["lm2"][l, m]] += p.amp_s1 * s1
+        if p.l_mag:
+            r, r_cmb, r_icb = rad["r"], rad["r_cmb"], rad["r_icb"]
+            l1m0 = self.maps["lm2"][1, 0]
+            l2m0 = self.maps["lm2"][2, 0]
+            b_tor = -4.0 / 3.0 * p.amp_b1 * math.sqrt(math.pi / 5.0)
+            b_pol = p.amp_b1 * math.sqrt(3.0 * math.pi) / 4.0
+            self.b[l1m0] += b_pol * (r ** 3 - 4.0 / 3.0 * r_cmb * r ** 2 + r_icb ** 4 / 3.0 / r)
+            self.aj[l2m0] += b_tor * r * np.sin(math.pi * (r - r_icb))
+
+    def add_noise(self, seed: int, amp: float):
+        """Seeded Gaussian noise on s for l>=1 (build extension, see module doc)."""
+        noise = seeded_noise(self.prm, self.maps, self.rad, seed, amp)
+        self.s += noise
+
+    # ------------------------------------------------------------ helpers
+    def _dr(self, f, D):
+        return f @ D.T
+
+    def setup_initial_state(self):
+        """``step_time.py:398-617`` (Boussinesq, not double-curl)."""
+        p, rad = self.prm, self.rad
+        dLh = self.lmarr["dLh"][:, None]
+        or1, or2, or3 = rad["or1"][None], rad["or2"][None], rad["or3"][None]
+        for td in self.td.values():
+            td["old"][:] = 0; td["impl"][:] = 0
+            td["expl"][0][:] = 0; td["expl"][1][:] = 0
+        ds, d2s = self._dr(self.s, self.D1), self._dr(self.s, self.D2)
+        self.ds[:] = ds
+        self.td["s"]["old"][:] = self.s
+        self.td["s"]["impl"][:] = p.opr * (d2s + 2.0 * or1 * ds - dLh * or2 * self.s)
+        dz, d2z = self._dr(self.z, self.D1), self._dr(self.z, self.D2)
+        self.dz[:] = dz
+        self.td["z"]["old"][:] = dLh * or2 * self.z
+        self.td["z"]["impl"][:] = dLh * or2 * (d2z - dLh * or2 * self.z)
+        dw, ddw, dddw = (self._dr(self.w, D) for D in (self.D1, self.D2, self.D3))
+        self.dw[:] = dw; self.ddw[:] = ddw
+        dp = self._dr(self.p, self.D1)
+        self.dp[:] = dp
+        self.td["w"]["old"][:] = dLh * or2 * self.w
+        self.td["p"]["old"][:] = -dLh * or2 * dw
+        self._w_p_impl(self.w, dw, ddw, dddw, self.p, dp, self.s)
+        if p.l_mag:
+            for f, df, ddf, key in ((self.b, self.db, self.ddb, "b"), (self.aj, self.dj, self.ddj, "j")):
+                df[:] = self._dr(f, self.D1)
+                ddf[:] = self._dr(f, self.D2)
+                self.td[key]["old"][:] = dLh * or2 * f
+                impl = self.td[key]["impl"]
+                impl[:] = 0
+                impl[:, 1:-1] = (p.opm * dLh * or2[:, 1:-1]
+                                 * (ddf[:, 1:-1] - dLh * or2[:, 1:-1] * f[:, 1:-1]))
+
+    def _w_p_impl(self, w, dw, ddw, dddw, pf, dp, s):
+        p, rad = self.prm, self.rad
+        dLh = self.lmarr["dLh"][:, None]
+        or2, or3 = rad["or2"][None], rad["or3"][None]
+        rgrav = rad["rgrav"][None]
+        wi = self.td["w"]["impl"]; pi_ = self.td["p"]["impl"]
+        wi[:] = -dp + dLh * or2 * (ddw - dLh * or2 * w) + p.BuoFac * rgrav * s
+        pi_[:] = dLh * or2 * pf + dLh * or2 * (-dddw + dLh * or2 * dw - 2.0 * dLh * or3 * w)
+        wi[self.lm_l0] = 0; pi_[self.lm_l0] = 0
+        wi[:, 0] = 0; wi[:, -1] = 0; pi_[:, 0] = 0; pi_[:, -1] = 0
+
+    def initialize_dt(self, dt: float):
+        """``step_time.py:1554-1571`` (intfac clamp for rotating runs)."""
+        if not self.prm.ek < 0.0:
+            dt = min(dt, self.prm.cnab2_intfac * self.prm.ek)
+        self.dt_hist[:] = dt
+        self.dtMax = dt
+
+    # ------------------------------------------------------------ r-loop
+    def radial_loop(self):
+        """``step_time.py:931-1212``; returns (dtrkc_min, dthkc_min)."""
+        p, rad, sht = self.prm, self.rad, self.sht
+        N = p.n_r_max
+        dLh = self.lmarr["dLh"][:, None]
+        or2 = rad["or2"][None]
+        sc = sht.scal_to_spat(self.s)
+        Q = np.concatenate([dLh * self.b, dLh * self.aj, dLh * self.w, dLh * self.z], axis=1)
+        S = np.concatenate([self.db, self.dj, self.dw, self.dz], axis=1)
+        T = np.concatenate([self.aj, or2 * dLh * self.b - self.ddb, self.z,
+                            or2 * dLh * self.w - self.ddw], axis=1)
+        ar, at, ap = sht.torpol_to_spat(Q, S, T)
+        br, bt, bp = ar[:N], at[:N], ap[:N]
+        cbr, cbt, cbp = ar[N:2 * N], at[N:2 * N], ap[N:2 * N]
+        vr, vt, vp = ar[2 * N:3 * N], at[2 * N:3 * N], ap[2 * N:3 * N]
+        cvr, cvt, cvp = ar[3 * N:], at[3 * N:], ap[3 * N:]
+        nl = get_nl(p, rad, self.grid, vr, vt, vp, cvr, cvt, cvp, sc, br, bt, bp, cbr, cbt, cbp)
+        Advr, Advt, Advp, VSr, VSt, VSp, VxBr, VxBt, VxBp = nl
+        bulk = slice(1, N - 1)
+        scal_out = sht.scal_to_SH(np.concatenate([Advr[bulk], VSr[bulk], VxBr[bulk]], axis=0))
+        S_out, T_out = sht.spat_to_sphertor(np.concatenate([Advt[bulk], VSt[bulk], VxBt[bulk]]),
+                                            np.concatenate([Advp[bulk], VSp[bulk], VxBp[bulk]]))
+        Nb = N - 2
+        L = p.lm_max
+        z = lambda: np.zeros((L, N), np.complex128)
+        AdvrLM, AdvtLM, AdvpLM, dVSrLM, VStLM, VxBrLM, VxBtLM, VxBpLM = (z() for _ in range(8))
+        AdvrLM[:, bulk] = scal_out[:, :Nb]
+        dVSrLM[:, bulk] = scal_out[:, Nb:2 * Nb]
+        VxBrLM[:, bulk] = scal_out[:, 2 * Nb:]
+        AdvtLM[:, bulk] = S_out[:, :Nb]
+        AdvpLM[:, bulk] = T_out[:, :Nb]
+        VStLM[:, bulk] = S_out[:, Nb:2 * Nb]
+        VxBtLM[:, bulk] = S_out[:, 2 * Nb:]
+        VxBpLM[:, bulk] = T_out[:, 2 * Nb:]
+        self.last_grid = dict(vr=vr, vt=vt, vp=vp, br=br, bt=bt, bp=bp, sc=sc)
+        self.last_spec = dict(AdvrLM=AdvrLM, AdvtLM=AdvtLM, AdvpLM=AdvpLM, dVSrLM=dVSrLM,
+                              VStLM=VStLM, VxBrLM=VxBrLM, VxBtLM=VxBtLM, VxBpLM=VxBpLM)
+        td = get_td(self, AdvrLM, AdvtLM, AdvpLM, dVSrLM, VStLM, VxBrLM, VxBtLM, VxBpLM)
+        for k, v in td.items():
+            self.td[k]["expl"][0][:] = v
+        dtrkc, dthkc = courant_check(p, rad, self.grid, vr, vt, vp, br, bt, bp)
+        return float(dtrkc.min()), float(dthkc.min())
+
+    # ------------------------------------------------------------ lm-loop
+    def _weights(self):
+        dt1, dt2 = self.dt_hist
+        a = self.prm.alpha
+        self.wimp = 1.0
+        self.wimp_lin = [a * dt1, (1.0 - a) * dt1]
+        self.wexp = [(1.0 + 0.5 * dt1 / dt2) * dt1, -0.5 * dt1 * dt1 / dt2]
+
+    def _rhs(self, key):
+        td = self.td[key]
+        return (self.wimp * td["old"] + self.wimp_lin[1] * td["impl"]
+                + self.wexp[0] * td["expl"][0] + self.wexp[1] * td["expl"][1])
+
+    def build_all_matrices(self):
+        """``step_time.py:1530-1548`` (Boussinesq Chebyshev insulating)."""
+        w0 = self.wimp_lin[0]
+        p, rad, mats = self.prm, self.rad, self.mats
+        self.inv = dict(s=matrices.build_s(p, rad, mats, w0), z=matrices.build_z(p, rad, mats, w0),
+                        wp=matrices.build_wp(p, rad, mats, w0), p0=matrices.build_p0(p, rad, mats))
+        if p.l_mag:
+            self.inv["b"], self.inv["j"] = matrices.build_bj(p, rad, mats, w0)
+
+    def _solve(self, inv_by_l, rhs):
+        l = self.maps["lm2l"]
+        return np.einsum("lij,lj->li", inv_by_l[l], rhs)
+
+    def lm_loop(self):
+        """``_fused_lm_loop_fast`` (``step_time.py:222-394``) + ``_updateB_insulating``."""
+        p, rad = self.prm, self.rad
+        N, L = p.n_r_max, p.lm_max
+        nc = p.n_cheb_max
+        dLh = self.lmarr["dLh"][:, None]
+        or1, or2, or3 = rad["or1"][None], rad["or2"][None], rad["or3"][None]
+        rgrav = rad["rgrav"]
+        rhs_s, rhs_z, rhs_w, rhs_p = (self._rhs(k) for k in ("s", "z", "w", "p"))
+        rhs_s[:, 0] = 0.0
+        rhs_s[:, N - 1] = 0.0
+        rhs_s[self.lm_l0, N - 1] = math.sqrt(4.0 * math.pi)
+        rhs_z[:, 0] = 0.0; rhs_z[:, N - 1] = 0.0
+        s_cheb = self._solve(self.inv["s"], rhs_s)
+        z_cheb = self._solve(self.inv["z"], rhs_z)
+        for c in (s_cheb, z_cheb):
+            c[self.m0] = c[self.m0].real
+            c[:, nc:] = 0.0
+        self.s[:] = radial.costf(s_cheb)
+        rc = np.zeros((L, 2 * N), np.complex128)
+        rc[:, 1:N - 1] = rhs_w[:, 1:N - 1] + self.wimp_lin[0] * p.BuoFac * rgrav[None, 1:N - 1] * self.s[:, 1:N - 1]
+        rc[:, N + 1:2 * N - 1] = rhs_p[:, 1:N - 1]
+        p0_rhs = np.zeros(N)
+        l0 = self.lm_l0
+        p0_rhs[1:] = p.BuoFac * rgrav[1:] * self.s[l0, 1:].real + self.td["w"]["expl"][0][l0, 1:].real
+        sol = self._solve(self.inv["wp"], rc)
+        sol[self.m0] = sol[self.m0].real
+        w_cheb = sol[:, :N].copy()
+        p_cheb = sol[:, N:].copy()
+        p_cheb[l0] = self.inv["p0"] @ p0_rhs
+        w_cheb[:, nc:] = 0.0
+        p_cheb[:, nc:] = 0.0
+        self.z[:] = radial.costf(z_cheb)
+        self.w[:] = radial.costf(w_cheb)
+        self.p[:] = radial.costf(p_cheb)
+        D1, D2, D3 = self.D1, self.D2, self.D3
+        self.ds[:] = self._dr(self.s, D1); d2s = self._dr(self.s, D2)
+        self.dz[:] = self._dr(self.z, D1); d2z = self._dr(self.z, D2)
+        self.dw[:] = self._dr(self.w, D1); self.ddw[:] = self._dr(self.w, D2)
+        dddw = self._dr(self.w, D3)
+        self.dp[:] = self._dr(self.p, D1)
+        for k in ("s", "z", "w", "p"):
+            self.td[k]["expl"][1][:] = self.td[k]["expl"][0]
+        self.td["s"]["old"][:] = self.s
+        self.td["z"]["old"][:] = dLh * or2 * self.z
+        self.td["w"]["old"][:] = dLh * or2 * self.w
+        self.td["p"]["old"][:] = -dLh * or2 * self.dw
+        self.td["s"]["impl"][:] = p.opr * (d2s + 2.0 * or1 * self.ds - dLh * or2 * self.s)
+        self.td["z"]["impl"][:] = dLh * or2 * d2z - (dLh * or2) ** 2 * self.z
+        self._w_p_impl(self.w, self.dw, self.ddw, dddw, self.p, self.dp, self.s)
+        if p.l_mag:
+            self._update_b()
+
+    def _update_b(self):
+        """``update_b.py:557-631`` (insulating)."""
+        p, rad = self.prm, self.rad
+        N = p.n_r_max
+        dLh = self.lmarr["dLh"][:, None]
+        or2 = rad["or2"][None]
+        rb, rj = self._rhs("b"), self._rhs("j")
+        for r_ in (rb, rj):
+            r_[:, 0] = 0.0; r_[:, N - 1] = 0.0
+        bc = self._solve(self.inv["b"], rb)
+        jc = self._solve(self.inv["j"], rj)
+        for c in (bc, jc):
+            c[self.m0] = c[self.m0].real
+            c[:, p.n_cheb_max:] = 0.0
+        self.b[:] = radial.costf(bc)
+        self.aj[:] = radial.costf(jc)
+        self.db[:] = self._dr(self.b, self.D1); self.ddb[:] = self._dr(self.b, self.D2)
+        self.dj[:] = self._dr(self.aj, self.D1); self.ddj[:] = self._dr(self.aj, self.D2)
+        for k in ("b", "j"):
+            self.td[k]["expl"][1][:] = self.td[k]["expl"][0]
+        self.td["b"]["old"][:] = dLh * or2 * self.b
+        self.td["j"]["old"][:] = dLh * or2 * self.aj
+        self.td["b"]["impl"][:] = p.opm * dLh * or2 * (self.ddb - dLh * or2 * self.b)
+        self.td["j"]["impl"][:] = p.opm * dLh * or2 * (self.ddj - dLh * or2 * self.aj)
+
+    # ------------------------------------------------------------ step
+    def one_step(self, n_time_step: int, dt: float) -> float:
+        """``_one_step_cnab2`` (``step_time.py:1602-1637``)."""
+        dtrkc, dthkc = self.radial_loop()
+        l_new_dt, dt_new = dt_courant(dt, self.dtMax, dtrkc, dthkc)
+        dt_old = self.dt_hist[0]
+        self.dt_hist[0] = dt_new
+        self.dt_hist[1] = dt_old
+        self._weights()
+        if n_time_step == 1 or l_new_dt:
+            self.build_all_matrices()
+        self.lm_loop()
+        return dt_new
+
+    # ------------------------------------------------------------ energies
+    def _cc2real(self, c):
+        m0 = self.m0[:, None]
+        return np.where(m0, c.real ** 2, 2.0 * (c.real ** 2 + c.imag ** 2))
+
+    def energies(self):
+        """(e_kin_pol, e_kin_tor, e_mag_pol, e_mag_tor) (``output.py:111-193``)."""
+        dLh = self.lmarr["dLh"][:, None]
+        or2 = self.rad["or2"][None]
+        q = self.q
+        ekp = 0.5 * dLh * (dLh * or2 * self._cc2real(self.w) + self._cc2real(self.dw))
+        ekt = 0.5 * dLh * self._cc2real(self.z)
+        f = 0.5 * self.prm.LFfac
+        emp = f * dLh * (dLh * or2 * self._cc2real(self.b) + self._cc2real(self.db))
+        emt = f * dLh * self._cc2real(self.aj)
+        return tuple(float(e.sum(axis=0) @ q) for e in (ekp, ekt, emp, emt))
+
+    def spectra(self):
+        """Per-l and per-m kinetic/magnetic energy spectra (``output.py:750-861`` aggregation)."""
+        dLh = self.lmarr["dLh"][:, None]
+        or2 = self.rad["or2"][None]
+        ek = 0.5 * dLh * (dLh * or2 * self._cc2real(self.w) + self._cc2real(self.dw)
+                          + self._cc2real(self.z))
+        em = 0.5 * self.prm.LFfac * dLh * (dLh * or2 * self._cc2real(self.b) + self._cc2real(self.db)
+                                         + self._cc2real(self.aj))
+        l = self.maps["lm2l"]; m = self.maps["lm2m"]
+        out = {}
+        for name, e in (("kin", ek), ("mag", em)):
+            per_lm = e @ self.q
+            out[name + "_l"] = np.bincount(l, weights=per_lm, minlength=self.prm.l_max + 1)
+            out[name + "_m"] = np.bincount(m, weights=per_lm, minlength=self.prm.l_max + 1)
+        return out
+
+
+# ---------------------------------------------------------------- kernels
+def seeded_noise(p: Params, maps, rad, seed: int, amp: float) -> np.ndarray:
+    """amp·N(0,1) (complex) for l>=1 with m=0 imag=0, shaped by (1-x²)³ in r.
+
+    Drawn on the CPU in FP64 from ``numpy.random.default_rng(seed)`` in the
+    m-major lm order, so the same (seed, l_max, n_r_max) gives the same field
+    in the oracle and in the HIP path.
+    """
+    rng = np.random.default_rng(seed)
+    L = p.lm_max
+    re = rng.standard_normal(L)
+    im = rng.standard_normal(L)
+    coef = amp * (re + 1j * im)
+    coef[maps["lm2m"] == 0] = coef[maps["lm2m"] == 0].real
+    coef[maps["lm2l"] == 0] = 0.0
+    r = rad["r"]
+    x = 2.0 * r - rad["r_cmb"] - rad["r_icb"]
+    shape = (1.0 - x ** 2) ** 3
+    return coef[:, None] * shape[None, :]
+
+
+def get_nl(p, rad, grid, vr, vt, vp, cvr, cvt, cvp, sc, br, bt, bp, cbr, cbt, cbp):
+    """Grid-space products (``get_nl.py:30-87``), Boussinesq, no composition."""
+    or4 = rad["or4"][:, None, None]
+    or2 = rad["or2"][:, None, None]
+    ost2 = grid["O_sin_theta_E2_grid"][None, :, None]
+    LF = p.LFfac
+    Advr = -ost2 * (cvt * vp - cvp * vt) + LF * ost2 * (cbt * bp - cbp * bt)
+    Advt = -or4 * (cvp * vr - cvr * vp) + LF * or4 * (cbp * br - cbr * bp)
+    Advp = -or4 * (cvr * vt - cvt * vr) + LF * or4 * (cbr * bt - cbt * br)
+    VSr = vr * sc
+    VSt = or2 * vt * sc
+    VSp = or2 * vp * sc
+    VxBr = ost2 * (vt * bp - vp * bt)
+    VxBt = or4 * (vp * br - vr * bp)
+    VxBp = or4 * (vr * bt - vt * br)
+    return Advr, Advt, Advp, VSr, VSt, VSp, VxBr, VxBt, VxBp
+
+
+def get_td(run: OracleRun, AdvrLM, AdvtLM, AdvpLM, dVSrLM, VStLM, VxBrLM, VxBtLM, VxBpLM):
+    """Explicit terms (``get_td.py:58-308``) + finish_exp of S and B."""
+    p, rad, A = run.prm, run.rad, run.lmarr
+    maps = run.maps
+    CorFac = p.CorFac
+    or1, or2, or4 = rad["or1"][None], rad["or2"][None], rad["or4"][None]
+    r2 = (rad["r"] ** 2)[None]
+    dLh = A["dLh"][:, None]
+    dPhi = A["dPhi"][:, None]
+    lmA = np.clip(maps["lm2lmA"], 0, None)
+    maskA = (maps["lm2lmA"] >= 0)[:, None].astype(np.float64)
+    lmS = maps["lm2lmS"]
+    gA = lambda f: f[lmA] * maskA
+    gS = lambda f: f[lmS]
+    w, dw, z = run.w, run.dw, run.z
+    c = {k: A[k][:, None] for k in ("dTheta2A", "dTheta2S", "dTheta3A", "dTheta3S",
+                                     "dTheta4A", "dTheta4S")}
+    dwdt = or2 * AdvrLM + 2.0 * CorFac * or1 * (dPhi * dw + c["dTheta2A"] * gA(z) - c["dTheta2S"] * gS(z))
+    dzdt = dLh * AdvpLM + 2.0 * CorFac * or2 * (
+        dPhi * z + c["dTheta3A"] * gA(dw) + or1 * c["dTheta4A"] * gA(w)
+        + c["dTheta3S"] * gS(dw) - or1 * c["dTheta4S"] * gS(w))
+    dpdt = -dLh * AdvtLM + 2.0 * CorFac * or2 * (
+        -dPhi * (dw + or1 * dLh * w) + c["dTheta3A"] * gA(z) + c["dTheta3S"] * gS(z))
+    dsdt = dLh * VStLM
+    dVSr = dVSrLM.copy()
+    dbdt = dLh * VxBpLM
+    djdt = dLh * or4 * VxBrLM
+    dVxBh = -dLh * VxBtLM * r2
+    for f in (dwdt, dzdt, dpdt, dsdt, dVSr, dbdt, djdt):
+        f[:, 0] = 0.0
+        f[:, -1] = 0.0
+    dVxBh[0, 0] = 0.0
+    dVxBh[0, -1] = 0.0
+    dsdt = dsdt - or2 * (dVSr @ run.D1.T)
+    djdt = djdt + or2 * (dVxBh @ run.D1.T)
+    return dict(w=dwdt, z=dzdt, p=dpdt, s=dsdt, b=dbdt, j=djdt)
+
+
+def courant_check(p, rad, grid, vr, vt, vp, br, bt, bp):
+    """Per-level Courant limits (``courant.py:25-127``, damped Alfvén branch)."""
+    or4 = rad["or4"][:, None, None]
+    or2 = rad["or2"][:, None, None]
+    ost2 = grid["O_sin_theta_E2_grid"][None, :, None]
+    cf2 = p.cnab2_courfac ** 2
+    vflr2 = vr * vr
+    vflh2 = (vt * vt + vp * vp) * ost2
+    if p.l_mag:
+        af2 = p.cnab2_alffac ** 2
+        valr = br * br * p.LFfac
+        valh2 = (bt * bt + bp * bp) * p.LFfac * ost2
+        if p.l_cour_alf_damp:
+            valri2 = ((0.5 * (1.0 + p.opm)) ** 2 / rad["delxr2"])[:, None, None]
+            valhi2 = ((0.5 * (1.0 + p.opm)) ** 2 / rad["delxh2"])[:, None, None]
+            valr2 = valr * valr / (valr + valri2)
+            valh2m = valh2 * valh2 / (valh2 + valhi2)
+        else:
+            valr2, valh2m = valr, valh2
+        vr2max = (or4 * (cf2 * vflr2 + af2 * valr2)).max(axis=(1, 2))
+        vh2max = (or2 * (cf2 * vflh2 + af2 * valh2m)).max(axis=(1, 2))
+    else:
+        vr2max = (cf2 * or4 * vflr2).max(axis=(1, 2))
+        vh2max = (cf2 * or2 * vflh2).max(axis=(1, 2))
+    big = 1e10
+    with np.errstate(divide="ignore", invalid="ignore"):
+        dtrkc = np.where(vr2max > 0, np.sqrt(rad["delxr2"] / vr2max), big)
+        dthkc = np.where(vh2max > 0, np.sqrt(rad["delxh2"] / vh2max), big)
+    return dtrkc, dthkc
+
+
+def dt_courant(dt, dtMax, dtrkc_min, dthkc_min):
+    """Hysteresis dt decision (``courant.py:130-155``)."""
+    dt_fac = 2.0
+    dtMin = min(dtrkc_min, dthkc_min)
+    dt_2 = min(0.5 * (1.0 / dt_fac + 1.0) * dtMin, dtMax)
+    if dt > dtMax:
+        return True, dtMax
+    if dt > dtMin:
+        return True, dt_2
+    if dt_fac * dt < dtMin and dt < dtMax:
+        return True, dt_2
+    return False, dt

@@ -1,0 +1,118 @@
+"""Radial discretisation (oracle): Chebyshev grid, DCT-I, derivative matrices.
+
+Restates ``chebyshev.py:20-107`` (Chebyshev-Gauss-Lobatto grid on
+[r_icb, r_cmb], rMat..d3rMat by the T_n recursion with the linear-map chain
+rule), ``cosine_transform.py:18-45`` (self-inverse DCT-I, scaled 1/sqrt(2N)),
+``radial_derivatives.py:71-106`` (D_k = d^k rMat · rMat^-1 with n_cheb_max
+truncation), ``integration.py:18-72`` (Clenshaw-Curtis rInt_R) and the
+Boussinesq part of ``radial_functions.py:18-69,189-202``.
+
+Pinned by ``tests/test_oracle_radial.py``: rMat against numpy's Chebyshev
+T_n, D1..D3 exact on polynomials of degree < N, costf against
+``scipy.fft.dct(type=1)`` times 1/sqrt(2N) and costf∘costf = I, rInt_R exact
+on polynomials.
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from .params import Params
+
+
+def cheb_grid(p: Params):
+    """Grid r[0]=r_cmb .. r[N-1]=r_icb (``chebyshev.py:20-45``)."""
+    r_cmb = 1.0 / (1.0 - p.radratio)
+    r_icb = r_cmb - 1.0
+    n = p.n_r_max - 1
+    x = np.cos(math.pi * np.arange(n + 1) / n)
+    r = 0.5 * (r_cmb - r_icb) * x + 0.5 * (r_icb + r_cmb)
+    return r, x, r_cmb, r_icb
+
+
+def cheb_mats(p: Params, x=None, r_cmb=None, r_icb=None):
+    """rMat, drMat, d2rMat, d3rMat (``chebyshev.py:53-107``)."""
+    if x is None:
+        _, x, r_cmb, r_icb = cheb_grid(p)
+    n = p.n_r_max
+    drx = 2.0 / (r_cmb - r_icb)
+    R = np.zeros((n, n)); D = np.zeros((n, n)); D2 = np.zeros((n, n)); D3 = np.zeros((n, n))
+    R[:, 0] = 1.0
+    R[:, 1] = x
+    D[:, 1] = drx
+    for c in range(2, n):
+        R[:, c] = 2 * x * R[:, c - 1] - R[:, c - 2]
+        D[:, c] = 2 * drx * R[:, c - 1] + 2 * x * D[:, c - 1] - D[:, c - 2]
+        D2[:, c] = 4 * drx * D[:, c - 1] + 2 * x * D2[:, c - 1] - D2[:, c - 2]
+        D3[:, c] = 6 * drx * D2[:, c - 1] + 2 * x * D3[:, c - 1] - D3[:, c - 2]
+    return R, D, D2, D3
+
+
+def costf(f: np.ndarray) -> np.ndarray:
+    """Self-inverse DCT-I along the last axis (``cosine_transform.py:18-45``)."""
+    n = f.shape[-1]
+    N = n - 1
+    ext = np.concatenate([f, f[..., 1:-1][..., ::-1]], axis=-1)
+    F = np.fft.fft(ext, axis=-1)[..., :n]
+    if not np.iscomplexobj(f):
+        F = F.real
+    return F / math.sqrt(2.0 * N)
+
+
+def costf_matrix(n: int) -> np.ndarray:
+    """Dense matrix C with costf(f) = f @ C.T for real f (same transform)."""
+    return costf(np.eye(n)).T
+
+
+def diff_mats(p: Params, mats=None):
+    """D1, D2, D3 acting on grid values (``radial_derivatives.py:71-106``)."""
+    R, D, D2, D3 = mats if mats is not None else cheb_mats(p)
+    Rinv = np.linalg.inv(R)
+    if p.n_cheb_max < p.n_r_max:
+        Rinv = Rinv.copy()
+        Rinv[p.n_cheb_max:, :] = 0.0
+    return D @ Rinv, D2 @ Rinv, D3 @ Rinv
+
+
+def cheb_int_weights(n: int) -> np.ndarray:
+    """Chebyshev integration weights (``integration.py:18-38``)."""
+    w = np.zeros(n)
+    w[0] = 1.0
+    for k in range(2, n, 2):
+        k1 = k + 1
+        w[k] = -1.0 / (k1 * (k1 - 2))
+    return w
+
+
+def rint_weights(p: Params, r_cmb: float, r_icb: float) -> np.ndarray:
+    """Vector q with rInt_R(f) = f @ q (``integration.py:45-72`` is linear in f)."""
+    n = p.n_r_max
+    drx = 2.0 / (r_cmb - r_icb)
+    rnorm = math.sqrt(2.0 / (n - 1))
+    w = cheb_int_weights(n)
+    C = costf_matrix(n)             # a = C @ g
+    bf = np.ones(n); bf[0] = 0.5; bf[-1] = 0.5
+    # result = 2 rnorm sum_k w_k bf_k (C @ (f/drx))_k
+    return 2.0 * rnorm * (C.T @ (w * bf)) / drx
+
+
+def rint(f: np.ndarray, q: np.ndarray):
+    """Clenshaw-Curtis integral over r of f(..., N) (``integration.py:45-72``)."""
+    return f @ q
+
+
+def radial_functions(p: Params):
+    """Boussinesq radial profiles and CFL spacings (``radial_functions.py:18-69,189-202``)."""
+    r, x, r_cmb, r_icb = cheb_grid(p)
+    or1 = 1.0 / r
+    dr = r[:-1] - r[1:]
+    delxr2 = np.empty(p.n_r_max)
+    delxr2[0] = dr[0] ** 2
+    delxr2[-1] = dr[-1] ** 2
+    delxr2[1:-1] = np.minimum(dr[:-1], dr[1:]) ** 2
+    return dict(r=r, x=x, r_cmb=r_cmb, r_icb=r_icb, or1=or1, or2=or1 ** 2, or3=or1 ** 3,
+                or4=or1 ** 4, rgrav=r / r_cmb,
+                delxh2=r ** 2 / float(p.l_max * (p.l_max + 1)), delxr2=delxr2,
+                rnorm=math.sqrt(2.0 / (p.n_r_max - 1)))
