@@ -1,0 +1,295 @@
+/*
+ * magic_hip.h — C ABI of libmagic_hip.so, the MI355X (gfx950) kernels behind
+ * the pseudo-spectral dynamo time step.
+ *
+ * Plain C: device pointers are passed as `double*` / `int*` (complex arrays
+ * are interleaved re/im doubles), sizes as ints, the HIP stream as `void*`
+ * (a hipStream_t, NULL = default stream).  Every entry point is stream
+ * ordered, allocates nothing, never synchronises and returns 0 on success,
+ * -1 (MH_EINVAL) for invalid arguments and -2 (MH_ELAUNCH) for a launch
+ * failure.  Callers own every buffer.
+ *
+ * The reference (rvilim/torch-magic, src/magic_torch/) has no native layer:
+ * each entry point replaces torch-dispatched work behind a reference Python
+ * function, cited per declaration.  Layouts:
+ *   spectral fields   (L, N) complex, m-major lm order  (blocking.py:29-35)
+ *   Fourier buffers   F[c][m][t][b] complex, t = interleaved N/S theta row
+ *   grid              G[c][b][t][phi] real               (sht.py:298-348)
+ */
+#ifndef MAGIC_HIP_H
+#define MAGIC_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MH_OK 0
+#define MH_EINVAL -1
+#define MH_ELAUNCH -2
+
+/* ------------------------------------------------------------ Legendre */
+
+/* P_lm, sin(theta) dP_lm/dtheta and their Gauss-weighted copies on the
+ * northern nodes, written into the synthesis / analysis table layouts.
+ * Replaces plms.build_plm_matrices (plms.py:158-186) and the padded/bucketed
+ * SHT matrices of sht.py:84-289.  mp_info[(m*2+p)*4 + {0..3}] =
+ * {K, syn_off, ana_off, K16} for parity p = (l-m)%2. */
+int mh_plm_tables(int l_max, int n_m, int H, int Hp, const double* cos_t, const double* sin_t,
+                  const double* wgt, const int* mp_info, double* synP, double* synD,
+                  double* anaW, double* anaWd, void* stream);
+
+typedef struct {
+  const double* x1; /* (L, ldx) complex */
+  const double* x2; /* mode 2 only */
+  int mode;         /* 0: x1   1: l(l+1) x1   2: -x1 + or2[b] l(l+1) x2 */
+  int pad;
+} MhSynIn;
+
+typedef struct {
+  MhSynIn in[4][3]; /* [group][Q, S, T]; scalar synthesis uses in[g][0] */
+  int ngroups, B, ldx, H, Hp;
+  const int* mp_info;
+  const int* lm_start;
+  const double* synP;
+  const double* synD;
+  const double* or2;
+  double* F;
+  long c_stride, m_stride;
+  int Bout, pad;
+} MhSynArgs;
+
+/* Legendre synthesis on FP64 MFMA: scalar (vector=0) replaces the bmm part of
+ * scal_to_spat (sht.py:298-338); vector QST (vector=1) replaces the bmm part
+ * of torpol_to_spat (sht.py:408-482), one launch for up to 4 field groups. */
+int mh_leg_synth(int vector, const MhSynArgs* args, int n_m, void* stream);
+
+typedef struct {
+  const double* F;
+  long c_stride, m_stride;
+  int Bin, B, ncomp;
+  int cf1[3], cf2[3];
+  int pad0;
+  double* outS[3];
+  double* outT[3];
+  int ldo, H, Hp, pad1;
+  const int* mp_info;
+  const int* lm_start;
+  const double* anaW;
+  const double* anaWd;
+  const int* tiles; /* (m, p, k0) triples */
+  int ntiles, pad2;
+} MhAnaArgs;
+
+/* Legendre analysis on FP64 MFMA: scalar replaces the bmm part of scal_to_SH
+ * (sht.py:351-405); vector replaces spat_to_sphertor's products and the
+ * division by l(l+1) (sht.py:575-667). */
+int mh_leg_analysis(int vector, const MhAnaArgs* args, void* stream);
+
+/* ------------------------------------------------------------ FFT */
+typedef struct {
+  int Mh, nstage;
+  int radix[16];
+  int n_m, T, B, ncomp, RB, pad;
+  const double* tw;  /* exp(-2 pi i k/Mh), complex, k < Mh */
+  const double* tw2; /* exp(-2 pi i k/(2Mh)), complex, k < Mh */
+  double* F;
+  long Fc_stride, Fm_stride;
+  int FB, pad0;
+  double* G;
+  long Gc_stride;
+  const int* comp_scale; /* forward: nonzero -> multiply by ost2[t] */
+  const double* ost2;
+} MhFftArgs;
+
+/* Azimuthal real FFT, norm="forward".  forward=0 replaces torch.fft.irfft in
+ * sht.py:341,484-486; forward=1 replaces torch.fft.rfft (and the 1/sin^2
+ * scaling) in sht.py:371,597-603. */
+int mh_fft(int forward, const MhFftArgs* args, void* stream);
+
+/* ------------------------------------------------------------ grid space */
+typedef struct {
+  const double* Gs;
+  const double* Gv;
+  long Gv_c_stride;
+  double* Go;
+  long Go_c_stride;
+  int N, T, P, pad;
+  const double* or2;
+  const double* or4;
+  const double* ost2;
+  const double* valri2;
+  const double* valhi2;
+  double LFfac, cf2, af2;
+  int l_mag, alf_damp;
+  unsigned long long* cfl; /* 2N: per-level maxima (bit patterns of doubles) */
+} MhNlArgs;
+
+/* Nonlinear products and per-level Courant maxima: replaces get_nl
+ * (get_nl.py:30-87) and the reductions of courant_check (courant.py:25-127). */
+int mh_get_nl(const MhNlArgs* args, void* stream);
+
+/* ------------------------------------------------------------ row operators */
+typedef struct {
+  const double* X;
+  const int* rows;
+  const double* MT;
+  double* Y[6];
+  int ldx, ldy, K, R, Rpad, rseg, nrows, pad;
+} MhRowJob;
+
+/* Y[lm][r] = sum_k M_g[r][k] X[lm][k] over a tile list of (job, row0, col0):
+ * the per-degree implicit solves of all systems with the DCT-I and the radial
+ * derivatives folded in (chunked_solve_complex algebra.py:271-310, costf
+ * cosine_transform.py:18-45, get_dr/get_ddr/get_dddr radial_derivatives.py:
+ * 18-68, the D123 matmul step_time.py:315-341). */
+int mh_rowop(const MhRowJob* jobs, const int* tiles, int ntiles, void* stream);
+
+/* ------------------------------------------------------------ spectral side */
+typedef struct {
+  const int* l;
+  const int* m;
+  const double *dLh, *dTh2A, *dTh2S, *dTh3A, *dTh3S, *dTh4A, *dTh4S;
+  int l_max, L;
+} MhLmCoef;
+
+typedef struct {
+  const double *or1, *or2, *or3, *or4, *r2, *rgrav;
+  int N, pad;
+} MhRadCoef;
+
+typedef struct {
+  MhLmCoef lc;
+  MhRadCoef rc;
+  const double *Advr, *Advt, *Advp, *VSt, *VxBr, *VxBp;
+  const double *dVSr_dr, *dVxBt_dr;
+  const double *w, *dw, *z;
+  double *ew, *ez, *ep, *es, *eb, *ej;
+  double CorFac;
+  int l_mag, pad;
+} MhTdArgs;
+
+/* Explicit terms: get_dwdt, get_dzdt, get_dpdt, get_dsdt, get_dbdt
+ * (get_td.py:58-308) with finish_exp_entropy (update_s.py:191-210) and
+ * finish_exp_mag (update_b.py:514-527). */
+int mh_td(const MhTdArgs* args, void* stream);
+
+typedef struct {
+  const double *old, *impl, *expl0, *expl1;
+} MhImexSys;
+
+typedef struct {
+  MhImexSys sys[6]; /* s, z, w, p, b, j */
+  double *rs, *rz, *rb, *rj, *rwp;
+  const int* m;
+  int L, N;
+  double wimp, wl1, we0, we1, bots_l0;
+  int l_mag, pad;
+} MhRhsArgs;
+
+/* IMEX right-hand sides with boundary rows and m=0 realness: replaces
+ * _assemble_rhs_mega and the BC/WP-rhs assembly of _fused_lm_loop_fast
+ * (step_time.py:186-219,239-280) and set_imex_rhs (time_scheme.py:75-90). */
+int mh_rhs(const MhRhsArgs* args, void* stream);
+
+typedef struct {
+  double* rwp;
+  const double* s;
+  const double* ew0;
+  const double* rgrav;
+  const int* m;
+  int L, N;
+  double fac, BuoFac;
+  int lm_l0, pad;
+} MhBuoyArgs;
+
+/* Implicit buoyancy coupling and the l=0 pressure right-hand side
+ * (step_time.py:274-289). */
+int mh_buoy(const MhBuoyArgs* args, void* stream);
+
+typedef struct {
+  MhLmCoef lc;
+  MhRadCoef rc;
+  const double *s, *ds, *d2s, *z, *d2z, *w, *dw, *ddw, *dddw, *p, *dp;
+  const double *b, *ddb, *aj, *ddj;
+  double *old_s, *impl_s, *old_z, *impl_z, *old_w, *impl_w, *old_p, *impl_p;
+  double *old_b, *impl_b, *old_j, *impl_j;
+  double opr, opm, BuoFac;
+  int l_mag, lm_l0;
+} MhImplArgs;
+
+/* Stored old states and implicit terms (step_time.py:352-394,
+ * update_b.py:619-631, setup_initial_state step_time.py:398-617). */
+int mh_impl(const MhImplArgs* args, void* stream);
+
+typedef struct {
+  const double *w, *dw, *z, *b, *db, *aj;
+  const int* lm_start;
+  const double* or2;
+  const double* qw;
+  double* out;
+  int l_max, N, by_m, l_mag;
+  double LFfac;
+} MhEnergyArgs;
+
+/* Kinetic / magnetic poloidal and toroidal energy per degree (by_m=0) or per
+ * order (by_m=1), radially integrated: get_e_kin_full, get_e_mag_oc_full
+ * (output.py:111-193) and the l/m aggregation of get_dlm (output.py:750-861). */
+int mh_energy(const MhEnergyArgs* args, void* stream);
+
+/* ------------------------------------------------------------ rebuild */
+typedef struct {
+  const double *R, *D1, *D2, *D3, *or1, *or2;
+  double* out;
+  int N, l_max, n_cheb, ktopv, kbotv, kind; /* kind 0 S 1 Z 2 B 3 J 4 WP 5 p0 */
+  double rnorm, wimp, opr, opm;
+} MhBuildArgs;
+
+/* Per-degree implicit matrices: build_s_matrices (update_s.py:66-188),
+ * build_z_matrices (update_z.py:89-240), build_wp_matrices
+ * (update_wp.py:197-368), build_p0_matrix (update_wp.py:80-170),
+ * _build_b_matrices_insulating (update_b.py:98-298). */
+int mh_build_matrices(const MhBuildArgs* args, void* stream);
+
+/* Batched equilibrated Gauss-Jordan inverse (precond 0 none, 1 rows, 2 rows
+ * and columns), out = Fc inv(Fr A Fc) Fr; replaces the torch.linalg.inv calls
+ * of the matrix builders.  work: nbatch*2n*n doubles, scal: nbatch*2n. */
+int mh_gj_inverse(const double* A, double* out, double* work, double* scal, int n, int nbatch,
+                  int precond, int* info, void* stream);
+
+typedef struct {
+  const double* A;
+  const double* B;
+  double* C;
+  long sAr, sAc, sAb, sBr, sBc, sBb, sCr, sCc, sCb;
+  int M, Nn, K, batch;
+  double alpha;
+} MhGemmArgs;
+
+/* Batched strided FP64 GEMM C = alpha A B on MFMA. */
+int mh_gemm_f64(const MhGemmArgs* args, void* stream);
+
+/* ------------------------------------------------------------ banded */
+
+/* LINPACK dgbfa order pivoted band LU of a batch of (2kl+ku+1, n) band
+ * matrices in place (negated multipliers, 1-based pivots, ZERO_TOLERANCE
+ * 1e-15): prepare_band (algebra.py:371-457).  info[b] = 0 or first zero
+ * pivot (1-based). */
+int mh_band_factor(double* abd, int* pivot, int* info, int n, int kl, int ku, int nbatch,
+                   void* stream);
+
+/* dgbsl forward/back substitution for complex right-hand sides grouped by
+ * degree, with row (and column) preconditioning: solve_band_real
+ * (algebra.py:460-506) inside banded_solve_by_l (algebra.py:672-723).
+ * rhs/sol: (L, n) complex rows; rows of degree l use factor l. */
+int mh_band_solve(const double* abd, const int* pivot, const double* fac_row,
+                  const double* fac_col, const int* lm2l, const double* rhs, double* sol, int L,
+                  int n, int kl, int ku, void* stream);
+
+/* Sizes of the argument structs as compiled (binding checks). */
+int mh_struct_sizes(int* out, int n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

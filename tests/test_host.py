@@ -1,0 +1,138 @@
+"""CPU tests of the product's host-side set-up and of the C ABI library."""
+
+import ctypes as C
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+from magic_amd import grid
+from magic_amd.params import Params
+from oracle.params import Params as OP
+from oracle import horizontal, radial as oradial
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.mark.parametrize("l_max,n_phi,L", [(16, 48, 153), (32, 96, 561), (64, 192, 2145),
+                                           (85, 256, 3741), (127, 384, 8256), (383, 1152, 73920)])
+def test_truncation_sizes(l_max, n_phi, L):
+    p = Params({"l_max": l_max})
+    assert (p.n_phi, p.n_theta, p.lm_max) == (n_phi, n_phi // 2, L)
+
+
+def test_unknown_key_rejected():
+    with pytest.raises(KeyError):
+        Params({"l_maxx": 3})
+
+
+def test_yaml_string_values_and_dt_alias():
+    p = Params({"l_max": "24", "ra": "1e5", "l_cour_alf_damp": "false", "dt": 2e-4})
+    assert p.l_max == 24 and p.ra == 1e5 and p.l_cour_alf_damp is False and p.dtmax == 2e-4
+
+
+@pytest.mark.parametrize("n", [24, 48, 96, 576])
+def test_gauss_legendre_matches_reference_newton(n):
+    x, w = grid.gauss_legendre(n)
+    th, wo = horizontal.gauleg(n)
+    assert np.abs(np.arccos(x) - th).max() < 1e-13
+    assert np.abs(w - wo).max() < 1e-14
+
+
+def test_radial_operators_match_oracle():
+    cfg = {"l_max": 16, "n_r_max": 33, "n_cheb_max": 31}
+    rad = grid.radial(Params(cfg))
+    op = OP(cfg)
+    orad = oradial.radial_functions(op)
+    mats = oradial.cheb_mats(op, orad["x"], orad["r_cmb"], orad["r_icb"])
+    D1, D2, D3 = oradial.diff_mats(op, mats)
+    assert np.abs(rad["r"] - orad["r"]).max() < 1e-15
+    for a, b in ((rad["T"], mats[0]), (rad["T1"], mats[1]), (rad["T2"], mats[2]), (rad["T3"], mats[3])):
+        assert np.abs(a - b).max() <= 1e-12 * np.abs(b).max()
+    for a, b in ((rad["D1"], D1), (rad["D2"], D2), (rad["D3"], D3)):
+        assert np.abs(a - b).max() <= 1e-11 * np.abs(b).max()
+    assert np.abs(rad["C"] - oradial.costf_matrix(33)).max() < 1e-14
+    q = oradial.rint_weights(op, orad["r_cmb"], orad["r_icb"])
+    assert np.abs(rad["q"] - q).max() < 1e-14
+
+
+def test_lm_coefficients_match_oracle():
+    p = Params({"l_max": 20})
+    maps = grid.lm_maps(p)
+    c = grid.lm_coefficients(p, maps)
+    oc = horizontal.lm_arrays(OP({"l_max": 20}))
+    for k in ("dTheta2A", "dTheta2S", "dTheta3A", "dTheta3S", "dTheta4A", "dTheta4S"):
+        assert np.abs(c["dTh" + k[6:]] - oc[k]).max() < 1e-12
+    om = horizontal.lm_maps(OP({"l_max": 20}))
+    assert np.array_equal(maps["l"], om["lm2l"]) and np.array_equal(maps["m"], om["lm2m"])
+    # rows of degree l are lm(l, m) for m = 0..l
+    for l in (0, 5, 20):
+        rows = maps["l_rows"][maps["l_rows_off"][l]:maps["l_rows_off"][l + 1]]
+        assert np.all(maps["l"][rows] == l) and np.array_equal(maps["m"][rows], np.arange(l + 1))
+
+
+def test_sht_layout_covers_every_mode():
+    p = Params({"l_max": 33})
+    lay = grid.sht_layout(p, grid.lm_maps(p))
+    info = lay["mp_info"].reshape(p.n_m, 2, 4)
+    assert info[:, :, 0].sum() == p.lm_max
+    tiles = lay["ana_tiles"].reshape(-1, 3)
+    for m in range(p.n_m):
+        for par in range(2):
+            K = info[m, par, 0]
+            ks = sorted(t[2] for t in tiles if t[0] == m and t[1] == par)
+            assert ks == list(range(0, K, 32))
+    assert lay["Hp"] % 32 == 0 and lay["Hp"] >= lay["H"]
+
+
+@pytest.mark.parametrize("n", [24, 48, 96, 128, 192, 576])
+def test_fft_radices(n):
+    assert math.prod(grid.fft_radices(n)) == n
+
+
+def test_noise_matches_oracle():
+    from magic_amd.dynamo import seeded_noise
+    from oracle.dynamo import seeded_noise as onoise
+
+    class R:
+        pass
+    p = Params({"l_max": 12, "n_r_max": 17})
+    r = R()
+    r.maps = grid.lm_maps(p)
+    r.rad = grid.radial(p)
+    op = OP({"l_max": 12, "n_r_max": 17})
+    a = seeded_noise(p, r, 7, 1e-5)
+    b = onoise(op, horizontal.lm_maps(op), oradial.radial_functions(op), 7, 1e-5)
+    assert np.abs(a - b).max() < 1e-20
+
+
+def test_conduction_state_matches_oracle():
+    from magic_amd.dynamo import conduction_state
+    from oracle import matrices
+
+    class R:
+        pass
+    cfg = {"l_max": 16, "n_r_max": 33}
+    p = Params(cfg)
+    r = R()
+    r.rad = grid.radial(p)
+    s0, p0 = conduction_state(p, r)
+    op = OP(cfg)
+    orad = oradial.radial_functions(op)
+    os0, op0 = matrices.ps_cond(op, orad, oradial.cheb_mats(op))
+    assert np.abs(s0 - os0).max() < 1e-12 and np.abs(p0 - op0).max() < 1e-9 * np.abs(op0).max()
+
+
+def test_library_exports_header_symbols():
+    from magic_amd import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("library not built")
+    hdr = open(os.path.join(ROOT, "include", "magic_hip.h")).read()
+    decl = set(re.findall(r"^int (mh_\w+)\(", hdr, flags=re.M))
+    assert decl == set(_lib.EXPORTED)
+    lib = C.CDLL(_lib.LIB_PATH)
+    for name in decl:
+        assert hasattr(lib, name), name
+    _lib.load()   # struct sizes agree with the binding

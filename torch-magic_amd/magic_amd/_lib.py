@@ -1,0 +1,184 @@
+"""ctypes binding of ``libmagic_hip.so`` (the C ABI declared in include/magic_hip.h).
+
+The structures below mirror the header field by field; :func:`load` checks
+their sizes against ``mh_struct_sizes`` so a layout drift fails loudly.
+There is no fallback: when the library is missing every product entry point
+raises :class:`MagicHipError`.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libmagic_hip.so")
+
+P = C.c_void_p
+I = C.c_int
+L = C.c_long
+D = C.c_double
+
+
+class MagicHipError(RuntimeError):
+    """Raised when the HIP library is missing or a kernel entry point fails."""
+
+
+class SynIn(C.Structure):
+    _fields_ = [("x1", P), ("x2", P), ("mode", I), ("pad", I)]
+
+
+class SynArgs(C.Structure):
+    _fields_ = [("in_", (SynIn * 3) * 4), ("ngroups", I), ("B", I), ("ldx", I), ("H", I),
+                ("Hp", I), ("mp_info", P), ("lm_start", P), ("synP", P), ("synD", P),
+                ("or2", P), ("F", P), ("c_stride", L), ("m_stride", L), ("Bout", I), ("pad", I)]
+
+
+class AnaArgs(C.Structure):
+    _fields_ = [("F", P), ("c_stride", L), ("m_stride", L), ("Bin", I), ("B", I), ("ncomp", I),
+                ("cf1", I * 3), ("cf2", I * 3), ("pad0", I), ("outS", P * 3), ("outT", P * 3),
+                ("ldo", I), ("H", I), ("Hp", I), ("pad1", I), ("mp_info", P), ("lm_start", P),
+                ("anaW", P), ("anaWd", P), ("tiles", P), ("ntiles", I), ("pad2", I)]
+
+
+class FftArgs(C.Structure):
+    _fields_ = [("Mh", I), ("nstage", I), ("radix", I * 16), ("n_m", I), ("T", I), ("B", I),
+                ("ncomp", I), ("RB", I), ("pad", I), ("tw", P), ("tw2", P), ("F", P),
+                ("Fc_stride", L), ("Fm_stride", L), ("FB", I), ("pad0", I), ("G", P),
+                ("Gc_stride", L), ("comp_scale", P), ("ost2", P)]
+
+
+class NlArgs(C.Structure):
+    _fields_ = [("Gs", P), ("Gv", P), ("Gv_c_stride", L), ("Go", P), ("Go_c_stride", L),
+                ("N", I), ("T", I), ("P", I), ("pad", I), ("or2", P), ("or4", P), ("ost2", P),
+                ("valri2", P), ("valhi2", P), ("LFfac", D), ("cf2", D), ("af2", D),
+                ("l_mag", I), ("alf_damp", I), ("cfl", P)]
+
+
+class RowJob(C.Structure):
+    _fields_ = [("X", P), ("rows", P), ("MT", P), ("Y", P * 6), ("ldx", I), ("ldy", I),
+                ("K", I), ("R", I), ("Rpad", I), ("rseg", I), ("nrows", I), ("pad", I)]
+
+
+class LmCoef(C.Structure):
+    _fields_ = [("l", P), ("m", P), ("dLh", P), ("dTh2A", P), ("dTh2S", P), ("dTh3A", P),
+                ("dTh3S", P), ("dTh4A", P), ("dTh4S", P), ("l_max", I), ("L", I)]
+
+
+class RadCoef(C.Structure):
+    _fields_ = [("or1", P), ("or2", P), ("or3", P), ("or4", P), ("r2", P), ("rgrav", P),
+                ("N", I), ("pad", I)]
+
+
+class TdArgs(C.Structure):
+    _fields_ = [("lc", LmCoef), ("rc", RadCoef), ("Advr", P), ("Advt", P), ("Advp", P),
+                ("VSt", P), ("VxBr", P), ("VxBp", P), ("dVSr_dr", P), ("dVxBt_dr", P),
+                ("w", P), ("dw", P), ("z", P), ("ew", P), ("ez", P), ("ep", P), ("es", P),
+                ("eb", P), ("ej", P), ("CorFac", D), ("l_mag", I), ("pad", I)]
+
+
+class ImexSys(C.Structure):
+    _fields_ = [("old", P), ("impl", P), ("expl0", P), ("expl1", P)]
+
+
+class RhsArgs(C.Structure):
+    _fields_ = [("sys", ImexSys * 6), ("rs", P), ("rz", P), ("rb", P), ("rj", P), ("rwp", P),
+                ("m", P), ("L", I), ("N", I), ("wimp", D), ("wl1", D), ("we0", D), ("we1", D),
+                ("bots_l0", D), ("l_mag", I), ("pad", I)]
+
+
+class BuoyArgs(C.Structure):
+    _fields_ = [("rwp", P), ("s", P), ("ew0", P), ("rgrav", P), ("m", P), ("L", I), ("N", I),
+                ("fac", D), ("BuoFac", D), ("lm_l0", I), ("pad", I)]
+
+
+class ImplArgs(C.Structure):
+    _fields_ = [("lc", LmCoef), ("rc", RadCoef)] + [(n, P) for n in (
+        "s", "ds", "d2s", "z", "d2z", "w", "dw", "ddw", "dddw", "p", "dp", "b", "ddb", "aj", "ddj",
+        "old_s", "impl_s", "old_z", "impl_z", "old_w", "impl_w", "old_p", "impl_p",
+        "old_b", "impl_b", "old_j", "impl_j")] + [("opr", D), ("opm", D), ("BuoFac", D),
+                                                  ("l_mag", I), ("lm_l0", I)]
+
+
+class EnergyArgs(C.Structure):
+    _fields_ = [("w", P), ("dw", P), ("z", P), ("b", P), ("db", P), ("aj", P), ("lm_start", P),
+                ("or2", P), ("qw", P), ("out", P), ("l_max", I), ("N", I), ("by_m", I),
+                ("l_mag", I), ("LFfac", D)]
+
+
+class BuildArgs(C.Structure):
+    _fields_ = [("R", P), ("D1", P), ("D2", P), ("D3", P), ("or1", P), ("or2", P), ("out", P),
+                ("N", I), ("l_max", I), ("n_cheb", I), ("ktopv", I), ("kbotv", I), ("kind", I),
+                ("rnorm", D), ("wimp", D), ("opr", D), ("opm", D)]
+
+
+class GemmArgs(C.Structure):
+    _fields_ = [("A", P), ("B", P), ("C", P)] + [(n, L) for n in (
+        "sAr", "sAc", "sAb", "sBr", "sBc", "sBb", "sCr", "sCc", "sCb")] + [
+        ("M", I), ("Nn", I), ("K", I), ("batch", I), ("alpha", D)]
+
+
+_STRUCT_ORDER = [SynArgs, AnaArgs, FftArgs, NlArgs, RowJob, TdArgs, RhsArgs, BuoyArgs,
+                 ImplArgs, EnergyArgs, BuildArgs, GemmArgs]
+
+_SIGS = {
+    "mh_plm_tables": [I, I, I, I, P, P, P, P, P, P, P, P, P],
+    "mh_leg_synth": [I, P, I, P],
+    "mh_leg_analysis": [I, P, P],
+    "mh_fft": [I, P, P],
+    "mh_get_nl": [P, P],
+    "mh_rowop": [P, P, I, P],
+    "mh_td": [P, P],
+    "mh_rhs": [P, P],
+    "mh_buoy": [P, P],
+    "mh_impl": [P, P],
+    "mh_energy": [P, P],
+    "mh_build_matrices": [P, P],
+    "mh_gj_inverse": [P, P, P, P, I, I, I, P, P],
+    "mh_gemm_f64": [P, P],
+    "mh_band_factor": [P, P, P, I, I, I, I, P],
+    "mh_band_solve": [P, P, P, P, P, P, P, I, I, I, I, P],
+    "mh_struct_sizes": [P, I],
+}
+
+EXPORTED = tuple(_SIGS)
+
+_lib = None
+
+
+def load(path: str | None = None):
+    """Load the library (once), bind signatures and verify struct sizes."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = path or LIB_PATH
+    if not os.path.exists(path):
+        raise MagicHipError(f"HIP library not built: {path} (run __graft_entry__.build())")
+    lib = C.CDLL(path)
+    for name, args in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = I
+    buf = (I * 32)()
+    n = lib.mh_struct_sizes(C.cast(buf, P), 32)
+    if n != len(_STRUCT_ORDER):
+        raise MagicHipError(f"struct table mismatch: library reports {n} structs")
+    for k, cls in enumerate(_STRUCT_ORDER):
+        if buf[k] != C.sizeof(cls):
+            raise MagicHipError(f"{cls.__name__}: binding size {C.sizeof(cls)} != library {buf[k]}")
+    _lib = lib
+    return lib
+
+
+def call(name: str, *args):
+    """Call an entry point and raise on a non-zero return code."""
+    rc = getattr(load(), name)(*args)
+    if rc != 0:
+        kind = {-1: "invalid argument", -2: "launch failure"}.get(rc, f"code {rc}")
+        raise MagicHipError(f"{name}: {kind}")
+    return rc
+
+
+def ref(struct) -> C.c_void_p:
+    """Host pointer to a ctypes structure, for the ``const Mh*Args*`` parameters."""
+    return C.cast(C.pointer(struct), P)

@@ -1,0 +1,292 @@
+// Legendre tables and FP64 MFMA Legendre synthesis / analysis for gfx950.
+//
+// Table layouts (per azimuthal order m and parity p = (l-m)%2, l = m+p+2k):
+//   synthesis  syn[m][p][k][j]  (k < K4 = roundup(K,4), j < Hp)  j fastest
+//   analysis   ana[m][p][j][k]  (j < Hp, k < K16 = roundup(K,16)) k fastest
+// j indexes the northern Gauss nodes (theta pairs), Hp = roundup(H,32).
+// mp_info[(m*2+p)*4 + {0,1,2,3}] = {K, syn_off, ana_off, K16}.
+//
+// Fourier buffers F[c][m][t][b] (complex) hold, per component c and order m,
+// the (theta, batch) matrix; t = 2j is the northern and t = 2j+1 the southern
+// node of pair j (interleaved rows, as in the grid layout).
+#include "common.h"
+
+// ---------------------------------------------------------------- P_lm tables
+// Orthonormal P_lm without Condon-Shortley phase and sin(theta) dP_lm/dtheta by
+// the three-term recurrence (the same recurrences as the reference plms.py).
+__global__ void k_plm_tables(int l_max, int H, int Hp, const double* __restrict__ cos_t,
+                             const double* __restrict__ sin_t, const double* __restrict__ wgt,
+                             const int* __restrict__ mp_info, double* __restrict__ synP,
+                             double* __restrict__ synD, double* __restrict__ anaW,
+                             double* __restrict__ anaWd) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int m = blockIdx.y;
+  if (j >= H) return;
+  const double c = cos_t[j], s = sin_t[j];
+  const double w2pi = 2.0 * M_PI * wgt[j];
+  const double dnorm = 0.28209479177387814347403972578;  // 1/sqrt(4 pi)
+  double fac = 1.0;
+  for (int q = 3; q <= 2 * m + 1; q += 2) fac = fac * (double)q / (double)(q - 1);
+  double pmm = sqrt(fac);
+  if (s != 0.0) {
+    double sp = 1.0;
+    for (int q = 0; q < m; ++q) sp *= s;
+    pmm *= sp;
+  } else if (m != 0) {
+    pmm = 0.0;
+  }
+  double pm1 = 0.0, p0 = pmm;  // P_{l-1}, P_l (unnormalised)
+  for (int l = m; l <= l_max; ++l) {
+    const int L1 = l + 1;  // P_{l+1}
+    double p1;
+    if ((L1 - m) * (L1 + m) > 0) {
+      const double dL = (double)L1, dM = (double)m;
+      const double a = sqrt((2.0 * dL - 1.0) * (2.0 * dL + 1.0) / ((dL - dM) * (dL + dM)));
+      const double b = sqrt((2.0 * dL + 1.0) * (dL + dM - 1.0) * (dL - dM - 1.0) /
+                            ((2.0 * dL - 3.0) * (dL - dM) * (dL + dM)));
+      p1 = c * a * p0 - b * pm1;
+    } else {
+      p1 = 0.0;
+    }
+    const double dl = (double)l, dm = (double)m;
+    const double cl1 = sqrt((dl + dm + 1.0) * (dl - dm + 1.0) / ((2.0 * dl + 1.0) * (2.0 * dl + 3.0)));
+    const double cl = (l > m) ? sqrt((dl + dm) * (dl - dm) / ((2.0 * dl - 1.0) * (2.0 * dl + 1.0))) : 0.0;
+    const double P = dnorm * p0;
+    const double dP = dnorm * ((double)l * cl1 * p1 - (double)(l + 1) * cl * pm1);
+    const int p = (l - m) & 1, k = (l - m) >> 1;
+    const int* info = mp_info + (m * 2 + p) * 4;
+    const long so = (long)info[1] + (long)k * Hp + j;
+    synP[so] = P;
+    synD[so] = dP;
+    const long ao = (long)info[2] + (long)j * info[3] + k;
+    anaW[ao] = w2pi * P;
+    anaWd[ao] = w2pi * dP;
+    pm1 = p0;
+    p0 = p1;
+  }
+}
+
+extern "C" int mh_plm_tables(int l_max, int n_m, int H, int Hp, const double* cos_t,
+                             const double* sin_t, const double* wgt, const int* mp_info,
+                             double* synP, double* synD, double* anaW, double* anaWd,
+                             void* stream) {
+  if (l_max < 0 || n_m != l_max + 1 || H <= 0 || Hp < H) return MH_EINVAL;
+  dim3 grid(mh_cdiv(H, 64), n_m);
+  hipLaunchKernelGGL(k_plm_tables, grid, dim3(64), 0, mh_stream(stream), l_max, H, Hp, cos_t, sin_t, wgt,
+                     mp_info, synP, synD, anaW, anaWd);
+  return mh_launch_status();
+}
+
+// ---------------------------------------------------------------- synthesis
+typedef MhSynIn SynIn;
+typedef MhSynArgs SynArgs;
+
+__device__ __forceinline__ double syn_load(const SynIn& in, long lm, int l, int b, int reim,
+                                           int ldx, const double* or2) {
+  const long off = lm * 2 * (long)ldx + 2 * b + reim;
+  const double dl = (double)l * (double)(l + 1);
+  if (in.mode == 0) return in.x1[off];
+  if (in.mode == 1) return dl * in.x1[off];
+  return -in.x1[off] + or2[b] * dl * in.x2[off];
+}
+
+// i*m*z where lane pairs (2q, 2q+1) hold (re, im) of z
+__device__ __forceinline__ double imul_pair(double v, double m, int lane) {
+  const double o = __shfl_xor(v, 1);
+  return (lane & 1) ? m * o : -m * o;
+}
+
+// WG = 4 waves; wave w covers theta pairs j0 + 16*(w&1) .. and complex batch
+// columns b0 + 8*(w>>1) .. ; WG tile = 32 theta pairs x 16 batch columns.
+template <bool VEC>
+__global__ __launch_bounds__(256) void k_syn(SynArgs a) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int m = blockIdx.z;
+  const int nbt = (a.B + 15) / 16;
+  const int g = blockIdx.x / nbt;
+  const int b0 = (blockIdx.x % nbt) * 16 + 8 * (w >> 1);
+  const int j0 = blockIdx.y * 32 + 16 * (w & 1);
+  const int col = lane & 15, kk = lane >> 4;
+  const int b = b0 + (col >> 1), reim = col & 1;
+  const bool bval = b < a.B;
+  const int bl = bval ? b : 0;
+  const long lms = a.lm_start[m];
+  double4_t acc[VEC ? 10 : 2];
+#pragma unroll
+  for (int q = 0; q < (VEC ? 10 : 2); ++q) acc[q] = (double4_t){0.0, 0.0, 0.0, 0.0};
+  const SynIn* in = a.in[g];
+#pragma unroll
+  for (int p = 0; p < 2; ++p) {
+    const int* info = a.mp_info + (m * 2 + p) * 4;
+    const int K = info[0];
+    const double* tP = a.synP + info[1];
+    const double* tD = a.synD + info[1];
+    for (int k0 = 0; k0 < K; k0 += 4) {
+      const int k = k0 + kk;
+      const double aP = tP[(long)k * a.Hp + j0 + col];
+      const bool v = bval && (k < K);
+      const int l = m + p + 2 * k;
+      const long lm = lms + p + 2 * k;
+      if (!VEC) {
+        const double x = v ? syn_load(in[0], lm, l, bl, reim, a.ldx, a.or2) : 0.0;
+        acc[p] = mfma_f64(aP, x, acc[p]);
+      } else {
+        const double aD = tD[(long)k * a.Hp + j0 + col];
+        const double xq = v ? syn_load(in[0], lm, l, bl, reim, a.ldx, a.or2) : 0.0;
+        const double xs = v ? syn_load(in[1], lm, l, bl, reim, a.ldx, a.or2) : 0.0;
+        const double xt = v ? syn_load(in[2], lm, l, bl, reim, a.ldx, a.or2) : 0.0;
+        acc[5 * p + 0] = mfma_f64(aP, xq, acc[5 * p + 0]);   // P Q
+        acc[5 * p + 1] = mfma_f64(aP, xt, acc[5 * p + 1]);   // P T
+        acc[5 * p + 2] = mfma_f64(aP, xs, acc[5 * p + 2]);   // P S
+        acc[5 * p + 3] = mfma_f64(aD, xs, acc[5 * p + 3]);   // dP S
+        acc[5 * p + 4] = mfma_f64(aD, xt, acc[5 * p + 4]);   // dP T
+      }
+    }
+  }
+  const double dm = (double)m;
+  const long rowlen = 2L * a.Bout;
+  const long cb = 2L * ((long)g * a.B + bl) + reim;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int j = j0 + mfma_row(lane, i);
+    const bool st = bval && (j < a.H);
+    double* base = a.F + (long)m * a.m_stride + cb;
+    const long oN = (long)(2 * j) * rowlen, oS = (long)(2 * j + 1) * rowlen;
+    if (!VEC) {
+      const double e = acc[0][i], o = acc[1][i];
+      if (st) {
+        base[oN] = e + o;
+        base[oS] = e - o;
+      }
+    } else {
+      const double Qe = acc[0][i], Qo = acc[5][i];
+      const double PTe = acc[1][i], PTo = acc[6][i];
+      const double PSe = acc[2][i], PSo = acc[7][i];
+      const double DSe = acc[3][i], DSo = acc[8][i];
+      const double DTe = acc[4][i], DTo = acc[9][i];
+      const double iPTp = imul_pair(PTe + PTo, dm, lane), iPTm = imul_pair(PTe - PTo, dm, lane);
+      const double iPSp = imul_pair(PSe + PSo, dm, lane), iPSm = imul_pair(PSe - PSo, dm, lane);
+      if (st) {
+        base[oN] = Qe + Qo;
+        base[oS] = Qe - Qo;
+        base[a.c_stride + oN] = (DSe + DSo) + iPTp;
+        base[a.c_stride + oS] = -(DSe - DSo) + iPTm;
+        base[2 * a.c_stride + oN] = iPSp - (DTe + DTo);
+        base[2 * a.c_stride + oS] = iPSm + (DTe - DTo);
+      }
+    }
+  }
+}
+
+extern "C" int mh_leg_synth(int vector, const MhSynArgs* args, int n_m, void* stream) {
+  const SynArgs& a = *args;
+  if (a.ngroups < 1 || a.ngroups > 4 || a.B < 1 || a.H < 1 || a.Hp % 32 != 0 || a.Hp < a.H)
+    return MH_EINVAL;
+  dim3 grid(a.ngroups * mh_cdiv(a.B, 16), mh_cdiv(a.H, 32), n_m);
+  if (vector)
+    hipLaunchKernelGGL(k_syn<true>, grid, dim3(256), 0, mh_stream(stream), a);
+  else
+    hipLaunchKernelGGL(k_syn<false>, grid, dim3(256), 0, mh_stream(stream), a);
+  return mh_launch_status();
+}
+
+
+// ---------------------------------------------------------------- analysis
+typedef MhAnaArgs AnaArgs;
+
+// WG = 4 waves: wave w covers k0 + 16*(w&1) .. (16 rows of l) and complex batch
+// columns b0 + 8*(w>>1) ..; K loop runs over theta pairs in steps of 4.
+template <bool VEC>
+__global__ __launch_bounds__(256) void k_ana(AnaArgs a) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int* tl = a.tiles + 3 * blockIdx.y;
+  const int m = tl[0], p = tl[1];
+  const int kbase = tl[2] + 16 * (w & 1);
+  const int nbt = (a.B + 15) / 16;
+  const int comp = blockIdx.x / nbt;
+  const int b0 = (blockIdx.x % nbt) * 16 + 8 * (w >> 1);
+  const int col = lane & 15, kk = lane >> 4;
+  const int b = b0 + (col >> 1), reim = col & 1;
+  const bool bval = b < a.B;
+  const int bl = bval ? b : 0;
+  const int* info = a.mp_info + (m * 2 + p) * 4;
+  const int K = info[0], K16 = info[3];
+  const double* tW = a.anaW + info[2];
+  const double* tD = a.anaWd + info[2];
+  const long rowlen = 2L * a.Bin;
+  const double* Fm = a.F + (long)m * a.m_stride + 2L * bl + reim;
+  const double sg = p ? -1.0 : 1.0;  // same-parity fold: N + sg*S
+  double4_t acc[VEC ? 4 : 1];
+#pragma unroll
+  for (int q = 0; q < (VEC ? 4 : 1); ++q) acc[q] = (double4_t){0.0, 0.0, 0.0, 0.0};
+  const int arow = kbase + col;  // row of A handled by this lane
+  if (kbase < K) {
+    if (!VEC) {
+      const double* Fc = Fm + (long)a.cf1[comp] * a.c_stride;
+      for (int j0 = 0; j0 < a.H; j0 += 4) {
+        const int j = j0 + kk;
+        const double aw = tW[(long)j * K16 + arow];
+        double x = 0.0;
+        if (bval && j < a.H) {
+          const double fN = Fc[(long)(2 * j) * rowlen], fS = Fc[(long)(2 * j + 1) * rowlen];
+          x = fN + sg * fS;
+        }
+        acc[0] = mfma_f64(aw, x, acc[0]);
+      }
+    } else {
+      const double* F1 = Fm + (long)a.cf1[comp] * a.c_stride;
+      const double* F2 = Fm + (long)a.cf2[comp] * a.c_stride;
+      for (int j0 = 0; j0 < a.H; j0 += 4) {
+        const int j = j0 + kk;
+        const double aw = tW[(long)j * K16 + arow];
+        const double ad = tD[(long)j * K16 + arow];
+        double f1s = 0.0, f1o = 0.0, f2s = 0.0, f2o = 0.0;
+        if (bval && j < a.H) {
+          const double f1N = F1[(long)(2 * j) * rowlen], f1S = F1[(long)(2 * j + 1) * rowlen];
+          const double f2N = F2[(long)(2 * j) * rowlen], f2S = F2[(long)(2 * j + 1) * rowlen];
+          f1s = f1N + sg * f1S;
+          f1o = f1N - sg * f1S;
+          f2s = f2N + sg * f2S;
+          f2o = f2N - sg * f2S;
+        }
+        acc[0] = mfma_f64(ad, f2o, acc[0]);  // wdP f2(opposite fold)
+        acc[1] = mfma_f64(ad, f1o, acc[1]);  // wdP f1(opposite fold)
+        acc[2] = mfma_f64(aw, f1s, acc[2]);  // wP f1(same fold)
+        acc[3] = mfma_f64(aw, f2s, acc[3]);  // wP f2(same fold)
+      }
+    }
+  }
+  const double dm = (double)m;
+  const long lms = a.lm_start[m];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int k = kbase + mfma_row(lane, i);
+    const int l = m + p + 2 * k;
+    const bool st = bval && (k < K);
+    const long off = (lms + p + 2 * k) * 2L * a.ldo + 2 * bl + reim;
+    if (!VEC) {
+      if (st) a.outS[comp][off] = acc[0][i];
+    } else {
+      const double inv = (l > 0) ? 1.0 / ((double)l * (double)(l + 1)) : 0.0;
+      const double Sv = acc[0][i] + imul_pair(acc[2][i], -dm, lane);
+      const double Tv = -acc[1][i] + imul_pair(acc[3][i], -dm, lane);
+      if (st) {
+        a.outS[comp][off] = Sv * inv;
+        if (a.outT[comp]) a.outT[comp][off] = Tv * inv;
+      }
+    }
+  }
+}
+
+extern "C" int mh_leg_analysis(int vector, const MhAnaArgs* args, void* stream) {
+  const AnaArgs& a = *args;
+  if (a.ncomp < 1 || a.ncomp > 3 || a.B < 1 || a.ntiles < 1 || a.Hp % 32 != 0 || a.Hp < a.H)
+    return MH_EINVAL;
+  dim3 grid(a.ncomp * mh_cdiv(a.B, 16), a.ntiles);
+  if (vector)
+    hipLaunchKernelGGL(k_ana<true>, grid, dim3(256), 0, mh_stream(stream), a);
+  else
+    hipLaunchKernelGGL(k_ana<false>, grid, dim3(256), 0, mh_stream(stream), a);
+  return mh_launch_status();
+}
+

@@ -1,0 +1,261 @@
+// Implicit-operator rebuild on the device (runs when dt changes) and banded
+// LINPACK-order factor/solve for the finite-difference per-degree systems.
+//
+//  * k_build_*: per-degree collocation matrices of the Boussinesq Chebyshev
+//    implicit systems (entropy, toroidal, poloidal/pressure 2N, magnetic b and
+//    j, l=0 pressure), with the boundary-coefficient factor 1/2 on the first
+//    and last Chebyshev columns.
+//  * k_gj_inverse: one workgroup per matrix, row (and optionally column)
+//    max-norm equilibration, Gauss-Jordan inversion with partial pivoting
+//    (first maximum), result rescaled so that inv @ rhs solves the original
+//    system.  info[b] = first zero pivot (1-based) or 0.
+//  * k_gemm_f64: batched C = alpha * A B with general strides on FP64 MFMA,
+//    used to fold the DCT-I and the radial derivative matrices into the
+//    inverses.
+#include "common.h"
+
+typedef MhBuildArgs BuildArgs;
+
+__device__ __forceinline__ double bfac(int n, int N) { return (n == 0 || n == N - 1) ? 0.5 : 1.0; }
+
+__global__ void k_build(BuildArgs a) {
+  const int N = a.N;
+  const int n2 = (a.kind == 4) ? 2 * N : N;
+  const long per = (long)n2 * n2;
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int nmat = (a.kind == 5) ? 1 : a.l_max + 1;
+  if (idx >= per * nmat) return;
+  const int l = (int)(idx / per);
+  const int rem = (int)(idx - (long)l * per);
+  const int i = rem / n2, j = rem - i * n2;
+  const double dL = (double)l * (double)(l + 1);
+  const double rn = a.rnorm, w0 = a.wimp;
+  double v = 0.0;
+  if (a.kind <= 3 || a.kind == 5) {
+    const int n = j;
+    const double R = a.R[i * N + n], D = a.D1[i * N + n], D2 = a.D2[i * N + n];
+    const double or1 = a.or1[i], or2 = a.or2[i];
+    const bool top = (i == 0), bot = (i == N - 1);
+    if (a.kind == 0) {
+      v = rn * (R - w0 * a.opr * (D2 + 2.0 * or1 * D - dL * or2 * R));
+      if (top || bot) v = rn * R;
+    } else if (a.kind == 1) {
+      if (l == 0) {
+        v = (i == j) ? 1.0 : 0.0;
+      } else {
+        v = rn * dL * or2 * (R - w0 * (D2 - dL * or2 * R));
+        if (top) v = (a.ktopv == 1) ? rn * (D - 2.0 * or1 * R) : rn * R;
+        if (bot) v = (a.kbotv == 1) ? rn * (D - 2.0 * or1 * R) : rn * R;
+      }
+    } else if (a.kind == 2 || a.kind == 3) {
+      if (l == 0) {
+        v = (i == j) ? 1.0 : 0.0;
+      } else {
+        v = rn * dL * or2 * (R - w0 * a.opm * (D2 - dL * or2 * R));
+        if (a.kind == 2) {
+          if (top) v = rn * (D + (double)l * or1 * R);
+          if (bot) v = rn * (D - (double)(l + 1) * or1 * R);
+        } else if (top || bot) {
+          v = rn * R;
+        }
+      }
+    } else {  // p0
+      v = top ? rn * R : rn * D;
+    }
+    const bool ident = (a.kind == 1 || a.kind == 2 || a.kind == 3) && l == 0;
+    if (!ident) {
+      if ((top || (bot && a.kind != 5)) && n >= a.n_cheb) v = 0.0;
+      v *= bfac(n, N);
+    }
+  } else {  // WP, 2N x 2N
+    const int bi = i / N, ii = i - bi * N;  // row block, row within block
+    const int bj = j / N, n = j - bj * N;   // column block, Chebyshev index
+    if (l == 0) {
+      v = (i == j) ? 1.0 : 0.0;
+    } else {
+      const double R = a.R[ii * N + n], D = a.D1[ii * N + n], D2 = a.D2[ii * N + n],
+                   D3 = a.D3[ii * N + n];
+      const double or1 = a.or1[ii], or2 = a.or2[ii];
+      const bool top = (ii == 0), bot = (ii == N - 1);
+      if (!top && !bot) {
+        if (bi == 0 && bj == 0) v = rn * dL * or2 * (R - w0 * (D2 - dL * or2 * R));
+        else if (bi == 0 && bj == 1) v = rn * w0 * D;
+        else if (bi == 1 && bj == 0)
+          v = rn * dL * or2 * (-D - w0 * (-D3 + dL * or2 * D - dL * or2 * 2.0 * or1 * R));
+        else v = -rn * w0 * dL * or2 * R;
+      } else if (bj == 0) {
+        if (bi == 0) {
+          v = rn * R;  // w = 0 on both boundaries
+        } else {
+          const int ks = top ? a.ktopv : a.kbotv;
+          v = (ks == 1) ? rn * (D2 - 2.0 * or1 * D) : rn * D;
+        }
+      }
+      if ((top || bot) && n >= a.n_cheb) v = 0.0;
+      v *= bfac(n, N);
+    }
+  }
+  a.out[idx] = v;
+}
+
+extern "C" int mh_build_matrices(const MhBuildArgs* args, void* stream) {
+  const BuildArgs& a = *args;
+  if (a.N < 3 || a.kind < 0 || a.kind > 5) return MH_EINVAL;
+  const int n2 = (a.kind == 4) ? 2 * a.N : a.N;
+  const long total = (long)n2 * n2 * ((a.kind == 5) ? 1 : a.l_max + 1);
+  hipLaunchKernelGGL(k_build, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, mh_stream(stream), a);
+  return mh_launch_status();
+}
+
+// ------------------------------------------------------------ Gauss-Jordan
+// work: per matrix n x 2n doubles; A and out may alias.
+__global__ __launch_bounds__(256) void k_gj_inverse(const double* __restrict__ A, double* out,
+                                                    double* __restrict__ work,
+                                                    double* __restrict__ scal, int n,
+                                                    int precond, int* info) {
+  const int bm = blockIdx.x;
+  const long nn = (long)n * n;
+  const double* Am = A + bm * nn;
+  double* W = work + bm * 2 * nn;
+  double* fr = scal + (long)bm * 2 * n;
+  double* fc = fr + n;
+  const int n2 = 2 * n;
+  __shared__ double sval[256];
+  __shared__ int sidx[256];
+  __shared__ int s_piv;
+  __shared__ double s_pv;
+  __shared__ int s_info;
+  if (threadIdx.x == 0) s_info = 0;
+  // row equilibration
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    double mx = 0.0;
+    for (int j = 0; j < n; ++j) mx = fmax(mx, fabs(Am[(long)i * n + j]));
+    fr[i] = (precond >= 1 && mx > 0.0) ? 1.0 / mx : 1.0;
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+    double mx = 0.0;
+    for (int i = 0; i < n; ++i) mx = fmax(mx, fabs(fr[i] * Am[(long)i * n + j]));
+    fc[j] = (precond >= 2 && mx > 0.0) ? 1.0 / mx : 1.0;
+  }
+  __syncthreads();
+  for (long e = threadIdx.x; e < (long)n * n2; e += blockDim.x) {
+    const int i = (int)(e / n2), j = (int)(e - (long)i * n2);
+    W[e] = (j < n) ? fr[i] * Am[(long)i * n + j] * fc[j] : ((j - n) == i ? 1.0 : 0.0);
+  }
+  __syncthreads();
+  for (int k = 0; k < n; ++k) {
+    // pivot: first maximum of |W[i][k]|, i >= k
+    double best = -1.0;
+    int bi = k;
+    for (int i = k + threadIdx.x; i < n; i += blockDim.x) {
+      const double v = fabs(W[(long)i * n2 + k]);
+      if (v > best) { best = v; bi = i; }
+    }
+    sval[threadIdx.x] = best;
+    sidx[threadIdx.x] = bi;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+      if (threadIdx.x < s) {
+        const double o = sval[threadIdx.x + s];
+        const int oi = sidx[threadIdx.x + s];
+        if (o > sval[threadIdx.x] || (o == sval[threadIdx.x] && oi < sidx[threadIdx.x])) {
+          sval[threadIdx.x] = o;
+          sidx[threadIdx.x] = oi;
+        }
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      s_piv = sidx[0];
+      s_pv = W[(long)sidx[0] * n2 + k];
+      if (fabs(s_pv) <= 1e-15 && s_info == 0) s_info = k + 1;
+    }
+    __syncthreads();
+    const int p = s_piv;
+    const double pinv = (s_pv != 0.0) ? 1.0 / s_pv : 0.0;
+    if (p != k) {
+      for (int j = threadIdx.x; j < n2; j += blockDim.x) {
+        const double t = W[(long)k * n2 + j];
+        W[(long)k * n2 + j] = W[(long)p * n2 + j];
+        W[(long)p * n2 + j] = t;
+      }
+      __syncthreads();
+    }
+    for (int j = threadIdx.x; j < n2; j += blockDim.x) W[(long)k * n2 + j] *= pinv;
+    __syncthreads();
+    // eliminate column k from all other rows (columns k+1 .. 2n-1 change)
+    const long ncol = n2 - k - 1;
+    for (long e = threadIdx.x; e < (long)n * ncol; e += blockDim.x) {
+      const int i = (int)(e / ncol);
+      const int j = k + 1 + (int)(e - (long)i * ncol);
+      if (i == k) continue;
+      W[(long)i * n2 + j] -= W[(long)i * n2 + k] * W[(long)k * n2 + j];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x)
+      if (i != k) W[(long)i * n2 + k] = 0.0;
+    __syncthreads();
+  }
+  double* O = out + bm * nn;
+  for (long e = threadIdx.x; e < nn; e += blockDim.x) {
+    const int i = (int)(e / n), j = (int)(e - (long)i * n);
+    O[e] = fc[i] * W[(long)i * n2 + n + j] * fr[j];
+  }
+  if (threadIdx.x == 0 && info) info[bm] = s_info;
+}
+
+extern "C" int mh_gj_inverse(const double* A, double* out, double* work, double* scal, int n,
+                             int nbatch, int precond, int* info, void* stream) {
+  if (n < 1 || nbatch < 1) return MH_EINVAL;
+  hipLaunchKernelGGL(k_gj_inverse, dim3(nbatch), dim3(256), 0, mh_stream(stream), A, out, work, scal, n,
+                     precond, info);
+  return mh_launch_status();
+}
+
+// ------------------------------------------------------------ batched GEMM
+typedef MhGemmArgs GemmArgs;
+
+// one wave per 16 x 64 tile of C
+__global__ __launch_bounds__(64) void k_gemm_f64(GemmArgs g) {
+  const int lane = threadIdx.x;
+  const int b = blockIdx.z;
+  const int i0 = blockIdx.y * 16, j0 = blockIdx.x * 64;
+  const int kk = lane >> 4, c16 = lane & 15;
+  const double* A = g.A + b * g.sAb;
+  const double* B = g.B + b * g.sBb;
+  double4_t acc[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) acc[q] = (double4_t){0.0, 0.0, 0.0, 0.0};
+  const int ia = i0 + c16;
+  for (int k0 = 0; k0 < g.K; k0 += 4) {
+    const int k = k0 + kk;
+    const double av = (ia < g.M && k < g.K) ? A[ia * g.sAr + k * g.sAc] : 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int jb = j0 + 16 * q + c16;
+      const double bv = (jb < g.Nn && k < g.K) ? B[k * g.sBr + jb * g.sBc] : 0.0;
+      acc[q] = mfma_f64(av, bv, acc[q]);
+    }
+  }
+  double* C = g.C + b * g.sCb;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int ic = i0 + mfma_row(lane, i);
+    if (ic >= g.M) continue;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int jc = j0 + 16 * q + c16;
+      if (jc < g.Nn) C[ic * g.sCr + jc * g.sCc] = g.alpha * acc[q][i];
+    }
+  }
+}
+
+extern "C" int mh_gemm_f64(const MhGemmArgs* args, void* stream) {
+  const GemmArgs& g = *args;
+  if (g.M < 1 || g.Nn < 1 || g.K < 1 || g.batch < 1) return MH_EINVAL;
+  dim3 grid(mh_cdiv(g.Nn, 64), mh_cdiv(g.M, 16), g.batch);
+  hipLaunchKernelGGL(k_gemm_f64, grid, dim3(64), 0, mh_stream(stream), g);
+  return mh_launch_status();
+}
+

@@ -1,0 +1,157 @@
+"""Device resources of one resolution and launchers for the SHT kernels.
+
+A :class:`Resolution` uploads the host set-up of :mod:`magic_amd.grid`, builds
+the Legendre tables on the GPU (``mh_plm_tables``) and the azimuthal FFT plan,
+and offers thin launchers around the C ABI that take torch device tensors.
+torch is used for device memory and the current stream only.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib, grid
+from .params import Params
+
+F64 = torch.float64
+C128 = torch.complex128
+I32 = torch.int32
+
+
+def ptr(t) -> int:
+    return 0 if t is None else t.data_ptr()
+
+
+def stream_handle() -> C.c_void_p:
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def require_gpu(device) -> torch.device:
+    device = torch.device(device)
+    if device.type != "cuda" or not torch.cuda.is_available():
+        raise _lib.MagicHipError("magic_amd runs on the GPU only (no CPU fallback); "
+                                 "no HIP device is available")
+    _lib.load()
+    return device
+
+
+class Resolution:
+    """Per-(l_max, n_r_max, radratio, n_cheb_max) tables resident on the device."""
+
+    def __init__(self, p: Params, device="cuda"):
+        self.device = dev = require_gpu(device)
+        self.p = p
+        self.maps = grid.lm_maps(p)
+        self.th = grid.theta_grid(p)
+        self.rad = grid.radial(p)
+        self.lmc = grid.lm_coefficients(p, self.maps)
+        self.lay = grid.sht_layout(p, self.maps)
+        self.L, self.N, self.M = p.lm_max, p.n_r_max, p.n_m
+        self.T, self.P, self.H, self.Hp = p.n_theta, p.n_phi, self.lay["H"], self.lay["Hp"]
+
+        def up(a, dt=F64):
+            return torch.as_tensor(np.ascontiguousarray(a), dtype=dt).to(dev)
+
+        self.d = {}
+        self.d["mp_info"] = up(self.lay["mp_info"], I32)
+        self.d["lm_start"] = up(self.maps["lm_start"], I32)
+        self.d["l"] = up(self.maps["l"], I32)
+        self.d["m"] = up(self.maps["m"], I32)
+        self.d["l_rows"] = up(self.maps["l_rows"], I32)
+        self.d["ana_tiles"] = up(self.lay["ana_tiles"], I32)
+        for k, v in self.lmc.items():
+            self.d[k] = up(v)
+        for k in ("or1", "or2", "or3", "or4", "r2", "rgrav", "q", "valri2", "valhi2"):
+            self.d[k] = up(self.rad[k])
+        self.d["ost2_grid"] = up(self.th["ost2_grid"])
+        # Legendre tables
+        self.synP = torch.zeros(self.lay["syn_size"], dtype=F64, device=dev)
+        self.synD = torch.zeros_like(self.synP)
+        self.anaW = torch.zeros(self.lay["ana_size"], dtype=F64, device=dev)
+        self.anaWd = torch.zeros_like(self.anaW)
+        cos_n, sin_n, w_n = up(self.th["cos_n"]), up(self.th["sin_n"]), up(self.th["w_n"])
+        _lib.call("mh_plm_tables", p.l_max, p.n_m, self.H, self.Hp, ptr(cos_n), ptr(sin_n),
+                  ptr(w_n), ptr(self.d["mp_info"]), ptr(self.synP), ptr(self.synD),
+                  ptr(self.anaW), ptr(self.anaWd), stream_handle())
+        # FFT plan: complex length Mh = n_phi/2, twiddles exp(-2 pi i k/Mh), exp(-2 pi i k/n_phi)
+        self.Mh = p.n_phi // 2
+        self.radix = grid.fft_radices(self.Mh)
+        k = np.arange(self.Mh)
+        tw = np.exp(-2j * np.pi * k / self.Mh)
+        tw2 = np.exp(-2j * np.pi * k / p.n_phi)
+        self.d["tw"] = up(np.stack([tw.real, tw.imag], -1).reshape(-1))
+        self.d["tw2"] = up(np.stack([tw2.real, tw2.imag], -1).reshape(-1))
+        self.fft_rb = max(1, min(16, 2048 // self.Mh))
+        self.d["comp_scale_v"] = up(np.array([0, 0, 0, 1, 1, 1, 1, 1, 1]), I32)
+        self.d["comp_scale_2"] = up(np.array([1, 1]), I32)
+
+    # ------------------------------------------------------------ launchers
+    def synth(self, groups, B, ldx, F, Bout, vector: bool, or2=None):
+        """Legendre synthesis.  groups: list of [(x1, x2, mode)] * (3 if vector else 1)."""
+        a = _lib.SynArgs()
+        for g, grp in enumerate(groups):
+            for c, (x1, x2, mode) in enumerate(grp):
+                a.in_[g][c].x1 = ptr(x1)
+                a.in_[g][c].x2 = ptr(x2)
+                a.in_[g][c].mode = mode
+        a.ngroups, a.B, a.ldx, a.H, a.Hp = len(groups), B, ldx, self.H, self.Hp
+        a.mp_info, a.lm_start = ptr(self.d["mp_info"]), ptr(self.d["lm_start"])
+        a.synP, a.synD = ptr(self.synP), ptr(self.synD)
+        a.or2 = ptr(or2 if or2 is not None else self.d["or2"])
+        a.F = ptr(F)
+        a.m_stride = 2 * self.T * Bout
+        a.c_stride = self.M * a.m_stride
+        a.Bout = Bout
+        _lib.call("mh_leg_synth", 1 if vector else 0, _lib.ref(a), self.M, stream_handle())
+
+    def analysis(self, F, Bin, B, comps, outS, outT, ldo, vector: bool):
+        """Legendre analysis of F[c][m][t][b] (comp stride = M*T*Bin complex).
+
+        comps: scalar -> list of component indices; vector -> list of (c_vtheta, c_vphi).
+        outS/outT: output tensors (or pointers as ints) per component / pair.
+        """
+        a = _lib.AnaArgs()
+        a.F = ptr(F)
+        a.m_stride = 2 * self.T * Bin
+        a.c_stride = self.M * a.m_stride
+        a.Bin, a.B, a.ncomp = Bin, B, len(comps)
+        for i, c in enumerate(comps):
+            if vector:
+                a.cf2[i], a.cf1[i] = c
+            else:
+                a.cf1[i] = c
+            a.outS[i] = outS[i] if isinstance(outS[i], int) else ptr(outS[i])
+            if vector and outT[i] is not None:
+                a.outT[i] = outT[i] if isinstance(outT[i], int) else ptr(outT[i])
+        a.ldo, a.H, a.Hp = ldo, self.H, self.Hp
+        a.mp_info, a.lm_start = ptr(self.d["mp_info"]), ptr(self.d["lm_start"])
+        a.anaW, a.anaWd = ptr(self.anaW), ptr(self.anaWd)
+        a.tiles, a.ntiles = ptr(self.d["ana_tiles"]), self.lay["n_ana_tiles"]
+        _lib.call("mh_leg_analysis", 1 if vector else 0, _lib.ref(a), stream_handle())
+
+    def fft(self, forward: bool, F, FB, G, ncomp, B, comp_scale=None):
+        """Azimuthal FFT between F[c][m][t][b] (FB columns) and G[c][b][t][phi]."""
+        a = _lib.FftArgs()
+        a.Mh, a.nstage = self.Mh, len(self.radix)
+        for i, r in enumerate(self.radix):
+            a.radix[i] = r
+        a.n_m, a.T, a.B, a.ncomp, a.RB = self.M, self.T, B, ncomp, self.fft_rb
+        a.tw, a.tw2 = ptr(self.d["tw"]), ptr(self.d["tw2"])
+        a.F = ptr(F)
+        a.Fm_stride = 2 * self.T * FB
+        a.Fc_stride = self.M * a.Fm_stride
+        a.FB = FB
+        a.G = ptr(G)
+        a.Gc_stride = B * self.T * self.P
+        a.comp_scale = ptr(comp_scale)
+        a.ost2 = ptr(self.d["ost2_grid"])
+        _lib.call("mh_fft", 1 if forward else 0, _lib.ref(a), stream_handle())
+
+    def fourier_buffer(self, ncomp, B):
+        return torch.empty(ncomp * self.M * self.T * B * 2, dtype=F64, device=self.device)
+
+    def grid_buffer(self, ncomp, B):
+        return torch.empty(ncomp * B * self.T * self.P, dtype=F64, device=self.device)

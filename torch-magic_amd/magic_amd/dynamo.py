@@ -1,0 +1,550 @@
+"""One dynamo run resident on the GPU: state, implicit operators and the CNAB2 step.
+
+:class:`DynamoRun` is the context object behind the reference's module-level
+state (``fields.py``, ``dt_fields.py``, the per-l matrices of ``update_*.py``)
+and step driver (``step_time.py``).  All per-step arithmetic runs in
+libmagic_hip kernels on one HIP stream; the host only launches kernels, reads
+back the 2*n_r_max Courant maxima once per step (the reference's
+``dtrkc.min().item()`` sync, ``step_time.py:1212``) and, when dt changes,
+re-runs the on-device operator rebuild.
+
+Per step (``_one_step_cnab2``, reference ``step_time.py:1602-1637``):
+
+radial loop (``step_time.py:931-1212``)
+  1. Legendre synthesis of s and of the QST groups (B, curl B, u, curl u)
+     into Fourier buffers (MFMA FP64), the l(l+1) and or2 prefactors applied
+     while loading;
+  2. inverse azimuthal FFT to the grid;
+  3. nonlinear products + per-level Courant maxima;
+  4. forward FFT of the 9 product components of the bulk levels;
+  5. Legendre analysis (scalar and spheroidal/toroidal);
+  6. radial derivatives of the two flux terms (row-operator GEMM);
+  7. explicit terms with Coriolis coupling.
+dt decision (``courant.py:130-155``), IMEX weights (``time_scheme.py:56-73``),
+operator rebuild if dt changed (``step_time.py:1530-1548``);
+lm loop (``step_time.py:222-394``, ``update_b.py:557-631``)
+  8. IMEX right-hand sides with boundary rows;
+  9. per-degree solves of S, Z, B, J with the DCT-I and the first two radial
+     derivatives folded into one operator per degree (row-operator GEMM);
+ 10. buoyancy coupling and the l=0 pressure right-hand side;
+ 11. per-degree (w, p) solve with w, dw, ddw, dddw, p, dp folded in;
+ 12. stored old states and implicit terms; explicit-history rotation is a
+     pointer swap.
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import torch
+
+from . import _lib
+from .device import C128, F64, I32, Resolution, ptr, require_gpu, stream_handle
+from .params import Params
+
+_FIELDS = ("s", "ds", "z", "dz", "w", "dw", "ddw", "p", "dp", "b", "db", "ddb", "aj", "dj", "ddj")
+_SYS = ("s", "z", "w", "p", "b", "j")
+
+
+def seeded_noise(p: Params, res: Resolution, seed: int, amp: float) -> np.ndarray:
+    """amp * complex N(0,1) per lm (l >= 1, m = 0 real) times (1 - x^2)^3 in r.
+
+    Drawn on the CPU in FP64 from ``numpy.random.default_rng(seed)`` (real
+    parts for all lm, then imaginary parts) in m-major lm order; the radial
+    shape vanishes at both boundaries so the fixed-entropy rows hold.
+    """
+    rng = np.random.default_rng(seed)
+    L = p.lm_max
+    coef = amp * (rng.standard_normal(L) + 1j * rng.standard_normal(L))
+    m, l = res.maps["m"], res.maps["l"]
+    coef[m == 0] = coef[m == 0].real
+    coef[l == 0] = 0.0
+    r = res.rad["r"]
+    x = 2.0 * r - res.rad["r_cmb"] - res.rad["r_icb"]
+    return coef[:, None] * ((1.0 - x * x) ** 3)[None, :]
+
+
+def conduction_state(p: Params, res: Resolution):
+    """Boussinesq conduction profile (s0, p0) on the radial grid.
+
+    Solves the reference's coupled Chebyshev collocation system
+    (``init_fields.py:108-180``): opr (s'' + 2/r s') = 0, p' = BuoFac rgrav s,
+    s(r_cmb) = 0, s(r_icb) = sqrt(4 pi), p(r_cmb) = 0, with the boundary
+    factor 1/2 on the first/last Chebyshev columns; host LAPACK, run once.
+    """
+    rad = res.rad
+    N = p.n_r_max
+    rn = rad["rnorm"]
+    T, T1, T2 = rad["T"], rad["T1"], rad["T2"]
+    A = np.zeros((2 * N, 2 * N))
+    A[:N, :N] = rn * p.opr * (T2 + (2.0 * rad["or1"])[:, None] * T1)
+    A[N:, :N] = -rn * (p.BuoFac * rad["rgrav"])[:, None] * T
+    A[N:, N:] = rn * T1
+    A[0, :] = 0.0; A[0, :N] = rn * T[0]
+    A[N - 1, :] = 0.0; A[N - 1, :N] = rn * T[N - 1]
+    A[N, :] = 0.0; A[N, N:] = rn * T[0]
+    for c in (0, N - 1, N, 2 * N - 1):
+        A[:, c] *= 0.5
+    rhs = np.zeros(2 * N)
+    rhs[N - 1] = math.sqrt(4.0 * math.pi)
+    fac = 1.0 / np.abs(A).max(axis=1)
+    x = np.linalg.solve(fac[:, None] * A, fac * rhs)
+    C = rad["C"]
+    return C @ x[:N], C @ x[N:]
+
+
+class DynamoRun:
+    """State and step of one Boussinesq Chebyshev CNAB2 dynamo run on the GPU."""
+
+    def __init__(self, cfg, device="cuda", res: Resolution | None = None):
+        self.p = p = cfg if isinstance(cfg, Params) else Params(cfg)
+        p.check_step_supported()
+        self.device = require_gpu(device)
+        self.res = res if res is not None else Resolution(p, self.device)
+        r = self.res
+        L, N, M, T, P = r.L, r.N, r.M, r.T, r.P
+        self.L, self.N = L, N
+        self.Nb = N - 2
+        dev = self.device
+        z = lambda cols=N: torch.zeros(L, cols, dtype=C128, device=dev)
+        self.f = {k: z() for k in _FIELDS}
+        self.tmp = {k: z() for k in ("d2s", "d2z", "dddw")}
+        self.old = {k: z() for k in _SYS}
+        self.impl = {k: z() for k in _SYS}
+        self.expl_cur = {k: z() for k in _SYS}
+        self.expl_prev = {k: z() for k in _SYS}
+        self.rhs = {k: z() for k in ("s", "z", "b", "j")}
+        self.rhs["wp"] = z(2 * N)
+        # radial-loop spectral outputs (boundary columns stay zero)
+        self.nl = {k: z() for k in ("Advr", "Advt", "Advp", "dVSr", "VSt", "VxBr", "VxBt",
+                                    "VxBp", "dVSr_dr", "dVxBt_dr")}
+        # grid-space work (FP64): Fourier buffers and grids
+        self.F_s = r.fourier_buffer(1, N)
+        self.F_v = r.fourier_buffer(3, 4 * N)
+        self.G_s = r.grid_buffer(1, N)
+        self.G_v = r.grid_buffer(3, 4 * N)
+        self.G_o = r.grid_buffer(9, self.Nb)
+        nFo = 9 * M * T * self.Nb * 2
+        self.F_o = self.F_v[:nFo] if nFo <= self.F_v.numel() else r.fourier_buffer(9, self.Nb)
+        self.cfl = torch.zeros(2 * N, dtype=torch.int64, device=dev)
+        self.cfl_host = torch.zeros(2 * N, dtype=torch.int64).pin_memory()
+        self._build_operator_storage()
+        self._build_jobs()
+        self.dt_hist = [0.0, 0.0]
+        self.dtMax = 0.0
+        self.wimp_lin0 = None
+        self.n_rebuild = 0
+        self.time = 0.0
+
+    # ------------------------------------------------------------ operators
+    def _build_operator_storage(self):
+        r, N, dev = self.res, self.N, self.device
+        lp1 = self.p.l_max + 1
+        self.K4 = -(-N // 4) * 4
+        self.K4wp = -(-(2 * N) // 4) * 4
+        self.Rpad3 = -(-(3 * N) // 64) * 64
+        self.Rpad6 = -(-(6 * N) // 64) * 64
+        self.MT = {k: torch.zeros(lp1, self.K4, self.Rpad3, dtype=F64, device=dev)
+                   for k in ("s", "z", "b", "j")}
+        self.MT["wp"] = torch.zeros(lp1, self.K4wp, self.Rpad6, dtype=F64, device=dev)
+        rad = r.rad
+        CT, D1, D2, D3 = rad["CT"], rad["D1"], rad["D2"], rad["D3"]
+        up = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=F64).to(dev)
+        self.Op3 = up(np.concatenate([CT, D1 @ CT, D2 @ CT]))            # (3N, N)
+        self.Opw = up(np.concatenate([CT, D1 @ CT, D2 @ CT, D3 @ CT]))   # (4N, N)
+        self.Opp = up(np.concatenate([CT, D1 @ CT]))                     # (2N, N)
+        self.cheb = {k: up(rad[k]) for k in ("T", "T1", "T2", "T3")}
+        self.mat_nn = torch.empty(lp1, N, N, dtype=F64, device=dev)
+        self.mat_wp = torch.empty(lp1, 2 * N, 2 * N, dtype=F64, device=dev)
+        self.gj_work = torch.empty(lp1 * 2 * (2 * N) * (2 * N), dtype=F64, device=dev)
+        self.gj_scal = torch.empty(lp1 * 2 * 2 * N, dtype=F64, device=dev)
+        self.gj_info = torch.zeros(lp1, dtype=I32, device=dev)
+        # uniform radial-derivative operators (transposed, padded)
+        r2 = rad["r2"]
+
+        def mt(ops):  # list of (N, N) blocks stacked as output rows -> MT[k][r]
+            Op = np.concatenate(ops)
+            R = Op.shape[0]
+            Rp = -(-R // 64) * 64
+            out = np.zeros((self.K4, Rp))
+            out[:N, :R] = Op.T
+            return up(out), R, Rp
+
+        self.MT_d1 = mt([D1])
+        self.MT_d1r2 = mt([D1 * r2[None, :]])
+        self.MT_d12 = mt([D1, D2])
+        self.MT_d123 = mt([D1, D2, D3])
+
+    def _gemm(self, A, sAr, sAc, sAb, B, sBr, sBc, sBb, Cp, sCr, sCc, sCb, M, Nn, K, batch):
+        g = _lib.GemmArgs()
+        g.A, g.B, g.C = A, B, Cp
+        g.sAr, g.sAc, g.sAb, g.sBr, g.sBc, g.sBb, g.sCr, g.sCc, g.sCb = (
+            sAr, sAc, sAb, sBr, sBc, sBb, sCr, sCc, sCb)
+        g.M, g.Nn, g.K, g.batch, g.alpha = M, Nn, K, batch, 1.0
+        _lib.call("mh_gemm_f64", _lib.ref(g), stream_handle())
+
+    def _build_kind(self, kind, out, wl0):
+        p, rad = self.p, self.res.rad
+        a = _lib.BuildArgs()
+        a.R, a.D1, a.D2, a.D3 = (ptr(self.cheb[k]) for k in ("T", "T1", "T2", "T3"))
+        a.or1, a.or2 = ptr(self.res.d["or1"]), ptr(self.res.d["or2"])
+        a.out = ptr(out)
+        a.N, a.l_max, a.n_cheb = self.N, p.l_max, p.n_cheb_max
+        a.ktopv, a.kbotv, a.kind = p.ktopv, p.kbotv, kind
+        a.rnorm, a.wimp, a.opr, a.opm = rad["rnorm"], wl0, p.opr, p.opm
+        _lib.call("mh_build_matrices", _lib.ref(a), stream_handle())
+
+    def build_all_matrices(self, wimp_lin0: float):
+        """Rebuild every per-degree operator for the implicit weight ``wimp_lin0``.
+
+        Device-side replacement of ``build_all_matrices`` (reference
+        ``step_time.py:1530-1548``): assemble, equilibrate and invert the
+        collocation matrices of all degrees at once, then fold the DCT-I and
+        the radial derivatives into the inverses.
+        """
+        N, lp1 = self.N, self.p.l_max + 1
+        st = stream_handle()
+        for kind, key in ((0, "s"), (1, "z"), (2, "b"), (3, "j")):
+            if key in ("b", "j") and not self.p.l_mag:
+                continue
+            self._build_kind(kind, self.mat_nn, wimp_lin0)
+            _lib.call("mh_gj_inverse", ptr(self.mat_nn), ptr(self.mat_nn), ptr(self.gj_work),
+                      ptr(self.gj_scal), N, lp1, 1, ptr(self.gj_info), st)
+            self._check_info(key)
+            MT = self.MT[key]
+            # MT[l][k][r] = sum_j inv_l[j][k] Op3[r][j]
+            self._gemm(ptr(self.mat_nn), 1, N, N * N, ptr(self.Op3), 1, N, 0,
+                       ptr(MT), self.Rpad3, 1, self.K4 * self.Rpad3, N, 3 * N, N, lp1)
+            if key != "s":
+                MT[0].zero_()
+        # coupled (w, p)
+        self._build_kind(4, self.mat_wp, wimp_lin0)
+        _lib.call("mh_gj_inverse", ptr(self.mat_wp), ptr(self.mat_wp), ptr(self.gj_work),
+                  ptr(self.gj_scal), 2 * N, lp1, 2, ptr(self.gj_info), st)
+        self._check_info("wp")
+        MT = self.MT["wp"]
+        sb = self.K4wp * self.Rpad6
+        self._gemm(ptr(self.mat_wp), 1, 2 * N, 4 * N * N, ptr(self.Opw), 1, N, 0,
+                   ptr(MT), self.Rpad6, 1, sb, 2 * N, 4 * N, N, lp1)
+        self._gemm(ptr(self.mat_wp) + 8 * N * 2 * N, 1, 2 * N, 4 * N * N, ptr(self.Opp), 1, N, 0,
+                   ptr(MT) + 8 * 4 * N, self.Rpad6, 1, sb, 2 * N, 2 * N, N, lp1)
+        MT[0].zero_()
+        # l = 0 pressure: p(l=0) from the p0 system acting on the pressure half
+        p0 = self.mat_nn[0]
+        self._build_kind(5, p0, wimp_lin0)
+        _lib.call("mh_gj_inverse", ptr(p0), ptr(p0), ptr(self.gj_work), ptr(self.gj_scal), N, 1, 1,
+                  ptr(self.gj_info), st)
+        self._check_info("p0", n=1)
+        self._gemm(ptr(p0), 1, N, 0, ptr(self.Opp), 1, N, 0,
+                   ptr(MT) + 8 * (N * self.Rpad6 + 4 * N), self.Rpad6, 1, 0, N, 2 * N, N, 1)
+        self.wimp_lin0 = wimp_lin0
+        self.n_rebuild += 1
+
+    def _check_info(self, name, n=None):
+        info = self.gj_info[: (n or self.p.l_max + 1)].cpu()
+        bad = torch.nonzero(info).flatten()
+        if bad.numel():
+            l = int(bad[0])
+            raise AssertionError(f"Singular {name} matrix for l={l}, info={int(info[l])}")
+
+    # ------------------------------------------------------------ job tables
+    def _build_jobs(self):
+        """Row-operator job descriptors and tile lists (pointers are fixed per run)."""
+        r, N, L, lp1 = self.res, self.N, self.L, self.p.l_max + 1
+        rows = r.d["l_rows"]
+        roff = r.maps["l_rows_off"]
+
+        def job(X, ldx, rows_ptr, nrows, MT, K, R, Rpad, Y, ldy, rseg):
+            j = _lib.RowJob()
+            j.X, j.rows, j.MT = X, rows_ptr, MT
+            for i, y in enumerate(Y):
+                j.Y[i] = y
+            j.ldx, j.ldy, j.K, j.R, j.Rpad, j.rseg, j.nrows = ldx, ldy, K, R, Rpad, rseg, nrows
+            return j
+
+        def tiles_for(jobs):
+            t = []
+            for ji, j in enumerate(jobs):
+                for r0 in range(0, j.nrows, 32):
+                    for c0 in range(0, j.R, 64):
+                        t.append((ji, r0, c0))
+            return t
+
+        def upload(jobs):
+            arr = (_lib.RowJob * len(jobs))(*jobs)
+            raw = bytes(arr)
+            dj = torch.frombuffer(bytearray(raw), dtype=torch.uint8).to(self.device)
+            tl = torch.as_tensor(np.asarray(tiles_for(jobs), np.int32).reshape(-1)).to(self.device)
+            return dj, tl, len(tiles_for(jobs))
+
+        f, tmp = self.f, self.tmp
+        # lm loop, stage 1: S, Z, B, J per degree
+        jobs = []
+        systems = [("s", self.rhs["s"], (f["s"], f["ds"], tmp["d2s"])),
+                   ("z", self.rhs["z"], (f["z"], f["dz"], tmp["d2z"]))]
+        if self.p.l_mag:
+            systems += [("b", self.rhs["b"], (f["b"], f["db"], f["ddb"])),
+                        ("j", self.rhs["j"], (f["aj"], f["dj"], f["ddj"]))]
+        for key, X, Y in systems:
+            MT = self.MT[key]
+            for l in range(lp1):
+                jobs.append(job(ptr(X), N, ptr(rows) + 4 * int(roff[l]), l + 1,
+                                ptr(MT) + 8 * l * self.K4 * self.Rpad3, N, 3 * N, self.Rpad3,
+                                [ptr(y) for y in Y], N, N))
+        self.jobs1 = upload(jobs)
+        # lm loop, stage 2: coupled (w, p) per degree
+        jobs = []
+        Y = [f["w"], f["dw"], f["ddw"], tmp["dddw"], f["p"], f["dp"]]
+        for l in range(lp1):
+            jobs.append(job(ptr(self.rhs["wp"]), 2 * N, ptr(rows) + 4 * int(roff[l]), l + 1,
+                            ptr(self.MT["wp"]) + 8 * l * self.K4wp * self.Rpad6, 2 * N, 6 * N,
+                            self.Rpad6, [ptr(y) for y in Y], N, N))
+        self.jobs2 = upload(jobs)
+        # radial loop: derivatives of the flux terms
+        nl = self.nl
+        jobs = [job(ptr(nl["dVSr"]), N, 0, L, ptr(self.MT_d1[0]), N, N, self.MT_d1[2],
+                    [ptr(nl["dVSr_dr"])], N, N)]
+        if self.p.l_mag:
+            jobs.append(job(ptr(nl["VxBt"]), N, 0, L, ptr(self.MT_d1r2[0]), N, N,
+                            self.MT_d1r2[2], [ptr(nl["dVxBt_dr"])], N, N))
+        self.jobs_td = upload(jobs)
+        # set-up: derivatives of the initial fields
+        jobs = [job(ptr(f["s"]), N, 0, L, ptr(self.MT_d12[0]), N, 2 * N, self.MT_d12[2],
+                    [ptr(f["ds"]), ptr(tmp["d2s"])], N, N),
+                job(ptr(f["z"]), N, 0, L, ptr(self.MT_d12[0]), N, 2 * N, self.MT_d12[2],
+                    [ptr(f["dz"]), ptr(tmp["d2z"])], N, N),
+                job(ptr(f["w"]), N, 0, L, ptr(self.MT_d123[0]), N, 3 * N, self.MT_d123[2],
+                    [ptr(f["dw"]), ptr(f["ddw"]), ptr(tmp["dddw"])], N, N),
+                job(ptr(f["p"]), N, 0, L, ptr(self.MT_d1[0]), N, N, self.MT_d1[2],
+                    [ptr(f["dp"])], N, N)]
+        if self.p.l_mag:
+            jobs += [job(ptr(f["b"]), N, 0, L, ptr(self.MT_d12[0]), N, 2 * N, self.MT_d12[2],
+                         [ptr(f["db"]), ptr(f["ddb"])], N, N),
+                     job(ptr(f["aj"]), N, 0, L, ptr(self.MT_d12[0]), N, 2 * N, self.MT_d12[2],
+                         [ptr(f["dj"]), ptr(f["ddj"])], N, N)]
+        self.jobs_setup = upload(jobs)
+
+    def _rowop(self, jt):
+        dj, tl, nt = jt
+        _lib.call("mh_rowop", ptr(dj), ptr(tl), nt, stream_handle())
+
+    # ------------------------------------------------------------ coefficient structs
+    def _lmcoef(self):
+        d = self.res.d
+        c = _lib.LmCoef()
+        c.l, c.m, c.dLh = ptr(d["l"]), ptr(d["m"]), ptr(d["dLh"])
+        c.dTh2A, c.dTh2S, c.dTh3A, c.dTh3S = (ptr(d[k]) for k in ("dTh2A", "dTh2S", "dTh3A", "dTh3S"))
+        c.dTh4A, c.dTh4S = ptr(d["dTh4A"]), ptr(d["dTh4S"])
+        c.l_max, c.L = self.p.l_max, self.L
+        return c
+
+    def _radcoef(self):
+        d = self.res.d
+        c = _lib.RadCoef()
+        c.or1, c.or2, c.or3, c.or4 = (ptr(d[k]) for k in ("or1", "or2", "or3", "or4"))
+        c.r2, c.rgrav, c.N = ptr(d["r2"]), ptr(d["rgrav"]), self.N
+        return c
+
+    # ------------------------------------------------------------ init
+    def initialize_fields(self, noise_amp: float | None = None, noise_seed: int | None = None):
+        """``initialize_fields`` (reference ``init_fields.py:470-532``) + optional noise."""
+        p, r = self.p, self.res
+        L, N = self.L, self.N
+        for t in self.f.values():
+            t.zero_()
+        s = np.zeros((L, N), np.complex128)
+        pr = np.zeros((L, N), np.complex128)
+        b = np.zeros((L, N), np.complex128)
+        aj = np.zeros((L, N), np.complex128)
+        rr, r_cmb, r_icb = r.rad["r"], r.rad["r_cmb"], r.rad["r_icb"]
+        lm = lambda l, m: int(r.maps["lm_start"][m] + (l - m))
+        if p.l_heat:
+            s0, p0 = conduction_state(p, r)
+            s[0], pr[0] = s0, p0
+            if p.init_s1 >= 100:
+                l, m = p.init_s1 // 100, p.init_s1 % 100
+                x = 2.0 * rr - r_cmb - r_icb
+                s[lm(l, m)] += p.amp_s1 * (1.0 - 3.0 * x ** 2 + 3.0 * x ** 4 - x ** 6)
+        if p.l_mag:
+            b_pol = p.amp_b1 * math.sqrt(3.0 * math.pi) / 4.0
+            b_tor = -4.0 / 3.0 * p.amp_b1 * math.sqrt(math.pi / 5.0)
+            b[lm(1, 0)] = b_pol * (rr ** 3 - 4.0 / 3.0 * r_cmb * rr ** 2 + r_icb ** 4 / (3.0 * rr))
+            aj[lm(2, 0)] = b_tor * rr * np.sin(math.pi * (rr - r_icb))
+        amp = p.noise_amp if noise_amp is None else noise_amp
+        seed = p.noise_seed if noise_seed is None else noise_seed
+        if amp:
+            s = s + seeded_noise(p, r, seed, amp)
+        for k, v in (("s", s), ("p", pr), ("b", b), ("aj", aj)):
+            self.f[k].copy_(torch.from_numpy(v))
+
+    def setup_initial_state(self):
+        """Derivatives, old states and implicit terms from the fields (``step_time.py:398-617``)."""
+        for d in (self.old, self.impl, self.expl_cur, self.expl_prev):
+            for t in d.values():
+                t.zero_()
+        self._rowop(self.jobs_setup)
+        self._impl()
+        # the reference's set-up fills the magnetic implicit terms on the
+        # interior levels only (step_time.py:582-595)
+        for k in ("b", "j"):
+            self.impl[k][:, 0] = 0.0
+            self.impl[k][:, -1] = 0.0
+
+    def initialize_dt(self, dt: float):
+        """dt <= intfac * E for rotating runs (``step_time.py:1554-1571``)."""
+        if not self.p.ek < 0.0:
+            dt = min(dt, self.p.int_fac * self.p.ek)
+        self.dt_hist = [dt, dt]
+        self.dtMax = dt
+
+    # ------------------------------------------------------------ kernels
+    def _impl(self):
+        a = _lib.ImplArgs()
+        a.lc, a.rc = self._lmcoef(), self._radcoef()
+        f, t = self.f, self.tmp
+        for k, v in (("s", f["s"]), ("ds", f["ds"]), ("d2s", t["d2s"]), ("z", f["z"]),
+                     ("d2z", t["d2z"]), ("w", f["w"]), ("dw", f["dw"]), ("ddw", f["ddw"]),
+                     ("dddw", t["dddw"]), ("p", f["p"]), ("dp", f["dp"]), ("b", f["b"]),
+                     ("ddb", f["ddb"]), ("aj", f["aj"]), ("ddj", f["ddj"])):
+            setattr(a, k, ptr(v))
+        for k in _SYS:
+            setattr(a, "old_" + k, ptr(self.old[k]))
+            setattr(a, "impl_" + k, ptr(self.impl[k]))
+        a.opr, a.opm, a.BuoFac = self.p.opr, self.p.opm, self.p.BuoFac
+        a.l_mag, a.lm_l0 = int(self.p.l_mag), 0
+        _lib.call("mh_impl", _lib.ref(a), stream_handle())
+
+    def radial_loop(self):
+        """Explicit terms of the current state; returns (dtrkc_min, dthkc_min)."""
+        r, f, N, Nb = self.res, self.f, self.N, self.Nb
+        # 1. Legendre synthesis
+        r.synth([[(f["s"], None, 0)]], N, N, self.F_s, N, vector=False)
+        groups = [[(f["b"], None, 1), (f["db"], None, 0), (f["aj"], None, 0)],
+                  [(f["aj"], None, 1), (f["dj"], None, 0), (f["ddb"], f["b"], 2)],
+                  [(f["w"], None, 1), (f["dw"], None, 0), (f["z"], None, 0)],
+                  [(f["z"], None, 1), (f["dz"], None, 0), (f["ddw"], f["w"], 2)]]
+        r.synth(groups, N, N, self.F_v, 4 * N, vector=True)
+        # 2. inverse FFT
+        r.fft(False, self.F_s, N, self.G_s, 1, N)
+        r.fft(False, self.F_v, 4 * N, self.G_v, 3, 4 * N)
+        # 3. nonlinear products + Courant maxima
+        a = _lib.NlArgs()
+        a.Gs, a.Gv, a.Gv_c_stride = ptr(self.G_s), ptr(self.G_v), 4 * N * r.T * r.P
+        a.Go, a.Go_c_stride = ptr(self.G_o), Nb * r.T * r.P
+        a.N, a.T, a.P = N, r.T, r.P
+        d = r.d
+        a.or2, a.or4, a.ost2 = ptr(d["or2"]), ptr(d["or4"]), ptr(d["ost2_grid"])
+        a.valri2, a.valhi2 = ptr(d["valri2"]), ptr(d["valhi2"])
+        a.LFfac, a.cf2, a.af2 = self.p.LFfac, self.p.cour_fac ** 2, self.p.alf_fac ** 2
+        a.l_mag, a.alf_damp = int(self.p.l_mag), int(self.p.l_cour_alf_damp)
+        a.cfl = ptr(self.cfl)
+        _lib.call("mh_get_nl", _lib.ref(a), stream_handle())
+        # 4. forward FFT of the bulk products
+        r.fft(True, self.F_o, Nb, self.G_o, 9, Nb, comp_scale=d["comp_scale_v"])
+        # 5. Legendre analysis into (L, N) arrays at columns 1..N-2
+        nl = self.nl
+        off = 16  # one complex column
+        r.analysis(self.F_o, Nb, Nb, [0, 1, 2],
+                   [ptr(nl["Advr"]) + off, ptr(nl["dVSr"]) + off, ptr(nl["VxBr"]) + off],
+                   [None, None, None], N, vector=False)
+        r.analysis(self.F_o, Nb, Nb, [(3, 4), (5, 6), (7, 8)],
+                   [ptr(nl["Advt"]) + off, ptr(nl["VSt"]) + off, ptr(nl["VxBt"]) + off],
+                   [ptr(nl["Advp"]) + off, None, ptr(nl["VxBp"]) + off], N, vector=True)
+        # 6. radial derivatives of the fluxes
+        self._rowop(self.jobs_td)
+        # 7. explicit terms
+        t = _lib.TdArgs()
+        t.lc, t.rc = self._lmcoef(), self._radcoef()
+        for k in ("Advr", "Advt", "Advp", "VSt", "VxBr", "VxBp", "dVSr_dr", "dVxBt_dr"):
+            setattr(t, k, ptr(nl[k]))
+        t.w, t.dw, t.z = ptr(f["w"]), ptr(f["dw"]), ptr(f["z"])
+        e = self.expl_cur
+        t.ew, t.ez, t.ep = ptr(e["w"]), ptr(e["z"]), ptr(e["p"])
+        t.es, t.eb, t.ej = ptr(e["s"]), ptr(e["b"]), ptr(e["j"])
+        t.CorFac, t.l_mag = self.p.CorFac, int(self.p.l_mag)
+        _lib.call("mh_td", _lib.ref(t), stream_handle())
+        # Courant limits (host decision, like the reference's .item())
+        self.cfl_host.copy_(self.cfl, non_blocking=True)
+        torch.cuda.current_stream().synchronize()
+        v = self.cfl_host.numpy().view(np.float64)
+        rad = r.rad
+        with np.errstate(divide="ignore"):
+            dtrkc = np.where(v[:N] > 0, np.sqrt(rad["delxr2"] / np.where(v[:N] > 0, v[:N], 1.0)), 1e10)
+            dthkc = np.where(v[N:] > 0, np.sqrt(rad["delxh2"] / np.where(v[N:] > 0, v[N:], 1.0)), 1e10)
+        return float(dtrkc.min()), float(dthkc.min())
+
+    def lm_loop(self):
+        """Implicit solves and post-processing (``_fused_lm_loop_fast`` + insulating B)."""
+        p = self.p
+        dt1, dt2 = self.dt_hist
+        wl0, wl1 = p.alpha * dt1, (1.0 - p.alpha) * dt1
+        we0, we1 = (1.0 + 0.5 * dt1 / dt2) * dt1, -0.5 * dt1 * dt1 / dt2
+        a = _lib.RhsArgs()
+        for i, k in enumerate(_SYS):
+            a.sys[i].old, a.sys[i].impl = ptr(self.old[k]), ptr(self.impl[k])
+            a.sys[i].expl0, a.sys[i].expl1 = ptr(self.expl_cur[k]), ptr(self.expl_prev[k])
+        a.rs, a.rz, a.rb, a.rj, a.rwp = (ptr(self.rhs[k]) for k in ("s", "z", "b", "j", "wp"))
+        a.m, a.L, a.N = ptr(self.res.d["m"]), self.L, self.N
+        a.wimp, a.wl1, a.we0, a.we1 = 1.0, wl1, we0, we1
+        a.bots_l0, a.l_mag = math.sqrt(4.0 * math.pi), int(p.l_mag)
+        _lib.call("mh_rhs", _lib.ref(a), stream_handle())
+        self._rowop(self.jobs1)
+        bq = _lib.BuoyArgs()
+        bq.rwp, bq.s, bq.ew0 = ptr(self.rhs["wp"]), ptr(self.f["s"]), ptr(self.expl_cur["w"])
+        bq.rgrav, bq.m, bq.L, bq.N = ptr(self.res.d["rgrav"]), ptr(self.res.d["m"]), self.L, self.N
+        bq.fac, bq.BuoFac, bq.lm_l0 = wl0 * p.BuoFac, p.BuoFac, 0
+        _lib.call("mh_buoy", _lib.ref(bq), stream_handle())
+        self._rowop(self.jobs2)
+        self._impl()
+        self.expl_cur, self.expl_prev = self.expl_prev, self.expl_cur
+
+    def one_step(self, n_time_step: int, dt: float) -> float:
+        """One CNAB2 step (reference ``step_time.py:1574-1637``); returns the dt used."""
+        dtrkc, dthkc = self.radial_loop()
+        l_new_dt, dt_new = dt_courant(dt, self.dtMax, dtrkc, dthkc)
+        self.dt_hist = [dt_new, self.dt_hist[0]]
+        wl0 = self.p.alpha * dt_new
+        if n_time_step == 1 or l_new_dt or self.wimp_lin0 != wl0:
+            self.build_all_matrices(wl0)
+        self.lm_loop()
+        self.time += dt_new
+        return dt_new
+
+    # ------------------------------------------------------------ diagnostics
+    def _energy_out(self, by_m: bool):
+        out = torch.zeros(4, self.p.l_max + 1, dtype=F64, device=self.device)
+        a = _lib.EnergyArgs()
+        f = self.f
+        a.w, a.dw, a.z, a.b, a.db, a.aj = (ptr(f[k]) for k in ("w", "dw", "z", "b", "db", "aj"))
+        a.lm_start = ptr(self.res.d["lm_start"])
+        a.or2, a.qw, a.out = ptr(self.res.d["or2"]), ptr(self.res.d["q"]), ptr(out)
+        a.l_max, a.N, a.by_m, a.l_mag, a.LFfac = self.p.l_max, self.N, int(by_m), int(self.p.l_mag), self.p.LFfac
+        _lib.call("mh_energy", _lib.ref(a), stream_handle())
+        return out
+
+    def energies(self):
+        """(e_kin_pol, e_kin_tor, e_mag_pol, e_mag_tor) (``output.py:111-193``)."""
+        e = self._energy_out(False).cpu().numpy()
+        return tuple(float(x) for x in e.sum(axis=1))
+
+    def spectra(self):
+        """Per-degree and per-order kinetic and magnetic energy spectra."""
+        el = self._energy_out(False).cpu().numpy()
+        em = self._energy_out(True).cpu().numpy()
+        return dict(kin_l=el[0] + el[1], mag_l=el[2] + el[3], kin_m=em[0] + em[1],
+                    mag_m=em[2] + em[3])
+
+
+def dt_courant(dt, dtMax, dtrkc_min, dthkc_min):
+    """Hysteresis dt decision (reference ``courant.py:130-155``)."""
+    dt_fac = 2.0
+    dtMin = min(dtrkc_min, dthkc_min)
+    dt_2 = min(0.5 * (1.0 / dt_fac + 1.0) * dtMin, dtMax)
+    if dt > dtMax:
+        return True, dtMax
+    if dt > dtMin:
+        return True, dt_2
+    if dt_fac * dt < dtMin and dt < dtMax:
+        return True, dt_2
+    return False, dt
