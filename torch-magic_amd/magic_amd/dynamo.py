@@ -136,6 +136,38 @@ class DynamoRun:
         self.wimp_lin0 = None
         self.n_rebuild = 0
         self.time = 0.0
+        self.timers = None   # {name: [(start_event, end_event), ...]} while timing
+
+    # ------------------------------------------------------------ timing
+    def enable_timers(self, on: bool = True):
+        """Record HIP events around each kernel section (on the launch stream)."""
+        self.timers = {} if on else None
+
+    def _t(self, name):
+        run = self
+
+        class _Sec:
+            def __enter__(self_):
+                if run.timers is not None:
+                    self_.ev = (torch.cuda.Event(enable_timing=True),
+                                torch.cuda.Event(enable_timing=True))
+                    self_.ev[0].record()
+
+            def __exit__(self_, *exc):
+                if run.timers is not None:
+                    self_.ev[1].record()
+                    run.timers.setdefault(name, []).append(self_.ev)
+                return False
+        return _Sec()
+
+    def timer_summary(self):
+        """{name: (launch count, mean ms)} of the recorded sections (synchronises)."""
+        torch.cuda.synchronize()
+        out = {}
+        for k, evs in (self.timers or {}).items():
+            ms = [a.elapsed_time(b) for a, b in evs]
+            out[k] = (len(ms), float(np.mean(ms)))
+        return out
 
     # ------------------------------------------------------------ operators
     def _build_operator_storage(self):
@@ -419,15 +451,18 @@ class DynamoRun:
         """Explicit terms of the current state; returns (dtrkc_min, dthkc_min)."""
         r, f, N, Nb = self.res, self.f, self.N, self.Nb
         # 1. Legendre synthesis
-        r.synth([[(f["s"], None, 0)]], N, N, self.F_s, N, vector=False)
+        with self._t("leg_syn_scal"):
+            r.synth([[(f["s"], None, 0)]], N, N, self.F_s, N, vector=False)
         groups = [[(f["b"], None, 1), (f["db"], None, 0), (f["aj"], None, 0)],
                   [(f["aj"], None, 1), (f["dj"], None, 0), (f["ddb"], f["b"], 2)],
                   [(f["w"], None, 1), (f["dw"], None, 0), (f["z"], None, 0)],
                   [(f["z"], None, 1), (f["dz"], None, 0), (f["ddw"], f["w"], 2)]]
-        r.synth(groups, N, N, self.F_v, 4 * N, vector=True)
+        with self._t("leg_syn_vec"):
+            r.synth(groups, N, N, self.F_v, 4 * N, vector=True)
         # 2. inverse FFT
-        r.fft(False, self.F_s, N, self.G_s, 1, N)
-        r.fft(False, self.F_v, 4 * N, self.G_v, 3, 4 * N)
+        with self._t("fft_inv"):
+            r.fft(False, self.F_s, N, self.G_s, 1, N)
+            r.fft(False, self.F_v, 4 * N, self.G_v, 3, 4 * N)
         # 3. nonlinear products + Courant maxima
         a = _lib.NlArgs()
         a.Gs, a.Gv, a.Gv_c_stride = ptr(self.G_s), ptr(self.G_v), 4 * N * r.T * r.P
@@ -439,20 +474,25 @@ class DynamoRun:
         a.LFfac, a.cf2, a.af2 = self.p.LFfac, self.p.cour_fac ** 2, self.p.alf_fac ** 2
         a.l_mag, a.alf_damp = int(self.p.l_mag), int(self.p.l_cour_alf_damp)
         a.cfl = ptr(self.cfl)
-        _lib.call("mh_get_nl", _lib.ref(a), stream_handle())
+        with self._t("get_nl"):
+            _lib.call("mh_get_nl", _lib.ref(a), stream_handle())
         # 4. forward FFT of the bulk products
-        r.fft(True, self.F_o, Nb, self.G_o, 9, Nb, comp_scale=d["comp_scale_v"])
+        with self._t("fft_fwd"):
+            r.fft(True, self.F_o, Nb, self.G_o, 9, Nb, comp_scale=d["comp_scale_v"])
         # 5. Legendre analysis into (L, N) arrays at columns 1..N-2
         nl = self.nl
         off = 16  # one complex column
-        r.analysis(self.F_o, Nb, Nb, [0, 1, 2],
-                   [ptr(nl["Advr"]) + off, ptr(nl["dVSr"]) + off, ptr(nl["VxBr"]) + off],
-                   [None, None, None], N, vector=False)
-        r.analysis(self.F_o, Nb, Nb, [(3, 4), (5, 6), (7, 8)],
-                   [ptr(nl["Advt"]) + off, ptr(nl["VSt"]) + off, ptr(nl["VxBt"]) + off],
-                   [ptr(nl["Advp"]) + off, None, ptr(nl["VxBp"]) + off], N, vector=True)
+        with self._t("leg_ana_scal"):
+            r.analysis(self.F_o, Nb, Nb, [0, 1, 2],
+                       [ptr(nl["Advr"]) + off, ptr(nl["dVSr"]) + off, ptr(nl["VxBr"]) + off],
+                       [None, None, None], N, vector=False)
+        with self._t("leg_ana_vec"):
+            r.analysis(self.F_o, Nb, Nb, [(3, 4), (5, 6), (7, 8)],
+                       [ptr(nl["Advt"]) + off, ptr(nl["VSt"]) + off, ptr(nl["VxBt"]) + off],
+                       [ptr(nl["Advp"]) + off, None, ptr(nl["VxBp"]) + off], N, vector=True)
         # 6. radial derivatives of the fluxes
-        self._rowop(self.jobs_td)
+        with self._t("rowop_dr"):
+            self._rowop(self.jobs_td)
         # 7. explicit terms
         t = _lib.TdArgs()
         t.lc, t.rc = self._lmcoef(), self._radcoef()
@@ -463,7 +503,8 @@ class DynamoRun:
         t.ew, t.ez, t.ep = ptr(e["w"]), ptr(e["z"]), ptr(e["p"])
         t.es, t.eb, t.ej = ptr(e["s"]), ptr(e["b"]), ptr(e["j"])
         t.CorFac, t.l_mag = self.p.CorFac, int(self.p.l_mag)
-        _lib.call("mh_td", _lib.ref(t), stream_handle())
+        with self._t("td"):
+            _lib.call("mh_td", _lib.ref(t), stream_handle())
         # Courant limits (host decision, like the reference's .item())
         self.cfl_host.copy_(self.cfl, non_blocking=True)
         torch.cuda.current_stream().synchronize()
@@ -488,15 +529,20 @@ class DynamoRun:
         a.m, a.L, a.N = ptr(self.res.d["m"]), self.L, self.N
         a.wimp, a.wl1, a.we0, a.we1 = 1.0, wl1, we0, we1
         a.bots_l0, a.l_mag = math.sqrt(4.0 * math.pi), int(p.l_mag)
-        _lib.call("mh_rhs", _lib.ref(a), stream_handle())
-        self._rowop(self.jobs1)
+        with self._t("rhs"):
+            _lib.call("mh_rhs", _lib.ref(a), stream_handle())
+        with self._t("rowop_szbj"):
+            self._rowop(self.jobs1)
         bq = _lib.BuoyArgs()
         bq.rwp, bq.s, bq.ew0 = ptr(self.rhs["wp"]), ptr(self.f["s"]), ptr(self.expl_cur["w"])
         bq.rgrav, bq.m, bq.L, bq.N = ptr(self.res.d["rgrav"]), ptr(self.res.d["m"]), self.L, self.N
         bq.fac, bq.BuoFac, bq.lm_l0 = wl0 * p.BuoFac, p.BuoFac, 0
-        _lib.call("mh_buoy", _lib.ref(bq), stream_handle())
-        self._rowop(self.jobs2)
-        self._impl()
+        with self._t("buoy"):
+            _lib.call("mh_buoy", _lib.ref(bq), stream_handle())
+        with self._t("rowop_wp"):
+            self._rowop(self.jobs2)
+        with self._t("impl"):
+            self._impl()
         self.expl_cur, self.expl_prev = self.expl_prev, self.expl_cur
 
     def one_step(self, n_time_step: int, dt: float) -> float:
