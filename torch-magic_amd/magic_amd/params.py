@@ -108,6 +108,9 @@ class Params:
         self.n_m = self.l_max + 1
         self.n_phi = smallest_235(2 * ((30 * self.l_max) // 20))
         self.n_theta = self.n_phi // 2
+        if self.n_theta % 2:
+            raise ValueError(f"l_max={self.l_max} gives an odd number of latitudes "
+                             f"({self.n_theta}); the N/S paired grid needs an even count")
         self.lm_max = (self.l_max + 1) * (self.l_max + 2) // 2
         self.l_mag = self.mode not in (1, 7)
         self.l_heat = self.mode != 7
