@@ -55,6 +55,9 @@ def section_work(p, N):
         "fft_inv": ("hbm", 13.0 * N * T * (M * 16.0 + P * 8.0)),
         "fft_fwd": ("hbm", 9.0 * Nb * T * (P * 8.0 + M * 16.0)),
         "get_nl": ("hbm", (13.0 * N + 9.0 * Nb) * T * P * 8.0),
+        # fused grid step: Fourier rows in (13 components, 6 on the 2 boundary
+        # levels) and out (9 components, bulk levels), M complex values each
+        "grid_step": ("hbm", ((13.0 * Nb + 6.0 * 2) + 9.0 * Nb) * T * M * 16.0),
     }
 
 

@@ -71,7 +71,7 @@ typedef struct {
   int pad0;
   double* outS[3];
   double* outT[3];
-  int ldo, H, Hp, pad1;
+  int ldo, H, Hp, n_m; /* n_m: number of azimuthal orders */
   const int* mp_info;
   const int* lm_start;
   const double* anaW;
@@ -127,6 +127,29 @@ typedef struct {
 /* Nonlinear products and per-level Courant maxima: replaces get_nl
  * (get_nl.py:30-87) and the reductions of courant_check (courant.py:25-127). */
 int mh_get_nl(const MhNlArgs* args, void* stream);
+
+typedef struct {
+  const double* Fs; /* s [1][M][T][N] complex (Fourier layout F[c][m][t][b]) */
+  const double* Fv; /* [3][M][T][4N] complex, batch g*N + r, groups B, curl B, u, curl u */
+  double* Fo;       /* [9][M][T][N-2] complex: Advr VSr VxBr Advt Advp VSt VSp VxBt VxBp */
+  int N, T, M, Mh;
+  int generic; /* 1: run the runtime-plan kernel even where a compiled plan exists */
+  int pad;
+  const double* tw;  /* exp(-2 pi i k/Mh) */
+  const double* tw2; /* exp(-2 pi i k/(2Mh)) */
+  const double *or2, *or4, *ost2, *valri2, *valhi2;
+  double LFfac, cf2, af2;
+  int l_mag, alf_damp;
+  unsigned long long* cfl; /* 2N per-level maxima (bit patterns of doubles) */
+} MhGridArgs;
+
+/* Fused grid-space step, one workgroup per (theta row, RB radial levels):
+ * inverse azimuthal FFTs of the 13 synthesised components in LDS, nonlinear
+ * products and Courant maxima in registers, forward FFTs of the 9 products;
+ * the grid never touches HBM.  Replaces irfft (sht.py:341,484-486), get_nl
+ * (get_nl.py:30-87), courant_check (courant.py:25-127) and rfft with the
+ * 1/sin^2 scaling (sht.py:371,597-603) of the radial loop. */
+int mh_grid_step(const MhGridArgs* args, void* stream);
 
 /* ------------------------------------------------------------ row operators */
 typedef struct {

@@ -23,9 +23,9 @@ def _load(name):
     return case, np.load(os.path.join(GOLD, f"traj_{name}.npz"))
 
 
-def _run(case):
+def _run(case, fused=True):
     from magic_amd.dynamo import DynamoRun
-    run = DynamoRun(case["cfg"])
+    run = DynamoRun(case["cfg"], fused_grid=fused)
     nz = case["noise"]
     if nz:
         run.initialize_fields(noise_amp=nz["amp"], noise_seed=nz["seed"])
@@ -40,10 +40,11 @@ def _rel(a, b):
     return np.abs(a - b).max() / np.abs(b).max()
 
 
-@pytest.mark.parametrize("name", ["bench16", "c1"])
-def test_step_matches_reference_trajectory(name):
+@pytest.mark.parametrize("name,fused", [("bench16", True), ("bench16", False), ("c1", True)])
+def test_step_matches_reference_trajectory(name, fused):
+    """fused: one grid-space kernel; otherwise separate FFT / products / FFT kernels."""
     case, g = _load(name)
-    run = _run(case)
+    run = _run(case, fused)
     for n in range(1, case["steps"] + 1):
         dt = run.one_step(n, case["dt"])
         assert dt == pytest.approx(g["dt"][n - 1], rel=1e-12)
@@ -66,3 +67,78 @@ def test_first_step_explicit_terms():
         got = run.expl_prev[k].cpu().numpy()
         scale = max(np.abs(ref).max(), 1e-300)
         assert np.abs(got - ref).max() / scale < 1e-10 or np.abs(ref).max() == 0.0, k
+
+
+_GRID_CFG = dict(ek=1e-3, ra=1e5, pr=1.0, prmag=5.0, radratio=0.35)
+_SPEC = dict(Advr="AdvrLM", Advt="AdvtLM", Advp="AdvpLM", dVSr="dVSrLM", VSt="VStLM",
+             VxBr="VxBrLM", VxBt="VxBtLM", VxBp="VxBpLM")
+
+
+def _random_fields(shape, seed):
+    from magic_amd.dynamo import _FIELDS
+    rng = np.random.default_rng(seed)
+    return {k: rng.standard_normal(shape) + 1j * rng.standard_normal(shape) for k in _FIELDS}
+
+
+def _radial_loop_outputs(l_max, mode, seed=11):
+    """One radial loop on random spectral fields; mode: fused, generic or separate."""
+    import torch
+    from magic_amd.dynamo import DynamoRun
+    run = DynamoRun(dict(_GRID_CFG, l_max=l_max, n_r_max=9), fused_grid=(mode != "separate"))
+    run.grid_generic = mode == "generic"
+    for k, a in _random_fields(tuple(run.f["s"].shape), seed).items():
+        run.f[k].copy_(torch.as_tensor(a))
+    cour = run.radial_loop()
+    torch.cuda.synchronize()
+    return {k: run.nl[k].cpu().numpy() for k in _SPEC}, np.array(cour)
+
+
+_VECTOR_OF = dict(Advr="Adv", Advt="Adv", Advp="Adv", dVSr="VS", VSt="VS",
+                  VxBr="VxB", VxBt="VxB", VxBp="VxB")
+
+
+def _rel_errs(out, ref):
+    """Max error of each output relative to the largest coefficient of its vector
+    (advection, entropy flux, induction): on random fields one component can be
+    far smaller than the others through cancellation."""
+    scale = {}
+    for k, g in _VECTOR_OF.items():
+        scale[g] = max(scale.get(g, 1e-300), np.abs(ref[k]).max())
+    return {k: float(np.abs(out[k] - ref[k]).max() / scale[_VECTOR_OF[k]]) for k in ref}
+
+
+def test_grid_step_matches_oracle_on_random_fields():
+    """All three grid-space paths (fused with compiled FFT plan, fused with the
+    run-time plan, separate kernels) against the oracle's radial loop
+    (irfft / get_nl / rfft of step_time.py:931-1212) on O(1) random spectral
+    fields at l_max=63 (n_phi/2 = 96), through the same Legendre transforms."""
+    from oracle.dynamo import OracleRun
+    orun = OracleRun(dict(_GRID_CFG, l_max=63, n_r_max=9))
+    fields = _random_fields((orun.prm.lm_max, 9), 11)
+    for k, a in fields.items():
+        setattr(orun, k, a.copy())
+    ocour = np.array(orun.radial_loop())
+    ref = {k: orun.last_spec[v] for k, v in _SPEC.items()}
+    for mode in ("fused", "generic", "separate"):
+        out, cour = _radial_loop_outputs(63, mode)
+        errs = _rel_errs(out, ref)
+        assert max(errs.values()) < 1e-11, (mode, errs)
+        assert np.allclose(cour, ocour, rtol=1e-12, atol=0), (mode, cour, ocour)
+
+
+@pytest.mark.parametrize("l_max", [16, 383])
+def test_fused_grid_step_matches_separate_kernels(l_max):
+    """Fused grid-space kernel (compiled and run-time FFT plans) against the
+    separate inverse FFT / get_nl / forward FFT kernels on random fields.
+    l_max 16 and 383 give n_phi/2 = 24 and 576.  The two fused variants run the
+    same arithmetic; against the separate kernels the summation order differs.
+    Errors are relative to the largest coefficient of each vector."""
+    ref, cref = _radial_loop_outputs(l_max, "separate")
+    fused, cf = _radial_loop_outputs(l_max, "fused")
+    gen, cg = _radial_loop_outputs(l_max, "generic")
+    e_sep = _rel_errs(fused, ref)
+    e_gen = _rel_errs(fused, gen)
+    assert max(e_sep.values()) < 1e-11, e_sep
+    assert max(e_gen.values()) < 1e-12, e_gen
+    assert np.allclose(cf, cref, rtol=1e-12, atol=0), (cf, cref)
+    assert np.allclose(cf, cg, rtol=1e-14, atol=0), (cf, cg)

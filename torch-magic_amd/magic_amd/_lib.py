@@ -37,7 +37,7 @@ class SynArgs(C.Structure):
 class AnaArgs(C.Structure):
     _fields_ = [("F", P), ("c_stride", L), ("m_stride", L), ("Bin", I), ("B", I), ("ncomp", I),
                 ("cf1", I * 3), ("cf2", I * 3), ("pad0", I), ("outS", P * 3), ("outT", P * 3),
-                ("ldo", I), ("H", I), ("Hp", I), ("pad1", I), ("mp_info", P), ("lm_start", P),
+                ("ldo", I), ("H", I), ("Hp", I), ("n_m", I), ("mp_info", P), ("lm_start", P),
                 ("anaW", P), ("anaWd", P), ("tiles", P), ("ntiles", I), ("pad2", I)]
 
 
@@ -53,6 +53,13 @@ class NlArgs(C.Structure):
                 ("N", I), ("T", I), ("P", I), ("pad", I), ("or2", P), ("or4", P), ("ost2", P),
                 ("valri2", P), ("valhi2", P), ("LFfac", D), ("cf2", D), ("af2", D),
                 ("l_mag", I), ("alf_damp", I), ("cfl", P)]
+
+
+class GridArgs(C.Structure):
+    _fields_ = [("Fs", P), ("Fv", P), ("Fo", P), ("N", I), ("T", I), ("M", I), ("Mh", I),
+                ("generic", I), ("pad", I), ("tw", P), ("tw2", P), ("or2", P),
+                ("or4", P), ("ost2", P), ("valri2", P), ("valhi2", P), ("LFfac", D), ("cf2", D),
+                ("af2", D), ("l_mag", I), ("alf_damp", I), ("cfl", P)]
 
 
 class RowJob(C.Structure):
@@ -119,7 +126,7 @@ class GemmArgs(C.Structure):
 
 
 _STRUCT_ORDER = [SynArgs, AnaArgs, FftArgs, NlArgs, RowJob, TdArgs, RhsArgs, BuoyArgs,
-                 ImplArgs, EnergyArgs, BuildArgs, GemmArgs]
+                 ImplArgs, EnergyArgs, BuildArgs, GemmArgs, GridArgs]
 
 _SIGS = {
     "mh_plm_tables": [I, I, I, I, P, P, P, P, P, P, P, P, P],
@@ -127,6 +134,7 @@ _SIGS = {
     "mh_leg_analysis": [I, P, P],
     "mh_fft": [I, P, P],
     "mh_get_nl": [P, P],
+    "mh_grid_step": [P, P],
     "mh_rowop": [P, P, I, P],
     "mh_td": [P, P],
     "mh_rhs": [P, P],

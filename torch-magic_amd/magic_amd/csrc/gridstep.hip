@@ -1,0 +1,604 @@
+// Fused grid-space part of the radial loop.  One workgroup takes one theta row
+// t and RB consecutive radial levels r0..r0+RB-1 (one "row" each):
+//
+//   13 inverse FFTs (u, s, B, curl B, curl u in sin(theta)-weighted form)
+//   -> nonlinear products and Courant maxima, point by point in registers
+//   -> 9 forward FFTs (Advr VSr VxBr Advt Advp VSt VSp VxBt VxBp), scaled by
+//      1/n_phi and, for the theta/phi components, 1/sin^2(theta)
+//
+// Fourier buffers are F[c][m][t][b]: for fixed (c, m, t) the RB levels of a
+// workgroup are RB consecutive complex values, so loads and stores move
+// RB*16-byte runs.  The grid never leaves the CU.
+//
+// Real FFT of n_phi = 2*Mh points = complex Stockham FFT of length Mh in LDS
+// (even/odd packing).  The last inverse pass and the first forward pass have
+// radix RL and run in registers with a fixed thread <-> butterfly map: thread
+// (row, q) owns the complex grid points q + i*TP (i < RL, TP = Mh/RL), i.e.
+// the real points 2(q+i*TP) and 2(q+i*TP)+1, for every component.  The
+// products therefore need no data exchange, and the forward transform starts
+// on the values the thread already holds.
+//
+// Register budget: at most 7 components of RL complex values are live at
+// once (u and B for the induction terms; u or B, the three advection
+// accumulators and the component being transformed for the force terms),
+// which is why B is transformed twice.
+#include "fft_lds.h"
+
+typedef MhGridArgs GridArgs;
+
+__global__ void k_zero_cfl(unsigned long long* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0ull;
+}
+
+namespace {
+
+constexpr int kMaxThreads = 768;
+constexpr int kLdsBytes = 160 * 1024 - 256;
+
+// FFT plan of the fused kernel for a half length Mh: radices in inverse order,
+// the last (RL) being the register pass; TP = Mh/RL threads per level; RB
+// levels per workgroup (two LDS buffers of RB rows of Mh+1 complex values).
+struct GPlan {
+  int RL, TP, RB, nst, threads;
+  int radix[16];
+};
+
+constexpr GPlan make_plan(int Mh) {
+  GPlan p{};
+  if (Mh < 2) return p;
+  const int RL = Mh % 3 == 0 ? 3 : Mh % 4 == 0 ? 4 : Mh % 2 == 0 ? 2 : Mh % 5 == 0 ? 5 : 0;
+  if (RL == 0) return p;
+  int n = Mh / RL, k = 0;
+  while (n % 4 == 0 && k < 15) { p.radix[k++] = 4; n /= 4; }
+  while (n % 2 == 0 && k < 15) { p.radix[k++] = 2; n /= 2; }
+  while (n % 3 == 0 && k < 15) { p.radix[k++] = 3; n /= 3; }
+  while (n % 5 == 0 && k < 15) { p.radix[k++] = 5; n /= 5; }
+  if (n != 1) return p;
+  p.radix[k++] = RL;
+  p.nst = k;
+  p.TP = Mh / RL;
+  for (int rb = 16; rb >= 1; --rb) {
+    const int th = (rb * p.TP + 63) / 64 * 64;
+    if (th <= kMaxThreads && 2 * rb * (Mh + 1) * 16 <= kLdsBytes) {
+      p.RB = rb;
+      p.threads = th;
+      break;
+    }
+  }
+  if (p.RB > 0) p.RL = RL;
+  return p;
+}
+
+// Half lengths with a compiled plan (all divisions and stage loops resolved
+// at compile time); other lengths run the same code with the run-time plan.
+#define MH_GRID_COMPILED(X) X(24) X(48) X(96) X(192) X(288) X(384) X(576) X(768)
+
+// Workgroup barrier that waits for this wave's LDS operations only, so the
+// global loads of the next component stay in flight across FFT passes.
+__device__ __forceinline__ void lds_barrier() {
+  asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+extern __shared__ double2 g_lds[];
+
+// Geometry of one launch; compile-time constants when MH > 0.
+struct Geo {
+  int Mh, M, RB, LS, TP, nst, threads;
+};
+
+template <int MH>
+__device__ __forceinline__ Geo make_geo(const GridArgs& a, const GPlan& rp) {
+  Geo g;
+  g.M = a.M;
+  if constexpr (MH > 0) {
+    constexpr GPlan P = make_plan(MH);
+    g.Mh = MH;
+    g.RB = P.RB;
+    g.TP = P.TP;
+    g.nst = P.nst;
+    g.threads = P.threads;
+  } else {
+    g.Mh = a.Mh;
+    g.RB = rp.RB;
+    g.TP = rp.TP;
+    g.nst = rp.nst;
+    g.threads = rp.threads;
+  }
+  g.LS = g.Mh + 1;
+  return g;
+}
+
+__device__ __forceinline__ double2* lbuf(const Geo& g, int w) { return g_lds + w * g.RB * g.LS; }
+
+struct Ctx {
+  const GPlan* rp;  // run-time plan (MH == 0)
+  int row, q;       // register-pass mapping of this thread (row < RB when active)
+  bool act;
+  int busy;         // LDS buffer that other threads may still be reading
+};
+
+// Z_k of the packed inverse real FFT, read from the raw spectrum X in LDS:
+// Z_k = E_k + i O_k, E_k = X_k + conj(X_{Mh-k}), O_k = (X_k - conj(X_{Mh-k})) e^{+2 pi i k/n}
+__device__ __forceinline__ double2 zload(const Geo& g, const GridArgs& a, const double2* X, int k) {
+  double2 x = make_double2(0.0, 0.0), xc = x;
+  if (k < g.M) x = X[k];
+  if (k == 0) x.y = 0.0;
+  const int kc = g.Mh - k;
+  if (k > 0 && kc < g.M) xc = cconj(X[kc]);
+  const double2 E = cadd(x, xc);
+  const double2 O = cmul(csub(x, xc), cconj(reinterpret_cast<const double2*>(a.tw2)[k]));
+  return make_double2(E.x - O.y, E.y + O.x);
+}
+
+template <int R>
+__device__ __forceinline__ void stage_first_inv(const Geo& g, const GridArgs& a, int win, int wout) {
+  const int nb = g.Mh / R;
+  const double2* in = lbuf(g, win);
+  double2* out = lbuf(g, wout);
+  for (int idx = threadIdx.x; idx < g.RB * nb; idx += g.threads) {
+    const int row = idx / nb, bf = idx - row * nb;
+    const double2* src = in + row * g.LS;
+    double2 v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = zload(g, a, src, bf + r * nb);
+    dft<R>(v, 1.0);
+    double2* dst = out + row * g.LS + bf * R;
+#pragma unroll
+    for (int r = 0; r < R; ++r) dst[r] = v[r];
+  }
+}
+
+template <int R>
+__device__ __forceinline__ void stage_mid(const Geo& g, const GridArgs& a, int win, int wout, int Ns,
+                                          double dir) {
+  const int nb = g.Mh / R, tstep = g.Mh / (Ns * R);
+  const double2* tw = reinterpret_cast<const double2*>(a.tw);
+  const double2* in = lbuf(g, win);
+  double2* out = lbuf(g, wout);
+  for (int idx = threadIdx.x; idx < g.RB * nb; idx += g.threads) {
+    const int row = idx / nb, bf = idx - row * nb;
+    const int k = bf % Ns;
+    const double2* src = in + row * g.LS;
+    double2 v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = src[bf + r * nb];
+#pragma unroll
+    for (int r = 1; r < R; ++r) {
+      double2 w = tw[r * k * tstep];
+      w.y *= -dir;  // tw holds exp(-i...); the inverse (dir = +1) conjugates
+      v[r] = cmul(v[r], w);
+    }
+    dft<R>(v, dir);
+    double2* dst = out + row * g.LS + (bf / Ns) * Ns * R + k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) dst[r * Ns] = v[r];
+  }
+}
+
+__device__ __forceinline__ void stage_mid_rt(const Geo& g, const GridArgs& a, int R, int win, int wout,
+                                             int Ns, double dir) {
+  if (R == 4) stage_mid<4>(g, a, win, wout, Ns, dir);
+  else if (R == 2) stage_mid<2>(g, a, win, wout, Ns, dir);
+  else if (R == 3) stage_mid<3>(g, a, win, wout, Ns, dir);
+  else stage_mid<5>(g, a, win, wout, Ns, dir);
+}
+
+__device__ __forceinline__ void stage_first_inv_rt(const Geo& g, const GridArgs& a, int R, int win,
+                                                   int wout) {
+  if (R == 4) stage_first_inv<4>(g, a, win, wout);
+  else if (R == 2) stage_first_inv<2>(g, a, win, wout);
+  else if (R == 3) stage_first_inv<3>(g, a, win, wout);
+  else stage_first_inv<5>(g, a, win, wout);
+}
+
+// Inverse stages 1 .. nst-2 of a compiled plan (S = stage, NS = its Ns).
+template <int MH, int S, int NS>
+__device__ __forceinline__ int inv_mid_ct(const Geo& g, const GridArgs& a, int w) {
+  constexpr GPlan P = make_plan(MH);
+  if constexpr (S < P.nst - 1) {
+    constexpr int R = P.radix[S];
+    stage_mid<R>(g, a, w, 1 - w, NS, 1.0);
+    lds_barrier();
+    return inv_mid_ct<MH, S + 1, NS * R>(g, a, 1 - w);
+  } else {
+    return w;
+  }
+}
+
+// Forward stages S = nst-2 .. 0 of a compiled plan.
+template <int MH, int S, int NS>
+__device__ __forceinline__ int fwd_mid_ct(const Geo& g, const GridArgs& a, int w) {
+  constexpr GPlan P = make_plan(MH);
+  if constexpr (S >= 0) {
+    constexpr int R = P.radix[S];
+    stage_mid<R>(g, a, w, 1 - w, NS, -1.0);
+    lds_barrier();
+    return fwd_mid_ct<MH, S - 1, NS * R>(g, a, 1 - w);
+  } else {
+    return w;
+  }
+}
+
+// Global loads of one component's raw spectrum for the workgroup's levels
+// into registers: element e = m*RB + j (level j of the block) at
+// base + ((m*T + t)*Bst + b0 + j)*2 doubles.
+template <int RL>
+struct Prefetch {
+  double2 v[RL];
+};
+
+template <int MH, int RL>
+__device__ __forceinline__ void fetch(const Geo& g, const GridArgs& a, Prefetch<RL>& pf,
+                                      const double* base, int Bst, int b0, int nvalid, int t) {
+  const int RB = g.RB, n = g.M * RB;
+#pragma unroll
+  for (int i = 0; i < RL; ++i) {
+    const int e = threadIdx.x + i * g.threads;
+    double2 x = make_double2(0.0, 0.0);
+    if (e < n) {
+      const int m = e / RB, j = e - m * RB;
+      if (j < nvalid)
+        x = *reinterpret_cast<const double2*>(base + (((long)m * a.T + t) * Bst + b0 + j) * 2);
+    }
+    pf.v[i] = x;
+  }
+}
+
+// Inverse transform of the prefetched component; returns this thread's RL
+// complex grid values (x_{2p} + i x_{2p+1} at p = q + i*TP).
+template <int MH, int RL>
+__device__ __forceinline__ void inverse(Ctx& c, const Geo& g, const GridArgs& a,
+                                        const Prefetch<RL>& pf, double2 (&z)[RL]) {
+  const int RB = g.RB, n = g.M * RB, LS = g.LS;
+  int w = 1 - c.busy;
+  {
+    double2* raw = lbuf(g, w);
+#pragma unroll
+    for (int i = 0; i < RL; ++i) {
+      const int e = threadIdx.x + i * g.threads;
+      if (e < n) {
+        const int m = e / RB, j = e - m * RB;
+        raw[j * LS + m] = pf.v[i];
+      }
+    }
+  }
+  lds_barrier();
+  if (g.nst > 1) {
+    if constexpr (MH > 0) {
+      constexpr GPlan P = make_plan(MH);
+      if constexpr (P.nst > 1) {
+        stage_first_inv<P.radix[0]>(g, a, w, 1 - w);
+        lds_barrier();
+        w = inv_mid_ct<MH, 1, P.radix[0]>(g, a, 1 - w);
+      }
+    } else {
+      const GPlan& P = *c.rp;
+      stage_first_inv_rt(g, a, P.radix[0], w, 1 - w);
+      int Ns = P.radix[0];
+      w = 1 - w;
+      lds_barrier();
+      for (int s = 1; s < P.nst - 1; ++s) {
+        stage_mid_rt(g, a, P.radix[s], w, 1 - w, Ns, 1.0);
+        Ns *= P.radix[s];
+        w = 1 - w;
+        lds_barrier();
+      }
+    }
+  }
+  // last pass (radix RL, Ns = TP): butterfly q of level `row`, twiddles w^{i q}
+  if (c.act) {
+    const double2* src = lbuf(g, w) + c.row * LS;
+    const double2* tw = reinterpret_cast<const double2*>(a.tw);
+#pragma unroll
+    for (int i = 0; i < RL; ++i) {
+      const int k = c.q + i * g.TP;
+      z[i] = (g.nst > 1) ? src[k] : zload(g, a, src, k);
+    }
+    if (g.nst > 1) {
+#pragma unroll
+      for (int i = 1; i < RL; ++i) z[i] = cmul(z[i], cconj(tw[i * c.q]));
+    }
+    dft<RL>(z, 1.0);
+  } else {
+#pragma unroll
+    for (int i = 0; i < RL; ++i) z[i] = make_double2(0.0, 0.0);
+  }
+  c.busy = w;
+}
+
+// Forward transform of the thread's grid values z; writes X_m (m < M) times
+// `scale` into output component `comp` of Fo for the bulk levels of the block.
+template <int MH, int RL>
+__device__ __forceinline__ void forward(Ctx& c, const Geo& g, const GridArgs& a, double2 (&z)[RL],
+                                        int comp, double scale, int t, int r0) {
+  const int LS = g.LS;
+  int w = 1 - c.busy;
+  // first pass (radix RL, Ns = 1): inputs at q + i*TP, outputs at q*RL + i
+  if (c.act) {
+    dft<RL>(z, -1.0);
+    double2* dst = lbuf(g, w) + c.row * LS + c.q * RL;
+#pragma unroll
+    for (int i = 0; i < RL; ++i) dst[i] = z[i];
+  }
+  lds_barrier();
+  if constexpr (MH > 0) {
+    constexpr GPlan P = make_plan(MH);
+    w = fwd_mid_ct<MH, P.nst - 2, RL>(g, a, w);
+  } else {
+    const GPlan& P = *c.rp;
+    int Ns = RL;
+    for (int s = P.nst - 2; s >= 0; --s) {
+      stage_mid_rt(g, a, P.radix[s], w, 1 - w, Ns, -1.0);
+      Ns *= P.radix[s];
+      w = 1 - w;
+      lds_barrier();
+    }
+  }
+  // unpack X_m = E_m + e^{-2 pi i m/n} O_m and store the bulk levels
+  const int RB = g.RB, Mh = g.Mh, N = a.N, No = N - 2, n = g.M * RB;
+  const double2* tw2 = reinterpret_cast<const double2*>(a.tw2);
+  double* Fo = a.Fo + (long)comp * g.M * a.T * No * 2;
+  const double2* res = lbuf(g, w);
+  for (int e = threadIdx.x; e < n; e += g.threads) {
+    const int m = e / RB, j = e - m * RB;
+    const int r = r0 + j;
+    if (r < 1 || r > N - 2) continue;
+    const double2* src = res + j * LS;
+    const double2 Z = src[m];
+    const double2 Zc = cconj(src[m == 0 ? 0 : Mh - m]);
+    const double2 E = cscale(cadd(Z, Zc), 0.5);
+    const double2 D = csub(Z, Zc);
+    const double2 O = make_double2(0.5 * D.y, -0.5 * D.x);  // (Z - Zc) / (2i)
+    const double2 X = cadd(E, cmul(tw2[m], O));
+    *reinterpret_cast<double2*>(Fo + (((long)m * a.T + t) * No + r - 1) * 2) = cscale(X, scale);
+  }
+  c.busy = w;
+}
+
+}  // namespace
+
+template <int MH, int RL>
+__global__ __launch_bounds__(kMaxThreads) void k_grid_step(GridArgs a, GPlan rp) {
+  __shared__ unsigned long long red[2 * 16];
+  const Geo g = make_geo<MH>(a, rp);
+  const int N = a.N, T = a.T, RB = g.RB;
+  const int nRB = (N + RB - 1) / RB;
+  // XCD-aware order: workgroups i and i+8 are dispatched to the same XCD, so
+  // consecutive logical indices (neighbouring level blocks of one theta row,
+  // which share 128-byte lines of F) are given to them.
+  const int nwg = T * nRB, per = (nwg + 7) / 8;
+  const int L = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  if (L >= nwg) return;  // uniform per workgroup
+  const int t = L / nRB, r0 = (L - t * nRB) * RB;
+  const int nvalid = min(RB, N - r0);
+
+  Ctx c;
+  c.rp = &rp;
+  c.row = threadIdx.x / g.TP;
+  c.q = threadIdx.x - c.row * g.TP;
+  c.act = c.row < nvalid;
+  c.busy = 0;
+  if (threadIdx.x < 2 * RB) red[threadIdx.x] = 0ull;
+
+  const int r = r0 + (c.act ? c.row : 0);
+  const double ost2 = a.ost2[t];
+  const double or2 = a.or2[r], or4 = a.or4[r];
+  const double LF = a.LFfac;
+  const int Bv = 4 * N;
+  const long vcs = (long)g.M * T * Bv * 2;  // component stride of Fv (doubles)
+  const double* Fv = a.Fv;
+  const bool mag = a.l_mag != 0;
+
+  Prefetch<RL> pf;
+  double2 vr[RL], vt[RL], vp[RL], tmp[RL];
+  const double inv_n = 1.0 / (2.0 * g.Mh);
+  // 1. u (group 2)
+  fetch<MH, RL>(g, a, pf, Fv, Bv, 2 * N + r0, nvalid, t);
+  inverse<MH, RL>(c, g, a, pf, vr);
+  fetch<MH, RL>(g, a, pf, Fv + vcs, Bv, 2 * N + r0, nvalid, t);
+  inverse<MH, RL>(c, g, a, pf, vt);
+  fetch<MH, RL>(g, a, pf, Fv + 2 * vcs, Bv, 2 * N + r0, nvalid, t);
+  inverse<MH, RL>(c, g, a, pf, vp);
+  // 2. s -> VSr, VSt, VSp
+  fetch<MH, RL>(g, a, pf, a.Fs, N, r0, nvalid, t);
+  inverse<MH, RL>(c, g, a, pf, tmp);
+  if (mag) fetch<MH, RL>(g, a, pf, Fv, Bv, r0, nvalid, t);  // B_r next
+  else fetch<MH, RL>(g, a, pf, Fv, Bv, 3 * N + r0, nvalid, t);  // curl u_r next
+  {
+    double2 o[RL];
+#pragma unroll
+    for (int i = 0; i < RL; ++i) o[i] = make_double2(vr[i].x * tmp[i].x, vr[i].y * tmp[i].y);
+    forward<MH, RL>(c, g, a, o, 1, inv_n, t, r0);
+#pragma unroll
+    for (int i = 0; i < RL; ++i)
+      o[i] = make_double2(or2 * vt[i].x * tmp[i].x, or2 * vt[i].y * tmp[i].y);
+    forward<MH, RL>(c, g, a, o, 5, inv_n * ost2, t, r0);
+#pragma unroll
+    for (int i = 0; i < RL; ++i)
+      o[i] = make_double2(or2 * vp[i].x * tmp[i].x, or2 * vp[i].y * tmp[i].y);
+    forward<MH, RL>(c, g, a, o, 6, inv_n * ost2, t, r0);
+  }
+  // 3. B -> Courant maxima (all levels incl. boundaries) and VxBr, VxBt, VxBp
+  {
+    double2 br[RL], bt[RL], bp[RL];
+    if (mag) {
+      inverse<MH, RL>(c, g, a, pf, br);
+      fetch<MH, RL>(g, a, pf, Fv + vcs, Bv, r0, nvalid, t);
+      inverse<MH, RL>(c, g, a, pf, bt);
+      fetch<MH, RL>(g, a, pf, Fv + 2 * vcs, Bv, r0, nvalid, t);
+      inverse<MH, RL>(c, g, a, pf, bp);
+      fetch<MH, RL>(g, a, pf, Fv, Bv, 3 * N + r0, nvalid, t);  // curl u_r next
+    } else {
+#pragma unroll
+      for (int i = 0; i < RL; ++i) br[i] = bt[i] = bp[i] = make_double2(0.0, 0.0);
+    }
+    if (c.act) {
+      double cr = 0.0, ch = 0.0;
+      const double vri = a.valri2[r], vhi = a.valhi2[r];
+#pragma unroll
+      for (int i = 0; i < RL; ++i) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const double Vr = h ? vr[i].y : vr[i].x, Vt = h ? vt[i].y : vt[i].x;
+          const double Vp = h ? vp[i].y : vp[i].x, Br = h ? br[i].y : br[i].x;
+          const double Bt = h ? bt[i].y : bt[i].x, Bp = h ? bp[i].y : bp[i].x;
+          const double vflr2 = Vr * Vr;
+          const double vflh2 = (Vt * Vt + Vp * Vp) * ost2;
+          double c1, c2;
+          if (mag) {
+            const double valr = Br * Br * LF;
+            const double valh2 = (Bt * Bt + Bp * Bp) * LF * ost2;
+            double valr2 = valr, valh2m = valh2;
+            if (a.alf_damp) {
+              valr2 = valr * valr / (valr + vri);
+              valh2m = valh2 * valh2 / (valh2 + vhi);
+            }
+            c1 = or4 * (a.cf2 * vflr2 + a.af2 * valr2);
+            c2 = or2 * (a.cf2 * vflh2 + a.af2 * valh2m);
+          } else {
+            c1 = a.cf2 * or4 * vflr2;
+            c2 = a.cf2 * or2 * vflh2;
+          }
+          cr = fmax(cr, c1);
+          ch = fmax(ch, c2);
+        }
+      }
+      atomicMax(&red[c.row], (unsigned long long)__double_as_longlong(cr));
+      atomicMax(&red[RB + c.row], (unsigned long long)__double_as_longlong(ch));
+    }
+    double2 o[RL];
+#pragma unroll
+    for (int i = 0; i < RL; ++i)
+      o[i] = make_double2(ost2 * (vt[i].x * bp[i].x - vp[i].x * bt[i].x),
+                          ost2 * (vt[i].y * bp[i].y - vp[i].y * bt[i].y));
+    forward<MH, RL>(c, g, a, o, 2, inv_n, t, r0);
+#pragma unroll
+    for (int i = 0; i < RL; ++i)
+      o[i] = make_double2(or4 * (vp[i].x * br[i].x - vr[i].x * bp[i].x),
+                          or4 * (vp[i].y * br[i].y - vr[i].y * bp[i].y));
+    forward<MH, RL>(c, g, a, o, 7, inv_n * ost2, t, r0);
+#pragma unroll
+    for (int i = 0; i < RL; ++i)
+      o[i] = make_double2(or4 * (vr[i].x * bt[i].x - vt[i].x * br[i].x),
+                          or4 * (vr[i].y * bt[i].y - vt[i].y * br[i].y));
+    forward<MH, RL>(c, g, a, o, 8, inv_n * ost2, t, r0);
+  }
+  // 4-5. Advection + Lorentz force, accumulated one component of curl u / curl B
+  // at a time (B is transformed a second time so that u and B are never live
+  // together with the accumulators):
+  //   Advr = -ost2 (Cvt Vp - Cvp Vt) + LF ost2 (Cbt Bp - Cbp Bt)
+  //   Advt = -or4 (Cvp Vr - Cvr Vp)  + LF or4 (Cbp Br - Cbr Bp)
+  //   Advp = -or4 (Cvr Vt - Cvt Vr)  + LF or4 (Cbr Bt - Cbt Br)
+  double2 ar[RL], at[RL], ap[RL];
+  inverse<MH, RL>(c, g, a, pf, tmp);  // curl u_r
+  fetch<MH, RL>(g, a, pf, Fv + vcs, Bv, 3 * N + r0, nvalid, t);
+#pragma unroll
+  for (int i = 0; i < RL; ++i) {
+    ar[i] = make_double2(0.0, 0.0);
+    at[i] = make_double2(or4 * tmp[i].x * vp[i].x, or4 * tmp[i].y * vp[i].y);
+    ap[i] = make_double2(-or4 * tmp[i].x * vt[i].x, -or4 * tmp[i].y * vt[i].y);
+  }
+  inverse<MH, RL>(c, g, a, pf, tmp);  // curl u_t
+  fetch<MH, RL>(g, a, pf, Fv + 2 * vcs, Bv, 3 * N + r0, nvalid, t);
+#pragma unroll
+  for (int i = 0; i < RL; ++i) {
+    ar[i].x -= ost2 * tmp[i].x * vp[i].x;
+    ar[i].y -= ost2 * tmp[i].y * vp[i].y;
+    ap[i].x += or4 * tmp[i].x * vr[i].x;
+    ap[i].y += or4 * tmp[i].y * vr[i].y;
+  }
+  inverse<MH, RL>(c, g, a, pf, tmp);  // curl u_p
+  if (mag) fetch<MH, RL>(g, a, pf, Fv, Bv, r0, nvalid, t);  // B_r next
+#pragma unroll
+  for (int i = 0; i < RL; ++i) {
+    ar[i].x += ost2 * tmp[i].x * vt[i].x;
+    ar[i].y += ost2 * tmp[i].y * vt[i].y;
+    at[i].x -= or4 * tmp[i].x * vr[i].x;
+    at[i].y -= or4 * tmp[i].y * vr[i].y;
+  }
+  if (mag) {
+    const double lr = LF * ost2, lt = LF * or4;
+    double2 br[RL], bt[RL], bp[RL];
+    inverse<MH, RL>(c, g, a, pf, br);
+    fetch<MH, RL>(g, a, pf, Fv + vcs, Bv, r0, nvalid, t);
+    inverse<MH, RL>(c, g, a, pf, bt);
+    fetch<MH, RL>(g, a, pf, Fv + 2 * vcs, Bv, r0, nvalid, t);
+    inverse<MH, RL>(c, g, a, pf, bp);
+    fetch<MH, RL>(g, a, pf, Fv, Bv, N + r0, nvalid, t);
+    inverse<MH, RL>(c, g, a, pf, tmp);  // curl B_r
+    fetch<MH, RL>(g, a, pf, Fv + vcs, Bv, N + r0, nvalid, t);
+#pragma unroll
+    for (int i = 0; i < RL; ++i) {
+      at[i].x -= lt * tmp[i].x * bp[i].x;
+      at[i].y -= lt * tmp[i].y * bp[i].y;
+      ap[i].x += lt * tmp[i].x * bt[i].x;
+      ap[i].y += lt * tmp[i].y * bt[i].y;
+    }
+    inverse<MH, RL>(c, g, a, pf, tmp);  // curl B_t
+    fetch<MH, RL>(g, a, pf, Fv + 2 * vcs, Bv, N + r0, nvalid, t);
+#pragma unroll
+    for (int i = 0; i < RL; ++i) {
+      ar[i].x += lr * tmp[i].x * bp[i].x;
+      ar[i].y += lr * tmp[i].y * bp[i].y;
+      ap[i].x -= lt * tmp[i].x * br[i].x;
+      ap[i].y -= lt * tmp[i].y * br[i].y;
+    }
+    inverse<MH, RL>(c, g, a, pf, tmp);  // curl B_p
+#pragma unroll
+    for (int i = 0; i < RL; ++i) {
+      ar[i].x -= lr * tmp[i].x * bt[i].x;
+      ar[i].y -= lr * tmp[i].y * bt[i].y;
+      at[i].x += lt * tmp[i].x * br[i].x;
+      at[i].y += lt * tmp[i].y * br[i].y;
+    }
+  }
+  forward<MH, RL>(c, g, a, ar, 0, inv_n, t, r0);
+  forward<MH, RL>(c, g, a, at, 3, inv_n * ost2, t, r0);
+  forward<MH, RL>(c, g, a, ap, 4, inv_n * ost2, t, r0);
+  lds_barrier();
+  if (threadIdx.x < 2 * nvalid) {
+    const int j = threadIdx.x % nvalid, h = threadIdx.x / nvalid;
+    atomicMax(a.cfl + h * N + r0 + j, red[h * RB + j]);
+  }
+}
+
+extern "C" int mh_grid_step(const MhGridArgs* args, void* stream) {
+  const GridArgs& a = *args;
+  if (a.N < 3 || a.T < 1 || a.Mh < 2 || a.M < 1 || a.M > a.Mh) return MH_EINVAL;
+  const GPlan P = make_plan(a.Mh);
+  if (P.RL == 0) return MH_EINVAL;
+  // every thread covers its share of a component's M*RB raw values in RL loads
+  if ((long)P.threads * P.RL < (long)a.M * P.RB) return MH_EINVAL;
+  const size_t lds = 2ull * P.RB * (a.Mh + 1) * sizeof(double2);
+  const int nRB = (a.N + P.RB - 1) / P.RB;
+  const long nwg = (long)a.T * nRB;
+  const long grid = ((nwg + 7) / 8) * 8;
+  if (grid > 0x7fffffffL) return MH_EINVAL;
+  hipLaunchKernelGGL(k_zero_cfl, dim3(mh_cdiv(2 * a.N, 256)), dim3(256), 0, mh_stream(stream),
+                     a.cfl, 2 * a.N);
+  const dim3 gd((unsigned)grid), bd(P.threads);
+  hipStream_t st = mh_stream(stream);
+  bool done = false;
+  if (!a.generic) {
+    switch (a.Mh) {
+#define MH_CASE(V)                                                                    \
+  case V:                                                                             \
+    hipLaunchKernelGGL((k_grid_step<V, make_plan(V).RL>), gd, bd, lds, st, a, P);     \
+    done = true;                                                                      \
+    break;
+      MH_GRID_COMPILED(MH_CASE)
+#undef MH_CASE
+      default: break;
+    }
+  }
+  if (!done) {
+    switch (P.RL) {
+      case 2: hipLaunchKernelGGL((k_grid_step<0, 2>), gd, bd, lds, st, a, P); break;
+      case 3: hipLaunchKernelGGL((k_grid_step<0, 3>), gd, bd, lds, st, a, P); break;
+      case 4: hipLaunchKernelGGL((k_grid_step<0, 4>), gd, bd, lds, st, a, P); break;
+      default: hipLaunchKernelGGL((k_grid_step<0, 5>), gd, bd, lds, st, a, P); break;
+    }
+  }
+  return mh_launch_status();
+}

@@ -96,12 +96,16 @@ __device__ __forceinline__ double imul_pair(double v, double m, int lane) {
   return (lane & 1) ? m * o : -m * o;
 }
 
+// Offset (in doubles) of the real part of F(c, m, t, b), layout F[c][m][t][b].
+__device__ __forceinline__ long fidx(long cs, long ms, int B, int c, int m, int t, int b) {
+  return c * cs + m * ms + (long)t * 2 * B + 2L * b;
+}
+
 // WG = 4 waves; wave w covers theta pairs j0 + 16*(w&1) .. and complex batch
 // columns b0 + 8*(w>>1) .. ; WG tile = 32 theta pairs x 16 batch columns.
 template <bool VEC>
-__global__ __launch_bounds__(256) void k_syn(SynArgs a) {
+__global__ __launch_bounds__(256) void k_syn(SynArgs a, int n_m) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int m = blockIdx.z;
   const int nbt = (a.B + 15) / 16;
   const int g = blockIdx.x / nbt;
   const int b0 = (blockIdx.x % nbt) * 16 + 8 * (w >> 1);
@@ -110,69 +114,71 @@ __global__ __launch_bounds__(256) void k_syn(SynArgs a) {
   const int b = b0 + (col >> 1), reim = col & 1;
   const bool bval = b < a.B;
   const int bl = bval ? b : 0;
-  const long lms = a.lm_start[m];
-  double4_t acc[VEC ? 10 : 2];
-#pragma unroll
-  for (int q = 0; q < (VEC ? 10 : 2); ++q) acc[q] = (double4_t){0.0, 0.0, 0.0, 0.0};
   const SynIn* in = a.in[g];
+  {
+    const int m = blockIdx.z;
+    const long lms = a.lm_start[m];
+    double4_t acc[VEC ? 10 : 2];
 #pragma unroll
-  for (int p = 0; p < 2; ++p) {
-    const int* info = a.mp_info + (m * 2 + p) * 4;
-    const int K = info[0];
-    const double* tP = a.synP + info[1];
-    const double* tD = a.synD + info[1];
-    for (int k0 = 0; k0 < K; k0 += 4) {
-      const int k = k0 + kk;
-      const double aP = tP[(long)k * a.Hp + j0 + col];
-      const bool v = bval && (k < K);
-      const int l = m + p + 2 * k;
-      const long lm = lms + p + 2 * k;
-      if (!VEC) {
-        const double x = v ? syn_load(in[0], lm, l, bl, reim, a.ldx, a.or2) : 0.0;
-        acc[p] = mfma_f64(aP, x, acc[p]);
-      } else {
-        const double aD = tD[(long)k * a.Hp + j0 + col];
-        const double xq = v ? syn_load(in[0], lm, l, bl, reim, a.ldx, a.or2) : 0.0;
-        const double xs = v ? syn_load(in[1], lm, l, bl, reim, a.ldx, a.or2) : 0.0;
-        const double xt = v ? syn_load(in[2], lm, l, bl, reim, a.ldx, a.or2) : 0.0;
-        acc[5 * p + 0] = mfma_f64(aP, xq, acc[5 * p + 0]);   // P Q
-        acc[5 * p + 1] = mfma_f64(aP, xt, acc[5 * p + 1]);   // P T
-        acc[5 * p + 2] = mfma_f64(aP, xs, acc[5 * p + 2]);   // P S
-        acc[5 * p + 3] = mfma_f64(aD, xs, acc[5 * p + 3]);   // dP S
-        acc[5 * p + 4] = mfma_f64(aD, xt, acc[5 * p + 4]);   // dP T
+    for (int q = 0; q < (VEC ? 10 : 2); ++q) acc[q] = (double4_t){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      const int* info = a.mp_info + (m * 2 + p) * 4;
+      const int K = info[0];
+      const double* tP = a.synP + info[1];
+      const double* tD = a.synD + info[1];
+      for (int k0 = 0; k0 < K; k0 += 4) {
+        const int k = k0 + kk;
+        const double aP = tP[(long)k * a.Hp + j0 + col];
+        const bool v = bval && (k < K);
+        const int l = m + p + 2 * k;
+        const long lm = lms + p + 2 * k;
+        if (!VEC) {
+          const double x = v ? syn_load(in[0], lm, l, bl, reim, a.ldx, a.or2) : 0.0;
+          acc[p] = mfma_f64(aP, x, acc[p]);
+        } else {
+          const double aD = tD[(long)k * a.Hp + j0 + col];
+          const double xq = v ? syn_load(in[0], lm, l, bl, reim, a.ldx, a.or2) : 0.0;
+          const double xs = v ? syn_load(in[1], lm, l, bl, reim, a.ldx, a.or2) : 0.0;
+          const double xt = v ? syn_load(in[2], lm, l, bl, reim, a.ldx, a.or2) : 0.0;
+          acc[5 * p + 0] = mfma_f64(aP, xq, acc[5 * p + 0]);  // P Q
+          acc[5 * p + 1] = mfma_f64(aP, xt, acc[5 * p + 1]);  // P T
+          acc[5 * p + 2] = mfma_f64(aP, xs, acc[5 * p + 2]);  // P S
+          acc[5 * p + 3] = mfma_f64(aD, xs, acc[5 * p + 3]);  // dP S
+          acc[5 * p + 4] = mfma_f64(aD, xt, acc[5 * p + 4]);  // dP T
+        }
       }
     }
-  }
-  const double dm = (double)m;
-  const long rowlen = 2L * a.Bout;
-  const long cb = 2L * ((long)g * a.B + bl) + reim;
+    const double dm = (double)m;
+    const int bo = g * a.B + bl;
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int j = j0 + mfma_row(lane, i);
-    const bool st = bval && (j < a.H);
-    double* base = a.F + (long)m * a.m_stride + cb;
-    const long oN = (long)(2 * j) * rowlen, oS = (long)(2 * j + 1) * rowlen;
-    if (!VEC) {
-      const double e = acc[0][i], o = acc[1][i];
-      if (st) {
-        base[oN] = e + o;
-        base[oS] = e - o;
-      }
-    } else {
-      const double Qe = acc[0][i], Qo = acc[5][i];
-      const double PTe = acc[1][i], PTo = acc[6][i];
-      const double PSe = acc[2][i], PSo = acc[7][i];
-      const double DSe = acc[3][i], DSo = acc[8][i];
-      const double DTe = acc[4][i], DTo = acc[9][i];
-      const double iPTp = imul_pair(PTe + PTo, dm, lane), iPTm = imul_pair(PTe - PTo, dm, lane);
-      const double iPSp = imul_pair(PSe + PSo, dm, lane), iPSm = imul_pair(PSe - PSo, dm, lane);
-      if (st) {
-        base[oN] = Qe + Qo;
-        base[oS] = Qe - Qo;
-        base[a.c_stride + oN] = (DSe + DSo) + iPTp;
-        base[a.c_stride + oS] = -(DSe - DSo) + iPTm;
-        base[2 * a.c_stride + oN] = iPSp - (DTe + DTo);
-        base[2 * a.c_stride + oS] = iPSm + (DTe - DTo);
+    for (int i = 0; i < 4; ++i) {
+      const int j = j0 + mfma_row(lane, i);
+      const bool st = bval && (j < a.H);
+      const long oN = fidx(a.c_stride, a.m_stride, a.Bout, 0, m, 2 * j, bo) + reim;
+      const long oS = fidx(a.c_stride, a.m_stride, a.Bout, 0, m, 2 * j + 1, bo) + reim;
+      if (!VEC) {
+        const double e = acc[0][i], o = acc[1][i];
+        if (st) {
+          a.F[oN] = e + o;
+          a.F[oS] = e - o;
+        }
+      } else {
+        const double Qe = acc[0][i], Qo = acc[5][i];
+        const double PTe = acc[1][i], PTo = acc[6][i];
+        const double PSe = acc[2][i], PSo = acc[7][i];
+        const double DSe = acc[3][i], DSo = acc[8][i];
+        const double DTe = acc[4][i], DTo = acc[9][i];
+        const double iPTp = imul_pair(PTe + PTo, dm, lane), iPTm = imul_pair(PTe - PTo, dm, lane);
+        const double iPSp = imul_pair(PSe + PSo, dm, lane), iPSm = imul_pair(PSe - PSo, dm, lane);
+        if (st) {
+          a.F[oN] = Qe + Qo;
+          a.F[oS] = Qe - Qo;
+          a.F[a.c_stride + oN] = (DSe + DSo) + iPTp;
+          a.F[a.c_stride + oS] = -(DSe - DSo) + iPTm;
+          a.F[2 * a.c_stride + oN] = iPSp - (DTe + DTo);
+          a.F[2 * a.c_stride + oS] = iPSm + (DTe - DTo);
+        }
       }
     }
   }
@@ -184,9 +190,9 @@ extern "C" int mh_leg_synth(int vector, const MhSynArgs* args, int n_m, void* st
     return MH_EINVAL;
   dim3 grid(a.ngroups * mh_cdiv(a.B, 16), mh_cdiv(a.H, 32), n_m);
   if (vector)
-    hipLaunchKernelGGL(k_syn<true>, grid, dim3(256), 0, mh_stream(stream), a);
+    hipLaunchKernelGGL(k_syn<true>, grid, dim3(256), 0, mh_stream(stream), a, n_m);
   else
-    hipLaunchKernelGGL(k_syn<false>, grid, dim3(256), 0, mh_stream(stream), a);
+    hipLaunchKernelGGL(k_syn<false>, grid, dim3(256), 0, mh_stream(stream), a, n_m);
   return mh_launch_status();
 }
 
@@ -197,10 +203,10 @@ typedef MhAnaArgs AnaArgs;
 // WG = 4 waves: wave w covers k0 + 16*(w&1) .. (16 rows of l) and complex batch
 // columns b0 + 8*(w>>1) ..; K loop runs over theta pairs in steps of 4.
 template <bool VEC>
-__global__ __launch_bounds__(256) void k_ana(AnaArgs a) {
+__global__ __launch_bounds__(256) void k_ana(AnaArgs a, int n_m) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int* tl = a.tiles + 3 * blockIdx.y;
-  const int m = tl[0], p = tl[1];
+  const int p = tl[1];
   const int kbase = tl[2] + 16 * (w & 1);
   const int nbt = (a.B + 15) / 16;
   const int comp = blockIdx.x / nbt;
@@ -209,70 +215,74 @@ __global__ __launch_bounds__(256) void k_ana(AnaArgs a) {
   const int b = b0 + (col >> 1), reim = col & 1;
   const bool bval = b < a.B;
   const int bl = bval ? b : 0;
-  const int* info = a.mp_info + (m * 2 + p) * 4;
-  const int K = info[0], K16 = info[3];
-  const double* tW = a.anaW + info[2];
-  const double* tD = a.anaWd + info[2];
-  const long rowlen = 2L * a.Bin;
-  const double* Fm = a.F + (long)m * a.m_stride + 2L * bl + reim;
   const double sg = p ? -1.0 : 1.0;  // same-parity fold: N + sg*S
-  double4_t acc[VEC ? 4 : 1];
+  const int arow = kbase + col;      // row of A handled by this lane
+  {
+    const int m = tl[0];
+    if (m >= n_m) return;
+    const int* info = a.mp_info + (m * 2 + p) * 4;
+    const int K = info[0], K16 = info[3];
+    const double* tW = a.anaW + info[2];
+    const double* tD = a.anaWd + info[2];
+    double4_t acc[VEC ? 4 : 1];
 #pragma unroll
-  for (int q = 0; q < (VEC ? 4 : 1); ++q) acc[q] = (double4_t){0.0, 0.0, 0.0, 0.0};
-  const int arow = kbase + col;  // row of A handled by this lane
-  if (kbase < K) {
-    if (!VEC) {
-      const double* Fc = Fm + (long)a.cf1[comp] * a.c_stride;
-      for (int j0 = 0; j0 < a.H; j0 += 4) {
-        const int j = j0 + kk;
-        const double aw = tW[(long)j * K16 + arow];
-        double x = 0.0;
-        if (bval && j < a.H) {
-          const double fN = Fc[(long)(2 * j) * rowlen], fS = Fc[(long)(2 * j + 1) * rowlen];
-          x = fN + sg * fS;
+    for (int q = 0; q < (VEC ? 4 : 1); ++q) acc[q] = (double4_t){0.0, 0.0, 0.0, 0.0};
+    if (kbase < K) {
+      if (!VEC) {
+        const int c1 = a.cf1[comp];
+        for (int j0 = 0; j0 < a.H; j0 += 4) {
+          const int j = j0 + kk;
+          const double aw = tW[(long)j * K16 + arow];
+          double x = 0.0;
+          if (bval && j < a.H) {
+            const double fN = a.F[fidx(a.c_stride, a.m_stride, a.Bin, c1, m, 2 * j, bl) + reim];
+            const double fS = a.F[fidx(a.c_stride, a.m_stride, a.Bin, c1, m, 2 * j + 1, bl) + reim];
+            x = fN + sg * fS;
+          }
+          acc[0] = mfma_f64(aw, x, acc[0]);
         }
-        acc[0] = mfma_f64(aw, x, acc[0]);
-      }
-    } else {
-      const double* F1 = Fm + (long)a.cf1[comp] * a.c_stride;
-      const double* F2 = Fm + (long)a.cf2[comp] * a.c_stride;
-      for (int j0 = 0; j0 < a.H; j0 += 4) {
-        const int j = j0 + kk;
-        const double aw = tW[(long)j * K16 + arow];
-        const double ad = tD[(long)j * K16 + arow];
-        double f1s = 0.0, f1o = 0.0, f2s = 0.0, f2o = 0.0;
-        if (bval && j < a.H) {
-          const double f1N = F1[(long)(2 * j) * rowlen], f1S = F1[(long)(2 * j + 1) * rowlen];
-          const double f2N = F2[(long)(2 * j) * rowlen], f2S = F2[(long)(2 * j + 1) * rowlen];
-          f1s = f1N + sg * f1S;
-          f1o = f1N - sg * f1S;
-          f2s = f2N + sg * f2S;
-          f2o = f2N - sg * f2S;
+      } else {
+        const int c1 = a.cf1[comp], c2 = a.cf2[comp];
+        for (int j0 = 0; j0 < a.H; j0 += 4) {
+          const int j = j0 + kk;
+          const double aw = tW[(long)j * K16 + arow];
+          const double ad = tD[(long)j * K16 + arow];
+          double f1s = 0.0, f1o = 0.0, f2s = 0.0, f2o = 0.0;
+          if (bval && j < a.H) {
+            const long iN = fidx(a.c_stride, a.m_stride, a.Bin, 0, m, 2 * j, bl) + reim;
+            const long iS = fidx(a.c_stride, a.m_stride, a.Bin, 0, m, 2 * j + 1, bl) + reim;
+            const double f1N = a.F[c1 * a.c_stride + iN], f1S = a.F[c1 * a.c_stride + iS];
+            const double f2N = a.F[c2 * a.c_stride + iN], f2S = a.F[c2 * a.c_stride + iS];
+            f1s = f1N + sg * f1S;
+            f1o = f1N - sg * f1S;
+            f2s = f2N + sg * f2S;
+            f2o = f2N - sg * f2S;
+          }
+          acc[0] = mfma_f64(ad, f2o, acc[0]);  // wdP f2(opposite fold)
+          acc[1] = mfma_f64(ad, f1o, acc[1]);  // wdP f1(opposite fold)
+          acc[2] = mfma_f64(aw, f1s, acc[2]);  // wP f1(same fold)
+          acc[3] = mfma_f64(aw, f2s, acc[3]);  // wP f2(same fold)
         }
-        acc[0] = mfma_f64(ad, f2o, acc[0]);  // wdP f2(opposite fold)
-        acc[1] = mfma_f64(ad, f1o, acc[1]);  // wdP f1(opposite fold)
-        acc[2] = mfma_f64(aw, f1s, acc[2]);  // wP f1(same fold)
-        acc[3] = mfma_f64(aw, f2s, acc[3]);  // wP f2(same fold)
       }
     }
-  }
-  const double dm = (double)m;
-  const long lms = a.lm_start[m];
+    const double dm = (double)m;
+    const long lms = a.lm_start[m];
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int k = kbase + mfma_row(lane, i);
-    const int l = m + p + 2 * k;
-    const bool st = bval && (k < K);
-    const long off = (lms + p + 2 * k) * 2L * a.ldo + 2 * bl + reim;
-    if (!VEC) {
-      if (st) a.outS[comp][off] = acc[0][i];
-    } else {
-      const double inv = (l > 0) ? 1.0 / ((double)l * (double)(l + 1)) : 0.0;
-      const double Sv = acc[0][i] + imul_pair(acc[2][i], -dm, lane);
-      const double Tv = -acc[1][i] + imul_pair(acc[3][i], -dm, lane);
-      if (st) {
-        a.outS[comp][off] = Sv * inv;
-        if (a.outT[comp]) a.outT[comp][off] = Tv * inv;
+    for (int i = 0; i < 4; ++i) {
+      const int k = kbase + mfma_row(lane, i);
+      const int l = m + p + 2 * k;
+      const bool st = bval && (k < K);
+      const long off = (lms + p + 2 * k) * 2L * a.ldo + 2 * bl + reim;
+      if (!VEC) {
+        if (st) a.outS[comp][off] = acc[0][i];
+      } else {
+        const double inv = (l > 0) ? 1.0 / ((double)l * (double)(l + 1)) : 0.0;
+        const double Sv = acc[0][i] + imul_pair(acc[2][i], -dm, lane);
+        const double Tv = -acc[1][i] + imul_pair(acc[3][i], -dm, lane);
+        if (st) {
+          a.outS[comp][off] = Sv * inv;
+          if (a.outT[comp]) a.outT[comp][off] = Tv * inv;
+        }
       }
     }
   }
@@ -280,13 +290,15 @@ __global__ __launch_bounds__(256) void k_ana(AnaArgs a) {
 
 extern "C" int mh_leg_analysis(int vector, const MhAnaArgs* args, void* stream) {
   const AnaArgs& a = *args;
-  if (a.ncomp < 1 || a.ncomp > 3 || a.B < 1 || a.ntiles < 1 || a.Hp % 32 != 0 || a.Hp < a.H)
+  if (a.ncomp < 1 || a.ncomp > 3 || a.B < 1 || a.ntiles < 1 || a.Hp % 32 != 0 || a.Hp < a.H ||
+      a.n_m < 1)
     return MH_EINVAL;
+  const int n_m = a.n_m;
   dim3 grid(a.ncomp * mh_cdiv(a.B, 16), a.ntiles);
   if (vector)
-    hipLaunchKernelGGL(k_ana<true>, grid, dim3(256), 0, mh_stream(stream), a);
+    hipLaunchKernelGGL(k_ana<true>, grid, dim3(256), 0, mh_stream(stream), a, n_m);
   else
-    hipLaunchKernelGGL(k_ana<false>, grid, dim3(256), 0, mh_stream(stream), a);
+    hipLaunchKernelGGL(k_ana<false>, grid, dim3(256), 0, mh_stream(stream), a, n_m);
   return mh_launch_status();
 }
 

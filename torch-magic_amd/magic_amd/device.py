@@ -89,6 +89,11 @@ class Resolution:
         self.d["comp_scale_2"] = up(np.array([1, 1]), I32)
 
     # ------------------------------------------------------------ launchers
+    def fstrides(self, Bout):
+        """(component stride, m stride) in doubles of a Fourier buffer F[c][m][t][b]."""
+        ms = 2 * self.T * Bout
+        return self.M * ms, ms
+
     def synth(self, groups, B, ldx, F, Bout, vector: bool, or2=None):
         """Legendre synthesis.  groups: list of [(x1, x2, mode)] * (3 if vector else 1)."""
         a = _lib.SynArgs()
@@ -102,8 +107,7 @@ class Resolution:
         a.synP, a.synD = ptr(self.synP), ptr(self.synD)
         a.or2 = ptr(or2 if or2 is not None else self.d["or2"])
         a.F = ptr(F)
-        a.m_stride = 2 * self.T * Bout
-        a.c_stride = self.M * a.m_stride
+        a.c_stride, a.m_stride = self.fstrides(Bout)
         a.Bout = Bout
         _lib.call("mh_leg_synth", 1 if vector else 0, _lib.ref(a), self.M, stream_handle())
 
@@ -115,8 +119,7 @@ class Resolution:
         """
         a = _lib.AnaArgs()
         a.F = ptr(F)
-        a.m_stride = 2 * self.T * Bin
-        a.c_stride = self.M * a.m_stride
+        a.c_stride, a.m_stride = self.fstrides(Bin)
         a.Bin, a.B, a.ncomp = Bin, B, len(comps)
         for i, c in enumerate(comps):
             if vector:
@@ -130,6 +133,7 @@ class Resolution:
         a.mp_info, a.lm_start = ptr(self.d["mp_info"]), ptr(self.d["lm_start"])
         a.anaW, a.anaWd = ptr(self.anaW), ptr(self.anaWd)
         a.tiles, a.ntiles = ptr(self.d["ana_tiles"]), self.lay["n_ana_tiles"]
+        a.n_m = self.M
         _lib.call("mh_leg_analysis", 1 if vector else 0, _lib.ref(a), stream_handle())
 
     def fft(self, forward: bool, F, FB, G, ncomp, B, comp_scale=None):
@@ -152,6 +156,24 @@ class Resolution:
 
     def fourier_buffer(self, ncomp, B):
         return torch.empty(ncomp * self.M * self.T * B * 2, dtype=F64, device=self.device)
+
+    def grid_step(self, Fs, Fv, Fo, N, cfl, p, generic=False):
+        """Fused inverse FFT + nonlinear products + Courant maxima + forward FFT.
+
+        generic: use the kernel with a run-time FFT plan even where the library
+        has one compiled for this n_phi (both give the same results).
+        """
+        a = _lib.GridArgs()
+        a.Fs, a.Fv, a.Fo = ptr(Fs), ptr(Fv), ptr(Fo)
+        a.N, a.T, a.M, a.Mh, a.generic = N, self.T, self.M, self.Mh, int(generic)
+        a.tw, a.tw2 = ptr(self.d["tw"]), ptr(self.d["tw2"])
+        d = self.d
+        a.or2, a.or4, a.ost2 = ptr(d["or2"]), ptr(d["or4"]), ptr(d["ost2_grid"])
+        a.valri2, a.valhi2 = ptr(d["valri2"]), ptr(d["valhi2"])
+        a.LFfac, a.cf2, a.af2 = p.LFfac, p.cour_fac ** 2, p.alf_fac ** 2
+        a.l_mag, a.alf_damp = int(p.l_mag), int(p.l_cour_alf_damp)
+        a.cfl = ptr(cfl)
+        _lib.call("mh_grid_step", _lib.ref(a), stream_handle())
 
     def grid_buffer(self, ncomp, B):
         return torch.empty(ncomp * B * self.T * self.P, dtype=F64, device=self.device)

@@ -97,7 +97,7 @@ def conduction_state(p: Params, res: Resolution):
 class DynamoRun:
     """State and step of one Boussinesq Chebyshev CNAB2 dynamo run on the GPU."""
 
-    def __init__(self, cfg, device="cuda", res: Resolution | None = None):
+    def __init__(self, cfg, device="cuda", res: Resolution | None = None, fused_grid: bool = True):
         self.p = p = cfg if isinstance(cfg, Params) else Params(cfg)
         p.check_step_supported()
         self.device = require_gpu(device)
@@ -119,14 +119,21 @@ class DynamoRun:
         # radial-loop spectral outputs (boundary columns stay zero)
         self.nl = {k: z() for k in ("Advr", "Advt", "Advp", "dVSr", "VSt", "VxBr", "VxBt",
                                     "VxBp", "dVSr_dr", "dVxBt_dr")}
-        # grid-space work (FP64): Fourier buffers and grids
+        # grid-space work (FP64) between Fourier buffers F[c][m][t][b].  Fused
+        # (default): one kernel for inverse FFT + products + forward FFT, no grid in
+        # HBM.  Separate: grids G[c][b][t][phi] and three kernels.
+        self.fused_grid = fused_grid
+        self.grid_generic = False  # force the run-time FFT plan of the fused kernel
         self.F_s = r.fourier_buffer(1, N)
         self.F_v = r.fourier_buffer(3, 4 * N)
-        self.G_s = r.grid_buffer(1, N)
-        self.G_v = r.grid_buffer(3, 4 * N)
-        self.G_o = r.grid_buffer(9, self.Nb)
-        nFo = 9 * M * T * self.Nb * 2
-        self.F_o = self.F_v[:nFo] if nFo <= self.F_v.numel() else r.fourier_buffer(9, self.Nb)
+        if fused_grid:
+            self.F_o = r.fourier_buffer(9, self.Nb)
+        else:
+            self.G_s = r.grid_buffer(1, N)
+            self.G_v = r.grid_buffer(3, 4 * N)
+            self.G_o = r.grid_buffer(9, self.Nb)
+            nFo = 9 * M * T * self.Nb * 2
+            self.F_o = self.F_v[:nFo] if nFo <= self.F_v.numel() else r.fourier_buffer(9, self.Nb)
         self.cfl = torch.zeros(2 * N, dtype=torch.int64, device=dev)
         self.cfl_host = torch.zeros(2 * N, dtype=torch.int64).pin_memory()
         self._build_operator_storage()
@@ -459,26 +466,14 @@ class DynamoRun:
                   [(f["z"], None, 1), (f["dz"], None, 0), (f["ddw"], f["w"], 2)]]
         with self._t("leg_syn_vec"):
             r.synth(groups, N, N, self.F_v, 4 * N, vector=True)
-        # 2. inverse FFT
-        with self._t("fft_inv"):
-            r.fft(False, self.F_s, N, self.G_s, 1, N)
-            r.fft(False, self.F_v, 4 * N, self.G_v, 3, 4 * N)
-        # 3. nonlinear products + Courant maxima
-        a = _lib.NlArgs()
-        a.Gs, a.Gv, a.Gv_c_stride = ptr(self.G_s), ptr(self.G_v), 4 * N * r.T * r.P
-        a.Go, a.Go_c_stride = ptr(self.G_o), Nb * r.T * r.P
-        a.N, a.T, a.P = N, r.T, r.P
         d = r.d
-        a.or2, a.or4, a.ost2 = ptr(d["or2"]), ptr(d["or4"]), ptr(d["ost2_grid"])
-        a.valri2, a.valhi2 = ptr(d["valri2"]), ptr(d["valhi2"])
-        a.LFfac, a.cf2, a.af2 = self.p.LFfac, self.p.cour_fac ** 2, self.p.alf_fac ** 2
-        a.l_mag, a.alf_damp = int(self.p.l_mag), int(self.p.l_cour_alf_damp)
-        a.cfl = ptr(self.cfl)
-        with self._t("get_nl"):
-            _lib.call("mh_get_nl", _lib.ref(a), stream_handle())
-        # 4. forward FFT of the bulk products
-        with self._t("fft_fwd"):
-            r.fft(True, self.F_o, Nb, self.G_o, 9, Nb, comp_scale=d["comp_scale_v"])
+        if self.fused_grid:
+            # 2-4. fused inverse FFT + nonlinear products + Courant + forward FFT
+            with self._t("grid_step"):
+                r.grid_step(self.F_s, self.F_v, self.F_o, N, self.cfl, self.p,
+                            generic=self.grid_generic)
+        else:
+            self._grid_separate()
         # 5. Legendre analysis into (L, N) arrays at columns 1..N-2
         nl = self.nl
         off = 16  # one complex column
@@ -490,6 +485,32 @@ class DynamoRun:
             r.analysis(self.F_o, Nb, Nb, [(3, 4), (5, 6), (7, 8)],
                        [ptr(nl["Advt"]) + off, ptr(nl["VSt"]) + off, ptr(nl["VxBt"]) + off],
                        [ptr(nl["Advp"]) + off, None, ptr(nl["VxBp"]) + off], N, vector=True)
+        return self._radial_finish()
+
+    def _grid_separate(self):
+        r, N, Nb, d = self.res, self.N, self.Nb, self.res.d
+        # 2. inverse FFT
+        with self._t("fft_inv"):
+            r.fft(False, self.F_s, N, self.G_s, 1, N)
+            r.fft(False, self.F_v, 4 * N, self.G_v, 3, 4 * N)
+        # 3. nonlinear products + Courant maxima
+        a = _lib.NlArgs()
+        a.Gs, a.Gv, a.Gv_c_stride = ptr(self.G_s), ptr(self.G_v), 4 * N * r.T * r.P
+        a.Go, a.Go_c_stride = ptr(self.G_o), Nb * r.T * r.P
+        a.N, a.T, a.P = N, r.T, r.P
+        a.or2, a.or4, a.ost2 = ptr(d["or2"]), ptr(d["or4"]), ptr(d["ost2_grid"])
+        a.valri2, a.valhi2 = ptr(d["valri2"]), ptr(d["valhi2"])
+        a.LFfac, a.cf2, a.af2 = self.p.LFfac, self.p.cour_fac ** 2, self.p.alf_fac ** 2
+        a.l_mag, a.alf_damp = int(self.p.l_mag), int(self.p.l_cour_alf_damp)
+        a.cfl = ptr(self.cfl)
+        with self._t("get_nl"):
+            _lib.call("mh_get_nl", _lib.ref(a), stream_handle())
+        # 4. forward FFT of the bulk products
+        with self._t("fft_fwd"):
+            r.fft(True, self.F_o, Nb, self.G_o, 9, Nb, comp_scale=d["comp_scale_v"])
+
+    def _radial_finish(self):
+        r, f, N, nl = self.res, self.f, self.N, self.nl
         # 6. radial derivatives of the fluxes
         with self._t("rowop_dr"):
             self._rowop(self.jobs_td)
