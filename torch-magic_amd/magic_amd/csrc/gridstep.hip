@@ -38,7 +38,8 @@ constexpr int kLdsBytes = 160 * 1024 - 256;
 
 // FFT plan of the fused kernel for a half length Mh: radices in inverse order,
 // the last (RL) being the register pass; TP = Mh/RL threads per level; RB
-// levels per workgroup (two LDS buffers of RB rows of Mh+1 complex values).
+// levels per workgroup.  LDS: two buffers of RB rows of Mh+1 complex values,
+// then the twiddles exp(-2 pi i k/Mh) and exp(-2 pi i k/(2Mh)), k < Mh.
 struct GPlan {
   int RL, TP, RB, nst, threads;
   int radix[16];
@@ -60,7 +61,7 @@ constexpr GPlan make_plan(int Mh) {
   p.TP = Mh / RL;
   for (int rb = 16; rb >= 1; --rb) {
     const int th = (rb * p.TP + 63) / 64 * 64;
-    if (th <= kMaxThreads && 2 * rb * (Mh + 1) * 16 <= kLdsBytes) {
+    if (th <= kMaxThreads && (2 * rb * (Mh + 1) + 2 * Mh) * 16 <= kLdsBytes) {
       p.RB = rb;
       p.threads = th;
       break;
@@ -110,6 +111,8 @@ __device__ __forceinline__ Geo make_geo(const GridArgs& a, const GPlan& rp) {
 }
 
 __device__ __forceinline__ double2* lbuf(const Geo& g, int w) { return g_lds + w * g.RB * g.LS; }
+__device__ __forceinline__ const double2* ltw(const Geo& g) { return g_lds + 2 * g.RB * g.LS; }
+__device__ __forceinline__ const double2* ltw2(const Geo& g) { return ltw(g) + g.Mh; }
 
 struct Ctx {
   const GPlan* rp;  // run-time plan (MH == 0)
@@ -127,7 +130,7 @@ __device__ __forceinline__ double2 zload(const Geo& g, const GridArgs& a, const 
   const int kc = g.Mh - k;
   if (k > 0 && kc < g.M) xc = cconj(X[kc]);
   const double2 E = cadd(x, xc);
-  const double2 O = cmul(csub(x, xc), cconj(reinterpret_cast<const double2*>(a.tw2)[k]));
+  const double2 O = cmul(csub(x, xc), cconj(ltw2(g)[k]));
   return make_double2(E.x - O.y, E.y + O.x);
 }
 
@@ -153,7 +156,7 @@ template <int R>
 __device__ __forceinline__ void stage_mid(const Geo& g, const GridArgs& a, int win, int wout, int Ns,
                                           double dir) {
   const int nb = g.Mh / R, tstep = g.Mh / (Ns * R);
-  const double2* tw = reinterpret_cast<const double2*>(a.tw);
+  const double2* tw = ltw(g);
   const double2* in = lbuf(g, win);
   double2* out = lbuf(g, wout);
   for (int idx = threadIdx.x; idx < g.RB * nb; idx += g.threads) {
@@ -289,7 +292,7 @@ __device__ __forceinline__ void inverse(Ctx& c, const Geo& g, const GridArgs& a,
   // last pass (radix RL, Ns = TP): butterfly q of level `row`, twiddles w^{i q}
   if (c.act) {
     const double2* src = lbuf(g, w) + c.row * LS;
-    const double2* tw = reinterpret_cast<const double2*>(a.tw);
+    const double2* tw = ltw(g);
 #pragma unroll
     for (int i = 0; i < RL; ++i) {
       const int k = c.q + i * g.TP;
@@ -337,7 +340,7 @@ __device__ __forceinline__ void forward(Ctx& c, const Geo& g, const GridArgs& a,
   }
   // unpack X_m = E_m + e^{-2 pi i m/n} O_m and store the bulk levels
   const int RB = g.RB, Mh = g.Mh, N = a.N, No = N - 2, n = g.M * RB;
-  const double2* tw2 = reinterpret_cast<const double2*>(a.tw2);
+  const double2* tw2 = ltw2(g);
   double* Fo = a.Fo + (long)comp * g.M * a.T * No * 2;
   const double2* res = lbuf(g, w);
   for (int e = threadIdx.x; e < n; e += g.threads) {
@@ -380,6 +383,15 @@ __global__ __launch_bounds__(kMaxThreads) void k_grid_step(GridArgs a, GPlan rp)
   c.act = c.row < nvalid;
   c.busy = 0;
   if (threadIdx.x < 2 * RB) red[threadIdx.x] = 0ull;
+  {  // twiddles into LDS (visible after the first barrier)
+    double2* tl = g_lds + 2 * RB * g.LS;
+    const double2* t1 = reinterpret_cast<const double2*>(a.tw);
+    const double2* t2 = reinterpret_cast<const double2*>(a.tw2);
+    for (int k = threadIdx.x; k < g.Mh; k += g.threads) {
+      tl[k] = t1[k];
+      tl[g.Mh + k] = t2[k];
+    }
+  }
 
   const int r = r0 + (c.act ? c.row : 0);
   const double ost2 = a.ost2[t];
@@ -570,7 +582,7 @@ extern "C" int mh_grid_step(const MhGridArgs* args, void* stream) {
   if (P.RL == 0) return MH_EINVAL;
   // every thread covers its share of a component's M*RB raw values in RL loads
   if ((long)P.threads * P.RL < (long)a.M * P.RB) return MH_EINVAL;
-  const size_t lds = 2ull * P.RB * (a.Mh + 1) * sizeof(double2);
+  const size_t lds = (2ull * P.RB * (a.Mh + 1) + 2ull * a.Mh) * sizeof(double2);
   const int nRB = (a.N + P.RB - 1) / P.RB;
   const long nwg = (long)a.T * nRB;
   const long grid = ((nwg + 7) / 8) * 8;

@@ -101,56 +101,83 @@ __device__ __forceinline__ long fidx(long cs, long ms, int B, int c, int m, int 
   return c * cs + m * ms + (long)t * 2 * B + 2L * b;
 }
 
-// WG = 4 waves; wave w covers theta pairs j0 + 16*(w&1) .. and complex batch
-// columns b0 + 8*(w>>1) .. ; WG tile = 32 theta pairs x 16 batch columns.
+// WG = 4 waves; wave w covers theta pairs j0 + 16*(w&1) .. and complex
+// columns b0 + 8*(w>>1) .. of the groups laid side by side (column bc is
+// batch column bc % B of group bc / B, which is also its column in F);
+// WG tile = 32 theta pairs x 16 complex columns.
+__device__ __forceinline__ SynIn syn_in(const SynArgs& a, int g, int c) {
+  // per-lane group select without dynamic indexing of the argument struct
+  SynIn r = a.in[0][c];
+  if (g == 1) r = a.in[1][c];
+  if (g == 2) r = a.in[2][c];
+  if (g == 3) r = a.in[3][c];
+  return r;
+}
+
 template <bool VEC>
-__global__ __launch_bounds__(256) void k_syn(SynArgs a, int n_m) {
+__global__ __launch_bounds__(256, 3) void k_syn(SynArgs a, int n_m) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int nbt = (a.B + 15) / 16;
-  const int g = blockIdx.x / nbt;
-  const int b0 = (blockIdx.x % nbt) * 16 + 8 * (w >> 1);
+  const int b0 = blockIdx.x * 16 + 8 * (w >> 1);
   const int j0 = blockIdx.y * 32 + 16 * (w & 1);
   const int col = lane & 15, kk = lane >> 4;
-  const int b = b0 + (col >> 1), reim = col & 1;
-  const bool bval = b < a.B;
-  const int bl = bval ? b : 0;
-  const SynIn* in = a.in[g];
+  const int bc = b0 + (col >> 1), reim = col & 1;
+  const bool bval = bc < a.ngroups * a.B;
+  const int g = bval ? bc / a.B : 0;
+  const int bl = bval ? bc - g * a.B : 0;
+  const SynIn in[3] = {syn_in(a, g, 0), syn_in(a, g, 1), syn_in(a, g, 2)};
   {
     const int m = blockIdx.z;
     const long lms = a.lm_start[m];
     double4_t acc[VEC ? 10 : 2];
 #pragma unroll
     for (int q = 0; q < (VEC ? 10 : 2); ++q) acc[q] = (double4_t){0.0, 0.0, 0.0, 0.0};
+    // Operands of one k-step (4 degrees); the next step's are loaded before
+    // the current step's MFMAs are issued, so their latency is hidden.
+    struct Ops {
+      double aP, aD, xq, xs, xt;
+    };
 #pragma unroll
     for (int p = 0; p < 2; ++p) {
       const int* info = a.mp_info + (m * 2 + p) * 4;
       const int K = info[0];
       const double* tP = a.synP + info[1];
       const double* tD = a.synD + info[1];
-      for (int k0 = 0; k0 < K; k0 += 4) {
+      auto load = [&](int k0) {
+        Ops o;
         const int k = k0 + kk;
-        const double aP = tP[(long)k * a.Hp + j0 + col];
         const bool v = bval && (k < K);
         const int l = m + p + 2 * k;
         const long lm = lms + p + 2 * k;
-        if (!VEC) {
-          const double x = v ? syn_load(in[0], lm, l, bl, reim, a.ldx, a.or2) : 0.0;
-          acc[p] = mfma_f64(aP, x, acc[p]);
+        o.aP = tP[(long)k * a.Hp + j0 + col];
+        o.xq = v ? syn_load(in[0], lm, l, bl, reim, a.ldx, a.or2) : 0.0;
+        if (VEC) {
+          o.aD = tD[(long)k * a.Hp + j0 + col];
+          o.xs = v ? syn_load(in[1], lm, l, bl, reim, a.ldx, a.or2) : 0.0;
+          o.xt = v ? syn_load(in[2], lm, l, bl, reim, a.ldx, a.or2) : 0.0;
         } else {
-          const double aD = tD[(long)k * a.Hp + j0 + col];
-          const double xq = v ? syn_load(in[0], lm, l, bl, reim, a.ldx, a.or2) : 0.0;
-          const double xs = v ? syn_load(in[1], lm, l, bl, reim, a.ldx, a.or2) : 0.0;
-          const double xt = v ? syn_load(in[2], lm, l, bl, reim, a.ldx, a.or2) : 0.0;
-          acc[5 * p + 0] = mfma_f64(aP, xq, acc[5 * p + 0]);  // P Q
-          acc[5 * p + 1] = mfma_f64(aP, xt, acc[5 * p + 1]);  // P T
-          acc[5 * p + 2] = mfma_f64(aP, xs, acc[5 * p + 2]);  // P S
-          acc[5 * p + 3] = mfma_f64(aD, xs, acc[5 * p + 3]);  // dP S
-          acc[5 * p + 4] = mfma_f64(aD, xt, acc[5 * p + 4]);  // dP T
+          o.aD = o.xs = o.xt = 0.0;
         }
+        return o;
+      };
+      if (K <= 0) continue;
+      Ops cur = load(0);
+      for (int k0 = 0; k0 < K; k0 += 4) {
+        Ops nxt = cur;
+        if (k0 + 4 < K) nxt = load(k0 + 4);
+        if (!VEC) {
+          acc[p] = mfma_f64(cur.aP, cur.xq, acc[p]);
+        } else {
+          acc[5 * p + 0] = mfma_f64(cur.aP, cur.xq, acc[5 * p + 0]);  // P Q
+          acc[5 * p + 1] = mfma_f64(cur.aP, cur.xt, acc[5 * p + 1]);  // P T
+          acc[5 * p + 2] = mfma_f64(cur.aP, cur.xs, acc[5 * p + 2]);  // P S
+          acc[5 * p + 3] = mfma_f64(cur.aD, cur.xs, acc[5 * p + 3]);  // dP S
+          acc[5 * p + 4] = mfma_f64(cur.aD, cur.xt, acc[5 * p + 4]);  // dP T
+        }
+        cur = nxt;
       }
     }
     const double dm = (double)m;
-    const int bo = g * a.B + bl;
+    const int bo = bval ? bc : 0;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int j = j0 + mfma_row(lane, i);
@@ -186,9 +213,10 @@ __global__ __launch_bounds__(256) void k_syn(SynArgs a, int n_m) {
 
 extern "C" int mh_leg_synth(int vector, const MhSynArgs* args, int n_m, void* stream) {
   const SynArgs& a = *args;
-  if (a.ngroups < 1 || a.ngroups > 4 || a.B < 1 || a.H < 1 || a.Hp % 32 != 0 || a.Hp < a.H)
+  if (a.ngroups < 1 || a.ngroups > 4 || a.B < 1 || a.H < 1 || a.Hp % 32 != 0 || a.Hp < a.H ||
+      a.Bout != a.ngroups * a.B)
     return MH_EINVAL;
-  dim3 grid(a.ngroups * mh_cdiv(a.B, 16), mh_cdiv(a.H, 32), n_m);
+  dim3 grid(mh_cdiv(a.ngroups * a.B, 16), mh_cdiv(a.H, 32), n_m);
   if (vector)
     hipLaunchKernelGGL(k_syn<true>, grid, dim3(256), 0, mh_stream(stream), a, n_m);
   else
@@ -203,7 +231,7 @@ typedef MhAnaArgs AnaArgs;
 // WG = 4 waves: wave w covers k0 + 16*(w&1) .. (16 rows of l) and complex batch
 // columns b0 + 8*(w>>1) ..; K loop runs over theta pairs in steps of 4.
 template <bool VEC>
-__global__ __launch_bounds__(256) void k_ana(AnaArgs a, int n_m) {
+__global__ __launch_bounds__(256, 6) void k_ana(AnaArgs a, int n_m) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int* tl = a.tiles + 3 * blockIdx.y;
   const int p = tl[1];
@@ -228,41 +256,45 @@ __global__ __launch_bounds__(256) void k_ana(AnaArgs a, int n_m) {
 #pragma unroll
     for (int q = 0; q < (VEC ? 4 : 1); ++q) acc[q] = (double4_t){0.0, 0.0, 0.0, 0.0};
     if (kbase < K) {
-      if (!VEC) {
-        const int c1 = a.cf1[comp];
-        for (int j0 = 0; j0 < a.H; j0 += 4) {
-          const int j = j0 + kk;
-          const double aw = tW[(long)j * K16 + arow];
-          double x = 0.0;
-          if (bval && j < a.H) {
-            const double fN = a.F[fidx(a.c_stride, a.m_stride, a.Bin, c1, m, 2 * j, bl) + reim];
-            const double fS = a.F[fidx(a.c_stride, a.m_stride, a.Bin, c1, m, 2 * j + 1, bl) + reim];
-            x = fN + sg * fS;
-          }
-          acc[0] = mfma_f64(aw, x, acc[0]);
-        }
-      } else {
-        const int c1 = a.cf1[comp], c2 = a.cf2[comp];
-        for (int j0 = 0; j0 < a.H; j0 += 4) {
-          const int j = j0 + kk;
-          const double aw = tW[(long)j * K16 + arow];
-          const double ad = tD[(long)j * K16 + arow];
-          double f1s = 0.0, f1o = 0.0, f2s = 0.0, f2o = 0.0;
-          if (bval && j < a.H) {
-            const long iN = fidx(a.c_stride, a.m_stride, a.Bin, 0, m, 2 * j, bl) + reim;
-            const long iS = fidx(a.c_stride, a.m_stride, a.Bin, 0, m, 2 * j + 1, bl) + reim;
-            const double f1N = a.F[c1 * a.c_stride + iN], f1S = a.F[c1 * a.c_stride + iS];
+      // Operands of one theta step (4 pairs); the next step's are loaded before
+      // the current step's MFMAs are issued.
+      struct Ops {
+        double aw, ad, x1, x2, x3, x4;
+      };
+      const int c1 = a.cf1[comp], c2 = VEC ? a.cf2[comp] : 0;
+      auto load = [&](int j0) {
+        Ops o;
+        const int j = j0 + kk;
+        o.aw = tW[(long)j * K16 + arow];
+        o.ad = VEC ? tD[(long)j * K16 + arow] : 0.0;
+        o.x1 = o.x2 = o.x3 = o.x4 = 0.0;
+        if (bval && j < a.H) {
+          const long iN = fidx(a.c_stride, a.m_stride, a.Bin, 0, m, 2 * j, bl) + reim;
+          const long iS = fidx(a.c_stride, a.m_stride, a.Bin, 0, m, 2 * j + 1, bl) + reim;
+          const double f1N = a.F[c1 * a.c_stride + iN], f1S = a.F[c1 * a.c_stride + iS];
+          o.x1 = f1N + sg * f1S;  // same fold
+          if (VEC) {
             const double f2N = a.F[c2 * a.c_stride + iN], f2S = a.F[c2 * a.c_stride + iS];
-            f1s = f1N + sg * f1S;
-            f1o = f1N - sg * f1S;
-            f2s = f2N + sg * f2S;
-            f2o = f2N - sg * f2S;
+            o.x2 = f1N - sg * f1S;  // opposite fold
+            o.x3 = f2N + sg * f2S;
+            o.x4 = f2N - sg * f2S;
           }
-          acc[0] = mfma_f64(ad, f2o, acc[0]);  // wdP f2(opposite fold)
-          acc[1] = mfma_f64(ad, f1o, acc[1]);  // wdP f1(opposite fold)
-          acc[2] = mfma_f64(aw, f1s, acc[2]);  // wP f1(same fold)
-          acc[3] = mfma_f64(aw, f2s, acc[3]);  // wP f2(same fold)
         }
+        return o;
+      };
+      Ops cur = load(0);
+      for (int j0 = 0; j0 < a.H; j0 += 4) {
+        Ops nxt = cur;
+        if (j0 + 4 < a.H) nxt = load(j0 + 4);
+        if (!VEC) {
+          acc[0] = mfma_f64(cur.aw, cur.x1, acc[0]);
+        } else {
+          acc[0] = mfma_f64(cur.ad, cur.x4, acc[0]);  // wdP f2(opposite fold)
+          acc[1] = mfma_f64(cur.ad, cur.x2, acc[1]);  // wdP f1(opposite fold)
+          acc[2] = mfma_f64(cur.aw, cur.x1, acc[2]);  // wP f1(same fold)
+          acc[3] = mfma_f64(cur.aw, cur.x3, acc[3]);  // wP f2(same fold)
+        }
+        cur = nxt;
       }
     }
     const double dm = (double)m;

@@ -205,11 +205,192 @@ __global__ __launch_bounds__(256) void k_gj_inverse(const double* __restrict__ A
   if (threadIdx.x == 0 && info) info[bm] = s_info;
 }
 
+
+// ------------------------------------------------- register-resident Gauss-Jordan
+// One workgroup of TR x TC threads holds the whole (equilibrated) matrix in
+// registers: thread (tr, tc) owns rows tr + TR*u (u < RPT) and columns
+// tc + TC*v (v < CPT).  In-place Gauss-Jordan with implicit partial pivoting
+// (first maximum of |a[i][k]| over the rows not yet used): at step k the
+// pivot column and the normalised pivot row are broadcast through LDS and
+// every thread updates its block; column k takes the overlay of the identity.
+// With r_k the pivot row of step k and pi(r_k) = k, the array ends holding
+// storage[s][c] = inv[pi(s)][r_c], which the final store un-permutes.  Two
+// barriers per step (LDS broadcast buffers alternate between steps).
+template <int TR, int TC, int RPT, int CPT>
+__global__ __launch_bounds__(TR * TC) void k_gj_reg(const double* A, double* out, int n,
+                                                    int precond, int* info) {
+  constexpr int RMAX = TR * RPT, CMAX = TC * CPT;
+  __shared__ double colbuf[2][RMAX];
+  __shared__ double rowbuf[2][CMAX];
+  __shared__ double cval[2][TR];
+  __shared__ int cidx[2][TR];
+  __shared__ int piv_row[CMAX];
+  __shared__ int step_of[RMAX];
+  __shared__ double s_fr[RMAX];
+  __shared__ unsigned long long s_fc[CMAX];
+  const int tid = threadIdx.x, tr = tid / TC, tc = tid - tr * TC;
+  const long nn = (long)n * n;
+  const double* Am = A + (long)blockIdx.x * nn;
+  double a[RPT][CPT];
+#pragma unroll
+  for (int u = 0; u < RPT; ++u)
+#pragma unroll
+    for (int v = 0; v < CPT; ++v) {
+      const int i = tr + TR * u, j = tc + TC * v;
+      a[u][v] = (i < n && j < n) ? Am[(long)i * n + j] : 0.0;
+    }
+  for (int j = tid; j < CMAX; j += TR * TC) s_fc[j] = 0ull;
+  // row equilibration: the TC threads of a row group are consecutive lanes
+  double fr[RPT];
+#pragma unroll
+  for (int u = 0; u < RPT; ++u) {
+    double mx = 0.0;
+#pragma unroll
+    for (int v = 0; v < CPT; ++v) mx = fmax(mx, fabs(a[u][v]));
+    for (int o = TC / 2; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o));
+    fr[u] = (precond >= 1 && mx > 0.0) ? 1.0 / mx : 1.0;
+#pragma unroll
+    for (int v = 0; v < CPT; ++v) a[u][v] *= fr[u];
+    const int i = tr + TR * u;
+    if (tc == 0 && i < RMAX) s_fr[i] = fr[u];
+  }
+  __syncthreads();
+  // column equilibration (max over the TR row groups through LDS)
+#pragma unroll
+  for (int v = 0; v < CPT; ++v) {
+    double mx = 0.0;
+#pragma unroll
+    for (int u = 0; u < RPT; ++u) mx = fmax(mx, fabs(a[u][v]));
+    atomicMax(&s_fc[tc + TC * v], (unsigned long long)__double_as_longlong(mx));
+  }
+  __syncthreads();
+  double fc[CPT];
+#pragma unroll
+  for (int v = 0; v < CPT; ++v) {
+    const double mx = __longlong_as_double((long long)s_fc[tc + TC * v]);
+    fc[v] = (precond >= 2 && mx > 0.0) ? 1.0 / mx : 1.0;
+#pragma unroll
+    for (int u = 0; u < RPT; ++u) a[u][v] *= fc[v];
+  }
+  __syncthreads();  // all reads of s_fc done before it is rewritten below
+#pragma unroll
+  for (int v = 0; v < CPT; ++v)
+    if (tr == 0) s_fc[tc + TC * v] = (unsigned long long)__double_as_longlong(fc[v]);
+  unsigned used = 0u;  // bit u: row tr + TR*u has been a pivot row
+  int first_zero = 0;
+  for (int k = 0; k < n; ++k) {
+    const int buf = k & 1;
+    const int kc = k % TC, kv = k / TC;
+    // (a) holders of column k: candidates, column values, clear column k
+    if (tc == kc) {
+      double best = -1.0;
+      int bi = 0x7fffffff;
+#pragma unroll
+      for (int u = 0; u < RPT; ++u) {
+        const int i = tr + TR * u;
+        double x = 0.0;
+#pragma unroll
+        for (int v = 0; v < CPT; ++v)
+          if (v == kv) x = a[u][v];
+        if (i < RMAX) colbuf[buf][i] = x;
+        if (i < n && !((used >> u) & 1u) && fabs(x) > best) {
+          best = fabs(x);
+          bi = i;
+        }
+#pragma unroll
+        for (int v = 0; v < CPT; ++v)
+          if (v == kv) a[u][v] = 0.0;
+      }
+      cval[buf][tr] = best;
+      cidx[buf][tr] = bi;
+    }
+    __syncthreads();
+    // (b) pivot (same decision in every thread), normalised pivot row
+    double best = -1.0;
+    int r = 0x7fffffff;
+#pragma unroll
+    for (int q = 0; q < TR; ++q) {
+      const double cv = cval[buf][q];
+      const int ci = cidx[buf][q];
+      if (cv > best || (cv == best && ci < r)) {
+        best = cv;
+        r = ci;
+      }
+    }
+    double pv = colbuf[buf][r];
+    if (!(fabs(pv) > 1e-15)) {
+      if (first_zero == 0) first_zero = k + 1;
+      if (pv == 0.0) pv = 1.0;
+    }
+    const double pinv = 1.0 / pv;
+    const int rtr = r % TR, ru = r / TR;
+    if (tr == rtr) {
+#pragma unroll
+      for (int u = 0; u < RPT; ++u)
+        if (u == ru) {
+          used |= 1u << u;
+#pragma unroll
+          for (int v = 0; v < CPT; ++v) {
+            const int j = tc + TC * v;
+            rowbuf[buf][j] = (j == k) ? pinv : a[u][v] * pinv;
+          }
+        }
+    }
+    if (tid == 0) {
+      piv_row[k] = r;
+      step_of[r] = k;
+    }
+    __syncthreads();
+    // (c) elimination
+    double rw[CPT];
+#pragma unroll
+    for (int v = 0; v < CPT; ++v) rw[v] = rowbuf[buf][tc + TC * v];
+#pragma unroll
+    for (int u = 0; u < RPT; ++u) {
+      const int i = tr + TR * u;
+      if (i == r) {
+#pragma unroll
+        for (int v = 0; v < CPT; ++v) a[u][v] = rw[v];
+      } else {
+        const double f = (i < RMAX) ? colbuf[buf][i] : 0.0;
+#pragma unroll
+        for (int v = 0; v < CPT; ++v) a[u][v] -= f * rw[v];
+      }
+    }
+  }
+  __syncthreads();
+  // out[pi(s)][r_c] = fc[pi(s)] * storage[s][c] * fr[r_c] (the equilibration of
+  // row pi(s) and column r_c of the original system)
+  double* O = out + (long)blockIdx.x * nn;
+#pragma unroll
+  for (int u = 0; u < RPT; ++u) {
+    const int s = tr + TR * u;
+    if (s >= n) continue;
+    const int oi = step_of[s];
+    const double fci = __longlong_as_double((long long)s_fc[oi]);
+#pragma unroll
+    for (int v = 0; v < CPT; ++v) {
+      const int c = tc + TC * v;
+      if (c >= n) continue;
+      const int oj = piv_row[c];
+      O[(long)oi * n + oj] = fci * a[u][v] * s_fr[oj];
+    }
+  }
+  if (tid == 0 && info) info[blockIdx.x] = first_zero;
+}
+
 extern "C" int mh_gj_inverse(const double* A, double* out, double* work, double* scal, int n,
                              int nbatch, int precond, int* info, void* stream) {
   if (n < 1 || nbatch < 1) return MH_EINVAL;
-  hipLaunchKernelGGL(k_gj_inverse, dim3(nbatch), dim3(256), 0, mh_stream(stream), A, out, work, scal, n,
-                     precond, info);
+  if (n <= 112)
+    hipLaunchKernelGGL((k_gj_reg<16, 16, 7, 7>), dim3(nbatch), dim3(256), 0, mh_stream(stream), A,
+                       out, n, precond, info);
+  else if (n <= 208)
+    hipLaunchKernelGGL((k_gj_reg<16, 32, 13, 7>), dim3(nbatch), dim3(512), 0, mh_stream(stream), A,
+                       out, n, precond, info);
+  else
+    hipLaunchKernelGGL(k_gj_inverse, dim3(nbatch), dim3(256), 0, mh_stream(stream), A, out, work,
+                       scal, n, precond, info);
   return mh_launch_status();
 }
 
