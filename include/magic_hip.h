@@ -55,7 +55,8 @@ typedef struct {
   const double* or2;
   double* F;
   long c_stride, m_stride;
-  int Bout, pad;
+  int Bout;
+  int prec; /* 0: FP64 tables and MFMA; 1: FP32 tables (synP/synD hold floats), FP32 MFMA */
 } MhSynArgs;
 
 /* Legendre synthesis on FP64 MFMA: scalar (vector=0) replaces the bmm part of
@@ -77,13 +78,19 @@ typedef struct {
   const double* anaW;
   const double* anaWd;
   const int* tiles; /* (m, p, k0) triples */
-  int ntiles, pad2;
+  int ntiles;
+  int prec; /* as MhSynArgs.prec (anaW/anaWd hold floats for 1) */
 } MhAnaArgs;
 
 /* Legendre analysis on FP64 MFMA: scalar replaces the bmm part of scal_to_SH
  * (sht.py:351-405); vector replaces spat_to_sphertor's products and the
  * division by l(l+1) (sht.py:575-667). */
 int mh_leg_analysis(int vector, const MhAnaArgs* args, void* stream);
+
+/* dst[i] = (float)src[i], i < n: the FP32 copies of the Legendre tables used by
+ * the mixed-precision transforms (no reference counterpart: the reference's
+ * transforms are FP64 only, sht.py:24-296). */
+int mh_tables_to_f32(const double* src, float* dst, long n, void* stream);
 
 /* ------------------------------------------------------------ FFT */
 typedef struct {

@@ -152,6 +152,35 @@ def test_sht_round_trip_lmax383():
     assert np.abs(T2 - T).max() / np.abs(T).max() < 1e-11
 
 
+@pytest.mark.parametrize("l_max", [63, 383])
+def test_sht_mixed_precision_against_fp64(l_max):
+    """FP32-MFMA Legendre transforms ("mixed") against the FP64 transforms on
+    (l+1)^-2 spectra (the config-2 input law): each of the four transforms and
+    the round trip stay within 1e-5 of the largest value (FP32 tables, inputs
+    rounded to FP32, FP32 accumulation over at most l_max/2 degrees or n_theta/2
+    nodes).  The FP64 transforms are pinned against the oracle above."""
+    torch = _torch()
+    from magic_amd.params import Params
+    from magic_amd.device import Resolution
+    from magic_amd.sht import SHT
+    p = Params({"l_max": l_max, "n_r_max": 9})
+    res = Resolution(p)
+    s64, s32 = SHT(res), SHT(res, precision="mixed")
+    rel = lambda a, b: float((a - b).abs().max() / b.abs().max())
+    S = torch.from_numpy(_rand_spec(p.lm_max, 5, res.maps, 7)).cuda()
+    T = torch.from_numpy(_rand_spec(p.lm_max, 5, res.maps, 8)).cuda()
+    Q = torch.from_numpy(_rand_spec(p.lm_max, 5, res.maps, 9)).cuda()
+    g64 = s64.scal_to_spat(S)
+    assert rel(s32.scal_to_spat(S), g64) < 1e-5
+    assert rel(s32.scal_to_SH(g64), s64.scal_to_SH(g64)) < 1e-5
+    v64 = s64.torpol_to_spat(Q, S, T)
+    for a, b in zip(s32.torpol_to_spat(Q, S, T), v64):
+        assert rel(a, b) < 1e-5
+    for a, b in zip(s32.spat_to_sphertor(v64[1], v64[2]), s64.spat_to_sphertor(v64[1], v64[2])):
+        assert rel(a, b) < 1e-5
+    assert rel(s32.scal_to_SH(s32.scal_to_spat(S)), S) < 1e-5
+
+
 @pytest.mark.parametrize("n", [5, 97, 112, 113, 194, 208, 230])
 def test_gj_inverse_matches_numpy(n):
     """Batched equilibrated Gauss-Jordan inverse (register-resident for n <= 208,

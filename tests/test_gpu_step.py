@@ -23,9 +23,9 @@ def _load(name):
     return case, np.load(os.path.join(GOLD, f"traj_{name}.npz"))
 
 
-def _run(case, fused=True):
+def _run(case, fused=True, precision="fp64"):
     from magic_amd.dynamo import DynamoRun
-    run = DynamoRun(case["cfg"], fused_grid=fused)
+    run = DynamoRun(dict(case["cfg"], precision=precision), fused_grid=fused)
     nz = case["noise"]
     if nz:
         run.initialize_fields(noise_amp=nz["amp"], noise_seed=nz["seed"])
@@ -55,6 +55,25 @@ def test_step_matches_reference_trajectory(name, fused):
             for k in ("s", "z", "w", "p", "b", "aj"):
                 got = run.f[k].cpu().numpy()
                 assert _rel(got, g[f"{k}_step{n}"]) < 1e-10, (n, k, _rel(got, g[f"{k}_step{n}"]))
+
+
+@pytest.mark.parametrize("name", ["bench16", "c1"])
+def test_mixed_precision_step_tracks_reference(name):
+    """Mixed precision (FP32-MFMA Legendre transforms, everything else FP64)
+    against the FP64 reference trajectory: energies within 1e-6 relative and
+    fields within 1e-5 of their largest coefficient at every recorded step."""
+    case, g = _load(name)
+    run = _run(case, precision="mixed")
+    for n in range(1, case["steps"] + 1):
+        dt = run.one_step(n, case["dt"])
+        assert dt == pytest.approx(g["dt"][n - 1], rel=1e-6)
+        e = np.array(run.energies())
+        ge = g["energies"][n - 1]
+        assert np.all(np.abs(e - ge) <= 1e-6 * np.abs(ge)), (n, e, ge)
+        if n in case["field_steps"]:
+            for k in ("s", "z", "w", "p", "b", "aj"):
+                got = run.f[k].cpu().numpy()
+                assert _rel(got, g[f"{k}_step{n}"]) < 1e-5, (n, k, _rel(got, g[f"{k}_step{n}"]))
 
 
 def test_first_step_explicit_terms():

@@ -31,14 +31,14 @@ class SynIn(C.Structure):
 class SynArgs(C.Structure):
     _fields_ = [("in_", (SynIn * 3) * 4), ("ngroups", I), ("B", I), ("ldx", I), ("H", I),
                 ("Hp", I), ("mp_info", P), ("lm_start", P), ("synP", P), ("synD", P),
-                ("or2", P), ("F", P), ("c_stride", L), ("m_stride", L), ("Bout", I), ("pad", I)]
+                ("or2", P), ("F", P), ("c_stride", L), ("m_stride", L), ("Bout", I), ("prec", I)]
 
 
 class AnaArgs(C.Structure):
     _fields_ = [("F", P), ("c_stride", L), ("m_stride", L), ("Bin", I), ("B", I), ("ncomp", I),
                 ("cf1", I * 3), ("cf2", I * 3), ("pad0", I), ("outS", P * 3), ("outT", P * 3),
                 ("ldo", I), ("H", I), ("Hp", I), ("n_m", I), ("mp_info", P), ("lm_start", P),
-                ("anaW", P), ("anaWd", P), ("tiles", P), ("ntiles", I), ("pad2", I)]
+                ("anaW", P), ("anaWd", P), ("tiles", P), ("ntiles", I), ("prec", I)]
 
 
 class FftArgs(C.Structure):
@@ -135,6 +135,7 @@ _SIGS = {
     "mh_fft": [I, P, P],
     "mh_get_nl": [P, P],
     "mh_grid_step": [P, P],
+    "mh_tables_to_f32": [P, P, L, P],
     "mh_rowop": [P, P, I, P],
     "mh_td": [P, P],
     "mh_rhs": [P, P],

@@ -7,6 +7,7 @@
 #include "magic_hip.h"
 
 typedef double double4_t __attribute__((ext_vector_type(4)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
 
 // Return the launch error (if any) of the kernel just enqueued.
 static inline int mh_launch_status() {
@@ -42,6 +43,14 @@ __device__ __forceinline__ double2 cimul(double2 z, double m) {
 
 // Row of the 16x16 FP64 accumulator held in element v of this lane.
 __device__ __forceinline__ int mfma_row(int lane, int v) { return (lane >> 4) + 4 * v; }
+
+// D(16x16) += A(16x4) * B(4x16) in FP32 on the matrix cores, FP32 accumulation.
+// A: row = lane%16, k = lane/16; B: col = lane%16, k = lane/16; element v of
+// a lane of C/D is row 4*(lane/16) + v, col lane%16 (see mfma_row_f32).
+__device__ __forceinline__ float4_t mfma_f32(float a, float b, float4_t c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ int mfma_row_f32(int lane, int v) { return 4 * (lane >> 4) + v; }
 
 static inline int mh_cdiv(int a, int b) { return (a + b - 1) / b; }
 

@@ -1,4 +1,5 @@
-// Legendre tables and FP64 MFMA Legendre synthesis / analysis for gfx950.
+// Legendre tables and MFMA Legendre synthesis / analysis (FP64, or FP32 with
+// FP32 accumulation) for gfx950.
 //
 // Table layouts (per azimuthal order m and parity p = (l-m)%2, l = m+p+2k):
 //   synthesis  syn[m][p][k][j]  (k < K4 = roundup(K,4), j < Hp)  j fastest
@@ -101,6 +102,22 @@ __device__ __forceinline__ long fidx(long cs, long ms, int B, int c, int m, int 
   return c * cs + m * ms + (long)t * 2 * B + 2L * b;
 }
 
+// Element type of the Legendre contraction: FP64 (16x16x4 f64 MFMA) or FP32
+// (16x16x4 f32 MFMA, FP32 accumulation; tables stored in FP32, inputs rounded
+// to FP32 as they are loaded).
+template <typename T> struct Mfma;
+template <> struct Mfma<double> {
+  typedef double4_t acc_t;
+  static __device__ __forceinline__ acc_t run(double a, double b, acc_t c) { return mfma_f64(a, b, c); }
+  // row of the 16x16 accumulator held in element v of a lane
+  static __device__ __forceinline__ int row(int lane, int v) { return mfma_row(lane, v); }
+};
+template <> struct Mfma<float> {
+  typedef float4_t acc_t;
+  static __device__ __forceinline__ acc_t run(float a, float b, acc_t c) { return mfma_f32(a, b, c); }
+  static __device__ __forceinline__ int row(int lane, int v) { return mfma_row_f32(lane, v); }
+};
+
 // WG = 4 waves; wave w covers theta pairs j0 + 16*(w&1) .. and complex
 // columns b0 + 8*(w>>1) .. of the groups laid side by side (column bc is
 // batch column bc % B of group bc / B, which is also its column in F);
@@ -114,8 +131,10 @@ __device__ __forceinline__ SynIn syn_in(const SynArgs& a, int g, int c) {
   return r;
 }
 
-template <bool VEC>
+template <bool VEC, typename T>
 __global__ __launch_bounds__(256, 3) void k_syn(SynArgs a, int n_m) {
+  typedef Mfma<T> MF;
+  typedef typename MF::acc_t acc_t;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int b0 = blockIdx.x * 16 + 8 * (w >> 1);
   const int j0 = blockIdx.y * 32 + 16 * (w & 1);
@@ -128,20 +147,20 @@ __global__ __launch_bounds__(256, 3) void k_syn(SynArgs a, int n_m) {
   {
     const int m = blockIdx.z;
     const long lms = a.lm_start[m];
-    double4_t acc[VEC ? 10 : 2];
+    acc_t acc[VEC ? 10 : 2];
 #pragma unroll
-    for (int q = 0; q < (VEC ? 10 : 2); ++q) acc[q] = (double4_t){0.0, 0.0, 0.0, 0.0};
+    for (int q = 0; q < (VEC ? 10 : 2); ++q) acc[q] = (acc_t){0, 0, 0, 0};
     // Operands of one k-step (4 degrees); the next step's are loaded before
     // the current step's MFMAs are issued, so their latency is hidden.
     struct Ops {
-      double aP, aD, xq, xs, xt;
+      T aP, aD, xq, xs, xt;
     };
 #pragma unroll
     for (int p = 0; p < 2; ++p) {
       const int* info = a.mp_info + (m * 2 + p) * 4;
       const int K = info[0];
-      const double* tP = a.synP + info[1];
-      const double* tD = a.synD + info[1];
+      const T* tP = reinterpret_cast<const T*>(a.synP) + info[1];
+      const T* tD = reinterpret_cast<const T*>(a.synD) + info[1];
       auto load = [&](int k0) {
         Ops o;
         const int k = k0 + kk;
@@ -149,13 +168,13 @@ __global__ __launch_bounds__(256, 3) void k_syn(SynArgs a, int n_m) {
         const int l = m + p + 2 * k;
         const long lm = lms + p + 2 * k;
         o.aP = tP[(long)k * a.Hp + j0 + col];
-        o.xq = v ? syn_load(in[0], lm, l, bl, reim, a.ldx, a.or2) : 0.0;
+        o.xq = v ? (T)syn_load(in[0], lm, l, bl, reim, a.ldx, a.or2) : (T)0;
         if (VEC) {
           o.aD = tD[(long)k * a.Hp + j0 + col];
-          o.xs = v ? syn_load(in[1], lm, l, bl, reim, a.ldx, a.or2) : 0.0;
-          o.xt = v ? syn_load(in[2], lm, l, bl, reim, a.ldx, a.or2) : 0.0;
+          o.xs = v ? (T)syn_load(in[1], lm, l, bl, reim, a.ldx, a.or2) : (T)0;
+          o.xt = v ? (T)syn_load(in[2], lm, l, bl, reim, a.ldx, a.or2) : (T)0;
         } else {
-          o.aD = o.xs = o.xt = 0.0;
+          o.aD = o.xs = o.xt = (T)0;
         }
         return o;
       };
@@ -165,13 +184,13 @@ __global__ __launch_bounds__(256, 3) void k_syn(SynArgs a, int n_m) {
         Ops nxt = cur;
         if (k0 + 4 < K) nxt = load(k0 + 4);
         if (!VEC) {
-          acc[p] = mfma_f64(cur.aP, cur.xq, acc[p]);
+          acc[p] = MF::run(cur.aP, cur.xq, acc[p]);
         } else {
-          acc[5 * p + 0] = mfma_f64(cur.aP, cur.xq, acc[5 * p + 0]);  // P Q
-          acc[5 * p + 1] = mfma_f64(cur.aP, cur.xt, acc[5 * p + 1]);  // P T
-          acc[5 * p + 2] = mfma_f64(cur.aP, cur.xs, acc[5 * p + 2]);  // P S
-          acc[5 * p + 3] = mfma_f64(cur.aD, cur.xs, acc[5 * p + 3]);  // dP S
-          acc[5 * p + 4] = mfma_f64(cur.aD, cur.xt, acc[5 * p + 4]);  // dP T
+          acc[5 * p + 0] = MF::run(cur.aP, cur.xq, acc[5 * p + 0]);  // P Q
+          acc[5 * p + 1] = MF::run(cur.aP, cur.xt, acc[5 * p + 1]);  // P T
+          acc[5 * p + 2] = MF::run(cur.aP, cur.xs, acc[5 * p + 2]);  // P S
+          acc[5 * p + 3] = MF::run(cur.aD, cur.xs, acc[5 * p + 3]);  // dP S
+          acc[5 * p + 4] = MF::run(cur.aD, cur.xt, acc[5 * p + 4]);  // dP T
         }
         cur = nxt;
       }
@@ -180,7 +199,7 @@ __global__ __launch_bounds__(256, 3) void k_syn(SynArgs a, int n_m) {
     const int bo = bval ? bc : 0;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      const int j = j0 + mfma_row(lane, i);
+      const int j = j0 + MF::row(lane, i);
       const bool st = bval && (j < a.H);
       const long oN = fidx(a.c_stride, a.m_stride, a.Bout, 0, m, 2 * j, bo) + reim;
       const long oS = fidx(a.c_stride, a.m_stride, a.Bout, 0, m, 2 * j + 1, bo) + reim;
@@ -216,11 +235,16 @@ extern "C" int mh_leg_synth(int vector, const MhSynArgs* args, int n_m, void* st
   if (a.ngroups < 1 || a.ngroups > 4 || a.B < 1 || a.H < 1 || a.Hp % 32 != 0 || a.Hp < a.H ||
       a.Bout != a.ngroups * a.B)
     return MH_EINVAL;
+  if (a.prec != 0 && a.prec != 1) return MH_EINVAL;
   dim3 grid(mh_cdiv(a.ngroups * a.B, 16), mh_cdiv(a.H, 32), n_m);
-  if (vector)
-    hipLaunchKernelGGL(k_syn<true>, grid, dim3(256), 0, mh_stream(stream), a, n_m);
-  else
-    hipLaunchKernelGGL(k_syn<false>, grid, dim3(256), 0, mh_stream(stream), a, n_m);
+  hipStream_t st = mh_stream(stream);
+  if (a.prec == 0) {
+    if (vector) hipLaunchKernelGGL((k_syn<true, double>), grid, dim3(256), 0, st, a, n_m);
+    else hipLaunchKernelGGL((k_syn<false, double>), grid, dim3(256), 0, st, a, n_m);
+  } else {
+    if (vector) hipLaunchKernelGGL((k_syn<true, float>), grid, dim3(256), 0, st, a, n_m);
+    else hipLaunchKernelGGL((k_syn<false, float>), grid, dim3(256), 0, st, a, n_m);
+  }
   return mh_launch_status();
 }
 
@@ -230,8 +254,10 @@ typedef MhAnaArgs AnaArgs;
 
 // WG = 4 waves: wave w covers k0 + 16*(w&1) .. (16 rows of l) and complex batch
 // columns b0 + 8*(w>>1) ..; K loop runs over theta pairs in steps of 4.
-template <bool VEC>
+template <bool VEC, typename T>
 __global__ __launch_bounds__(256, 6) void k_ana(AnaArgs a, int n_m) {
+  typedef Mfma<T> MF;
+  typedef typename MF::acc_t acc_t;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int* tl = a.tiles + 3 * blockIdx.y;
   const int p = tl[1];
@@ -250,34 +276,34 @@ __global__ __launch_bounds__(256, 6) void k_ana(AnaArgs a, int n_m) {
     if (m >= n_m) return;
     const int* info = a.mp_info + (m * 2 + p) * 4;
     const int K = info[0], K16 = info[3];
-    const double* tW = a.anaW + info[2];
-    const double* tD = a.anaWd + info[2];
-    double4_t acc[VEC ? 4 : 1];
+    const T* tW = reinterpret_cast<const T*>(a.anaW) + info[2];
+    const T* tD = reinterpret_cast<const T*>(a.anaWd) + info[2];
+    acc_t acc[VEC ? 4 : 1];
 #pragma unroll
-    for (int q = 0; q < (VEC ? 4 : 1); ++q) acc[q] = (double4_t){0.0, 0.0, 0.0, 0.0};
+    for (int q = 0; q < (VEC ? 4 : 1); ++q) acc[q] = (acc_t){0, 0, 0, 0};
     if (kbase < K) {
       // Operands of one theta step (4 pairs); the next step's are loaded before
       // the current step's MFMAs are issued.
       struct Ops {
-        double aw, ad, x1, x2, x3, x4;
+        T aw, ad, x1, x2, x3, x4;
       };
       const int c1 = a.cf1[comp], c2 = VEC ? a.cf2[comp] : 0;
       auto load = [&](int j0) {
         Ops o;
         const int j = j0 + kk;
         o.aw = tW[(long)j * K16 + arow];
-        o.ad = VEC ? tD[(long)j * K16 + arow] : 0.0;
-        o.x1 = o.x2 = o.x3 = o.x4 = 0.0;
+        o.ad = VEC ? tD[(long)j * K16 + arow] : (T)0;
+        o.x1 = o.x2 = o.x3 = o.x4 = (T)0;
         if (bval && j < a.H) {
           const long iN = fidx(a.c_stride, a.m_stride, a.Bin, 0, m, 2 * j, bl) + reim;
           const long iS = fidx(a.c_stride, a.m_stride, a.Bin, 0, m, 2 * j + 1, bl) + reim;
           const double f1N = a.F[c1 * a.c_stride + iN], f1S = a.F[c1 * a.c_stride + iS];
-          o.x1 = f1N + sg * f1S;  // same fold
+          o.x1 = (T)(f1N + sg * f1S);  // same fold
           if (VEC) {
             const double f2N = a.F[c2 * a.c_stride + iN], f2S = a.F[c2 * a.c_stride + iS];
-            o.x2 = f1N - sg * f1S;  // opposite fold
-            o.x3 = f2N + sg * f2S;
-            o.x4 = f2N - sg * f2S;
+            o.x2 = (T)(f1N - sg * f1S);  // opposite fold
+            o.x3 = (T)(f2N + sg * f2S);
+            o.x4 = (T)(f2N - sg * f2S);
           }
         }
         return o;
@@ -287,12 +313,12 @@ __global__ __launch_bounds__(256, 6) void k_ana(AnaArgs a, int n_m) {
         Ops nxt = cur;
         if (j0 + 4 < a.H) nxt = load(j0 + 4);
         if (!VEC) {
-          acc[0] = mfma_f64(cur.aw, cur.x1, acc[0]);
+          acc[0] = MF::run(cur.aw, cur.x1, acc[0]);
         } else {
-          acc[0] = mfma_f64(cur.ad, cur.x4, acc[0]);  // wdP f2(opposite fold)
-          acc[1] = mfma_f64(cur.ad, cur.x2, acc[1]);  // wdP f1(opposite fold)
-          acc[2] = mfma_f64(cur.aw, cur.x1, acc[2]);  // wP f1(same fold)
-          acc[3] = mfma_f64(cur.aw, cur.x3, acc[3]);  // wP f2(same fold)
+          acc[0] = MF::run(cur.ad, cur.x4, acc[0]);  // wdP f2(opposite fold)
+          acc[1] = MF::run(cur.ad, cur.x2, acc[1]);  // wdP f1(opposite fold)
+          acc[2] = MF::run(cur.aw, cur.x1, acc[2]);  // wP f1(same fold)
+          acc[3] = MF::run(cur.aw, cur.x3, acc[3]);  // wP f2(same fold)
         }
         cur = nxt;
       }
@@ -301,16 +327,16 @@ __global__ __launch_bounds__(256, 6) void k_ana(AnaArgs a, int n_m) {
     const long lms = a.lm_start[m];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      const int k = kbase + mfma_row(lane, i);
+      const int k = kbase + MF::row(lane, i);
       const int l = m + p + 2 * k;
       const bool st = bval && (k < K);
       const long off = (lms + p + 2 * k) * 2L * a.ldo + 2 * bl + reim;
       if (!VEC) {
-        if (st) a.outS[comp][off] = acc[0][i];
+        if (st) a.outS[comp][off] = (double)acc[0][i];
       } else {
         const double inv = (l > 0) ? 1.0 / ((double)l * (double)(l + 1)) : 0.0;
-        const double Sv = acc[0][i] + imul_pair(acc[2][i], -dm, lane);
-        const double Tv = -acc[1][i] + imul_pair(acc[3][i], -dm, lane);
+        const double Sv = (double)acc[0][i] + imul_pair((double)acc[2][i], -dm, lane);
+        const double Tv = -(double)acc[1][i] + imul_pair((double)acc[3][i], -dm, lane);
         if (st) {
           a.outS[comp][off] = Sv * inv;
           if (a.outT[comp]) a.outT[comp][off] = Tv * inv;
@@ -327,10 +353,30 @@ extern "C" int mh_leg_analysis(int vector, const MhAnaArgs* args, void* stream) 
     return MH_EINVAL;
   const int n_m = a.n_m;
   dim3 grid(a.ncomp * mh_cdiv(a.B, 16), a.ntiles);
-  if (vector)
-    hipLaunchKernelGGL(k_ana<true>, grid, dim3(256), 0, mh_stream(stream), a, n_m);
-  else
-    hipLaunchKernelGGL(k_ana<false>, grid, dim3(256), 0, mh_stream(stream), a, n_m);
+  if (a.prec != 0 && a.prec != 1) return MH_EINVAL;
+  hipStream_t st = mh_stream(stream);
+  if (a.prec == 0) {
+    if (vector) hipLaunchKernelGGL((k_ana<true, double>), grid, dim3(256), 0, st, a, n_m);
+    else hipLaunchKernelGGL((k_ana<false, double>), grid, dim3(256), 0, st, a, n_m);
+  } else {
+    if (vector) hipLaunchKernelGGL((k_ana<true, float>), grid, dim3(256), 0, st, a, n_m);
+    else hipLaunchKernelGGL((k_ana<false, float>), grid, dim3(256), 0, st, a, n_m);
+  }
   return mh_launch_status();
 }
 
+
+// ---------------------------------------------------------------- FP32 tables
+__global__ void k_to_f32(const double* __restrict__ src, float* __restrict__ dst, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = (float)src[i];
+}
+
+extern "C" int mh_tables_to_f32(const double* src, float* dst, long n, void* stream) {
+  if (n < 0) return MH_EINVAL;
+  if (n == 0) return MH_OK;
+  const long nb = (n + 255) / 256;
+  if (nb > 0x7fffffffL) return MH_EINVAL;
+  hipLaunchKernelGGL(k_to_f32, dim3((unsigned)nb), dim3(256), 0, mh_stream(stream), src, dst, n);
+  return mh_launch_status();
+}

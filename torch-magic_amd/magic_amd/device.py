@@ -88,14 +88,28 @@ class Resolution:
         self.d["comp_scale_v"] = up(np.array([0, 0, 0, 1, 1, 1, 1, 1, 1]), I32)
         self.d["comp_scale_2"] = up(np.array([1, 1]), I32)
 
+    def f32_tables(self):
+        """FP32 copies of the four Legendre tables (built on first use)."""
+        if getattr(self, "_t32", None) is None:
+            t = {}
+            for k, src in (("synP", self.synP), ("synD", self.synD), ("anaW", self.anaW),
+                           ("anaWd", self.anaWd)):
+                dst = torch.empty(src.numel(), dtype=torch.float32, device=self.device)
+                _lib.call("mh_tables_to_f32", ptr(src), ptr(dst), src.numel(), stream_handle())
+                t[k] = dst
+            self._t32 = t
+        return self._t32
+
     # ------------------------------------------------------------ launchers
     def fstrides(self, Bout):
         """(component stride, m stride) in doubles of a Fourier buffer F[c][m][t][b]."""
         ms = 2 * self.T * Bout
         return self.M * ms, ms
 
-    def synth(self, groups, B, ldx, F, Bout, vector: bool, or2=None):
-        """Legendre synthesis.  groups: list of [(x1, x2, mode)] * (3 if vector else 1)."""
+    def synth(self, groups, B, ldx, F, Bout, vector: bool, or2=None, prec=0):
+        """Legendre synthesis.  groups: list of [(x1, x2, mode)] * (3 if vector else 1).
+
+        prec 0: FP64 MFMA; 1: FP32 tables and MFMA with FP32 accumulation."""
         a = _lib.SynArgs()
         for g, grp in enumerate(groups):
             for c, (x1, x2, mode) in enumerate(grp):
@@ -104,14 +118,19 @@ class Resolution:
                 a.in_[g][c].mode = mode
         a.ngroups, a.B, a.ldx, a.H, a.Hp = len(groups), B, ldx, self.H, self.Hp
         a.mp_info, a.lm_start = ptr(self.d["mp_info"]), ptr(self.d["lm_start"])
-        a.synP, a.synD = ptr(self.synP), ptr(self.synD)
+        if prec:
+            t32 = self.f32_tables()
+            a.synP, a.synD = ptr(t32["synP"]), ptr(t32["synD"])
+        else:
+            a.synP, a.synD = ptr(self.synP), ptr(self.synD)
+        a.prec = prec
         a.or2 = ptr(or2 if or2 is not None else self.d["or2"])
         a.F = ptr(F)
         a.c_stride, a.m_stride = self.fstrides(Bout)
         a.Bout = Bout
         _lib.call("mh_leg_synth", 1 if vector else 0, _lib.ref(a), self.M, stream_handle())
 
-    def analysis(self, F, Bin, B, comps, outS, outT, ldo, vector: bool):
+    def analysis(self, F, Bin, B, comps, outS, outT, ldo, vector: bool, prec=0):
         """Legendre analysis of F[c][m][t][b] (comp stride = M*T*Bin complex).
 
         comps: scalar -> list of component indices; vector -> list of (c_vtheta, c_vphi).
@@ -131,7 +150,12 @@ class Resolution:
                 a.outT[i] = outT[i] if isinstance(outT[i], int) else ptr(outT[i])
         a.ldo, a.H, a.Hp = ldo, self.H, self.Hp
         a.mp_info, a.lm_start = ptr(self.d["mp_info"]), ptr(self.d["lm_start"])
-        a.anaW, a.anaWd = ptr(self.anaW), ptr(self.anaWd)
+        if prec:
+            t32 = self.f32_tables()
+            a.anaW, a.anaWd = ptr(t32["anaW"]), ptr(t32["anaWd"])
+        else:
+            a.anaW, a.anaWd = ptr(self.anaW), ptr(self.anaWd)
+        a.prec = prec
         a.tiles, a.ntiles = ptr(self.d["ana_tiles"]), self.lay["n_ana_tiles"]
         a.n_m = self.M
         _lib.call("mh_leg_analysis", 1 if vector else 0, _lib.ref(a), stream_handle())

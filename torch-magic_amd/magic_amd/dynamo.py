@@ -123,6 +123,9 @@ class DynamoRun:
         # (default): one kernel for inverse FFT + products + forward FFT, no grid in
         # HBM.  Separate: grids G[c][b][t][phi] and three kernels.
         self.fused_grid = fused_grid
+        # mixed precision: Legendre transforms on FP32 MFMA with FP32 accumulation;
+        # FFTs, products, solves, derivatives and diagnostics stay FP64
+        self.leg_prec = 1 if p.precision == "mixed" else 0
         self.grid_generic = False  # force the run-time FFT plan of the fused kernel
         self.F_s = r.fourier_buffer(1, N)
         self.F_v = r.fourier_buffer(3, 4 * N)
@@ -459,13 +462,13 @@ class DynamoRun:
         r, f, N, Nb = self.res, self.f, self.N, self.Nb
         # 1. Legendre synthesis
         with self._t("leg_syn_scal"):
-            r.synth([[(f["s"], None, 0)]], N, N, self.F_s, N, vector=False)
+            r.synth([[(f["s"], None, 0)]], N, N, self.F_s, N, vector=False, prec=self.leg_prec)
         groups = [[(f["b"], None, 1), (f["db"], None, 0), (f["aj"], None, 0)],
                   [(f["aj"], None, 1), (f["dj"], None, 0), (f["ddb"], f["b"], 2)],
                   [(f["w"], None, 1), (f["dw"], None, 0), (f["z"], None, 0)],
                   [(f["z"], None, 1), (f["dz"], None, 0), (f["ddw"], f["w"], 2)]]
         with self._t("leg_syn_vec"):
-            r.synth(groups, N, N, self.F_v, 4 * N, vector=True)
+            r.synth(groups, N, N, self.F_v, 4 * N, vector=True, prec=self.leg_prec)
         d = r.d
         if self.fused_grid:
             # 2-4. fused inverse FFT + nonlinear products + Courant + forward FFT
@@ -480,11 +483,12 @@ class DynamoRun:
         with self._t("leg_ana_scal"):
             r.analysis(self.F_o, Nb, Nb, [0, 1, 2],
                        [ptr(nl["Advr"]) + off, ptr(nl["dVSr"]) + off, ptr(nl["VxBr"]) + off],
-                       [None, None, None], N, vector=False)
+                       [None, None, None], N, vector=False, prec=self.leg_prec)
         with self._t("leg_ana_vec"):
             r.analysis(self.F_o, Nb, Nb, [(3, 4), (5, 6), (7, 8)],
                        [ptr(nl["Advt"]) + off, ptr(nl["VSt"]) + off, ptr(nl["VxBt"]) + off],
-                       [ptr(nl["Advp"]) + off, None, ptr(nl["VxBp"]) + off], N, vector=True)
+                       [ptr(nl["Advp"]) + off, None, ptr(nl["VxBp"]) + off], N, vector=True,
+                       prec=self.leg_prec)
         return self._radial_finish()
 
     def _grid_separate(self):

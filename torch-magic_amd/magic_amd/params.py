@@ -147,6 +147,8 @@ class Params:
             bad.append("angular-momentum correction")
         if self.ktops != 1 or self.kbots != 1:
             bad.append("fixed-flux entropy boundaries")
+        if self.precision not in ("fp64", "mixed"):
+            bad.append(f"precision {self.precision!r} (fp64 or mixed)")
         if bad:
             raise NotImplementedError("device step does not support: " + ", ".join(bad))
 

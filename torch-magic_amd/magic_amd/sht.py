@@ -42,8 +42,15 @@ def _grid(x, T, P):
 
 
 class SHT:
-    def __init__(self, res: Resolution):
+    """Transforms of one resolution.  precision "fp64" (the reference's) or
+    "mixed": Legendre contractions on FP32 MFMA with FP32 accumulation and FP32
+    tables, FFTs in FP64; inputs and outputs keep the reference's dtypes."""
+
+    def __init__(self, res: Resolution, precision: str = "fp64"):
+        if precision not in ("fp64", "mixed"):
+            raise ValueError(f"precision must be 'fp64' or 'mixed', not {precision!r}")
         self.res = res
+        self.prec = 1 if precision == "mixed" else 0
 
     def _on(self, x):
         return x.to(self.res.device) if x.device != self.res.device else x
@@ -54,7 +61,7 @@ class SHT:
         S = self._on(S)
         B = S.shape[1]
         F = r.fourier_buffer(1, B)
-        r.synth([[(S, None, 0)]], B, B, F, B, vector=False)
+        r.synth([[(S, None, 0)]], B, B, F, B, vector=False, prec=self.prec)
         G = torch.empty(B, r.T, r.P, dtype=F64, device=r.device)
         r.fft(False, F, B, G, 1, B)
         return G if batched else G[0]
@@ -67,7 +74,7 @@ class SHT:
         F = r.fourier_buffer(1, B)
         r.fft(True, F, B, g, 1, B)
         out = torch.zeros(r.L, B, dtype=C128, device=r.device)
-        r.analysis(F, B, B, [0], [out], [None], B, vector=False)
+        r.analysis(F, B, B, [0], [out], [None], B, vector=False, prec=self.prec)
         return out if batched else out[:, 0]
 
     def torpol_to_spat(self, Qlm, Slm, Tlm, lcut=None):
@@ -78,7 +85,8 @@ class SHT:
         Q, S, T = self._on(Q), self._on(S), self._on(T)
         B = Q.shape[1]
         F = r.fourier_buffer(3, B)
-        r.synth([[(Q, None, 0), (S, None, 0), (T, None, 0)]], B, B, F, B, vector=True)
+        r.synth([[(Q, None, 0), (S, None, 0), (T, None, 0)]], B, B, F, B, vector=True,
+                prec=self.prec)
         G = torch.empty(3, B, r.T, r.P, dtype=F64, device=r.device)
         r.fft(False, F, B, G, 3, B)
         out = (G[0], G[1], G[2])
@@ -96,7 +104,7 @@ class SHT:
         r.fft(True, F, B, G, 2, B, comp_scale=r.d["comp_scale_2"])
         S = torch.zeros(r.L, B, dtype=C128, device=r.device)
         T = torch.zeros(r.L, B, dtype=C128, device=r.device)
-        r.analysis(F, B, B, [(0, 1)], [S], [T], B, vector=True)
+        r.analysis(F, B, B, [(0, 1)], [S], [T], B, vector=True, prec=self.prec)
         return (S, T) if batched else (S[:, 0], T[:, 0])
 
 
