@@ -10,11 +10,16 @@ Synthetic data: fields initialised on the device from the seed, no I/O.
 A "step" is one full ``one_step`` (radial loop with the SHTs, nonlinear
 terms and CFL, dt decision, operator rebuild if dt changed, implicit solves).
 W warm-up steps are run untimed, then exactly K steps are timed between two
-device synchronisations.  Per-section HIP-event timings from the same timed
+device synchronisations, first in plain FP64, then in mixed precision
+(Legendre transforms on FP32 MFMA, everything else FP64), each from the same
+seeded initial state.  The headline value is the mixed-precision steps/s;
+the FP64 rate, their ratio and the largest relative energy difference after
+W+K steps are in ``config``.  Per-section HIP-event timings from the timed
 region give the roofline of the dominant kernel.  The CPU baseline is the
 FP64 numpy oracle on a bounded sample of the same workload.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision both|mixed|fp64]
+                       [--no-cpu-baseline]
 """
 
 import argparse
@@ -34,6 +39,7 @@ WORKLOAD = dict(l_max=383, n_r_max=97, ek=1e-5, ra=2e8, pr=1.0, prmag=1.0, radra
 NOISE = dict(seed=20261202, amp=1e-4)
 
 PEAK_FP64_TFLOPS = 78.6
+PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 
 
@@ -87,11 +93,59 @@ def cpu_baseline(sample_levels):
                 seconds=dt)
 
 
+def run_case(precision, steps, warmup):
+    """Time `steps` CNAB2 steps after `warmup` untimed ones; per-section HIP events."""
+    import torch
+    from magic_amd.dynamo import DynamoRun
+    run = DynamoRun(dict(WORKLOAD, precision=precision), device="cuda:0")
+    run.initialize_fields(noise_amp=NOISE["amp"], noise_seed=NOISE["seed"])
+    run.setup_initial_state()
+    run.initialize_dt(WORKLOAD["dtmax"])
+    dt = run.dtMax
+    n = 0
+    for _ in range(warmup):
+        n += 1
+        dt = run.one_step(n, dt)
+    torch.cuda.synchronize()
+    run.enable_timers(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        n += 1
+        dt = run.one_step(n, dt)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    secs = run.timer_summary()
+    run.enable_timers(False)
+    out = dict(elapsed=t1 - t0, secs=secs, energies=run.energies(), dt=dt,
+               rebuilds=run.n_rebuild, p=run.p, N=run.N)
+    del run
+    torch.cuda.empty_cache()
+    return out
+
+
+def roofline(res, precision):
+    """Roofline of the dominant kernel section (largest total time)."""
+    secs, work = res["secs"], section_work(res["p"], res["N"])
+    dom = max(secs, key=lambda k: secs[k][0] * secs[k][1])
+    cnt, ms = secs[dom]
+    kind, amount = work.get(dom, ("hbm", float("nan")))
+    if kind == "mfma":
+        peak = PEAK_FP32_TFLOPS if (precision == "mixed" and dom.startswith("leg_")) else PEAK_FP64_TFLOPS
+        ach = amount / (ms * 1e-3) / 1e12
+        return dict(bound="mfma", achieved=ach, peak=peak, unit="TFLOP/s", frac=ach / peak,
+                    traffic=None, kernel=dom, launch_ms=ms, algorithmic_flops=amount)
+    ach = amount / (ms * 1e-3) / 1e9
+    return dict(bound="hbm", achieved=ach, peak=PEAK_HBM_GBS, unit="GB/s", frac=ach / PEAK_HBM_GBS,
+                traffic=None, kernel=dom, launch_ms=ms, algorithmic_bytes=amount)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--precision", choices=("both", "mixed", "fp64"), default="both")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-levels", type=int, default=9)
     args = ap.parse_args()
@@ -100,71 +154,46 @@ def main():
         sys.exit(2)
 
     import numpy as np
-    import torch
-    from magic_amd.dynamo import DynamoRun
-
-    run = DynamoRun(WORKLOAD, device="cuda:0")
-    run.initialize_fields(noise_amp=NOISE["amp"], noise_seed=NOISE["seed"])
-    run.setup_initial_state()
-    run.initialize_dt(WORKLOAD["dtmax"])
-    dt = run.dtMax
-    n = 0
-    for _ in range(args.warmup):
-        n += 1
-        dt = run.one_step(n, dt)
-    torch.cuda.synchronize()
-    run.enable_timers(True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        n += 1
-        dt = run.one_step(n, dt)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    secs = run.timer_summary()
-    run.enable_timers(False)
-    elapsed = t1 - t0
-    ms_per_step = 1e3 * elapsed / args.steps
-    energies = run.energies()
-    p, N = run.p, run.N
-    work = section_work(p, N)
-    # dominant kernel section by total time in the timed region
-    dom = max(secs, key=lambda k: secs[k][0] * secs[k][1])
-    cnt, ms = secs[dom]
-    kind, amount = work.get(dom, ("hbm", float("nan")))
-    if kind == "mfma":
-        ach = amount / (ms * 1e-3) / 1e12
-        roof = dict(bound="mfma", achieved=ach, peak=PEAK_FP64_TFLOPS, unit="TFLOP/s",
-                    frac=ach / PEAK_FP64_TFLOPS, traffic=None, kernel=dom,
-                    launch_ms=ms, algorithmic_flops=amount)
-    else:
-        ach = amount / (ms * 1e-3) / 1e9
-        roof = dict(bound="hbm", achieved=ach, peak=PEAK_HBM_GBS, unit="GB/s",
-                    frac=ach / PEAK_HBM_GBS, traffic=None, kernel=dom,
-                    launch_ms=ms, algorithmic_bytes=amount)
-    breakdown = {k: round(v[1], 4) for k, v in sorted(secs.items(), key=lambda kv: -kv[1][1])}
+    precs = ("fp64", "mixed") if args.precision == "both" else (args.precision,)
+    res = {pr: run_case(pr, args.steps, args.warmup) for pr in precs}
+    head = "mixed" if "mixed" in res else "fp64"
+    h = res[head]
+    sps = {pr: args.steps / r["elapsed"] for pr, r in res.items()}
+    breakdown = {pr: {k: round(v[1], 4) for k, v in sorted(r["secs"].items(), key=lambda kv: -kv[1][1])}
+                 for pr, r in res.items()}
+    cfg = {"workload": "Boussinesq CNAB2 dynamo step, l_max=383, n_r_max=97, n_theta=576, "
+                       "n_phi=1152, E=1e-5, Ra=2e8, Pr=1, Pm=1",
+           "l_max": 383, "n_r_max": 97, "precision": head,
+           "precision_split": "Legendre synthesis/analysis on FP32 MFMA with FP32 accumulation; "
+                              "FFTs, nonlinear products, solves, derivatives, diagnostics FP64"
+                              if head == "mixed" else "all FP64",
+           "steps_per_s": {pr: v for pr, v in sps.items()},
+           "ms_per_step": {pr: 1e3 * r["elapsed"] / args.steps for pr, r in res.items()},
+           "dt": h["dt"], "rebuilds": {pr: r["rebuilds"] for pr, r in res.items()},
+           "energies": {pr: r["energies"] for pr, r in res.items()},
+           "section_ms": breakdown}
+    if len(res) == 2:
+        e64, emx = np.array(res["fp64"]["energies"]), np.array(res["mixed"]["energies"])
+        cfg["mixed_over_fp64"] = sps["mixed"] / sps["fp64"]
+        cfg["energy_rel_diff_mixed_vs_fp64"] = float(np.max(np.abs(emx - e64) / np.abs(e64)))
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(args.cpu_levels)
     line = {
         "metric": "steps/s at l_max=383 mixed vs FP64; ensemble p50/p99 step latency",
-        "value": args.steps / elapsed,
+        "value": sps[head],
         "unit": "steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": 1e3 * h["elapsed"] / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64" if head == "mixed" else "f64",
         "data": "synthetic (seeded noise + dipole seed, generated on device)",
-        "config": {"workload": "Boussinesq CNAB2 dynamo step, l_max=383, n_r_max=97, "
-                               "n_theta=576, n_phi=1152, E=1e-5, Ra=2e8, Pr=1, Pm=1, FP64",
-                   "l_max": 383, "n_r_max": 97, "precision": "fp64", "dt": dt,
-                   "rebuilds": run.n_rebuild, "energies": energies,
-                   "section_ms": breakdown},
-        "roofline": roof,
+        "config": cfg,
+        "roofline": roofline(h, head),
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

@@ -77,7 +77,7 @@ typedef struct {
   const int* lm_start;
   const double* anaW;
   const double* anaWd;
-  const int* tiles; /* (m, p, k0) triples */
+  const int* tiles; /* (m, p, k0) triples, k0 in steps of 96 degrees (one workgroup each) */
   int ntiles;
   int prec; /* as MhSynArgs.prec (anaW/anaWd hold floats for 1) */
 } MhAnaArgs;
@@ -214,6 +214,7 @@ typedef struct {
   int L, N;
   double wimp, wl1, we0, we1, bots_l0;
   int l_mag, pad;
+  const double* wdev; /* optional: device weights {wimp, wl1, we0, we1, ...} replace the four above */
 } MhRhsArgs;
 
 /* IMEX right-hand sides with boundary rows and m=0 realness: replaces
@@ -230,6 +231,7 @@ typedef struct {
   int L, N;
   double fac, BuoFac;
   int lm_l0, pad;
+  const double* wdev; /* optional: device weights, wdev[4] replaces fac */
 } MhBuoyArgs;
 
 /* Implicit buoyancy coupling and the l=0 pressure right-hand side

@@ -91,12 +91,12 @@ class ImexSys(C.Structure):
 class RhsArgs(C.Structure):
     _fields_ = [("sys", ImexSys * 6), ("rs", P), ("rz", P), ("rb", P), ("rj", P), ("rwp", P),
                 ("m", P), ("L", I), ("N", I), ("wimp", D), ("wl1", D), ("we0", D), ("we1", D),
-                ("bots_l0", D), ("l_mag", I), ("pad", I)]
+                ("bots_l0", D), ("l_mag", I), ("pad", I), ("wdev", P)]
 
 
 class BuoyArgs(C.Structure):
     _fields_ = [("rwp", P), ("s", P), ("ew0", P), ("rgrav", P), ("m", P), ("L", I), ("N", I),
-                ("fac", D), ("BuoFac", D), ("lm_l0", I), ("pad", I)]
+                ("fac", D), ("BuoFac", D), ("lm_l0", I), ("pad", I), ("wdev", P)]
 
 
 class ImplArgs(C.Structure):
@@ -179,8 +179,36 @@ def load(path: str | None = None):
     return lib
 
 
+_recording = None  # list collecting (section, name, args) while a launch plan is recorded
+
+
+class record:
+    """Collect the entry-point calls made inside the block instead of running
+    them: ``with record() as plan: ...``; each item is (section, name, args
+    without the trailing stream).  Every entry point takes the stream last."""
+
+    def __init__(self):
+        self.calls = []
+        self.section = None
+
+    def __enter__(self):
+        global _recording
+        if _recording is not None:
+            raise RuntimeError("nested launch-plan recording")
+        _recording = self
+        return self
+
+    def __exit__(self, *exc):
+        global _recording
+        _recording = None
+        return False
+
+
 def call(name: str, *args):
     """Call an entry point and raise on a non-zero return code."""
+    if _recording is not None:
+        _recording.calls.append((_recording.section, name, args[:-1]))
+        return 0
     rc = getattr(load(), name)(*args)
     if rc != 0:
         kind = {-1: "invalid argument", -2: "launch failure"}.get(rc, f"code {rc}")
@@ -191,3 +219,22 @@ def call(name: str, *args):
 def ref(struct) -> C.c_void_p:
     """Host pointer to a ctypes structure, for the ``const Mh*Args*`` parameters."""
     return C.cast(C.pointer(struct), P)
+
+
+def run_plan(calls, stream, on_section=None):
+    """Run recorded calls on `stream` (a c_void_p); on_section(name, begin) brackets
+    each run of consecutive calls of one section (for timers)."""
+    lib = load()
+    cur = None
+    for sec, name, args in calls:
+        if on_section is not None and sec != cur:
+            if cur is not None:
+                on_section(cur, False)
+            on_section(sec, True)
+            cur = sec
+        rc = getattr(lib, name)(*args, stream)
+        if rc != 0:
+            kind = {-1: "invalid argument", -2: "launch failure"}.get(rc, f"code {rc}")
+            raise MagicHipError(f"{name}: {kind}")
+    if on_section is not None and cur is not None:
+        on_section(cur, False)

@@ -118,12 +118,8 @@ template <> struct Mfma<float> {
   static __device__ __forceinline__ int row(int lane, int v) { return mfma_row_f32(lane, v); }
 };
 
-// WG = 4 waves; wave w covers theta pairs j0 + 16*(w&1) .. and complex
-// columns b0 + 8*(w>>1) .. of the groups laid side by side (column bc is
-// batch column bc % B of group bc / B, which is also its column in F);
-// WG tile = 32 theta pairs x 16 complex columns.
+// Per-lane group select without dynamic indexing of the argument struct.
 __device__ __forceinline__ SynIn syn_in(const SynArgs& a, int g, int c) {
-  // per-lane group select without dynamic indexing of the argument struct
   SynIn r = a.in[0][c];
   if (g == 1) r = a.in[1][c];
   if (g == 2) r = a.in[2][c];
@@ -131,100 +127,138 @@ __device__ __forceinline__ SynIn syn_in(const SynArgs& a, int g, int c) {
   return r;
 }
 
+// Synthesis, one workgroup per (order m, 96 theta pairs, 8 complex columns).
+// Columns of the groups are laid side by side (column bc is batch column
+// bc % B of group bc / B, which is also its column in F).  The input tile of
+// 32 degrees x 16 real columns x (1 or 3) fields is staged in LDS with the
+// l(l+1) / or2 prefactors applied once per element (double-buffered: the next
+// chunk is loaded into registers while the current one feeds the MFMAs); wave
+// w takes theta pairs j0 + 16w and reads the table operand from global memory.
+constexpr int kSynWaves = 6, kSynJ = 16 * kSynWaves, kChunk = 32;
+
 template <bool VEC, typename T>
-__global__ __launch_bounds__(256, 3) void k_syn(SynArgs a, int n_m) {
+__global__ __launch_bounds__(64 * kSynWaves, 2) void k_syn(SynArgs a, int n_m) {
   typedef Mfma<T> MF;
   typedef typename MF::acc_t acc_t;
+  constexpr int NF = VEC ? 3 : 1;           // fields per group (Q, S, T)
+  constexpr int NT = 64 * kSynWaves;        // threads
+  constexpr int NV = NF * kChunk * 16;      // staged values per chunk
+  constexpr int PER = (NV + NT - 1) / NT;   // staged values per thread
+  __shared__ T xs[2][NF][kChunk][16];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int b0 = blockIdx.x * 16 + 8 * (w >> 1);
-  const int j0 = blockIdx.y * 32 + 16 * (w & 1);
+  const int bc0 = blockIdx.x * 8;
+  const int j0 = blockIdx.y * kSynJ + 16 * w;
   const int col = lane & 15, kk = lane >> 4;
-  const int bc = b0 + (col >> 1), reim = col & 1;
-  const bool bval = bc < a.ngroups * a.B;
-  const int g = bval ? bc / a.B : 0;
-  const int bl = bval ? bc - g * a.B : 0;
-  const SynIn in[3] = {syn_in(a, g, 0), syn_in(a, g, 1), syn_in(a, g, 2)};
-  {
-    const int m = blockIdx.z;
-    const long lms = a.lm_start[m];
-    acc_t acc[VEC ? 10 : 2];
+  const int m = blockIdx.z;
+  const long lms = a.lm_start[m];
+  const int ncol = a.ngroups * a.B;
+  acc_t acc[VEC ? 10 : 2];
 #pragma unroll
-    for (int q = 0; q < (VEC ? 10 : 2); ++q) acc[q] = (acc_t){0, 0, 0, 0};
-    // Operands of one k-step (4 degrees); the next step's are loaded before
-    // the current step's MFMAs are issued, so their latency is hidden.
-    struct Ops {
-      T aP, aD, xq, xs, xt;
+  for (int q = 0; q < (VEC ? 10 : 2); ++q) acc[q] = (acc_t){0, 0, 0, 0};
+  // staging map of this thread: value v = threadIdx.x + NT*i -> (field, k, real column)
+  T stg[PER];
+#pragma unroll 1
+  for (int p = 0; p < 2; ++p) {
+    const int* info = a.mp_info + (m * 2 + p) * 4;
+    const int K = info[0];
+    if (K <= 0) continue;
+    const T* tP = reinterpret_cast<const T*>(a.synP) + info[1];
+    const T* tD = reinterpret_cast<const T*>(a.synD) + info[1];
+    auto fetch = [&](int c0) {
+#pragma unroll
+      for (int i = 0; i < PER; ++i) {
+        const int v = threadIdx.x + NT * i;
+        T x = (T)0;
+        if (v < NV) {
+          const int f = v / (kChunk * 16), rem = v - f * kChunk * 16;
+          const int k = c0 + rem / 16, c16 = rem & 15;
+          const int bc = bc0 + (c16 >> 1);
+          if (k < K && bc < ncol) {
+            const int g = bc / a.B, b = bc - g * a.B;
+            x = (T)syn_load(syn_in(a, g, f), lms + p + 2 * k, m + p + 2 * k, b, c16 & 1, a.ldx,
+                            a.or2);
+          }
+        }
+        stg[i] = x;
+      }
     };
+    auto store = [&](int buf) {
 #pragma unroll
-    for (int p = 0; p < 2; ++p) {
-      const int* info = a.mp_info + (m * 2 + p) * 4;
-      const int K = info[0];
-      const T* tP = reinterpret_cast<const T*>(a.synP) + info[1];
-      const T* tD = reinterpret_cast<const T*>(a.synD) + info[1];
-      auto load = [&](int k0) {
-        Ops o;
-        const int k = k0 + kk;
-        const bool v = bval && (k < K);
-        const int l = m + p + 2 * k;
-        const long lm = lms + p + 2 * k;
-        o.aP = tP[(long)k * a.Hp + j0 + col];
-        o.xq = v ? (T)syn_load(in[0], lm, l, bl, reim, a.ldx, a.or2) : (T)0;
-        if (VEC) {
-          o.aD = tD[(long)k * a.Hp + j0 + col];
-          o.xs = v ? (T)syn_load(in[1], lm, l, bl, reim, a.ldx, a.or2) : (T)0;
-          o.xt = v ? (T)syn_load(in[2], lm, l, bl, reim, a.ldx, a.or2) : (T)0;
-        } else {
-          o.aD = o.xs = o.xt = (T)0;
+      for (int i = 0; i < PER; ++i) {
+        const int v = threadIdx.x + NT * i;
+        if (v < NV) {
+          const int f = v / (kChunk * 16), rem = v - f * kChunk * 16;
+          xs[buf][f][rem / 16][rem & 15] = stg[i];
         }
-        return o;
-      };
-      if (K <= 0) continue;
-      Ops cur = load(0);
-      for (int k0 = 0; k0 < K; k0 += 4) {
-        Ops nxt = cur;
-        if (k0 + 4 < K) nxt = load(k0 + 4);
+      }
+    };
+    fetch(0);
+    __syncthreads();  // previous parity's readers are done with both buffers
+    store(0);
+    __syncthreads();
+    const int nch = (K + kChunk - 1) / kChunk;
+    for (int ch = 0; ch < nch; ++ch) {
+      const int buf = ch & 1, c0 = ch * kChunk;
+      if (ch + 1 < nch) fetch(c0 + kChunk);
+      // table rows are padded to a multiple of 4; waves past the last theta
+      // pair only help with the staging
+      const int ksteps = (j0 < a.H) ? min(kChunk, K - c0 + 3) / 4 : 0;
+      for (int s = 0; s < ksteps; ++s) {
+        const int k = c0 + 4 * s + kk;
+        const long to = (long)k * a.Hp + j0 + col;
+        const bool jv = j0 + col < a.Hp;
+        const T aP = jv ? tP[to] : (T)0;
+        const T xq = xs[buf][0][4 * s + kk][col];
         if (!VEC) {
-          acc[p] = MF::run(cur.aP, cur.xq, acc[p]);
+          acc[p] = MF::run(aP, xq, acc[p]);
         } else {
-          acc[5 * p + 0] = MF::run(cur.aP, cur.xq, acc[5 * p + 0]);  // P Q
-          acc[5 * p + 1] = MF::run(cur.aP, cur.xt, acc[5 * p + 1]);  // P T
-          acc[5 * p + 2] = MF::run(cur.aP, cur.xs, acc[5 * p + 2]);  // P S
-          acc[5 * p + 3] = MF::run(cur.aD, cur.xs, acc[5 * p + 3]);  // dP S
-          acc[5 * p + 4] = MF::run(cur.aD, cur.xt, acc[5 * p + 4]);  // dP T
+          const T aD = jv ? tD[to] : (T)0;
+          const T xsv = xs[buf][1][4 * s + kk][col];
+          const T xt = xs[buf][2][4 * s + kk][col];
+          acc[5 * p + 0] = MF::run(aP, xq, acc[5 * p + 0]);   // P Q
+          acc[5 * p + 1] = MF::run(aP, xt, acc[5 * p + 1]);   // P T
+          acc[5 * p + 2] = MF::run(aP, xsv, acc[5 * p + 2]);  // P S
+          acc[5 * p + 3] = MF::run(aD, xsv, acc[5 * p + 3]);  // dP S
+          acc[5 * p + 4] = MF::run(aD, xt, acc[5 * p + 4]);   // dP T
         }
-        cur = nxt;
+      }
+      if (ch + 1 < nch) {
+        store(buf ^ 1);
+        __syncthreads();
       }
     }
-    const double dm = (double)m;
-    const int bo = bval ? bc : 0;
+  }
+  const double dm = (double)m;
+  const int bc = bc0 + (col >> 1), reim = col & 1;
+  const bool bval = bc < ncol;
+  const int bo = bval ? bc : 0;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int j = j0 + MF::row(lane, i);
-      const bool st = bval && (j < a.H);
-      const long oN = fidx(a.c_stride, a.m_stride, a.Bout, 0, m, 2 * j, bo) + reim;
-      const long oS = fidx(a.c_stride, a.m_stride, a.Bout, 0, m, 2 * j + 1, bo) + reim;
-      if (!VEC) {
-        const double e = acc[0][i], o = acc[1][i];
-        if (st) {
-          a.F[oN] = e + o;
-          a.F[oS] = e - o;
-        }
-      } else {
-        const double Qe = acc[0][i], Qo = acc[5][i];
-        const double PTe = acc[1][i], PTo = acc[6][i];
-        const double PSe = acc[2][i], PSo = acc[7][i];
-        const double DSe = acc[3][i], DSo = acc[8][i];
-        const double DTe = acc[4][i], DTo = acc[9][i];
-        const double iPTp = imul_pair(PTe + PTo, dm, lane), iPTm = imul_pair(PTe - PTo, dm, lane);
-        const double iPSp = imul_pair(PSe + PSo, dm, lane), iPSm = imul_pair(PSe - PSo, dm, lane);
-        if (st) {
-          a.F[oN] = Qe + Qo;
-          a.F[oS] = Qe - Qo;
-          a.F[a.c_stride + oN] = (DSe + DSo) + iPTp;
-          a.F[a.c_stride + oS] = -(DSe - DSo) + iPTm;
-          a.F[2 * a.c_stride + oN] = iPSp - (DTe + DTo);
-          a.F[2 * a.c_stride + oS] = iPSm + (DTe - DTo);
-        }
+  for (int i = 0; i < 4; ++i) {
+    const int j = j0 + MF::row(lane, i);
+    const bool st = bval && (j < a.H);
+    const long oN = fidx(a.c_stride, a.m_stride, a.Bout, 0, m, 2 * j, bo) + reim;
+    const long oS = fidx(a.c_stride, a.m_stride, a.Bout, 0, m, 2 * j + 1, bo) + reim;
+    if (!VEC) {
+      const double e = acc[0][i], o = acc[1][i];
+      if (st) {
+        a.F[oN] = e + o;
+        a.F[oS] = e - o;
+      }
+    } else {
+      const double Qe = acc[0][i], Qo = acc[5][i];
+      const double PTe = acc[1][i], PTo = acc[6][i];
+      const double PSe = acc[2][i], PSo = acc[7][i];
+      const double DSe = acc[3][i], DSo = acc[8][i];
+      const double DTe = acc[4][i], DTo = acc[9][i];
+      const double iPTp = imul_pair(PTe + PTo, dm, lane), iPTm = imul_pair(PTe - PTo, dm, lane);
+      const double iPSp = imul_pair(PSe + PSo, dm, lane), iPSm = imul_pair(PSe - PSo, dm, lane);
+      if (st) {
+        a.F[oN] = Qe + Qo;
+        a.F[oS] = Qe - Qo;
+        a.F[a.c_stride + oN] = (DSe + DSo) + iPTp;
+        a.F[a.c_stride + oS] = -(DSe - DSo) + iPTm;
+        a.F[2 * a.c_stride + oN] = iPSp - (DTe + DTo);
+        a.F[2 * a.c_stride + oS] = iPSm + (DTe - DTo);
       }
     }
   }
@@ -236,14 +270,15 @@ extern "C" int mh_leg_synth(int vector, const MhSynArgs* args, int n_m, void* st
       a.Bout != a.ngroups * a.B)
     return MH_EINVAL;
   if (a.prec != 0 && a.prec != 1) return MH_EINVAL;
-  dim3 grid(mh_cdiv(a.ngroups * a.B, 16), mh_cdiv(a.H, 32), n_m);
+  dim3 grid(mh_cdiv(a.ngroups * a.B, 8), mh_cdiv(a.H, kSynJ), n_m);
+  const dim3 blk(64 * kSynWaves);
   hipStream_t st = mh_stream(stream);
   if (a.prec == 0) {
-    if (vector) hipLaunchKernelGGL((k_syn<true, double>), grid, dim3(256), 0, st, a, n_m);
-    else hipLaunchKernelGGL((k_syn<false, double>), grid, dim3(256), 0, st, a, n_m);
+    if (vector) hipLaunchKernelGGL((k_syn<true, double>), grid, blk, 0, st, a, n_m);
+    else hipLaunchKernelGGL((k_syn<false, double>), grid, blk, 0, st, a, n_m);
   } else {
-    if (vector) hipLaunchKernelGGL((k_syn<true, float>), grid, dim3(256), 0, st, a, n_m);
-    else hipLaunchKernelGGL((k_syn<false, float>), grid, dim3(256), 0, st, a, n_m);
+    if (vector) hipLaunchKernelGGL((k_syn<true, float>), grid, blk, 0, st, a, n_m);
+    else hipLaunchKernelGGL((k_syn<false, float>), grid, blk, 0, st, a, n_m);
   }
   return mh_launch_status();
 }
@@ -252,95 +287,132 @@ extern "C" int mh_leg_synth(int vector, const MhSynArgs* args, int n_m, void* st
 // ---------------------------------------------------------------- analysis
 typedef MhAnaArgs AnaArgs;
 
-// WG = 4 waves: wave w covers k0 + 16*(w&1) .. (16 rows of l) and complex batch
-// columns b0 + 8*(w>>1) ..; K loop runs over theta pairs in steps of 4.
+// Analysis, one workgroup per (tile (m, p, k0), component, 8 complex columns):
+// wave w takes the 16 degrees k0 + 16w (96 per tile).  The hemisphere folds
+// of the Fourier rows (N + sg S and N - sg S, sg = +1 for even and -1 for odd
+// l - m) for 32 theta pairs x 16 real columns are staged in LDS once per chunk
+// (double-buffered) and shared by the six waves; the weight tables are read
+// from global memory.
+constexpr int kAnaWaves = 6;
+
 template <bool VEC, typename T>
-__global__ __launch_bounds__(256, 6) void k_ana(AnaArgs a, int n_m) {
+__global__ __launch_bounds__(64 * kAnaWaves, 2) void k_ana(AnaArgs a, int n_m) {
   typedef Mfma<T> MF;
   typedef typename MF::acc_t acc_t;
+  constexpr int NF = VEC ? 4 : 1;          // f1 same, f1 opposite, f2 same, f2 opposite
+  constexpr int NT = 64 * kAnaWaves;
+  constexpr int NP = kChunk * 16;          // (pair, real column) slots per chunk
+  constexpr int PER = (NP + NT - 1) / NT;
+  __shared__ T fs[2][NF][kChunk][16];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int* tl = a.tiles + 3 * blockIdx.y;
-  const int p = tl[1];
-  const int kbase = tl[2] + 16 * (w & 1);
-  const int nbt = (a.B + 15) / 16;
+  const int m = tl[0], p = tl[1];
+  if (m >= n_m) return;  // uniform per workgroup
+  const int kbase = tl[2] + 16 * w;
+  const int nbt = (a.B + 7) / 8;
   const int comp = blockIdx.x / nbt;
-  const int b0 = (blockIdx.x % nbt) * 16 + 8 * (w >> 1);
+  const int bc0 = (blockIdx.x - comp * nbt) * 8;
   const int col = lane & 15, kk = lane >> 4;
-  const int b = b0 + (col >> 1), reim = col & 1;
-  const bool bval = b < a.B;
-  const int bl = bval ? b : 0;
-  const double sg = p ? -1.0 : 1.0;  // same-parity fold: N + sg*S
-  const int arow = kbase + col;      // row of A handled by this lane
-  {
-    const int m = tl[0];
-    if (m >= n_m) return;
-    const int* info = a.mp_info + (m * 2 + p) * 4;
-    const int K = info[0], K16 = info[3];
-    const T* tW = reinterpret_cast<const T*>(a.anaW) + info[2];
-    const T* tD = reinterpret_cast<const T*>(a.anaWd) + info[2];
-    acc_t acc[VEC ? 4 : 1];
+  const double sg = p ? -1.0 : 1.0;
+  const int* info = a.mp_info + (m * 2 + p) * 4;
+  const int K = info[0], K16 = info[3];
+  const T* tW = reinterpret_cast<const T*>(a.anaW) + info[2];
+  const T* tD = reinterpret_cast<const T*>(a.anaWd) + info[2];
+  const int c1 = a.cf1[comp], c2 = VEC ? a.cf2[comp] : 0;
+  acc_t acc[VEC ? 4 : 1];
 #pragma unroll
-    for (int q = 0; q < (VEC ? 4 : 1); ++q) acc[q] = (acc_t){0, 0, 0, 0};
-    if (kbase < K) {
-      // Operands of one theta step (4 pairs); the next step's are loaded before
-      // the current step's MFMAs are issued.
-      struct Ops {
-        T aw, ad, x1, x2, x3, x4;
-      };
-      const int c1 = a.cf1[comp], c2 = VEC ? a.cf2[comp] : 0;
-      auto load = [&](int j0) {
-        Ops o;
-        const int j = j0 + kk;
-        o.aw = tW[(long)j * K16 + arow];
-        o.ad = VEC ? tD[(long)j * K16 + arow] : (T)0;
-        o.x1 = o.x2 = o.x3 = o.x4 = (T)0;
-        if (bval && j < a.H) {
-          const long iN = fidx(a.c_stride, a.m_stride, a.Bin, 0, m, 2 * j, bl) + reim;
-          const long iS = fidx(a.c_stride, a.m_stride, a.Bin, 0, m, 2 * j + 1, bl) + reim;
+  for (int q = 0; q < (VEC ? 4 : 1); ++q) acc[q] = (acc_t){0, 0, 0, 0};
+  T stg[PER][NF];
+  auto fetch = [&](int j0) {
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int v = threadIdx.x + NT * i;
+#pragma unroll
+      for (int f = 0; f < NF; ++f) stg[i][f] = (T)0;
+      if (v < NP) {
+        const int j = j0 + v / 16, c16 = v & 15;
+        const int b = bc0 + (c16 >> 1);
+        if (j < a.H && b < a.B) {
+          const long iN = fidx(a.c_stride, a.m_stride, a.Bin, 0, m, 2 * j, b) + (c16 & 1);
+          const long iS = iN + 2L * a.Bin;  // row 2j+1
           const double f1N = a.F[c1 * a.c_stride + iN], f1S = a.F[c1 * a.c_stride + iS];
-          o.x1 = (T)(f1N + sg * f1S);  // same fold
+          stg[i][0] = (T)(f1N + sg * f1S);
           if (VEC) {
             const double f2N = a.F[c2 * a.c_stride + iN], f2S = a.F[c2 * a.c_stride + iS];
-            o.x2 = (T)(f1N - sg * f1S);  // opposite fold
-            o.x3 = (T)(f2N + sg * f2S);
-            o.x4 = (T)(f2N - sg * f2S);
+            stg[i][1] = (T)(f1N - sg * f1S);
+            stg[i][2] = (T)(f2N + sg * f2S);
+            stg[i][3] = (T)(f2N - sg * f2S);
           }
         }
-        return o;
-      };
-      Ops cur = load(0);
-      for (int j0 = 0; j0 < a.H; j0 += 4) {
-        Ops nxt = cur;
-        if (j0 + 4 < a.H) nxt = load(j0 + 4);
-        if (!VEC) {
-          acc[0] = MF::run(cur.aw, cur.x1, acc[0]);
-        } else {
-          acc[0] = MF::run(cur.ad, cur.x4, acc[0]);  // wdP f2(opposite fold)
-          acc[1] = MF::run(cur.ad, cur.x2, acc[1]);  // wdP f1(opposite fold)
-          acc[2] = MF::run(cur.aw, cur.x1, acc[2]);  // wP f1(same fold)
-          acc[3] = MF::run(cur.aw, cur.x3, acc[3]);  // wP f2(same fold)
-        }
-        cur = nxt;
       }
     }
-    const double dm = (double)m;
-    const long lms = a.lm_start[m];
+  };
+  auto store = [&](int buf) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int k = kbase + MF::row(lane, i);
-      const int l = m + p + 2 * k;
-      const bool st = bval && (k < K);
-      const long off = (lms + p + 2 * k) * 2L * a.ldo + 2 * bl + reim;
-      if (!VEC) {
-        if (st) a.outS[comp][off] = (double)acc[0][i];
-      } else {
-        const double inv = (l > 0) ? 1.0 / ((double)l * (double)(l + 1)) : 0.0;
-        const double Sv = (double)acc[0][i] + imul_pair((double)acc[2][i], -dm, lane);
-        const double Tv = -(double)acc[1][i] + imul_pair((double)acc[3][i], -dm, lane);
-        if (st) {
-          a.outS[comp][off] = Sv * inv;
-          if (a.outT[comp]) a.outT[comp][off] = Tv * inv;
+    for (int i = 0; i < PER; ++i) {
+      const int v = threadIdx.x + NT * i;
+      if (v < NP) {
+#pragma unroll
+        for (int f = 0; f < NF; ++f) fs[buf][f][v / 16][v & 15] = stg[i][f];
+      }
+    }
+  };
+  const bool active = kbase < K;  // uniform per wave
+  const int arow = kbase + col;
+  fetch(0);
+  store(0);
+  __syncthreads();
+  const int nch = (a.H + kChunk - 1) / kChunk;
+  for (int ch = 0; ch < nch; ++ch) {
+    const int buf = ch & 1, j0 = ch * kChunk;
+    if (ch + 1 < nch) fetch(j0 + kChunk);
+    if (active) {
+      const int jsteps = min(kChunk, a.H - j0 + 3) / 4;  // tables hold Hp >= H rows
+      for (int s = 0; s < jsteps; ++s) {
+        const long to = (long)(j0 + 4 * s + kk) * K16 + arow;
+        const T aw = tW[to];
+        const T x1 = fs[buf][0][4 * s + kk][col];
+        if (!VEC) {
+          acc[0] = MF::run(aw, x1, acc[0]);
+        } else {
+          const T ad = tD[to];
+          const T x2 = fs[buf][1][4 * s + kk][col];
+          const T x3 = fs[buf][2][4 * s + kk][col];
+          const T x4 = fs[buf][3][4 * s + kk][col];
+          acc[0] = MF::run(ad, x4, acc[0]);  // wdP f2 (opposite fold)
+          acc[1] = MF::run(ad, x2, acc[1]);  // wdP f1 (opposite fold)
+          acc[2] = MF::run(aw, x1, acc[2]);  // wP f1 (same fold)
+          acc[3] = MF::run(aw, x3, acc[3]);  // wP f2 (same fold)
         }
+      }
+    }
+    if (ch + 1 < nch) {
+      // buf ^ 1 was last read in chunk ch - 1, before the previous barrier
+      store(buf ^ 1);
+      __syncthreads();
+    }
+  }
+  if (!active) return;
+  const double dm = (double)m;
+  const long lms = a.lm_start[m];
+  const int b = bc0 + (col >> 1), reim = col & 1;
+  const bool bval = b < a.B;
+  const int bl = bval ? b : 0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int k = kbase + MF::row(lane, i);
+    const int l = m + p + 2 * k;
+    const bool st = bval && (k < K);
+    const long off = (lms + p + 2 * k) * 2L * a.ldo + 2 * bl + reim;
+    if (!VEC) {
+      if (st) a.outS[comp][off] = (double)acc[0][i];
+    } else {
+      const double inv = (l > 0) ? 1.0 / ((double)l * (double)(l + 1)) : 0.0;
+      const double Sv = (double)acc[0][i] + imul_pair((double)acc[2][i], -dm, lane);
+      const double Tv = -(double)acc[1][i] + imul_pair((double)acc[3][i], -dm, lane);
+      if (st) {
+        a.outS[comp][off] = Sv * inv;
+        if (a.outT[comp]) a.outT[comp][off] = Tv * inv;
       }
     }
   }
@@ -352,15 +424,16 @@ extern "C" int mh_leg_analysis(int vector, const MhAnaArgs* args, void* stream) 
       a.n_m < 1)
     return MH_EINVAL;
   const int n_m = a.n_m;
-  dim3 grid(a.ncomp * mh_cdiv(a.B, 16), a.ntiles);
+  dim3 grid(a.ncomp * mh_cdiv(a.B, 8), a.ntiles);
+  const dim3 blk(64 * kAnaWaves);
   if (a.prec != 0 && a.prec != 1) return MH_EINVAL;
   hipStream_t st = mh_stream(stream);
   if (a.prec == 0) {
-    if (vector) hipLaunchKernelGGL((k_ana<true, double>), grid, dim3(256), 0, st, a, n_m);
-    else hipLaunchKernelGGL((k_ana<false, double>), grid, dim3(256), 0, st, a, n_m);
+    if (vector) hipLaunchKernelGGL((k_ana<true, double>), grid, blk, 0, st, a, n_m);
+    else hipLaunchKernelGGL((k_ana<false, double>), grid, blk, 0, st, a, n_m);
   } else {
-    if (vector) hipLaunchKernelGGL((k_ana<true, float>), grid, dim3(256), 0, st, a, n_m);
-    else hipLaunchKernelGGL((k_ana<false, float>), grid, dim3(256), 0, st, a, n_m);
+    if (vector) hipLaunchKernelGGL((k_ana<true, float>), grid, blk, 0, st, a, n_m);
+    else hipLaunchKernelGGL((k_ana<false, float>), grid, blk, 0, st, a, n_m);
   }
   return mh_launch_status();
 }

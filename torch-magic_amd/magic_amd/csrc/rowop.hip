@@ -13,7 +13,7 @@
 typedef MhRowJob RowJob;
 
 // tiles: (job, row0, col0) triples; WG = 4 waves x (8 rows, 64 columns).
-__global__ __launch_bounds__(256) void k_rowop(const RowJob* __restrict__ jobs,
+__global__ __launch_bounds__(256, 4) void k_rowop(const RowJob* __restrict__ jobs,
                                                const int* __restrict__ tiles) {
   const int* tl = tiles + 3 * blockIdx.x;
   const RowJob& J = jobs[tl[0]];
@@ -30,12 +30,25 @@ __global__ __launch_bounds__(256) void k_rowop(const RowJob* __restrict__ jobs,
 #pragma unroll
   for (int q = 0; q < 4; ++q) acc[q] = (double4_t){0.0, 0.0, 0.0, 0.0};
   const double* mt = J.MT + col0 + c16;
-  for (int k0 = 0; k0 < J.K; k0 += 4) {
+  // operands of k-step k0 (4 k values); the next step's are loaded before the
+  // current step's MFMAs are issued
+  auto load = [&](int k0, double& av, double (&bv)[4]) {
     const int k = k0 + kk;
-    const double av = (va && k < J.K) ? xrow[2 * k] : 0.0;
-    const double* mk = mt + (long)k * J.Rpad;
+    av = (va && k < J.K) ? xrow[2 * k] : 0.0;
+    const double* mk = mt + (long)k * J.Rpad;  // MT rows are padded to a multiple of 4
 #pragma unroll
-    for (int q = 0; q < 4; ++q) acc[q] = mfma_f64(av, mk[16 * q], acc[q]);
+    for (int q = 0; q < 4; ++q) bv[q] = mk[16 * q];
+  };
+  double av, bv[4];
+  load(0, av, bv);
+  for (int k0 = 0; k0 < J.K; k0 += 4) {
+    double an = av, bn[4] = {bv[0], bv[1], bv[2], bv[3]};
+    if (k0 + 4 < J.K) load(k0 + 4, an, bn);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[q] = mfma_f64(av, bv[q], acc[q]);
+    av = an;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) bv[q] = bn[q];
   }
 #pragma unroll
   for (int i = 0; i < 4; ++i) {

@@ -92,6 +92,12 @@ __device__ __forceinline__ double2 imex(const ImexSys& s, long i, const RhsArgs&
 }
 
 __global__ __launch_bounds__(256) void k_rhs(RhsArgs a) {
+  if (a.wdev) {  // weights of this step from device memory (launch arguments fixed)
+    a.wimp = a.wdev[0];
+    a.wl1 = a.wdev[1];
+    a.we0 = a.wdev[2];
+    a.we1 = a.wdev[3];
+  }
   const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const int N = a.N;
   if (idx >= (long)a.L * N) return;
@@ -123,6 +129,7 @@ extern "C" int mh_rhs(const MhRhsArgs* args, void* stream) {
 typedef MhBuoyArgs BuoyArgs;
 
 __global__ __launch_bounds__(256) void k_buoy(BuoyArgs a) {
+  if (a.wdev) a.fac = a.wdev[4];
   const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const int N = a.N;
   if (idx >= (long)a.L * N) return;

@@ -147,6 +147,18 @@ class DynamoRun:
         self.n_rebuild = 0
         self.time = 0.0
         self.timers = None   # {name: [(start_event, end_event), ...]} while timing
+        # launch plans: the radial and lm phases as recorded C-ABI calls, one per
+        # parity of the explicit-history buffers (they alternate every step);
+        # the dt-dependent IMEX weights come from a device buffer, so a plan (and
+        # its HIP graph) stays valid for the whole run
+        self._E = (self.expl_cur, self.expl_prev)
+        self._par = 0
+        self.wdev = torch.zeros(8, dtype=F64, device=dev)
+        self.w_host = torch.zeros(8, dtype=F64).pin_memory()
+        self._plans = {}
+        self._graphs = {}
+        self.use_graphs = False
+        self._sec_ev = None
 
     # ------------------------------------------------------------ timing
     def enable_timers(self, on: bool = True):
@@ -158,17 +170,64 @@ class DynamoRun:
 
         class _Sec:
             def __enter__(self_):
+                rec = _lib._recording
+                if rec is not None:
+                    self_.prev = rec.section
+                    rec.section = name
+                    return
                 if run.timers is not None:
                     self_.ev = (torch.cuda.Event(enable_timing=True),
                                 torch.cuda.Event(enable_timing=True))
                     self_.ev[0].record()
 
             def __exit__(self_, *exc):
+                rec = _lib._recording
+                if rec is not None:
+                    rec.section = self_.prev
+                    return False
                 if run.timers is not None:
                     self_.ev[1].record()
                     run.timers.setdefault(name, []).append(self_.ev)
                 return False
+
         return _Sec()
+
+    def _on_section(self, name, begin):
+        if begin:
+            self._sec_ev = torch.cuda.Event(enable_timing=True)
+            self._sec_ev.record()
+        else:
+            end = torch.cuda.Event(enable_timing=True)
+            end.record()
+            self.timers.setdefault(name, []).append((self._sec_ev, end))
+
+    def _exec(self, kind):
+        """Run the radial or lm phase: recorded once per parity, then replayed
+        eagerly (with section timers when enabled) or as a captured HIP graph."""
+        key = (kind, self._par)
+        plan = self._plans.get(key)
+        if plan is None:
+            with _lib.record() as rec:
+                if kind == "radial":
+                    self._radial_launches()
+                else:
+                    self._lm_launches()
+            plan = self._plans[key] = rec.calls
+        if self.use_graphs and self.timers is None:
+            g = self._graphs.get(key)
+            if g is None:
+                cur = torch.cuda.current_stream()
+                side = torch.cuda.Stream(device=self.device)  # default priority
+                side.wait_stream(cur)
+                g = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(g, stream=side):
+                    _lib.run_plan(plan, stream_handle())
+                cur.wait_stream(side)
+                self._graphs[key] = g
+            g.replay()
+        else:
+            _lib.run_plan(plan, stream_handle(),
+                          self._on_section if self.timers is not None else None)
 
     def timer_summary(self):
         """{name: (launch count, mean ms)} of the recorded sections (synchronises)."""
@@ -422,6 +481,7 @@ class DynamoRun:
 
     def setup_initial_state(self):
         """Derivatives, old states and implicit terms from the fields (``step_time.py:398-617``)."""
+        self._par = 0 if self.expl_cur is self._E[0] else 1
         for d in (self.old, self.impl, self.expl_cur, self.expl_prev):
             for t in d.values():
                 t.zero_()
@@ -459,6 +519,10 @@ class DynamoRun:
 
     def radial_loop(self):
         """Explicit terms of the current state; returns (dtrkc_min, dthkc_min)."""
+        self._exec("radial")
+        return self._cfl_decision()
+
+    def _radial_launches(self):
         r, f, N, Nb = self.res, self.f, self.N, self.Nb
         # 1. Legendre synthesis
         with self._t("leg_syn_scal"):
@@ -489,7 +553,7 @@ class DynamoRun:
                        [ptr(nl["Advt"]) + off, ptr(nl["VSt"]) + off, ptr(nl["VxBt"]) + off],
                        [ptr(nl["Advp"]) + off, None, ptr(nl["VxBp"]) + off], N, vector=True,
                        prec=self.leg_prec)
-        return self._radial_finish()
+        self._td_launches()
 
     def _grid_separate(self):
         r, N, Nb, d = self.res, self.N, self.Nb, self.res.d
@@ -513,7 +577,7 @@ class DynamoRun:
         with self._t("fft_fwd"):
             r.fft(True, self.F_o, Nb, self.G_o, 9, Nb, comp_scale=d["comp_scale_v"])
 
-    def _radial_finish(self):
+    def _td_launches(self):
         r, f, N, nl = self.res, self.f, self.N, self.nl
         # 6. radial derivatives of the fluxes
         with self._t("rowop_dr"):
@@ -530,7 +594,10 @@ class DynamoRun:
         t.CorFac, t.l_mag = self.p.CorFac, int(self.p.l_mag)
         with self._t("td"):
             _lib.call("mh_td", _lib.ref(t), stream_handle())
-        # Courant limits (host decision, like the reference's .item())
+
+    def _cfl_decision(self):
+        """Courant limits (host decision, like the reference's .item())."""
+        r, N = self.res, self.N
         self.cfl_host.copy_(self.cfl, non_blocking=True)
         torch.cuda.current_stream().synchronize()
         v = self.cfl_host.numpy().view(np.float64)
@@ -544,15 +611,28 @@ class DynamoRun:
         """Implicit solves and post-processing (``_fused_lm_loop_fast`` + insulating B)."""
         p = self.p
         dt1, dt2 = self.dt_hist
-        wl0, wl1 = p.alpha * dt1, (1.0 - p.alpha) * dt1
-        we0, we1 = (1.0 + 0.5 * dt1 / dt2) * dt1, -0.5 * dt1 * dt1 / dt2
+        w = self.w_host
+        w[0] = 1.0                                    # wimp
+        w[1] = (1.0 - p.alpha) * dt1                  # wimp_lin[1]
+        w[2] = (1.0 + 0.5 * dt1 / dt2) * dt1          # wexp[0]
+        w[3] = -0.5 * dt1 * dt1 / dt2                 # wexp[1]
+        w[4] = p.alpha * dt1 * p.BuoFac               # wimp_lin[0] * BuoFac (buoyancy)
+        # the host writes w_host only after the step's Courant synchronisation, so
+        # the previous step's copy has completed
+        self.wdev.copy_(w, non_blocking=True)
+        self._exec("lm")
+        self.expl_cur, self.expl_prev = self.expl_prev, self.expl_cur
+        self._par ^= 1
+
+    def _lm_launches(self):
+        p = self.p
         a = _lib.RhsArgs()
         for i, k in enumerate(_SYS):
             a.sys[i].old, a.sys[i].impl = ptr(self.old[k]), ptr(self.impl[k])
             a.sys[i].expl0, a.sys[i].expl1 = ptr(self.expl_cur[k]), ptr(self.expl_prev[k])
         a.rs, a.rz, a.rb, a.rj, a.rwp = (ptr(self.rhs[k]) for k in ("s", "z", "b", "j", "wp"))
         a.m, a.L, a.N = ptr(self.res.d["m"]), self.L, self.N
-        a.wimp, a.wl1, a.we0, a.we1 = 1.0, wl1, we0, we1
+        a.wdev = ptr(self.wdev)
         a.bots_l0, a.l_mag = math.sqrt(4.0 * math.pi), int(p.l_mag)
         with self._t("rhs"):
             _lib.call("mh_rhs", _lib.ref(a), stream_handle())
@@ -561,14 +641,13 @@ class DynamoRun:
         bq = _lib.BuoyArgs()
         bq.rwp, bq.s, bq.ew0 = ptr(self.rhs["wp"]), ptr(self.f["s"]), ptr(self.expl_cur["w"])
         bq.rgrav, bq.m, bq.L, bq.N = ptr(self.res.d["rgrav"]), ptr(self.res.d["m"]), self.L, self.N
-        bq.fac, bq.BuoFac, bq.lm_l0 = wl0 * p.BuoFac, p.BuoFac, 0
+        bq.BuoFac, bq.lm_l0, bq.wdev = p.BuoFac, 0, ptr(self.wdev)
         with self._t("buoy"):
             _lib.call("mh_buoy", _lib.ref(bq), stream_handle())
         with self._t("rowop_wp"):
             self._rowop(self.jobs2)
         with self._t("impl"):
             self._impl()
-        self.expl_cur, self.expl_prev = self.expl_prev, self.expl_cur
 
     def one_step(self, n_time_step: int, dt: float) -> float:
         """One CNAB2 step (reference ``step_time.py:1574-1637``); returns the dt used."""

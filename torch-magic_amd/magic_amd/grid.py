@@ -174,7 +174,7 @@ def sht_layout(p: Params, maps: dict) -> dict:
             info[m, par] = (K, syn_off, ana_off, K16)
             syn_off += K4 * Hp
             ana_off += K16 * Hp
-            for k0 in range(0, K, 32):
+            for k0 in range(0, K, 96):  # one analysis workgroup = 6 waves x 16 degrees
                 tiles.append((m, par, k0))
     if syn_off >= 2 ** 31 or ana_off >= 2 ** 31:
         raise ValueError("Legendre tables exceed 32-bit offsets")
