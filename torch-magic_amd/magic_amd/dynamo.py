@@ -159,6 +159,7 @@ class DynamoRun:
         self._graphs = {}
         self.use_graphs = False
         self._sec_ev = None
+        self._cfl_event = None
 
     # ------------------------------------------------------------ timing
     def enable_timers(self, on: bool = True):
@@ -597,9 +598,23 @@ class DynamoRun:
 
     def _cfl_decision(self):
         """Courant limits (host decision, like the reference's .item())."""
-        r, N = self.res, self.N
+        self._cfl_copy()
+        return self._cfl_values()
+
+    def _cfl_copy(self):
+        """Queue the copy of the per-level Courant maxima to pinned host memory."""
         self.cfl_host.copy_(self.cfl, non_blocking=True)
-        torch.cuda.current_stream().synchronize()
+        if self._cfl_event is None:
+            self._cfl_event = torch.cuda.Event()
+        self._cfl_event.record()
+
+    def cfl_ready(self) -> bool:
+        """True once the queued Courant maxima have reached the host."""
+        return self._cfl_event is None or self._cfl_event.query()
+
+    def _cfl_values(self):
+        r, N = self.res, self.N
+        self._cfl_event.synchronize()
         v = self.cfl_host.numpy().view(np.float64)
         rad = r.rad
         with np.errstate(divide="ignore"):
@@ -651,7 +666,17 @@ class DynamoRun:
 
     def one_step(self, n_time_step: int, dt: float) -> float:
         """One CNAB2 step (reference ``step_time.py:1574-1637``); returns the dt used."""
-        dtrkc, dthkc = self.radial_loop()
+        self.begin_step()
+        return self.finish_step(n_time_step, dt)
+
+    def begin_step(self):
+        """Queue the radial loop and the copy of the Courant maxima (no host wait)."""
+        self._exec("radial")
+        self._cfl_copy()
+
+    def finish_step(self, n_time_step: int, dt: float) -> float:
+        """Wait for the Courant maxima, decide dt, rebuild if needed, queue the lm loop."""
+        dtrkc, dthkc = self._cfl_values()
         l_new_dt, dt_new = dt_courant(dt, self.dtMax, dtrkc, dthkc)
         self.dt_hist = [dt_new, self.dt_hist[0]]
         wl0 = self.p.alpha * dt_new
