@@ -157,7 +157,7 @@ __global__ __launch_bounds__(64 * kSynWaves, 2) void k_syn(SynArgs a, int n_m) {
   for (int q = 0; q < (VEC ? 10 : 2); ++q) acc[q] = (acc_t){0, 0, 0, 0};
   // staging map of this thread: value v = threadIdx.x + NT*i -> (field, k, real column)
   T stg[PER];
-#pragma unroll 1
+#pragma unroll  // static accumulator indices (acc[5p + q] stays in registers)
   for (int p = 0; p < 2; ++p) {
     const int* info = a.mp_info + (m * 2 + p) * 4;
     const int K = info[0];
@@ -196,31 +196,41 @@ __global__ __launch_bounds__(64 * kSynWaves, 2) void k_syn(SynArgs a, int n_m) {
     __syncthreads();  // previous parity's readers are done with both buffers
     store(0);
     __syncthreads();
+    // table rows are padded to a multiple of 4; waves past the last theta pair
+    // only help with the staging
+    const int nsteps = (j0 < a.H) ? (K + 3) / 4 : 0;
+    const bool jv = j0 + col < a.Hp;
+    // table operands of step t (degrees 4t .. 4t+3), loaded one step ahead
+    auto tload = [&](int t, T& p_, T& d_) {
+      const long to = (long)(4 * t + kk) * a.Hp + j0 + col;
+      p_ = jv ? tP[to] : (T)0;
+      d_ = (VEC && jv) ? tD[to] : (T)0;
+    };
+    T aP = (T)0, aD = (T)0;
+    if (nsteps > 0) tload(0, aP, aD);
     const int nch = (K + kChunk - 1) / kChunk;
     for (int ch = 0; ch < nch; ++ch) {
-      const int buf = ch & 1, c0 = ch * kChunk;
-      if (ch + 1 < nch) fetch(c0 + kChunk);
-      // table rows are padded to a multiple of 4; waves past the last theta
-      // pair only help with the staging
-      const int ksteps = (j0 < a.H) ? min(kChunk, K - c0 + 3) / 4 : 0;
-      for (int s = 0; s < ksteps; ++s) {
-        const int k = c0 + 4 * s + kk;
-        const long to = (long)k * a.Hp + j0 + col;
-        const bool jv = j0 + col < a.Hp;
-        const T aP = jv ? tP[to] : (T)0;
-        const T xq = xs[buf][0][4 * s + kk][col];
+      const int buf = ch & 1;
+      if (ch + 1 < nch) fetch((ch + 1) * kChunk);
+      const int t0 = ch * (kChunk / 4), t1 = min(nsteps, t0 + kChunk / 4);
+      for (int t = t0; t < t1; ++t) {
+        T nP = aP, nD = aD;
+        if (t + 1 < nsteps) tload(t + 1, nP, nD);
+        const int kr = 4 * (t - t0) + kk;
+        const T xq = xs[buf][0][kr][col];
         if (!VEC) {
           acc[p] = MF::run(aP, xq, acc[p]);
         } else {
-          const T aD = jv ? tD[to] : (T)0;
-          const T xsv = xs[buf][1][4 * s + kk][col];
-          const T xt = xs[buf][2][4 * s + kk][col];
+          const T xsv = xs[buf][1][kr][col];
+          const T xt = xs[buf][2][kr][col];
           acc[5 * p + 0] = MF::run(aP, xq, acc[5 * p + 0]);   // P Q
           acc[5 * p + 1] = MF::run(aP, xt, acc[5 * p + 1]);   // P T
           acc[5 * p + 2] = MF::run(aP, xsv, acc[5 * p + 2]);  // P S
           acc[5 * p + 3] = MF::run(aD, xsv, acc[5 * p + 3]);  // dP S
           acc[5 * p + 4] = MF::run(aD, xt, acc[5 * p + 4]);   // dP T
         }
+        aP = nP;
+        aD = nD;
       }
       if (ch + 1 < nch) {
         store(buf ^ 1);
@@ -359,6 +369,16 @@ __global__ __launch_bounds__(64 * kAnaWaves, 2) void k_ana(AnaArgs a, int n_m) {
   };
   const bool active = kbase < K;  // uniform per wave
   const int arow = kbase + col;
+  // weight-table operands of theta step t (pairs 4t .. 4t+3), loaded one step
+  // ahead; the tables hold Hp >= roundup(H, 4) rows
+  const int nsteps = (a.H + 3) / 4;
+  auto tload = [&](int t, T& w_, T& d_) {
+    const long to = (long)(4 * t + kk) * K16 + arow;
+    w_ = tW[to];
+    d_ = VEC ? tD[to] : (T)0;
+  };
+  T aw = (T)0, ad = (T)0;
+  if (active) tload(0, aw, ad);
   fetch(0);
   store(0);
   __syncthreads();
@@ -367,23 +387,25 @@ __global__ __launch_bounds__(64 * kAnaWaves, 2) void k_ana(AnaArgs a, int n_m) {
     const int buf = ch & 1, j0 = ch * kChunk;
     if (ch + 1 < nch) fetch(j0 + kChunk);
     if (active) {
-      const int jsteps = min(kChunk, a.H - j0 + 3) / 4;  // tables hold Hp >= H rows
-      for (int s = 0; s < jsteps; ++s) {
-        const long to = (long)(j0 + 4 * s + kk) * K16 + arow;
-        const T aw = tW[to];
-        const T x1 = fs[buf][0][4 * s + kk][col];
+      const int t0 = ch * (kChunk / 4), t1 = min(nsteps, t0 + kChunk / 4);
+      for (int t = t0; t < t1; ++t) {
+        T nw = aw, nd = ad;
+        if (t + 1 < nsteps) tload(t + 1, nw, nd);
+        const int jr = 4 * (t - t0) + kk;
+        const T x1 = fs[buf][0][jr][col];
         if (!VEC) {
           acc[0] = MF::run(aw, x1, acc[0]);
         } else {
-          const T ad = tD[to];
-          const T x2 = fs[buf][1][4 * s + kk][col];
-          const T x3 = fs[buf][2][4 * s + kk][col];
-          const T x4 = fs[buf][3][4 * s + kk][col];
+          const T x2 = fs[buf][1][jr][col];
+          const T x3 = fs[buf][2][jr][col];
+          const T x4 = fs[buf][3][jr][col];
           acc[0] = MF::run(ad, x4, acc[0]);  // wdP f2 (opposite fold)
           acc[1] = MF::run(ad, x2, acc[1]);  // wdP f1 (opposite fold)
           acc[2] = MF::run(aw, x1, acc[2]);  // wP f1 (same fold)
           acc[3] = MF::run(aw, x3, acc[3]);  // wP f2 (same fold)
         }
+        aw = nw;
+        ad = nd;
       }
     }
     if (ch + 1 < nch) {
