@@ -312,10 +312,12 @@ int mh_band_factor(double* abd, int* pivot, int* info, int n, int kl, int ku, in
 /* dgbsl forward/back substitution for complex right-hand sides grouped by
  * degree, with row (and column) preconditioning: solve_band_real
  * (algebra.py:460-506) inside banded_solve_by_l (algebra.py:672-723).
- * rhs/sol: (L, n) complex rows; rows of degree l use factor l. */
+ * rhs/sol: (L, n) complex rows (sol may alias rhs); rows of degree l use factor l.
+ * rows: optional processing order of the L rows (grouped by degree, so that
+ * the lanes of a wave share factors); kl, ku <= 4. */
 int mh_band_solve(const double* abd, const int* pivot, const double* fac_row,
-                  const double* fac_col, const int* lm2l, const double* rhs, double* sol, int L,
-                  int n, int kl, int ku, void* stream);
+                  const double* fac_col, const int* lm2l, const int* rows, const double* rhs,
+                  double* sol, int L, int n, int kl, int ku, void* stream);
 
 /* Sizes of the argument structs as compiled (binding checks). */
 int mh_struct_sizes(int* out, int n);

@@ -146,7 +146,7 @@ _SIGS = {
     "mh_gj_inverse": [P, P, P, P, I, I, I, P, P],
     "mh_gemm_f64": [P, P],
     "mh_band_factor": [P, P, P, I, I, I, I, P],
-    "mh_band_solve": [P, P, P, P, P, P, P, I, I, I, I, P],
+    "mh_band_solve": [P, P, P, P, P, P, P, P, I, I, I, I, P],
     "mh_struct_sizes": [P, I],
 }
 

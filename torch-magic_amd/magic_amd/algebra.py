@@ -113,8 +113,9 @@ def banded_solve_by_l(abd_by_l, pivot_by_l, l_index, rhs_complex, n, kl, ku,
     fr = None if fac_row_by_l is None else torch.as_tensor(fac_row_by_l, dtype=F64).to(dev).contiguous()
     fc = None if fac_col_by_l is None else torch.as_tensor(fac_col_by_l, dtype=F64).to(dev).contiguous()
     sol = torch.empty_like(rhs)
-    _lib.call("mh_band_solve", ptr(abd), ptr(piv), ptr(fr), ptr(fc), ptr(lidx), ptr(rhs),
-              ptr(sol), rhs.shape[0], n, kl, ku, stream_handle())
+    rows = torch.argsort(lidx, stable=True).to(I32)   # group the rows by degree
+    _lib.call("mh_band_solve", ptr(abd), ptr(piv), ptr(fr), ptr(fc), ptr(lidx), ptr(rows),
+              ptr(rhs), ptr(sol), rhs.shape[0], n, kl, ku, stream_handle())
     return sol.real.contiguous() if real else sol
 
 

@@ -146,10 +146,17 @@ __global__ __launch_bounds__(64 * kSynWaves, 2) void k_syn(SynArgs a, int n_m) {
   constexpr int PER = (NV + NT - 1) / NT;   // staged values per thread
   __shared__ T xs[2][NF][kChunk][16];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int bc0 = blockIdx.x * 8;
-  const int j0 = blockIdx.y * kSynJ + 16 * w;
+  // XCD-aware order: workgroups b and b+8 run on the same XCD; XCD x takes the
+  // orders m = x, x+8, ... (all tiles of one m on one XCD, so the m's tables
+  // are fetched into that XCD's L2 once), which also balances the K ~ l_max-m work
+  const int nx = (a.ngroups * a.B + 7) / 8, ny = (a.H + kSynJ - 1) / kSynJ;
+  const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+  const int m = (slot / (nx * ny)) * 8 + xcd;
+  if (m >= n_m) return;  // uniform per workgroup
+  const int rem = slot % (nx * ny), ty = rem / nx, tx = rem - ty * nx;
+  const int bc0 = tx * 8;
+  const int j0 = ty * kSynJ + 16 * w;
   const int col = lane & 15, kk = lane >> 4;
-  const int m = blockIdx.z;
   const long lms = a.lm_start[m];
   const int ncol = a.ngroups * a.B;
   acc_t acc[VEC ? 10 : 2];
@@ -280,7 +287,9 @@ extern "C" int mh_leg_synth(int vector, const MhSynArgs* args, int n_m, void* st
       a.Bout != a.ngroups * a.B)
     return MH_EINVAL;
   if (a.prec != 0 && a.prec != 1) return MH_EINVAL;
-  dim3 grid(mh_cdiv(a.ngroups * a.B, 8), mh_cdiv(a.H, kSynJ), n_m);
+  const long units = (long)mh_cdiv(a.ngroups * a.B, 8) * mh_cdiv(a.H, kSynJ) * mh_cdiv(n_m, 8) * 8;
+  if (units > 0x7fffffffL) return MH_EINVAL;
+  dim3 grid((unsigned)units);
   const dim3 blk(64 * kSynWaves);
   hipStream_t st = mh_stream(stream);
   if (a.prec == 0) {
@@ -315,13 +324,18 @@ __global__ __launch_bounds__(64 * kAnaWaves, 2) void k_ana(AnaArgs a, int n_m) {
   constexpr int PER = (NP + NT - 1) / NT;
   __shared__ T fs[2][NF][kChunk][16];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int* tl = a.tiles + 3 * blockIdx.y;
+  // XCD-aware order: the tile list holds 8 equal-length lists, list x with the
+  // tiles of orders m = x (mod 8) (padding entries have m = n_m); workgroups
+  // b and b+8 run on the same XCD, which works through its list in order
+  const int nbt = (a.B + 7) / 8, gx = a.ncomp * nbt, tpx = a.ntiles / 8;
+  const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+  const int* tl = a.tiles + 3 * (xcd * tpx + slot / gx);
   const int m = tl[0], p = tl[1];
   if (m >= n_m) return;  // uniform per workgroup
   const int kbase = tl[2] + 16 * w;
-  const int nbt = (a.B + 7) / 8;
-  const int comp = blockIdx.x / nbt;
-  const int bc0 = (blockIdx.x - comp * nbt) * 8;
+  const int cx = slot % gx;
+  const int comp = cx / nbt;
+  const int bc0 = (cx - comp * nbt) * 8;
   const int col = lane & 15, kk = lane >> 4;
   const double sg = p ? -1.0 : 1.0;
   const int* info = a.mp_info + (m * 2 + p) * 4;
@@ -446,7 +460,8 @@ extern "C" int mh_leg_analysis(int vector, const MhAnaArgs* args, void* stream) 
       a.n_m < 1)
     return MH_EINVAL;
   const int n_m = a.n_m;
-  dim3 grid(a.ncomp * mh_cdiv(a.B, 8), a.ntiles);
+  if (a.ntiles % 8 != 0) return MH_EINVAL;
+  dim3 grid((unsigned)(a.ncomp * mh_cdiv(a.B, 8) * a.ntiles));
   const dim3 blk(64 * kAnaWaves);
   if (a.prec != 0 && a.prec != 1) return MH_EINVAL;
   hipStream_t st = mh_stream(stream);

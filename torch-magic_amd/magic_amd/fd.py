@@ -259,12 +259,15 @@ class FDOperators:
                                   fc=torch.as_tensor(fc, dtype=F64).to(dev).contiguous())
         return self
 
-    def solve(self, name, rhs, l_index, out=None):
+    def solve(self, name, rhs, l_index, out=None, rows=None):
         """Solve every row lm of rhs (L, n) complex with the degree l_index[lm]
-        factors of system S, Z or W, all rows in one launch."""
+        factors of system S, Z or W, all rows in one launch.  rows: the rows
+        grouped by degree (computed from l_index when not given)."""
         s = self.dev[name]
         out = torch.empty_like(rhs) if out is None else out
+        if rows is None:
+            rows = torch.argsort(l_index.to(torch.int64), stable=True).to(I32)
         _lib.call("mh_band_solve", ptr(s["abd"]), ptr(s["piv"]), ptr(s["fr"]),
-                  ptr(s["fc"]) if name == "W" else 0, ptr(l_index), ptr(rhs), ptr(out),
-                  rhs.shape[0], self.n, s["k"], s["k"], stream_handle())
+                  ptr(s["fc"]) if name == "W" else 0, ptr(l_index), ptr(rows), ptr(rhs),
+                  ptr(out), rhs.shape[0], self.n, s["k"], s["k"], stream_handle())
         return out

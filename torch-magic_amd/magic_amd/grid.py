@@ -166,6 +166,7 @@ def sht_layout(p: Params, maps: dict) -> dict:
     info = np.zeros((p.n_m, 2, 4), np.int32)
     syn_off = ana_off = 0
     tiles = []
+    per_xcd = [[] for _ in range(8)]
     for m in range(p.n_m):
         for par in range(2):
             K = (p.l_max - m - par) // 2 + 1 if p.l_max - m - par >= 0 else 0
@@ -175,7 +176,12 @@ def sht_layout(p: Params, maps: dict) -> dict:
             syn_off += K4 * Hp
             ana_off += K16 * Hp
             for k0 in range(0, K, 96):  # one analysis workgroup = 6 waves x 16 degrees
-                tiles.append((m, par, k0))
+                per_xcd[m % 8].append((m, par, k0))
+    # analysis tiles as 8 equal-length lists (list x: orders m = x mod 8; the
+    # kernel gives list x to the workgroups of XCD x), padded with m = n_m
+    tpx = max(len(t) for t in per_xcd)
+    for t in per_xcd:
+        tiles += t + [(p.n_m, 0, 0)] * (tpx - len(t))
     if syn_off >= 2 ** 31 or ana_off >= 2 ** 31:
         raise ValueError("Legendre tables exceed 32-bit offsets")
     return dict(H=H, Hp=Hp, mp_info=info.reshape(-1), syn_size=syn_off, ana_size=ana_off,
