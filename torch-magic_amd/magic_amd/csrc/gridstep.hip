@@ -121,37 +121,6 @@ struct Ctx {
   int busy;         // LDS buffer that other threads may still be reading
 };
 
-// Z_k of the packed inverse real FFT, read from the raw spectrum X in LDS:
-// Z_k = E_k + i O_k, E_k = X_k + conj(X_{Mh-k}), O_k = (X_k - conj(X_{Mh-k})) e^{+2 pi i k/n}
-__device__ __forceinline__ double2 zload(const Geo& g, const GridArgs& a, const double2* X, int k) {
-  double2 x = make_double2(0.0, 0.0), xc = x;
-  if (k < g.M) x = X[k];
-  if (k == 0) x.y = 0.0;
-  const int kc = g.Mh - k;
-  if (k > 0 && kc < g.M) xc = cconj(X[kc]);
-  const double2 E = cadd(x, xc);
-  const double2 O = cmul(csub(x, xc), cconj(ltw2(g)[k]));
-  return make_double2(E.x - O.y, E.y + O.x);
-}
-
-template <int R>
-__device__ __forceinline__ void stage_first_inv(const Geo& g, const GridArgs& a, int win, int wout) {
-  const int nb = g.Mh / R;
-  const double2* in = lbuf(g, win);
-  double2* out = lbuf(g, wout);
-  for (int idx = threadIdx.x; idx < g.RB * nb; idx += g.threads) {
-    const int row = idx / nb, bf = idx - row * nb;
-    const double2* src = in + row * g.LS;
-    double2 v[R];
-#pragma unroll
-    for (int r = 0; r < R; ++r) v[r] = zload(g, a, src, bf + r * nb);
-    dft<R>(v, 1.0);
-    double2* dst = out + row * g.LS + bf * R;
-#pragma unroll
-    for (int r = 0; r < R; ++r) dst[r] = v[r];
-  }
-}
-
 template <int R>
 __device__ __forceinline__ void stage_mid(const Geo& g, const GridArgs& a, int win, int wout, int Ns,
                                           double dir) {
@@ -166,11 +135,16 @@ __device__ __forceinline__ void stage_mid(const Geo& g, const GridArgs& a, int w
     double2 v[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) v[r] = src[bf + r * nb];
+    if (Ns > 1) {
+      // w^r by recurrence from one table value (one LDS read per butterfly)
+      double2 w1 = tw[k * tstep];
+      w1.y *= -dir;  // tw holds exp(-i...); the inverse (dir = +1) conjugates
+      double2 wr = w1;
 #pragma unroll
-    for (int r = 1; r < R; ++r) {
-      double2 w = tw[r * k * tstep];
-      w.y *= -dir;  // tw holds exp(-i...); the inverse (dir = +1) conjugates
-      v[r] = cmul(v[r], w);
+      for (int r = 1; r < R; ++r) {
+        v[r] = cmul(v[r], wr);
+        if (r + 1 < R) wr = cmul(wr, w1);
+      }
     }
     dft<R>(v, dir);
     double2* dst = out + row * g.LS + (bf / Ns) * Ns * R + k;
@@ -187,15 +161,7 @@ __device__ __forceinline__ void stage_mid_rt(const Geo& g, const GridArgs& a, in
   else stage_mid<5>(g, a, win, wout, Ns, dir);
 }
 
-__device__ __forceinline__ void stage_first_inv_rt(const Geo& g, const GridArgs& a, int R, int win,
-                                                   int wout) {
-  if (R == 4) stage_first_inv<4>(g, a, win, wout);
-  else if (R == 2) stage_first_inv<2>(g, a, win, wout);
-  else if (R == 3) stage_first_inv<3>(g, a, win, wout);
-  else stage_first_inv<5>(g, a, win, wout);
-}
-
-// Inverse stages 1 .. nst-2 of a compiled plan (S = stage, NS = its Ns).
+// Inverse stages 0 .. nst-2 of a compiled plan (S = stage, NS = its Ns).
 template <int MH, int S, int NS>
 __device__ __forceinline__ int inv_mid_ct(const Geo& g, const GridArgs& a, int w) {
   constexpr GPlan P = make_plan(MH);
@@ -223,28 +189,34 @@ __device__ __forceinline__ int fwd_mid_ct(const Geo& g, const GridArgs& a, int w
   }
 }
 
-// Global loads of one component's raw spectrum for the workgroup's levels
-// into registers: element e = m*RB + j (level j of the block) at
-// base + ((m*T + t)*Bst + b0 + j)*2 doubles.
+// Global loads of one component's spectrum for the workgroup's levels, by
+// conjugate pairs: item e = p*RB + j (pair p = 0 .. Mh/2, level j) holds X_p
+// and X_{Mh-p} (zero where the order is >= M or the level is past the end),
+// read from base + ((m*T + t)*Bst + b0 + j)*2 doubles.
 template <int RL>
 struct Prefetch {
-  double2 v[RL];
+  static constexpr int PP = (RL + 2) / 2;  // items per thread (RL = 2, 3, 4)
+  double2 x[PP], xc[PP];
 };
 
 template <int MH, int RL>
 __device__ __forceinline__ void fetch(const Geo& g, const GridArgs& a, Prefetch<RL>& pf,
                                       const double* base, int Bst, int b0, int nvalid, int t) {
-  const int RB = g.RB, n = g.M * RB;
+  const int RB = g.RB, n = (g.Mh / 2 + 1) * RB;
+  auto ld = [&](int m, int j) {
+    return (m < g.M && j < nvalid)
+               ? *reinterpret_cast<const double2*>(base + (((long)m * a.T + t) * Bst + b0 + j) * 2)
+               : make_double2(0.0, 0.0);
+  };
 #pragma unroll
-  for (int i = 0; i < RL; ++i) {
+  for (int i = 0; i < Prefetch<RL>::PP; ++i) {
     const int e = threadIdx.x + i * g.threads;
-    double2 x = make_double2(0.0, 0.0);
+    pf.x[i] = pf.xc[i] = make_double2(0.0, 0.0);
     if (e < n) {
-      const int m = e / RB, j = e - m * RB;
-      if (j < nvalid)
-        x = *reinterpret_cast<const double2*>(base + (((long)m * a.T + t) * Bst + b0 + j) * 2);
+      const int p = e / RB, j = e - p * RB;
+      pf.x[i] = ld(p, j);
+      if (p > 0 && 2 * p < g.Mh) pf.xc[i] = ld(g.Mh - p, j);
     }
-    pf.v[i] = x;
   }
 }
 
@@ -253,35 +225,39 @@ __device__ __forceinline__ void fetch(const Geo& g, const GridArgs& a, Prefetch<
 template <int MH, int RL>
 __device__ __forceinline__ void inverse(Ctx& c, const Geo& g, const GridArgs& a,
                                         const Prefetch<RL>& pf, double2 (&z)[RL]) {
-  const int RB = g.RB, n = g.M * RB, LS = g.LS;
+  const int RB = g.RB, LS = g.LS;
   int w = 1 - c.busy;
   {
-    double2* raw = lbuf(g, w);
+    // packed spectrum of the real inverse FFT, written by conjugate pairs:
+    // Z_k = E + iO, E = X_k + conj(X_{Mh-k}), O = (X_k - conj(X_{Mh-k})) e^{+2 pi i k/n},
+    // and Z_{Mh-k} = conj(E) + i conj(O) (X_0 is real: its imaginary part is dropped)
+    double2* zb = lbuf(g, w);
+    const double2* tw2 = ltw2(g);
+    const int n = (g.Mh / 2 + 1) * RB;
 #pragma unroll
-    for (int i = 0; i < RL; ++i) {
+    for (int i = 0; i < Prefetch<RL>::PP; ++i) {
       const int e = threadIdx.x + i * g.threads;
       if (e < n) {
-        const int m = e / RB, j = e - m * RB;
-        raw[j * LS + m] = pf.v[i];
+        const int p = e / RB, j = e - p * RB;
+        double2 x = pf.x[i];
+        if (p == 0) x.y = 0.0;
+        const double2 xc = (2 * p == g.Mh) ? cconj(x) : cconj(pf.xc[i]);
+        const double2 E = cadd(x, xc);
+        const double2 O = cmul(csub(x, xc), cconj(tw2[p]));
+        double2* row = zb + j * LS;
+        row[p] = make_double2(E.x - O.y, E.y + O.x);
+        if (p > 0 && 2 * p < g.Mh) row[g.Mh - p] = make_double2(E.x + O.y, -E.y + O.x);
       }
     }
   }
   lds_barrier();
   if (g.nst > 1) {
     if constexpr (MH > 0) {
-      constexpr GPlan P = make_plan(MH);
-      if constexpr (P.nst > 1) {
-        stage_first_inv<P.radix[0]>(g, a, w, 1 - w);
-        lds_barrier();
-        w = inv_mid_ct<MH, 1, P.radix[0]>(g, a, 1 - w);
-      }
+      w = inv_mid_ct<MH, 0, 1>(g, a, w);
     } else {
       const GPlan& P = *c.rp;
-      stage_first_inv_rt(g, a, P.radix[0], w, 1 - w);
-      int Ns = P.radix[0];
-      w = 1 - w;
-      lds_barrier();
-      for (int s = 1; s < P.nst - 1; ++s) {
+      int Ns = 1;
+      for (int s = 0; s < P.nst - 1; ++s) {
         stage_mid_rt(g, a, P.radix[s], w, 1 - w, Ns, 1.0);
         Ns *= P.radix[s];
         w = 1 - w;
@@ -294,13 +270,15 @@ __device__ __forceinline__ void inverse(Ctx& c, const Geo& g, const GridArgs& a,
     const double2* src = lbuf(g, w) + c.row * LS;
     const double2* tw = ltw(g);
 #pragma unroll
-    for (int i = 0; i < RL; ++i) {
-      const int k = c.q + i * g.TP;
-      z[i] = (g.nst > 1) ? src[k] : zload(g, a, src, k);
-    }
+    for (int i = 0; i < RL; ++i) z[i] = src[c.q + i * g.TP];
     if (g.nst > 1) {
+      const double2 w1 = cconj(tw[c.q]);
+      double2 wi = w1;
 #pragma unroll
-      for (int i = 1; i < RL; ++i) z[i] = cmul(z[i], cconj(tw[i * c.q]));
+      for (int i = 1; i < RL; ++i) {
+        z[i] = cmul(z[i], wi);
+        if (i + 1 < RL) wi = cmul(wi, w1);
+      }
     }
     dft<RL>(z, 1.0);
   } else {
@@ -338,23 +316,31 @@ __device__ __forceinline__ void forward(Ctx& c, const Geo& g, const GridArgs& a,
       lds_barrier();
     }
   }
-  // unpack X_m = E_m + e^{-2 pi i m/n} O_m and store the bulk levels
-  const int RB = g.RB, Mh = g.Mh, N = a.N, No = N - 2, n = g.M * RB;
+  // unpack by conjugate pairs: X_m = E + e^{-2 pi i m/n} O and
+  // X_{Mh-m} = conj(E - e^{-2 pi i m/n} O), E = (Z_m + conj Z_{Mh-m})/2,
+  // O = (Z_m - conj Z_{Mh-m})/(2i); store the orders < M of the bulk levels
+  const int RB = g.RB, Mh = g.Mh, N = a.N, No = N - 2, n = (Mh / 2 + 1) * RB;
   const double2* tw2 = ltw2(g);
   double* Fo = a.Fo + (long)comp * g.M * a.T * No * 2;
   const double2* res = lbuf(g, w);
   for (int e = threadIdx.x; e < n; e += g.threads) {
-    const int m = e / RB, j = e - m * RB;
+    const int p = e / RB, j = e - p * RB;
     const int r = r0 + j;
     if (r < 1 || r > N - 2) continue;
     const double2* src = res + j * LS;
-    const double2 Z = src[m];
-    const double2 Zc = cconj(src[m == 0 ? 0 : Mh - m]);
+    const int pc = (p == 0) ? 0 : Mh - p;
+    const double2 Z = src[p];
+    const double2 Zc = cconj(src[pc]);
     const double2 E = cscale(cadd(Z, Zc), 0.5);
     const double2 D = csub(Z, Zc);
-    const double2 O = make_double2(0.5 * D.y, -0.5 * D.x);  // (Z - Zc) / (2i)
-    const double2 X = cadd(E, cmul(tw2[m], O));
-    *reinterpret_cast<double2*>(Fo + (((long)m * a.T + t) * No + r - 1) * 2) = cscale(X, scale);
+    const double2 O = make_double2(0.5 * D.y, -0.5 * D.x);
+    const double2 Tm = cmul(tw2[p], O);
+    if (p < g.M)
+      *reinterpret_cast<double2*>(Fo + (((long)p * a.T + t) * No + r - 1) * 2) =
+          cscale(cadd(E, Tm), scale);
+    if (p > 0 && 2 * p < Mh && pc < g.M)
+      *reinterpret_cast<double2*>(Fo + (((long)pc * a.T + t) * No + r - 1) * 2) =
+          cscale(cconj(csub(E, Tm)), scale);
   }
   c.busy = w;
 }
@@ -580,8 +566,9 @@ extern "C" int mh_grid_step(const MhGridArgs* args, void* stream) {
   if (a.N < 3 || a.T < 1 || a.Mh < 2 || a.M < 1 || a.M > a.Mh) return MH_EINVAL;
   const GPlan P = make_plan(a.Mh);
   if (P.RL == 0) return MH_EINVAL;
-  // every thread covers its share of a component's M*RB raw values in RL loads
-  if ((long)P.threads * P.RL < (long)a.M * P.RB) return MH_EINVAL;
+  // every thread covers its share of the (Mh/2 + 1) * RB conjugate-pair items
+  if (a.Mh % 2 != 0 || (long)P.threads * ((P.RL + 2) / 2) < (long)(a.Mh / 2 + 1) * P.RB)
+    return MH_EINVAL;
   const size_t lds = (2ull * P.RB * (a.Mh + 1) + 2ull * a.Mh) * sizeof(double2);
   const int nRB = (a.N + P.RB - 1) / P.RB;
   const long nwg = (long)a.T * nRB;
