@@ -131,19 +131,21 @@ __device__ __forceinline__ SynIn syn_in(const SynArgs& a, int g, int c) {
 // Columns of the groups are laid side by side (column bc is batch column
 // bc % B of group bc / B, which is also its column in F).  The input tile of
 // 32 degrees x 16 real columns x (1 or 3) fields is staged in LDS with the
-// l(l+1) / or2 prefactors applied once per element (double-buffered: the next
-// chunk is loaded into registers while the current one feeds the MFMAs); wave
-// w takes theta pairs j0 + 16w and reads the table operand from global memory.
-constexpr int kSynWaves = 6, kSynJ = 16 * kSynWaves, kChunk = 32;
+// l(l+1) / or2 prefactors applied once per element, double-buffered: the loads
+// of the next chunk are issued before the current chunk's MFMAs and combined
+// and stored after them.  Wave w takes theta pairs j0 + 16w and streams its
+// table operands from global memory through a ring of kSynDepth steps.
+constexpr int kSynWaves = 6, kSynJ = 16 * kSynWaves, kChunk = 32, kSynDepth = 4;
 
 template <bool VEC, typename T>
-__global__ __launch_bounds__(64 * kSynWaves, 2) void k_syn(SynArgs a, int n_m) {
+__global__ __launch_bounds__(64 * kSynWaves, 3) void k_syn(SynArgs a, int n_m) {
   typedef Mfma<T> MF;
   typedef typename MF::acc_t acc_t;
   constexpr int NF = VEC ? 3 : 1;           // fields per group (Q, S, T)
   constexpr int NT = 64 * kSynWaves;        // threads
-  constexpr int NV = NF * kChunk * 16;      // staged values per chunk
-  constexpr int PER = (NV + NT - 1) / NT;   // staged values per thread
+  constexpr int SPC = kChunk / 4;           // MFMA steps per chunk
+  constexpr int D = kSynDepth;
+  static_assert(NT % 16 == 0 && SPC % D == 0, "staging / ring layout");
   __shared__ T xs[2][NF][kChunk][16];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   // XCD-aware order: workgroups b and b+8 run on the same XCD; XCD x takes the
@@ -159,11 +161,35 @@ __global__ __launch_bounds__(64 * kSynWaves, 2) void k_syn(SynArgs a, int n_m) {
   const int col = lane & 15, kk = lane >> 4;
   const long lms = a.lm_start[m];
   const int ncol = a.ngroups * a.B;
+  // staging map: thread stages real column c16 = tid % 16 of chunk rows
+  // tid / 16 (slot 0) and 24 + tid / 16 (slot 1, tid < 128) of every field
+  static_assert(NT == 384 && kChunk == 32, "staging map assumes 384 threads, 32-row chunks");
+  const int c16 = threadIdx.x & 15, srow = threadIdx.x >> 4;
+  const bool s2 = threadIdx.x < 128;
+  const int sbc = bc0 + (c16 >> 1);
+  const bool scol = sbc < ncol;
+  const int sg = scol ? sbc / a.B : 0;
+  const int sb = scol ? sbc - sg * a.B : 0;
+  const long soff = 2L * sb + (c16 & 1);
+  const double sor2 = scol ? a.or2[sb] : 0.0;
+  // this thread's input pointers (column offset folded in) and modes, resolved
+  // once: per-lane selection inside the chunk loop would re-read the argument
+  // block through memory on every chunk
+  const double* sx1[NF];
+  const double* sx2 = nullptr;
+  int smodes = 0;  // 2 bits per field
+#pragma unroll
+  for (int f = 0; f < NF; ++f) {
+    const SynIn in = syn_in(a, sg, f);
+    sx1[f] = in.x1 + soff;
+    if (f == 2) sx2 = in.x2 + soff;
+    smodes |= in.mode << (2 * f);
+  }
   acc_t acc[VEC ? 10 : 2];
 #pragma unroll
   for (int q = 0; q < (VEC ? 10 : 2); ++q) acc[q] = (acc_t){0, 0, 0, 0};
-  // staging map of this thread: value v = threadIdx.x + NT*i -> (field, k, real column)
-  T stg[PER];
+  double r1[NF][2], r2[2];  // raw inputs of the next chunk (r2: second input of T)
+  const bool wact = j0 < a.H;  // uniform per wave; then j0 + 15 < Hp
 #pragma unroll  // static accumulator indices (acc[5p + q] stays in registers)
   for (int p = 0; p < 2; ++p) {
     const int* info = a.mp_info + (m * 2 + p) * 4;
@@ -173,57 +199,67 @@ __global__ __launch_bounds__(64 * kSynWaves, 2) void k_syn(SynArgs a, int n_m) {
     const T* tD = reinterpret_cast<const T*>(a.synD) + info[1];
     auto fetch = [&](int c0) {
 #pragma unroll
-      for (int i = 0; i < PER; ++i) {
-        const int v = threadIdx.x + NT * i;
-        T x = (T)0;
-        if (v < NV) {
-          const int f = v / (kChunk * 16), rem = v - f * kChunk * 16;
-          const int k = c0 + rem / 16, c16 = rem & 15;
-          const int bc = bc0 + (c16 >> 1);
-          if (k < K && bc < ncol) {
-            const int g = bc / a.B, b = bc - g * a.B;
-            x = (T)syn_load(syn_in(a, g, f), lms + p + 2 * k, m + p + 2 * k, b, c16 & 1, a.ldx,
-                            a.or2);
+      for (int f = 0; f < NF; ++f) {
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+          const int k = c0 + 24 * q + srow;
+          r1[f][q] = 0.0;
+          if (f == 2) r2[q] = 0.0;
+          if ((q == 0 || s2) && scol && k < K) {
+            const long o = (lms + p + 2 * k) * 2 * (long)a.ldx;
+            r1[f][q] = sx1[f][o];
+            if (f == 2 && ((smodes >> 4) & 3) == 2) r2[q] = sx2[o];
           }
         }
-        stg[i] = x;
       }
     };
-    auto store = [&](int buf) {
+    auto store = [&](int buf, int c0) {
 #pragma unroll
-      for (int i = 0; i < PER; ++i) {
-        const int v = threadIdx.x + NT * i;
-        if (v < NV) {
-          const int f = v / (kChunk * 16), rem = v - f * kChunk * 16;
-          xs[buf][f][rem / 16][rem & 15] = stg[i];
+      for (int f = 0; f < NF; ++f) {
+        const int mode = (smodes >> (2 * f)) & 3;
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+          if (q == 0 || s2) {
+            const int kr = 24 * q + srow;
+            const int l = m + p + 2 * (c0 + kr);
+            const double dl = (double)l * (double)(l + 1);
+            const double x1 = r1[f][q], x2 = (f == 2) ? r2[q] : 0.0;
+            const double x = mode == 0 ? x1 : mode == 1 ? dl * x1 : -x1 + sor2 * dl * x2;
+            xs[buf][f][kr][c16] = (T)x;
+          }
         }
       }
     };
     fetch(0);
     __syncthreads();  // previous parity's readers are done with both buffers
-    store(0);
+    store(0, 0);
     __syncthreads();
     // table rows are padded to a multiple of 4; waves past the last theta pair
     // only help with the staging
-    const int nsteps = (j0 < a.H) ? (K + 3) / 4 : 0;
-    const bool jv = j0 + col < a.Hp;
-    // table operands of step t (degrees 4t .. 4t+3), loaded one step ahead
-    auto tload = [&](int t, T& p_, T& d_) {
-      const long to = (long)(4 * t + kk) * a.Hp + j0 + col;
-      p_ = jv ? tP[to] : (T)0;
-      d_ = (VEC && jv) ? tD[to] : (T)0;
-    };
-    T aP = (T)0, aD = (T)0;
-    if (nsteps > 0) tload(0, aP, aD);
+    const int nsteps = wact ? (K + 3) / 4 : 0;
+    const T* pP = tP + (long)kk * a.Hp + j0 + col;
+    const T* pD = tD + (long)kk * a.Hp + j0 + col;
+    const long tstep = 4L * a.Hp;
+    T rP[D], rD[D];  // table operands of steps t .. t+D-1 (slot t % D)
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      rP[d] = (T)0;
+      rD[d] = (T)0;
+      if (d < nsteps) {
+        rP[d] = pP[d * tstep];
+        if (VEC) rD[d] = pD[d * tstep];
+      }
+    }
     const int nch = (K + kChunk - 1) / kChunk;
     for (int ch = 0; ch < nch; ++ch) {
       const int buf = ch & 1;
-      if (ch + 1 < nch) fetch((ch + 1) * kChunk);
-      const int t0 = ch * (kChunk / 4), t1 = min(nsteps, t0 + kChunk / 4);
-      for (int t = t0; t < t1; ++t) {
-        T nP = aP, nD = aD;
-        if (t + 1 < nsteps) tload(t + 1, nP, nD);
-        const int kr = 4 * (t - t0) + kk;
+      const bool more = ch + 1 < nch;
+      // one MFMA step: degrees 4t .. 4t+3 against LDS rows 4s .. 4s+3; the
+      // slot is refilled with step t + D after its last use, straight from
+      // global memory into the ring register (row clamped to the table)
+      auto step = [&](const int s, const int t) {
+        const T aP = rP[s % D], aD = rD[s % D];
+        const int kr = 4 * s + kk;
         const T xq = xs[buf][0][kr][col];
         if (!VEC) {
           acc[p] = MF::run(aP, xq, acc[p]);
@@ -236,13 +272,28 @@ __global__ __launch_bounds__(64 * kSynWaves, 2) void k_syn(SynArgs a, int n_m) {
           acc[5 * p + 3] = MF::run(aD, xsv, acc[5 * p + 3]);  // dP S
           acc[5 * p + 4] = MF::run(aD, xt, acc[5 * p + 4]);   // dP T
         }
-        aP = nP;
-        aD = nD;
+        const long tn = (long)min(t + D, nsteps - 1) * tstep;
+        rP[s % D] = pP[tn];
+        if (VEC) rD[s % D] = pD[tn];
+      };
+      // the next chunk's inputs are requested after the first step, so that
+      // step's operands are not queued behind them
+      // (the staging store is repeated in both branches so that its wait
+      // covers only the staged loads, not the table refills issued after them)
+      if ((ch + 1) * SPC <= nsteps) {  // full chunk (uniform per wave): straight-line steps
+        step(0, ch * SPC);
+        if (more) fetch((ch + 1) * kChunk);
+#pragma unroll
+        for (int s = 1; s < SPC; ++s) step(s, ch * SPC + s);
+        if (more) store(buf ^ 1, (ch + 1) * kChunk);
+      } else {  // last chunk of an active wave, or an idle wave
+        if (more) fetch((ch + 1) * kChunk);
+#pragma unroll
+        for (int s = 0; s < SPC; ++s)
+          if (ch * SPC + s < nsteps) step(s, ch * SPC + s);
+        if (more) store(buf ^ 1, (ch + 1) * kChunk);
       }
-      if (ch + 1 < nch) {
-        store(buf ^ 1);
-        __syncthreads();
-      }
+      if (more) __syncthreads();
     }
   }
   const double dm = (double)m;
@@ -287,6 +338,11 @@ extern "C" int mh_leg_synth(int vector, const MhSynArgs* args, int n_m, void* st
       a.Bout != a.ngroups * a.B)
     return MH_EINVAL;
   if (a.prec != 0 && a.prec != 1) return MH_EINVAL;
+  // mode 2 (two inputs) is supported for the T field of a vector group only
+  for (int g = 0; g < a.ngroups; ++g)
+    for (int f = 0; f < (vector ? 3 : 1); ++f)
+      if (a.in[g][f].mode < 0 || a.in[g][f].mode > 2 || (a.in[g][f].mode == 2 && !(vector && f == 2)))
+        return MH_EINVAL;
   const long units = (long)mh_cdiv(a.ngroups * a.B, 8) * mh_cdiv(a.H, kSynJ) * mh_cdiv(n_m, 8) * 8;
   if (units > 0x7fffffffL) return MH_EINVAL;
   dim3 grid((unsigned)units);
@@ -320,8 +376,6 @@ __global__ __launch_bounds__(64 * kAnaWaves, 2) void k_ana(AnaArgs a, int n_m) {
   typedef typename MF::acc_t acc_t;
   constexpr int NF = VEC ? 4 : 1;          // f1 same, f1 opposite, f2 same, f2 opposite
   constexpr int NT = 64 * kAnaWaves;
-  constexpr int NP = kChunk * 16;          // (pair, real column) slots per chunk
-  constexpr int PER = (NP + NT - 1) / NT;
   __shared__ T fs[2][NF][kChunk][16];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   // XCD-aware order: the tile list holds 8 equal-length lists, list x with the
@@ -346,87 +400,111 @@ __global__ __launch_bounds__(64 * kAnaWaves, 2) void k_ana(AnaArgs a, int n_m) {
   acc_t acc[VEC ? 4 : 1];
 #pragma unroll
   for (int q = 0; q < (VEC ? 4 : 1); ++q) acc[q] = (acc_t){0, 0, 0, 0};
-  T stg[PER][NF];
+  // staging map: thread stages real column c16 = tid % 16 of theta pairs
+  // tid / 16 (slot 0) and 24 + tid / 16 (slot 1, tid < 128) of the chunk; the
+  // raw N/S values are loaded before the chunk's MFMAs and folded after them
+  static_assert(NT == 384 && kChunk == 32, "staging map assumes 384 threads, 32-row chunks");
+  const int c16 = threadIdx.x & 15, srow = threadIdx.x >> 4;
+  const bool s2 = threadIdx.x < 128;
+  const int sb = bc0 + (c16 >> 1);
+  const bool scol = sb < a.B;
+  const double* f1p = a.F + c1 * a.c_stride + fidx(0, a.m_stride, a.Bin, 0, m, 0, scol ? sb : 0) + (c16 & 1);
+  const double* f2p = a.F + c2 * a.c_stride + fidx(0, a.m_stride, a.Bin, 0, m, 0, scol ? sb : 0) + (c16 & 1);
+  const long rstride = 2L * a.Bin;  // one grid row t
+  double raw[2][VEC ? 4 : 2];       // (f1N, f1S, f2N, f2S) of the next chunk
   auto fetch = [&](int j0) {
 #pragma unroll
-    for (int i = 0; i < PER; ++i) {
-      const int v = threadIdx.x + NT * i;
+    for (int q = 0; q < 2; ++q) {
+      const int j = j0 + 24 * q + srow;
 #pragma unroll
-      for (int f = 0; f < NF; ++f) stg[i][f] = (T)0;
-      if (v < NP) {
-        const int j = j0 + v / 16, c16 = v & 15;
-        const int b = bc0 + (c16 >> 1);
-        if (j < a.H && b < a.B) {
-          const long iN = fidx(a.c_stride, a.m_stride, a.Bin, 0, m, 2 * j, b) + (c16 & 1);
-          const long iS = iN + 2L * a.Bin;  // row 2j+1
-          const double f1N = a.F[c1 * a.c_stride + iN], f1S = a.F[c1 * a.c_stride + iS];
-          stg[i][0] = (T)(f1N + sg * f1S);
-          if (VEC) {
-            const double f2N = a.F[c2 * a.c_stride + iN], f2S = a.F[c2 * a.c_stride + iS];
-            stg[i][1] = (T)(f1N - sg * f1S);
-            stg[i][2] = (T)(f2N + sg * f2S);
-            stg[i][3] = (T)(f2N - sg * f2S);
-          }
+      for (int f = 0; f < (VEC ? 4 : 2); ++f) raw[q][f] = 0.0;
+      if ((q == 0 || s2) && scol && j < a.H) {
+        const long iN = 2L * j * rstride, iS = iN + rstride;
+        raw[q][0] = f1p[iN];
+        raw[q][1] = f1p[iS];
+        if (VEC) {
+          raw[q][2] = f2p[iN];
+          raw[q][3] = f2p[iS];
         }
       }
     }
   };
   auto store = [&](int buf) {
 #pragma unroll
-    for (int i = 0; i < PER; ++i) {
-      const int v = threadIdx.x + NT * i;
-      if (v < NP) {
-#pragma unroll
-        for (int f = 0; f < NF; ++f) fs[buf][f][v / 16][v & 15] = stg[i][f];
+    for (int q = 0; q < 2; ++q) {
+      if (q == 0 || s2) {
+        const int jr = 24 * q + srow;
+        fs[buf][0][jr][c16] = (T)(raw[q][0] + sg * raw[q][1]);
+        if (VEC) {
+          fs[buf][1][jr][c16] = (T)(raw[q][0] - sg * raw[q][1]);
+          fs[buf][2][jr][c16] = (T)(raw[q][2] + sg * raw[q][3]);
+          fs[buf][3][jr][c16] = (T)(raw[q][2] - sg * raw[q][3]);
+        }
       }
     }
   };
-  const bool active = kbase < K;  // uniform per wave
-  const int arow = kbase + col;
-  // weight-table operands of theta step t (pairs 4t .. 4t+3), loaded one step
-  // ahead; the tables hold Hp >= roundup(H, 4) rows
+  const bool active = kbase < K;  // uniform per wave; then kbase + 15 < K16
+  // weight-table operands of theta step t (pairs 4t .. 4t+3) stream through a
+  // ring of D steps; the tables hold Hp >= roundup(H, 4) rows
+  constexpr int SPC = kChunk / 4, D = kSynDepth;
   const int nsteps = (a.H + 3) / 4;
-  auto tload = [&](int t, T& w_, T& d_) {
-    const long to = (long)(4 * t + kk) * K16 + arow;
-    w_ = tW[to];
-    d_ = VEC ? tD[to] : (T)0;
-  };
-  T aw = (T)0, ad = (T)0;
-  if (active) tload(0, aw, ad);
+  const T* pW = tW + (long)kk * K16 + kbase + col;
+  const T* pD = tD + (long)kk * K16 + kbase + col;
+  const long tstep = 4L * K16;
+  T rW[D], rD[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    rW[d] = (T)0;
+    rD[d] = (T)0;
+    if (active && d < nsteps) {
+      rW[d] = pW[d * tstep];
+      if (VEC) rD[d] = pD[d * tstep];
+    }
+  }
   fetch(0);
   store(0);
   __syncthreads();
   const int nch = (a.H + kChunk - 1) / kChunk;
   for (int ch = 0; ch < nch; ++ch) {
-    const int buf = ch & 1, j0 = ch * kChunk;
-    if (ch + 1 < nch) fetch(j0 + kChunk);
-    if (active) {
-      const int t0 = ch * (kChunk / 4), t1 = min(nsteps, t0 + kChunk / 4);
-      for (int t = t0; t < t1; ++t) {
-        T nw = aw, nd = ad;
-        if (t + 1 < nsteps) tload(t + 1, nw, nd);
-        const int jr = 4 * (t - t0) + kk;
-        const T x1 = fs[buf][0][jr][col];
-        if (!VEC) {
-          acc[0] = MF::run(aw, x1, acc[0]);
-        } else {
-          const T x2 = fs[buf][1][jr][col];
-          const T x3 = fs[buf][2][jr][col];
-          const T x4 = fs[buf][3][jr][col];
-          acc[0] = MF::run(ad, x4, acc[0]);  // wdP f2 (opposite fold)
-          acc[1] = MF::run(ad, x2, acc[1]);  // wdP f1 (opposite fold)
-          acc[2] = MF::run(aw, x1, acc[2]);  // wP f1 (same fold)
-          acc[3] = MF::run(aw, x3, acc[3]);  // wP f2 (same fold)
-        }
-        aw = nw;
-        ad = nd;
+    const int buf = ch & 1;
+    const bool more = ch + 1 < nch;
+    auto step = [&](const int s, const int t) {
+      const T aw = rW[s % D], ad = rD[s % D];
+      const int jr = 4 * s + kk;
+      const T x1 = fs[buf][0][jr][col];
+      if (!VEC) {
+        acc[0] = MF::run(aw, x1, acc[0]);
+      } else {
+        const T x2 = fs[buf][1][jr][col];
+        const T x3 = fs[buf][2][jr][col];
+        const T x4 = fs[buf][3][jr][col];
+        acc[0] = MF::run(ad, x4, acc[0]);  // wdP f2 (opposite fold)
+        acc[1] = MF::run(ad, x2, acc[1]);  // wdP f1 (opposite fold)
+        acc[2] = MF::run(aw, x1, acc[2]);  // wP f1 (same fold)
+        acc[3] = MF::run(aw, x3, acc[3]);  // wP f2 (same fold)
       }
+      const long tn = (long)min(t + D, nsteps - 1) * tstep;
+      rW[s % D] = pW[tn];
+      if (VEC) rD[s % D] = pD[tn];
+    };
+    // buf ^ 1 was last read in chunk ch - 1, before the previous barrier; the
+    // store is repeated per branch so its wait covers the staged loads only
+    if (active && (ch + 1) * SPC <= nsteps) {  // full chunk: straight-line steps
+      step(0, ch * SPC);
+      if (more) fetch((ch + 1) * kChunk);
+#pragma unroll
+      for (int s = 1; s < SPC; ++s) step(s, ch * SPC + s);
+      if (more) store(buf ^ 1);
+    } else {
+      if (more) fetch((ch + 1) * kChunk);
+      if (active) {
+#pragma unroll
+        for (int s = 0; s < SPC; ++s)
+          if (ch * SPC + s < nsteps) step(s, ch * SPC + s);
+      }
+      if (more) store(buf ^ 1);
     }
-    if (ch + 1 < nch) {
-      // buf ^ 1 was last read in chunk ch - 1, before the previous barrier
-      store(buf ^ 1);
-      __syncthreads();
-    }
+    if (more) __syncthreads();
   }
   if (!active) return;
   const double dm = (double)m;
