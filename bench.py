@@ -168,6 +168,22 @@ def fd_banded_case():
                 info_ok=all(int(ops.dev[k]["info"].abs().sum()) == 0 for k in ("S", "Z", "W")))
 
 
+def ensemble_case(n_seq):
+    """Config 5: 64 Boussinesq runs (l_max in {32, 64, 85, 127}, n_r in {33, 49, 65},
+    E log-uniform [1e-4, 1e-3], Ra_mod uniform [60, 200], 500 steps each) arriving as a
+    seeded Poisson process at 2/s, served with continuous admission (16 slots, HBM
+    budget); baseline: the first n_seq of the same requests run back to back, one at
+    a time, in plain FP64 (arrivals ignored)."""
+    from magic_amd.serve import EnsembleServer, config5_requests, serve_sequential
+    srv = EnsembleServer(max_slots=16)
+    _, mix = srv.serve(config5_requests(), realtime=True)
+    _, seq = serve_sequential(config5_requests()[:n_seq])
+    return dict(workload="64 runs x 500 steps, Poisson arrivals 2/s, l_max {32,64,85,127}, "
+                         "n_r {33,49,65}, FP64",
+                server=dict(mix, max_active=max(n for _, n in srv.rounds)),
+                sequential=dict(seq, sample=f"first {n_seq} of the 64 requests back to back"))
+
+
 def roofline(res, precision):
     """Roofline of the dominant kernel section (largest total time)."""
     secs, work = res["secs"], section_work(res["p"], res["N"])
@@ -193,6 +209,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fd", action="store_true", help="skip the config-3 banded-solve timing")
     ap.add_argument("--cpu-levels", type=int, default=9)
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the config-5 ensemble")
+    ap.add_argument("--ensemble-seq", type=int, default=16,
+                    help="requests in the sequential ensemble baseline sample")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only; --gpus must be 1", file=sys.stderr)
@@ -200,7 +219,11 @@ def main():
 
     import numpy as np
     precs = ("fp64", "mixed") if args.precision == "both" else (args.precision,)
-    res = {pr: run_case(pr, args.steps, args.warmup) for pr in precs}
+    res = {}
+    for pr in precs:
+        res[pr] = run_case(pr, args.steps, args.warmup)
+        print(f"bench: {pr} {args.steps} steps in {res[pr]['elapsed']:.2f} s", file=sys.stderr,
+              flush=True)
     head = "mixed" if "mixed" in res else "fp64"
     h = res[head]
     sps = {pr: args.steps / r["elapsed"] for pr, r in res.items()}
@@ -223,6 +246,10 @@ def main():
         cfg["energy_rel_diff_mixed_vs_fp64"] = float(np.max(np.abs(emx - e64) / np.abs(e64)))
     if not args.no_fd:
         cfg["fd_banded"] = fd_banded_case()
+    if not args.no_ensemble:
+        cfg["ensemble"] = ensemble_case(args.ensemble_seq)
+        print(f"bench: ensemble wall {cfg['ensemble']['server']['wall_s']:.1f} s", file=sys.stderr,
+              flush=True)
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(args.cpu_levels)

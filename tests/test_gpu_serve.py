@@ -43,3 +43,29 @@ def test_admission_respects_slots_and_budget():
     done, summary = srv.serve(reqs)
     assert summary["runs"] == 4
     assert max(n for _, n in srv.rounds) == 1  # only one run fits the budget at a time
+
+
+def test_runs_of_one_resolution_overlapping_match_alone():
+    """Three runs of the same resolution (l_max=32, n_r=33; parameters from the
+    ensemble workload) admitted together, so their kernels share the compute
+    units: each must end bit-identical to the same run alone."""
+    import torch
+    from magic_amd.serve import EnsembleServer, Request, serve_sequential
+    base = dict(l_max=32, n_r_max=33, pr=1.0, prmag=5.0, radratio=0.35, dtmax=1e-4)
+    cfgs = [dict(base, ek=1.92e-4, ra=8.43e5), dict(base, ek=3.01e-4, ra=5.37e5),
+            dict(base, ek=1.82e-4, ra=8.58e5)]
+
+    def reqs():
+        return [Request(i, c, n_steps=60, arrival=0.0, noise_amp=1e-5, noise_seed=1006 + i)
+                for i, c in enumerate(cfgs)]
+
+    srv = EnsembleServer(max_slots=3)
+    mix, _ = srv.serve(reqs(), realtime=False, keep_fields=True)
+    alone, _ = serve_sequential(reqs(), keep_fields=True)
+    assert max(n for _, n in srv.rounds) == 3
+    by_id = {r.rid: r for r in alone}
+    for r in mix:
+        a = by_id[r.rid]
+        assert r.energies == a.energies, r.rid
+        for k, v in r.final_fields.items():
+            assert torch.equal(v, a.final_fields[k]), (r.rid, k)

@@ -126,20 +126,23 @@ def _rel_errs(out, ref):
     return {k: float(np.abs(out[k] - ref[k]).max() / scale[_VECTOR_OF[k]]) for k in ref}
 
 
-def test_grid_step_matches_oracle_on_random_fields():
+@pytest.mark.parametrize("l_max", [63, 85])
+def test_grid_step_matches_oracle_on_random_fields(l_max):
     """All three grid-space paths (fused with compiled FFT plan, fused with the
     run-time plan, separate kernels) against the oracle's radial loop
     (irfft / get_nl / rfft of step_time.py:931-1212) on O(1) random spectral
-    fields at l_max=63 (n_phi/2 = 96), through the same Legendre transforms."""
+    fields, through the same Legendre transforms.  l_max=63 gives n_phi/2 = 96
+    (radix-3 register pass, compiled plan); l_max=85 gives n_phi/2 = 128
+    (radix-4 register pass, run-time plan only)."""
     from oracle.dynamo import OracleRun
-    orun = OracleRun(dict(_GRID_CFG, l_max=63, n_r_max=9))
+    orun = OracleRun(dict(_GRID_CFG, l_max=l_max, n_r_max=9))
     fields = _random_fields((orun.prm.lm_max, 9), 11)
     for k, a in fields.items():
         setattr(orun, k, a.copy())
     ocour = np.array(orun.radial_loop())
     ref = {k: orun.last_spec[v] for k, v in _SPEC.items()}
     for mode in ("fused", "generic", "separate"):
-        out, cour = _radial_loop_outputs(63, mode)
+        out, cour = _radial_loop_outputs(l_max, mode)
         errs = _rel_errs(out, ref)
         assert max(errs.values()) < 1e-11, (mode, errs)
         assert np.allclose(cour, ocour, rtol=1e-12, atol=0), (mode, cour, ocour)

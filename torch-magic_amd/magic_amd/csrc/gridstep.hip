@@ -369,7 +369,7 @@ __global__ __launch_bounds__(kMaxThreads) void k_grid_step(GridArgs a, GPlan rp)
   c.act = c.row < nvalid;
   c.busy = 0;
   if (threadIdx.x < 2 * RB) red[threadIdx.x] = 0ull;
-  {  // twiddles into LDS (visible after the first barrier)
+  {  // twiddles into LDS
     double2* tl = g_lds + 2 * RB * g.LS;
     const double2* t1 = reinterpret_cast<const double2*>(a.tw);
     const double2* t2 = reinterpret_cast<const double2*>(a.tw2);
@@ -378,6 +378,10 @@ __global__ __launch_bounds__(kMaxThreads) void k_grid_step(GridArgs a, GPlan rp)
       tl[g.Mh + k] = t2[k];
     }
   }
+  // the first inverse reads tw2 from LDS while it writes its packed spectrum,
+  // before its own first barrier: every twiddle (and the zeroed maxima) must
+  // be in place first
+  __syncthreads();
 
   const int r = r0 + (c.act ? c.row : 0);
   const double ost2 = a.ost2[t];

@@ -87,6 +87,9 @@ class Resolution:
         self.fft_rb = max(1, min(16, 2048 // self.Mh))
         self.d["comp_scale_v"] = up(np.array([0, 0, 0, 1, 1, 1, 1, 1, 1]), I32)
         self.d["comp_scale_2"] = up(np.array([1, 1]), I32)
+        # the tables are shared by every run of this resolution, each on its own
+        # stream: make them complete before any other stream can read them
+        torch.cuda.current_stream(dev).synchronize()
 
     def f32_tables(self):
         """FP32 copies of the four Legendre tables (built on first use)."""
@@ -97,6 +100,7 @@ class Resolution:
                 dst = torch.empty(src.numel(), dtype=torch.float32, device=self.device)
                 _lib.call("mh_tables_to_f32", ptr(src), ptr(dst), src.numel(), stream_handle())
                 t[k] = dst
+            torch.cuda.current_stream(self.device).synchronize()  # shared across streams
             self._t32 = t
         return self._t32
 

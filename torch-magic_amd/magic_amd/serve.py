@@ -128,7 +128,7 @@ class EnsembleServer:
 
     def _resolution(self, p):
         from .device import Resolution
-        key = (p.l_max, p.n_r_max, p.radratio, p.n_cheb_max)
+        key = (p.l_max, p.n_r_max, p.radratio, p.n_cheb_max, p.opm)  # opm: Alfven Courant factors
         r = self._res_cache.get(key)
         if r is None:
             r = self._res_cache[key] = Resolution(p, self.device)
@@ -195,7 +195,11 @@ class EnsembleServer:
                 for sl in list(waiting):
                     if sl.run.cfl_ready():
                         with torch.cuda.stream(sl.stream):
-                            sl.dt = sl.run.finish_step(sl.n + 1, sl.dt)
+                            try:
+                                sl.dt = sl.run.finish_step(sl.n + 1, sl.dt)
+                            except AssertionError as e:
+                                raise AssertionError(f"request {sl.req.rid} step {sl.n + 1} "
+                                                     f"(dt {sl.dt:.6g}): {e}") from e
                         sl.n += 1
                         waiting.remove(sl)
                 if waiting:
