@@ -9,17 +9,21 @@ Synthetic data: fields initialised on the device from the seed, no I/O.
 
 A "step" is one full ``one_step`` (radial loop with the SHTs, nonlinear
 terms and CFL, dt decision, operator rebuild if dt changed, implicit solves).
-W warm-up steps are run untimed, then exactly K steps are timed between two
-device synchronisations, first in plain FP64, then in mixed precision
-(Legendre transforms on FP32 MFMA, everything else FP64), each from the same
-seeded initial state.  The headline value is the mixed-precision steps/s;
-the FP64 rate, their ratio and the largest relative energy difference after
-W+K steps are in ``config``.  Per-section HIP-event timings from the timed
-region give the roofline of the dominant kernel.  The CPU baseline is the
-FP64 numpy oracle on a bounded sample of the same workload.
+W warm-up steps are run untimed, then exactly K steps (default 1000) are
+timed between two device synchronisations, first in plain FP64, then in mixed
+precision (Legendre transforms on FP32 MFMA, everything else FP64), each from
+the same seeded initial state; the mixed run replays the FP64 run's step
+sizes so that the two differ by arithmetic precision only.  The headline value
+is the mixed-precision steps/s; the FP64 rate, their ratio and the relative
+energy and spectrum differences after W+K steps are in ``config``.
+Per-section HIP-event timings from the timed region give the roofline of the
+dominant kernel.  Also reported: the finite-difference banded solves (FD4,
+n_r=512, l_max=255) and the 64-run ensemble served with continuous admission
+against a sequential sample.  The CPU baseline is the FP64 numpy oracle on a
+bounded sample of the same workload.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision both|mixed|fp64]
-                       [--no-cpu-baseline]
+                       [--no-cpu-baseline] [--no-fd] [--no-ensemble]
 """
 
 import argparse
@@ -93,11 +97,13 @@ def cpu_baseline(sample_levels):
                 seconds=dt)
 
 
-def run_case(precision, steps, warmup):
-    """Time `steps` CNAB2 steps after `warmup` untimed ones; per-section HIP events."""
+def run_case(precision, steps, warmup, dt_schedule=None):
+    """Time `steps` CNAB2 steps after `warmup` untimed ones; per-section HIP events.
+    dt_schedule: step sizes to replay instead of the run's own Courant decisions."""
     import torch
     from magic_amd.dynamo import DynamoRun
     run = DynamoRun(dict(WORKLOAD, precision=precision), device="cuda:0")
+    run.dt_schedule = dt_schedule
     run.initialize_fields(noise_amp=NOISE["amp"], noise_seed=NOISE["seed"])
     run.setup_initial_state()
     run.initialize_dt(WORKLOAD["dtmax"])
@@ -117,8 +123,8 @@ def run_case(precision, steps, warmup):
     t1 = time.perf_counter()
     secs = run.timer_summary()
     run.enable_timers(False)
-    out = dict(elapsed=t1 - t0, secs=secs, energies=run.energies(), dt=dt,
-               rebuilds=run.n_rebuild, p=run.p, N=run.N)
+    out = dict(elapsed=t1 - t0, secs=secs, energies=run.energies(), spectra=run.spectra(), dt=dt,
+               rebuilds=run.n_rebuild, p=run.p, N=run.N, dt_log=list(run.dt_log))
     del run
     torch.cuda.empty_cache()
     return out
@@ -174,13 +180,20 @@ def ensemble_case(n_seq):
     seeded Poisson process at 2/s, served with continuous admission (16 slots, HBM
     budget); baseline: the first n_seq of the same requests run back to back, one at
     a time, in plain FP64 (arrivals ignored)."""
+    import numpy as np
     from magic_amd.serve import EnsembleServer, config5_requests, serve_sequential
     srv = EnsembleServer(max_slots=16)
     _, mix = srv.serve(config5_requests(), realtime=True)
+    # the same 64 runs all waiting at t=0: throughput of the server when saturated
+    sat_srv = EnsembleServer(max_slots=16)
+    _, sat = sat_srv.serve(config5_requests(), realtime=False)
     _, seq = serve_sequential(config5_requests()[:n_seq])
     return dict(workload="64 runs x 500 steps, Poisson arrivals 2/s, l_max {32,64,85,127}, "
                          "n_r {33,49,65}, FP64",
-                server=dict(mix, max_active=max(n for _, n in srv.rounds)),
+                server=dict(mix, max_active=max(n for _, n in srv.rounds),
+                            admit_ms_mean=1e3 * float(np.mean(srv.admit_s)),
+                            admit_ms_max=1e3 * float(np.max(srv.admit_s))),
+                saturated=dict(sat, max_active=max(n for _, n in sat_srv.rounds)),
                 sequential=dict(seq, sample=f"first {n_seq} of the 64 requests back to back"))
 
 
@@ -203,7 +216,7 @@ def roofline(res, precision):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--precision", choices=("both", "mixed", "fp64"), default="both")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -221,7 +234,11 @@ def main():
     precs = ("fp64", "mixed") if args.precision == "both" else (args.precision,)
     res = {}
     for pr in precs:
-        res[pr] = run_case(pr, args.steps, args.warmup)
+        # the mixed run replays the FP64 run's step sizes, so the two trajectories
+        # differ by arithmetic precision only (the Courant dt = 0.75 dt_min moves
+        # with the last bits of the velocity maxima)
+        sched = res["fp64"]["dt_log"] if (pr == "mixed" and "fp64" in res) else None
+        res[pr] = run_case(pr, args.steps, args.warmup, sched)
         print(f"bench: {pr} {args.steps} steps in {res[pr]['elapsed']:.2f} s", file=sys.stderr,
               flush=True)
     head = "mixed" if "mixed" in res else "fp64"
@@ -244,6 +261,13 @@ def main():
         e64, emx = np.array(res["fp64"]["energies"]), np.array(res["mixed"]["energies"])
         cfg["mixed_over_fp64"] = sps["mixed"] / sps["fp64"]
         cfg["energy_rel_diff_mixed_vs_fp64"] = float(np.max(np.abs(emx - e64) / np.abs(e64)))
+        cfg["energy_rel_diff_note"] = (f"after {args.warmup + args.steps} steps, mixed run on the "
+                                       "FP64 run's dt sequence; max over e_kin_pol/tor, e_mag_pol/tor")
+        sp = {}
+        for k in ("kin_l", "mag_l", "kin_m", "mag_m"):
+            a, b = res["fp64"]["spectra"][k], res["mixed"]["spectra"][k]
+            sp[k] = float(np.max(np.abs(b - a)) / np.max(np.abs(a)))
+        cfg["spectrum_rel_diff_mixed_vs_fp64"] = sp
     if not args.no_fd:
         cfg["fd_banded"] = fd_banded_case()
     if not args.no_ensemble:

@@ -10,6 +10,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import threading
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libmagic_hip.so")
@@ -179,35 +180,40 @@ def load(path: str | None = None):
     return lib
 
 
-_recording = None  # list collecting (section, name, args) while a launch plan is recorded
+_tls = threading.local()  # per thread: the launch plan being recorded (or None)
+
+
+def recording():
+    """The launch plan this thread is recording, or None."""
+    return getattr(_tls, "rec", None)
 
 
 class record:
-    """Collect the entry-point calls made inside the block instead of running
-    them: ``with record() as plan: ...``; each item is (section, name, args
-    without the trailing stream).  Every entry point takes the stream last."""
+    """Collect the entry-point calls made inside the block (in this thread)
+    instead of running them: ``with record() as plan: ...``; each item is
+    (section, name, args without the trailing stream).  Every entry point
+    takes the stream last."""
 
     def __init__(self):
         self.calls = []
         self.section = None
 
     def __enter__(self):
-        global _recording
-        if _recording is not None:
+        if recording() is not None:
             raise RuntimeError("nested launch-plan recording")
-        _recording = self
+        _tls.rec = self
         return self
 
     def __exit__(self, *exc):
-        global _recording
-        _recording = None
+        _tls.rec = None
         return False
 
 
 def call(name: str, *args):
     """Call an entry point and raise on a non-zero return code."""
-    if _recording is not None:
-        _recording.calls.append((_recording.section, name, args[:-1]))
+    rec = recording()
+    if rec is not None:
+        rec.calls.append((rec.section, name, args[:-1]))
         return 0
     rc = getattr(load(), name)(*args)
     if rc != 0:

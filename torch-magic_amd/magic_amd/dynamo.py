@@ -94,6 +94,9 @@ def conduction_state(p: Params, res: Resolution):
     return C @ x[:N], C @ x[N:]
 
 
+_INFO_ROWS = ("s", "z", "b", "j", "wp", "p0")
+
+
 class DynamoRun:
     """State and step of one Boussinesq Chebyshev CNAB2 dynamo run on the GPU."""
 
@@ -160,6 +163,8 @@ class DynamoRun:
         self.use_graphs = False
         self._sec_ev = None
         self._cfl_event = None
+        self.dt_log = []         # dt used by each step
+        self.dt_schedule = None  # optional step sizes to use instead of the Courant decision
 
     # ------------------------------------------------------------ timing
     def enable_timers(self, on: bool = True):
@@ -171,7 +176,7 @@ class DynamoRun:
 
         class _Sec:
             def __enter__(self_):
-                rec = _lib._recording
+                rec = _lib.recording()
                 if rec is not None:
                     self_.prev = rec.section
                     rec.section = name
@@ -182,7 +187,7 @@ class DynamoRun:
                     self_.ev[0].record()
 
             def __exit__(self_, *exc):
-                rec = _lib._recording
+                rec = _lib.recording()
                 if rec is not None:
                     rec.section = self_.prev
                     return False
@@ -202,9 +207,8 @@ class DynamoRun:
             end.record()
             self.timers.setdefault(name, []).append((self._sec_ev, end))
 
-    def _exec(self, kind):
-        """Run the radial or lm phase: recorded once per parity, then replayed
-        eagerly (with section timers when enabled) or as a captured HIP graph."""
+    def _plan(self, kind):
+        """The recorded launch plan of a phase for the current parity."""
         key = (kind, self._par)
         plan = self._plans.get(key)
         if plan is None:
@@ -214,18 +218,35 @@ class DynamoRun:
                 else:
                     self._lm_launches()
             plan = self._plans[key] = rec.calls
-        if self.use_graphs and self.timers is None:
-            g = self._graphs.get(key)
-            if g is None:
-                cur = torch.cuda.current_stream()
-                side = torch.cuda.Stream(device=self.device)  # default priority
-                side.wait_stream(cur)
-                g = torch.cuda.CUDAGraph()
-                with torch.cuda.graph(g, stream=side):
+        return key, plan
+
+    def _graph(self, key, plan):
+        """The captured HIP graph of a recorded plan (captured once)."""
+        g = self._graphs.get(key)
+        if g is None:
+            # capture on a side stream (default priority); the low-level capture
+            # calls avoid the device-wide synchronisation of torch.cuda.graph,
+            # which would stall every other run on the GPU
+            cur = torch.cuda.current_stream()
+            side = torch.cuda.Stream(device=self.device)
+            side.wait_stream(cur)
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.stream(side):
+                g.capture_begin()
+                try:
                     _lib.run_plan(plan, stream_handle())
-                cur.wait_stream(side)
-                self._graphs[key] = g
-            g.replay()
+                finally:
+                    g.capture_end()
+            cur.wait_stream(side)
+            self._graphs[key] = g
+        return g
+
+    def _exec(self, kind):
+        """Run the radial or lm phase: recorded once per parity, then replayed
+        eagerly (with section timers when enabled) or as a captured HIP graph."""
+        key, plan = self._plan(kind)
+        if self.use_graphs and self.timers is None:
+            self._graph(key, plan).replay()
         else:
             _lib.run_plan(plan, stream_handle(),
                           self._on_section if self.timers is not None else None)
@@ -261,7 +282,11 @@ class DynamoRun:
         self.mat_wp = torch.empty(lp1, 2 * N, 2 * N, dtype=F64, device=dev)
         self.gj_work = torch.empty(lp1 * 2 * (2 * N) * (2 * N), dtype=F64, device=dev)
         self.gj_scal = torch.empty(lp1 * 2 * 2 * N, dtype=F64, device=dev)
-        self.gj_info = torch.zeros(lp1, dtype=I32, device=dev)
+        # first zero pivot per degree, one row per system (s, z, b, j, wp, p0)
+        self.gj_info = torch.zeros(len(_INFO_ROWS), lp1, dtype=I32, device=dev)
+        self._info_host = torch.zeros(len(_INFO_ROWS), lp1, dtype=I32).pin_memory()
+        self._info_event = None
+        self.defer_checks = False  # check the pivots of a rebuild at the next step (no host wait)
         # uniform radial-derivative operators (transposed, padded)
         r2 = rad["r2"]
 
@@ -312,8 +337,7 @@ class DynamoRun:
                 continue
             self._build_kind(kind, self.mat_nn, wimp_lin0)
             _lib.call("mh_gj_inverse", ptr(self.mat_nn), ptr(self.mat_nn), ptr(self.gj_work),
-                      ptr(self.gj_scal), N, lp1, 1, ptr(self.gj_info), st)
-            self._check_info(key)
+                      ptr(self.gj_scal), N, lp1, 1, ptr(self.gj_info[_INFO_ROWS.index(key)]), st)
             MT = self.MT[key]
             # MT[l][k][r] = sum_j inv_l[j][k] Op3[r][j]
             self._gemm(ptr(self.mat_nn), 1, N, N * N, ptr(self.Op3), 1, N, 0,
@@ -323,8 +347,7 @@ class DynamoRun:
         # coupled (w, p)
         self._build_kind(4, self.mat_wp, wimp_lin0)
         _lib.call("mh_gj_inverse", ptr(self.mat_wp), ptr(self.mat_wp), ptr(self.gj_work),
-                  ptr(self.gj_scal), 2 * N, lp1, 2, ptr(self.gj_info), st)
-        self._check_info("wp")
+                  ptr(self.gj_scal), 2 * N, lp1, 2, ptr(self.gj_info[_INFO_ROWS.index("wp")]), st)
         MT = self.MT["wp"]
         sb = self.K4wp * self.Rpad6
         self._gemm(ptr(self.mat_wp), 1, 2 * N, 4 * N * N, ptr(self.Opw), 1, N, 0,
@@ -336,19 +359,37 @@ class DynamoRun:
         p0 = self.mat_nn[0]
         self._build_kind(5, p0, wimp_lin0)
         _lib.call("mh_gj_inverse", ptr(p0), ptr(p0), ptr(self.gj_work), ptr(self.gj_scal), N, 1, 1,
-                  ptr(self.gj_info), st)
-        self._check_info("p0", n=1)
+                  ptr(self.gj_info[_INFO_ROWS.index("p0")]), st)
         self._gemm(ptr(p0), 1, N, 0, ptr(self.Opp), 1, N, 0,
                    ptr(MT) + 8 * (N * self.Rpad6 + 4 * N), self.Rpad6, 1, 0, N, 2 * N, N, 1)
         self.wimp_lin0 = wimp_lin0
         self.n_rebuild += 1
+        # pivots: one host read for all systems, now or (defer_checks) at the next step
+        self._info_host.copy_(self.gj_info, non_blocking=True)
+        if self._info_event is None:
+            self._info_event = torch.cuda.Event()
+        self._info_event.record()
+        self._info_pending = True
+        if not self.defer_checks:
+            self._check_info()
 
-    def _check_info(self, name, n=None):
-        info = self.gj_info[: (n or self.p.l_max + 1)].cpu()
-        bad = torch.nonzero(info).flatten()
-        if bad.numel():
-            l = int(bad[0])
-            raise AssertionError(f"Singular {name} matrix for l={l}, info={int(info[l])}")
+    def _check_info(self):
+        """Raise the reference's AssertionError for the first singular system
+        (``update_s.py:152`` message form) of the last rebuild."""
+        if not getattr(self, "_info_pending", False):
+            return
+        self._info_event.synchronize()
+        self._info_pending = False
+        info = self._info_host.numpy()
+        lp1 = self.p.l_max + 1
+        for row, name in enumerate(_INFO_ROWS):
+            if name in ("b", "j") and not self.p.l_mag:
+                continue
+            n = 1 if name == "p0" else lp1
+            bad = np.nonzero(info[row, :n])[0]
+            if bad.size:
+                l = int(bad[0])
+                raise AssertionError(f"Singular {name} matrix for l={l}, info={int(info[row, l])}")
 
     # ------------------------------------------------------------ job tables
     def _build_jobs(self):
@@ -676,8 +717,14 @@ class DynamoRun:
 
     def finish_step(self, n_time_step: int, dt: float) -> float:
         """Wait for the Courant maxima, decide dt, rebuild if needed, queue the lm loop."""
+        self._check_info()  # a deferred check of the previous rebuild (done by now)
         dtrkc, dthkc = self._cfl_values()
         l_new_dt, dt_new = dt_courant(dt, self.dtMax, dtrkc, dthkc)
+        if self.dt_schedule is not None and n_time_step <= len(self.dt_schedule):
+            # replay another run's step sizes (precision comparisons on one dt sequence)
+            dt_new = float(self.dt_schedule[n_time_step - 1])
+            l_new_dt = dt_new != dt
+        self.dt_log.append(dt_new)
         self.dt_hist = [dt_new, self.dt_hist[0]]
         wl0 = self.p.alpha * dt_new
         if n_time_step == 1 or l_new_dt or self.wimp_lin0 != wl0:

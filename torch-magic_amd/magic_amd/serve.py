@@ -134,19 +134,23 @@ class EnsembleServer:
             r = self._res_cache[key] = Resolution(p, self.device)
         return r
 
-    def _admit(self, req: Request, t: float, fp: int) -> _Slot:
+    def _admit(self, req: Request, fp: int) -> _Slot:
+        """Build and initialise a run on its own stream."""
         from .dynamo import DynamoRun
         from .params import Params
         torch = self.torch
+        t0 = self.clock()
         p = Params(dict(req.cfg))
         stream = torch.cuda.Stream(device=self.device)
         with torch.cuda.stream(stream):
             run = DynamoRun(p, self.device, res=self._resolution(p))
             run.use_graphs = self.use_graphs
+            run.defer_checks = True  # singular pivots surface at the next step, without a host wait
             run.initialize_fields(noise_amp=req.noise_amp, noise_seed=req.noise_seed)
             run.setup_initial_state()
             run.initialize_dt(p.dtmax)
-        return _Slot(req, run, stream, t, fp)
+        self.admit_s.append(self.clock() - t0)
+        return _Slot(req, run, stream, 0.0, fp)
 
     def serve(self, requests, realtime: bool = True, keep_fields: bool = False):
         """Serve `requests` (arrival times honoured when realtime); returns the
@@ -159,7 +163,6 @@ class EnsembleServer:
         and arrivals are admitted between rounds.  A run's step latency is the
         time between the ends of its consecutive rounds (for its first step:
         from admission)."""
-        torch = self.torch
         pending = sorted(requests, key=lambda r: (r.arrival, r.rid))
         for r in pending:
             if footprint_bytes(r.cfg) > self.budget:
@@ -169,6 +172,7 @@ class EnsembleServer:
         done: list[Result] = []
         used = 0
         self.rounds = []
+        self.admit_s = []  # host time of each admission (run construction and set-up)
         t0 = self.clock()
         while pending or active:
             now = self.clock() - t0
@@ -182,53 +186,78 @@ class EnsembleServer:
                     break
                 pending.pop(0)
                 used += fp
-                active.append(self._admit(req, self.clock() - t0, fp))
+                sl = self._admit(req, fp)
+                sl.last_done = sl.res.admitted = self.clock() - t0
+                active.append(sl)
             if not active:
                 time.sleep(max(0.0, min(0.01, pending[0].arrival - (self.clock() - t0))))
                 continue
-            # one round: queue every radial phase, then finish runs as their maxima land
-            for sl in active:
-                with torch.cuda.stream(sl.stream):
-                    sl.run.begin_step()
-            waiting = list(active)
-            while waiting:
-                for sl in list(waiting):
-                    if sl.run.cfl_ready():
-                        with torch.cuda.stream(sl.stream):
-                            try:
-                                sl.dt = sl.run.finish_step(sl.n + 1, sl.dt)
-                            except AssertionError as e:
-                                raise AssertionError(f"request {sl.req.rid} step {sl.n + 1} "
-                                                     f"(dt {sl.dt:.6g}): {e}") from e
-                        sl.n += 1
-                        waiting.remove(sl)
-                if waiting:
-                    time.sleep(0)
-            t_end = self.clock() - t0
-            self.rounds.append((t_end, len(active)))
-            for sl in list(active):
-                sl.res.step_latencies.append(t_end - sl.last_done)
-                sl.last_done = t_end
-                if sl.n >= sl.req.n_steps:
-                    with torch.cuda.stream(sl.stream):
-                        sl.res.energies = sl.run.energies()
-                        if keep_fields:
-                            sl.res.final_fields = {k: sl.run.f[k].clone()
-                                                   for k in ("s", "w", "z", "b", "aj")}
-                    sl.res.finished = self.clock() - t0
-                    done.append(sl.res)
-                    active.remove(sl)
-                    used -= sl.res.footprint
-                    sl.run = None
+            used -= self._round(active, done, t0, keep_fields)
         wall = self.clock() - t0
         return done, summarize(done, wall)
 
+    def _round(self, active, done, t0, keep_fields) -> int:
+        """One round: every active run advances one step (all radial phases are
+        queued first, then each run's lm phase as soon as its Courant maxima
+        arrive).  Finished runs leave; returns the HBM footprint they free."""
+        torch = self.torch
+        for sl in active:
+            with torch.cuda.stream(sl.stream):
+                sl.run.begin_step()
+        waiting = list(active)
+        while waiting:
+            for sl in list(waiting):
+                if sl.run.cfl_ready():
+                    with torch.cuda.stream(sl.stream):
+                        try:
+                            sl.dt = sl.run.finish_step(sl.n + 1, sl.dt)
+                        except AssertionError as e:
+                            raise AssertionError(f"request {sl.req.rid} step {sl.n + 1} "
+                                                 f"(dt {sl.dt:.6g}): {e}") from e
+                    sl.n += 1
+                    waiting.remove(sl)
+            if waiting:
+                time.sleep(0)
+        t_end = self.clock() - t0
+        self.rounds.append((t_end, len(active)))
+        freed = 0
+        for sl in list(active):
+            sl.res.step_latencies.append(t_end - sl.last_done)
+            sl.last_done = t_end
+            if sl.n >= sl.req.n_steps:
+                with torch.cuda.stream(sl.stream):
+                    try:
+                        sl.run._check_info()  # the last rebuild's pivots
+                    except AssertionError as e:
+                        raise AssertionError(f"request {sl.req.rid} step {sl.n}: {e}") from e
+                    sl.res.energies = sl.run.energies()
+                    if keep_fields:
+                        sl.res.final_fields = {k: sl.run.f[k].clone()
+                                               for k in ("s", "w", "z", "b", "aj")}
+                sl.res.finished = self.clock() - t0
+                done.append(sl.res)
+                active.remove(sl)
+                freed += sl.res.footprint
+                sl.run = None
+        return freed
+
 
 def summarize(results, wall: float) -> dict:
+    """Throughput and step-latency percentiles over all runs, and the latency
+    percentiles of each resolution class (l_max, n_r_max) in the mix."""
     lat = np.concatenate([np.asarray(r.step_latencies, float) for r in results]) if results else \
         np.zeros(0)
     steps = int(lat.size)
-    return dict(runs=len(results), steps=steps, wall_s=wall,
+    by_class = {}
+    cls = lambda r: (r.cfg.get("l_max"), r.cfg.get("n_r_max"))
+    for key in sorted({cls(r) for r in results}, key=str):
+        x = np.concatenate([np.asarray(r.step_latencies, float) for r in results if cls(r) == key])
+        if x.size:
+            p50, p99 = np.percentile(x, 50), np.percentile(x, 99)
+            by_class[f"l{key[0]}_n{key[1]}"] = dict(steps=int(x.size), p50_ms=float(p50 * 1e3),
+                                                     p99_ms=float(p99 * 1e3),
+                                                     p99_over_p50=float(p99 / p50))
+    return dict(runs=len(results), steps=steps, wall_s=wall, by_class=by_class,
                 runs_per_hour=3600.0 * len(results) / wall if wall > 0 else math.nan,
                 steps_per_s=steps / wall if wall > 0 else math.nan,
                 p50_ms=float(np.percentile(lat, 50) * 1e3) if steps else math.nan,
