@@ -42,6 +42,15 @@ WORKLOAD = dict(l_max=383, n_r_max=97, ek=1e-5, ra=2e8, pr=1.0, prmag=1.0, radra
                 dtmax=1e-4)
 NOISE = dict(seed=20261202, amp=1e-4)
 
+# HBM bytes per launch (read + written) of the main kernels at this workload, from
+# rocprofv3 FETCH_SIZE / WRITE_SIZE passes scaled by a 256 MiB copy (FETCH x 2,
+# WRITE x 1); see profiles/hbm_traffic_l383_nr97_pmc.md
+PMC_TRAFFIC_BYTES = {
+    ("grid_step", "mixed"): 6.461e9 + 3.239e9, ("grid_step", "fp64"): 6.448e9 + 3.226e9,
+    ("leg_syn_vec", "mixed"): 3.822e9 + 4.373e9, ("leg_syn_vec", "fp64"): 4.425e9 + 4.132e9,
+    ("leg_ana_vec", "mixed"): 5.559e9 + 0.609e9, ("leg_ana_vec", "fp64"): 6.191e9 + 0.667e9,
+}
+
 PEAK_FP64_TFLOPS = 78.6
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
@@ -207,10 +216,12 @@ def roofline(res, precision):
         peak = PEAK_FP32_TFLOPS if (precision == "mixed" and dom.startswith("leg_")) else PEAK_FP64_TFLOPS
         ach = amount / (ms * 1e-3) / 1e12
         return dict(bound="mfma", achieved=ach, peak=peak, unit="TFLOP/s", frac=ach / peak,
-                    traffic=None, kernel=dom, launch_ms=ms, algorithmic_flops=amount)
+                    traffic=PMC_TRAFFIC_BYTES.get((dom, precision)), kernel=dom, launch_ms=ms,
+                    algorithmic_flops=amount)
     ach = amount / (ms * 1e-3) / 1e9
     return dict(bound="hbm", achieved=ach, peak=PEAK_HBM_GBS, unit="GB/s", frac=ach / PEAK_HBM_GBS,
-                traffic=None, kernel=dom, launch_ms=ms, algorithmic_bytes=amount)
+                traffic=PMC_TRAFFIC_BYTES.get((dom, precision)), kernel=dom, launch_ms=ms,
+                algorithmic_bytes=amount)
 
 
 def main():
