@@ -17,13 +17,13 @@ sizes so that the two differ by arithmetic precision only.  The headline value
 is the mixed-precision steps/s; the FP64 rate, their ratio and the relative
 energy and spectrum differences after W+K steps are in ``config``.
 Per-section HIP-event timings from the timed region give the roofline of the
-dominant kernel.  Also reported: the finite-difference banded solves (FD4,
-n_r=512, l_max=255) and the 64-run ensemble served with continuous admission
+dominant kernel.  Also reported: the SHT round trip at l_max=383 on 291 fields,
+the finite-difference banded solves (FD4, n_r=512, l_max=255) and the 64-run ensemble served with continuous admission
 against a sequential sample.  The CPU baseline is the FP64 numpy oracle on a
 bounded sample of the same workload.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision both|mixed|fp64]
-                       [--no-cpu-baseline] [--no-fd] [--no-ensemble]
+                       [--no-cpu-baseline] [--no-fd] [--no-ensemble] [--no-sht]
 """
 
 import argparse
@@ -80,7 +80,7 @@ def section_work(p, N):
     }
 
 
-def cpu_baseline(sample_levels):
+def cpu_baseline(sample_levels, steps=3):
     """Time the FP64 oracle on a bounded sample of the workload (see 'sample')."""
     import numpy as np
     from oracle.dynamo import OracleRun
@@ -90,17 +90,18 @@ def cpu_baseline(sample_levels):
     run.add_noise(NOISE["seed"], NOISE["amp"])
     run.setup_initial_state()
     run.initialize_dt(WORKLOAD["dtmax"])
-    run.one_step(1, WORKLOAD["dtmax"])   # builds the matrices
+    dt_step = run.one_step(1, WORKLOAD["dtmax"])   # builds the matrices
     t0 = time.perf_counter()
-    run.one_step(2, WORKLOAD["dtmax"])
-    dt = time.perf_counter() - t0
+    for n in range(2, 2 + steps):
+        dt_step = run.one_step(n, dt_step)
+    dt = (time.perf_counter() - t0) / steps
     try:
         import torch
         cores = torch.get_num_threads()
     except Exception:
         cores = os.cpu_count()
     return dict(value=1.0 / dt, unit="steps/s", cores=cores, kind="port",
-                sample=f"one CNAB2 step of the numpy FP64 oracle at l_max=383 with n_r_max="
+                sample=f"{steps} CNAB2 steps of the numpy FP64 oracle at l_max=383 with n_r_max="
                        f"{sample_levels} (the full 576x1152 angular grid); the GPU value "
                        f"is at n_r_max=97, so scale by ~{sample_levels}/97 for equal work",
                 seconds=dt)
@@ -183,6 +184,61 @@ def fd_banded_case():
                 info_ok=all(int(ops.dev[k]["info"].abs().sum()) == 0 for k in ("S", "Z", "W")))
 
 
+def sht_case():
+    """Config 2: SHT round trip at l_max=383 (n_theta=576, n_phi=1152) on 3 x 97 = 291
+    fields with seeded Gaussian coefficients scaled by (l+1)^-2 (m=0 real), in FP64 and
+    in mixed precision: ms per direction (synthesis + inverse FFT; forward FFT +
+    analysis) and the max / RMS round-trip error relative to the largest coefficient
+    (analysis(synthesis(x)) = x exactly for band-limited x), plus the mixed grid
+    against the FP64 grid."""
+    import numpy as np
+    import torch
+    from magic_amd.device import Resolution
+    from magic_amd.params import Params
+    from magic_amd.sht import SHT
+    p = Params({"l_max": 383, "n_r_max": 97})
+    res = Resolution(p)
+    B = 3 * 97
+    rng = np.random.default_rng(NOISE["seed"])
+    l, m = res.maps["l"], res.maps["m"]
+    S = (rng.standard_normal((p.lm_max, B)) + 1j * rng.standard_normal((p.lm_max, B))) \
+        * ((l + 1.0) ** -2.0)[:, None]
+    S[m == 0] = S[m == 0].real
+    dS = torch.from_numpy(S).cuda()
+    smax = np.abs(S).max()
+    ev = lambda: torch.cuda.Event(enable_timing=True)
+    out, grids = {}, {}
+    flops = 4.0 * p.lm_max * (p.n_theta // 2) * B
+    for prec in ("fp64", "mixed"):
+        sht = SHT(res, precision=prec)
+        g = sht.scal_to_spat(dS)
+        back = sht.scal_to_SH(g)                       # warm-up (tables, module load)
+        torch.cuda.synchronize()
+        t = {"syn": [], "ana": []}
+        for _ in range(3):
+            a, b, c = ev(), ev(), ev()
+            a.record()
+            g = sht.scal_to_spat(dS)
+            b.record()
+            back = sht.scal_to_SH(g)
+            c.record()
+            torch.cuda.synchronize()
+            t["syn"].append(a.elapsed_time(b))
+            t["ana"].append(b.elapsed_time(c))
+        err = (back - dS).abs().cpu().numpy()
+        out[prec] = dict(synthesis_ms=float(np.min(t["syn"])), analysis_ms=float(np.min(t["ana"])),
+                         roundtrip_max_err=float(err.max() / smax),
+                         roundtrip_rms_err=float(np.sqrt((err ** 2).mean()) / smax),
+                         legendre_tflops_synthesis_upper=flops / (np.min(t["syn"]) * 1e-3) / 1e12)
+        grids[prec] = g
+    d = (grids["mixed"] - grids["fp64"]).abs()
+    gmax = grids["fp64"].abs().max()
+    out["mixed_grid_vs_fp64"] = dict(max_err=float(d.max() / gmax),
+                                     rms_err=float(d.pow(2).mean().sqrt() / gmax))
+    out["workload"] = "l_max=383, 576 x 1152 grid, 291 fields, coefficients N(0,1) (l+1)^-2"
+    return out
+
+
 def ensemble_case(n_seq):
     """Config 5: 64 Boussinesq runs (l_max in {32, 64, 85, 127}, n_r in {33, 49, 65},
     E log-uniform [1e-4, 1e-3], Ra_mod uniform [60, 200], 500 steps each) arriving as a
@@ -234,6 +290,7 @@ def main():
     ap.add_argument("--no-fd", action="store_true", help="skip the config-3 banded-solve timing")
     ap.add_argument("--cpu-levels", type=int, default=9)
     ap.add_argument("--no-ensemble", action="store_true", help="skip the config-5 ensemble")
+    ap.add_argument("--no-sht", action="store_true", help="skip the config-2 SHT round trip")
     ap.add_argument("--ensemble-seq", type=int, default=16,
                     help="requests in the sequential ensemble baseline sample")
     args = ap.parse_args()
@@ -279,8 +336,12 @@ def main():
             a, b = res["fp64"]["spectra"][k], res["mixed"]["spectra"][k]
             sp[k] = float(np.max(np.abs(b - a)) / np.max(np.abs(a)))
         cfg["spectrum_rel_diff_mixed_vs_fp64"] = sp
+    if not args.no_sht:
+        cfg["sht_roundtrip"] = sht_case()
+        print("bench: SHT round trip done", file=sys.stderr, flush=True)
     if not args.no_fd:
         cfg["fd_banded"] = fd_banded_case()
+        print("bench: FD banded solves done", file=sys.stderr, flush=True)
     if not args.no_ensemble:
         cfg["ensemble"] = ensemble_case(args.ensemble_seq)
         print(f"bench: ensemble wall {cfg['ensemble']['server']['wall_s']:.1f} s", file=sys.stderr,
@@ -288,6 +349,7 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(args.cpu_levels)
+        print(f"bench: CPU baseline step {cpu['seconds']:.1f} s", file=sys.stderr, flush=True)
     line = {
         "metric": "steps/s at l_max=383 mixed vs FP64; ensemble p50/p99 step latency",
         "value": sps[head],
