@@ -8,6 +8,8 @@
 // up to kl, ku = 4 (the 9-band double-curl operator) are supported.
 #include "common.h"
 
+#include <type_traits>
+
 #define AB(i, j) abd[(long)(i) * n + (j)]  // 0-based band row i, column j
 
 __global__ void k_band_factor(double* __restrict__ abd_all, int* __restrict__ piv_all,
@@ -76,11 +78,23 @@ extern "C" int mh_band_factor(double* abd, int* pivot, int* info, int n, int kl,
   return mh_launch_status();
 }
 
-// dgbsl for one right-hand-side row per lane, streamed through register
-// windows: the forward elimination keeps b(k .. k+kl), the back substitution
-// b(k-kl-ku .. k); every element of the row is read and written once per pass.
-// `rows` (optional) orders the rows by degree so that the lanes of a wave
-// share their factors (broadcast loads).
+// dgbsl over LDS tiles of 16 columns.  The factors are real, so the real and
+// imaginary parts of a complex right-hand side are independent recurrences:
+// lane 2r+c runs component c of row r, and a wave owns 32 rows.  Each tile is
+// loaded (and stored) cooperatively, sixteen lanes moving one row's 256
+// contiguous bytes; each lane then runs its recurrence out of LDS.  Forward
+// elimination keeps b(k .. k+kl) in a register window and emits x(k-1) kl
+// columns behind the input; the back substitution streams the columns
+// downwards with the window x(k-m .. k-1), m = kl+ku+1, emitting x(k-1) m
+// columns above the input.  Outputs collect in a ring of LDS tiles that is
+// stored as soon as a tile is complete; the forward result is the back
+// substitution's input (in `sol`).  `rows` (optional) orders the rows by
+// degree so that the lanes of a wave share their factors (broadcast loads).
+// In place (rhs == sol) is allowed: every tile is read before any output
+// lands on it.
+constexpr int kBT = 16;  // columns per tile
+constexpr int kBR = 32;  // rows per wave
+
 template <int KL, int KU>
 __global__ __launch_bounds__(64) void k_band_solve(const double* __restrict__ abd_all,
                                                    const int* __restrict__ piv_all,
@@ -91,71 +105,228 @@ __global__ __launch_bounds__(64) void k_band_solve(const double* __restrict__ ab
                                                    const double2* rhs, double2* sol, int L,
                                                    int n) {
   constexpr int M = KL + KU + 1;
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= L) return;
-  const int lm = rows ? rows[idx] : idx;
+  constexpr int NR = 2 * KL + KU + 1;
+  constexpr int TP = kBT + 1;                    // padded LDS row (conflict-free columns)
+  constexpr int NO = (M + kBT - 1) / kBT + 1;    // output ring slots (back substitution)
+  constexpr int CR = 64 / kBT;                   // rows per cooperative access
+  constexpr int CI = kBR / CR;                   // cooperative accesses per lane
+  __shared__ double2 tin[kBR][TP];
+  __shared__ double2 tout[NO > 2 ? NO : 2][kBR][TP];
+  const int lane = threadIdx.x;
+  const int r0 = blockIdx.x * kBR;
+  // this lane's row and component, and its degree's factors
+  const int row = lane >> 1, comp = lane & 1;
+  const int ri = r0 + row;
+  const int lm = ri < L ? (rows ? rows[ri] : ri) : 0;
   const int l = lm2l[lm];
-  const double* abd = abd_all + (long)l * (2 * KL + KU + 1) * n;
+  const double* abd = abd_all + (long)l * NR * n;
   const int* pivot = piv_all + (long)l * n;
-  const double* fr = fac_row ? fac_row + (long)l * n : nullptr;
-  const double* fc = fac_col ? fac_col + (long)l * n : nullptr;
-  const double2* b = rhs + (long)lm * n;
-  double2* x = sol + (long)lm * n;
-  auto bin = [&](int i) {  // scaled right-hand side b(i), 0-based, zero past the end
-    if (i >= n) return make_double2(0.0, 0.0);
-    const double2 v = b[i];
-    return fr ? cscale(v, fr[i]) : v;
+  const double* fcl = fac_col ? fac_col + (long)l * n : nullptr;
+  auto in_at = [&](int c) { return reinterpret_cast<const double*>(&tin[row][c])[comp]; };
+  auto out_at = [&](int slot, int pos, double v) {
+    reinterpret_cast<double*>(&tout[slot][row][pos])[comp] = v;
   };
-  // forward elimination, window W[j] = b(k+j), k = 1 .. n (1-based)
+  // cooperative map: lane -> sub-row cs (rows cs + CR*it) and column cc of a tile
+  static_assert(64 % kBT == 0 && kBR % CR == 0, "cooperative tile map");
+  const int cs = lane / kBT, cc = lane % kBT;
+  long cbase[CI];
+  const double* cfr[CI];
+  bool cok[CI];
+#pragma unroll
+  for (int it = 0; it < CI; ++it) {
+    const int rr = r0 + cs + CR * it;
+    cok[it] = rr < L;
+    const int lmr = cok[it] ? (rows ? rows[rr] : rr) : 0;
+    cbase[it] = (long)lmr * n;
+    cfr[it] = fac_row ? fac_row + (long)lm2l[lmr] * n : nullptr;
+  }
+  const int ntile = (n + kBT - 1) / kBT;
+  auto coop_load = [&](const double2* src, int tile, bool scale, double2 (&v)[CI]) {
+    const int col = tile * kBT + cc;
+#pragma unroll
+    for (int it = 0; it < CI; ++it) {
+      v[it] = make_double2(0.0, 0.0);
+      if (cok[it] && col < n) {
+        v[it] = src[cbase[it] + col];
+        if (scale && cfr[it]) v[it] = cscale(v[it], cfr[it][col]);
+      }
+    }
+  };
+  auto coop_put = [&](const double2 (&v)[CI]) {
+#pragma unroll
+    for (int it = 0; it < CI; ++it) tin[cs + CR * it][cc] = v[it];
+  };
+  auto coop_store = [&](int slot, int tile) {
+    const int col = tile * kBT + cc;
+#pragma unroll
+    for (int it = 0; it < CI; ++it)
+      if (cok[it] && col < n) sol[cbase[it] + col] = tout[slot][cs + CR * it][cc];
+  };
+  double2 pre[CI];
+
+  // ---- forward elimination: x = L^-1 P b (rows scaled by fac_row)
   if (KL > 0) {
-    double2 W[KL + 1];
+    double W[KL + 1];
 #pragma unroll
-    for (int j = 0; j <= KL; ++j) W[j] = bin(j);
-    for (int k = 1; k <= n - 1; ++k) {
-      const int lmk = min(KL, n - k);
-      const int lp = pivot[k - 1] - k;  // 0 .. lmk
+    for (int j = 0; j <= KL; ++j) W[j] = 0.0;
+    // iteration k (1-based): pivot swap, update of b(k+1 .. k+lmk), emit x(k-1)
+    // (full: an interior iteration, k <= n - KL, so every update applies)
+    auto iter = [&](auto full, int k, int slot, int pos) __attribute__((always_inline)) {
+      constexpr bool FULL = decltype(full)::value;
+      const int lmk = FULL ? KL : min(KL, n - k);
+      const int lp = pivot[k - 1] - k;
+      // swap W[0] and W[lp] with selects (an indexed swap would put W in scratch)
+      const double w0 = W[0];
+      double pv = w0;
+#pragma unroll
+      for (int j = 1; j <= KL; ++j) {
+        const bool s = j == lp;
+        pv = s ? W[j] : pv;
+        W[j] = s ? w0 : W[j];
+      }
+      W[0] = pv;
 #pragma unroll
       for (int j = 1; j <= KL; ++j)
-        if (j == lp) {
-          const double2 t = W[0];
-          W[0] = W[j];
-          W[j] = t;
-        }
-      const double2 t = W[0];
-#pragma unroll
-      for (int j = 1; j <= KL; ++j)
-        if (j <= lmk) {
-          const double a = AB(M + j - 1, k - 1);
-          W[j] = make_double2(W[j].x + t.x * a, W[j].y + t.y * a);
-        }
-      x[k - 1] = W[0];
+        if (FULL || j <= lmk) W[j] += pv * abd[(long)(M + j - 1) * n + (k - 1)];
+      out_at(slot, pos, W[0]);
 #pragma unroll
       for (int j = 0; j < KL; ++j) W[j] = W[j + 1];
-      W[KL] = bin(k + KL);
-    }
-    x[n - 1] = W[0];
-  } else {
-    for (int i = 0; i < n; ++i) x[i] = bin(i);
-  }
-  // back substitution, window V[M-1-j] = b(k-j), k = n .. 1
-  double2 V[M];
+      W[KL] = 0.0;
+    };
+    int flushed = 0;  // output tiles < flushed are stored
+    auto ftile = [&](int t) __attribute__((always_inline)) {
+      __syncthreads();  // the previous tile's readers are done
+      coop_put(pre);
+      __syncthreads();
+      if (t + 1 < ntile) coop_load(rhs, t + 1, true, pre);
+      if (t >= 1 && (t + 1) * kBT <= n) {
+        // interior tile: straight-line iterations (their loads can be issued early)
 #pragma unroll
-  for (int j = 0; j < M; ++j) V[M - 1 - j] = (n - 1 - j >= 0) ? x[n - 1 - j] : make_double2(0.0, 0.0);
-  for (int k = n; k >= 1; --k) {
-    const double d = AB(M - 1, k - 1);
-    V[M - 1] = make_double2(V[M - 1].x / d, V[M - 1].y / d);
-    const int lmk = min(k, M) - 1;
-    const double2 t = make_double2(-V[M - 1].x, -V[M - 1].y);
+        for (int c = 0; c < kBT; ++c) {
+          W[KL] = in_at(c);
+          const int o = t * kBT + c - KL;
+          iter(std::true_type{}, o + 1, (o / kBT) & 1, (c - KL + kBT) % kBT);
+        }
+      } else {
 #pragma unroll
-    for (int j = 1; j < M; ++j)
-      if (j <= lmk) {
-        const double a = AB(M - j - 1, k - 1);
-        V[M - 1 - j] = make_double2(V[M - 1 - j].x + t.x * a, V[M - 1 - j].y + t.y * a);
+      for (int c = 0; c < kBT; ++c) {
+        const int j = t * kBT + c;  // input column entering the window
+        if (j < n) {
+          W[KL] = in_at(c);
+          if (j >= KL) {
+            // iteration k = j - KL + 1 emits column j - KL
+            const int o = j - KL;
+            iter(std::false_type{}, o + 1, (o / kBT) & 1, (c - KL + kBT) % kBT);
+          } else {
+            // filling: b(0 .. KL-1) move down as they arrive
+#pragma unroll
+            for (int q = 0; q < KL; ++q) W[q] = W[q + 1];
+          }
+        }
       }
-    x[k - 1] = fc ? cscale(V[M - 1], fc[k - 1]) : V[M - 1];
+      }
+      // outputs so far cover columns <= hi: store every complete tile (a tile's
+      // ring slot is written again only two input tiles later)
+      const int hi = min(t * kBT + kBT - 1, n - 1) - KL;
+      __syncthreads();
+      while (flushed < ntile && flushed * kBT + kBT - 1 <= hi) {
+        coop_store(flushed & 1, flushed);
+        ++flushed;
+      }
+    };
+    coop_load(rhs, 0, true, pre);
+    for (int t = 0; t < ntile; ++t) ftile(t);
+    // fewer columns than KL: finish the fill with zeros
+    for (int j = n; j < KL; ++j) {
 #pragma unroll
-    for (int j = M - 1; j > 0; --j) V[j] = V[j - 1];
-    V[0] = (k - M - 1 >= 0) ? x[k - M - 1] : make_double2(0.0, 0.0);
+      for (int q = 0; q < KL; ++q) W[q] = W[q + 1];
+      W[KL] = 0.0;
+    }
+    // the last iterations (nothing left to enter), then x(n-1) = W[0]
+    for (int k = max(1, n - KL + 1); k <= n - 1; ++k) {
+      const int o = k - 1;
+      iter(std::false_type{}, k, (o / kBT) & 1, o % kBT);
+    }
+    out_at(((n - 1) / kBT) & 1, (n - 1) % kBT, W[0]);
+    __syncthreads();
+    while (flushed < ntile) {
+      coop_store(flushed & 1, flushed);
+      ++flushed;
+    }
+  }
+
+  // ---- back substitution: x = U^-1 y (columns scaled by fac_col)
+  {
+    const double2* src = (KL > 0) ? sol : rhs;
+    const bool scale_in = (KL == 0);  // without a forward pass the row scaling is applied here
+    double V[M];
+#pragma unroll
+    for (int j = 0; j < M; ++j) V[j] = 0.0;
+    // iteration k (1-based, descending): finish x(k-1), update x(k-2 .. k-1-lmk)
+    // (full: k > M, so every update applies)
+    auto biter = [&](auto full, int k) __attribute__((always_inline)) {
+      constexpr bool FULL = decltype(full)::value;
+      V[M - 1] /= abd[(long)(M - 1) * n + (k - 1)];
+      const int lmk = FULL ? M - 1 : min(k, M) - 1;
+      const double t = -V[M - 1];
+#pragma unroll
+      for (int j = 1; j < M; ++j)
+        if (FULL || j <= lmk) V[M - 1 - j] += t * abd[(long)(M - j - 1) * n + (k - 1)];
+      const int o = k - 1;
+      out_at((o / kBT) % NO, o % kBT, fcl ? V[M - 1] * fcl[o] : V[M - 1]);
+#pragma unroll
+      for (int j = M - 1; j > 0; --j) V[j] = V[j - 1];
+      V[0] = 0.0;
+    };
+    int flushed = ntile;  // output tiles >= flushed are stored
+    auto btile = [&](int u) __attribute__((always_inline)) {
+      __syncthreads();
+      coop_put(pre);
+      __syncthreads();
+      if (u > 0) coop_load(src, u - 1, scale_in, pre);
+      if ((u + 1) * kBT + M <= n) {
+        // interior tile: straight-line iterations (k = col + M + 1 <= n, k > M)
+#pragma unroll
+        for (int c = kBT - 1; c >= 0; --c) {
+          const double v = in_at(c);
+          biter(std::true_type{}, u * kBT + c + M + 1);
+          V[0] = v;
+        }
+      } else {
+#pragma unroll
+      for (int c = kBT - 1; c >= 0; --c) {
+        const int col = u * kBT + c;  // input column entering the window from below
+        if (col < n) {
+          const double v = in_at(c);
+          const int k = col + M + 1;
+          if (k <= n) {
+            biter(std::false_type{}, k);
+            V[0] = v;
+          } else {
+            // initial fill of x(n-1) .. x(n-M), by distance from the top
+            const int pos = M - 1 - (n - 1 - col);
+#pragma unroll
+            for (int j = 0; j < M; ++j)
+              if (j == pos) V[j] = v;
+          }
+        }
+      }
+      }
+      // outputs so far cover columns >= u*kBT + M: store every complete tile
+      __syncthreads();
+      while (flushed - 1 >= 0 && (flushed - 1) * kBT >= u * kBT + M) {
+        --flushed;
+        coop_store(flushed % NO, flushed);
+      }
+    };
+    coop_load(src, ntile - 1, scale_in, pre);
+    for (int u = ntile - 1; u >= 0; --u) btile(u);
+    for (int k = min(M, n); k >= 1; --k) biter(std::false_type{}, k);
+    __syncthreads();
+    while (flushed - 1 >= 0) {
+      --flushed;
+      coop_store(flushed % NO, flushed);
+    }
   }
 }
 
@@ -164,7 +335,7 @@ extern "C" int mh_band_solve(const double* abd, const int* pivot, const double* 
                              const double* rhs, double* sol, int L, int n, int kl, int ku,
                              void* stream) {
   if (L < 1 || n < 1 || kl < 0 || ku < 0 || kl > 4 || ku > 4) return MH_EINVAL;
-  const dim3 g(mh_cdiv(L, 64)), blk(64);
+  const dim3 g(mh_cdiv(L, kBR)), blk(64);  // one wave of 32 rows (two lanes each) per workgroup
   hipStream_t st = mh_stream(stream);
   const double2* r = reinterpret_cast<const double2*>(rhs);
   double2* o = reinterpret_cast<double2*>(sol);
