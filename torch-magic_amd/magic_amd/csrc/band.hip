@@ -2,10 +2,11 @@
 // per-degree systems.  Band storage abd[(2kl+ku+1) x n] (row kl+ku is the
 // diagonal, the first kl rows hold fill-in), multipliers stored negated,
 // pivots 1-based, singular pivot threshold 1e-15.  The factorisation is
-// sequential in the column index by nature; one lane factors one matrix.
-// The solve runs one lane per right-hand-side row (complex), all rows of all
-// degrees in one launch, each row using its degree's factors.  Band widths
-// up to kl, ku = 4 (the 9-band double-curl operator) are supported.
+// sequential in the column index by nature: one wave factors one matrix held
+// in LDS (one lane per matrix when the band does not fit in 64 KB).  The solve
+// runs all right-hand-side rows of all degrees in one launch, each row using
+// its degree's factors, over coalesced LDS tiles.  Band widths up to kl, ku = 4
+// (the 9-band double-curl operator) are supported by the solve.
 #include "common.h"
 
 #include <type_traits>
@@ -70,11 +71,94 @@ __global__ void k_band_factor(double* __restrict__ abd_all, int* __restrict__ pi
   info_all[b] = info;
 }
 
+// The same factorisation with one wave per matrix: the band is held in LDS
+// (loaded and stored coalesced), every lane finds the pivot (broadcast reads),
+// lanes scale the multipliers of column k and update one later column each.
+// Same operations in the same order per element as k_band_factor.
+__global__ __launch_bounds__(64) void k_band_factor_lds(double* __restrict__ abd_all,
+                                                        int* __restrict__ piv_all,
+                                                        int* __restrict__ info_all, int n, int kl,
+                                                        int ku) {
+  extern __shared__ double sb[];  // [nrow][n]
+  const int b = blockIdx.x, lane = threadIdx.x;
+  const int nrow = 2 * kl + ku + 1;
+  const int m = kl + ku + 1;  // 1-based diagonal row
+  double* g = abd_all + (long)b * nrow * n;
+  int* pivot = piv_all + (long)b * n;
+#define SB(i, j) sb[(long)(i) * n + (j)]  // 0-based band row i, column j
+  for (int e = lane; e < nrow * n; e += 64) sb[e] = g[e];
+  __syncthreads();
+  const int j0 = ku + 2, j1 = (n < m ? n : m) - 1;
+  for (int jz = j0; jz <= j1; ++jz)
+    for (int i = m + 1 - jz + lane; i <= kl; i += 64) SB(i - 1, jz - 1) = 0.0;
+  int info = 0, jz = j1, ju = 0;
+  for (int k = 1; k <= n - 1; ++k) {
+    jz = jz + 1;
+    __syncthreads();  // the previous step's updates are visible
+    if (jz <= n)
+      for (int i = lane; i < kl; i += 64) SB(i, jz - 1) = 0.0;
+    __syncthreads();
+    const int lm = kl < (n - k) ? kl : (n - k);
+    // first maximum of |abd(m..m+lm, k)|, found by every lane
+    int l = m;
+    double best = fabs(SB(m - 1, k - 1));
+    for (int i = 1; i <= lm; ++i) {
+      const double v = fabs(SB(m - 1 + i, k - 1));
+      if (v > best) {
+        best = v;
+        l = m + i;
+      }
+    }
+    const int pk = l + k - m;
+    if (lane == 0) pivot[k - 1] = pk;
+    const double pv = SB(l - 1, k - 1), dg = SB(m - 1, k - 1);
+    if (fabs(pv) > 1.0e-15) {  // uniform: every lane read the same column
+      __syncthreads();         // every lane has read the column before it changes
+      if (lane == 0 && l != m) {
+        SB(l - 1, k - 1) = dg;
+        SB(m - 1, k - 1) = pv;
+      }
+      __syncthreads();
+      const double t = -1.0 / pv;
+      for (int i = lane; i < lm; i += 64) SB(m + i, k - 1) *= t;
+      int jmax = ku + pk;
+      if (ju > jmax) jmax = ju;
+      ju = jmax < n ? jmax : n;
+      __syncthreads();  // multipliers in place
+      for (int j = k + 1 + lane; j <= ju; j += 64) {
+        const int ll = l - (j - k), mm = m - (j - k);
+        const double t2 = SB(ll - 1, j - 1);
+        if (ll != mm) {
+          SB(ll - 1, j - 1) = SB(mm - 1, j - 1);
+          SB(mm - 1, j - 1) = t2;
+        }
+        for (int i = 0; i < lm; ++i) SB(mm + i, j - 1) += t2 * SB(m + i, k - 1);
+      }
+    } else {
+      info = k;
+    }
+  }
+  __syncthreads();
+  if (lane == 0) {
+    pivot[n - 1] = n;
+    if (fabs(SB(m - 1, n - 1)) <= 1.0e-15) info = n;
+    info_all[b] = info;
+  }
+  for (int e = lane; e < nrow * n; e += 64) g[e] = sb[e];
+#undef SB
+}
+
 extern "C" int mh_band_factor(double* abd, int* pivot, int* info, int n, int kl, int ku,
                               int nbatch, void* stream) {
   if (n < 1 || kl < 0 || ku < 0 || nbatch < 1) return MH_EINVAL;
-  hipLaunchKernelGGL(k_band_factor, dim3(mh_cdiv(nbatch, 64)), dim3(64), 0, mh_stream(stream),
-                     abd, pivot, info, n, kl, ku, nbatch);
+  const size_t lds = (size_t)(2 * kl + ku + 1) * n * sizeof(double);
+  if (lds <= 64 * 1024) {
+    hipLaunchKernelGGL(k_band_factor_lds, dim3(nbatch), dim3(64), lds, mh_stream(stream), abd,
+                       pivot, info, n, kl, ku);
+  } else {
+    hipLaunchKernelGGL(k_band_factor, dim3(mh_cdiv(nbatch, 64)), dim3(64), 0, mh_stream(stream),
+                       abd, pivot, info, n, kl, ku, nbatch);
+  }
   return mh_launch_status();
 }
 
