@@ -12,6 +12,19 @@
 
 typedef MhRowJob RowJob;
 
+// The job descriptors live in global memory, so the compiler cannot tell that
+// their pointers are global: without this cast every access is a flat access,
+// which counts against both wait counters and forces a full drain per step.
+#define MH_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ const MH_GLOBAL T* gconst(const T* p) {
+  return (const MH_GLOBAL T*)p;
+}
+template <typename T>
+__device__ __forceinline__ MH_GLOBAL T* gmut(T* p) {
+  return (MH_GLOBAL T*)p;
+}
+
 // tiles: (job, row0, col0) triples; WG = 4 waves x (8 rows, 64 columns).
 __global__ __launch_bounds__(256, 4) void k_rowop(const RowJob* __restrict__ jobs,
                                                const int* __restrict__ tiles) {
@@ -24,44 +37,58 @@ __global__ __launch_bounds__(256, 4) void k_rowop(const RowJob* __restrict__ job
   // A operand: row = c16 -> (row index rbase + c16/2, re/im c16&1)
   const int ra = rbase + (c16 >> 1);
   const bool va = ra < J.nrows;
-  const long lma = va ? (J.rows ? J.rows[ra] : ra) : 0;
-  const double* xrow = J.X + lma * 2L * J.ldx + (c16 & 1);
+  const int lma = va ? (J.rows ? gconst(J.rows)[ra] : ra) : 0;
+  // uniform bases (scalar registers) and 32-bit per-lane element offsets
+  // (a field or RHS array holds fewer than 2^31 doubles)
+  const MH_GLOBAL double* xb = gconst(J.X);
+  const int xo = lma * 2 * J.ldx + (c16 & 1);
   double4_t acc[4];
 #pragma unroll
   for (int q = 0; q < 4; ++q) acc[q] = (double4_t){0.0, 0.0, 0.0, 0.0};
-  const double* mt = J.MT + col0 + c16;
-  // operands of k-step k0 (4 k values); the next step's are loaded before the
-  // current step's MFMAs are issued
-  auto load = [&](int k0, double& av, double (&bv)[4]) {
-    const int k = k0 + kk;
-    av = (va && k < J.K) ? xrow[2 * k] : 0.0;
-    const double* mk = mt + (long)k * J.Rpad;  // MT rows are padded to a multiple of 4
+  const MH_GLOBAL double* mtb = gconst(J.MT) + col0;
+  const int mo = kk * J.Rpad + c16;
+  // operands of k-step t (k = 4t + kk) stream through a ring of D steps: the
+  // loads are unconditional (k clamped to the last row; MT rows are padded to
+  // a multiple of 4) and a slot is refilled right after its MFMAs, so each
+  // MFMA waits only for loads issued D steps earlier; rows past K or past the
+  // job's rows are zeroed where the value is used
+  constexpr int D = 4;
+  const int nst = (J.K + 3) / 4;
+  double xa[D], rb[D][4];
+  auto load = [&](int t, double& av, double (&bv)[4]) __attribute__((always_inline)) {
+    av = xb[xo + 2 * min(4 * t + kk, J.K - 1)];
+    const int m = mo + 4 * t * J.Rpad;  // t is uniform: one scalar multiply per step
 #pragma unroll
-    for (int q = 0; q < 4; ++q) bv[q] = mk[16 * q];
+    for (int q = 0; q < 4; ++q) bv[q] = mtb[m + 16 * q];
   };
-  double av, bv[4];
-  load(0, av, bv);
-  for (int k0 = 0; k0 < J.K; k0 += 4) {
-    double an = av, bn[4] = {bv[0], bv[1], bv[2], bv[3]};
-    if (k0 + 4 < J.K) load(k0 + 4, an, bn);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) acc[q] = mfma_f64(av, bv[q], acc[q]);
-    av = an;
+  for (int d = 0; d < D; ++d) load(min(d, nst - 1), xa[d], rb[d]);
+  auto step = [&](int t, int d) __attribute__((always_inline)) {
+    const double a = (va && 4 * t + kk < J.K) ? xa[d] : 0.0;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) bv[q] = bn[q];
+    for (int q = 0; q < 4; ++q) acc[q] = mfma_f64(a, rb[d][q], acc[q]);
+    load(min(t + D, nst - 1), xa[d], rb[d]);
+  };
+  int t0 = 0;
+  for (; t0 + D <= nst; t0 += D) {  // full groups: straight-line steps
+#pragma unroll
+    for (int d = 0; d < D; ++d) step(t0 + d, d);
   }
+#pragma unroll
+  for (int d = 0; d < D; ++d)
+    if (t0 + d < nst) step(t0 + d, d);
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const int row = mfma_row(lane, i);  // 16 rows = 8 lm rows x (re, im)
     const int ro = rbase + (row >> 1);
     if (ro >= J.nrows) continue;
-    const long lmo = J.rows ? J.rows[ro] : ro;
+    const long lmo = J.rows ? gconst(J.rows)[ro] : ro;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int r = col0 + 16 * q + c16;
       if (r < J.R) {
         const int seg = r / J.rseg, rc = r - seg * J.rseg;
-        J.Y[seg][lmo * 2L * J.ldy + 2 * rc + (row & 1)] = acc[q][i];
+        gmut(J.Y[seg])[lmo * 2L * J.ldy + 2 * rc + (row & 1)] = acc[q][i];
       }
     }
   }
