@@ -137,7 +137,10 @@ __device__ __forceinline__ SynIn syn_in(const SynArgs& a, int g, int c) {
 // table operands from global memory through a ring of kSynDepth steps.
 constexpr int kSynWaves = 6, kSynJ = 16 * kSynWaves, kChunk = 32, kSynDepth = 4;
 
-template <bool VEC, typename T>
+// NCT column tiles of 8 complex columns per workgroup: every table operand
+// feeds NCT MFMAs (the launches use one tile; two, which FP32 accumulators
+// allow, measured no faster on MI355X).
+template <bool VEC, typename T, int NCT>
 __global__ __launch_bounds__(64 * kSynWaves, 3) void k_syn(SynArgs a, int n_m) {
   typedef Mfma<T> MF;
   typedef typename MF::acc_t acc_t;
@@ -145,32 +148,34 @@ __global__ __launch_bounds__(64 * kSynWaves, 3) void k_syn(SynArgs a, int n_m) {
   constexpr int NT = 64 * kSynWaves;        // threads
   constexpr int SPC = kChunk / 4;           // MFMA steps per chunk
   constexpr int D = kSynDepth;
-  static_assert(NT % 16 == 0 && SPC % D == 0, "staging / ring layout");
-  __shared__ T xs[2][NF][kChunk][16];
+  constexpr int CW = 16 * NCT;              // real columns per workgroup
+  constexpr int RPT = NT / CW;              // chunk rows staged per pass
+  constexpr int NPS = (kChunk + RPT - 1) / RPT;  // staging passes
+  constexpr int NA = (VEC ? 10 : 2) * NCT;  // accumulators
+  static_assert(NT % CW == 0 && SPC % D == 0, "staging / ring layout");
+  __shared__ T xs[2][NF][kChunk][CW];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   // XCD-aware order: workgroups b and b+8 run on the same XCD; XCD x takes the
   // orders m = x, x+8, ... (all tiles of one m on one XCD, so the m's tables
   // are fetched into that XCD's L2 once), which also balances the K ~ l_max-m work
-  const int nx = (a.ngroups * a.B + 7) / 8, ny = (a.H + kSynJ - 1) / kSynJ;
+  const int nx = (a.ngroups * a.B + 8 * NCT - 1) / (8 * NCT), ny = (a.H + kSynJ - 1) / kSynJ;
   const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
   const int m = (slot / (nx * ny)) * 8 + xcd;
   if (m >= n_m) return;  // uniform per workgroup
   const int rem = slot % (nx * ny), ty = rem / nx, tx = rem - ty * nx;
-  const int bc0 = tx * 8;
+  const int bc0 = tx * 8 * NCT;
   const int j0 = ty * kSynJ + 16 * w;
   const int col = lane & 15, kk = lane >> 4;
   const long lms = a.lm_start[m];
   const int ncol = a.ngroups * a.B;
-  // staging map: thread stages real column c16 = tid % 16 of chunk rows
-  // tid / 16 (slot 0) and 24 + tid / 16 (slot 1, tid < 128) of every field
-  static_assert(NT == 384 && kChunk == 32, "staging map assumes 384 threads, 32-row chunks");
-  const int c16 = threadIdx.x & 15, srow = threadIdx.x >> 4;
-  const bool s2 = threadIdx.x < 128;
-  const int sbc = bc0 + (c16 >> 1);
+  // staging map: thread stages real column cw = tid % CW of chunk rows
+  // tid / CW + RPT*q (q < NPS, rows < kChunk) of every field
+  const int cw = threadIdx.x % CW, srow = threadIdx.x / CW;
+  const int sbc = bc0 + (cw >> 1);
   const bool scol = sbc < ncol;
   const int sg = scol ? sbc / a.B : 0;
   const int sb = scol ? sbc - sg * a.B : 0;
-  const long soff = 2L * sb + (c16 & 1);
+  const long soff = 2L * sb + (cw & 1);
   const double sor2 = scol ? a.or2[sb] : 0.0;
   // this thread's input pointers (column offset folded in) and modes, resolved
   // once: per-lane selection inside the chunk loop would re-read the argument
@@ -185,12 +190,12 @@ __global__ __launch_bounds__(64 * kSynWaves, 3) void k_syn(SynArgs a, int n_m) {
     if (f == 2) sx2 = in.x2 + soff;
     smodes |= in.mode << (2 * f);
   }
-  acc_t acc[VEC ? 10 : 2];
+  acc_t acc[NA];  // acc[(5p + q) * NCT + ct] (vector), acc[p * NCT + ct] (scalar)
 #pragma unroll
-  for (int q = 0; q < (VEC ? 10 : 2); ++q) acc[q] = (acc_t){0, 0, 0, 0};
-  double r1[NF][2], r2[2];  // raw inputs of the next chunk (r2: second input of T)
+  for (int q = 0; q < NA; ++q) acc[q] = (acc_t){0, 0, 0, 0};
+  double r1[NF][NPS], r2[NPS];  // raw inputs of the next chunk (r2: second input of T)
   const bool wact = j0 < a.H;  // uniform per wave; then j0 + 15 < Hp
-#pragma unroll  // static accumulator indices (acc[5p + q] stays in registers)
+#pragma unroll  // static accumulator indices (acc[] stays in registers)
   for (int p = 0; p < 2; ++p) {
     const int* info = a.mp_info + (m * 2 + p) * 4;
     const int K = info[0];
@@ -201,11 +206,11 @@ __global__ __launch_bounds__(64 * kSynWaves, 3) void k_syn(SynArgs a, int n_m) {
 #pragma unroll
       for (int f = 0; f < NF; ++f) {
 #pragma unroll
-        for (int q = 0; q < 2; ++q) {
-          const int k = c0 + 24 * q + srow;
+        for (int q = 0; q < NPS; ++q) {
+          const int kr = srow + RPT * q, k = c0 + kr;
           r1[f][q] = 0.0;
           if (f == 2) r2[q] = 0.0;
-          if ((q == 0 || s2) && scol && k < K) {
+          if (kr < kChunk && scol && k < K) {
             const long o = (lms + p + 2 * k) * 2 * (long)a.ldx;
             r1[f][q] = sx1[f][o];
             if (f == 2 && ((smodes >> 4) & 3) == 2) r2[q] = sx2[o];
@@ -218,14 +223,14 @@ __global__ __launch_bounds__(64 * kSynWaves, 3) void k_syn(SynArgs a, int n_m) {
       for (int f = 0; f < NF; ++f) {
         const int mode = (smodes >> (2 * f)) & 3;
 #pragma unroll
-        for (int q = 0; q < 2; ++q) {
-          if (q == 0 || s2) {
-            const int kr = 24 * q + srow;
+        for (int q = 0; q < NPS; ++q) {
+          const int kr = srow + RPT * q;
+          if (kr < kChunk) {
             const int l = m + p + 2 * (c0 + kr);
             const double dl = (double)l * (double)(l + 1);
             const double x1 = r1[f][q], x2 = (f == 2) ? r2[q] : 0.0;
             const double x = mode == 0 ? x1 : mode == 1 ? dl * x1 : -x1 + sor2 * dl * x2;
-            xs[buf][f][kr][c16] = (T)x;
+            xs[buf][f][kr][cw] = (T)x;
           }
         }
       }
@@ -260,17 +265,21 @@ __global__ __launch_bounds__(64 * kSynWaves, 3) void k_syn(SynArgs a, int n_m) {
       auto step = [&](const int s, const int t) {
         const T aP = rP[s % D], aD = rD[s % D];
         const int kr = 4 * s + kk;
-        const T xq = xs[buf][0][kr][col];
-        if (!VEC) {
-          acc[p] = MF::run(aP, xq, acc[p]);
-        } else {
-          const T xsv = xs[buf][1][kr][col];
-          const T xt = xs[buf][2][kr][col];
-          acc[5 * p + 0] = MF::run(aP, xq, acc[5 * p + 0]);   // P Q
-          acc[5 * p + 1] = MF::run(aP, xt, acc[5 * p + 1]);   // P T
-          acc[5 * p + 2] = MF::run(aP, xsv, acc[5 * p + 2]);  // P S
-          acc[5 * p + 3] = MF::run(aD, xsv, acc[5 * p + 3]);  // dP S
-          acc[5 * p + 4] = MF::run(aD, xt, acc[5 * p + 4]);   // dP T
+#pragma unroll
+        for (int ct = 0; ct < NCT; ++ct) {
+          const int cc = 16 * ct + col;
+          const T xq = xs[buf][0][kr][cc];
+          if (!VEC) {
+            acc[p * NCT + ct] = MF::run(aP, xq, acc[p * NCT + ct]);
+          } else {
+            const T xsv = xs[buf][1][kr][cc];
+            const T xt = xs[buf][2][kr][cc];
+            acc[(5 * p + 0) * NCT + ct] = MF::run(aP, xq, acc[(5 * p + 0) * NCT + ct]);   // P Q
+            acc[(5 * p + 1) * NCT + ct] = MF::run(aP, xt, acc[(5 * p + 1) * NCT + ct]);   // P T
+            acc[(5 * p + 2) * NCT + ct] = MF::run(aP, xsv, acc[(5 * p + 2) * NCT + ct]);  // P S
+            acc[(5 * p + 3) * NCT + ct] = MF::run(aD, xsv, acc[(5 * p + 3) * NCT + ct]);  // dP S
+            acc[(5 * p + 4) * NCT + ct] = MF::run(aD, xt, acc[(5 * p + 4) * NCT + ct]);   // dP T
+          }
         }
         const long tn = (long)min(t + D, nsteps - 1) * tstep;
         rP[s % D] = pP[tn];
@@ -297,9 +306,13 @@ __global__ __launch_bounds__(64 * kSynWaves, 3) void k_syn(SynArgs a, int n_m) {
     }
   }
   const double dm = (double)m;
-  const int bc = bc0 + (col >> 1), reim = col & 1;
+  const int reim = col & 1;
+#pragma unroll
+  for (int ct = 0; ct < NCT; ++ct) {
+  const int bc = bc0 + 8 * ct + (col >> 1);
   const bool bval = bc < ncol;
   const int bo = bval ? bc : 0;
+  auto A = [&](int q, int i) { return (double)acc[q * NCT + ct][i]; };
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const int j = j0 + MF::row(lane, i);
@@ -307,17 +320,17 @@ __global__ __launch_bounds__(64 * kSynWaves, 3) void k_syn(SynArgs a, int n_m) {
     const long oN = fidx(a.c_stride, a.m_stride, a.Bout, 0, m, 2 * j, bo) + reim;
     const long oS = fidx(a.c_stride, a.m_stride, a.Bout, 0, m, 2 * j + 1, bo) + reim;
     if (!VEC) {
-      const double e = acc[0][i], o = acc[1][i];
+      const double e = A(0, i), o = A(1, i);
       if (st) {
         a.F[oN] = e + o;
         a.F[oS] = e - o;
       }
     } else {
-      const double Qe = acc[0][i], Qo = acc[5][i];
-      const double PTe = acc[1][i], PTo = acc[6][i];
-      const double PSe = acc[2][i], PSo = acc[7][i];
-      const double DSe = acc[3][i], DSo = acc[8][i];
-      const double DTe = acc[4][i], DTo = acc[9][i];
+      const double Qe = A(0, i), Qo = A(5, i);
+      const double PTe = A(1, i), PTo = A(6, i);
+      const double PSe = A(2, i), PSo = A(7, i);
+      const double DSe = A(3, i), DSo = A(8, i);
+      const double DTe = A(4, i), DTo = A(9, i);
       const double iPTp = imul_pair(PTe + PTo, dm, lane), iPTm = imul_pair(PTe - PTo, dm, lane);
       const double iPSp = imul_pair(PSe + PSo, dm, lane), iPSm = imul_pair(PSe - PSo, dm, lane);
       if (st) {
@@ -329,6 +342,7 @@ __global__ __launch_bounds__(64 * kSynWaves, 3) void k_syn(SynArgs a, int n_m) {
         a.F[2 * a.c_stride + oS] = iPSm + (DTe - DTo);
       }
     }
+  }
   }
 }
 
@@ -343,17 +357,19 @@ extern "C" int mh_leg_synth(int vector, const MhSynArgs* args, int n_m, void* st
     for (int f = 0; f < (vector ? 3 : 1); ++f)
       if (a.in[g][f].mode < 0 || a.in[g][f].mode > 2 || (a.in[g][f].mode == 2 && !(vector && f == 2)))
         return MH_EINVAL;
-  const long units = (long)mh_cdiv(a.ngroups * a.B, 8) * mh_cdiv(a.H, kSynJ) * mh_cdiv(n_m, 8) * 8;
+  const int nct = 1;  // column tiles per workgroup (two measured no faster for FP32)
+  const long units =
+      (long)mh_cdiv(a.ngroups * a.B, 8 * nct) * mh_cdiv(a.H, kSynJ) * mh_cdiv(n_m, 8) * 8;
   if (units > 0x7fffffffL) return MH_EINVAL;
   dim3 grid((unsigned)units);
   const dim3 blk(64 * kSynWaves);
   hipStream_t st = mh_stream(stream);
   if (a.prec == 0) {
-    if (vector) hipLaunchKernelGGL((k_syn<true, double>), grid, blk, 0, st, a, n_m);
-    else hipLaunchKernelGGL((k_syn<false, double>), grid, blk, 0, st, a, n_m);
+    if (vector) hipLaunchKernelGGL((k_syn<true, double, 1>), grid, blk, 0, st, a, n_m);
+    else hipLaunchKernelGGL((k_syn<false, double, 1>), grid, blk, 0, st, a, n_m);
   } else {
-    if (vector) hipLaunchKernelGGL((k_syn<true, float>), grid, blk, 0, st, a, n_m);
-    else hipLaunchKernelGGL((k_syn<false, float>), grid, blk, 0, st, a, n_m);
+    if (vector) hipLaunchKernelGGL((k_syn<true, float, 1>), grid, blk, 0, st, a, n_m);
+    else hipLaunchKernelGGL((k_syn<false, float, 1>), grid, blk, 0, st, a, n_m);
   }
   return mh_launch_status();
 }
