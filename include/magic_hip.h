@@ -56,7 +56,9 @@ typedef struct {
   double* F;
   long c_stride, m_stride;
   int Bout;
-  int prec; /* 0: FP64 tables and MFMA; 1: FP32 tables (synP/synD hold floats), FP32 MFMA */
+  int prec;  /* 0: FP64 tables and MFMA; 1: FP32 tables (synP/synD hold floats), FP32 MFMA */
+  int fprec; /* 0: F holds doubles; 1: F holds floats (FP32 Fourier buffers, needs prec = 1) */
+  int pad;
 } MhSynArgs;
 
 /* Legendre synthesis on FP64 MFMA: scalar (vector=0) replaces the bmm part of
@@ -79,7 +81,9 @@ typedef struct {
   const double* anaWd;
   const int* tiles; /* (m, p, k0) triples, k0 in steps of 96 degrees (one workgroup each) */
   int ntiles;
-  int prec; /* as MhSynArgs.prec (anaW/anaWd hold floats for 1) */
+  int prec;  /* as MhSynArgs.prec (anaW/anaWd hold floats for 1) */
+  int fprec; /* as MhSynArgs.fprec: F holds floats for 1 (needs prec = 1) */
+  int pad;
 } MhAnaArgs;
 
 /* Legendre analysis on FP64 MFMA: scalar replaces the bmm part of scal_to_SH
@@ -141,7 +145,7 @@ typedef struct {
   double* Fo;       /* [9][M][T][N-2] complex: Advr VSr VxBr Advt Advp VSt VSp VxBt VxBp */
   int N, T, M, Mh;
   int generic; /* 1: run the runtime-plan kernel even where a compiled plan exists */
-  int pad;
+  int fprec;   /* 0: Fs/Fv/Fo and the transforms in FP64; 1: FP32 (the pointers hold floats) */
   const double* tw;  /* exp(-2 pi i k/Mh) */
   const double* tw2; /* exp(-2 pi i k/(2Mh)) */
   const double *or2, *or4, *ost2, *valri2, *valhi2;

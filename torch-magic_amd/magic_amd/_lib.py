@@ -32,14 +32,14 @@ class SynIn(C.Structure):
 class SynArgs(C.Structure):
     _fields_ = [("in_", (SynIn * 3) * 4), ("ngroups", I), ("B", I), ("ldx", I), ("H", I),
                 ("Hp", I), ("mp_info", P), ("lm_start", P), ("synP", P), ("synD", P),
-                ("or2", P), ("F", P), ("c_stride", L), ("m_stride", L), ("Bout", I), ("prec", I)]
+                ("or2", P), ("F", P), ("c_stride", L), ("m_stride", L), ("Bout", I), ("prec", I), ("fprec", I), ("pad", I)]
 
 
 class AnaArgs(C.Structure):
     _fields_ = [("F", P), ("c_stride", L), ("m_stride", L), ("Bin", I), ("B", I), ("ncomp", I),
                 ("cf1", I * 3), ("cf2", I * 3), ("pad0", I), ("outS", P * 3), ("outT", P * 3),
                 ("ldo", I), ("H", I), ("Hp", I), ("n_m", I), ("mp_info", P), ("lm_start", P),
-                ("anaW", P), ("anaWd", P), ("tiles", P), ("ntiles", I), ("prec", I)]
+                ("anaW", P), ("anaWd", P), ("tiles", P), ("ntiles", I), ("prec", I), ("fprec", I), ("pad", I)]
 
 
 class FftArgs(C.Structure):
@@ -58,7 +58,7 @@ class NlArgs(C.Structure):
 
 class GridArgs(C.Structure):
     _fields_ = [("Fs", P), ("Fv", P), ("Fo", P), ("N", I), ("T", I), ("M", I), ("Mh", I),
-                ("generic", I), ("pad", I), ("tw", P), ("tw2", P), ("or2", P),
+                ("generic", I), ("fprec", I), ("tw", P), ("tw2", P), ("or2", P),
                 ("or4", P), ("ost2", P), ("valri2", P), ("valhi2", P), ("LFfac", D), ("cf2", D),
                 ("af2", D), ("l_mag", I), ("alf_damp", I), ("cfl", P)]
 

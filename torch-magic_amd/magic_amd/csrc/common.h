@@ -41,6 +41,22 @@ __device__ __forceinline__ double2 cimul(double2 z, double m) {
   return make_double2(-m * z.y, m * z.x);
 }
 
+// Complex helpers in FP32 (the mixed-precision grid step and Fourier buffers).
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+__device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+
+// Complex vector type of a real type, and its constructor.
+template <typename T> struct vec2;
+template <> struct vec2<double> { typedef double2 type; };
+template <> struct vec2<float> { typedef float2 type; };
+__device__ __forceinline__ double2 mk2(double x, double y) { return make_double2(x, y); }
+__device__ __forceinline__ float2 mk2(float x, float y) { return make_float2(x, y); }
+
 // Row of the 16x16 FP64 accumulator held in element v of this lane.
 __device__ __forceinline__ int mfma_row(int lane, int v) { return (lane >> 4) + 4 * v; }
 

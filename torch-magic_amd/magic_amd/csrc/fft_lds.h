@@ -4,42 +4,47 @@
 
 #include "common.h"
 
-template <int R>
-__device__ __forceinline__ void dft(double2* v, double dir) {
+// In-register DFT of R points, X_k = sum_n x_n exp(dir * 2 pi i nk / R), on
+// complex values of type V (double2 or float2; the constants are rounded to
+// the element type, so an FP32 transform stays in FP32).
+template <int R, typename V>
+__device__ __forceinline__ void dft(V* v, double dir) {
+  typedef decltype(v[0].x) S;
   if (R == 2) {
-    const double2 a = v[0], b = v[1];
+    const V a = v[0], b = v[1];
     v[0] = cadd(a, b);
     v[1] = csub(a, b);
   } else if (R == 3) {
-    const double c1 = -0.5, s1 = dir * 0.86602540378443864676372317075294;
-    const double2 a = v[0], b = v[1], c = v[2];
-    const double2 bc = cadd(b, c), bmc = csub(b, c);
-    const double2 t1 = make_double2(a.x + c1 * bc.x, a.y + c1 * bc.y);
-    const double2 t2 = make_double2(-s1 * bmc.y, s1 * bmc.x);  // i*s1*(b-c)
+    const S c1 = (S)-0.5, s1 = (S)(dir * 0.86602540378443864676372317075294);
+    const V a = v[0], b = v[1], c = v[2];
+    const V bc = cadd(b, c), bmc = csub(b, c);
+    const V t1 = mk2(a.x + c1 * bc.x, a.y + c1 * bc.y);
+    const V t2 = mk2(-s1 * bmc.y, s1 * bmc.x);  // i*s1*(b-c)
     v[0] = cadd(a, bc);
     v[1] = cadd(t1, t2);
     v[2] = csub(t1, t2);
   } else if (R == 4) {
-    const double2 a = v[0], b = v[1], c = v[2], d = v[3];
-    const double2 apc = cadd(a, c), amc = csub(a, c), bpd = cadd(b, d), bmd = csub(b, d);
-    const double2 ibmd = make_double2(-dir * bmd.y, dir * bmd.x);  // i*dir*(b-d)
+    const S d = (S)dir;
+    const V a = v[0], b = v[1], c = v[2], e = v[3];
+    const V apc = cadd(a, c), amc = csub(a, c), bpd = cadd(b, e), bmd = csub(b, e);
+    const V ibmd = mk2(-d * bmd.y, d * bmd.x);  // i*dir*(b-d)
     v[0] = cadd(apc, bpd);
     v[1] = cadd(amc, ibmd);
     v[2] = csub(apc, bpd);
     v[3] = csub(amc, ibmd);
   } else if (R == 5) {
-    const double c1 = 0.30901699437494742410229341718282, c2 = -0.80901699437494742410229341718282;
-    const double s1 = dir * 0.95105651629515357211643933337938;
-    const double s2 = dir * 0.58778525229247312916870595463907;
-    const double2 a = v[0];
-    const double2 t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
-    const double2 t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
-    const double2 r1 = make_double2(a.x + c1 * t1.x + c2 * t2.x, a.y + c1 * t1.y + c2 * t2.y);
-    const double2 r2 = make_double2(a.x + c2 * t1.x + c1 * t2.x, a.y + c2 * t1.y + c1 * t2.y);
-    const double2 q1 = make_double2(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y);
-    const double2 q2 = make_double2(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y);
-    const double2 iq1 = make_double2(-q1.y, q1.x), iq2 = make_double2(-q2.y, q2.x);
-    v[0] = make_double2(a.x + t1.x + t2.x, a.y + t1.y + t2.y);
+    const S c1 = (S)0.30901699437494742410229341718282, c2 = (S)-0.80901699437494742410229341718282;
+    const S s1 = (S)(dir * 0.95105651629515357211643933337938);
+    const S s2 = (S)(dir * 0.58778525229247312916870595463907);
+    const V a = v[0];
+    const V t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
+    const V t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
+    const V r1 = mk2(a.x + c1 * t1.x + c2 * t2.x, a.y + c1 * t1.y + c2 * t2.y);
+    const V r2 = mk2(a.x + c2 * t1.x + c1 * t2.x, a.y + c2 * t1.y + c1 * t2.y);
+    const V q1 = mk2(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y);
+    const V q2 = mk2(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y);
+    const V iq1 = mk2(-q1.y, q1.x), iq2 = mk2(-q2.y, q2.x);
+    v[0] = mk2(a.x + t1.x + t2.x, a.y + t1.y + t2.y);
     v[1] = cadd(r1, iq1);
     v[4] = csub(r1, iq1);
     v[2] = cadd(r2, iq2);

@@ -140,8 +140,9 @@ constexpr int kSynWaves = 6, kSynJ = 16 * kSynWaves, kChunk = 32, kSynDepth = 4;
 // NCT column tiles of 8 complex columns per workgroup: every table operand
 // feeds NCT MFMAs (the launches use one tile; two, which FP32 accumulators
 // allow, measured no faster on MI355X).
-template <bool VEC, typename T, int NCT>
+template <bool VEC, typename T, int NCT, typename FT>
 __global__ __launch_bounds__(64 * kSynWaves, 3) void k_syn(SynArgs a, int n_m) {
+  FT* const Fo = reinterpret_cast<FT*>(a.F);  // Fourier buffer elements (double or float)
   typedef Mfma<T> MF;
   typedef typename MF::acc_t acc_t;
   constexpr int NF = VEC ? 3 : 1;           // fields per group (Q, S, T)
@@ -322,8 +323,8 @@ __global__ __launch_bounds__(64 * kSynWaves, 3) void k_syn(SynArgs a, int n_m) {
     if (!VEC) {
       const double e = A(0, i), o = A(1, i);
       if (st) {
-        a.F[oN] = e + o;
-        a.F[oS] = e - o;
+        Fo[oN] = (FT)(e + o);
+        Fo[oS] = (FT)(e - o);
       }
     } else {
       const double Qe = A(0, i), Qo = A(5, i);
@@ -334,12 +335,12 @@ __global__ __launch_bounds__(64 * kSynWaves, 3) void k_syn(SynArgs a, int n_m) {
       const double iPTp = imul_pair(PTe + PTo, dm, lane), iPTm = imul_pair(PTe - PTo, dm, lane);
       const double iPSp = imul_pair(PSe + PSo, dm, lane), iPSm = imul_pair(PSe - PSo, dm, lane);
       if (st) {
-        a.F[oN] = Qe + Qo;
-        a.F[oS] = Qe - Qo;
-        a.F[a.c_stride + oN] = (DSe + DSo) + iPTp;
-        a.F[a.c_stride + oS] = -(DSe - DSo) + iPTm;
-        a.F[2 * a.c_stride + oN] = iPSp - (DTe + DTo);
-        a.F[2 * a.c_stride + oS] = iPSm + (DTe - DTo);
+        Fo[oN] = (FT)(Qe + Qo);
+        Fo[oS] = (FT)(Qe - Qo);
+        Fo[a.c_stride + oN] = (FT)((DSe + DSo) + iPTp);
+        Fo[a.c_stride + oS] = (FT)(-(DSe - DSo) + iPTm);
+        Fo[2 * a.c_stride + oN] = (FT)(iPSp - (DTe + DTo));
+        Fo[2 * a.c_stride + oS] = (FT)(iPSm + (DTe - DTo));
       }
     }
   }
@@ -352,6 +353,7 @@ extern "C" int mh_leg_synth(int vector, const MhSynArgs* args, int n_m, void* st
       a.Bout != a.ngroups * a.B)
     return MH_EINVAL;
   if (a.prec != 0 && a.prec != 1) return MH_EINVAL;
+  if (a.fprec != 0 && (a.fprec != 1 || a.prec != 1)) return MH_EINVAL;
   // mode 2 (two inputs) is supported for the T field of a vector group only
   for (int g = 0; g < a.ngroups; ++g)
     for (int f = 0; f < (vector ? 3 : 1); ++f)
@@ -365,11 +367,14 @@ extern "C" int mh_leg_synth(int vector, const MhSynArgs* args, int n_m, void* st
   const dim3 blk(64 * kSynWaves);
   hipStream_t st = mh_stream(stream);
   if (a.prec == 0) {
-    if (vector) hipLaunchKernelGGL((k_syn<true, double, 1>), grid, blk, 0, st, a, n_m);
-    else hipLaunchKernelGGL((k_syn<false, double, 1>), grid, blk, 0, st, a, n_m);
+    if (vector) hipLaunchKernelGGL((k_syn<true, double, 1, double>), grid, blk, 0, st, a, n_m);
+    else hipLaunchKernelGGL((k_syn<false, double, 1, double>), grid, blk, 0, st, a, n_m);
+  } else if (a.fprec == 0) {
+    if (vector) hipLaunchKernelGGL((k_syn<true, float, 1, double>), grid, blk, 0, st, a, n_m);
+    else hipLaunchKernelGGL((k_syn<false, float, 1, double>), grid, blk, 0, st, a, n_m);
   } else {
-    if (vector) hipLaunchKernelGGL((k_syn<true, float, 1>), grid, blk, 0, st, a, n_m);
-    else hipLaunchKernelGGL((k_syn<false, float, 1>), grid, blk, 0, st, a, n_m);
+    if (vector) hipLaunchKernelGGL((k_syn<true, float, 1, float>), grid, blk, 0, st, a, n_m);
+    else hipLaunchKernelGGL((k_syn<false, float, 1, float>), grid, blk, 0, st, a, n_m);
   }
   return mh_launch_status();
 }
@@ -386,8 +391,9 @@ typedef MhAnaArgs AnaArgs;
 // from global memory.
 constexpr int kAnaWaves = 6;
 
-template <bool VEC, typename T>
+template <bool VEC, typename T, typename FT>
 __global__ __launch_bounds__(64 * kAnaWaves, 2) void k_ana(AnaArgs a, int n_m) {
+  const FT* const Fi = reinterpret_cast<const FT*>(a.F);  // Fourier buffer elements
   typedef Mfma<T> MF;
   typedef typename MF::acc_t acc_t;
   constexpr int NF = VEC ? 4 : 1;          // f1 same, f1 opposite, f2 same, f2 opposite
@@ -424,8 +430,8 @@ __global__ __launch_bounds__(64 * kAnaWaves, 2) void k_ana(AnaArgs a, int n_m) {
   const bool s2 = threadIdx.x < 128;
   const int sb = bc0 + (c16 >> 1);
   const bool scol = sb < a.B;
-  const double* f1p = a.F + c1 * a.c_stride + fidx(0, a.m_stride, a.Bin, 0, m, 0, scol ? sb : 0) + (c16 & 1);
-  const double* f2p = a.F + c2 * a.c_stride + fidx(0, a.m_stride, a.Bin, 0, m, 0, scol ? sb : 0) + (c16 & 1);
+  const FT* f1p = Fi + c1 * a.c_stride + fidx(0, a.m_stride, a.Bin, 0, m, 0, scol ? sb : 0) + (c16 & 1);
+  const FT* f2p = Fi + c2 * a.c_stride + fidx(0, a.m_stride, a.Bin, 0, m, 0, scol ? sb : 0) + (c16 & 1);
   const long rstride = 2L * a.Bin;  // one grid row t
   double raw[2][VEC ? 4 : 2];       // (f1N, f1S, f2N, f2S) of the next chunk
   auto fetch = [&](int j0) {
@@ -558,13 +564,17 @@ extern "C" int mh_leg_analysis(int vector, const MhAnaArgs* args, void* stream) 
   dim3 grid((unsigned)(a.ncomp * mh_cdiv(a.B, 8) * a.ntiles));
   const dim3 blk(64 * kAnaWaves);
   if (a.prec != 0 && a.prec != 1) return MH_EINVAL;
+  if (a.fprec != 0 && (a.fprec != 1 || a.prec != 1)) return MH_EINVAL;
   hipStream_t st = mh_stream(stream);
   if (a.prec == 0) {
-    if (vector) hipLaunchKernelGGL((k_ana<true, double>), grid, blk, 0, st, a, n_m);
-    else hipLaunchKernelGGL((k_ana<false, double>), grid, blk, 0, st, a, n_m);
+    if (vector) hipLaunchKernelGGL((k_ana<true, double, double>), grid, blk, 0, st, a, n_m);
+    else hipLaunchKernelGGL((k_ana<false, double, double>), grid, blk, 0, st, a, n_m);
+  } else if (a.fprec == 0) {
+    if (vector) hipLaunchKernelGGL((k_ana<true, float, double>), grid, blk, 0, st, a, n_m);
+    else hipLaunchKernelGGL((k_ana<false, float, double>), grid, blk, 0, st, a, n_m);
   } else {
-    if (vector) hipLaunchKernelGGL((k_ana<true, float>), grid, blk, 0, st, a, n_m);
-    else hipLaunchKernelGGL((k_ana<false, float>), grid, blk, 0, st, a, n_m);
+    if (vector) hipLaunchKernelGGL((k_ana<true, float, float>), grid, blk, 0, st, a, n_m);
+    else hipLaunchKernelGGL((k_ana<false, float, float>), grid, blk, 0, st, a, n_m);
   }
   return mh_launch_status();
 }

@@ -128,6 +128,7 @@ class Resolution:
         else:
             a.synP, a.synD = ptr(self.synP), ptr(self.synD)
         a.prec = prec
+        a.fprec = 1 if F.dtype == torch.float32 else 0  # FP32 Fourier buffer (needs prec 1)
         a.or2 = ptr(or2 if or2 is not None else self.d["or2"])
         a.F = ptr(F)
         a.c_stride, a.m_stride = self.fstrides(Bout)
@@ -142,6 +143,7 @@ class Resolution:
         """
         a = _lib.AnaArgs()
         a.F = ptr(F)
+        a.fprec = 1 if F.dtype == torch.float32 else 0  # FP32 Fourier buffer (needs prec 1)
         a.c_stride, a.m_stride = self.fstrides(Bin)
         a.Bin, a.B, a.ncomp = Bin, B, len(comps)
         for i, c in enumerate(comps):
@@ -182,8 +184,10 @@ class Resolution:
         a.ost2 = ptr(self.d["ost2_grid"])
         _lib.call("mh_fft", 1 if forward else 0, _lib.ref(a), stream_handle())
 
-    def fourier_buffer(self, ncomp, B):
-        return torch.empty(ncomp * self.M * self.T * B * 2, dtype=F64, device=self.device)
+    def fourier_buffer(self, ncomp, B, dtype=F64):
+        """F[c][m][t][b] complex as interleaved (re, im) of `dtype` (float64, or
+        float32 for the mixed-precision radial loop)."""
+        return torch.empty(ncomp * self.M * self.T * B * 2, dtype=dtype, device=self.device)
 
     def grid_step(self, Fs, Fv, Fo, N, cfl, p, generic=False):
         """Fused inverse FFT + nonlinear products + Courant maxima + forward FFT.
@@ -193,6 +197,9 @@ class Resolution:
         """
         a = _lib.GridArgs()
         a.Fs, a.Fv, a.Fo = ptr(Fs), ptr(Fv), ptr(Fo)
+        if not (Fs.dtype == Fv.dtype == Fo.dtype):
+            raise ValueError("grid_step: Fourier buffers of mixed dtypes")
+        a.fprec = 1 if Fs.dtype == torch.float32 else 0
         a.N, a.T, a.M, a.Mh, a.generic = N, self.T, self.M, self.Mh, int(generic)
         a.tw, a.tw2 = ptr(self.d["tw"]), ptr(self.d["tw2"])
         d = self.d

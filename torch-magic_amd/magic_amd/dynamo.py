@@ -126,14 +126,16 @@ class DynamoRun:
         # (default): one kernel for inverse FFT + products + forward FFT, no grid in
         # HBM.  Separate: grids G[c][b][t][phi] and three kernels.
         self.fused_grid = fused_grid
-        # mixed precision: Legendre transforms on FP32 MFMA with FP32 accumulation;
-        # FFTs, products, solves, derivatives and diagnostics stay FP64
+        # mixed precision: Legendre transforms on FP32 MFMA with FP32 accumulation,
+        # and (fused grid path) FP32 Fourier buffers, azimuthal FFTs and nonlinear
+        # products; solves, radial derivatives, IMEX terms and diagnostics stay FP64
         self.leg_prec = 1 if p.precision == "mixed" else 0
         self.grid_generic = False  # force the run-time FFT plan of the fused kernel
-        self.F_s = r.fourier_buffer(1, N)
-        self.F_v = r.fourier_buffer(3, 4 * N)
+        fdt = torch.float32 if (self.leg_prec and fused_grid) else F64
+        self.F_s = r.fourier_buffer(1, N, fdt)
+        self.F_v = r.fourier_buffer(3, 4 * N, fdt)
         if fused_grid:
-            self.F_o = r.fourier_buffer(9, self.Nb)
+            self.F_o = r.fourier_buffer(9, self.Nb, fdt)
         else:
             self.G_s = r.grid_buffer(1, N)
             self.G_v = r.grid_buffer(3, 4 * N)
