@@ -444,9 +444,12 @@ __global__ __launch_bounds__(kMaxThreads, 3) void k_grid_step(GridArgs a, GPlan 
       o[i] = mk2(or2 * vp[i].x * tmp[i].x, or2 * vp[i].y * tmp[i].y);
     forward<MH, RL, T>(c, g, a, o, 6, inv_n * ost2, t, r0);
   }
-  // 3. B -> Courant maxima (all levels incl. boundaries) and VxBr, VxBt, VxBp
+  // 3. B -> Courant maxima (all levels incl. boundaries) and VxBr, VxBt, VxBp.
+  // In FP32 the B values stay in registers for the Lorentz force below; in
+  // FP64 they are transformed again there (register budget).
+  constexpr bool keepB = sizeof(T) == 4;
+  V br[RL], bt[RL], bp[RL];
   {
-    V br[RL], bt[RL], bp[RL];
     if (mag) {
       inverse<MH, RL, T>(c, g, a, pf, br);
       fetch<MH, RL, T>(g, a, pf, Fv + vcs, Bv, r0, nvalid, t);
@@ -534,7 +537,10 @@ __global__ __launch_bounds__(kMaxThreads, 3) void k_grid_step(GridArgs a, GPlan 
     ap[i].y += or4 * tmp[i].y * vr[i].y;
   }
   inverse<MH, RL, T>(c, g, a, pf, tmp);  // curl u_p
-  if (mag) fetch<MH, RL, T>(g, a, pf, Fv, Bv, r0, nvalid, t);  // B_r next
+  if (mag) {
+    if (keepB) fetch<MH, RL, T>(g, a, pf, Fv, Bv, N + r0, nvalid, t);  // curl B_r next
+    else fetch<MH, RL, T>(g, a, pf, Fv, Bv, r0, nvalid, t);            // B_r next
+  }
 #pragma unroll
   for (int i = 0; i < RL; ++i) {
     ar[i].x += ost2 * tmp[i].x * vt[i].x;
@@ -544,13 +550,14 @@ __global__ __launch_bounds__(kMaxThreads, 3) void k_grid_step(GridArgs a, GPlan 
   }
   if (mag) {
     const T lr = LF * ost2, lt = LF * or4;
-    V br[RL], bt[RL], bp[RL];
-    inverse<MH, RL, T>(c, g, a, pf, br);
-    fetch<MH, RL, T>(g, a, pf, Fv + vcs, Bv, r0, nvalid, t);
-    inverse<MH, RL, T>(c, g, a, pf, bt);
-    fetch<MH, RL, T>(g, a, pf, Fv + 2 * vcs, Bv, r0, nvalid, t);
-    inverse<MH, RL, T>(c, g, a, pf, bp);
-    fetch<MH, RL, T>(g, a, pf, Fv, Bv, N + r0, nvalid, t);
+    if (!keepB) {
+      inverse<MH, RL, T>(c, g, a, pf, br);
+      fetch<MH, RL, T>(g, a, pf, Fv + vcs, Bv, r0, nvalid, t);
+      inverse<MH, RL, T>(c, g, a, pf, bt);
+      fetch<MH, RL, T>(g, a, pf, Fv + 2 * vcs, Bv, r0, nvalid, t);
+      inverse<MH, RL, T>(c, g, a, pf, bp);
+      fetch<MH, RL, T>(g, a, pf, Fv, Bv, N + r0, nvalid, t);
+    }
     inverse<MH, RL, T>(c, g, a, pf, tmp);  // curl B_r
     fetch<MH, RL, T>(g, a, pf, Fv + vcs, Bv, N + r0, nvalid, t);
 #pragma unroll
