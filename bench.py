@@ -11,8 +11,9 @@ A "step" is one full ``one_step`` (radial loop with the SHTs, nonlinear
 terms and CFL, dt decision, operator rebuild if dt changed, implicit solves).
 W warm-up steps are run untimed, then exactly K steps (default 1000) are
 timed between two device synchronisations, first in plain FP64, then in mixed
-precision (Legendre transforms on FP32 MFMA, everything else FP64), each from
-the same seeded initial state; the mixed run replays the FP64 run's step
+precision (Legendre transforms on FP32 MFMA, FP32 Fourier buffers, azimuthal
+FFTs and nonlinear products; solves, radial derivatives, IMEX terms and
+diagnostics FP64), each from the same seeded initial state; the mixed run replays the FP64 run's step
 sizes so that the two differ by arithmetic precision only.  The headline value
 is the mixed-precision steps/s; the FP64 rate, their ratio and the relative
 energy and spectrum differences after W+K steps are in ``config``.
@@ -46,9 +47,12 @@ NOISE = dict(seed=20261202, amp=1e-4)
 # rocprofv3 FETCH_SIZE / WRITE_SIZE passes scaled by a 256 MiB copy (FETCH x 2,
 # WRITE x 1); see profiles/hbm_traffic_l383_nr97_pmc.md
 PMC_TRAFFIC_BYTES = {
-    ("grid_step", "mixed"): 6.461e9 + 3.239e9, ("grid_step", "fp64"): 6.448e9 + 3.226e9,
-    ("leg_syn_vec", "mixed"): 3.822e9 + 4.373e9, ("leg_syn_vec", "fp64"): 4.425e9 + 4.132e9,
-    ("leg_ana_vec", "mixed"): 5.559e9 + 0.609e9, ("leg_ana_vec", "fp64"): 6.191e9 + 0.667e9,
+    ("grid_step", "mixed"): 2.526e9 + 1.750e9,
+    ("grid_step", "fp64"): 6.465e9 + 3.225e9,
+    ("leg_syn_vec", "mixed"): 3.370e9 + 2.323e9,
+    ("leg_syn_vec", "fp64"): 4.414e9 + 4.132e9,
+    ("leg_ana_vec", "mixed"): 1.404e9 + 0.590e9,
+    ("leg_ana_vec", "fp64"): 6.196e9 + 0.668e9,
 }
 
 PEAK_FP64_TFLOPS = 78.6
@@ -56,8 +60,9 @@ PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 
 
-def section_work(p, N):
-    """Algorithmic FLOPs (MFMA-bound sections) or HBM bytes (bandwidth-bound) per launch."""
+def section_work(p, N, fbytes=16.0):
+    """Algorithmic FLOPs (MFMA-bound sections) or HBM bytes (bandwidth-bound) per launch;
+    fbytes: bytes per complex value of the Fourier buffers (16 FP64, 8 in mixed precision)."""
     L, H, T, P, M = p.lm_max, p.n_theta // 2, p.n_theta, p.n_phi, p.n_m
     Nb = N - 2
     return {
@@ -76,7 +81,7 @@ def section_work(p, N):
         "get_nl": ("hbm", (13.0 * N + 9.0 * Nb) * T * P * 8.0),
         # fused grid step: Fourier rows in (13 components, 6 on the 2 boundary
         # levels) and out (9 components, bulk levels), M complex values each
-        "grid_step": ("hbm", ((13.0 * Nb + 6.0 * 2) + 9.0 * Nb) * T * M * 16.0),
+        "grid_step": ("hbm", ((13.0 * Nb + 6.0 * 2) + 9.0 * Nb) * T * M * fbytes),
     }
 
 
@@ -264,7 +269,7 @@ def ensemble_case(n_seq):
 
 def roofline(res, precision):
     """Roofline of the dominant kernel section (largest total time)."""
-    secs, work = res["secs"], section_work(res["p"], res["N"])
+    secs, work = res["secs"], section_work(res["p"], res["N"], 8.0 if precision == "mixed" else 16.0)
     dom = max(secs, key=lambda k: secs[k][0] * secs[k][1])
     cnt, ms = secs[dom]
     kind, amount = work.get(dom, ("hbm", float("nan")))
@@ -318,7 +323,9 @@ def main():
                        "n_phi=1152, E=1e-5, Ra=2e8, Pr=1, Pm=1",
            "l_max": 383, "n_r_max": 97, "precision": head,
            "precision_split": "Legendre synthesis/analysis on FP32 MFMA with FP32 accumulation; "
-                              "FFTs, nonlinear products, solves, derivatives, diagnostics FP64"
+                              "Fourier buffers, azimuthal FFTs, nonlinear products and Courant "
+                              "maxima FP32; implicit solves, radial derivatives, IMEX terms and "
+                              "energy/spectrum diagnostics FP64"
                               if head == "mixed" else "all FP64",
            "steps_per_s": {pr: v for pr, v in sps.items()},
            "ms_per_step": {pr: 1e3 * r["elapsed"] / args.steps for pr, r in res.items()},
