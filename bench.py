@@ -7,8 +7,10 @@ Pm=1, radius ratio 0.35, CNAB2 with adaptive Courant dt (dt <= 0.15 E),
 seeded Gaussian noise 1e-4 on the entropy plus the dipole seed field.
 Synthetic data: fields initialised on the device from the seed, no I/O.
 
-A "step" is one full ``one_step`` (radial loop with the SHTs, nonlinear
-terms and CFL, dt decision, operator rebuild if dt changed, implicit solves).
+A "step" is one full CNAB2 step (radial loop with the SHTs, nonlinear terms
+and CFL, dt decision, operator rebuild if dt changed, implicit solves), run
+as one HIP graph replay with the dt decision taken on the device
+(``DynamoRun.step_async``), so the host queues steps without waiting.
 W warm-up steps are run untimed, then exactly K steps (default 1000) are
 timed between two device synchronisations, first in plain FP64, then in mixed
 precision (Legendre transforms on FP32 MFMA, FP32 Fourier buffers, azimuthal
@@ -17,8 +19,8 @@ diagnostics FP64), each from the same seeded initial state; the mixed run replay
 sizes so that the two differ by arithmetic precision only.  The headline value
 is the mixed-precision steps/s; the FP64 rate, their ratio and the relative
 energy and spectrum differences after W+K steps are in ``config``.
-Per-section HIP-event timings from the timed region give the roofline of the
-dominant kernel.  Also reported: the SHT round trip at l_max=383 on 291 fields,
+Per-section HIP-event timings, from 20 further steps run eagerly with events
+between the sections, give the roofline of the dominant kernel.  Also reported: the SHT round trip at l_max=383 on 291 fields,
 the finite-difference banded solves (FD4, n_r=512, l_max=255) and the 64-run ensemble served with continuous admission
 against a sequential sample.  The CPU baseline is the FP64 numpy oracle on a
 bounded sample of the same workload.
@@ -112,34 +114,35 @@ def cpu_baseline(sample_levels, steps=3):
                 seconds=dt)
 
 
-def run_case(precision, steps, warmup, dt_schedule=None):
-    """Time `steps` CNAB2 steps after `warmup` untimed ones; per-section HIP events.
+def run_case(precision, steps, warmup, dt_schedule=None, section_steps=20):
+    """Time `steps` CNAB2 steps (one HIP graph each, dt decided on the device)
+    after `warmup` untimed ones, then time the kernel sections over
+    `section_steps` further steps run eagerly with HIP events between sections.
     dt_schedule: step sizes to replay instead of the run's own Courant decisions."""
     import torch
     from magic_amd.dynamo import DynamoRun
     run = DynamoRun(dict(WORKLOAD, precision=precision), device="cuda:0")
     run.dt_schedule = dt_schedule
+    run.use_graphs = True
     run.initialize_fields(noise_amp=NOISE["amp"], noise_seed=NOISE["seed"])
     run.setup_initial_state()
     run.initialize_dt(WORKLOAD["dtmax"])
-    dt = run.dtMax
-    n = 0
-    for _ in range(warmup):
-        n += 1
-        dt = run.one_step(n, dt)
-    torch.cuda.synchronize()
-    run.enable_timers(True)
+    dt = run.run_steps(warmup, run.dtMax)           # includes the graph captures
+    run.ctl_begin(dt)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
-        n += 1
-        dt = run.one_step(n, dt)
+        run.step_async()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    secs = run.timer_summary()
+    dt = run.ctl_end()
+    out = dict(elapsed=t1 - t0, energies=run.energies(), spectra=run.spectra(), dt=dt,
+               rebuilds=run.n_rebuild, p=run.p, N=run.N)
+    run.enable_timers(True)
+    run.run_steps(section_steps, dt)
+    out["secs"] = run.timer_summary()
+    out["dt_log"] = list(run.dt_log)
     run.enable_timers(False)
-    out = dict(elapsed=t1 - t0, secs=secs, energies=run.energies(), spectra=run.spectra(), dt=dt,
-               rebuilds=run.n_rebuild, p=run.p, N=run.N, dt_log=list(run.dt_log))
     del run
     torch.cuda.empty_cache()
     return out
@@ -252,18 +255,26 @@ def ensemble_case(n_seq):
     a time, in plain FP64 (arrivals ignored)."""
     import numpy as np
     from magic_amd.serve import EnsembleServer, config5_requests, serve_sequential
+    cfgs = [r.cfg for r in config5_requests()]
     srv = EnsembleServer(max_slots=16)
+    tw = time.perf_counter()
+    srv.warm(cfgs)                 # device tables of the served resolutions, before serving
+    warm_s = time.perf_counter() - tw
     _, mix = srv.serve(config5_requests(), realtime=True)
     # the same 64 runs all waiting at t=0: throughput of the server when saturated
     sat_srv = EnsembleServer(max_slots=16)
+    sat_srv.warm(cfgs)
     _, sat = sat_srv.serve(config5_requests(), realtime=False)
-    _, seq = serve_sequential(config5_requests()[:n_seq])
+    _, seq = serve_sequential(config5_requests()[:n_seq], warm=True)
     return dict(workload="64 runs x 500 steps, Poisson arrivals 2/s, l_max {32,64,85,127}, "
-                         "n_r {33,49,65}, FP64",
-                server=dict(mix, max_active=max(n for _, n in srv.rounds),
+                         "n_r {33,49,65}, FP64; each step one HIP graph with the dt decision "
+                         "on the device, one step per run per round",
+                table_warm_s=warm_s,
+                server=dict(mix, max_active=max(n for _, n in srv.occupancy),
                             admit_ms_mean=1e3 * float(np.mean(srv.admit_s)),
-                            admit_ms_max=1e3 * float(np.max(srv.admit_s))),
-                saturated=dict(sat, max_active=max(n for _, n in sat_srv.rounds)),
+                            admit_ms_max=1e3 * float(np.max(srv.admit_s)),
+                            admit_stage_ms_max=1e3 * float(np.max(srv.stage_s))),
+                saturated=dict(sat, max_active=max(n for _, n in sat_srv.occupancy)),
                 sequential=dict(seq, sample=f"first {n_seq} of the 64 requests back to back"))
 
 

@@ -278,6 +278,8 @@ typedef struct {
   double* out;
   int N, l_max, n_cheb, ktopv, kbotv, kind; /* kind 0 S 1 Z 2 B 3 J 4 WP 5 p0 */
   double rnorm, wimp, opr, opm;
+  const double* wimp_dev; /* optional: read the implicit weight here instead of wimp */
+  const int* when;        /* optional: the kernel does nothing unless *when != 0 */
 } MhBuildArgs;
 
 /* Per-degree implicit matrices: build_s_matrices (update_s.py:66-188),
@@ -288,9 +290,10 @@ int mh_build_matrices(const MhBuildArgs* args, void* stream);
 
 /* Batched equilibrated Gauss-Jordan inverse (precond 0 none, 1 rows, 2 rows
  * and columns), out = Fc inv(Fr A Fc) Fr; replaces the torch.linalg.inv calls
- * of the matrix builders.  work: nbatch*2n*n doubles, scal: nbatch*2n. */
+ * of the matrix builders.  work: nbatch*2n*n doubles, scal: nbatch*2n.
+ * when: optional, as in MhBuildArgs. */
 int mh_gj_inverse(const double* A, double* out, double* work, double* scal, int n, int nbatch,
-                  int precond, int* info, void* stream);
+                  int precond, int* info, const int* when, void* stream);
 
 typedef struct {
   const double* A;
@@ -299,10 +302,59 @@ typedef struct {
   long sAr, sAc, sAb, sBr, sBc, sBb, sCr, sCc, sCb;
   int M, Nn, K, batch;
   double alpha;
+  const int* when; /* optional: the kernel does nothing unless *when != 0 */
 } MhGemmArgs;
 
 /* Batched strided FP64 GEMM C = alpha A B on MFMA. */
 int mh_gemm_f64(const MhGemmArgs* args, void* stream);
+
+/* ------------------------------------------------------------ step control */
+
+/* Device-resident state of the time stepping of one run (read and written by
+ * mh_step_control). */
+typedef struct {
+  double dt;        /* dt of the last step (the dt the next decision starts from) */
+  double dtMax;
+  double dt1, dt2;  /* dt history: last step, the step before */
+  double wimp_lin0; /* implicit weight the operators were built for (NaN: none yet) */
+  double time;
+  int n_step;       /* steps taken */
+  int rebuild;      /* 1 while the current step rebuilds the operators */
+  int n_rebuild;
+  int new_dt;       /* 1 when the current step's Courant decision changed dt */
+  int sing_step;    /* first step whose rebuild met a zero pivot (0: none yet) */
+  int sing_sys;     /* its system: row of the pivot-flag array */
+  int sing_l;       /* its degree */
+  int sing_info;    /* its first zero pivot (1-based) */
+} MhStepState;
+
+typedef struct {
+  const unsigned long long* cfl; /* 2N per-level Courant maxima (FP64 bits): radial, then horizontal */
+  const double* delxr2;          /* N: squared radial grid spacing (courant.py) */
+  const double* delxh2;          /* N: squared horizontal grid spacing */
+  MhStepState* state;
+  double* w;                     /* out: wimp, wimp_lin[1], wexp[0], wexp[1], wimp_lin[0]*BuoFac */
+  const double* dt_schedule;     /* optional: n_schedule step sizes replacing the decision */
+  double* dt_log;                /* optional: ring of log_cap entries, dt of step n at (n-1) % log_cap */
+  const int* info;               /* optional: pivot flags [info_rows][info_cols] of the last rebuild */
+  int N, n_schedule, log_cap, info_rows, info_cols, pad;
+  double alpha, BuoFac;
+} MhStepCtlArgs;
+
+/* The host decisions of one CNAB2 step taken on the device, so that a whole
+ * step is one stream of kernels (one HIP graph): Courant limits from the
+ * per-level maxima (courant.py:80-127), the hysteresis dt decision of
+ * dt_courant (courant.py:130-155), the rebuild decision and dt roll of
+ * one_step (step_time.py:1574-1637) and the CNAB2 weights of set_weights
+ * (time_scheme.py:56-109).  state->rebuild and state->wimp_lin0 are the
+ * `when` flag and `wimp_dev` of the rebuild launches that follow.  When the
+ * previous step rebuilt, its pivot flags are scanned first and the first
+ * singular system is kept in state->sing_* (the reference raises there,
+ * update_s.py:152; the host raises when it reads the state). */
+int mh_step_control(const MhStepCtlArgs* args, void* stream);
+
+/* p[0..n) = v, unless `when` (optional) points at 0. */
+int mh_fill_f64(double* p, long n, double v, const int* when, void* stream);
 
 /* ------------------------------------------------------------ banded */
 

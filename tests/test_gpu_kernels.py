@@ -202,7 +202,7 @@ def test_gj_inverse_matches_numpy(n):
         scal = torch.empty(batch * 2 * n, dtype=torch.float64, device="cuda")
         info = torch.full((batch,), -1, dtype=torch.int32, device="cuda")
         _lib.call("mh_gj_inverse", ptr(d), ptr(d), ptr(work), ptr(scal), n, batch, precond,
-                  ptr(info), stream_handle())
+                  ptr(info), None, stream_handle())
         got = d.cpu().numpy()
         assert np.all(info.cpu().numpy() == 0)
         for b in range(batch):
@@ -226,5 +226,5 @@ def test_gj_inverse_reports_singular_pivot():
         scal = torch.empty(2 * n, dtype=torch.float64, device="cuda")
         info = torch.zeros(1, dtype=torch.int32, device="cuda")
         _lib.call("mh_gj_inverse", ptr(d), ptr(d), ptr(work), ptr(scal), n, 1, 1, ptr(info),
-                  stream_handle())
+                  None, stream_handle())
         assert int(info.cpu()[0]) == 1

@@ -42,7 +42,7 @@ def test_admission_respects_slots_and_budget():
     srv = EnsembleServer(max_slots=4, hbm_budget=budget)
     done, summary = srv.serve(reqs)
     assert summary["runs"] == 4
-    assert max(n for _, n in srv.rounds) == 1  # only one run fits the budget at a time
+    assert max(n for _, n in srv.occupancy) == 1  # only one run fits the budget at a time
 
 
 def test_runs_of_one_resolution_overlapping_match_alone():
@@ -62,7 +62,7 @@ def test_runs_of_one_resolution_overlapping_match_alone():
     srv = EnsembleServer(max_slots=3)
     mix, _ = srv.serve(reqs(), realtime=False, keep_fields=True)
     alone, _ = serve_sequential(reqs(), keep_fields=True)
-    assert max(n for _, n in srv.rounds) == 3
+    assert max(n for _, n in srv.occupancy) == 3
     by_id = {r.rid: r for r in alone}
     for r in mix:
         a = by_id[r.rid]

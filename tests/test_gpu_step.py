@@ -205,3 +205,73 @@ def test_graph_replay_matches_eager(name):
         for k in ("s", "z", "w", "p", "b", "aj"):
             assert torch.equal(graph.f[k], eager.f[k]), (n, k)
     assert len(graph._graphs) == 4
+
+
+# Courant-limited case: noise of amplitude 1 on s makes the flow strong enough
+# that the Courant rule changes dt six times in 30 steps (l_max=16, n_r=33)
+_DT_CASE = dict(cfg=dict(l_max=16, n_r_max=33, ek=1e-3, ra=1e5, pr=1.0, prmag=5.0,
+                         radratio=0.35, dtmax=1e-3),
+                dt=1e-3, noise=dict(amp=1.0, seed=7), steps=30)
+
+
+def test_device_step_control_matches_host_steps():
+    """Steps with the dt decision on the device, each step one HIP graph (radial
+    loop, step control, rebuild launches that run only when dt changed, lm loop),
+    against host-decided eager steps: same dt sequence, rebuild count, time and
+    fields bit for bit, in two batches (state read back and pushed again in
+    between); the dt sequence also follows the oracle's."""
+    import torch
+    case = _DT_CASE
+    host = _run(case)
+    dev = _run(case)
+    dev.use_graphs = True
+    dt = case["dt"]
+    for n in range(1, case["steps"] + 1):
+        dt = host.one_step(n, dt)
+    dev.run_steps(10, case["dt"])
+    dev.run_steps(case["steps"] - 10, dev.dt_log[-1])
+    assert dev.dt_log == host.dt_log
+    assert dev.n_rebuild == host.n_rebuild and host.n_rebuild >= 6, host.n_rebuild
+    assert dev.time == host.time and dev.n_step == host.n_step == case["steps"]
+    for k in ("s", "z", "w", "p", "b", "aj", "db", "ddw"):
+        assert torch.equal(dev.f[k], host.f[k]), k
+    assert dev.energies() == host.energies()
+    assert sorted(k[0] for k in dev._graphs) == ["step", "step"]
+    # the oracle's decisions on the same initial state
+    from oracle.dynamo import OracleRun
+    orc = OracleRun(dict(case["cfg"]))
+    orc.initialize_fields()
+    orc.add_noise(case["noise"]["seed"], case["noise"]["amp"])
+    orc.setup_initial_state()
+    orc.initialize_dt(case["dt"])
+    dt = case["dt"]
+    odts = []
+    for n in range(1, case["steps"] + 1):
+        dt = orc.one_step(n, dt)
+        odts.append(dt)
+    np.testing.assert_allclose(dev.dt_log, odts, rtol=1e-9)
+    e, oe = np.array(dev.energies()), np.array(orc.energies())
+    assert np.all(np.abs(e - oe) <= 1e-9 * np.abs(oe)), (e, oe)
+
+
+def test_device_step_control_replays_schedule_with_timers():
+    """dt_schedule replay through the device step control, run eagerly with
+    section timers (the step's launch sequence outside a graph), equals the
+    host-decided run on the same schedule."""
+    import torch
+    case = _DT_CASE
+    sched = [4e-5] * 4 + [3e-5] * 3 + [2.5e-5] * 5  # below the case's Courant limits
+    host = _run(case)
+    dev = _run(case)
+    host.dt_schedule = dev.dt_schedule = sched
+    dt = case["dt"]
+    for n in range(1, len(sched) + 1):
+        dt = host.one_step(n, dt)
+    dev.enable_timers(True)
+    dev.run_steps(len(sched), case["dt"])
+    assert dev.dt_log == host.dt_log == sched
+    assert dev.n_rebuild == host.n_rebuild == 3
+    for k in ("s", "z", "w", "b", "aj"):
+        assert torch.equal(dev.f[k], host.f[k]), k
+    t = dev.timer_summary()
+    assert "step_ctl" in t and "rebuild" in t

@@ -1,5 +1,5 @@
 """Host-side logic of the ensemble server (no GPU): workload generator,
-footprint estimate, latency summary."""
+footprint estimate, latency summary, row-job packing."""
 
 import numpy as np
 
@@ -33,3 +33,39 @@ def test_summary_percentiles():
     assert s["runs"] == 4 and s["steps"] == 12
     assert abs(s["p50_ms"] - 10.0) < 1e-9
     assert s["runs_per_hour"] == 7200.0
+
+
+def test_row_job_packing_matches_struct_layout():
+    """The numpy packing of row-operator jobs equals MhRowJob records built one
+    by one with ctypes, and the tile list enumerates (job, 32-row chunk,
+    64-column chunk) in job order."""
+    import ctypes as C
+
+    from magic_amd import _lib
+    from magic_amd.dynamo import pack_row_jobs
+    ls = np.arange(40)
+    specs = [dict(X=0x1000, ldx=33, rows=0x2000 + 4 * ls * (ls + 1) // 2, nrows=ls + 1,
+                  MT=0x3000 + 8 * ls * 777, K=33, R=99, Rpad=128, Y=[0x10, 0x20, 0x30],
+                  ldy=33, rseg=33),
+             dict(X=0x5000, ldx=33, rows=0, nrows=561, MT=0x6000, K=33, R=33, Rpad=64,
+                  Y=[0x40], ldy=33, rseg=33)]
+    arr, tiles = pack_row_jobs(specs)
+    ref, ref_tiles = [], []
+    for sp in specs:
+        for i in range(np.size(sp["nrows"])):
+            j = _lib.RowJob()
+            pick = lambda v: int(np.asarray(v).reshape(-1)[i]) if np.size(v) > 1 else int(v)
+            j.X, j.rows, j.MT = sp["X"], pick(sp["rows"]) or None, pick(sp["MT"])
+            for k, y in enumerate(sp["Y"]):
+                j.Y[k] = y
+            j.ldx, j.ldy, j.K, j.R, j.Rpad, j.rseg = (sp[k] for k in ("ldx", "ldy", "K", "R", "Rpad", "rseg"))
+            j.nrows = pick(sp["nrows"])
+            ref.append(j)
+    raw = bytes((_lib.RowJob * len(ref))(*ref))
+    assert arr.tobytes() == raw
+    for ji, j in enumerate(ref):
+        for r0 in range(0, j.nrows, 32):
+            for c0 in range(0, j.R, 64):
+                ref_tiles.append((ji, r0, c0))
+    assert tiles.tolist() == [list(t) for t in ref_tiles]
+    assert C.sizeof(_lib.RowJob) == arr.dtype.itemsize

@@ -117,17 +117,30 @@ class EnergyArgs(C.Structure):
 class BuildArgs(C.Structure):
     _fields_ = [("R", P), ("D1", P), ("D2", P), ("D3", P), ("or1", P), ("or2", P), ("out", P),
                 ("N", I), ("l_max", I), ("n_cheb", I), ("ktopv", I), ("kbotv", I), ("kind", I),
-                ("rnorm", D), ("wimp", D), ("opr", D), ("opm", D)]
+                ("rnorm", D), ("wimp", D), ("opr", D), ("opm", D), ("wimp_dev", P), ("when", P)]
 
 
 class GemmArgs(C.Structure):
     _fields_ = [("A", P), ("B", P), ("C", P)] + [(n, L) for n in (
         "sAr", "sAc", "sAb", "sBr", "sBc", "sBb", "sCr", "sCc", "sCb")] + [
-        ("M", I), ("Nn", I), ("K", I), ("batch", I), ("alpha", D)]
+        ("M", I), ("Nn", I), ("K", I), ("batch", I), ("alpha", D), ("when", P)]
+
+
+class StepState(C.Structure):
+    _fields_ = [("dt", D), ("dtMax", D), ("dt1", D), ("dt2", D), ("wimp_lin0", D), ("time", D),
+                ("n_step", I), ("rebuild", I), ("n_rebuild", I), ("new_dt", I),
+                ("sing_step", I), ("sing_sys", I), ("sing_l", I), ("sing_info", I)]
+
+
+class StepCtlArgs(C.Structure):
+    _fields_ = [("cfl", P), ("delxr2", P), ("delxh2", P), ("state", P), ("w", P),
+                ("dt_schedule", P), ("dt_log", P), ("info", P), ("N", I), ("n_schedule", I),
+                ("log_cap", I), ("info_rows", I), ("info_cols", I), ("pad", I), ("alpha", D),
+                ("BuoFac", D)]
 
 
 _STRUCT_ORDER = [SynArgs, AnaArgs, FftArgs, NlArgs, RowJob, TdArgs, RhsArgs, BuoyArgs,
-                 ImplArgs, EnergyArgs, BuildArgs, GemmArgs, GridArgs]
+                 ImplArgs, EnergyArgs, BuildArgs, GemmArgs, GridArgs, StepState, StepCtlArgs]
 
 _SIGS = {
     "mh_plm_tables": [I, I, I, I, P, P, P, P, P, P, P, P, P],
@@ -144,8 +157,10 @@ _SIGS = {
     "mh_impl": [P, P],
     "mh_energy": [P, P],
     "mh_build_matrices": [P, P],
-    "mh_gj_inverse": [P, P, P, P, I, I, I, P, P],
+    "mh_gj_inverse": [P, P, P, P, I, I, I, P, P, P],
     "mh_gemm_f64": [P, P],
+    "mh_step_control": [P, P],
+    "mh_fill_f64": [P, L, D, P, P],
     "mh_band_factor": [P, P, P, I, I, I, I, P],
     "mh_band_solve": [P, P, P, P, P, P, P, P, I, I, I, I, P],
     "mh_struct_sizes": [P, I],

@@ -19,6 +19,7 @@ typedef MhBuildArgs BuildArgs;
 __device__ __forceinline__ double bfac(int n, int N) { return (n == 0 || n == N - 1) ? 0.5 : 1.0; }
 
 __global__ void k_build(BuildArgs a) {
+  if (a.when && *a.when == 0) return;
   const int N = a.N;
   const int n2 = (a.kind == 4) ? 2 * N : N;
   const long per = (long)n2 * n2;
@@ -29,7 +30,7 @@ __global__ void k_build(BuildArgs a) {
   const int rem = (int)(idx - (long)l * per);
   const int i = rem / n2, j = rem - i * n2;
   const double dL = (double)l * (double)(l + 1);
-  const double rn = a.rnorm, w0 = a.wimp;
+  const double rn = a.rnorm, w0 = a.wimp_dev ? *a.wimp_dev : a.wimp;
   double v = 0.0;
   if (a.kind <= 3 || a.kind == 5) {
     const int n = j;
@@ -112,7 +113,8 @@ extern "C" int mh_build_matrices(const MhBuildArgs* args, void* stream) {
 __global__ __launch_bounds__(256) void k_gj_inverse(const double* __restrict__ A, double* out,
                                                     double* __restrict__ work,
                                                     double* __restrict__ scal, int n,
-                                                    int precond, int* info) {
+                                                    int precond, int* info, const int* when) {
+  if (when && *when == 0) return;
   const int bm = blockIdx.x;
   const long nn = (long)n * n;
   const double* Am = A + bm * nn;
@@ -218,7 +220,8 @@ __global__ __launch_bounds__(256) void k_gj_inverse(const double* __restrict__ A
 // barriers per step (LDS broadcast buffers alternate between steps).
 template <int TR, int TC, int RPT, int CPT>
 __global__ __launch_bounds__(TR * TC) void k_gj_reg(const double* A, double* out, int n,
-                                                    int precond, int* info) {
+                                                    int precond, int* info, const int* when) {
+  if (when && *when == 0) return;
   constexpr int RMAX = TR * RPT, CMAX = TC * CPT;
   __shared__ double colbuf[2][RMAX];
   __shared__ double rowbuf[2][CMAX];
@@ -380,17 +383,17 @@ __global__ __launch_bounds__(TR * TC) void k_gj_reg(const double* A, double* out
 }
 
 extern "C" int mh_gj_inverse(const double* A, double* out, double* work, double* scal, int n,
-                             int nbatch, int precond, int* info, void* stream) {
+                             int nbatch, int precond, int* info, const int* when, void* stream) {
   if (n < 1 || nbatch < 1) return MH_EINVAL;
   if (n <= 112)
     hipLaunchKernelGGL((k_gj_reg<16, 16, 7, 7>), dim3(nbatch), dim3(256), 0, mh_stream(stream), A,
-                       out, n, precond, info);
+                       out, n, precond, info, when);
   else if (n <= 208)
     hipLaunchKernelGGL((k_gj_reg<16, 32, 13, 7>), dim3(nbatch), dim3(512), 0, mh_stream(stream), A,
-                       out, n, precond, info);
+                       out, n, precond, info, when);
   else
     hipLaunchKernelGGL(k_gj_inverse, dim3(nbatch), dim3(256), 0, mh_stream(stream), A, out, work,
-                       scal, n, precond, info);
+                       scal, n, precond, info, when);
   return mh_launch_status();
 }
 
@@ -399,6 +402,7 @@ typedef MhGemmArgs GemmArgs;
 
 // one wave per 16 x 64 tile of C
 __global__ __launch_bounds__(64) void k_gemm_f64(GemmArgs g) {
+  if (g.when && *g.when == 0) return;
   const int lane = threadIdx.x;
   const int b = blockIdx.z;
   const int i0 = blockIdx.y * 16, j0 = blockIdx.x * 64;

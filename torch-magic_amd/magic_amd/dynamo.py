@@ -34,6 +34,7 @@ lm loop (``step_time.py:222-394``, ``update_b.py:557-631``)
 
 from __future__ import annotations
 
+import ctypes as C
 import math
 
 import numpy as np
@@ -45,6 +46,46 @@ from .params import Params
 
 _FIELDS = ("s", "ds", "z", "dz", "w", "dw", "ddw", "p", "dp", "b", "db", "ddb", "aj", "dj", "ddj")
 _SYS = ("s", "z", "w", "p", "b", "j")
+
+
+def _rowjob_dtype():
+    """numpy record layout of MhRowJob (from the ctypes mirror's field offsets)."""
+    names, formats, offsets = [], [], []
+    for name, ct in _lib.RowJob._fields_:
+        names.append(name)
+        offsets.append(getattr(_lib.RowJob, name).offset)
+        formats.append((np.uint64, 6) if name == "Y" else
+                       (np.uint64 if ct is _lib.P else np.int32))
+    return np.dtype(dict(names=names, formats=formats, offsets=offsets,
+                         itemsize=C.sizeof(_lib.RowJob)))
+
+
+_ROWJOB_DTYPE = _rowjob_dtype()
+
+
+def pack_row_jobs(specs):
+    """Pack row-operator job columns into MhRowJob records and build the tile
+    list (job, first row, first column): rows in chunks of 32, operator columns
+    in chunks of 64, in job order.  Each spec is a dict of MhRowJob fields whose
+    rows / nrows / MT entries may be arrays (one job per element)."""
+    n = [int(np.size(sp["nrows"])) for sp in specs]
+    tot = sum(n)
+    arr = np.zeros(tot, dtype=_ROWJOB_DTYPE)
+    k = 0
+    for sp, m in zip(specs, n):
+        sl = arr[k:k + m]
+        for name in ("X", "rows", "MT", "ldx", "ldy", "K", "R", "Rpad", "rseg", "nrows"):
+            sl[name] = sp[name]
+        sl["Y"] = np.asarray(list(sp["Y"]) + [0] * (6 - len(sp["Y"])), np.uint64)[None, :]
+        k += m
+    nr = -(-arr["nrows"].astype(np.int64) // 32)
+    nc = -(-arr["R"].astype(np.int64) // 64)
+    per = nr * nc
+    ji = np.repeat(np.arange(tot), per)
+    q = np.arange(per.sum()) - np.repeat(np.cumsum(per) - per, per)
+    ncj = np.repeat(nc, per)
+    tiles = np.stack([ji, 32 * (q // ncj), 64 * (q % ncj)], axis=1).astype(np.int32)
+    return arr, tiles
 
 
 def seeded_noise(p: Params, res: Resolution, seed: int, amp: float) -> np.ndarray:
@@ -101,6 +142,18 @@ class DynamoRun:
     """State and step of one Boussinesq Chebyshev CNAB2 dynamo run on the GPU."""
 
     def __init__(self, cfg, device="cuda", res: Resolution | None = None, fused_grid: bool = True):
+        for _ in self._construct(cfg, device, res, fused_grid):
+            pass
+
+    @classmethod
+    def staged(cls, cfg, device="cuda", res: Resolution | None = None, fused_grid: bool = True):
+        """(run, stages): the run object and a generator that builds it in three
+        stages (buffers, operator storage, job tables), one per next(), so that
+        a server can interleave the construction with other work."""
+        run = cls.__new__(cls)
+        return run, run._construct(cfg, device, res, fused_grid)
+
+    def _construct(self, cfg, device, res, fused_grid):
         self.p = p = cfg if isinstance(cfg, Params) else Params(cfg)
         p.check_step_supported()
         self.device = require_gpu(device)
@@ -110,7 +163,18 @@ class DynamoRun:
         self.L, self.N = L, N
         self.Nb = N - 2
         dev = self.device
-        z = lambda cols=N: torch.zeros(L, cols, dtype=C128, device=dev)
+        # every (L, cols) complex array of the run is a view of one zeroed pool,
+        # each starting on a 256-byte boundary
+        slot = lambda cols: -(-(L * cols) // 16) * 16
+        n_arr = len(_FIELDS) + 3 + 4 * len(_SYS) + 4 + 10
+        self._pool = torch.zeros(n_arr * slot(N) + slot(2 * N), dtype=C128, device=dev)
+        used = [0]
+
+        def z(cols=N):
+            a = self._pool[used[0]:used[0] + L * cols].view(L, cols)
+            used[0] += slot(cols)
+            return a
+
         self.f = {k: z() for k in _FIELDS}
         self.tmp = {k: z() for k in ("d2s", "d2z", "dddw")}
         self.old = {k: z() for k in _SYS}
@@ -142,9 +206,12 @@ class DynamoRun:
             self.G_o = r.grid_buffer(9, self.Nb)
             nFo = 9 * M * T * self.Nb * 2
             self.F_o = self.F_v[:nFo] if nFo <= self.F_v.numel() else r.fourier_buffer(9, self.Nb)
+        assert used[0] == self._pool.numel()
         self.cfl = torch.zeros(2 * N, dtype=torch.int64, device=dev)
         self.cfl_host = torch.zeros(2 * N, dtype=torch.int64).pin_memory()
+        yield
         self._build_operator_storage()
+        yield
         self._build_jobs()
         self.dt_hist = [0.0, 0.0]
         self.dtMax = 0.0
@@ -167,6 +234,8 @@ class DynamoRun:
         self._cfl_event = None
         self.dt_log = []         # dt used by each step
         self.dt_schedule = None  # optional step sizes to use instead of the Courant decision
+        self.n_step = 0          # steps taken
+        self._ctl = None         # device step control buffers (run_steps / step_async)
 
     # ------------------------------------------------------------ timing
     def enable_timers(self, on: bool = True):
@@ -217,8 +286,10 @@ class DynamoRun:
             with _lib.record() as rec:
                 if kind == "radial":
                     self._radial_launches()
-                else:
+                elif kind == "lm":
                     self._lm_launches()
+                else:
+                    self._step_launches()
             plan = self._plans[key] = rec.calls
         return key, plan
 
@@ -242,6 +313,18 @@ class DynamoRun:
             cur.wait_stream(side)
             self._graphs[key] = g
         return g
+
+    def prepare_graph(self, kind, par):
+        """Record and capture the launch plan of phase `kind` ("radial", "lm" or
+        "step") for explicit-history parity `par`, without running it."""
+        saved = (self._par, self.expl_cur, self.expl_prev)
+        self._par, self.expl_cur, self.expl_prev = par, self._E[par], self._E[1 - par]
+        try:
+            key, plan = self._plan(kind)
+            if self.use_graphs:
+                self._graph(key, plan)
+        finally:
+            self._par, self.expl_cur, self.expl_prev = saved
 
     def _exec(self, kind):
         """Run the radial or lm phase: recorded once per parity, then replayed
@@ -273,13 +356,10 @@ class DynamoRun:
         self.MT = {k: torch.zeros(lp1, self.K4, self.Rpad3, dtype=F64, device=dev)
                    for k in ("s", "z", "b", "j")}
         self.MT["wp"] = torch.zeros(lp1, self.K4wp, self.Rpad6, dtype=F64, device=dev)
-        rad = r.rad
-        CT, D1, D2, D3 = rad["CT"], rad["D1"], rad["D2"], rad["D3"]
-        up = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=F64).to(dev)
-        self.Op3 = up(np.concatenate([CT, D1 @ CT, D2 @ CT]))            # (3N, N)
-        self.Opw = up(np.concatenate([CT, D1 @ CT, D2 @ CT, D3 @ CT]))   # (4N, N)
-        self.Opp = up(np.concatenate([CT, D1 @ CT]))                     # (2N, N)
-        self.cheb = {k: up(rad[k]) for k in ("T", "T1", "T2", "T3")}
+        ops = self._radial_operators()
+        self.Op3, self.Opw, self.Opp, self.cheb = ops["Op3"], ops["Opw"], ops["Opp"], ops["cheb"]
+        self.MT_d1, self.MT_d1r2, self.MT_d12, self.MT_d123 = (
+            ops[k] for k in ("MT_d1", "MT_d1r2", "MT_d12", "MT_d123"))
         self.mat_nn = torch.empty(lp1, N, N, dtype=F64, device=dev)
         self.mat_wp = torch.empty(lp1, 2 * N, 2 * N, dtype=F64, device=dev)
         self.gj_work = torch.empty(lp1 * 2 * (2 * N) * (2 * N), dtype=F64, device=dev)
@@ -289,31 +369,48 @@ class DynamoRun:
         self._info_host = torch.zeros(len(_INFO_ROWS), lp1, dtype=I32).pin_memory()
         self._info_event = None
         self.defer_checks = False  # check the pivots of a rebuild at the next step (no host wait)
-        # uniform radial-derivative operators (transposed, padded)
+    def _radial_operators(self):
+        """Radial operators of the resolution, uploaded once and shared by every
+        run of it (read-only): the DCT-I / derivative stacks folded into the
+        inverses, the Chebyshev collocation matrices of the builders, and the
+        uniform radial-derivative operators (transposed, padded)."""
+        r, N, dev = self.res, self.N, self.device
+        ops = getattr(r, "_radial_ops", None)
+        if ops is not None:
+            return ops
+        rad = r.rad
+        CT, D1, D2, D3 = rad["CT"], rad["D1"], rad["D2"], rad["D3"]
+        up = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=F64).to(dev)
+        ops = dict(Op3=up(np.concatenate([CT, D1 @ CT, D2 @ CT])),            # (3N, N)
+                   Opw=up(np.concatenate([CT, D1 @ CT, D2 @ CT, D3 @ CT])),   # (4N, N)
+                   Opp=up(np.concatenate([CT, D1 @ CT])),                     # (2N, N)
+                   cheb={k: up(rad[k]) for k in ("T", "T1", "T2", "T3")})
         r2 = rad["r2"]
 
-        def mt(ops):  # list of (N, N) blocks stacked as output rows -> MT[k][r]
-            Op = np.concatenate(ops)
+        def mt(blocks):  # list of (N, N) blocks stacked as output rows -> MT[k][r]
+            Op = np.concatenate(blocks)
             R = Op.shape[0]
             Rp = -(-R // 64) * 64
             out = np.zeros((self.K4, Rp))
             out[:N, :R] = Op.T
             return up(out), R, Rp
 
-        self.MT_d1 = mt([D1])
-        self.MT_d1r2 = mt([D1 * r2[None, :]])
-        self.MT_d12 = mt([D1, D2])
-        self.MT_d123 = mt([D1, D2, D3])
+        ops.update(MT_d1=mt([D1]), MT_d1r2=mt([D1 * r2[None, :]]), MT_d12=mt([D1, D2]),
+                   MT_d123=mt([D1, D2, D3]))
+        r._radial_ops = ops
+        return ops
 
-    def _gemm(self, A, sAr, sAc, sAb, B, sBr, sBc, sBb, Cp, sCr, sCc, sCb, M, Nn, K, batch):
+    def _gemm(self, A, sAr, sAc, sAb, B, sBr, sBc, sBb, Cp, sCr, sCc, sCb, M, Nn, K, batch,
+              when=None):
         g = _lib.GemmArgs()
+        g.when = when
         g.A, g.B, g.C = A, B, Cp
         g.sAr, g.sAc, g.sAb, g.sBr, g.sBc, g.sBb, g.sCr, g.sCc, g.sCb = (
             sAr, sAc, sAb, sBr, sBc, sBb, sCr, sCc, sCb)
         g.M, g.Nn, g.K, g.batch, g.alpha = M, Nn, K, batch, 1.0
         _lib.call("mh_gemm_f64", _lib.ref(g), stream_handle())
 
-    def _build_kind(self, kind, out, wl0):
+    def _build_kind(self, kind, out, wl0, ctl=None):
         p, rad = self.p, self.res.rad
         a = _lib.BuildArgs()
         a.R, a.D1, a.D2, a.D3 = (ptr(self.cheb[k]) for k in ("T", "T1", "T2", "T3"))
@@ -322,7 +419,12 @@ class DynamoRun:
         a.N, a.l_max, a.n_cheb = self.N, p.l_max, p.n_cheb_max
         a.ktopv, a.kbotv, a.kind = p.ktopv, p.kbotv, kind
         a.rnorm, a.wimp, a.opr, a.opm = rad["rnorm"], wl0, p.opr, p.opm
+        if ctl is not None:
+            a.wimp_dev, a.when = ctl
         _lib.call("mh_build_matrices", _lib.ref(a), stream_handle())
+
+    def _zero(self, t, when=None):
+        _lib.call("mh_fill_f64", ptr(t), t.numel(), 0.0, when, stream_handle())
 
     def build_all_matrices(self, wimp_lin0: float):
         """Rebuild every per-degree operator for the implicit weight ``wimp_lin0``.
@@ -332,41 +434,54 @@ class DynamoRun:
         collocation matrices of all degrees at once, then fold the DCT-I and
         the radial derivatives into the inverses.
         """
+        self._rebuild_launches(wimp_lin0)
+        self.wimp_lin0 = wimp_lin0
+        self.n_rebuild += 1
+        self._queue_info_check()
+
+    def _rebuild_launches(self, wimp_lin0, ctl=None):
+        """The launches of a rebuild.  ctl = (wimp_dev, when) device pointers: the
+        weight is read on the device and every kernel returns at once unless
+        *when != 0 (the rebuild inside a captured step)."""
         N, lp1 = self.N, self.p.l_max + 1
         st = stream_handle()
+        when = None if ctl is None else ctl[1]
         for kind, key in ((0, "s"), (1, "z"), (2, "b"), (3, "j")):
             if key in ("b", "j") and not self.p.l_mag:
                 continue
-            self._build_kind(kind, self.mat_nn, wimp_lin0)
+            self._build_kind(kind, self.mat_nn, wimp_lin0, ctl)
             _lib.call("mh_gj_inverse", ptr(self.mat_nn), ptr(self.mat_nn), ptr(self.gj_work),
-                      ptr(self.gj_scal), N, lp1, 1, ptr(self.gj_info[_INFO_ROWS.index(key)]), st)
+                      ptr(self.gj_scal), N, lp1, 1, ptr(self.gj_info[_INFO_ROWS.index(key)]),
+                      when, st)
             MT = self.MT[key]
             # MT[l][k][r] = sum_j inv_l[j][k] Op3[r][j]
             self._gemm(ptr(self.mat_nn), 1, N, N * N, ptr(self.Op3), 1, N, 0,
-                       ptr(MT), self.Rpad3, 1, self.K4 * self.Rpad3, N, 3 * N, N, lp1)
+                       ptr(MT), self.Rpad3, 1, self.K4 * self.Rpad3, N, 3 * N, N, lp1, when)
             if key != "s":
-                MT[0].zero_()
+                self._zero(MT[0], when)
         # coupled (w, p)
-        self._build_kind(4, self.mat_wp, wimp_lin0)
+        self._build_kind(4, self.mat_wp, wimp_lin0, ctl)
         _lib.call("mh_gj_inverse", ptr(self.mat_wp), ptr(self.mat_wp), ptr(self.gj_work),
-                  ptr(self.gj_scal), 2 * N, lp1, 2, ptr(self.gj_info[_INFO_ROWS.index("wp")]), st)
+                  ptr(self.gj_scal), 2 * N, lp1, 2, ptr(self.gj_info[_INFO_ROWS.index("wp")]),
+                  when, st)
         MT = self.MT["wp"]
         sb = self.K4wp * self.Rpad6
         self._gemm(ptr(self.mat_wp), 1, 2 * N, 4 * N * N, ptr(self.Opw), 1, N, 0,
-                   ptr(MT), self.Rpad6, 1, sb, 2 * N, 4 * N, N, lp1)
+                   ptr(MT), self.Rpad6, 1, sb, 2 * N, 4 * N, N, lp1, when)
         self._gemm(ptr(self.mat_wp) + 8 * N * 2 * N, 1, 2 * N, 4 * N * N, ptr(self.Opp), 1, N, 0,
-                   ptr(MT) + 8 * 4 * N, self.Rpad6, 1, sb, 2 * N, 2 * N, N, lp1)
-        MT[0].zero_()
+                   ptr(MT) + 8 * 4 * N, self.Rpad6, 1, sb, 2 * N, 2 * N, N, lp1, when)
+        self._zero(MT[0], when)
         # l = 0 pressure: p(l=0) from the p0 system acting on the pressure half
         p0 = self.mat_nn[0]
-        self._build_kind(5, p0, wimp_lin0)
+        self._build_kind(5, p0, wimp_lin0, ctl)
         _lib.call("mh_gj_inverse", ptr(p0), ptr(p0), ptr(self.gj_work), ptr(self.gj_scal), N, 1, 1,
-                  ptr(self.gj_info[_INFO_ROWS.index("p0")]), st)
+                  ptr(self.gj_info[_INFO_ROWS.index("p0")]), when, st)
         self._gemm(ptr(p0), 1, N, 0, ptr(self.Opp), 1, N, 0,
-                   ptr(MT) + 8 * (N * self.Rpad6 + 4 * N), self.Rpad6, 1, 0, N, 2 * N, N, 1)
-        self.wimp_lin0 = wimp_lin0
-        self.n_rebuild += 1
-        # pivots: one host read for all systems, now or (defer_checks) at the next step
+                   ptr(MT) + 8 * (N * self.Rpad6 + 4 * N), self.Rpad6, 1, 0, N, 2 * N, N, 1, when)
+
+    def _queue_info_check(self):
+        """Queue the host copy of the pivot flags of all systems; they are checked
+        now or (defer_checks) at the next step."""
         self._info_host.copy_(self.gj_info, non_blocking=True)
         if self._info_event is None:
             self._info_event = torch.cuda.Event()
@@ -401,27 +516,15 @@ class DynamoRun:
         roff = r.maps["l_rows_off"]
 
         def job(X, ldx, rows_ptr, nrows, MT, K, R, Rpad, Y, ldy, rseg):
-            j = _lib.RowJob()
-            j.X, j.rows, j.MT = X, rows_ptr, MT
-            for i, y in enumerate(Y):
-                j.Y[i] = y
-            j.ldx, j.ldy, j.K, j.R, j.Rpad, j.rseg, j.nrows = ldx, ldy, K, R, Rpad, rseg, nrows
-            return j
+            """Job columns; rows_ptr, nrows and MT may be per-job arrays."""
+            return dict(X=X, ldx=ldx, rows=rows_ptr, nrows=nrows, MT=MT, K=K, R=R, Rpad=Rpad,
+                        Y=list(Y), ldy=ldy, rseg=rseg)
 
-        def tiles_for(jobs):
-            t = []
-            for ji, j in enumerate(jobs):
-                for r0 in range(0, j.nrows, 32):
-                    for c0 in range(0, j.R, 64):
-                        t.append((ji, r0, c0))
-            return t
-
-        def upload(jobs):
-            arr = (_lib.RowJob * len(jobs))(*jobs)
-            raw = bytes(arr)
-            dj = torch.frombuffer(bytearray(raw), dtype=torch.uint8).to(self.device)
-            tl = torch.as_tensor(np.asarray(tiles_for(jobs), np.int32).reshape(-1)).to(self.device)
-            return dj, tl, len(tiles_for(jobs))
+        def upload(specs):
+            arr, tiles = pack_row_jobs(specs)
+            dj = torch.frombuffer(bytearray(arr.tobytes()), dtype=torch.uint8).to(self.device)
+            tl = torch.as_tensor(tiles.reshape(-1)).to(self.device)
+            return dj, tl, int(tiles.shape[0])
 
         f, tmp = self.f, self.tmp
         # lm loop, stage 1: S, Z, B, J per degree
@@ -431,21 +534,18 @@ class DynamoRun:
         if self.p.l_mag:
             systems += [("b", self.rhs["b"], (f["b"], f["db"], f["ddb"])),
                         ("j", self.rhs["j"], (f["aj"], f["dj"], f["ddj"]))]
+        ls = np.arange(lp1, dtype=np.int64)
+        row_ptrs = ptr(rows) + 4 * np.asarray(roff[:lp1], np.int64)
         for key, X, Y in systems:
             MT = self.MT[key]
-            for l in range(lp1):
-                jobs.append(job(ptr(X), N, ptr(rows) + 4 * int(roff[l]), l + 1,
-                                ptr(MT) + 8 * l * self.K4 * self.Rpad3, N, 3 * N, self.Rpad3,
-                                [ptr(y) for y in Y], N, N))
+            jobs.append(job(ptr(X), N, row_ptrs, ls + 1, ptr(MT) + 8 * ls * self.K4 * self.Rpad3,
+                            N, 3 * N, self.Rpad3, [ptr(y) for y in Y], N, N))
         self.jobs1 = upload(jobs)
         # lm loop, stage 2: coupled (w, p) per degree
-        jobs = []
         Y = [f["w"], f["dw"], f["ddw"], tmp["dddw"], f["p"], f["dp"]]
-        for l in range(lp1):
-            jobs.append(job(ptr(self.rhs["wp"]), 2 * N, ptr(rows) + 4 * int(roff[l]), l + 1,
-                            ptr(self.MT["wp"]) + 8 * l * self.K4wp * self.Rpad6, 2 * N, 6 * N,
-                            self.Rpad6, [ptr(y) for y in Y], N, N))
-        self.jobs2 = upload(jobs)
+        self.jobs2 = upload([job(ptr(self.rhs["wp"]), 2 * N, row_ptrs, ls + 1,
+                                 ptr(self.MT["wp"]) + 8 * ls * self.K4wp * self.Rpad6, 2 * N,
+                                 6 * N, self.Rpad6, [ptr(y) for y in Y], N, N)])
         # radial loop: derivatives of the flux terms
         nl = self.nl
         jobs = [job(ptr(nl["dVSr"]), N, 0, L, ptr(self.MT_d1[0]), N, N, self.MT_d1[2],
@@ -727,6 +827,7 @@ class DynamoRun:
             dt_new = float(self.dt_schedule[n_time_step - 1])
             l_new_dt = dt_new != dt
         self.dt_log.append(dt_new)
+        self.n_step = n_time_step
         self.dt_hist = [dt_new, self.dt_hist[0]]
         wl0 = self.p.alpha * dt_new
         if n_time_step == 1 or l_new_dt or self.wimp_lin0 != wl0:
@@ -734,6 +835,137 @@ class DynamoRun:
         self.lm_loop()
         self.time += dt_new
         return dt_new
+
+    # ------------------------------------------------------------ device step control
+    LOG_CAP = 1024  # dt log ring on the device; the host reads it at least this often
+
+    def _ctl_buffers(self):
+        """Device state of the step control, the Courant spacings, the dt log
+        ring and the replayed step sizes (created once; re-uploaded in place)."""
+        dev, rad = self.device, self.res.rad
+        c = self._ctl
+        if c is None:
+            up = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=F64).to(dev)
+            c = self._ctl = dict(state=torch.zeros(C.sizeof(_lib.StepState), dtype=torch.uint8,
+                                                   device=dev),
+                                 delxr2=up(rad["delxr2"]), delxh2=up(rad["delxh2"]),
+                                 log=torch.zeros(self.LOG_CAP, dtype=F64, device=dev),
+                                 sched=None, n_sched=0, pulled=0)
+        sch = self.dt_schedule
+        n = 0 if sch is None else len(sch)
+        if n != c["n_sched"]:
+            # the schedule's length is a launch argument: new plans and graphs
+            c["sched"] = torch.zeros(max(n, 1), dtype=F64, device=dev)
+            c["n_sched"] = n
+            for key in [k for k in self._plans if k[0] == "step"]:
+                self._plans.pop(key)
+                self._graphs.pop(key, None)
+        if n:
+            c["sched"].copy_(torch.as_tensor(np.asarray(sch, dtype=np.float64)))
+        return c
+
+    def _ctl_launch(self):
+        c = self._ctl
+        a = _lib.StepCtlArgs()
+        a.cfl, a.delxr2, a.delxh2 = ptr(self.cfl), ptr(c["delxr2"]), ptr(c["delxh2"])
+        a.state, a.w = ptr(c["state"]), ptr(self.wdev)
+        if c["n_sched"]:
+            a.dt_schedule, a.n_schedule = ptr(c["sched"]), c["n_sched"]
+        a.dt_log, a.log_cap = ptr(c["log"]), self.LOG_CAP
+        a.info, a.info_rows, a.info_cols = ptr(self.gj_info), self.gj_info.shape[0], self.gj_info.shape[1]
+        a.N, a.alpha, a.BuoFac = self.N, self.p.alpha, self.p.BuoFac
+        with self._t("step_ctl"):
+            _lib.call("mh_step_control", _lib.ref(a), stream_handle())
+
+    def _step_launches(self):
+        """A whole step as one launch sequence: radial loop, device step control,
+        the rebuild launches (no-ops unless the step control asks for a
+        rebuild) and the lm loop."""
+        self._radial_launches()
+        self._ctl_launch()
+        base = ptr(self._ctl["state"])
+        with self._t("rebuild"):
+            self._rebuild_launches(0.0, (base + _lib.StepState.wimp_lin0.offset,
+                                         base + _lib.StepState.rebuild.offset))
+        self._lm_launches()
+
+    def _ctl_push(self, dt):
+        """Upload the host's stepping state (before device-controlled steps)."""
+        c = self._ctl_buffers()
+        st = _lib.StepState()
+        st.dt = float(dt)
+        st.dtMax = self.dtMax
+        st.dt1, st.dt2 = self.dt_hist
+        st.wimp_lin0 = math.nan if self.wimp_lin0 is None else self.wimp_lin0
+        st.time = self.time
+        st.n_step, st.n_rebuild = self.n_step, self.n_rebuild
+        st.rebuild = 0
+        host = torch.frombuffer(bytearray(bytes(st)), dtype=torch.uint8)
+        c["state"].copy_(host)   # synchronous upload from pageable memory
+        c["pulled"] = self.n_step
+
+    def _ctl_pull(self):
+        """Read the device stepping state and the dt log back (synchronises the
+        stream), check the pivots, and return the dt of the last step."""
+        c = self._ctl
+        st = _lib.StepState.from_buffer_copy(c["state"].cpu().numpy().tobytes())
+        n_new = st.n_step - c["pulled"]
+        if n_new > self.LOG_CAP:
+            raise RuntimeError("dt log overrun: read the device state at least every "
+                               f"{self.LOG_CAP} steps")
+        log = c["log"].cpu().numpy()
+        for n in range(c["pulled"] + 1, st.n_step + 1):
+            self.dt_log.append(float(log[(n - 1) % self.LOG_CAP]))
+        c["pulled"] = st.n_step
+        self.n_step, self.n_rebuild, self.time = st.n_step, st.n_rebuild, st.time
+        self.dt_hist = [st.dt1, st.dt2]
+        self.wimp_lin0 = st.wimp_lin0
+        if st.sing_step:
+            name = _INFO_ROWS[st.sing_sys]
+            raise AssertionError(f"Singular {name} matrix for l={st.sing_l}, info={st.sing_info} "
+                                 f"(rebuild of step {st.sing_step})")
+        if st.rebuild:  # the last step's rebuild has not been scanned on the device
+            self._info_host.copy_(self.gj_info, non_blocking=True)
+            if self._info_event is None:
+                self._info_event = torch.cuda.Event()
+            self._info_event.record()
+            self._info_pending = True
+            self._check_info()
+        return st.dt
+
+    def step_async(self):
+        """Queue one whole step with the decisions on the device (one HIP graph
+        when use_graphs); no host wait.  Call ctl_begin(dt) once before and
+        ctl_end() to bring the stepping state back to the host."""
+        self._exec("step")
+        self.expl_cur, self.expl_prev = self.expl_prev, self.expl_cur
+        self._par ^= 1
+        self._ctl["queued"] += 1
+        if self._ctl["queued"] - self._ctl["pulled"] >= self.LOG_CAP:
+            self._ctl_pull()
+
+    def ctl_begin(self, dt: float):
+        """Start device-controlled stepping; dt: the dt passed to the first
+        step's decision, as for one_step."""
+        self._check_info()
+        self._ctl_push(dt)
+        self._ctl["queued"] = self.n_step
+
+    def ctl_end(self) -> float:
+        """Finish device-controlled stepping: read the state back (dt_log, time,
+        n_rebuild, dt history), check the pivots; returns the last step's dt."""
+        return self._ctl_pull()
+
+    def run_steps(self, n_steps: int, dt: float) -> float:
+        """Advance n_steps steps with every decision on the device: each step is
+        one launch sequence (one HIP graph replay when use_graphs) and the host
+        never waits inside the loop.  Same decisions, weights and results as
+        n_steps calls of one_step; dt as for one_step's first call.  Returns the
+        dt of the last step."""
+        self.ctl_begin(dt)
+        for _ in range(n_steps):
+            self.step_async()
+        return self.ctl_end()
 
     # ------------------------------------------------------------ diagnostics
     def _energy_out(self, by_m: bool):

@@ -6,15 +6,18 @@ steps) arriving at a given time.  The server
 * admits requests in arrival order while a slot is free and the run's HBM
   footprint fits the budget (admission by footprint, continuous: a request
   that arrives while others are stepping is admitted as soon as a slot and
-  memory free up);
+  memory free up).  Admission is done in stages (device buffers, operator
+  storage, job tables, initial state, capture of the two step graphs) while
+  the device executes a round of the running runs, so the host work of an
+  admission overlaps their steps;
 * keeps every admitted run's spectral state, IMEX history and operators
   resident in HBM until its last step, each run on its own HIP stream (default
-  priority), with the step captured as HIP graphs;
-* steps the active runs in rounds (every run advances one step per round, as
-  iteration-level batching): a step is the radial phase (queued with the
-  asynchronous copy of its Courant maxima) followed, once those maxima are on
-  the host, by the dt decision and the lm phase; all runs' radial phases are
-  queued before any host wait, so the runs' kernels overlap on the device;
+  priority);
+* steps the runs with every decision on the device: a step is one HIP graph
+  (radial loop, Courant dt decision, operator rebuild when dt changes, lm
+  loop), so the host never waits inside a step.  The active runs advance in
+  rounds, one step each per round (iteration-level batching); their streams
+  overlap on the device;
 * reports runs/hour and the p50/p99 per-step latency (wall time between the
   ends of a run's consecutive rounds).
 
@@ -95,12 +98,12 @@ def footprint_bytes(cfg) -> int:
 
 
 class _Slot:
-    def __init__(self, req: Request, run, stream, t_adm: float, fp: int):
-        self.req, self.run, self.stream = req, run, stream
+    def __init__(self, req: Request, fp: int):
+        self.req, self.fp = req, fp
+        self.run = self.stream = None
         self.n = 0                      # completed steps
-        self.dt = run.dtMax
-        self.last_done = t_adm
-        self.res = Result(req.rid, req.cfg, req.n_steps, req.arrival, t_adm, 0.0, footprint=fp)
+        self.last_done = 0.0
+        self.res = None
 
 
 class EnsembleServer:
@@ -108,7 +111,7 @@ class EnsembleServer:
 
     hbm_budget: bytes available to resident runs (default: 80% of the free
     device memory at start); max_slots: runs stepping at once; use_graphs:
-    replay each run's phases as HIP graphs.
+    replay each step as a HIP graph.
     """
 
     def __init__(self, device="cuda", hbm_budget: int | None = None, max_slots: int = 16,
@@ -134,35 +137,62 @@ class EnsembleServer:
             r = self._res_cache[key] = Resolution(p, self.device)
         return r
 
-    def _admit(self, req: Request, fp: int) -> _Slot:
-        """Build and initialise a run on its own stream."""
+    def warm(self, cfgs):
+        """Build the shared device tables of every resolution in `cfgs` ahead of
+        serving (as a server loads what it serves before taking requests)."""
+        from .params import Params
+        for cfg in cfgs:
+            self._resolution(Params(dict(cfg)))
+        self.torch.cuda.synchronize(self.device)
+
+    def _admit_stages(self, sl: _Slot):
+        """Admission of one run in stages (a generator: one stage per next()):
+        device buffers, operator storage, job tables (each on the run's own
+        stream), initial state and device step control, then the capture of
+        each of its two step graphs."""
         from .dynamo import DynamoRun
         from .params import Params
         torch = self.torch
-        t0 = self.clock()
+        req = sl.req
         p = Params(dict(req.cfg))
-        stream = torch.cuda.Stream(device=self.device)
-        with torch.cuda.stream(stream):
-            run = DynamoRun(p, self.device, res=self._resolution(p))
+        sl.stream = torch.cuda.Stream(device=self.device)
+        res = self._resolution(p)
+        run, build = DynamoRun.staged(p, self.device, res=res)
+        while True:
+            with torch.cuda.stream(sl.stream):
+                done = next(build, StopIteration) is StopIteration
+            if done:
+                break
+            yield
+        yield
+        with torch.cuda.stream(sl.stream):
             run.use_graphs = self.use_graphs
-            run.defer_checks = True  # singular pivots surface at the next step, without a host wait
+            run.defer_checks = True
             run.initialize_fields(noise_amp=req.noise_amp, noise_seed=req.noise_seed)
             run.setup_initial_state()
             run.initialize_dt(p.dtmax)
-        self.admit_s.append(self.clock() - t0)
-        return _Slot(req, run, stream, 0.0, fp)
+            run.ctl_begin(run.dtMax)
+        if self.use_graphs:
+            for par in (0, 1):
+                yield
+                with torch.cuda.stream(sl.stream):
+                    run.prepare_graph("step", par)
+        sl.run = run
 
     def serve(self, requests, realtime: bool = True, keep_fields: bool = False):
         """Serve `requests` (arrival times honoured when realtime); returns the
         list of Result in completion order and a summary dict (keep_fields:
         attach copies of each run's final s, w, z, b, aj fields).
 
-        Work proceeds in rounds, as iteration-level batching: in a round every
-        active run advances exactly one step (all radial phases are queued
-        first, then each run's lm phase as soon as its Courant maxima arrive),
-        and arrivals are admitted between rounds.  A run's step latency is the
+        Work proceeds in rounds (iteration-level batching): every active run
+        queues exactly one step (one HIP graph) on its stream; while the round
+        runs on the device the host advances the admission in progress by
+        stages (arrival order, by free slot and footprint); the round ends when
+        every run's step event has completed.  A run's step latency is the
         time between the ends of its consecutive rounds (for its first step:
-        from admission)."""
+        from the end of its admission).  A run admitted during a round steps
+        from the next round on."""
+        torch = self.torch
         pending = sorted(requests, key=lambda r: (r.arrival, r.rid))
         for r in pending:
             if footprint_bytes(r.cfg) > self.budget:
@@ -170,76 +200,88 @@ class EnsembleServer:
                                   f"more than the whole budget {self.budget}")
         active: list[_Slot] = []
         done: list[Result] = []
-        used = 0
-        self.rounds = []
-        self.admit_s = []  # host time of each admission (run construction and set-up)
+        used = [0]
+        admitting = [None]            # (slot, stage generator, start time)
+        self.occupancy = []           # (time, active runs) at the end of each round
+        self.admit_s = []             # host time of each admission (all its stages)
+        self.stage_s = []             # host time of each admission stage
         t0 = self.clock()
-        while pending or active:
-            now = self.clock() - t0
-            # admission between rounds: arrival order, by free slot and footprint
-            while pending and len(active) < self.max_slots:
+
+        def admission_stage():
+            """Start an admission if one may start, else advance the current one
+            by a stage; True if host work was done."""
+            if admitting[0] is None:
+                if not pending or len(active) >= self.max_slots:
+                    return False
                 req = pending[0]
-                if realtime and req.arrival > now:
-                    break
                 fp = footprint_bytes(req.cfg)
-                if used + fp > self.budget:
-                    break
+                if (realtime and req.arrival > self.clock() - t0) or used[0] + fp > self.budget:
+                    return False
                 pending.pop(0)
-                used += fp
-                sl = self._admit(req, fp)
-                sl.last_done = sl.res.admitted = self.clock() - t0
+                used[0] += fp
+                sl = _Slot(req, fp)
+                admitting[0] = (sl, self._admit_stages(sl), self.clock())
+            sl, gen, ta = admitting[0]
+            ts = self.clock()
+            try:
+                next(gen)
+                self.stage_s.append(self.clock() - ts)
+            except StopIteration:
+                self.stage_s.append(self.clock() - ts)
+                self.admit_s.append(self.clock() - ta)
+                admitting[0] = None
+                sl.last_done = self.clock() - t0
+                sl.res = Result(sl.req.rid, sl.req.cfg, sl.req.n_steps, sl.req.arrival,
+                                sl.last_done, 0.0, footprint=sl.fp)
                 active.append(sl)
-            if not active:
-                time.sleep(max(0.0, min(0.01, pending[0].arrival - (self.clock() - t0))))
+            return True
+
+        while pending or active or admitting[0] is not None:
+            stepping = list(active)
+            if not stepping:
+                if not admission_stage() and pending:
+                    time.sleep(max(0.0, min(0.01, pending[0].arrival - (self.clock() - t0))))
                 continue
-            used -= self._round(active, done, t0, keep_fields)
+            events = []
+            for sl in stepping:
+                with torch.cuda.stream(sl.stream):
+                    try:
+                        sl.run.step_async()
+                    except AssertionError as e:  # a singular rebuild read back early
+                        raise AssertionError(f"request {sl.req.rid}: {e}") from e
+                    ev = torch.cuda.Event()
+                    ev.record()
+                events.append(ev)
+            # host admission work overlaps the round on the device
+            while not all(ev.query() for ev in events):
+                if not admission_stage():
+                    time.sleep(0)
+            t_end = self.clock() - t0
+            self.occupancy.append((t_end, len(stepping)))
+            for sl in stepping:
+                sl.res.step_latencies.append(t_end - sl.last_done)
+                sl.last_done = t_end
+                sl.n += 1
+                if sl.n >= sl.req.n_steps:
+                    self._retire(sl, keep_fields)
+                    sl.res.finished = self.clock() - t0
+                    done.append(sl.res)
+                    active.remove(sl)
+                    used[0] -= sl.fp
         wall = self.clock() - t0
         return done, summarize(done, wall)
 
-    def _round(self, active, done, t0, keep_fields) -> int:
-        """One round: every active run advances one step (all radial phases are
-        queued first, then each run's lm phase as soon as its Courant maxima
-        arrive).  Finished runs leave; returns the HBM footprint they free."""
-        torch = self.torch
-        for sl in active:
-            with torch.cuda.stream(sl.stream):
-                sl.run.begin_step()
-        waiting = list(active)
-        while waiting:
-            for sl in list(waiting):
-                if sl.run.cfl_ready():
-                    with torch.cuda.stream(sl.stream):
-                        try:
-                            sl.dt = sl.run.finish_step(sl.n + 1, sl.dt)
-                        except AssertionError as e:
-                            raise AssertionError(f"request {sl.req.rid} step {sl.n + 1} "
-                                                 f"(dt {sl.dt:.6g}): {e}") from e
-                    sl.n += 1
-                    waiting.remove(sl)
-            if waiting:
-                time.sleep(0)
-        t_end = self.clock() - t0
-        self.rounds.append((t_end, len(active)))
-        freed = 0
-        for sl in list(active):
-            sl.res.step_latencies.append(t_end - sl.last_done)
-            sl.last_done = t_end
-            if sl.n >= sl.req.n_steps:
-                with torch.cuda.stream(sl.stream):
-                    try:
-                        sl.run._check_info()  # the last rebuild's pivots
-                    except AssertionError as e:
-                        raise AssertionError(f"request {sl.req.rid} step {sl.n}: {e}") from e
-                    sl.res.energies = sl.run.energies()
-                    if keep_fields:
-                        sl.res.final_fields = {k: sl.run.f[k].clone()
-                                               for k in ("s", "w", "z", "b", "aj")}
-                sl.res.finished = self.clock() - t0
-                done.append(sl.res)
-                active.remove(sl)
-                freed += sl.res.footprint
-                sl.run = None
-        return freed
+    def _retire(self, sl: _Slot, keep_fields: bool):
+        """Bring a finished run's state back (pivot checks), read its energies."""
+        with self.torch.cuda.stream(sl.stream):
+            try:
+                sl.run.ctl_end()
+            except AssertionError as e:
+                raise AssertionError(f"request {sl.req.rid}: {e}") from e
+            sl.res.energies = sl.run.energies()
+            if keep_fields:
+                sl.res.final_fields = {k: sl.run.f[k].clone() for k in ("s", "w", "z", "b", "aj")}
+        sl.run = None
 
 
 def summarize(results, wall: float) -> dict:
@@ -265,7 +307,10 @@ def summarize(results, wall: float) -> dict:
                 p99_over_p50=float(np.percentile(lat, 99) / np.percentile(lat, 50)) if steps else math.nan)
 
 
-def serve_sequential(requests, device="cuda", use_graphs=True, keep_fields=False):
-    """Baseline: the same runs one after another (no overlap, arrivals ignored)."""
+def serve_sequential(requests, device="cuda", use_graphs=True, keep_fields=False, warm=False):
+    """Baseline: the same runs one after another (no overlap, arrivals ignored;
+    warm: build the device tables of their resolutions first)."""
     srv = EnsembleServer(device=device, max_slots=1, use_graphs=use_graphs)
+    if warm:
+        srv.warm([r.cfg for r in requests])
     return srv.serve(requests, realtime=False, keep_fields=keep_fields)
