@@ -1,0 +1,141 @@
+"""Host-side bounds check of the recorded launch plans (no GPU).
+
+A DynamoRun is built on a CPU stand-in device with every C-ABI call recorded
+instead of run; the test then checks, from the recorded arguments, that every
+element range the rebuild kernels (matrix build, Gauss-Jordan inverse, fold
+GEMM, fill), the row-operator jobs and the step control touch lies inside a
+tensor of the run or its resolution, for every resolution the ensemble
+workload serves and the headline one.
+"""
+
+import ctypes as C
+
+import numpy as np
+import pytest
+import torch
+
+from magic_amd import _lib
+from magic_amd import device as devmod
+from magic_amd import dynamo as dyn
+
+
+class _NoStream:
+    def synchronize(self):
+        pass
+
+
+@pytest.fixture
+def cpu_standin(monkeypatch):
+    def require(d):
+        _lib.load()
+        return torch.device("cpu")
+    monkeypatch.setattr(devmod, "require_gpu", require)
+    monkeypatch.setattr(dyn, "require_gpu", require)
+    monkeypatch.setattr(devmod, "stream_handle", lambda: C.c_void_p(0))
+    monkeypatch.setattr(dyn, "stream_handle", lambda: C.c_void_p(0))
+    monkeypatch.setattr(torch.cuda, "current_stream", lambda *a, **k: _NoStream())
+    monkeypatch.setattr(torch.Tensor, "pin_memory", lambda self: self)
+
+
+def _extents(run):
+    out = []
+
+    def add(v):
+        if isinstance(v, torch.Tensor):
+            st = v.untyped_storage()
+            out.append((st.data_ptr(), st.data_ptr() + st.nbytes()))
+        elif isinstance(v, dict):
+            for w in v.values():
+                add(w)
+        elif isinstance(v, (tuple, list)):
+            for w in v:
+                add(w)
+
+    for obj in (run, run.res):
+        for v in vars(obj).values():
+            add(v)
+    return out
+
+
+def _inside(ext, lo, hi, what):
+    assert any(a <= lo and hi <= b for a, b in ext), f"{what}: [{lo:#x}, {hi:#x}) outside every tensor"
+
+
+def _check_rebuild_calls(calls, ext):
+    n = 0
+    for _, name, args in calls:
+        if name == "mh_build_matrices":
+            a = C.cast(args[0], C.POINTER(_lib.BuildArgs)).contents
+            N, lp1 = a.N, a.l_max + 1
+            size = {4: lp1 * 4 * N * N, 5: N * N}.get(a.kind, lp1 * N * N)
+            _inside(ext, a.out, a.out + 8 * size, f"build kind {a.kind}")
+            for p in (a.R, a.D1, a.D2, a.D3):
+                _inside(ext, p, p + 8 * N * N, "Chebyshev matrix")
+            for p in (a.or1, a.or2):
+                _inside(ext, p, p + 8 * N, "radial factor")
+        elif name == "mh_gj_inverse":
+            A, out, work, scal, n_, nb, pre, info = args[:8]
+            assert n_ <= 208  # register kernels (no work space)
+            _inside(ext, A, A + 8 * nb * n_ * n_, "inverse input")
+            _inside(ext, out, out + 8 * nb * n_ * n_, "inverse output")
+            _inside(ext, info, info + 4 * nb, "pivot flags")
+        elif name == "mh_gemm_f64":
+            g = C.cast(args[0], C.POINTER(_lib.GemmArgs)).contents
+            last = g.batch - 1
+            for p, sb, sr, sc, rows, cols in ((g.A, g.sAb, g.sAr, g.sAc, g.M, g.K),
+                                              (g.B, g.sBb, g.sBr, g.sBc, g.K, g.Nn),
+                                              (g.C, g.sCb, g.sCr, g.sCc, g.M, g.Nn)):
+                hi = p + 8 * (last * sb + (rows - 1) * sr + (cols - 1) * sc + 1)
+                _inside(ext, p, hi, "gemm operand")
+        elif name == "mh_fill_f64":
+            p, nf = args[0], args[1]
+            _inside(ext, p, p + 8 * nf, "fill")
+        else:
+            continue
+        n += 1
+    return n
+
+
+def _check_row_jobs(run, ext):
+    for key in ("jobs1", "jobs2", "jobs_td", "jobs_setup"):
+        dj, tl, nt = getattr(run, key)
+        jobs = np.frombuffer(dj.numpy().tobytes(), dtype=dyn._ROWJOB_DTYPE)
+        tiles = tl.numpy().reshape(-1, 3)
+        assert len(tiles) == nt and tiles[:, 0].max() < len(jobs)
+        for j in jobs:
+            nrows, ldx, ldy, K, R, Rpad, rseg = (int(j[k]) for k in
+                                                 ("nrows", "ldx", "ldy", "K", "R", "Rpad", "rseg"))
+            if j["rows"]:
+                _inside(ext, int(j["rows"]), int(j["rows"]) + 4 * nrows, f"{key} rows")
+                last = int(np.frombuffer(C.string_at(int(j["rows"]), 4 * nrows), np.int32).max())
+            else:
+                last = nrows - 1
+            _inside(ext, int(j["X"]), int(j["X"]) + 8 * (2 * last * ldx + 2 * K), f"{key} X")
+            _inside(ext, int(j["MT"]), int(j["MT"]) + 8 * (-(-K // 4) * 4) * Rpad, f"{key} MT")
+            for s in range(-(-R // rseg)):
+                y = int(j["Y"][s])
+                _inside(ext, y, y + 8 * (2 * last * ldy + 2 * rseg), f"{key} Y{s}")
+
+
+@pytest.mark.parametrize("l_max,n_r", [(16, 17), (32, 33), (64, 49), (85, 65), (127, 65), (383, 97)])
+def test_recorded_launches_stay_inside_their_tensors(cpu_standin, l_max, n_r):
+    cfg = dict(l_max=l_max, n_r_max=n_r, ek=1e-4, ra=1e6, pr=1.0, prmag=5.0, radratio=0.35,
+               dtmax=1e-4)
+    with _lib.record():
+        run = dyn.DynamoRun(cfg, device="cpu")
+        run.initialize_dt(cfg["dtmax"])
+        run._ctl_push(cfg["dtmax"])
+    ext = _extents(run)
+    _check_row_jobs(run, ext)
+    base = devmod.ptr(run._ctl["state"])
+    with _lib.record() as rec:
+        run._rebuild_launches(0.3)
+        run._rebuild_launches(0.0, (base + _lib.StepState.wimp_lin0.offset,
+                                    base + _lib.StepState.rebuild.offset))
+        run._ctl_launch()
+    assert _check_rebuild_calls(rec.calls, ext) >= 2 * 17
+    a = C.cast(rec.calls[-1][2][0], C.POINTER(_lib.StepCtlArgs)).contents
+    _inside(ext, a.cfl, a.cfl + 16 * a.N, "Courant maxima")
+    _inside(ext, a.info, a.info + 4 * a.info_rows * a.info_cols, "pivot flags")
+    _inside(ext, a.dt_log, a.dt_log + 8 * a.log_cap, "dt log")
+    _inside(ext, a.state, a.state + C.sizeof(_lib.StepState), "step state")
