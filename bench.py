@@ -143,6 +143,7 @@ def run_case(precision, steps, warmup, dt_schedule=None, section_steps=20):
     out["secs"] = run.timer_summary()
     out["dt_log"] = list(run.dt_log)
     run.enable_timers(False)
+    run.close()
     del run
     torch.cuda.empty_cache()
     return out

@@ -139,3 +139,29 @@ def test_recorded_launches_stay_inside_their_tensors(cpu_standin, l_max, n_r):
     _inside(ext, a.info, a.info + 4 * a.info_rows * a.info_cols, "pivot flags")
     _inside(ext, a.dt_log, a.dt_log + 8 * a.log_cap, "dt log")
     _inside(ext, a.state, a.state + C.sizeof(_lib.StepState), "step state")
+
+
+def test_run_is_freed_with_its_last_reference(cpu_standin):
+    """Recording launch plans (sections, timers) leaves no reference cycle: a run
+    and its buffers (and, on the GPU, its captured graphs) are released when the
+    last reference goes, never later at an arbitrary garbage collection, which
+    could fall inside another run's graph capture."""
+    import gc
+    import weakref
+    cfg = dict(l_max=16, n_r_max=17, ek=1e-3, ra=1e5, pr=1.0, prmag=5.0, radratio=0.35,
+               dtmax=1e-4)
+    was = gc.isenabled()
+    gc.disable()
+    try:
+        with _lib.record():
+            run = dyn.DynamoRun(cfg, device="cpu")
+            run.initialize_dt(cfg["dtmax"])
+            run._ctl_push(cfg["dtmax"])
+        for kind in ("step", "radial", "lm"):
+            run._plan(kind)
+        ref = weakref.ref(run)
+        del run
+        assert ref() is None
+    finally:
+        if was:
+            gc.enable()

@@ -35,6 +35,7 @@ lm loop (``step_time.py:222-394``, ``update_b.py:557-631``)
 from __future__ import annotations
 
 import ctypes as C
+import gc
 import math
 
 import numpy as np
@@ -138,6 +139,38 @@ def conduction_state(p: Params, res: Resolution):
 _INFO_ROWS = ("s", "z", "b", "j", "wp", "p0")
 
 
+class _Section:
+    """Context of one kernel section: names the calls recorded into a launch
+    plan, or brackets eager launches with HIP events when timers are on.  A
+    module-level class, so that a section holds the run without forming a
+    reference cycle (a run must be freed as soon as its last reference goes,
+    not at some later garbage collection)."""
+
+    __slots__ = ("run", "name", "prev", "ev")
+
+    def __init__(self, run, name):
+        self.run, self.name = run, name
+
+    def __enter__(self):
+        rec = _lib.recording()
+        if rec is not None:
+            self.prev = rec.section
+            rec.section = self.name
+        elif self.run.timers is not None:
+            self.ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+            self.ev[0].record()
+        return self
+
+    def __exit__(self, *exc):
+        rec = _lib.recording()
+        if rec is not None:
+            rec.section = self.prev
+        elif self.run.timers is not None:
+            self.ev[1].record()
+            self.run.timers.setdefault(self.name, []).append(self.ev)
+        return False
+
+
 class DynamoRun:
     """State and step of one Boussinesq Chebyshev CNAB2 dynamo run on the GPU."""
 
@@ -229,6 +262,7 @@ class DynamoRun:
         self.w_host = torch.zeros(8, dtype=F64).pin_memory()
         self._plans = {}
         self._graphs = {}
+        self._cap_stream = None
         self.use_graphs = False
         self._sec_ev = None
         self._cfl_event = None
@@ -243,31 +277,7 @@ class DynamoRun:
         self.timers = {} if on else None
 
     def _t(self, name):
-        run = self
-
-        class _Sec:
-            def __enter__(self_):
-                rec = _lib.recording()
-                if rec is not None:
-                    self_.prev = rec.section
-                    rec.section = name
-                    return
-                if run.timers is not None:
-                    self_.ev = (torch.cuda.Event(enable_timing=True),
-                                torch.cuda.Event(enable_timing=True))
-                    self_.ev[0].record()
-
-            def __exit__(self_, *exc):
-                rec = _lib.recording()
-                if rec is not None:
-                    rec.section = self_.prev
-                    return False
-                if run.timers is not None:
-                    self_.ev[1].record()
-                    run.timers.setdefault(name, []).append(self_.ev)
-                return False
-
-        return _Sec()
+        return _Section(self, name)
 
     def _on_section(self, name, begin):
         if begin:
@@ -294,25 +304,54 @@ class DynamoRun:
         return key, plan
 
     def _graph(self, key, plan):
-        """The captured HIP graph of a recorded plan (captured once)."""
+        """The captured HIP graph of a recorded plan (captured once).
+
+        Captured on the current stream when it is not the device's default
+        stream (a run served on its own stream), else on a capture stream of
+        this run; no device-wide synchronisation (torch.cuda.graph would stall
+        every other run on the GPU).  The garbage collector is off during the
+        capture: nothing may free device memory or destroy graphs while a
+        stream capture is open."""
         g = self._graphs.get(key)
         if g is None:
-            # capture on a side stream (default priority); the low-level capture
-            # calls avoid the device-wide synchronisation of torch.cuda.graph,
-            # which would stall every other run on the GPU
             cur = torch.cuda.current_stream()
-            side = torch.cuda.Stream(device=self.device)
-            side.wait_stream(cur)
+            cap = cur
+            if cur == torch.cuda.default_stream(self.device):
+                if self._cap_stream is None:
+                    self._cap_stream = torch.cuda.Stream(device=self.device)
+                cap = self._cap_stream
+                cap.wait_stream(cur)
             g = torch.cuda.CUDAGraph()
-            with torch.cuda.stream(side):
-                g.capture_begin()
-                try:
-                    _lib.run_plan(plan, stream_handle())
-                finally:
-                    g.capture_end()
-            cur.wait_stream(side)
+            gc_on = gc.isenabled()
+            gc.disable()
+            try:
+                with torch.cuda.stream(cap):
+                    g.capture_begin()
+                    try:
+                        _lib.run_plan(plan, stream_handle())
+                    finally:
+                        g.capture_end()
+            finally:
+                if gc_on:
+                    gc.enable()
+            if cap is not cur:
+                cur.wait_stream(cap)
             self._graphs[key] = g
         return g
+
+    def close(self):
+        """Release the run now: wait for its queued work, destroy its graphs and
+        drop its device buffers (a server retiring a run, or a benchmark
+        between cases).  The run cannot step afterwards."""
+        torch.cuda.current_stream().synchronize()
+        self._graphs.clear()
+        self._plans.clear()
+        for k in ("_pool", "f", "tmp", "old", "impl", "expl_cur", "expl_prev", "rhs", "nl", "_E",
+                  "F_s", "F_v", "F_o", "MT", "mat_nn", "mat_wp", "gj_work", "gj_scal", "jobs1",
+                  "jobs2", "jobs_td", "jobs_setup", "_ctl"):
+            if hasattr(self, k):
+                setattr(self, k, None)
+        self.closed = True
 
     def prepare_graph(self, kind, par):
         """Record and capture the launch plan of phase `kind` ("radial", "lm" or

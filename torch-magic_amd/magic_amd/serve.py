@@ -281,6 +281,7 @@ class EnsembleServer:
             sl.res.energies = sl.run.energies()
             if keep_fields:
                 sl.res.final_fields = {k: sl.run.f[k].clone() for k in ("s", "w", "z", "b", "aj")}
+            sl.run.close()  # graphs and buffers released here, outside any graph capture
         sl.run = None
 
 
