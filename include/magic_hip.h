@@ -276,10 +276,13 @@ int mh_energy(const MhEnergyArgs* args, void* stream);
 typedef struct {
   const double *R, *D1, *D2, *D3, *or1, *or2;
   double* out;
-  int N, l_max, n_cheb, ktopv, kbotv, kind; /* kind 0 S 1 Z 2 B 3 J 4 WP 5 p0 */
+  int N, l_max, n_cheb, ktopv, kbotv, kind; /* kind 0 S 1 Z 2 B 3 J 4 WP 5 p0 6 all */
   double rnorm, wimp, opr, opm;
   const double* wimp_dev; /* optional: read the implicit weight here instead of wimp */
   const int* when;        /* optional: the kernel does nothing unless *when != 0 */
+  double* out_wp;         /* kind 6: the (w,p) matrices (out then holds systems 0..nsys-1 */
+  double* out_p0;         /*   stacked by system and degree); the l=0 pressure matrix */
+  int nsys, pad;          /* kind 6: number of N x N systems (2: S, Z; 4: S, Z, B, J) */
 } MhBuildArgs;
 
 /* Per-degree implicit matrices: build_s_matrices (update_s.py:66-188),
@@ -353,8 +356,9 @@ typedef struct {
  * update_s.py:152; the host raises when it reads the state). */
 int mh_step_control(const MhStepCtlArgs* args, void* stream);
 
-/* p[0..n) = v, unless `when` (optional) points at 0. */
-int mh_fill_f64(double* p, long n, double v, const int* when, void* stream);
+/* p[k*stride + i] = v for k < count, i < n, unless `when` (optional) points at 0. */
+int mh_fill_f64(double* p, long n, int count, long stride, double v, const int* when,
+                void* stream);
 
 /* ------------------------------------------------------------ banded */
 

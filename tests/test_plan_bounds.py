@@ -67,8 +67,13 @@ def _check_rebuild_calls(calls, ext):
         if name == "mh_build_matrices":
             a = C.cast(args[0], C.POINTER(_lib.BuildArgs)).contents
             N, lp1 = a.N, a.l_max + 1
-            size = {4: lp1 * 4 * N * N, 5: N * N}.get(a.kind, lp1 * N * N)
-            _inside(ext, a.out, a.out + 8 * size, f"build kind {a.kind}")
+            if a.kind == 6:  # all systems: stacked N x N systems, (w,p), l=0 pressure
+                _inside(ext, a.out, a.out + 8 * a.nsys * lp1 * N * N, "build stack")
+                _inside(ext, a.out_wp, a.out_wp + 8 * lp1 * 4 * N * N, "build (w,p)")
+                _inside(ext, a.out_p0, a.out_p0 + 8 * N * N, "build p0")
+            else:
+                size = {4: lp1 * 4 * N * N, 5: N * N}.get(a.kind, lp1 * N * N)
+                _inside(ext, a.out, a.out + 8 * size, f"build kind {a.kind}")
             for p in (a.R, a.D1, a.D2, a.D3):
                 _inside(ext, p, p + 8 * N * N, "Chebyshev matrix")
             for p in (a.or1, a.or2):
@@ -88,8 +93,9 @@ def _check_rebuild_calls(calls, ext):
                 hi = p + 8 * (last * sb + (rows - 1) * sr + (cols - 1) * sc + 1)
                 _inside(ext, p, hi, "gemm operand")
         elif name == "mh_fill_f64":
-            p, nf = args[0], args[1]
-            _inside(ext, p, p + 8 * nf, "fill")
+            p, nf, count, stride = args[:4]
+            for k in range(count):
+                _inside(ext, p + 8 * k * stride, p + 8 * (k * stride + nf), f"fill block {k}")
         else:
             continue
         n += 1
@@ -133,7 +139,7 @@ def test_recorded_launches_stay_inside_their_tensors(cpu_standin, l_max, n_r):
         run._rebuild_launches(0.0, (base + _lib.StepState.wimp_lin0.offset,
                                     base + _lib.StepState.rebuild.offset))
         run._ctl_launch()
-    assert _check_rebuild_calls(rec.calls, ext) >= 2 * 17
+    assert _check_rebuild_calls(rec.calls, ext) == 2 * 10  # ten launches per rebuild
     a = C.cast(rec.calls[-1][2][0], C.POINTER(_lib.StepCtlArgs)).contents
     _inside(ext, a.cfl, a.cfl + 16 * a.N, "Courant maxima")
     _inside(ext, a.info, a.info + 4 * a.info_rows * a.info_cols, "pivot flags")

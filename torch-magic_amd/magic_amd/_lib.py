@@ -117,7 +117,8 @@ class EnergyArgs(C.Structure):
 class BuildArgs(C.Structure):
     _fields_ = [("R", P), ("D1", P), ("D2", P), ("D3", P), ("or1", P), ("or2", P), ("out", P),
                 ("N", I), ("l_max", I), ("n_cheb", I), ("ktopv", I), ("kbotv", I), ("kind", I),
-                ("rnorm", D), ("wimp", D), ("opr", D), ("opm", D), ("wimp_dev", P), ("when", P)]
+                ("rnorm", D), ("wimp", D), ("opr", D), ("opm", D), ("wimp_dev", P), ("when", P),
+                ("out_wp", P), ("out_p0", P), ("nsys", I), ("pad", I)]
 
 
 class GemmArgs(C.Structure):
@@ -160,7 +161,7 @@ _SIGS = {
     "mh_gj_inverse": [P, P, P, P, I, I, I, P, P, P],
     "mh_gemm_f64": [P, P],
     "mh_step_control": [P, P],
-    "mh_fill_f64": [P, L, D, P, P],
+    "mh_fill_f64": [P, L, I, L, D, P, P],
     "mh_band_factor": [P, P, P, I, I, I, I, P],
     "mh_band_solve": [P, P, P, P, P, P, P, P, I, I, I, I, P],
     "mh_struct_sizes": [P, I],

@@ -18,29 +18,21 @@ typedef MhBuildArgs BuildArgs;
 
 __device__ __forceinline__ double bfac(int n, int N) { return (n == 0 || n == N - 1) ? 0.5 : 1.0; }
 
-__global__ void k_build(BuildArgs a) {
-  if (a.when && *a.when == 0) return;
+// Entry (i, j) of the collocation matrix of system `kind` at degree l.
+__device__ double build_entry(const BuildArgs& a, int kind, int l, int i, int j) {
   const int N = a.N;
-  const int n2 = (a.kind == 4) ? 2 * N : N;
-  const long per = (long)n2 * n2;
-  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const int nmat = (a.kind == 5) ? 1 : a.l_max + 1;
-  if (idx >= per * nmat) return;
-  const int l = (int)(idx / per);
-  const int rem = (int)(idx - (long)l * per);
-  const int i = rem / n2, j = rem - i * n2;
   const double dL = (double)l * (double)(l + 1);
   const double rn = a.rnorm, w0 = a.wimp_dev ? *a.wimp_dev : a.wimp;
   double v = 0.0;
-  if (a.kind <= 3 || a.kind == 5) {
+  if (kind <= 3 || kind == 5) {
     const int n = j;
     const double R = a.R[i * N + n], D = a.D1[i * N + n], D2 = a.D2[i * N + n];
     const double or1 = a.or1[i], or2 = a.or2[i];
     const bool top = (i == 0), bot = (i == N - 1);
-    if (a.kind == 0) {
+    if (kind == 0) {
       v = rn * (R - w0 * a.opr * (D2 + 2.0 * or1 * D - dL * or2 * R));
       if (top || bot) v = rn * R;
-    } else if (a.kind == 1) {
+    } else if (kind == 1) {
       if (l == 0) {
         v = (i == j) ? 1.0 : 0.0;
       } else {
@@ -48,12 +40,12 @@ __global__ void k_build(BuildArgs a) {
         if (top) v = (a.ktopv == 1) ? rn * (D - 2.0 * or1 * R) : rn * R;
         if (bot) v = (a.kbotv == 1) ? rn * (D - 2.0 * or1 * R) : rn * R;
       }
-    } else if (a.kind == 2 || a.kind == 3) {
+    } else if (kind == 2 || kind == 3) {
       if (l == 0) {
         v = (i == j) ? 1.0 : 0.0;
       } else {
         v = rn * dL * or2 * (R - w0 * a.opm * (D2 - dL * or2 * R));
-        if (a.kind == 2) {
+        if (kind == 2) {
           if (top) v = rn * (D + (double)l * or1 * R);
           if (bot) v = rn * (D - (double)(l + 1) * or1 * R);
         } else if (top || bot) {
@@ -63,9 +55,9 @@ __global__ void k_build(BuildArgs a) {
     } else {  // p0
       v = top ? rn * R : rn * D;
     }
-    const bool ident = (a.kind == 1 || a.kind == 2 || a.kind == 3) && l == 0;
+    const bool ident = (kind == 1 || kind == 2 || kind == 3) && l == 0;
     if (!ident) {
-      if ((top || (bot && a.kind != 5)) && n >= a.n_cheb) v = 0.0;
+      if ((top || (bot && kind != 5)) && n >= a.n_cheb) v = 0.0;
       v *= bfac(n, N);
     }
   } else {  // WP, 2N x 2N
@@ -96,14 +88,66 @@ __global__ void k_build(BuildArgs a) {
       v *= bfac(n, N);
     }
   }
-  a.out[idx] = v;
+  return v;
+}
+
+// kind 0..5: one system into a.out; kind 6: systems 0..nsys-1 stacked in
+// a.out ([k][l][i][j]), the (w,p) system in a.out_wp and p0 in a.out_p0.
+__global__ void k_build(BuildArgs a) {
+  if (a.when && *a.when == 0) return;
+  const int N = a.N, lp1 = a.l_max + 1;
+  const long nn = (long)N * N, ww = 4L * nn;
+  long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  int kind = a.kind, l, i, j;
+  double* out;
+  if (kind != 6) {
+    const int n2 = (kind == 4) ? 2 * N : N;
+    const long per = (long)n2 * n2;
+    if (idx >= per * ((kind == 5) ? 1 : lp1)) return;
+    l = (int)(idx / per);
+    const int rem = (int)(idx - (long)l * per);
+    i = rem / n2;
+    j = rem - i * n2;
+    out = a.out + idx;
+  } else {
+    const long n_stack = (long)a.nsys * lp1 * nn;
+    if (idx < n_stack) {
+      const long b = idx / nn;
+      kind = (int)(b / lp1);
+      l = (int)(b - (long)kind * lp1);
+      const int rem = (int)(idx - b * nn);
+      i = rem / N;
+      j = rem - i * N;
+      out = a.out + idx;
+    } else if ((idx -= n_stack) < lp1 * ww) {
+      kind = 4;
+      l = (int)(idx / ww);
+      const int rem = (int)(idx - (long)l * ww);
+      i = rem / (2 * N);
+      j = rem - i * (2 * N);
+      out = a.out_wp + idx;
+    } else if ((idx -= lp1 * ww) < nn) {
+      kind = 5;
+      l = 0;
+      i = (int)(idx / N);
+      j = (int)(idx - (long)i * N);
+      out = a.out_p0 + idx;
+    } else {
+      return;
+    }
+  }
+  *out = build_entry(a, kind, l, i, j);
 }
 
 extern "C" int mh_build_matrices(const MhBuildArgs* args, void* stream) {
   const BuildArgs& a = *args;
-  if (a.N < 3 || a.kind < 0 || a.kind > 5) return MH_EINVAL;
-  const int n2 = (a.kind == 4) ? 2 * a.N : a.N;
-  const long total = (long)n2 * n2 * ((a.kind == 5) ? 1 : a.l_max + 1);
+  if (a.N < 3 || a.kind < 0 || a.kind > 6) return MH_EINVAL;
+  if (a.kind == 6 && (a.nsys < 1 || a.nsys > 4 || !a.out_wp || !a.out_p0)) return MH_EINVAL;
+  const long nn = (long)a.N * a.N, lp1 = a.l_max + 1;
+  long total;
+  if (a.kind == 6) total = a.nsys * lp1 * nn + lp1 * 4 * nn + nn;
+  else if (a.kind == 4) total = lp1 * 4 * nn;
+  else total = nn * ((a.kind == 5) ? 1 : lp1);
   hipLaunchKernelGGL(k_build, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, mh_stream(stream), a);
   return mh_launch_status();
 }

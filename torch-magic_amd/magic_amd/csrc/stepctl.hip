@@ -15,8 +15,8 @@
 //    is evaluated in the order the host evaluates it, with no fused
 //    multiply-add, so that host and device decisions and weights agree bit
 //    for bit.
-//  * k_fill: p[i] = v, skipped when the rebuild flag is 0 (clears operator
-//    blocks inside a captured step without a memset node).
+//  * k_fill: p[k*stride + i] = v, skipped when the rebuild flag is 0 (clears
+//    operator blocks inside a captured step without a memset node).
 #include "common.h"
 
 #pragma clang fp contract(off)
@@ -121,17 +121,20 @@ extern "C" int mh_step_control(const MhStepCtlArgs* args, void* stream) {
   return mh_launch_status();
 }
 
-__global__ void k_fill(double* p, long n, double v, const int* when) {
+__global__ void k_fill(double* p, long n, long stride, double v, const int* when) {
   if (when && *when == 0) return;
+  double* q = p + (long)blockIdx.y * stride;
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
-    p[i] = v;
+    q[i] = v;
 }
 
-extern "C" int mh_fill_f64(double* p, long n, double v, const int* when, void* stream) {
-  if (n < 0 || (n > 0 && p == nullptr)) return MH_EINVAL;
-  if (n == 0) return 0;
+extern "C" int mh_fill_f64(double* p, long n, int count, long stride, double v, const int* when,
+                           void* stream) {
+  if (n < 0 || count < 0 || (n > 0 && count > 0 && p == nullptr) || (count > 1 && stride < n))
+    return MH_EINVAL;
+  if (n == 0 || count == 0) return 0;
   const long blocks = (n + 255) / 256;
-  hipLaunchKernelGGL(k_fill, dim3((unsigned)(blocks < 1024 ? blocks : 1024)), dim3(256), 0,
-                     mh_stream(stream), p, n, v, when);
+  hipLaunchKernelGGL(k_fill, dim3((unsigned)(blocks < 1024 ? blocks : 1024), (unsigned)count),
+                     dim3(256), 0, mh_stream(stream), p, n, stride, v, when);
   return mh_launch_status();
 }

@@ -392,17 +392,27 @@ class DynamoRun:
         self.K4wp = -(-(2 * N) // 4) * 4
         self.Rpad3 = -(-(3 * N) // 64) * 64
         self.Rpad6 = -(-(6 * N) // 64) * 64
-        self.MT = {k: torch.zeros(lp1, self.K4, self.Rpad3, dtype=F64, device=dev)
-                   for k in ("s", "z", "b", "j")}
+        # S, Z, B, J operators stacked [system][l][k][r] so that one rebuild launch
+        # covers the four systems
+        self.nsys = 4 if self.p.l_mag else 2
+        self.MT4 = torch.zeros(4, lp1, self.K4, self.Rpad3, dtype=F64, device=dev)
+        self.MT = {k: self.MT4[i] for i, k in enumerate(("s", "z", "b", "j"))}
         self.MT["wp"] = torch.zeros(lp1, self.K4wp, self.Rpad6, dtype=F64, device=dev)
         ops = self._radial_operators()
         self.Op3, self.Opw, self.Opp, self.cheb = ops["Op3"], ops["Opw"], ops["Opp"], ops["cheb"]
         self.MT_d1, self.MT_d1r2, self.MT_d12, self.MT_d123 = (
             ops[k] for k in ("MT_d1", "MT_d1r2", "MT_d12", "MT_d123"))
-        self.mat_nn = torch.empty(lp1, N, N, dtype=F64, device=dev)
+        self.mat_nn = torch.empty(self.nsys * lp1, N, N, dtype=F64, device=dev)
         self.mat_wp = torch.empty(lp1, 2 * N, 2 * N, dtype=F64, device=dev)
-        self.gj_work = torch.empty(lp1 * 2 * (2 * N) * (2 * N), dtype=F64, device=dev)
-        self.gj_scal = torch.empty(lp1 * 2 * 2 * N, dtype=F64, device=dev)
+        self.mat_p0 = torch.empty(N, N, dtype=F64, device=dev)
+        # Gauss-Jordan work space: only the kernel for n > 208 uses it
+        nwork = 0
+        if 2 * N > 208:
+            nwork = lp1 * 2 * (2 * N) ** 2
+        if N > 208:
+            nwork = max(nwork, self.nsys * lp1 * 2 * N * N)
+        self.gj_work = torch.empty(max(nwork, 1), dtype=F64, device=dev)
+        self.gj_scal = torch.empty(max(self.nsys * lp1, 2 * lp1) * 2 * N * 2, dtype=F64, device=dev)
         # first zero pivot per degree, one row per system (s, z, b, j, wp, p0)
         self.gj_info = torch.zeros(len(_INFO_ROWS), lp1, dtype=I32, device=dev)
         self._info_host = torch.zeros(len(_INFO_ROWS), lp1, dtype=I32).pin_memory()
@@ -449,21 +459,24 @@ class DynamoRun:
         g.M, g.Nn, g.K, g.batch, g.alpha = M, Nn, K, batch, 1.0
         _lib.call("mh_gemm_f64", _lib.ref(g), stream_handle())
 
-    def _build_kind(self, kind, out, wl0, ctl=None):
+    def _build_all(self, wl0, ctl=None):
+        """Collocation matrices of every system in one launch: S, Z (B, J)
+        stacked in mat_nn, (w,p) in mat_wp, the l=0 pressure system in mat_p0."""
         p, rad = self.p, self.res.rad
         a = _lib.BuildArgs()
         a.R, a.D1, a.D2, a.D3 = (ptr(self.cheb[k]) for k in ("T", "T1", "T2", "T3"))
         a.or1, a.or2 = ptr(self.res.d["or1"]), ptr(self.res.d["or2"])
-        a.out = ptr(out)
-        a.N, a.l_max, a.n_cheb = self.N, p.l_max, p.n_cheb_max
-        a.ktopv, a.kbotv, a.kind = p.ktopv, p.kbotv, kind
+        a.out, a.out_wp, a.out_p0 = ptr(self.mat_nn), ptr(self.mat_wp), ptr(self.mat_p0)
+        a.N, a.l_max, a.n_cheb, a.nsys = self.N, p.l_max, p.n_cheb_max, self.nsys
+        a.ktopv, a.kbotv, a.kind = p.ktopv, p.kbotv, 6
         a.rnorm, a.wimp, a.opr, a.opm = rad["rnorm"], wl0, p.opr, p.opm
         if ctl is not None:
             a.wimp_dev, a.when = ctl
         _lib.call("mh_build_matrices", _lib.ref(a), stream_handle())
 
-    def _zero(self, t, when=None):
-        _lib.call("mh_fill_f64", ptr(t), t.numel(), 0.0, when, stream_handle())
+    def _zero(self, t, when=None, count=1, stride=0):
+        n = t[0].numel() if count > 1 else t.numel()
+        _lib.call("mh_fill_f64", ptr(t), n, count, stride, 0.0, when, stream_handle())
 
     def build_all_matrices(self, wimp_lin0: float):
         """Rebuild every per-degree operator for the implicit weight ``wimp_lin0``.
@@ -479,30 +492,28 @@ class DynamoRun:
         self._queue_info_check()
 
     def _rebuild_launches(self, wimp_lin0, ctl=None):
-        """The launches of a rebuild.  ctl = (wimp_dev, when) device pointers: the
-        weight is read on the device and every kernel returns at once unless
-        *when != 0 (the rebuild inside a captured step)."""
-        N, lp1 = self.N, self.p.l_max + 1
+        """The launches of a rebuild (ten, whatever the number of systems).
+        ctl = (wimp_dev, when) device pointers: the weight is read on the device
+        and every kernel returns at once unless *when != 0 (the rebuild inside
+        a captured step)."""
+        N, lp1, ns = self.N, self.p.l_max + 1, self.nsys
         st = stream_handle()
         when = None if ctl is None else ctl[1]
-        for kind, key in ((0, "s"), (1, "z"), (2, "b"), (3, "j")):
-            if key in ("b", "j") and not self.p.l_mag:
-                continue
-            self._build_kind(kind, self.mat_nn, wimp_lin0, ctl)
-            _lib.call("mh_gj_inverse", ptr(self.mat_nn), ptr(self.mat_nn), ptr(self.gj_work),
-                      ptr(self.gj_scal), N, lp1, 1, ptr(self.gj_info[_INFO_ROWS.index(key)]),
-                      when, st)
-            MT = self.MT[key]
-            # MT[l][k][r] = sum_j inv_l[j][k] Op3[r][j]
-            self._gemm(ptr(self.mat_nn), 1, N, N * N, ptr(self.Op3), 1, N, 0,
-                       ptr(MT), self.Rpad3, 1, self.K4 * self.Rpad3, N, 3 * N, N, lp1, when)
-            if key != "s":
-                self._zero(MT[0], when)
+        info = lambda key: ptr(self.gj_info[_INFO_ROWS.index(key)])
+        self._build_all(wimp_lin0, ctl)
+        # S, Z (B, J): all degrees of all systems in one inverse and one fold;
+        # their pivot-flag rows are consecutive
+        _lib.call("mh_gj_inverse", ptr(self.mat_nn), ptr(self.mat_nn), ptr(self.gj_work),
+                  ptr(self.gj_scal), N, ns * lp1, 1, info("s"), when, st)
+        # MT[k][l][c][r] = sum_j inv_kl[j][c] Op3[r][j]
+        self._gemm(ptr(self.mat_nn), 1, N, N * N, ptr(self.Op3), 1, N, 0,
+                   ptr(self.MT4), self.Rpad3, 1, self.K4 * self.Rpad3, N, 3 * N, N, ns * lp1, when)
+        # Z, B, J have no l = 0 equation
+        sb = lp1 * self.K4 * self.Rpad3
+        self._zero(self.MT4[1:ns, 0], when, count=ns - 1, stride=sb)
         # coupled (w, p)
-        self._build_kind(4, self.mat_wp, wimp_lin0, ctl)
         _lib.call("mh_gj_inverse", ptr(self.mat_wp), ptr(self.mat_wp), ptr(self.gj_work),
-                  ptr(self.gj_scal), 2 * N, lp1, 2, ptr(self.gj_info[_INFO_ROWS.index("wp")]),
-                  when, st)
+                  ptr(self.gj_scal), 2 * N, lp1, 2, info("wp"), when, st)
         MT = self.MT["wp"]
         sb = self.K4wp * self.Rpad6
         self._gemm(ptr(self.mat_wp), 1, 2 * N, 4 * N * N, ptr(self.Opw), 1, N, 0,
@@ -511,10 +522,9 @@ class DynamoRun:
                    ptr(MT) + 8 * 4 * N, self.Rpad6, 1, sb, 2 * N, 2 * N, N, lp1, when)
         self._zero(MT[0], when)
         # l = 0 pressure: p(l=0) from the p0 system acting on the pressure half
-        p0 = self.mat_nn[0]
-        self._build_kind(5, p0, wimp_lin0, ctl)
+        p0 = self.mat_p0
         _lib.call("mh_gj_inverse", ptr(p0), ptr(p0), ptr(self.gj_work), ptr(self.gj_scal), N, 1, 1,
-                  ptr(self.gj_info[_INFO_ROWS.index("p0")]), when, st)
+                  info("p0"), when, st)
         self._gemm(ptr(p0), 1, N, 0, ptr(self.Opp), 1, N, 0,
                    ptr(MT) + 8 * (N * self.Rpad6 + 4 * N), self.Rpad6, 1, 0, N, 2 * N, N, 1, when)
 

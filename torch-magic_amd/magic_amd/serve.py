@@ -93,7 +93,7 @@ def footprint_bytes(cfg) -> int:
     K4, K4wp = -(-N // 4) * 4, -(-(2 * N) // 4) * 4
     Rpad3, Rpad6 = -(-(3 * N) // 64) * 64, -(-(6 * N) // 64) * 64
     ops = (p.l_max + 1) * 8 * (4 * K4 * Rpad3 + K4wp * Rpad6)
-    solve_work = (p.l_max + 1) * 8 * (N * N + 4 * N * N) * 2
+    solve_work = (p.l_max + 1) * 8 * (4 * N * N + 4 * N * N + (8 * N * N if 2 * N > 208 else 0))
     return int(spec * n_spec + fourier + ops + solve_work + (16 << 20))
 
 
