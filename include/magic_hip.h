@@ -152,6 +152,8 @@ typedef struct {
   double LFfac, cf2, af2;
   int l_mag, alf_damp;
   unsigned long long* cfl; /* 2N per-level maxima (bit patterns of doubles) */
+  int rb;  /* radial levels per workgroup: 0 = the most the FFT plan fits, else 1..that */
+  int pad;
 } MhGridArgs;
 
 /* Fused grid-space step, one workgroup per (theta row, RB radial levels):
@@ -161,6 +163,12 @@ typedef struct {
  * (get_nl.py:30-87), courant_check (courant.py:25-127) and rfft with the
  * 1/sin^2 scaling (sht.py:371,597-603) of the radial loop. */
 int mh_grid_step(const MhGridArgs* args, void* stream);
+
+/* FFT plan of the grid step for half length Mh: out[0] = register radix,
+ * out[1] = threads per level, out[2] = the most levels per workgroup (the
+ * default level block; MhGridArgs.rb may ask for 1 .. out[2]).  Host only.
+ * No counterpart in the reference: it exists for the launch-shape tuner. */
+int mh_grid_plan(int Mh, int* out);
 
 /* ------------------------------------------------------------ row operators */
 typedef struct {

@@ -275,3 +275,39 @@ def test_device_step_control_replays_schedule_with_timers():
         assert torch.equal(dev.f[k], host.f[k]), k
     t = dev.timer_summary()
     assert "step_ctl" in t and "rebuild" in t
+
+
+@pytest.mark.parametrize("l_max,precision", [(32, "fp64"), (32, "mixed"), (63, "fp64"),
+                                             (85, "mixed"), (383, "mixed"), (383, "fp64")])
+def test_grid_level_blocks_give_identical_results(l_max, precision):
+    """Every level block the tuner may choose (1, 2, 4, ... up to the plan's own)
+    gives the same Fourier outputs and Courant maxima bit for bit: compiled plans
+    (n_phi 96, 192, 1152, with their 1- and 2-level instantiations) and the
+    run-time plan (n_phi 256, and the other counts)."""
+    import torch
+    from magic_amd import tune
+    from magic_amd.device import Resolution
+    from magic_amd.params import Params
+    N = 9
+    p = Params(dict(_GRID_CFG, l_max=l_max, n_r_max=N, precision=precision))
+    res = Resolution(p, "cuda")
+    fdt = torch.float32 if precision == "mixed" else torch.float64
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(3)
+    Fs = torch.randn(res.fourier_buffer(1, N, fdt).shape, generator=gen, device="cuda",
+                     dtype=fdt) * 1e-2
+    Fv = torch.randn(res.fourier_buffer(3, 4 * N, fdt).shape, generator=gen, device="cuda",
+                     dtype=fdt) * 1e-2
+    own = res.grid_plan()[2]
+    outs = {}
+    for rb in tune.grid_candidates(own):
+        Fo = res.fourier_buffer(9, N - 2, fdt).zero_()
+        cfl = torch.zeros(2 * N, dtype=torch.int64, device="cuda")
+        res.grid_step(Fs, Fv, Fo, N, cfl, p, rb=0 if rb == own else rb)
+        torch.cuda.synchronize()
+        outs[rb] = (Fo, cfl)
+    ref_Fo, ref_cfl = outs[own]
+    assert torch.isfinite(ref_Fo).all() and (ref_cfl > 0).all()
+    for rb, (Fo, cfl) in outs.items():
+        assert torch.equal(Fo, ref_Fo), rb
+        assert torch.equal(cfl, ref_cfl), rb

@@ -35,6 +35,7 @@ def cpu_standin(monkeypatch):
     monkeypatch.setattr(dyn, "stream_handle", lambda: C.c_void_p(0))
     monkeypatch.setattr(torch.cuda, "current_stream", lambda *a, **k: _NoStream())
     monkeypatch.setattr(torch.Tensor, "pin_memory", lambda self: self)
+    monkeypatch.setenv("MAGIC_AMD_TUNE", "0")  # no launch-shape timing without a GPU
 
 
 def _extents(run):

@@ -60,7 +60,7 @@ class GridArgs(C.Structure):
     _fields_ = [("Fs", P), ("Fv", P), ("Fo", P), ("N", I), ("T", I), ("M", I), ("Mh", I),
                 ("generic", I), ("fprec", I), ("tw", P), ("tw2", P), ("or2", P),
                 ("or4", P), ("ost2", P), ("valri2", P), ("valhi2", P), ("LFfac", D), ("cf2", D),
-                ("af2", D), ("l_mag", I), ("alf_damp", I), ("cfl", P)]
+                ("af2", D), ("l_mag", I), ("alf_damp", I), ("cfl", P), ("rb", I), ("pad", I)]
 
 
 class RowJob(C.Structure):
@@ -150,6 +150,7 @@ _SIGS = {
     "mh_fft": [I, P, P],
     "mh_get_nl": [P, P],
     "mh_grid_step": [P, P],
+    "mh_grid_plan": [I, P],
     "mh_tables_to_f32": [P, P, L, P],
     "mh_rowop": [P, P, I, P],
     "mh_td": [P, P],

@@ -189,11 +189,12 @@ class Resolution:
         float32 for the mixed-precision radial loop)."""
         return torch.empty(ncomp * self.M * self.T * B * 2, dtype=dtype, device=self.device)
 
-    def grid_step(self, Fs, Fv, Fo, N, cfl, p, generic=False):
+    def grid_step(self, Fs, Fv, Fo, N, cfl, p, generic=False, rb=0):
         """Fused inverse FFT + nonlinear products + Courant maxima + forward FFT.
 
         generic: use the kernel with a run-time FFT plan even where the library
-        has one compiled for this n_phi (both give the same results).
+        has one compiled for this n_phi; rb: radial levels per workgroup (0: the
+        most the plan fits).  Every choice gives the same results.
         """
         a = _lib.GridArgs()
         a.Fs, a.Fv, a.Fo = ptr(Fs), ptr(Fv), ptr(Fo)
@@ -208,7 +209,16 @@ class Resolution:
         a.LFfac, a.cf2, a.af2 = p.LFfac, p.cour_fac ** 2, p.alf_fac ** 2
         a.l_mag, a.alf_damp = int(p.l_mag), int(p.l_cour_alf_damp)
         a.cfl = ptr(cfl)
+        a.rb = int(rb)
         _lib.call("mh_grid_step", _lib.ref(a), stream_handle())
+
+    def grid_plan(self):
+        """(register radix, threads per level, most levels per workgroup) of the
+        grid step's FFT plan for this n_phi."""
+        out = (C.c_int * 3)()
+        if _lib.load().mh_grid_plan(self.Mh, C.cast(out, C.c_void_p)) != 0:  # host query, never recorded
+            raise _lib.MagicHipError(f"mh_grid_plan: no FFT plan for n_phi = {2 * self.Mh}")
+        return tuple(out)
 
     def grid_buffer(self, ncomp, B):
         return torch.empty(ncomp * B * self.T * self.P, dtype=F64, device=self.device)

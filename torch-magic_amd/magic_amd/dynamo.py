@@ -242,6 +242,12 @@ class DynamoRun:
         assert used[0] == self._pool.numel()
         self.cfl = torch.zeros(2 * N, dtype=torch.int64, device=dev)
         self.cfl_host = torch.zeros(2 * N, dtype=torch.int64).pin_memory()
+        # radial levels per workgroup of the grid step: timed once per shape
+        self.grid_rb = 0
+        if fused_grid:
+            from . import tune
+            self.grid_rb = tune.grid_level_block(r, N, p, fdt,
+                                                 bufs=(self.F_s, self.F_v, self.F_o, self.cfl))
         yield
         self._build_operator_storage()
         yield
@@ -731,7 +737,7 @@ class DynamoRun:
             # 2-4. fused inverse FFT + nonlinear products + Courant + forward FFT
             with self._t("grid_step"):
                 r.grid_step(self.F_s, self.F_v, self.F_o, N, self.cfl, self.p,
-                            generic=self.grid_generic)
+                            generic=self.grid_generic, rb=self.grid_rb)
         else:
             self._grid_separate()
         # 5. Legendre analysis into (L, N) arrays at columns 1..N-2

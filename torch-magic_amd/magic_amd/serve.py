@@ -138,11 +138,16 @@ class EnsembleServer:
         return r
 
     def warm(self, cfgs):
-        """Build the shared device tables of every resolution in `cfgs` ahead of
-        serving (as a server loads what it serves before taking requests)."""
+        """Build the shared device tables of every resolution in `cfgs` and tune
+        their grid-step launch shape ahead of serving (as a server loads what it
+        serves before taking requests)."""
+        from . import tune
         from .params import Params
         for cfg in cfgs:
-            self._resolution(Params(dict(cfg)))
+            p = Params(dict(cfg))
+            res = self._resolution(p)
+            fdt = self.torch.float32 if p.precision == "mixed" else self.torch.float64
+            tune.grid_level_block(res, p.n_r_max, p, fdt)
         self.torch.cuda.synchronize(self.device)
 
     def _admit_stages(self, sl: _Slot):
