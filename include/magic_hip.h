@@ -58,7 +58,7 @@ typedef struct {
   int Bout;
   int prec;  /* 0: FP64 tables and MFMA; 1: FP32 tables (synP/synD hold floats), FP32 MFMA */
   int fprec; /* 0: F holds doubles; 1: F holds floats (FP32 Fourier buffers, needs prec = 1) */
-  int pad;
+  int waves; /* waves per workgroup (16 theta pairs each): 0 = default (4), 4 or 6 */
 } MhSynArgs;
 
 /* Legendre synthesis on FP64 MFMA: scalar (vector=0) replaces the bmm part of

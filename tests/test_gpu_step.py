@@ -311,3 +311,32 @@ def test_grid_level_blocks_give_identical_results(l_max, precision):
     for rb, (Fo, cfl) in outs.items():
         assert torch.equal(Fo, ref_Fo), rb
         assert torch.equal(cfl, ref_cfl), rb
+
+
+@pytest.mark.parametrize("l_max,prec", [(32, 0), (32, 1), (85, 0), (383, 0), (383, 1)])
+def test_synthesis_workgroup_sizes_give_identical_results(l_max, prec):
+    """The vector and scalar Legendre synthesis with 4 and 6 waves per workgroup
+    (the tuner's candidates) write the same Fourier coefficients bit for bit,
+    FP64 and FP32 (FP32 tables, FP32 Fourier buffer)."""
+    import torch
+    from magic_amd.device import Resolution
+    from magic_amd.params import Params
+    N = 9
+    p = Params(dict(_GRID_CFG, l_max=l_max, n_r_max=N, precision="mixed" if prec else "fp64"))
+    res = Resolution(p, "cuda")
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(5)
+    xs = [torch.randn(res.L, N, dtype=torch.complex128, generator=gen, device="cuda")
+          for _ in range(4)]
+    fdt = torch.float32 if prec else torch.float64
+    groups = [[(xs[0], None, 1), (xs[1], None, 0), (xs[2], xs[3], 2)] for _ in range(4)]
+    outs = {}
+    for w in (4, 6):
+        Fv = res.fourier_buffer(3, 4 * N, fdt).zero_()
+        Fs = res.fourier_buffer(1, N, fdt).zero_()
+        res.synth(groups, N, N, Fv, 4 * N, vector=True, prec=prec, waves=w)
+        res.synth([[(xs[0], None, 0)]], N, N, Fs, N, vector=False, prec=prec, waves=w)
+        torch.cuda.synchronize()
+        outs[w] = (Fv, Fs)
+    assert torch.isfinite(outs[4][0]).all() and outs[4][0].abs().max() > 0
+    assert torch.equal(outs[4][0], outs[6][0]) and torch.equal(outs[4][1], outs[6][1])

@@ -12,11 +12,22 @@ def test_candidates_are_powers_of_two_below_the_plan_block():
     assert tune.grid_candidates(16) == [1, 2, 4, 8, 16]
 
 
-def test_choice_is_the_fastest_and_ties_keep_the_plan_block():
-    assert tune.choose({1: 2.9, 2: 3.2, 4: 3.6}, 4) == 1
-    assert tune.choose({1: 3.6, 2: 3.2, 4: 3.6}, 4) == 2
-    assert tune.choose({1: 3.0, 2: 3.2, 4: 3.0}, 4) == 0   # tie: the plan's own (rb = 0)
-    assert tune.choose({4: 1.0}, 4) == 0
+def test_choice_is_the_fastest_and_ties_keep_the_default():
+    assert tune.fastest({1: 2.9, 2: 3.2, 4: 3.6}, 4) == 1
+    assert tune.fastest({1: 3.6, 2: 3.2, 4: 3.6}, 4) == 2
+    assert tune.fastest({1: 3.0, 2: 3.2, 4: 3.0}, 4) == 4   # tie: the default
+    assert tune.fastest({4: 1.0, 6: 1.0}, 4) == 4
+
+
+def test_cached_choice_is_not_timed_again(tmp_path, monkeypatch):
+    monkeypatch.setenv("MAGIC_AMD_TUNE_CACHE", str(tmp_path / "t.json"))
+    tune._write_file("shape", dict(choice=6, ms={"4": 1.2, "6": 1.0}))
+    calls = []
+    assert tune.timed_choice("shape", [4, 6], calls.append, 4) == 6
+    assert calls == [] and tune._MEM.pop("shape") == 6
+    # a stale entry (not a candidate any more) is ignored: one candidate, no timing
+    assert tune.timed_choice("shape", [4], calls.append, 4) == 4
+    assert calls == [] and tune._MEM.pop("shape") == 4
 
 
 def test_cache_file_round_trip(tmp_path, monkeypatch):

@@ -36,10 +36,7 @@ constexpr GPlan make_plan(int Mh) {
   p.radix[k++] = RL;
   p.nst = k;
   p.TP = Mh / RL;
-#ifndef MH_GRID_RB_MAX
-#define MH_GRID_RB_MAX 16
-#endif
-  for (int rb = MH_GRID_RB_MAX; rb >= 1; --rb) {
+  for (int rb = 16; rb >= 1; --rb) {
     const int th = (rb * p.TP + 63) / 64 * 64;
     if (th <= kMaxThreads && (2 * rb * (Mh + 1) + 2 * Mh) * 16 <= kLdsBytes) {
       p.RB = rb;

@@ -135,18 +135,21 @@ __device__ __forceinline__ SynIn syn_in(const SynArgs& a, int g, int c) {
 // of the next chunk are issued before the current chunk's MFMAs and combined
 // and stored after them.  Wave w takes theta pairs j0 + 16w and streams its
 // table operands from global memory through a ring of kSynDepth steps.
-constexpr int kSynWaves = 6, kSynJ = 16 * kSynWaves, kChunk = 32, kSynDepth = 4;
+// Waves per synthesis workgroup: each wave takes 16 theta pairs.  Four (the
+// default) or six; both give identical results, the tuner may pick either.
+constexpr int kSynWavesDefault = 4, kChunk = 32, kSynDepth = 4;
 
 // NCT column tiles of 8 complex columns per workgroup: every table operand
 // feeds NCT MFMAs (the launches use one tile; two, which FP32 accumulators
 // allow, measured no faster on MI355X).
-template <bool VEC, typename T, int NCT, typename FT>
-__global__ __launch_bounds__(64 * kSynWaves, 3) void k_syn(SynArgs a, int n_m) {
+template <bool VEC, typename T, int NCT, typename FT, int WAVES>
+__global__ __launch_bounds__(64 * WAVES, 3) void k_syn(SynArgs a, int n_m) {
   FT* const Fo = reinterpret_cast<FT*>(a.F);  // Fourier buffer elements (double or float)
   typedef Mfma<T> MF;
   typedef typename MF::acc_t acc_t;
   constexpr int NF = VEC ? 3 : 1;           // fields per group (Q, S, T)
-  constexpr int NT = 64 * kSynWaves;        // threads
+  constexpr int NT = 64 * WAVES;            // threads
+  constexpr int kSynJ = 16 * WAVES;         // theta pairs per workgroup
   constexpr int SPC = kChunk / 4;           // MFMA steps per chunk
   constexpr int D = kSynDepth;
   constexpr int CW = 16 * NCT;              // real columns per workgroup
@@ -360,22 +363,31 @@ extern "C" int mh_leg_synth(int vector, const MhSynArgs* args, int n_m, void* st
       if (a.in[g][f].mode < 0 || a.in[g][f].mode > 2 || (a.in[g][f].mode == 2 && !(vector && f == 2)))
         return MH_EINVAL;
   const int nct = 1;  // column tiles per workgroup (two measured no faster for FP32)
+  const int waves = a.waves == 0 ? kSynWavesDefault : a.waves;
+  if (waves != 4 && waves != 6) return MH_EINVAL;
   const long units =
-      (long)mh_cdiv(a.ngroups * a.B, 8 * nct) * mh_cdiv(a.H, kSynJ) * mh_cdiv(n_m, 8) * 8;
+      (long)mh_cdiv(a.ngroups * a.B, 8 * nct) * mh_cdiv(a.H, 16 * waves) * mh_cdiv(n_m, 8) * 8;
   if (units > 0x7fffffffL) return MH_EINVAL;
   dim3 grid((unsigned)units);
-  const dim3 blk(64 * kSynWaves);
+  const dim3 blk(64 * waves);
   hipStream_t st = mh_stream(stream);
-  if (a.prec == 0) {
-    if (vector) hipLaunchKernelGGL((k_syn<true, double, 1, double>), grid, blk, 0, st, a, n_m);
-    else hipLaunchKernelGGL((k_syn<false, double, 1, double>), grid, blk, 0, st, a, n_m);
-  } else if (a.fprec == 0) {
-    if (vector) hipLaunchKernelGGL((k_syn<true, float, 1, double>), grid, blk, 0, st, a, n_m);
-    else hipLaunchKernelGGL((k_syn<false, float, 1, double>), grid, blk, 0, st, a, n_m);
-  } else {
-    if (vector) hipLaunchKernelGGL((k_syn<true, float, 1, float>), grid, blk, 0, st, a, n_m);
-    else hipLaunchKernelGGL((k_syn<false, float, 1, float>), grid, blk, 0, st, a, n_m);
+#define MH_SYN_LAUNCH(W)                                                                       \
+  if (a.prec == 0) {                                                                           \
+    if (vector) hipLaunchKernelGGL((k_syn<true, double, 1, double, W>), grid, blk, 0, st, a, n_m); \
+    else hipLaunchKernelGGL((k_syn<false, double, 1, double, W>), grid, blk, 0, st, a, n_m);   \
+  } else if (a.fprec == 0) {                                                                   \
+    if (vector) hipLaunchKernelGGL((k_syn<true, float, 1, double, W>), grid, blk, 0, st, a, n_m); \
+    else hipLaunchKernelGGL((k_syn<false, float, 1, double, W>), grid, blk, 0, st, a, n_m);    \
+  } else {                                                                                     \
+    if (vector) hipLaunchKernelGGL((k_syn<true, float, 1, float, W>), grid, blk, 0, st, a, n_m); \
+    else hipLaunchKernelGGL((k_syn<false, float, 1, float, W>), grid, blk, 0, st, a, n_m);     \
   }
+  if (waves == 4) {
+    MH_SYN_LAUNCH(4)
+  } else {
+    MH_SYN_LAUNCH(6)
+  }
+#undef MH_SYN_LAUNCH
   return mh_launch_status();
 }
 

@@ -242,12 +242,15 @@ class DynamoRun:
         assert used[0] == self._pool.numel()
         self.cfl = torch.zeros(2 * N, dtype=torch.int64, device=dev)
         self.cfl_host = torch.zeros(2 * N, dtype=torch.int64).pin_memory()
-        # radial levels per workgroup of the grid step: timed once per shape
+        # launch shapes (timed once per shape, same results whatever the choice):
+        # radial levels per workgroup of the grid step, waves per workgroup of
+        # the Legendre synthesis
+        from . import tune
         self.grid_rb = 0
         if fused_grid:
-            from . import tune
             self.grid_rb = tune.grid_level_block(r, N, p, fdt,
                                                  bufs=(self.F_s, self.F_v, self.F_o, self.cfl))
+        self.syn_waves = tune.synth_waves(r, N, self.leg_prec, fdt, bufs=(self.f["s"], self.F_v))
         yield
         self._build_operator_storage()
         yield
@@ -725,13 +728,15 @@ class DynamoRun:
         r, f, N, Nb = self.res, self.f, self.N, self.Nb
         # 1. Legendre synthesis
         with self._t("leg_syn_scal"):
-            r.synth([[(f["s"], None, 0)]], N, N, self.F_s, N, vector=False, prec=self.leg_prec)
+            r.synth([[(f["s"], None, 0)]], N, N, self.F_s, N, vector=False, prec=self.leg_prec,
+                    waves=self.syn_waves)
         groups = [[(f["b"], None, 1), (f["db"], None, 0), (f["aj"], None, 0)],
                   [(f["aj"], None, 1), (f["dj"], None, 0), (f["ddb"], f["b"], 2)],
                   [(f["w"], None, 1), (f["dw"], None, 0), (f["z"], None, 0)],
                   [(f["z"], None, 1), (f["dz"], None, 0), (f["ddw"], f["w"], 2)]]
         with self._t("leg_syn_vec"):
-            r.synth(groups, N, N, self.F_v, 4 * N, vector=True, prec=self.leg_prec)
+            r.synth(groups, N, N, self.F_v, 4 * N, vector=True, prec=self.leg_prec,
+                    waves=self.syn_waves)
         d = r.d
         if self.fused_grid:
             # 2-4. fused inverse FFT + nonlinear products + Courant + forward FFT

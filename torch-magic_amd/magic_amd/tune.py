@@ -1,14 +1,18 @@
 """Launch-shape tuner.
 
-The fused grid step (``mh_grid_step``) can give a workgroup 1, 2, ... up to
-the most radial levels its FFT plan fits (``mh_grid_plan``).  The best count
-depends on n_phi, the number of levels and the precision (registers, LDS per
-workgroup, workgroups per theta row), and every count gives the same result,
-so it is chosen by timing the candidates on the device, once per (device,
-n_phi, n_theta, orders, levels, precision), and remembered: in memory for the
-process and in a JSON file (``MAGIC_AMD_TUNE_CACHE``, default
-``~/.cache/magic_amd/tune.json``; not written when the directory is not
-writable).  ``MAGIC_AMD_TUNE=0`` keeps the plan's own level block.
+Two launch shapes are chosen by timing on the device, once per shape and
+precision, because the best one depends on the sizes and every choice gives
+the same results bit for bit:
+
+* the fused grid step (``mh_grid_step``): radial levels per workgroup, 1, 2,
+  ... up to the most its FFT plan fits (``mh_grid_plan``);
+* the Legendre synthesis (``mh_leg_synth``): waves per workgroup, 4 or 6
+  (16 theta pairs each).
+
+Choices are remembered in memory for the process and in a JSON file
+(``MAGIC_AMD_TUNE_CACHE``, default ``~/.cache/magic_amd/tune.json``; not
+written when the directory is not writable).  ``MAGIC_AMD_TUNE=0`` keeps the
+library defaults.
 """
 
 from __future__ import annotations
@@ -61,16 +65,72 @@ def grid_candidates(own_rb: int) -> list[int]:
     return sorted(c)
 
 
-def choose(times: dict[int, float], own_rb: int) -> int:
-    """The fastest candidate as the `rb` argument (0 when it is the plan's own);
-    a tie keeps the plan's own block."""
-    best = min(times, key=lambda rb: (times[rb], rb != own_rb))
-    return 0 if best == own_rb else best
+def fastest(times: dict, default):
+    """The candidate with the least time; a tie keeps `default`."""
+    return min(times, key=lambda c: (times[c], c != default))
+
+
+def timed_choice(key: str, candidates, launch, default, reps: int = 3):
+    """The fastest of `candidates` for `key` (cached): launch(c) issues one run
+    of candidate c on the current stream.  Ties keep `default`."""
+    with _LOCK:
+        if key in _MEM:
+            return _MEM[key]
+        hit = _read_file().get(key)
+        if hit is not None and hit.get("choice") in candidates:
+            _MEM[key] = hit["choice"]
+            return _MEM[key]
+    if len(candidates) == 1:
+        best = candidates[0]
+    else:
+        times = {}
+        for c in candidates:
+            launch(c)  # warm-up
+            ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+            ev[0].record()
+            for _ in range(reps):
+                launch(c)
+            ev[1].record()
+            ev[1].synchronize()
+            times[c] = ev[0].elapsed_time(ev[1]) / reps
+        best = fastest(times, default)
+        _write_file(key, dict(choice=best, ms={str(k): round(v, 4) for k, v in times.items()}))
+    with _LOCK:
+        _MEM[key] = best
+    return best
+
+
+def _device_name(res) -> str:
+    return torch.cuda.get_device_name(res.device).replace("|", "/")
+
+
+def synth_waves(res, N: int, prec: int, fdtype=torch.float64, bufs=None) -> int:
+    """Waves per workgroup for the vector synthesis of 4 groups of N levels
+    (the dynamo step's), FP64 (prec 0) or FP32 tables (prec 1) writing Fourier
+    buffers of `fdtype`.  bufs: (a spectral (L, N) array, the F buffer)."""
+    if os.environ.get("MAGIC_AMD_TUNE", "1") == "0":
+        return 0
+    fprec = 1 if fdtype == torch.float32 else 0
+    key = (f"{_device_name(res)}|leg_synth_vec|L={res.L}|H={res.H}|M={res.M}|B={4 * N}"
+           f"|prec={prec}|fprec={fprec}")
+    buf = {}
+
+    def launch(w):  # buffers made on first use (none on a cache hit)
+        if not buf:
+            if bufs is None:
+                buf["x"] = torch.zeros(res.L, N, dtype=torch.complex128, device=res.device)
+                buf["F"] = res.fourier_buffer(3, 4 * N, fdtype)
+            else:
+                buf["x"], buf["F"] = bufs
+        x = buf["x"]
+        groups = [[(x, None, 1), (x, None, 0), (x, x, 2)] for _ in range(4)]
+        res.synth(groups, N, N, buf["F"], 4 * N, vector=True, prec=prec, waves=w)
+
+    return timed_choice(key, [4, 6], launch, 4)
 
 
 def grid_key(res, N: int, fprec: int, l_mag: bool) -> str:
-    name = torch.cuda.get_device_name(res.device).replace("|", "/")
-    return f"{name}|grid_step|Mh={res.Mh}|T={res.T}|M={res.M}|N={N}|fprec={fprec}|mag={int(l_mag)}"
+    return f"{_device_name(res)}|grid_step|Mh={res.Mh}|T={res.T}|M={res.M}|N={N}|fprec={fprec}|mag={int(l_mag)}"
 
 
 def grid_level_block(res, N: int, p, fdtype=torch.float64, bufs=None, reps: int = 3) -> int:
@@ -83,37 +143,20 @@ def grid_level_block(res, N: int, p, fdtype=torch.float64, bufs=None, reps: int 
         return 0
     fprec = 1 if fdtype == torch.float32 else 0
     own = res.grid_plan()[2]
-    key = grid_key(res, N, fprec, p.l_mag)
-    with _LOCK:
-        if key in _MEM:
-            return _MEM[key]
-        hit = _read_file().get(key)
-        if hit is not None and hit.get("rb", -1) in [0] + grid_candidates(own):
-            _MEM[key] = int(hit["rb"])
-            return _MEM[key]
     cands = grid_candidates(own)
     if len(cands) == 1:
-        rb = 0
-    else:
-        if bufs is None:
-            Fs = res.fourier_buffer(1, N, fdtype).normal_(0.0, 1e-3)
-            Fv = res.fourier_buffer(3, 4 * N, fdtype).normal_(0.0, 1e-3)
-            Fo = res.fourier_buffer(9, N - 2, fdtype)
-            cfl = torch.zeros(2 * N, dtype=torch.int64, device=res.device)
-        else:
-            Fs, Fv, Fo, cfl = bufs
-        times = {}
-        for c in cands:
-            res.grid_step(Fs, Fv, Fo, N, cfl, p, rb=0 if c == own else c)  # warm-up
-            ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
-            ev[0].record()
-            for _ in range(reps):
-                res.grid_step(Fs, Fv, Fo, N, cfl, p, rb=0 if c == own else c)
-            ev[1].record()
-            ev[1].synchronize()
-            times[c] = ev[0].elapsed_time(ev[1]) / reps
-        rb = choose(times, own)
-        _write_file(key, dict(rb=rb, ms={str(k): round(v, 4) for k, v in times.items()}))
-    with _LOCK:
-        _MEM[key] = rb
-    return rb
+        return 0
+    buf = []
+
+    def launch(c):  # buffers made on first use (none on a cache hit)
+        if not buf:
+            buf.extend(bufs if bufs is not None else (
+                res.fourier_buffer(1, N, fdtype).normal_(0.0, 1e-3),
+                res.fourier_buffer(3, 4 * N, fdtype).normal_(0.0, 1e-3),
+                res.fourier_buffer(9, N - 2, fdtype),
+                torch.zeros(2 * N, dtype=torch.int64, device=res.device)))
+        Fs, Fv, Fo, cfl = buf
+        res.grid_step(Fs, Fv, Fo, N, cfl, p, rb=0 if c == own else c)
+
+    best = timed_choice(grid_key(res, N, fprec, p.l_mag), cands, launch, own, reps)
+    return 0 if best == own else best
