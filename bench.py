@@ -138,7 +138,7 @@ def run_case(precision, steps, warmup, dt_schedule=None, section_steps=20):
     dt = run.ctl_end()
     out = dict(elapsed=t1 - t0, energies=run.energies(), spectra=run.spectra(), dt=dt,
                rebuilds=run.n_rebuild, p=run.p, N=run.N, grid_rb=run.grid_rb,
-               syn_waves=run.syn_waves)
+               syn_waves=run.syn_waves, ana_waves=run.ana_waves)
     run.enable_timers(True)
     run.run_steps(section_steps, dt)
     out["secs"] = run.timer_summary()
@@ -345,6 +345,7 @@ def main():
            "dt": h["dt"], "rebuilds": {pr: r["rebuilds"] for pr, r in res.items()},
            "grid_levels_per_workgroup": {pr: r["grid_rb"] or "plan default" for pr, r in res.items()},
            "synthesis_waves_per_workgroup": {pr: r["syn_waves"] or "default" for pr, r in res.items()},
+           "analysis_waves_per_workgroup": {pr: r["ana_waves"] or "default" for pr, r in res.items()},
            "energies": {pr: r["energies"] for pr, r in res.items()},
            "section_ms": breakdown}
     if len(res) == 2:

@@ -83,7 +83,7 @@ typedef struct {
   int ntiles;
   int prec;  /* as MhSynArgs.prec (anaW/anaWd hold floats for 1) */
   int fprec; /* as MhSynArgs.fprec: F holds floats for 1 (needs prec = 1) */
-  int pad;
+  int waves; /* waves per workgroup: 0 = default (6), 6 or 4; tiles must hold 16 * waves degrees */
 } MhAnaArgs;
 
 /* Legendre analysis on FP64 MFMA: scalar replaces the bmm part of scal_to_SH

@@ -340,3 +340,35 @@ def test_synthesis_workgroup_sizes_give_identical_results(l_max, prec):
         outs[w] = (Fv, Fs)
     assert torch.isfinite(outs[4][0]).all() and outs[4][0].abs().max() > 0
     assert torch.equal(outs[4][0], outs[6][0]) and torch.equal(outs[4][1], outs[6][1])
+
+
+@pytest.mark.parametrize("l_max,prec", [(32, 0), (32, 1), (85, 0), (383, 0), (383, 1)])
+def test_analysis_workgroup_sizes_give_identical_results(l_max, prec):
+    """The vector and scalar Legendre analysis with 6 and 4 waves per workgroup
+    (each with its own tile list) write the same coefficients bit for bit and
+    leave the boundary columns untouched."""
+    import torch
+    from magic_amd.device import Resolution
+    from magic_amd.params import Params
+    N = 9
+    Nb = N - 2
+    p = Params(dict(_GRID_CFG, l_max=l_max, n_r_max=N, precision="mixed" if prec else "fp64"))
+    res = Resolution(p, "cuda")
+    fdt = torch.float32 if prec else torch.float64
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(6)
+    F = torch.randn(res.fourier_buffer(9, Nb, fdt).shape, generator=gen, device="cuda", dtype=fdt)
+    outs = {}
+    for w in (6, 4):
+        y = [torch.zeros(res.L, N, dtype=torch.complex128, device="cuda") for _ in range(8)]
+        o = lambda t: t.data_ptr() + 16
+        res.analysis(F, Nb, Nb, [(3, 4), (5, 6), (7, 8)], [o(y[0]), o(y[1]), o(y[2])],
+                     [o(y[3]), None, o(y[4])], N, vector=True, prec=prec, waves=w)
+        res.analysis(F, Nb, Nb, [0, 1, 2], [o(y[5]), o(y[6]), o(y[7])], [None, None, None], N,
+                     vector=False, prec=prec, waves=w)
+        torch.cuda.synchronize()
+        outs[w] = y
+    for a, b in zip(outs[6], outs[4]):
+        assert torch.equal(a, b)
+        assert not a[:, 0].any() and not a[:, N - 1].any()
+    assert outs[6][0].abs().max() > 0

@@ -73,17 +73,27 @@ def test_lm_coefficients_match_oracle():
         assert np.all(maps["l"][rows] == l) and np.array_equal(maps["m"][rows], np.arange(l + 1))
 
 
-def test_sht_layout_covers_every_mode():
-    p = Params({"l_max": 33})
+@pytest.mark.parametrize("l_max", [33, 383])
+def test_sht_layout_covers_every_mode(l_max):
+    """Every (m, parity) is covered once by analysis tiles of 16 * waves degrees
+    (waves 6 and 4), in 8 equal-length lists (list x: m = x mod 8), padded
+    with m = n_m."""
+    p = Params({"l_max": l_max})
     lay = grid.sht_layout(p, grid.lm_maps(p))
     info = lay["mp_info"].reshape(p.n_m, 2, 4)
     assert info[:, :, 0].sum() == p.lm_max
-    tiles = lay["ana_tiles"].reshape(-1, 3)
-    for m in range(p.n_m):
-        for par in range(2):
-            K = info[m, par, 0]
-            ks = sorted(t[2] for t in tiles if t[0] == m and t[1] == par)
-            assert ks == list(range(0, K, 32))
+    for waves in (6, 4):
+        tiles = lay[f"ana_tiles_{waves}"].reshape(8, -1, 3)
+        assert tiles.shape[1] * 8 == lay[f"n_ana_tiles_{waves}"]
+        for x in range(8):
+            real = tiles[x][tiles[x][:, 0] < p.n_m]
+            assert np.all(real[:, 0] % 8 == x)
+        flat = tiles.reshape(-1, 3)
+        for m in range(p.n_m):
+            for par in range(2):
+                K = info[m, par, 0]
+                ks = sorted(t[2] for t in flat if t[0] == m and t[1] == par)
+                assert ks == list(range(0, K, 16 * waves))
     assert lay["Hp"] % 32 == 0 and lay["Hp"] >= lay["H"]
 
 

@@ -39,7 +39,7 @@ class AnaArgs(C.Structure):
     _fields_ = [("F", P), ("c_stride", L), ("m_stride", L), ("Bin", I), ("B", I), ("ncomp", I),
                 ("cf1", I * 3), ("cf2", I * 3), ("pad0", I), ("outS", P * 3), ("outT", P * 3),
                 ("ldo", I), ("H", I), ("Hp", I), ("n_m", I), ("mp_info", P), ("lm_start", P),
-                ("anaW", P), ("anaWd", P), ("tiles", P), ("ntiles", I), ("prec", I), ("fprec", I), ("pad", I)]
+                ("anaW", P), ("anaWd", P), ("tiles", P), ("ntiles", I), ("prec", I), ("fprec", I), ("waves", I)]
 
 
 class FftArgs(C.Structure):

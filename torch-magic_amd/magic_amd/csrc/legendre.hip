@@ -401,15 +401,17 @@ typedef MhAnaArgs AnaArgs;
 // l - m) for 32 theta pairs x 16 real columns are staged in LDS once per chunk
 // (double-buffered) and shared by the six waves; the weight tables are read
 // from global memory.
-constexpr int kAnaWaves = 6;
+// Waves per analysis workgroup (16 degrees each): six (the default) or four;
+// the tile list passed in must have tiles of 16 * waves degrees.
+constexpr int kAnaWavesDefault = 6;
 
-template <bool VEC, typename T, typename FT>
-__global__ __launch_bounds__(64 * kAnaWaves, 2) void k_ana(AnaArgs a, int n_m) {
+template <bool VEC, typename T, typename FT, int WAVES>
+__global__ __launch_bounds__(64 * WAVES, 2) void k_ana(AnaArgs a, int n_m) {
   const FT* const Fi = reinterpret_cast<const FT*>(a.F);  // Fourier buffer elements
   typedef Mfma<T> MF;
   typedef typename MF::acc_t acc_t;
   constexpr int NF = VEC ? 4 : 1;          // f1 same, f1 opposite, f2 same, f2 opposite
-  constexpr int NT = 64 * kAnaWaves;
+  constexpr int NT = 64 * WAVES;
   __shared__ T fs[2][NF][kChunk][16];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   // XCD-aware order: the tile list holds 8 equal-length lists, list x with the
@@ -435,11 +437,13 @@ __global__ __launch_bounds__(64 * kAnaWaves, 2) void k_ana(AnaArgs a, int n_m) {
 #pragma unroll
   for (int q = 0; q < (VEC ? 4 : 1); ++q) acc[q] = (acc_t){0, 0, 0, 0};
   // staging map: thread stages real column c16 = tid % 16 of theta pairs
-  // tid / 16 (slot 0) and 24 + tid / 16 (slot 1, tid < 128) of the chunk; the
-  // raw N/S values are loaded before the chunk's MFMAs and folded after them
-  static_assert(NT == 384 && kChunk == 32, "staging map assumes 384 threads, 32-row chunks");
+  // RP*q + tid / 16 (q = 0, 1; RP = NT / 16 rows per pass; rows < 32) of the
+  // chunk; the raw N/S values are loaded before the chunk's MFMAs and folded
+  // after them
+  constexpr int RP = NT / 16;
+  static_assert(2 * RP >= kChunk && RP <= kChunk, "two staging passes cover a 32-row chunk");
   const int c16 = threadIdx.x & 15, srow = threadIdx.x >> 4;
-  const bool s2 = threadIdx.x < 128;
+  const bool s2 = RP + srow < kChunk;
   const int sb = bc0 + (c16 >> 1);
   const bool scol = sb < a.B;
   const FT* f1p = Fi + c1 * a.c_stride + fidx(0, a.m_stride, a.Bin, 0, m, 0, scol ? sb : 0) + (c16 & 1);
@@ -449,7 +453,7 @@ __global__ __launch_bounds__(64 * kAnaWaves, 2) void k_ana(AnaArgs a, int n_m) {
   auto fetch = [&](int j0) {
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
-      const int j = j0 + 24 * q + srow;
+      const int j = j0 + RP * q + srow;
 #pragma unroll
       for (int f = 0; f < (VEC ? 4 : 2); ++f) raw[q][f] = 0.0;
       if ((q == 0 || s2) && scol && j < a.H) {
@@ -467,7 +471,7 @@ __global__ __launch_bounds__(64 * kAnaWaves, 2) void k_ana(AnaArgs a, int n_m) {
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
       if (q == 0 || s2) {
-        const int jr = 24 * q + srow;
+        const int jr = RP * q + srow;
         fs[buf][0][jr][c16] = (T)(raw[q][0] + sg * raw[q][1]);
         if (VEC) {
           fs[buf][1][jr][c16] = (T)(raw[q][0] - sg * raw[q][1]);
@@ -573,21 +577,30 @@ extern "C" int mh_leg_analysis(int vector, const MhAnaArgs* args, void* stream) 
     return MH_EINVAL;
   const int n_m = a.n_m;
   if (a.ntiles % 8 != 0) return MH_EINVAL;
+  const int waves = a.waves == 0 ? kAnaWavesDefault : a.waves;
+  if (waves != 4 && waves != 6) return MH_EINVAL;
   dim3 grid((unsigned)(a.ncomp * mh_cdiv(a.B, 8) * a.ntiles));
-  const dim3 blk(64 * kAnaWaves);
+  const dim3 blk(64 * waves);
   if (a.prec != 0 && a.prec != 1) return MH_EINVAL;
   if (a.fprec != 0 && (a.fprec != 1 || a.prec != 1)) return MH_EINVAL;
   hipStream_t st = mh_stream(stream);
-  if (a.prec == 0) {
-    if (vector) hipLaunchKernelGGL((k_ana<true, double, double>), grid, blk, 0, st, a, n_m);
-    else hipLaunchKernelGGL((k_ana<false, double, double>), grid, blk, 0, st, a, n_m);
-  } else if (a.fprec == 0) {
-    if (vector) hipLaunchKernelGGL((k_ana<true, float, double>), grid, blk, 0, st, a, n_m);
-    else hipLaunchKernelGGL((k_ana<false, float, double>), grid, blk, 0, st, a, n_m);
-  } else {
-    if (vector) hipLaunchKernelGGL((k_ana<true, float, float>), grid, blk, 0, st, a, n_m);
-    else hipLaunchKernelGGL((k_ana<false, float, float>), grid, blk, 0, st, a, n_m);
+#define MH_ANA_LAUNCH(W)                                                                      \
+  if (a.prec == 0) {                                                                          \
+    if (vector) hipLaunchKernelGGL((k_ana<true, double, double, W>), grid, blk, 0, st, a, n_m); \
+    else hipLaunchKernelGGL((k_ana<false, double, double, W>), grid, blk, 0, st, a, n_m);     \
+  } else if (a.fprec == 0) {                                                                  \
+    if (vector) hipLaunchKernelGGL((k_ana<true, float, double, W>), grid, blk, 0, st, a, n_m); \
+    else hipLaunchKernelGGL((k_ana<false, float, double, W>), grid, blk, 0, st, a, n_m);      \
+  } else {                                                                                    \
+    if (vector) hipLaunchKernelGGL((k_ana<true, float, float, W>), grid, blk, 0, st, a, n_m); \
+    else hipLaunchKernelGGL((k_ana<false, float, float, W>), grid, blk, 0, st, a, n_m);       \
   }
+  if (waves == 6) {
+    MH_ANA_LAUNCH(6)
+  } else {
+    MH_ANA_LAUNCH(4)
+  }
+#undef MH_ANA_LAUNCH
   return mh_launch_status();
 }
 

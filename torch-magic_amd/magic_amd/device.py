@@ -61,7 +61,8 @@ class Resolution:
         self.d["l"] = up(self.maps["l"], I32)
         self.d["m"] = up(self.maps["m"], I32)
         self.d["l_rows"] = up(self.maps["l_rows"], I32)
-        self.d["ana_tiles"] = up(self.lay["ana_tiles"], I32)
+        for w in (6, 4):
+            self.d[f"ana_tiles_{w}"] = up(self.lay[f"ana_tiles_{w}"], I32)
         for k, v in self.lmc.items():
             self.d[k] = up(v)
         for k in ("or1", "or2", "or3", "or4", "r2", "rgrav", "q", "valri2", "valhi2"):
@@ -137,11 +138,12 @@ class Resolution:
         a.waves = int(waves)
         _lib.call("mh_leg_synth", 1 if vector else 0, _lib.ref(a), self.M, stream_handle())
 
-    def analysis(self, F, Bin, B, comps, outS, outT, ldo, vector: bool, prec=0):
+    def analysis(self, F, Bin, B, comps, outS, outT, ldo, vector: bool, prec=0, waves=0):
         """Legendre analysis of F[c][m][t][b] (comp stride = M*T*Bin complex).
 
         comps: scalar -> list of component indices; vector -> list of (c_vtheta, c_vphi).
         outS/outT: output tensors (or pointers as ints) per component / pair.
+        waves: waves per workgroup (0: the library default, 6; or 4), same results.
         """
         a = _lib.AnaArgs()
         a.F = ptr(F)
@@ -164,7 +166,9 @@ class Resolution:
         else:
             a.anaW, a.anaWd = ptr(self.anaW), ptr(self.anaWd)
         a.prec = prec
-        a.tiles, a.ntiles = ptr(self.d["ana_tiles"]), self.lay["n_ana_tiles"]
+        w = waves or 6
+        a.tiles, a.ntiles = ptr(self.d[f"ana_tiles_{w}"]), self.lay[f"n_ana_tiles_{w}"]
+        a.waves = w
         a.n_m = self.M
         _lib.call("mh_leg_analysis", 1 if vector else 0, _lib.ref(a), stream_handle())
 

@@ -251,6 +251,8 @@ class DynamoRun:
             self.grid_rb = tune.grid_level_block(r, N, p, fdt,
                                                  bufs=(self.F_s, self.F_v, self.F_o, self.cfl))
         self.syn_waves = tune.synth_waves(r, N, self.leg_prec, fdt, bufs=(self.f["s"], self.F_v))
+        self.ana_waves = tune.analysis_waves(r, N, self.leg_prec, fdt,
+                                             bufs=(self.F_o, self.nl["Advr"])) if fused_grid else 0
         yield
         self._build_operator_storage()
         yield
@@ -751,12 +753,13 @@ class DynamoRun:
         with self._t("leg_ana_scal"):
             r.analysis(self.F_o, Nb, Nb, [0, 1, 2],
                        [ptr(nl["Advr"]) + off, ptr(nl["dVSr"]) + off, ptr(nl["VxBr"]) + off],
-                       [None, None, None], N, vector=False, prec=self.leg_prec)
+                       [None, None, None], N, vector=False, prec=self.leg_prec,
+                       waves=self.ana_waves)
         with self._t("leg_ana_vec"):
             r.analysis(self.F_o, Nb, Nb, [(3, 4), (5, 6), (7, 8)],
                        [ptr(nl["Advt"]) + off, ptr(nl["VSt"]) + off, ptr(nl["VxBt"]) + off],
                        [ptr(nl["Advp"]) + off, None, ptr(nl["VxBp"]) + off], N, vector=True,
-                       prec=self.leg_prec)
+                       prec=self.leg_prec, waves=self.ana_waves)
         self._td_launches()
 
     def _grid_separate(self):

@@ -165,8 +165,6 @@ def sht_layout(p: Params, maps: dict) -> dict:
     Hp = -(-H // 32) * 32
     info = np.zeros((p.n_m, 2, 4), np.int32)
     syn_off = ana_off = 0
-    tiles = []
-    per_xcd = [[] for _ in range(8)]
     for m in range(p.n_m):
         for par in range(2):
             K = (p.l_max - m - par) // 2 + 1 if p.l_max - m - par >= 0 else 0
@@ -175,14 +173,28 @@ def sht_layout(p: Params, maps: dict) -> dict:
             info[m, par] = (K, syn_off, ana_off, K16)
             syn_off += K4 * Hp
             ana_off += K16 * Hp
-            for k0 in range(0, K, 96):  # one analysis workgroup = 6 waves x 16 degrees
-                per_xcd[m % 8].append((m, par, k0))
-    # analysis tiles as 8 equal-length lists (list x: orders m = x mod 8; the
-    # kernel gives list x to the workgroups of XCD x), padded with m = n_m
-    tpx = max(len(t) for t in per_xcd)
-    for t in per_xcd:
-        tiles += t + [(p.n_m, 0, 0)] * (tpx - len(t))
     if syn_off >= 2 ** 31 or ana_off >= 2 ** 31:
         raise ValueError("Legendre tables exceed 32-bit offsets")
-    return dict(H=H, Hp=Hp, mp_info=info.reshape(-1), syn_size=syn_off, ana_size=ana_off,
-                ana_tiles=np.asarray(tiles, np.int32).reshape(-1), n_ana_tiles=len(tiles))
+    out = dict(H=H, Hp=Hp, mp_info=info.reshape(-1), syn_size=syn_off, ana_size=ana_off)
+    for waves in (6, 4):
+        tl = analysis_tiles(p, info, 16 * waves)
+        out[f"ana_tiles_{waves}"] = tl.reshape(-1)
+        out[f"n_ana_tiles_{waves}"] = len(tl)
+    return out
+
+
+def analysis_tiles(p: Params, info: np.ndarray, tile: int) -> np.ndarray:
+    """Analysis work list: one workgroup per (m, parity, first degree index k0)
+    with `tile` degrees (16 per wave), as 8 equal-length lists (list x holds
+    the orders m = x mod 8; the kernel gives list x to the workgroups of XCD
+    x), padded with m = n_m."""
+    per_xcd = [[] for _ in range(8)]
+    for m in range(p.n_m):
+        for par in range(2):
+            for k0 in range(0, int(info[m, par, 0]), tile):
+                per_xcd[m % 8].append((m, par, k0))
+    tpx = max(len(t) for t in per_xcd)
+    tiles = []
+    for t in per_xcd:
+        tiles += t + [(p.n_m, 0, 0)] * (tpx - len(t))
+    return np.asarray(tiles, np.int32)

@@ -149,6 +149,7 @@ class EnsembleServer:
             fdt = self.torch.float32 if p.precision == "mixed" else self.torch.float64
             tune.grid_level_block(res, p.n_r_max, p, fdt)
             tune.synth_waves(res, p.n_r_max, 1 if p.precision == "mixed" else 0, fdt)
+            tune.analysis_waves(res, p.n_r_max, 1 if p.precision == "mixed" else 0, fdt)
         self.torch.cuda.synchronize(self.device)
 
     def _admit_stages(self, sl: _Slot):

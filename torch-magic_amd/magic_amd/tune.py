@@ -7,7 +7,9 @@ the same results bit for bit:
 * the fused grid step (``mh_grid_step``): radial levels per workgroup, 1, 2,
   ... up to the most its FFT plan fits (``mh_grid_plan``);
 * the Legendre synthesis (``mh_leg_synth``): waves per workgroup, 4 or 6
-  (16 theta pairs each).
+  (16 theta pairs each);
+* the Legendre analysis (``mh_leg_analysis``): waves per workgroup, 6 or 4
+  (16 degrees each).
 
 Choices are remembered in memory for the process and in a JSON file
 (``MAGIC_AMD_TUNE_CACHE``, default ``~/.cache/magic_amd/tune.json``; not
@@ -127,6 +129,33 @@ def synth_waves(res, N: int, prec: int, fdtype=torch.float64, bufs=None) -> int:
         res.synth(groups, N, N, buf["F"], 4 * N, vector=True, prec=prec, waves=w)
 
     return timed_choice(key, [4, 6], launch, 4)
+
+
+def analysis_waves(res, N: int, prec: int, fdtype=torch.float64, bufs=None) -> int:
+    """Waves per workgroup for the vector analysis of 3 pairs of N-2 levels (the
+    dynamo step's), FP64 (prec 0) or FP32 tables (prec 1) reading Fourier
+    buffers of `fdtype`.  bufs: (the F buffer of 9 components, an (L, N)
+    output array whose boundary columns 0 and N-1 are left untouched)."""
+    if os.environ.get("MAGIC_AMD_TUNE", "1") == "0":
+        return 0
+    fprec = 1 if fdtype == torch.float32 else 0
+    Nb = N - 2
+    key = (f"{_device_name(res)}|leg_analysis_vec|L={res.L}|H={res.H}|M={res.M}|B={Nb}"
+           f"|prec={prec}|fprec={fprec}")
+    buf = {}
+
+    def launch(w):  # buffers made on first use (none on a cache hit)
+        if not buf:
+            if bufs is None:
+                buf["F"] = res.fourier_buffer(9, Nb, fdtype).zero_()
+                buf["y"] = torch.zeros(res.L, N, dtype=torch.complex128, device=res.device)
+            else:
+                buf["F"], buf["y"] = bufs
+        y = buf["y"].data_ptr() + 16  # columns 1 .. N-2, as the step writes them
+        res.analysis(buf["F"], Nb, Nb, [(3, 4), (5, 6), (7, 8)], [y, y, y], [y, None, y], N,
+                     vector=True, prec=prec, waves=w)
+
+    return timed_choice(key, [6, 4], launch, 6)
 
 
 def grid_key(res, N: int, fprec: int, l_mag: bool) -> str:
