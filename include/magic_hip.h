@@ -183,8 +183,10 @@ typedef struct {
  * the per-degree implicit solves of all systems with the DCT-I and the radial
  * derivatives folded in (chunked_solve_complex algebra.py:271-310, costf
  * cosine_transform.py:18-45, get_dr/get_ddr/get_dddr radial_derivatives.py:
- * 18-68, the D123 matmul step_time.py:315-341). */
-int mh_rowop(const MhRowJob* jobs, const int* tiles, int ntiles, void* stream);
+ * 18-68, the D123 matmul step_time.py:315-341).  wide = 0: tiles of 32 rows x
+ * 64 columns (col0 multiples of 64); wide = 1: tiles of 32 rows x 128 columns
+ * (col0 multiples of 128), fewer operand loads per MFMA.  Same results. */
+int mh_rowop(const MhRowJob* jobs, const int* tiles, int ntiles, int wide, void* stream);
 
 /* ------------------------------------------------------------ spectral side */
 typedef struct {

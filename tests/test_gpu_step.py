@@ -372,3 +372,29 @@ def test_analysis_workgroup_sizes_give_identical_results(l_max, prec):
         assert torch.equal(a, b)
         assert not a[:, 0].any() and not a[:, N - 1].any()
     assert outs[6][0].abs().max() > 0
+
+
+@pytest.mark.parametrize("l_max,n_r", [(32, 33), (127, 65), (383, 97)])
+def test_row_operator_tile_shapes_give_identical_results(l_max, n_r):
+    """The row operator with 32 x 64 and with 32 x 128 tiles (the tuner's
+    candidates) writes the same results bit for bit, for every job table of a
+    run: the S/Z/B/J and (w,p) solves with their derivatives, the flux
+    derivatives and the set-up derivatives, on random operators and inputs."""
+    import torch
+    from magic_amd.dynamo import DynamoRun
+    run = DynamoRun(dict(_GRID_CFG, l_max=l_max, n_r_max=n_r))
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(9)
+    for t in (run.MT4, run.MT["wp"], run.MT_d1[0], run.MT_d1r2[0], run.MT_d12[0], run.MT_d123[0]):
+        t.copy_(torch.randn(t.shape, generator=gen, device="cuda", dtype=t.dtype))
+    start = torch.randn(run._pool.shape, generator=gen, device="cuda", dtype=run._pool.dtype)
+    for name in ("jobs1", "jobs2", "jobs_td", "jobs_setup"):
+        out = []
+        for wide in (0, 1):
+            run._pool.copy_(start)
+            run._rowop(getattr(run, name), wide)
+            torch.cuda.synchronize()
+            out.append(run._pool.clone())
+        assert not torch.equal(out[0], start), name
+        assert torch.equal(out[0], out[1]), name
+    run.close()

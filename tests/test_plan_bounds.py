@@ -105,10 +105,14 @@ def _check_rebuild_calls(calls, ext):
 
 def _check_row_jobs(run, ext):
     for key in ("jobs1", "jobs2", "jobs_td", "jobs_setup"):
-        dj, tl, nt = getattr(run, key)
+        dj, tile_lists = getattr(run, key)
         jobs = np.frombuffer(dj.numpy().tobytes(), dtype=dyn._ROWJOB_DTYPE)
-        tiles = tl.numpy().reshape(-1, 3)
-        assert len(tiles) == nt and tiles[:, 0].max() < len(jobs)
+        for tc, (tl, nt) in tile_lists.items():
+            tiles = tl.numpy().reshape(-1, 3)
+            assert len(tiles) == nt and tiles[:, 0].max() < len(jobs)
+            # every tile starts inside its job: rows below nrows, columns below R
+            assert np.all(tiles[:, 1] < jobs["nrows"][tiles[:, 0]])
+            assert np.all(tiles[:, 2] < jobs["R"][tiles[:, 0]]) and np.all(tiles[:, 2] % tc == 0)
         for j in jobs:
             nrows, ldx, ldy, K, R, Rpad, rseg = (int(j[k]) for k in
                                                  ("nrows", "ldx", "ldy", "K", "R", "Rpad", "rseg"))

@@ -152,7 +152,7 @@ _SIGS = {
     "mh_grid_step": [P, P],
     "mh_grid_plan": [I, P],
     "mh_tables_to_f32": [P, P, L, P],
-    "mh_rowop": [P, P, I, P],
+    "mh_rowop": [P, P, I, I, P],
     "mh_td": [P, P],
     "mh_rhs": [P, P],
     "mh_buoy": [P, P],

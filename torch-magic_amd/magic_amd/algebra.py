@@ -59,7 +59,7 @@ def chunked_solve_complex(inv_by_l, l_index, rhs_complex, chunk_size=None):
     arr = (_lib.RowJob * len(jobs))(*jobs)
     dj = torch.frombuffer(bytearray(bytes(arr)), dtype=torch.uint8).to(dev)
     dt = torch.as_tensor(np.asarray(tiles, np.int32).reshape(-1)).to(dev)
-    _lib.call("mh_rowop", ptr(dj), ptr(dt), len(tiles), stream_handle())
+    _lib.call("mh_rowop", ptr(dj), ptr(dt), len(tiles), 0, stream_handle())
     torch.cuda.current_stream().synchronize()
     return out
 

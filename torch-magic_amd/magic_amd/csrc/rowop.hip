@@ -94,10 +94,99 @@ __global__ __launch_bounds__(256, 4) void k_rowop(const RowJob* __restrict__ job
   }
 }
 
-extern "C" int mh_rowop(const MhRowJob* jobs, const int* tiles, int ntiles, void* stream) {
-  if (ntiles < 0) return MH_EINVAL;
+// Wide tiles: (job, row0, col0) with col0 a multiple of 128; WG = 4 waves,
+// wave w takes lm rows row0 + 16*(w & 1) .. + 15 (two MFMA row tiles of 8 lm
+// rows x re/im) and columns col0 + 64*(w >> 1) .. + 63 (four MFMA column
+// tiles), so each A operand feeds four MFMAs and each B operand two: 6 loads
+// per 8 MFMAs instead of 5 per 4.  Same arithmetic order per output as
+// k_rowop (k ascending in steps of 4), hence the same results bit for bit.
+__global__ __launch_bounds__(256, 4) void k_rowop_wide(const RowJob* __restrict__ jobs,
+                                                    const int* __restrict__ tiles) {
+  const int* tl = tiles + 3 * blockIdx.x;
+  const RowJob& J = jobs[tl[0]];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int rbase = tl[1] + 16 * (w & 1);
+  const int col0 = tl[2] + 64 * (w >> 1);
+  // no barriers in this kernel: a wave with no rows or past the padded
+  // operator width (a multiple of 64) has nothing to do
+  if (rbase >= J.nrows || col0 >= J.Rpad) return;
+  const int kk = lane >> 4, c16 = lane & 15;
+  const MH_GLOBAL double* xb = gconst(J.X);
+  int xo[2];
+  bool va[2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const int ra = rbase + 8 * i + (c16 >> 1);
+    va[i] = ra < J.nrows;
+    const int lma = va[i] ? (J.rows ? gconst(J.rows)[ra] : ra) : 0;
+    xo[i] = lma * 2 * J.ldx + (c16 & 1);
+  }
+  double4_t acc[2][4];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[i][q] = (double4_t){0.0, 0.0, 0.0, 0.0};
+  const MH_GLOBAL double* mtb = gconst(J.MT) + col0;
+  const int mo = kk * J.Rpad + c16;
+  constexpr int D = 4;
+  const int nst = (J.K + 3) / 4;
+  double xa[D][2], rb[D][4];
+  auto load = [&](int t, double (&av)[2], double (&bv)[4]) __attribute__((always_inline)) {
+    const int k = 2 * min(4 * t + kk, J.K - 1);
+    av[0] = xb[xo[0] + k];
+    av[1] = xb[xo[1] + k];
+    const int m = mo + 4 * t * J.Rpad;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) bv[q] = mtb[m + 16 * q];
+  };
+#pragma unroll
+  for (int d = 0; d < D; ++d) load(min(d, nst - 1), xa[d], rb[d]);
+  auto step = [&](int t, int d) __attribute__((always_inline)) {
+    const bool kin = 4 * t + kk < J.K;
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const double a = (va[i] && kin) ? xa[d][i] : 0.0;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) acc[i][q] = mfma_f64(a, rb[d][q], acc[i][q]);
+    }
+    load(min(t + D, nst - 1), xa[d], rb[d]);
+  };
+  int t0 = 0;
+  for (; t0 + D <= nst; t0 += D) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) step(t0 + d, d);
+  }
+#pragma unroll
+  for (int d = 0; d < D; ++d)
+    if (t0 + d < nst) step(t0 + d, d);
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int row = mfma_row(lane, v);  // 16 rows = 8 lm rows x (re, im)
+      const int ro = rbase + 8 * i + (row >> 1);
+      if (ro >= J.nrows) continue;
+      const long lmo = J.rows ? gconst(J.rows)[ro] : ro;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int r = col0 + 16 * q + c16;
+        if (r < J.R) {
+          const int seg = r / J.rseg, rc = r - seg * J.rseg;
+          gmut(J.Y[seg])[lmo * 2L * J.ldy + 2 * rc + (row & 1)] = acc[i][q][v];
+        }
+      }
+    }
+  }
+}
+
+extern "C" int mh_rowop(const MhRowJob* jobs, const int* tiles, int ntiles, int wide,
+                        void* stream) {
+  if (ntiles < 0 || (wide != 0 && wide != 1)) return MH_EINVAL;
   if (ntiles == 0) return MH_OK;
-  hipLaunchKernelGGL(k_rowop, dim3(ntiles), dim3(256), 0, mh_stream(stream), jobs, tiles);
+  if (wide)
+    hipLaunchKernelGGL(k_rowop_wide, dim3(ntiles), dim3(256), 0, mh_stream(stream), jobs, tiles);
+  else
+    hipLaunchKernelGGL(k_rowop, dim3(ntiles), dim3(256), 0, mh_stream(stream), jobs, tiles);
   return mh_launch_status();
 }
 

@@ -64,11 +64,12 @@ def _rowjob_dtype():
 _ROWJOB_DTYPE = _rowjob_dtype()
 
 
-def pack_row_jobs(specs):
+def pack_row_jobs(specs, tile_cols=64):
     """Pack row-operator job columns into MhRowJob records and build the tile
     list (job, first row, first column): rows in chunks of 32, operator columns
-    in chunks of 64, in job order.  Each spec is a dict of MhRowJob fields whose
-    rows / nrows / MT entries may be arrays (one job per element)."""
+    in chunks of `tile_cols` (64, or 128 for the wide kernel), in job order.
+    Each spec is a dict of MhRowJob fields whose rows / nrows / MT entries may
+    be arrays (one job per element)."""
     n = [int(np.size(sp["nrows"])) for sp in specs]
     tot = sum(n)
     arr = np.zeros(tot, dtype=_ROWJOB_DTYPE)
@@ -80,12 +81,12 @@ def pack_row_jobs(specs):
         sl["Y"] = np.asarray(list(sp["Y"]) + [0] * (6 - len(sp["Y"])), np.uint64)[None, :]
         k += m
     nr = -(-arr["nrows"].astype(np.int64) // 32)
-    nc = -(-arr["R"].astype(np.int64) // 64)
+    nc = -(-arr["R"].astype(np.int64) // tile_cols)
     per = nr * nc
     ji = np.repeat(np.arange(tot), per)
     q = np.arange(per.sum()) - np.repeat(np.cumsum(per) - per, per)
     ncj = np.repeat(nc, per)
-    tiles = np.stack([ji, 32 * (q // ncj), 64 * (q % ncj)], axis=1).astype(np.int32)
+    tiles = np.stack([ji, 32 * (q // ncj), tile_cols * (q % ncj)], axis=1).astype(np.int32)
     return arr, tiles
 
 
@@ -257,6 +258,9 @@ class DynamoRun:
         self._build_operator_storage()
         yield
         self._build_jobs()
+        # row-operator tile shape (32 x 64 or 32 x 128), timed on the S, Z, B, J
+        # solves (the fields are still zero: nothing to overwrite)
+        self.rowop_wide = tune.rowop_wide(self)
         self.dt_hist = [0.0, 0.0]
         self.dtMax = 0.0
         self.wimp_lin0 = None
@@ -581,10 +585,13 @@ class DynamoRun:
                         Y=list(Y), ldy=ldy, rseg=rseg)
 
         def upload(specs):
-            arr, tiles = pack_row_jobs(specs)
+            """(job records, {tile columns: (tile list, count)}) on the device."""
+            out = {}
+            for tc in (64, 128):
+                arr, tiles = pack_row_jobs(specs, tc)
+                out[tc] = (torch.as_tensor(tiles.reshape(-1)).to(self.device), int(tiles.shape[0]))
             dj = torch.frombuffer(bytearray(arr.tobytes()), dtype=torch.uint8).to(self.device)
-            tl = torch.as_tensor(tiles.reshape(-1)).to(self.device)
-            return dj, tl, int(tiles.shape[0])
+            return dj, out
 
         f, tmp = self.f, self.tmp
         # lm loop, stage 1: S, Z, B, J per degree
@@ -630,9 +637,11 @@ class DynamoRun:
                          [ptr(f["dj"]), ptr(f["ddj"])], N, N)]
         self.jobs_setup = upload(jobs)
 
-    def _rowop(self, jt):
-        dj, tl, nt = jt
-        _lib.call("mh_rowop", ptr(dj), ptr(tl), nt, stream_handle())
+    def _rowop(self, jt, wide=None):
+        dj, tiles = jt
+        wide = self.rowop_wide if wide is None else wide
+        tl, nt = tiles[128 if wide else 64]
+        _lib.call("mh_rowop", ptr(dj), ptr(tl), nt, int(wide), stream_handle())
 
     # ------------------------------------------------------------ coefficient structs
     def _lmcoef(self):

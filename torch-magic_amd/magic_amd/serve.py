@@ -141,15 +141,16 @@ class EnsembleServer:
         """Build the shared device tables of every resolution in `cfgs` and tune
         their grid-step launch shape ahead of serving (as a server loads what it
         serves before taking requests)."""
-        from . import tune
+        from .dynamo import DynamoRun
         from .params import Params
+        seen = set()
         for cfg in cfgs:
             p = Params(dict(cfg))
             res = self._resolution(p)
-            fdt = self.torch.float32 if p.precision == "mixed" else self.torch.float64
-            tune.grid_level_block(res, p.n_r_max, p, fdt)
-            tune.synth_waves(res, p.n_r_max, 1 if p.precision == "mixed" else 0, fdt)
-            tune.analysis_waves(res, p.n_r_max, 1 if p.precision == "mixed" else 0, fdt)
+            key = (id(res), p.precision, p.l_mag)
+            if key not in seen:  # a throw-away run times every launch shape once
+                seen.add(key)
+                DynamoRun(p, self.device, res=res).close()
         self.torch.cuda.synchronize(self.device)
 
     def _admit_stages(self, sl: _Slot):

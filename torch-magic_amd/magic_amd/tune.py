@@ -9,7 +9,8 @@ the same results bit for bit:
 * the Legendre synthesis (``mh_leg_synth``): waves per workgroup, 4 or 6
   (16 theta pairs each);
 * the Legendre analysis (``mh_leg_analysis``): waves per workgroup, 6 or 4
-  (16 degrees each).
+  (16 degrees each);
+* the row operator (``mh_rowop``): tiles of 32 rows x 64 or x 128 columns.
 
 Choices are remembered in memory for the process and in a JSON file
 (``MAGIC_AMD_TUNE_CACHE``, default ``~/.cache/magic_amd/tune.json``; not
@@ -156,6 +157,17 @@ def analysis_waves(res, N: int, prec: int, fdtype=torch.float64, bufs=None) -> i
                      vector=True, prec=prec, waves=w)
 
     return timed_choice(key, [6, 4], launch, 6)
+
+
+def rowop_wide(run) -> int:
+    """1 when the wide row-operator tiles (32 x 128) beat the narrow ones
+    (32 x 64) on the run's S, Z, B, J solves.  Runs the solves on the run's
+    current (zero) state, so call it before the fields are set."""
+    if os.environ.get("MAGIC_AMD_TUNE", "1") == "0":
+        return 0
+    r = run.res
+    key = f"{_device_name(r)}|rowop|L={r.L}|N={run.N}|l_max={run.p.l_max}|mag={int(run.p.l_mag)}"
+    return timed_choice(key, [0, 1], lambda w: run._rowop(run.jobs1, w), 0)
 
 
 def grid_key(res, N: int, fprec: int, l_mag: bool) -> str:
