@@ -398,3 +398,35 @@ def test_row_operator_tile_shapes_give_identical_results(l_max, n_r):
         assert not torch.equal(out[0], start), name
         assert torch.equal(out[0], out[1]), name
     run.close()
+
+
+def test_headline_configuration_mixed_tracks_fp64():
+    """The headline configuration (l_max=383, n_r=97, E=1e-5, Ra=2e8, Pm=1,
+    seeded noise 1e-4 and the dipole seed): 10 device-controlled steps (one HIP
+    graph each) in FP64 and in mixed precision on the FP64 run's step sizes;
+    energies within 1e-6 relative (the mixed-precision target), every field
+    finite, and the same dt sequence and rebuild count."""
+    import math
+    import torch
+    from magic_amd.dynamo import DynamoRun
+    cfg = dict(l_max=383, n_r_max=97, ek=1e-5, ra=2e8, pr=1.0, prmag=1.0, radratio=0.35,
+               dtmax=1e-4)
+    out = {}
+    sched = None
+    for precision in ("fp64", "mixed"):
+        run = DynamoRun(dict(cfg, precision=precision))
+        run.dt_schedule = sched
+        run.use_graphs = True
+        run.initialize_fields(noise_amp=1e-4, noise_seed=20261202)
+        run.setup_initial_state()
+        run.initialize_dt(cfg["dtmax"])
+        run.run_steps(10, run.dtMax)
+        for k in ("s", "w", "z", "b", "aj"):
+            assert torch.isfinite(torch.view_as_real(run.f[k])).all(), (precision, k)
+        out[precision] = (run.energies(), list(run.dt_log), run.n_rebuild)
+        sched = list(run.dt_log)
+        run.close()
+    (e64, dt64, nr64), (emx, dtmx, nrmx) = out["fp64"], out["mixed"]
+    assert dt64 == dtmx and nr64 == nrmx
+    for a, b in zip(e64, emx):
+        assert math.isfinite(b) and abs(b - a) <= 1e-6 * abs(a), (a, b)
