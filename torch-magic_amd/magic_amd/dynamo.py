@@ -97,6 +97,12 @@ def seeded_noise(p: Params, res: Resolution, seed: int, amp: float) -> np.ndarra
     parts for all lm, then imaginary parts) in m-major lm order; the radial
     shape vanishes at both boundaries so the fixed-entropy rows hold.
     """
+    coef, shape = noise_factors(p, res, seed, amp)
+    return coef[:, None] * shape[None, :]
+
+
+def noise_factors(p: Params, res: Resolution, seed: int, amp: float):
+    """The seeded noise as its two factors: (coefficient per lm, radial shape)."""
     rng = np.random.default_rng(seed)
     L = p.lm_max
     coef = amp * (rng.standard_normal(L) + 1j * rng.standard_normal(L))
@@ -105,7 +111,7 @@ def seeded_noise(p: Params, res: Resolution, seed: int, amp: float) -> np.ndarra
     coef[l == 0] = 0.0
     r = res.rad["r"]
     x = 2.0 * r - res.rad["r_cmb"] - res.rad["r_icb"]
-    return coef[:, None] * ((1.0 - x * x) ** 3)[None, :]
+    return coef, (1.0 - x * x) ** 3
 
 
 def conduction_state(p: Params, res: Resolution):
@@ -663,34 +669,33 @@ class DynamoRun:
     # ------------------------------------------------------------ init
     def initialize_fields(self, noise_amp: float | None = None, noise_seed: int | None = None):
         """``initialize_fields`` (reference ``init_fields.py:470-532``) + optional noise."""
-        p, r = self.p, self.res
-        L, N = self.L, self.N
-        for t in self.f.values():
+        p, r, f = self.p, self.res, self.f
+        for t in f.values():
             t.zero_()
-        s = np.zeros((L, N), np.complex128)
-        pr = np.zeros((L, N), np.complex128)
-        b = np.zeros((L, N), np.complex128)
-        aj = np.zeros((L, N), np.complex128)
+        up = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=C128).to(self.device)
         rr, r_cmb, r_icb = r.rad["r"], r.rad["r_cmb"], r.rad["r_icb"]
         lm = lambda l, m: int(r.maps["lm_start"][m] + (l - m))
-        if p.l_heat:
-            s0, p0 = conduction_state(p, r)
-            s[0], pr[0] = s0, p0
-            if p.init_s1 >= 100:
-                l, m = p.init_s1 // 100, p.init_s1 % 100
-                x = 2.0 * rr - r_cmb - r_icb
-                s[lm(l, m)] += p.amp_s1 * (1.0 - 3.0 * x ** 2 + 3.0 * x ** 4 - x ** 6)
-        if p.l_mag:
-            b_pol = p.amp_b1 * math.sqrt(3.0 * math.pi) / 4.0
-            b_tor = -4.0 / 3.0 * p.amp_b1 * math.sqrt(math.pi / 5.0)
-            b[lm(1, 0)] = b_pol * (rr ** 3 - 4.0 / 3.0 * r_cmb * rr ** 2 + r_icb ** 4 / (3.0 * rr))
-            aj[lm(2, 0)] = b_tor * rr * np.sin(math.pi * (rr - r_icb))
+        # the fields are zero but for a few rows and the rank-one seeded noise,
+        # so only those go to the device (no (L, N) host arrays): s = rows + noise
         amp = p.noise_amp if noise_amp is None else noise_amp
         seed = p.noise_seed if noise_seed is None else noise_seed
         if amp:
-            s = s + seeded_noise(p, r, seed, amp)
-        for k, v in (("s", s), ("p", pr), ("b", b), ("aj", aj)):
-            self.f[k].copy_(torch.from_numpy(v))
+            coef, shape = noise_factors(p, r, seed, amp)
+            f["s"].add_(up(coef)[:, None] * up(shape)[None, :])
+        if p.l_heat:
+            s0, p0 = conduction_state(p, r)
+            f["s"][0].add_(up(s0))
+            f["p"][0].copy_(up(p0))
+            if p.init_s1 >= 100:
+                l, m = p.init_s1 // 100, p.init_s1 % 100
+                x = 2.0 * rr - r_cmb - r_icb
+                f["s"][lm(l, m)].add_(up(p.amp_s1 * (1.0 - 3.0 * x ** 2 + 3.0 * x ** 4 - x ** 6)))
+        if p.l_mag:
+            b_pol = p.amp_b1 * math.sqrt(3.0 * math.pi) / 4.0
+            b_tor = -4.0 / 3.0 * p.amp_b1 * math.sqrt(math.pi / 5.0)
+            f["b"][lm(1, 0)].copy_(up(b_pol * (rr ** 3 - 4.0 / 3.0 * r_cmb * rr ** 2
+                                               + r_icb ** 4 / (3.0 * rr))))
+            f["aj"][lm(2, 0)].copy_(up(b_tor * rr * np.sin(math.pi * (rr - r_icb))))
 
     def setup_initial_state(self):
         """Derivatives, old states and implicit terms from the fields (``step_time.py:398-617``)."""
