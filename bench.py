@@ -138,7 +138,8 @@ def run_case(precision, steps, warmup, dt_schedule=None, section_steps=20):
     dt = run.ctl_end()
     out = dict(elapsed=t1 - t0, energies=run.energies(), spectra=run.spectra(), dt=dt,
                rebuilds=run.n_rebuild, p=run.p, N=run.N, grid_rb=run.grid_rb,
-               syn_waves=run.syn_waves, ana_waves=run.ana_waves)
+               syn_shape=(run.syn_vec_shape, run.syn_scal_shape), ana_waves=run.ana_waves,
+               rowop_wide=run.rowop_wide)
     run.enable_timers(True)
     run.run_steps(section_steps, dt)
     out["secs"] = run.timer_summary()
@@ -344,7 +345,10 @@ def main():
            "ms_per_step": {pr: 1e3 * r["elapsed"] / args.steps for pr, r in res.items()},
            "dt": h["dt"], "rebuilds": {pr: r["rebuilds"] for pr, r in res.items()},
            "grid_levels_per_workgroup": {pr: r["grid_rb"] or "plan default" for pr, r in res.items()},
-           "synthesis_waves_per_workgroup": {pr: r["syn_waves"] or "default" for pr, r in res.items()},
+           "synthesis_waves_and_column_tiles": {pr: {"vector": list(r["syn_shape"][0]),
+                                                     "scalar": list(r["syn_shape"][1])}
+                                                for pr, r in res.items()},
+           "row_operator_tile": {pr: "32x128" if r["rowop_wide"] else "32x64" for pr, r in res.items()},
            "analysis_waves_per_workgroup": {pr: r["ana_waves"] or "default" for pr, r in res.items()},
            "energies": {pr: r["energies"] for pr, r in res.items()},
            "section_ms": breakdown}

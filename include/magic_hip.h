@@ -59,6 +59,8 @@ typedef struct {
   int prec;  /* 0: FP64 tables and MFMA; 1: FP32 tables (synP/synD hold floats), FP32 MFMA */
   int fprec; /* 0: F holds doubles; 1: F holds floats (FP32 Fourier buffers, needs prec = 1) */
   int waves; /* waves per workgroup (16 theta pairs each): 0 = default (4), 4 or 6 */
+  int nct;   /* column tiles of 8 complex columns per workgroup: 0/1, or 2 (FP32 vector only) */
+  int pad;
 } MhSynArgs;
 
 /* Legendre synthesis on FP64 MFMA: scalar (vector=0) replaces the bmm part of

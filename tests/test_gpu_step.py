@@ -315,9 +315,10 @@ def test_grid_level_blocks_give_identical_results(l_max, precision):
 
 @pytest.mark.parametrize("l_max,prec", [(32, 0), (32, 1), (85, 0), (383, 0), (383, 1)])
 def test_synthesis_workgroup_sizes_give_identical_results(l_max, prec):
-    """The vector and scalar Legendre synthesis with 4 and 6 waves per workgroup
-    (the tuner's candidates) write the same Fourier coefficients bit for bit,
-    FP64 and FP32 (FP32 tables, FP32 Fourier buffer)."""
+    """The vector and scalar Legendre synthesis with 4 and 6 waves per workgroup,
+    and for FP32 vector synthesis one or two column tiles per workgroup (the
+    tuner's candidates), write the same Fourier coefficients bit for bit, FP64
+    and FP32 (FP32 tables, FP32 Fourier buffer)."""
     import torch
     from magic_amd.device import Resolution
     from magic_amd.params import Params
@@ -331,15 +332,18 @@ def test_synthesis_workgroup_sizes_give_identical_results(l_max, prec):
     fdt = torch.float32 if prec else torch.float64
     groups = [[(xs[0], None, 1), (xs[1], None, 0), (xs[2], xs[3], 2)] for _ in range(4)]
     outs = {}
-    for w in (4, 6):
+    shapes = [(4, 1), (6, 1)] + ([(4, 2), (6, 2)] if prec else [])
+    for w, nct in shapes:
         Fv = res.fourier_buffer(3, 4 * N, fdt).zero_()
         Fs = res.fourier_buffer(1, N, fdt).zero_()
-        res.synth(groups, N, N, Fv, 4 * N, vector=True, prec=prec, waves=w)
+        res.synth(groups, N, N, Fv, 4 * N, vector=True, prec=prec, waves=w, nct=nct)
         res.synth([[(xs[0], None, 0)]], N, N, Fs, N, vector=False, prec=prec, waves=w)
         torch.cuda.synchronize()
-        outs[w] = (Fv, Fs)
-    assert torch.isfinite(outs[4][0]).all() and outs[4][0].abs().max() > 0
-    assert torch.equal(outs[4][0], outs[6][0]) and torch.equal(outs[4][1], outs[6][1])
+        outs[(w, nct)] = (Fv, Fs)
+    ref = outs[(4, 1)]
+    assert torch.isfinite(ref[0]).all() and ref[0].abs().max() > 0
+    for key, (Fv, Fs) in outs.items():
+        assert torch.equal(Fv, ref[0]) and torch.equal(Fs, ref[1]), key
 
 
 @pytest.mark.parametrize("l_max,prec", [(32, 0), (32, 1), (85, 0), (383, 0), (383, 1)])

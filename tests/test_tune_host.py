@@ -40,3 +40,11 @@ def test_cache_file_round_trip(tmp_path, monkeypatch):
     assert tune._read_file() == data
     path.write_text("not json")
     assert tune._read_file() == {}
+
+
+def test_tuple_choices_survive_the_json_cache(tmp_path, monkeypatch):
+    monkeypatch.setenv("MAGIC_AMD_TUNE_CACHE", str(tmp_path / "t.json"))
+    tune._write_file("syn", dict(choice=[4, 2], ms={}))
+    calls = []
+    assert tune.timed_choice("syn", [(4, 1), (6, 1), (4, 2)], calls.append, (4, 1)) == (4, 2)
+    assert calls == [] and tune._MEM.pop("syn") == (4, 2)

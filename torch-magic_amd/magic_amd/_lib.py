@@ -32,7 +32,7 @@ class SynIn(C.Structure):
 class SynArgs(C.Structure):
     _fields_ = [("in_", (SynIn * 3) * 4), ("ngroups", I), ("B", I), ("ldx", I), ("H", I),
                 ("Hp", I), ("mp_info", P), ("lm_start", P), ("synP", P), ("synD", P),
-                ("or2", P), ("F", P), ("c_stride", L), ("m_stride", L), ("Bout", I), ("prec", I), ("fprec", I), ("waves", I)]
+                ("or2", P), ("F", P), ("c_stride", L), ("m_stride", L), ("Bout", I), ("prec", I), ("fprec", I), ("waves", I), ("nct", I), ("pad", I)]
 
 
 class AnaArgs(C.Structure):

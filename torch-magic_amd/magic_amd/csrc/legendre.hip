@@ -140,8 +140,8 @@ __device__ __forceinline__ SynIn syn_in(const SynArgs& a, int g, int c) {
 constexpr int kSynWavesDefault = 4, kChunk = 32, kSynDepth = 4;
 
 // NCT column tiles of 8 complex columns per workgroup: every table operand
-// feeds NCT MFMAs (the launches use one tile; two, which FP32 accumulators
-// allow, measured no faster on MI355X).
+// feeds NCT MFMAs.  Two are instantiated for the FP32 vector synthesis (its
+// 20 accumulators fit; FP64 would not), chosen by the tuner.
 template <bool VEC, typename T, int NCT, typename FT, int WAVES>
 __global__ __launch_bounds__(64 * WAVES, 3) void k_syn(SynArgs a, int n_m) {
   FT* const Fo = reinterpret_cast<FT*>(a.F);  // Fourier buffer elements (double or float)
@@ -362,7 +362,9 @@ extern "C" int mh_leg_synth(int vector, const MhSynArgs* args, int n_m, void* st
     for (int f = 0; f < (vector ? 3 : 1); ++f)
       if (a.in[g][f].mode < 0 || a.in[g][f].mode > 2 || (a.in[g][f].mode == 2 && !(vector && f == 2)))
         return MH_EINVAL;
-  const int nct = 1;  // column tiles per workgroup (two measured no faster for FP32)
+  // column tiles per workgroup: one, or two for the FP32 vector synthesis
+  const int nct = a.nct == 0 ? 1 : a.nct;
+  if (nct != 1 && (nct != 2 || a.prec != 1 || !vector)) return MH_EINVAL;
   const int waves = a.waves == 0 ? kSynWavesDefault : a.waves;
   if (waves != 4 && waves != 6) return MH_EINVAL;
   const long units =
@@ -372,7 +374,10 @@ extern "C" int mh_leg_synth(int vector, const MhSynArgs* args, int n_m, void* st
   const dim3 blk(64 * waves);
   hipStream_t st = mh_stream(stream);
 #define MH_SYN_LAUNCH(W)                                                                       \
-  if (a.prec == 0) {                                                                           \
+  if (nct == 2) {                                                                              \
+    if (a.fprec == 0) hipLaunchKernelGGL((k_syn<true, float, 2, double, W>), grid, blk, 0, st, a, n_m); \
+    else hipLaunchKernelGGL((k_syn<true, float, 2, float, W>), grid, blk, 0, st, a, n_m);       \
+  } else if (a.prec == 0) {                                                                    \
     if (vector) hipLaunchKernelGGL((k_syn<true, double, 1, double, W>), grid, blk, 0, st, a, n_m); \
     else hipLaunchKernelGGL((k_syn<false, double, 1, double, W>), grid, blk, 0, st, a, n_m);   \
   } else if (a.fprec == 0) {                                                                   \

@@ -111,11 +111,12 @@ class Resolution:
         ms = 2 * self.T * Bout
         return self.M * ms, ms
 
-    def synth(self, groups, B, ldx, F, Bout, vector: bool, or2=None, prec=0, waves=0):
+    def synth(self, groups, B, ldx, F, Bout, vector: bool, or2=None, prec=0, waves=0, nct=0):
         """Legendre synthesis.  groups: list of [(x1, x2, mode)] * (3 if vector else 1).
 
         prec 0: FP64 MFMA; 1: FP32 tables and MFMA with FP32 accumulation.
-        waves: waves per workgroup (0: the library default; 4 or 6), same results."""
+        waves: waves per workgroup (0: the library default; 4 or 6); nct: column
+        tiles per workgroup (0/1, or 2 for the FP32 vector synthesis); same results."""
         a = _lib.SynArgs()
         for g, grp in enumerate(groups):
             for c, (x1, x2, mode) in enumerate(grp):
@@ -135,7 +136,7 @@ class Resolution:
         a.F = ptr(F)
         a.c_stride, a.m_stride = self.fstrides(Bout)
         a.Bout = Bout
-        a.waves = int(waves)
+        a.waves, a.nct = int(waves), int(nct)
         _lib.call("mh_leg_synth", 1 if vector else 0, _lib.ref(a), self.M, stream_handle())
 
     def analysis(self, F, Bin, B, comps, outS, outT, ldo, vector: bool, prec=0, waves=0):

@@ -257,7 +257,9 @@ class DynamoRun:
         if fused_grid:
             self.grid_rb = tune.grid_level_block(r, N, p, fdt,
                                                  bufs=(self.F_s, self.F_v, self.F_o, self.cfl))
-        self.syn_waves = tune.synth_waves(r, N, self.leg_prec, fdt, bufs=(self.f["s"], self.F_v))
+        self.syn_vec_shape = tune.synth_shape(r, N, self.leg_prec, fdt, bufs=(self.f["s"], self.F_v))
+        self.syn_scal_shape = tune.synth_shape(r, N, self.leg_prec, fdt, bufs=(self.f["s"], self.F_s),
+                                               vector=False)
         self.ana_waves = tune.analysis_waves(r, N, self.leg_prec, fdt,
                                              bufs=(self.F_o, self.nl["Advr"])) if fused_grid else 0
         yield
@@ -745,14 +747,14 @@ class DynamoRun:
         # 1. Legendre synthesis
         with self._t("leg_syn_scal"):
             r.synth([[(f["s"], None, 0)]], N, N, self.F_s, N, vector=False, prec=self.leg_prec,
-                    waves=self.syn_waves)
+                    waves=self.syn_scal_shape[0], nct=self.syn_scal_shape[1])
         groups = [[(f["b"], None, 1), (f["db"], None, 0), (f["aj"], None, 0)],
                   [(f["aj"], None, 1), (f["dj"], None, 0), (f["ddb"], f["b"], 2)],
                   [(f["w"], None, 1), (f["dw"], None, 0), (f["z"], None, 0)],
                   [(f["z"], None, 1), (f["dz"], None, 0), (f["ddw"], f["w"], 2)]]
         with self._t("leg_syn_vec"):
             r.synth(groups, N, N, self.F_v, 4 * N, vector=True, prec=self.leg_prec,
-                    waves=self.syn_waves)
+                    waves=self.syn_vec_shape[0], nct=self.syn_vec_shape[1])
         d = r.d
         if self.fused_grid:
             # 2-4. fused inverse FFT + nonlinear products + Courant + forward FFT

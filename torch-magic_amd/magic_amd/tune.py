@@ -7,7 +7,8 @@ the same results bit for bit:
 * the fused grid step (``mh_grid_step``): radial levels per workgroup, 1, 2,
   ... up to the most its FFT plan fits (``mh_grid_plan``);
 * the Legendre synthesis (``mh_leg_synth``): waves per workgroup, 4 or 6
-  (16 theta pairs each);
+  (16 theta pairs each), and for the FP32 vector synthesis one or two column
+  tiles per workgroup; the vector and the scalar synthesis are tuned apart;
 * the Legendre analysis (``mh_leg_analysis``): waves per workgroup, 6 or 4
   (16 degrees each);
 * the row operator (``mh_rowop``): tiles of 32 rows x 64 or x 128 columns.
@@ -80,9 +81,11 @@ def timed_choice(key: str, candidates, launch, default, reps: int = 3):
         if key in _MEM:
             return _MEM[key]
         hit = _read_file().get(key)
-        if hit is not None and hit.get("choice") in candidates:
-            _MEM[key] = hit["choice"]
-            return _MEM[key]
+        choice = hit.get("choice") if hit is not None else None
+        choice = tuple(choice) if isinstance(choice, list) else choice  # JSON lists
+        if choice is not None and choice in candidates:
+            _MEM[key] = choice
+            return choice
     if len(candidates) == 1:
         best = candidates[0]
     else:
@@ -107,29 +110,34 @@ def _device_name(res) -> str:
     return torch.cuda.get_device_name(res.device).replace("|", "/")
 
 
-def synth_waves(res, N: int, prec: int, fdtype=torch.float64, bufs=None) -> int:
-    """Waves per workgroup for the vector synthesis of 4 groups of N levels
-    (the dynamo step's), FP64 (prec 0) or FP32 tables (prec 1) writing Fourier
-    buffers of `fdtype`.  bufs: (a spectral (L, N) array, the F buffer)."""
+def synth_shape(res, N: int, prec: int, fdtype=torch.float64, bufs=None,
+                vector: bool = True) -> tuple:
+    """(waves, column tiles) per workgroup for the dynamo step's synthesis:
+    vector (4 groups of N levels) or scalar (one field of N levels), FP64
+    (prec 0) or FP32 tables (prec 1) writing Fourier buffers of `fdtype`.
+    bufs: (a spectral (L, N) array, the F buffer)."""
     if os.environ.get("MAGIC_AMD_TUNE", "1") == "0":
-        return 0
+        return (0, 0)
     fprec = 1 if fdtype == torch.float32 else 0
-    key = (f"{_device_name(res)}|leg_synth_vec|L={res.L}|H={res.H}|M={res.M}|B={4 * N}"
-           f"|prec={prec}|fprec={fprec}")
+    ng = 4 if vector else 1
+    key = (f"{_device_name(res)}|leg_synth_{'vec' if vector else 'scal'}|L={res.L}|H={res.H}"
+           f"|M={res.M}|B={ng * N}|prec={prec}|fprec={fprec}")
+    cands = [(4, 1), (6, 1)] + ([(4, 2), (6, 2)] if (vector and prec == 1) else [])
     buf = {}
 
-    def launch(w):  # buffers made on first use (none on a cache hit)
+    def launch(c):  # buffers made on first use (none on a cache hit)
         if not buf:
             if bufs is None:
                 buf["x"] = torch.zeros(res.L, N, dtype=torch.complex128, device=res.device)
-                buf["F"] = res.fourier_buffer(3, 4 * N, fdtype)
+                buf["F"] = res.fourier_buffer(3 if vector else 1, ng * N, fdtype)
             else:
                 buf["x"], buf["F"] = bufs
         x = buf["x"]
-        groups = [[(x, None, 1), (x, None, 0), (x, x, 2)] for _ in range(4)]
-        res.synth(groups, N, N, buf["F"], 4 * N, vector=True, prec=prec, waves=w)
+        groups = ([[(x, None, 1), (x, None, 0), (x, x, 2)] for _ in range(4)] if vector
+                  else [[(x, None, 0)]])
+        res.synth(groups, N, N, buf["F"], ng * N, vector=vector, prec=prec, waves=c[0], nct=c[1])
 
-    return timed_choice(key, [4, 6], launch, 4)
+    return timed_choice(key, cands, launch, (4, 1))
 
 
 def analysis_waves(res, N: int, prec: int, fdtype=torch.float64, bufs=None) -> int:
