@@ -74,9 +74,11 @@ def fastest(times: dict, default):
     return min(times, key=lambda c: (times[c], c != default))
 
 
-def timed_choice(key: str, candidates, launch, default, reps: int = 3):
+def timed_choice(key: str, candidates, launch, default, reps: int = 3, trials: int = 3):
     """The fastest of `candidates` for `key` (cached): launch(c) issues one run
-    of candidate c on the current stream.  Ties keep `default`."""
+    of candidate c on the current stream.  Each candidate's time is the least
+    of `trials` interleaved measurements of `reps` runs (a transient slowdown
+    cannot decide the choice).  Ties keep `default`."""
     with _LOCK:
         if key in _MEM:
             return _MEM[key]
@@ -89,16 +91,19 @@ def timed_choice(key: str, candidates, launch, default, reps: int = 3):
     if len(candidates) == 1:
         best = candidates[0]
     else:
-        times = {}
         for c in candidates:
-            launch(c)  # warm-up
-            ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
-            ev[0].record()
-            for _ in range(reps):
-                launch(c)
-            ev[1].record()
-            ev[1].synchronize()
-            times[c] = ev[0].elapsed_time(ev[1]) / reps
+            launch(c)  # warm-up (code object load, first-touch)
+        times = {}
+        for _ in range(trials):
+            for c in candidates:
+                ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+                ev[0].record()
+                for _ in range(reps):
+                    launch(c)
+                ev[1].record()
+                ev[1].synchronize()
+                t = ev[0].elapsed_time(ev[1]) / reps
+                times[c] = min(t, times.get(c, t))
         best = fastest(times, default)
         _write_file(key, dict(choice=best, ms={str(k): round(v, 4) for k, v in times.items()}))
     with _LOCK:
@@ -107,7 +112,11 @@ def timed_choice(key: str, candidates, launch, default, reps: int = 3):
 
 
 def _device_name(res) -> str:
-    return torch.cuda.get_device_name(res.device).replace("|", "/")
+    """The device in a cache key: architecture and compute-unit count (stable
+    whatever name the runtime reports)."""
+    pr = torch.cuda.get_device_properties(res.device)
+    arch = getattr(pr, "gcnArchName", "") or pr.name
+    return f"{arch.split(':')[0]}x{pr.multi_processor_count}".replace("|", "/")
 
 
 def synth_shape(res, N: int, prec: int, fdtype=torch.float64, bufs=None,
