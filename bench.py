@@ -49,12 +49,12 @@ NOISE = dict(seed=20261202, amp=1e-4)
 # rocprofv3 FETCH_SIZE / WRITE_SIZE passes scaled by a 256 MiB copy (FETCH x 2,
 # WRITE x 1); see profiles/hbm_traffic_l383_nr97_pmc.md
 PMC_TRAFFIC_BYTES = {
-    ("grid_step", "mixed"): 2.441e9 + 3.679e9,
-    ("grid_step", "fp64"): 5.846e9 + 4.151e9,
-    ("leg_syn_vec", "mixed"): 4.495e9 + 2.158e9,
-    ("leg_syn_vec", "fp64"): 5.716e9 + 4.191e9,
-    ("leg_ana_vec", "mixed"): 1.360e9 + 0.599e9,
-    ("leg_ana_vec", "fp64"): 7.730e9 + 0.620e9,
+    ("grid_step", "mixed"): 2.442e9 + 3.722e9,
+    ("grid_step", "fp64"): 5.847e9 + 4.146e9,
+    ("leg_syn_vec", "mixed"): 4.184e9 + 2.191e9,
+    ("leg_syn_vec", "fp64"): 5.710e9 + 4.192e9,
+    ("leg_ana_vec", "mixed"): 1.432e9 + 0.599e9,
+    ("leg_ana_vec", "fp64"): 7.715e9 + 0.621e9,
 }
 
 PEAK_FP64_TFLOPS = 78.6
