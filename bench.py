@@ -281,6 +281,50 @@ def ensemble_case(n_seq):
                 sequential=dict(seq, sample=f"first {n_seq} of the 64 requests back to back"))
 
 
+def measured_peaks():
+    """Peaks measured on this device, beside the published ones: HBM from a
+    2 GiB device-to-device copy (bytes read + written), FP64 and FP32 matrix
+    rates from 8192^3 library GEMMs (torch.matmul, TF32 off); best of the
+    repetitions, timed with HIP events on the current stream."""
+    import torch
+    dev = torch.device("cuda")
+
+    def best(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t = float("inf")
+        for _ in range(reps):
+            e0.record()
+            fn()
+            e1.record()
+            e1.synchronize()
+            t = min(t, e0.elapsed_time(e1) * 1e-3)
+        return t
+
+    out = {}
+    n = (2 << 30) // 8
+    a = torch.ones(n, dtype=torch.float64, device=dev)
+    b = torch.empty_like(a)
+    out["hbm_copy_GBs"] = 2 * n * 8 / best(lambda: b.copy_(a), 10) / 1e9
+    del a, b
+    tf32 = torch.backends.cuda.matmul.allow_tf32
+    torch.backends.cuda.matmul.allow_tf32 = False
+    m = 8192
+    for name, dt in (("fp64", torch.float64), ("fp32", torch.float32)):
+        g = torch.Generator(device=dev).manual_seed(1)
+        A = torch.randn(m, m, dtype=dt, device=dev, generator=g)
+        B = torch.randn(m, m, dtype=dt, device=dev, generator=g)
+        C = torch.empty(m, m, dtype=dt, device=dev)
+        out[f"{name}_gemm_TFLOPs"] = 2 * m ** 3 / best(lambda: torch.matmul(A, B, out=C), 5) / 1e12
+        del A, B, C
+    torch.backends.cuda.matmul.allow_tf32 = tf32
+    torch.cuda.empty_cache()
+    out["method"] = ("2 GiB float64 copy_ (read + write bytes); 8192^3 torch.matmul in FP64 and "
+                     "FP32 (TF32 off); best of 10 / 5 repetitions")
+    return out
+
+
 def roofline(res, precision):
     """Roofline of the dominant kernel section (largest total time)."""
     secs, work = res["secs"], section_work(res["p"], res["N"], 8.0 if precision == "mixed" else 16.0)
@@ -310,6 +354,8 @@ def main():
     ap.add_argument("--cpu-levels", type=int, default=9)
     ap.add_argument("--no-ensemble", action="store_true", help="skip the config-5 ensemble")
     ap.add_argument("--no-sht", action="store_true", help="skip the config-2 SHT round trip")
+    ap.add_argument("--no-peaks", action="store_true",
+                    help="skip the measured HBM / GEMM peaks")
     ap.add_argument("--ensemble-seq", type=int, default=16,
                     help="requests in the sequential ensemble baseline sample")
     args = ap.parse_args()
@@ -373,6 +419,15 @@ def main():
         cfg["ensemble"] = ensemble_case(args.ensemble_seq)
         print(f"bench: ensemble wall {cfg['ensemble']['server']['wall_s']:.1f} s", file=sys.stderr,
               flush=True)
+    rl = roofline(h, head)
+    if not args.no_peaks:
+        pk = measured_peaks()
+        cfg["measured_peaks"] = pk
+        meas = {"GB/s": pk["hbm_copy_GBs"],
+                "TFLOP/s": pk["fp32_gemm_TFLOPs" if head == "mixed" and rl["kernel"].startswith("leg_")
+                              else "fp64_gemm_TFLOPs"]}[rl["unit"]]
+        rl["frac_of_measured_peak"] = rl["achieved"] / meas
+        print("bench: measured peaks done", file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(args.cpu_levels)
@@ -391,7 +446,7 @@ def main():
         "dtype": "f32+f64" if head == "mixed" else "f64",
         "data": "synthetic (seeded noise + dipole seed, generated on device)",
         "config": cfg,
-        "roofline": roofline(h, head),
+        "roofline": rl,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

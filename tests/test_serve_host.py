@@ -49,7 +49,7 @@ def test_row_job_packing_matches_struct_layout():
                   ldy=33, rseg=33),
              dict(X=0x5000, ldx=33, rows=0, nrows=561, MT=0x6000, K=33, R=33, Rpad=64,
                   Y=[0x40], ldy=33, rseg=33)]
-    arr, tiles = pack_row_jobs(specs)
+    arr, tiles = pack_row_jobs(specs, n_xcd=1)
     ref, ref_tiles = [], []
     for sp in specs:
         for i in range(np.size(sp["nrows"])):
@@ -69,3 +69,34 @@ def test_row_job_packing_matches_struct_layout():
                 ref_tiles.append((ji, r0, c0))
     assert tiles.tolist() == [list(t) for t in ref_tiles]
     assert C.sizeof(_lib.RowJob) == arr.dtype.itemsize
+    # the default order is a permutation of the same tiles
+    arr8, tiles8 = pack_row_jobs(specs)
+    assert arr8.tobytes() == raw
+    assert sorted(map(tuple, tiles8.tolist())) == sorted(ref_tiles)
+
+
+def test_xcd_tile_order_keeps_each_job_on_one_xcd():
+    """Workgroup i runs on XCD i mod 8: every tile of an unsplit job gets the
+    same i mod 8, a job larger than 1/64 of the tiles is cut at row-chunk
+    boundaries, and the eight XCDs get equal tile counts up to the largest
+    unit."""
+    from magic_amd.dynamo import xcd_tile_order
+    ls = np.arange(384)
+    # per-degree solves at l_max=383 (4 fields, 3 column tiles per 32-row chunk)
+    per = np.repeat(3 * (-(-(ls + 1) // 32)), 4)
+    nc = np.full(per.size, 3)
+    o = xcd_tile_order(per, nc)
+    assert sorted(o.tolist()) == list(range(int(per.sum())))
+    job = np.repeat(np.arange(per.size), per)[o]
+    pos = np.arange(o.size)
+    for j in range(per.size):
+        assert np.unique(pos[job == j] % 8).size == 1
+    counts = np.bincount(pos % 8, minlength=8)
+    assert counts.max() - counts.min() <= 1
+    # two derivative jobs over all 73 920 rows: cut into pieces of 72 tiles
+    per = np.array([2310, 2310])
+    o = xcd_tile_order(per, np.array([1, 1]))
+    assert sorted(o.tolist()) == list(range(4620))
+    first = o[np.arange(0, o.size, 8)]          # XCD 0's tiles in dispatch order
+    runs = np.split(first, np.nonzero(np.diff(first) != 1)[0] + 1)
+    assert all(r.size <= 72 for r in runs) and sum(r.size for r in runs) == first.size

@@ -64,12 +64,17 @@ def _rowjob_dtype():
 _ROWJOB_DTYPE = _rowjob_dtype()
 
 
-def pack_row_jobs(specs, tile_cols=64):
+def pack_row_jobs(specs, tile_cols=64, n_xcd=8):
     """Pack row-operator job columns into MhRowJob records and build the tile
     list (job, first row, first column): rows in chunks of 32, operator columns
-    in chunks of `tile_cols` (64, or 128 for the wide kernel), in job order.
-    Each spec is a dict of MhRowJob fields whose rows / nrows / MT entries may
-    be arrays (one job per element)."""
+    in chunks of `tile_cols` (64, or 128 for the wide kernel).  Each spec is a
+    dict of MhRowJob fields whose rows / nrows / MT entries may be arrays (one
+    job per element).
+
+    The hardware hands workgroup i to XCD i mod 8, and each XCD has its own L2.
+    So that the operator of a job is read from HBM by one XCD only (and its
+    right-hand-side rows by one XCD only), the tiles are ordered XCD by XCD
+    (`xcd_tile_order`); `n_xcd=1` keeps plain job order."""
     n = [int(np.size(sp["nrows"])) for sp in specs]
     tot = sum(n)
     arr = np.zeros(tot, dtype=_ROWJOB_DTYPE)
@@ -87,7 +92,45 @@ def pack_row_jobs(specs, tile_cols=64):
     q = np.arange(per.sum()) - np.repeat(np.cumsum(per) - per, per)
     ncj = np.repeat(nc, per)
     tiles = np.stack([ji, 32 * (q // ncj), tile_cols * (q % ncj)], axis=1).astype(np.int32)
+    if n_xcd > 1 and len(tiles):
+        tiles = tiles[xcd_tile_order(per, nc, n_xcd)]
     return arr, tiles
+
+
+def xcd_tile_order(per, nc, n_xcd=8):
+    """Permutation of a job-ordered tile list (job j owns `per[j]` consecutive
+    tiles, `nc[j]` per 32-row chunk) that gives each unit of work to one XCD.
+
+    Units are whole jobs, except that a job holding more than 1/64 of all tiles
+    is cut at row-chunk boundaries into pieces of about that size.  Units are
+    dealt largest first to the least-loaded of `n_xcd` queues (ties: the lower
+    queue), each unit's tiles staying consecutive in its queue; the queues are
+    then interleaved, position i of the result taking the next tile of queue
+    i mod n_xcd while every queue has tiles left.  The few tiles left when the
+    queues' lengths differ follow at the end, in queue order."""
+    per = np.asarray(per, np.int64)
+    nc = np.asarray(nc, np.int64)
+    total = int(per.sum())
+    start = np.cumsum(per) - per
+    cap = max(1, total // 64)
+    units = []  # (size, first tile, job)
+    for j in np.nonzero(per)[0]:
+        step = max(1, cap // int(nc[j])) * int(nc[j])
+        for s0 in range(0, int(per[j]), step):
+            units.append((min(step, int(per[j]) - s0), int(start[j]) + s0))
+    order = sorted(range(len(units)), key=lambda u: (-units[u][0], u))
+    load = [0] * n_xcd
+    queues = [[] for _ in range(n_xcd)]
+    for u in order:
+        x = min(range(n_xcd), key=lambda i: (load[i], i))
+        size, s0 = units[u]
+        queues[x].append(np.arange(s0, s0 + size))
+        load[x] += size
+    qs = [np.concatenate(qx) if qx else np.zeros(0, np.int64) for qx in queues]
+    m = min(len(qx) for qx in qs)
+    head = np.stack([qx[:m] for qx in qs], axis=1).reshape(-1)
+    tail = [qx[m:] for qx in qs]
+    return np.concatenate([head] + tail).astype(np.int64)
 
 
 def seeded_noise(p: Params, res: Resolution, seed: int, amp: float) -> np.ndarray:
