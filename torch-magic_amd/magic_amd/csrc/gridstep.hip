@@ -71,6 +71,11 @@ extern "C" int mh_grid_step(const MhGridArgs* args, void* stream) {
   if (a.Mh % 2 != 0 || (long)P.threads * ((P.RL + 2) / 2) < (long)(a.Mh / 2 + 1) * P.RB)
     return MH_EINVAL;
   if (a.fprec != 0 && a.fprec != 1) return MH_EINVAL;
+  // the kernel forms buffer offsets in 32 bits with 24-bit multiplies: one
+  // order's stride (T*4N*2 elements in Fv, the widest) below 2^24 and each
+  // component (M orders) below 2^31 elements
+  if ((long)a.T * 4 * a.N * 2 >= (1L << 24) || (long)a.M * a.T * 4 * a.N * 2 >= (1L << 31))
+    return MH_EINVAL;
   const size_t esz = a.fprec ? sizeof(float2) : sizeof(double2);
   const size_t lds = (2ull * P.RB * (a.Mh + 1) + 2ull * a.Mh) * esz;
   const int nRB = (a.N + P.RB - 1) / P.RB;

@@ -186,7 +186,11 @@ __device__ __forceinline__ int fwd_mid_ct(const Geo& g, const GridArgs& a, int w
 // Global loads of one component's spectrum for the workgroup's levels, by
 // conjugate pairs: item e = p*RB + j (pair p = 0 .. Mh/2, level j) holds X_p
 // and X_{Mh-p} (zero where the order is >= M or the level is past the end),
-// read from base + ((m*T + t)*Bst + b0 + j)*2 elements.
+// read from base + ((m*T + t)*Bst + b0 + j)*2 elements.  The offset is formed
+// in 32 bits as m*(T*Bst*2) + ((t*Bst + b0 + j)*2) with a 24-bit multiply
+// (full rate, where a 64-bit product costs four quarter-rate instructions);
+// the launcher checks T*Bst*2 < 2^24 and that a component holds fewer than
+// 2^31 elements, which makes both exact.
 template <int RL, typename T>
 struct Prefetch {
   static constexpr int PP = (RL + 2) / 2;  // items per thread (RL = 2, 3, 4)
@@ -198,9 +202,12 @@ __device__ __forceinline__ void fetch(const Geo& g, const GridArgs& a, Prefetch<
                                       const T* base, int Bst, int b0, int nvalid, int t) {
   typedef typename vec2<T>::type V;
   const int RB = g.RB, n = (g.Mh / 2 + 1) * RB;
+  const unsigned S = (unsigned)(a.T * Bst * 2);
+  const unsigned tb = (unsigned)((t * Bst + b0) * 2);
   auto ld = [&](int m, int j) {
     return (m < g.M && j < nvalid)
-               ? *reinterpret_cast<const V*>(base + (((long)m * a.T + t) * Bst + b0 + j) * 2)
+               ? *reinterpret_cast<const V*>(base + ((unsigned)__umul24((unsigned)m, S) + tb +
+                                                     2u * (unsigned)j))
                : mk2((T)0, (T)0);
   };
 #pragma unroll
@@ -215,11 +222,18 @@ __device__ __forceinline__ void fetch(const Geo& g, const GridArgs& a, Prefetch<
   }
 }
 
+struct NoFetch {
+  __device__ void operator()() const {}
+};
+
 // Inverse transform of the prefetched component; returns this thread's RL
-// complex grid values (x_{2p} + i x_{2p+1} at p = q + i*TP).
-template <int MH, int RL, typename T>
+// complex grid values (x_{2p} + i x_{2p+1} at p = q + i*TP).  `next` issues
+// the loads of the following component once the prefetched values are in
+// LDS, so that they are in flight during this transform's passes.
+template <int MH, int RL, typename T, typename F = NoFetch>
 __device__ __forceinline__ void inverse(Ctx& c, const Geo& g, const GridArgs& a,
-                                        const Prefetch<RL, T>& pf, typename vec2<T>::type (&z)[RL]) {
+                                        const Prefetch<RL, T>& pf, typename vec2<T>::type (&z)[RL],
+                                        F next = F()) {
   typedef typename vec2<T>::type V;
   const int RB = g.RB, LS = g.LS;
   int w = 1 - c.busy;
@@ -246,6 +260,7 @@ __device__ __forceinline__ void inverse(Ctx& c, const Geo& g, const GridArgs& a,
       }
     }
   }
+  next();
   lds_barrier();
   if (g.nst > 1) {
     if constexpr (MH > 0) {
@@ -320,6 +335,9 @@ __device__ __forceinline__ void forward(Ctx& c, const Geo& g, const GridArgs& a,
   const int RB = g.RB, Mh = g.Mh, N = a.N, No = N - 2, n = (Mh / 2 + 1) * RB;
   const V* tw2 = ltw2<T>(g);
   T* Fo = reinterpret_cast<T*>(a.Fo) + (long)comp * g.M * a.T * No * 2;
+  // 32-bit offsets as in fetch(): p*(T*No*2) + (t*No + r - 1)*2
+  const unsigned So = (unsigned)(a.T * No * 2);
+  const unsigned to = (unsigned)((t * No + r0 - 1) * 2);
   const V* res = lbuf<T>(g, w);
   for (int e = threadIdx.x; e < n; e += g.threads) {
     const int p = e / RB, j = e - p * RB;
@@ -333,11 +351,12 @@ __device__ __forceinline__ void forward(Ctx& c, const Geo& g, const GridArgs& a,
     const V D = csub(Z, Zc);
     const V O = mk2((T)0.5 * D.y, (T)-0.5 * D.x);
     const V Tm = cmul(tw2[p], O);
+    const unsigned oj = to + 2u * (unsigned)j;
     if (p < g.M)
-      *reinterpret_cast<V*>(Fo + (((long)p * a.T + t) * No + r - 1) * 2) =
+      *reinterpret_cast<V*>(Fo + ((unsigned)__umul24((unsigned)p, So) + oj)) =
           cscale(cadd(E, Tm), scale);
     if (p > 0 && 2 * p < Mh && pc < g.M)
-      *reinterpret_cast<V*>(Fo + (((long)pc * a.T + t) * No + r - 1) * 2) =
+      *reinterpret_cast<V*>(Fo + ((unsigned)__umul24((unsigned)pc, So) + oj)) =
           cscale(cconj(csub(E, Tm)), scale);
   }
   c.busy = w;
@@ -394,20 +413,20 @@ __global__ __launch_bounds__(kMaxThreads, 3) void k_grid_step(GridArgs a, GPlan 
   const bool mag = a.l_mag != 0;
 
   Prefetch<RL, T> pf;
+  // loads of a component, issued by inverse() after it has packed the last one
+  auto next = [&](const T* base, int Bst, int b0) {
+    return [&, base, Bst, b0]() { fetch<MH, RL, T>(g, a, pf, base, Bst, b0, nvalid, t); };
+  };
   V vr[RL], vt[RL], vp[RL], tmp[RL];
   const T inv_n = (T)(1.0 / (2.0 * g.Mh));
   // 1. u (group 2)
   fetch<MH, RL, T>(g, a, pf, Fv, Bv, 2 * N + r0, nvalid, t);
-  inverse<MH, RL, T>(c, g, a, pf, vr);
-  fetch<MH, RL, T>(g, a, pf, Fv + vcs, Bv, 2 * N + r0, nvalid, t);
-  inverse<MH, RL, T>(c, g, a, pf, vt);
-  fetch<MH, RL, T>(g, a, pf, Fv + 2 * vcs, Bv, 2 * N + r0, nvalid, t);
-  inverse<MH, RL, T>(c, g, a, pf, vp);
+  inverse<MH, RL, T>(c, g, a, pf, vr, next(Fv + vcs, Bv, 2 * N + r0));
+  inverse<MH, RL, T>(c, g, a, pf, vt, next(Fv + 2 * vcs, Bv, 2 * N + r0));
+  inverse<MH, RL, T>(c, g, a, pf, vp, next(Fs, N, r0));
   // 2. s -> VSr, VSt, VSp
-  fetch<MH, RL, T>(g, a, pf, Fs, N, r0, nvalid, t);
-  inverse<MH, RL, T>(c, g, a, pf, tmp);
-  if (mag) fetch<MH, RL, T>(g, a, pf, Fv, Bv, r0, nvalid, t);  // B_r next
-  else fetch<MH, RL, T>(g, a, pf, Fv, Bv, 3 * N + r0, nvalid, t);  // curl u_r next
+  // B_r next, or curl u_r without a magnetic field
+  inverse<MH, RL, T>(c, g, a, pf, tmp, next(Fv, Bv, (mag ? 0 : 3 * N) + r0));
   {
     V o[RL];
 #pragma unroll
@@ -429,12 +448,9 @@ __global__ __launch_bounds__(kMaxThreads, 3) void k_grid_step(GridArgs a, GPlan 
   V br[RL], bt[RL], bp[RL];
   {
     if (mag) {
-      inverse<MH, RL, T>(c, g, a, pf, br);
-      fetch<MH, RL, T>(g, a, pf, Fv + vcs, Bv, r0, nvalid, t);
-      inverse<MH, RL, T>(c, g, a, pf, bt);
-      fetch<MH, RL, T>(g, a, pf, Fv + 2 * vcs, Bv, r0, nvalid, t);
-      inverse<MH, RL, T>(c, g, a, pf, bp);
-      fetch<MH, RL, T>(g, a, pf, Fv, Bv, 3 * N + r0, nvalid, t);  // curl u_r next
+      inverse<MH, RL, T>(c, g, a, pf, br, next(Fv + vcs, Bv, r0));
+      inverse<MH, RL, T>(c, g, a, pf, bt, next(Fv + 2 * vcs, Bv, r0));
+      inverse<MH, RL, T>(c, g, a, pf, bp, next(Fv, Bv, 3 * N + r0));  // curl u_r next
     } else {
 #pragma unroll
       for (int i = 0; i < RL; ++i) br[i] = bt[i] = bp[i] = mk2((T)0, (T)0);
@@ -497,16 +513,14 @@ __global__ __launch_bounds__(kMaxThreads, 3) void k_grid_step(GridArgs a, GPlan 
   //   Advt = -or4 (Cvp Vr - Cvr Vp)  + LF or4 (Cbp Br - Cbr Bp)
   //   Advp = -or4 (Cvr Vt - Cvt Vr)  + LF or4 (Cbr Bt - Cbt Br)
   V ar[RL], at[RL], ap[RL];
-  inverse<MH, RL, T>(c, g, a, pf, tmp);  // curl u_r
-  fetch<MH, RL, T>(g, a, pf, Fv + vcs, Bv, 3 * N + r0, nvalid, t);
+  inverse<MH, RL, T>(c, g, a, pf, tmp, next(Fv + vcs, Bv, 3 * N + r0));  // curl u_r
 #pragma unroll
   for (int i = 0; i < RL; ++i) {
     ar[i] = mk2((T)0, (T)0);
     at[i] = mk2(or4 * tmp[i].x * vp[i].x, or4 * tmp[i].y * vp[i].y);
     ap[i] = mk2(-or4 * tmp[i].x * vt[i].x, -or4 * tmp[i].y * vt[i].y);
   }
-  inverse<MH, RL, T>(c, g, a, pf, tmp);  // curl u_t
-  fetch<MH, RL, T>(g, a, pf, Fv + 2 * vcs, Bv, 3 * N + r0, nvalid, t);
+  inverse<MH, RL, T>(c, g, a, pf, tmp, next(Fv + 2 * vcs, Bv, 3 * N + r0));  // curl u_t
 #pragma unroll
   for (int i = 0; i < RL; ++i) {
     ar[i].x -= ost2 * tmp[i].x * vp[i].x;
@@ -514,11 +528,10 @@ __global__ __launch_bounds__(kMaxThreads, 3) void k_grid_step(GridArgs a, GPlan 
     ap[i].x += or4 * tmp[i].x * vr[i].x;
     ap[i].y += or4 * tmp[i].y * vr[i].y;
   }
-  inverse<MH, RL, T>(c, g, a, pf, tmp);  // curl u_p
-  if (mag) {
-    if (keepB) fetch<MH, RL, T>(g, a, pf, Fv, Bv, N + r0, nvalid, t);  // curl B_r next
-    else fetch<MH, RL, T>(g, a, pf, Fv, Bv, r0, nvalid, t);            // B_r next
-  }
+  // curl u_p; then curl B_r (B kept in registers) or B_r (B transformed again)
+  inverse<MH, RL, T>(c, g, a, pf, tmp, [&]() {
+    if (mag) fetch<MH, RL, T>(g, a, pf, Fv, Bv, (keepB ? N : 0) + r0, nvalid, t);
+  });
 #pragma unroll
   for (int i = 0; i < RL; ++i) {
     ar[i].x += ost2 * tmp[i].x * vt[i].x;
@@ -529,15 +542,11 @@ __global__ __launch_bounds__(kMaxThreads, 3) void k_grid_step(GridArgs a, GPlan 
   if (mag) {
     const T lr = LF * ost2, lt = LF * or4;
     if (!keepB) {
-      inverse<MH, RL, T>(c, g, a, pf, br);
-      fetch<MH, RL, T>(g, a, pf, Fv + vcs, Bv, r0, nvalid, t);
-      inverse<MH, RL, T>(c, g, a, pf, bt);
-      fetch<MH, RL, T>(g, a, pf, Fv + 2 * vcs, Bv, r0, nvalid, t);
-      inverse<MH, RL, T>(c, g, a, pf, bp);
-      fetch<MH, RL, T>(g, a, pf, Fv, Bv, N + r0, nvalid, t);
+      inverse<MH, RL, T>(c, g, a, pf, br, next(Fv + vcs, Bv, r0));
+      inverse<MH, RL, T>(c, g, a, pf, bt, next(Fv + 2 * vcs, Bv, r0));
+      inverse<MH, RL, T>(c, g, a, pf, bp, next(Fv, Bv, N + r0));
     }
-    inverse<MH, RL, T>(c, g, a, pf, tmp);  // curl B_r
-    fetch<MH, RL, T>(g, a, pf, Fv + vcs, Bv, N + r0, nvalid, t);
+    inverse<MH, RL, T>(c, g, a, pf, tmp, next(Fv + vcs, Bv, N + r0));  // curl B_r
 #pragma unroll
     for (int i = 0; i < RL; ++i) {
       at[i].x -= lt * tmp[i].x * bp[i].x;
@@ -545,8 +554,7 @@ __global__ __launch_bounds__(kMaxThreads, 3) void k_grid_step(GridArgs a, GPlan 
       ap[i].x += lt * tmp[i].x * bt[i].x;
       ap[i].y += lt * tmp[i].y * bt[i].y;
     }
-    inverse<MH, RL, T>(c, g, a, pf, tmp);  // curl B_t
-    fetch<MH, RL, T>(g, a, pf, Fv + 2 * vcs, Bv, N + r0, nvalid, t);
+    inverse<MH, RL, T>(c, g, a, pf, tmp, next(Fv + 2 * vcs, Bv, N + r0));  // curl B_t
 #pragma unroll
     for (int i = 0; i < RL; ++i) {
       ar[i].x += lr * tmp[i].x * bp[i].x;
