@@ -47,14 +47,17 @@ NOISE = dict(seed=20261202, amp=1e-4)
 
 # HBM bytes per launch (read + written) of the main kernels at this workload, from
 # rocprofv3 FETCH_SIZE / WRITE_SIZE passes scaled by a 256 MiB copy (FETCH x 2,
-# WRITE x 1); see profiles/hbm_traffic_l383_nr97_pmc.md
+# WRITE x 1); see profiles/hbm_traffic_l383_nr97_pmc.md.  Grid-step entries are
+# per radial levels per workgroup (the tuner's choice): the FP32 one-level figure
+# is from a pass with one level per workgroup (same loads and stores per level).
 PMC_TRAFFIC_BYTES = {
-    ("grid_step", "mixed"): 2.442e9 + 3.722e9,
-    ("grid_step", "fp64"): 5.847e9 + 4.146e9,
-    ("leg_syn_vec", "mixed"): 4.184e9 + 2.191e9,
-    ("leg_syn_vec", "fp64"): 5.710e9 + 4.192e9,
-    ("leg_ana_vec", "mixed"): 1.432e9 + 0.599e9,
-    ("leg_ana_vec", "fp64"): 7.715e9 + 0.621e9,
+    ("grid_step", "mixed", 1): 2.442e9 + 3.722e9,
+    ("grid_step", "mixed", 2): 2.374e9 + 2.013e9,
+    ("grid_step", "fp64", 1): 5.881e9 + 4.619e9,
+    ("leg_syn_vec", "mixed"): 4.194e9 + 2.192e9,
+    ("leg_syn_vec", "fp64"): 5.713e9 + 4.189e9,
+    ("leg_ana_vec", "mixed"): 1.414e9 + 0.599e9,
+    ("leg_ana_vec", "fp64"): 7.751e9 + 0.621e9,
 }
 
 PEAK_FP64_TFLOPS = 78.6
@@ -330,16 +333,17 @@ def roofline(res, precision):
     secs, work = res["secs"], section_work(res["p"], res["N"], 8.0 if precision == "mixed" else 16.0)
     dom = max(secs, key=lambda k: secs[k][0] * secs[k][1])
     cnt, ms = secs[dom]
+    tkey = (dom, precision, res["grid_rb"]) if dom == "grid_step" else (dom, precision)
     kind, amount = work.get(dom, ("hbm", float("nan")))
     if kind == "mfma":
         peak = PEAK_FP32_TFLOPS if (precision == "mixed" and dom.startswith("leg_")) else PEAK_FP64_TFLOPS
         ach = amount / (ms * 1e-3) / 1e12
         return dict(bound="mfma", achieved=ach, peak=peak, unit="TFLOP/s", frac=ach / peak,
-                    traffic=PMC_TRAFFIC_BYTES.get((dom, precision)), kernel=dom, launch_ms=ms,
+                    traffic=PMC_TRAFFIC_BYTES.get(tkey), kernel=dom, launch_ms=ms,
                     algorithmic_flops=amount)
     ach = amount / (ms * 1e-3) / 1e9
     return dict(bound="hbm", achieved=ach, peak=PEAK_HBM_GBS, unit="GB/s", frac=ach / PEAK_HBM_GBS,
-                traffic=PMC_TRAFFIC_BYTES.get((dom, precision)), kernel=dom, launch_ms=ms,
+                traffic=PMC_TRAFFIC_BYTES.get(tkey), kernel=dom, launch_ms=ms,
                 algorithmic_bytes=amount)
 
 
