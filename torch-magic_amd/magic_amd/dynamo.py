@@ -1066,6 +1066,17 @@ class DynamoRun:
         if self._ctl["queued"] - self._ctl["pulled"] >= self.LOG_CAP:
             self._ctl_pull()
 
+    def save_checkpoint(self, path, step: int | None = None, sim_time: float | None = None):
+        """Restart file in the reference's .pt layout (``main.py:189-223``)."""
+        from .checkpoint import save_checkpoint
+        save_checkpoint(self, path, step, sim_time)
+
+    def load_checkpoint(self, path):
+        """Restore from a restart file (``main.py:226-267``); returns (step, sim_time).
+        Continue with one_step(step + 1, self.dt_hist[0]) or run_steps."""
+        from .checkpoint import load_checkpoint
+        return load_checkpoint(self, path)
+
     def ctl_begin(self, dt: float):
         """Start device-controlled stepping; dt: the dt passed to the first
         step's decision, as for one_step."""
