@@ -64,6 +64,11 @@ def test_restart_continues_bit_for_bit(tmp_path, precision):
     for k in ("s", "ds", "z", "dz", "w", "dw", "ddw", "p", "dp", "b", "db", "ddb", "aj", "dj"):
         assert torch.equal(a.f[k], ref.f[k]), k
     assert a.energies() == ref.energies()
+    # the time arrays too (the reference's restart test compares them slot by slot)
+    for k in ("s", "z", "w", "p", "b", "j"):
+        assert torch.equal(a.old[k], ref.old[k]), ("old", k)
+        assert torch.equal(a.impl[k], ref.impl[k]), ("impl", k)
+        assert torch.equal(a.expl_prev[k], ref.expl_prev[k]), ("expl", k)
 
     # device-controlled continuation (one HIP graph per step)
     b = _fresh(precision)
