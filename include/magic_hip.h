@@ -281,7 +281,11 @@ typedef struct {
 
 /* Kinetic / magnetic poloidal and toroidal energy per degree (by_m=0) or per
  * order (by_m=1), radially integrated: get_e_kin_full, get_e_mag_oc_full
- * (output.py:111-193) and the l/m aggregation of get_dlm (output.py:750-861). */
+ * (output.py:111-193) and the l/m aggregation of get_dlm (output.py:750-861);
+ * out[q*(l_max+1) + i], q = kin pol, kin tor, mag pol, mag tor.  by_m=2: per
+ * degree and split into bin = 2*(m == 0) + ((l + m) odd), out[(4q + bin)*(l_max+1)
+ * + l] (16 rows), from which the axisymmetric and equatorially symmetric parts
+ * of output.py:128-137,172-180 are sums. */
 int mh_energy(const MhEnergyArgs* args, void* stream);
 
 /* ------------------------------------------------------------ rebuild */

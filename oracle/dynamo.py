@@ -17,7 +17,9 @@ it in module globals: ``fields.py:14-67``, ``dt_fields.py:55-123``).
 * ``one_step`` — ``_one_step_cnab2`` (``step_time.py:1602-1637``) with
   ``dt_courant`` (``courant.py:130-155``) and ``CNAB2.set_weights``
   (``time_scheme.py:56-73``).
-* energies — ``get_e_kin_full``/``get_e_mag_oc_full`` (``output.py:111-193``).
+* energies — ``get_e_kin_full``/``get_e_mag_oc_full`` (``output.py:111-193``);
+  ``energy_components`` adds the masked parts (pinned by single-mode fields
+  whose energy the (l, m) parity places in known columns).
 
 Pinned by ``tests/test_oracle_step.py`` against the committed fixtures in
 ``tests/golden/`` written by ``tests/golden/make_golden.py`` from the
@@ -330,6 +332,33 @@ class OracleRun:
         emp = f * dLh * (dLh * or2 * self._cc2real(self.b) + self._cc2real(self.db))
         emt = f * dLh * self._cc2real(self.aj)
         return tuple(float(e.sum(axis=0) @ q) for e in (ekp, ekt, emp, emt))
+
+    def energy_components(self):
+        """The 8 e_kin.TAG and 12 e_mag_oc.TAG components (``output.py:111-193``):
+        per-mode radial profiles integrated with the radial weights, summed under
+        the reference's masks (m = 0; (l+m) even / odd; m = 0 with l even / odd),
+        plus the outside-shell poloidal magnetic energy from the CMB values."""
+        dLh = self.lmarr["dLh"][:, None]
+        or2 = self.rad["or2"][None]
+        q = self.q
+        l = self.maps["lm2l"]; m = self.maps["lm2m"]
+        m0 = m == 0
+        lpm_even = (l + m) % 2 == 0
+        f = 0.5 * self.prm.LFfac
+        ekp = (0.5 * dLh * (dLh * or2 * self._cc2real(self.w) + self._cc2real(self.dw))) @ q
+        ekt = (0.5 * dLh * self._cc2real(self.z)) @ q
+        emp = (f * dLh * (dLh * or2 * self._cc2real(self.b) + self._cc2real(self.db))) @ q
+        emt = (f * dLh * self._cc2real(self.aj)) @ q
+        s = lambda e, mask=None: float(e.sum() if mask is None else e[mask].sum())
+        ekin = (s(ekp), s(ekt), s(ekp, m0), s(ekt, m0), s(ekp, lpm_even), s(ekt, ~lpm_even),
+                s(ekp, m0 & (l % 2 == 0)), s(ekt, m0 & (l % 2 == 1)))
+        bc = self.b[:, 0]
+        cc = np.where(m0, bc.real ** 2, 2.0 * (bc.real ** 2 + bc.imag ** 2))
+        os_ = f / self.rad["r_cmb"] * l.astype(float) ** 2 * (l + 1.0) * cc
+        emag = (s(emp), s(emt), s(emp, m0), s(emt, m0), float(os_.sum()), float(os_[m0].sum()),
+                s(emp, ~lpm_even), s(emt, lpm_even), s(emp, m0 & (l % 2 == 1)),
+                s(emt, m0 & (l % 2 == 0)), 0.0, 0.0)
+        return ekin, emag
 
     def spectra(self):
         """Per-l and per-m kinetic/magnetic energy spectra (``output.py:750-861`` aggregation)."""

@@ -1,0 +1,89 @@
+"""Pins of the oracle's energy components (reference ``output.py:111-193``):
+a field holding a single (l, m) mode puts its energy into the columns that the
+parity of l+m and m = 0 fix, the totals equal the pinned ``energies()``, and the
+outside-shell term of a pure dipole has its closed form."""
+
+import numpy as np
+import pytest
+
+from oracle.dynamo import OracleRun
+
+CFG = dict(l_max=8, n_r_max=17, ek=1e-3, ra=1e5, pr=1.0, prmag=5.0, radratio=0.35)
+
+
+def _single(orc, name, l, m, value=0.3 + 0.2j):
+    for k in ("w", "dw", "z", "b", "db", "aj"):
+        getattr(orc, k)[:] = 0.0
+    lm = int(np.nonzero((orc.maps["lm2l"] == l) & (orc.maps["lm2m"] == m))[0][0])
+    x = orc.rad["x"] if "x" in orc.rad else np.linspace(-1, 1, orc.prm.n_r_max)
+    prof = 1.0 - 0.5 * np.asarray(x) ** 2
+    getattr(orc, name)[lm, :] = (value.real if m == 0 else value) * prof
+    return orc.energy_components()
+
+
+@pytest.fixture(scope="module")
+def orc():
+    return OracleRun(dict(CFG))
+
+
+# (field, l, m, expected nonzero EKin / EMagOC columns besides the total)
+CASES = [
+    ("w", 3, 1, "kin", 0, {4}),        # poloidal, l+m even: equatorially symmetric
+    ("w", 2, 0, "kin", 0, {2, 4, 6}),  # m=0, l even: axisymmetric, ES and its m=0 part
+    ("w", 3, 0, "kin", 0, {2}),        # m=0, l odd: axisymmetric only
+    ("z", 2, 1, "kin", 1, {5}),        # toroidal, l+m odd: ES
+    ("z", 1, 0, "kin", 1, {3, 5, 7}),  # m=0, l odd
+    ("z", 2, 0, "kin", 1, {3}),
+    ("b", 1, 0, "mag", 0, {2, 4, 5, 6, 8}),  # dipole: l+m odd is magnetic ES; CMB terms
+    ("b", 2, 1, "mag", 0, {4, 6}),     # l+m odd: magnetic ES; an outside-shell term (m>0)
+    ("aj", 2, 0, "mag", 1, {3, 7, 9}),  # toroidal m=0, l even
+    ("aj", 3, 1, "mag", 1, {7}),        # toroidal l+m even: ES
+]
+
+
+@pytest.mark.parametrize("name,l,m,kind,tot,cols", CASES)
+def test_single_mode_lands_in_its_columns(orc, name, l, m, kind, tot, cols):
+    ekin, emag = _single(orc, name, l, m)
+    e = ekin if kind == "kin" else emag
+    other = emag if kind == "kin" else ekin
+    assert e[tot] > 0
+    expect = set(cols) | {tot}
+    # the magnetic outside-shell pair (columns 4, 5) is a CMB value, not a share of the total
+    cmb = {4, 5} if kind == "mag" else set()
+    for i, v in enumerate(e):
+        if i in expect and i in cmb:
+            assert v > 0, (i, e)
+        elif i in expect:
+            assert v == pytest.approx(e[tot], rel=1e-14), (i, e)
+        else:
+            assert v == 0.0, (i, e)
+    assert all(v == 0.0 for v in other)
+    # totals are the (pinned) energies()
+    en = orc.energies()
+    assert ekin[0] == pytest.approx(en[0], rel=1e-14, abs=0) or en[0] == 0
+    assert emag[1] == pytest.approx(en[3], rel=1e-14, abs=0) or en[3] == 0
+
+
+def test_dipole_outside_shell_energy_closed_form(orc):
+    b0 = 0.7
+    for k in ("w", "dw", "z", "b", "db", "aj"):
+        getattr(orc, k)[:] = 0.0
+    lm = int(np.nonzero((orc.maps["lm2l"] == 1) & (orc.maps["lm2m"] == 0))[0][0])
+    orc.b[lm, 0] = b0   # radial index 0 is the CMB
+    _, emag = orc.energy_components()
+    expect = 0.5 * orc.prm.LFfac / orc.rad["r_cmb"] * 1.0 * 2.0 * b0 * b0
+    assert emag[4] == pytest.approx(expect, rel=1e-14)
+    assert emag[5] == pytest.approx(expect, rel=1e-14)
+
+
+def test_log_lines_have_the_fortran_widths():
+    import io
+    from magic_amd import output
+    f = io.StringIO()
+    output.write_e_kin_line(f, 0.125, output.EKin(*range(1, 9)))
+    output.write_e_mag_oc_line(f, 0.125, output.EMagOC(*range(1, 13)))
+    output.write_timestep_line(f, 0.125, 1e-4)
+    a, b, c = f.getvalue().splitlines()
+    assert len(a) == 20 + 8 * 16 and len(b) == 20 + 12 * 16 and len(c) == 36
+    assert a.startswith("  1.250000000000E-01  1.00000000E+00")
+    assert c == "  1.250000000000E-01  1.00000000E-04"

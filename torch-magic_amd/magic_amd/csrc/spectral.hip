@@ -217,39 +217,73 @@ __device__ __forceinline__ double cc2real(double2 c, bool m0) {
 __global__ __launch_bounds__(256) void k_energy(EnergyArgs a) {
   const int i = blockIdx.x;  // l or m
   const int N = a.N;
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  // by_m 2: per degree, each quantity split into four parts by
+  // bin = 2*(m == 0) + ((l + m) odd), out[(4*quantity + bin)*(l_max+1) + l]
+  const bool parts = a.by_m == 2;
+  double acc[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) acc[q] = 0.0;
   // work items: (m, r) for a fixed l, or (l, r) for a fixed m
-  const int nitem = (a.by_m ? (a.l_max - i + 1) : (i + 1)) * N;
+  const int nitem = (a.by_m == 1 ? (a.l_max - i + 1) : (i + 1)) * N;
   for (int it = threadIdx.x; it < nitem; it += blockDim.x) {
     const int q = it / N, r = it - q * N;
     int l, m;
-    if (a.by_m) { m = i; l = i + q; } else { l = i; m = q; }
+    if (a.by_m == 1) { m = i; l = i + q; } else { l = i; m = q; }
     const long lm = a.lm_start[m] + (l - m);
     const long idx = lm * N + r;
     const bool m0 = m == 0;
     const double dLh = (double)l * (double)(l + 1);
     const double wq = a.qw[r], or2 = a.or2[r];
-    acc[0] += wq * 0.5 * dLh * (dLh * or2 * cc2real(C2(a.w)[idx], m0) + cc2real(C2(a.dw)[idx], m0));
-    acc[1] += wq * 0.5 * dLh * cc2real(C2(a.z)[idx], m0);
-    if (a.l_mag) {
-      acc[2] += wq * 0.5 * a.LFfac * dLh * (dLh * or2 * cc2real(C2(a.b)[idx], m0) + cc2real(C2(a.db)[idx], m0));
-      acc[3] += wq * 0.5 * a.LFfac * dLh * cc2real(C2(a.aj)[idx], m0);
+    if (!parts) {
+      acc[0] += wq * 0.5 * dLh * (dLh * or2 * cc2real(C2(a.w)[idx], m0) + cc2real(C2(a.dw)[idx], m0));
+      acc[1] += wq * 0.5 * dLh * cc2real(C2(a.z)[idx], m0);
+      if (a.l_mag) {
+        acc[2] += wq * 0.5 * a.LFfac * dLh * (dLh * or2 * cc2real(C2(a.b)[idx], m0) + cc2real(C2(a.db)[idx], m0));
+        acc[3] += wq * 0.5 * a.LFfac * dLh * cc2real(C2(a.aj)[idx], m0);
+      }
+    } else {
+      double e[4];
+      e[0] = wq * 0.5 * dLh * (dLh * or2 * cc2real(C2(a.w)[idx], m0) + cc2real(C2(a.dw)[idx], m0));
+      e[1] = wq * 0.5 * dLh * cc2real(C2(a.z)[idx], m0);
+      e[2] = e[3] = 0.0;
+      if (a.l_mag) {
+        e[2] = wq * 0.5 * a.LFfac * dLh * (dLh * or2 * cc2real(C2(a.b)[idx], m0) + cc2real(C2(a.db)[idx], m0));
+        e[3] = wq * 0.5 * a.LFfac * dLh * cc2real(C2(a.aj)[idx], m0);
+      }
+      const int bin = 2 * (m0 ? 1 : 0) + ((l + m) & 1);
+      // selects, not an indexed register array (which would go to scratch)
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int bb = 0; bb < 4; ++bb) acc[4 * k + bb] += bb == bin ? e[k] : 0.0;
     }
   }
   __shared__ double sh[4][256];
-  for (int q = 0; q < 4; ++q) sh[q][threadIdx.x] = acc[q];
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (threadIdx.x < s)
-      for (int q = 0; q < 4; ++q) sh[q][threadIdx.x] += sh[q][threadIdx.x + s];
+  const int npass = parts ? 4 : 1;
+  for (int pass = 0; pass < npass; ++pass) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      double v = acc[q];
+#pragma unroll
+      for (int pp = 1; pp < 4; ++pp)
+        if (pass == pp) v = acc[4 * pp + q];
+      sh[q][threadIdx.x] = v;
+    }
     __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+      if (threadIdx.x < s)
+        for (int q = 0; q < 4; ++q) sh[q][threadIdx.x] += sh[q][threadIdx.x + s];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0)
+      for (int q = 0; q < 4; ++q) a.out[(4 * pass + q) * (a.l_max + 1) + i] = sh[q][0];
+    __syncthreads();  // the next pass overwrites sh
   }
-  if (threadIdx.x == 0)
-    for (int q = 0; q < 4; ++q) a.out[q * (a.l_max + 1) + i] = sh[q][0];
 }
 
 extern "C" int mh_energy(const MhEnergyArgs* args, void* stream) {
   const EnergyArgs& a = *args;
+  if (a.by_m < 0 || a.by_m > 2 || a.l_max < 0 || a.N < 1) return MH_EINVAL;
   hipLaunchKernelGGL(k_energy, dim3(a.l_max + 1), dim3(256), 0, mh_stream(stream), a);
   return mh_launch_status();
 }

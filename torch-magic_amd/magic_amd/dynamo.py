@@ -1101,8 +1101,8 @@ class DynamoRun:
         return self.ctl_end()
 
     # ------------------------------------------------------------ diagnostics
-    def _energy_out(self, by_m: bool):
-        out = torch.zeros(4, self.p.l_max + 1, dtype=F64, device=self.device)
+    def _energy_out(self, by_m: int):
+        out = torch.zeros(16 if by_m == 2 else 4, self.p.l_max + 1, dtype=F64, device=self.device)
         a = _lib.EnergyArgs()
         f = self.f
         a.w, a.dw, a.z, a.b, a.db, a.aj = (ptr(f[k]) for k in ("w", "dw", "z", "b", "db", "aj"))
@@ -1114,13 +1114,40 @@ class DynamoRun:
 
     def energies(self):
         """(e_kin_pol, e_kin_tor, e_mag_pol, e_mag_tor) (``output.py:111-193``)."""
-        e = self._energy_out(False).cpu().numpy()
+        e = self._energy_out(0).cpu().numpy()
         return tuple(float(x) for x in e.sum(axis=1))
+
+    def energy_components(self):
+        """All components of the reference's e_kin.TAG (8) and e_mag_oc.TAG (12)
+        lines (``output.py:111-193``): totals, axisymmetric (m = 0), equatorially
+        symmetric ((l+m) even for kinetic poloidal and magnetic toroidal, odd for
+        the other two), their m = 0 parts, and the poloidal magnetic energy of the
+        potential field outside the shell from the CMB values.  Returns
+        (ekin, emag) as tuples in the reference's column order."""
+        e = self._energy_out(2).cpu().numpy().reshape(4, 4, -1).sum(axis=2)  # [q][bin]
+        # bins: 0 m>0 (l+m) even, 1 m>0 odd, 2 m=0 l even, 3 m=0 l odd
+        tot = e.sum(axis=1)
+        as_ = e[:, 2] + e[:, 3]
+        even, odd = e[:, 0] + e[:, 2], e[:, 1] + e[:, 3]
+        ekin = (tot[0], tot[1], as_[0], as_[1], even[0], odd[1], e[0, 2], e[1, 3])
+        if self.p.l_mag:
+            r, maps = self.res, self.res.maps
+            l = np.asarray(maps["l"], dtype=np.float64)
+            m0 = np.asarray(maps["m"]) == 0
+            bc = self.f["b"][:, 0].cpu().numpy()  # radial index 0 is the CMB
+            cc = np.where(m0, bc.real ** 2, 2.0 * (bc.real ** 2 + bc.imag ** 2))
+            w = 0.5 * self.p.LFfac / r.rad["r_cmb"] * l * l * (l + 1.0) * cc
+            e_os, e_as_os = float(w.sum()), float(w[m0].sum())
+            emag = (tot[2], tot[3], as_[2], as_[3], e_os, e_as_os, odd[2], even[3], e[2, 3],
+                    e[3, 2], 0.0, 0.0)
+        else:
+            emag = (0.0,) * 12
+        return tuple(float(x) for x in ekin), tuple(float(x) for x in emag)
 
     def spectra(self):
         """Per-degree and per-order kinetic and magnetic energy spectra."""
-        el = self._energy_out(False).cpu().numpy()
-        em = self._energy_out(True).cpu().numpy()
+        el = self._energy_out(0).cpu().numpy()
+        em = self._energy_out(1).cpu().numpy()
         return dict(kin_l=el[0] + el[1], mag_l=el[2] + el[3], kin_m=em[0] + em[1],
                     mag_m=em[2] + em[3])
 
