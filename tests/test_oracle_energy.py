@@ -87,3 +87,20 @@ def test_log_lines_have_the_fortran_widths():
     assert len(a) == 20 + 8 * 16 and len(b) == 20 + 12 * 16 and len(c) == 36
     assert a.startswith("  1.250000000000E-01  1.00000000E+00")
     assert c == "  1.250000000000E-01  1.00000000E-04"
+
+
+def test_log_files_append_one_line_per_call(tmp_path):
+    import types
+    from magic_amd import output
+    run = types.SimpleNamespace(
+        res=types.SimpleNamespace(rad={"r": np.linspace(1.5, 0.5, 5)}),
+        p=types.SimpleNamespace(l_mag=True),
+        energy_components=lambda: (tuple(range(8)), tuple(range(12))))
+    logs = output.LogFiles(run, str(tmp_path), "t1")
+    logs.log(0.0, 1e-4)
+    logs.log(1e-4, 1e-4)
+    assert len((tmp_path / "e_kin.t1").read_text().splitlines()) == 2
+    assert len((tmp_path / "e_mag_oc.t1").read_text().splitlines()) == 2
+    assert (tmp_path / "signal.t1").read_text() == "NOT\n"
+    rad = (tmp_path / "radius.t1").read_text().splitlines()
+    assert len(rad) == 5 and rad[0] == "   1  1.50000000E+00"

@@ -70,3 +70,28 @@ def write_signal_file(path):
     """signal.TAG: the word NOT (no signal pending)."""
     with open(path, "w") as f:
         f.write("NOT\n")
+
+
+class LogFiles:
+    """The e_kin / e_mag_oc / timestep log files of one run, ``<name>.<tag>`` in
+    `directory`, with radius.TAG and signal.TAG written at creation (the files
+    the reference's main loop writes every log step).  ``log(time, dt)`` appends
+    one line to each from the run's current state."""
+
+    def __init__(self, run, directory, tag):
+        import os
+        self.run = run
+        self.paths = {n: os.path.join(directory, f"{n}.{tag}")
+                      for n in ("e_kin", "e_mag_oc", "timestep")}
+        write_radius_file(os.path.join(directory, f"radius.{tag}"), run.res.rad["r"])
+        write_signal_file(os.path.join(directory, f"signal.{tag}"))
+
+    def log(self, time, dt):
+        ek, em = self.run.energy_components()
+        with open(self.paths["e_kin"], "a") as f:
+            write_e_kin_line(f, time, EKin(*ek))
+        if self.run.p.l_mag:
+            with open(self.paths["e_mag_oc"], "a") as f:
+                write_e_mag_oc_line(f, time, EMagOC(*em))
+        with open(self.paths["timestep"], "a") as f:
+            write_timestep_line(f, time, dt)
