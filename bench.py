@@ -26,7 +26,10 @@ against a sequential sample.  The CPU baseline is the FP64 numpy oracle on a
 bounded sample of the same workload.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision both|mixed|fp64]
-                       [--no-cpu-baseline] [--no-fd] [--no-ensemble] [--no-sht]
+                       [--no-cpu-baseline] [--no-fd] [--no-ensemble] [--no-sht] [--no-peaks]
+
+The measured peaks (a device copy, FP64 and FP32 GEMMs) are reported beside the
+published ones, and the roofline gives the dominant kernel's fraction of both.
 """
 
 import argparse
