@@ -77,7 +77,7 @@ extern "C" int mh_grid_step(const MhGridArgs* args, void* stream) {
   if ((long)a.T * 4 * a.N * 2 >= (1L << 24) || (long)a.M * a.T * 4 * a.N * 2 >= (1L << 31))
     return MH_EINVAL;
   const size_t esz = a.fprec ? sizeof(float2) : sizeof(double2);
-  const size_t lds = (2ull * P.RB * (a.Mh + 1) + 2ull * a.Mh) * esz;
+  const size_t lds = (2ull * P.RB * (a.Mh + 2) + 2ull * a.Mh) * esz;
   const int nRB = (a.N + P.RB - 1) / P.RB;
   const long nwg = (long)a.T * nRB;
   const long grid = ((nwg + 7) / 8) * 8;

@@ -16,7 +16,8 @@ constexpr int kLdsBytes = 160 * 1024 - 256;
 // FFT plan of the fused kernel for a half length Mh: radices in inverse order,
 // the last (RL) being the register pass; TP = Mh/RL threads per level; RB
 // levels per workgroup.  LDS: two buffers of RB rows of Mh+1 complex values,
-// then the twiddles exp(-2 pi i k/Mh) and exp(-2 pi i k/(2Mh)), k < Mh.
+// then the twiddles exp(-2 pi i k/Mh) and exp(-2 pi i k/(2Mh)), k < Mh.  Rows
+// are Mh+2 values long (even, so every row starts on 16 bytes in FP32).
 struct GPlan {
   int RL, TP, RB, nst, threads;
   int radix[16];
@@ -38,7 +39,7 @@ constexpr GPlan make_plan(int Mh) {
   p.TP = Mh / RL;
   for (int rb = 16; rb >= 1; --rb) {
     const int th = (rb * p.TP + 63) / 64 * 64;
-    if (th <= kMaxThreads && (2 * rb * (Mh + 1) + 2 * Mh) * 16 <= kLdsBytes) {
+    if (th <= kMaxThreads && (2 * rb * (Mh + 2) + 2 * Mh) * 16 <= kLdsBytes) {
       p.RB = rb;
       p.threads = th;
       break;
@@ -83,7 +84,7 @@ __device__ __forceinline__ Geo make_geo(const GridArgs& a, const GPlan& rp) {
     g.nst = rp.nst;
     g.threads = rp.threads;
   }
-  g.LS = g.Mh + 1;
+  g.LS = g.Mh + 2;
   return g;
 }
 
@@ -141,6 +142,16 @@ __device__ __forceinline__ void stage_mid(const Geo& g, const GridArgs& a, int w
     }
     dft<R>(v, dir);
     V* dst = out + row * g.LS + (bf / Ns) * Ns * R + k;
+    if constexpr (sizeof(T) == 4 && R % 2 == 0) {
+      if (Ns == 1) {
+        // consecutive outputs from an even element: 16-byte stores of pairs
+        // (half the store instructions at this pass's stride of R values)
+#pragma unroll
+        for (int r = 0; r < R; r += 2)
+          *reinterpret_cast<float4*>(dst + r) = make_float4(v[r].x, v[r].y, v[r + 1].x, v[r + 1].y);
+        continue;
+      }
+    }
 #pragma unroll
     for (int r = 0; r < R; ++r) dst[r * Ns] = v[r];
   }
