@@ -55,12 +55,12 @@ NOISE = dict(seed=20261202, amp=1e-4)
 # is from a pass with one level per workgroup (same loads and stores per level).
 PMC_TRAFFIC_BYTES = {
     ("grid_step", "mixed", 1): 2.442e9 + 3.722e9,
-    ("grid_step", "mixed", 2): 2.374e9 + 2.013e9,
-    ("grid_step", "fp64", 1): 5.881e9 + 4.619e9,
-    ("leg_syn_vec", "mixed"): 4.194e9 + 2.192e9,
-    ("leg_syn_vec", "fp64"): 5.713e9 + 4.189e9,
+    ("grid_step", "mixed", 2): 2.372e9 + 2.013e9,
+    ("grid_step", "fp64", 1): 5.892e9 + 4.642e9,
+    ("leg_syn_vec", "mixed"): 4.196e9 + 2.192e9,
+    ("leg_syn_vec", "fp64"): 5.718e9 + 4.190e9,
     ("leg_ana_vec", "mixed"): 1.414e9 + 0.599e9,
-    ("leg_ana_vec", "fp64"): 7.751e9 + 0.621e9,
+    ("leg_ana_vec", "fp64"): 7.759e9 + 0.620e9,
 }
 
 PEAK_FP64_TFLOPS = 78.6
