@@ -146,3 +146,26 @@ def test_library_exports_header_symbols():
     for name in decl:
         assert hasattr(lib, name), name
     _lib.load()   # struct sizes agree with the binding
+
+
+def test_grid_step_rejects_offsets_beyond_32_bits():
+    """mh_grid_step forms buffer offsets in 32 bits with 24-bit multiplies and
+    refuses, before any launch, shapes whose per-order stride (T*4N*2 elements)
+    reaches 2^24 or whose component reaches 2^31 elements; invalid level blocks
+    and precisions are refused the same way."""
+    from magic_amd import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("library not built")
+    lib = _lib.load()
+
+    def call(**kw):
+        a = _lib.GridArgs()
+        a.N, a.T, a.M, a.Mh = 97, 576, 384, 576
+        for k, v in kw.items():
+            setattr(a, k, v)
+        return lib.mh_grid_step(C.byref(a), None)
+
+    assert call(N=2731) == -1          # 576 * 4 * 2731 * 2 = 12.58e6 < 2^24 but M*... >= 2^31
+    assert call(T=1536, N=1366) == -1  # 1536 * 4 * 1366 * 2 = 16.79e6 >= 2^24
+    assert call(rb=99) == -1
+    assert call(fprec=2) == -1
