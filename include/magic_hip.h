@@ -285,7 +285,11 @@ typedef struct {
  * out[q*(l_max+1) + i], q = kin pol, kin tor, mag pol, mag tor.  by_m=2: per
  * degree and split into bin = 2*(m == 0) + ((l + m) odd), out[(4q + bin)*(l_max+1)
  * + l] (16 rows), from which the axisymmetric and equatorially symmetric parts
- * of output.py:128-137,172-180 are sums. */
+ * of output.py:128-137,172-180 are sums.  by_m=3: radial profiles per degree,
+ * not integrated (qw unused), out[(q*(l_max+1) + l)*N + r] for q = kin pol,
+ * kin tor, kin pol over m > 0, kin tor over m > 0, then the same four magnetic
+ * (8 planes): the e_lr, e_pol_lr and e_lr_c arrays of get_dlm
+ * (output.py:768-800), halved (the reference's fac = 1/2). */
 int mh_energy(const MhEnergyArgs* args, void* stream);
 
 /* ------------------------------------------------------------ rebuild */

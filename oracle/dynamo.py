@@ -19,14 +19,16 @@ it in module globals: ``fields.py:14-67``, ``dt_fields.py:55-123``).
   (``time_scheme.py:56-73``).
 * energies — ``get_e_kin_full``/``get_e_mag_oc_full`` (``output.py:111-193``);
   ``energy_components`` adds the masked parts (pinned by single-mode fields
-  whose energy the (l, m) parity places in known columns).
+  whose energy the (l, m) parity places in known columns); ``get_dlm``
+  (``output.py:750-861``) the integral length scales.
 
-Pinned by ``tests/test_oracle_step.py`` against the committed fixtures in
+Pinned by ``tests/test_oracle.py`` against the committed fixtures in
 ``tests/golden/`` written by ``tests/golden/make_golden.py`` from the
 reference itself at l_max=16 and l_max=32 (the composed trajectory is the
-part only the reference can pin), plus the self-contained invariants listed
-there (conduction state is a steady solution of the s/p equations, m=0
-realness, boundary conditions).
+part only the reference can pin; the ``dtchange`` fixture pins the Courant
+decisions, the variable-step weights and the rebuild on a dt change), and by
+``tests/test_oracle_energy.py`` (energy components, spectra and the
+``get_dlm`` length scales against the fixtures and exact identities).
 """
 
 from __future__ import annotations
@@ -359,6 +361,68 @@ class OracleRun:
                 s(emp, ~lpm_even), s(emt, lpm_even), s(emp, m0 & (l % 2 == 1)),
                 s(emt, m0 & (l % 2 == 0)), 0.0, 0.0)
         return ekin, emag
+
+    def get_dlm(self, switch="V"):
+        """Integral length scales (``output.py:750-861``): (dl, dlR, dm, dlc,
+        dlPolPeak, dlRc, dlPolPeakR) of the flow (switch 'V': w, dw, z, degrees
+        from 1) or of the field ('B': b, db, aj, degrees from 2, no LFfac)."""
+        p = self.prm
+        dLh = self.lmarr["dLh"][:, None]
+        or2 = self.rad["or2"][None]
+        a, da, c = (self.w, self.dw, self.z) if switch == "V" else (self.b, self.db, self.aj)
+        e_p = dLh * (dLh * or2 * self._cc2real(a) + self._cc2real(da))
+        e_t = dLh * self._cc2real(c)
+        l_first = 1 if switch == "V" else 2
+        e = e_p + e_t
+        l = self.maps["lm2l"]; m = self.maps["lm2m"]
+        lp1, N = p.l_max + 1, p.n_r_max
+        def by(idx, x, n):
+            out = np.zeros((n, N))
+            np.add.at(out, idx, x)
+            return out
+        e_lr = by(l, e, lp1)
+        e_pol_lr = by(l, e_p, lp1)
+        e_mr = by(m, e, p.l_max + 1)
+        e_lr_c = by(l, e * (m != 0)[:, None], lp1)
+        q = self.q
+        fac = 0.5
+        ls = np.arange(l_first, lp1, dtype=np.float64)
+        E_l = np.zeros(lp1); E_pol_l = np.zeros(lp1)
+        for ll in range(l_first, lp1):
+            E_l[ll] = fac * (e_lr[ll] @ q)
+            E_pol_l[ll] = fac * (e_pol_lr[ll] @ q)
+        E, EL = E_l[l_first:].sum(), (ls * E_l[l_first:]).sum()
+        dl = math.pi * E / EL if EL != 0.0 else 0.0
+        lpeak = int(np.argmax(E_pol_l[1:])) + 1
+        dlPolPeak = math.pi / lpeak
+        eps = np.finfo(np.float64).eps
+        if switch == "V":
+            E_c = np.zeros(lp1)
+            for ll in range(l_first, lp1):
+                E_c[ll] = fac * (e_lr_c[ll] @ q)
+            Ec, ELc = E_c[l_first:].sum(), (ls * E_c[l_first:]).sum()
+            dlc = math.pi * Ec / ELc if ELc != 0.0 else 0.0
+        else:
+            dlc = 0.0
+        E_m = EM_m = 0.0
+        for mm in range(p.minc, p.l_max + 1, p.minc):
+            em = fac * (e_mr[mm] @ q)
+            E_m += em
+            EM_m += mm * em
+        dm = math.pi * E_m / EM_m if EM_m != 0.0 else 0.0
+        if switch == "V":
+            ER = fac * e_lr[l_first:].sum(axis=0)
+            ELR = fac * (ls[:, None] * e_lr[l_first:]).sum(axis=0)
+            dlR = np.where(ELR > 10.0 * eps, math.pi * ER / np.where(ELR != 0, ELR, 1.0), 0.0)
+            ERc = fac * e_lr_c[l_first:].sum(axis=0)
+            ELRc = fac * (ls[:, None] * e_lr_c[l_first:]).sum(axis=0)
+            dlRc = np.where(ELRc > 10.0 * eps, math.pi * ERc / np.where(ELRc != 0, ELRc, 1.0), 0.0)
+            per_r = e_pol_lr[1:]
+            mx, ix = per_r.max(axis=0), per_r.argmax(axis=0) + 1
+            dlPolPeakR = np.where(mx > 10.0 * eps, math.pi / ix, 0.0)
+        else:
+            dlR = np.zeros(N); dlRc = np.zeros(N); dlPolPeakR = np.zeros(N)
+        return dl, dlR, dm, dlc, dlPolPeak, dlRc, dlPolPeakR
 
     def spectra(self):
         """Per-l and per-m kinetic/magnetic energy spectra (``output.py:750-861`` aggregation)."""

@@ -1,24 +1,54 @@
-"""Write golden trajectory fixtures by running the reference step on the CPU.
+"""Write golden fixtures by running the reference on the CPU.
 
-Only the composed multi-step trajectory of the reference's ``one_step``
-(``step_time.py:1574-1637``: radial loop, CFL, IMEX weights, fused lm loop,
-insulating B update) cannot be pinned by closed forms or library routines,
-so this script imports the reference package from ``/root/reference/src``
-(pure Python, FP64, ``MAGIC_DEVICE=cpu``) in a child process per
-configuration and stores small ``.npz`` vectors here.  It is run by hand in
-the development container; tests only read the ``.npz`` files.
+What closed forms and library routines cannot pin is generated here from the
+reference package itself (``/root/reference/src``, pure Python, FP64,
+``MAGIC_DEVICE=cpu``), one child process per case, and stored as small
+``.npz`` vectors beside this script.  It is run by hand in the development
+container; tests only read the ``.npz`` files.
 
-Cases (each: ``initialize_fields`` → optional seeded noise on s →
-``setup_initial_state`` → ``initialize_dt`` → ``one_step`` × n):
+Trajectory cases (``step_time.one_step``, ``step_time.py:1574-1637``: radial
+loop, CFL, IMEX weights, rebuild on a dt change, fused lm loop, insulating B
+update).  Each: ``initialize_fields`` -> optional seeded noise on s ->
+``setup_initial_state`` -> ``initialize_dt`` -> ``one_step`` x n, with the dt
+that each step returns passed to the next (as ``main.py`` does):
 
 * ``bench16``: dynamo benchmark l_max=16, n_r_max=33, no noise, 20 steps;
   energies every step, fields at steps 1 and 20, explicit terms after the
   first radial loop.
 * ``c1``: l_max=32, n_r_max=33, E=1e-3, Ra=1e5, Pr=1, Pm=5, dt=1e-4,
   noise 1e-5 (seed 20261202), 100 steps; energies every step, fields at
-  step 100.
+  step 100, and at step 100 the full e_kin / e_mag_oc tuples
+  (``output.py:111-193``) and the length scales of ``get_dlm``
+  (``output.py:750-861``) for the flow ('V') and the field ('B').
+* ``dtchange``: l_max=16, n_r_max=33, dtmax=1e-3, noise 1.0 (seed 7), 30
+  steps: the Courant limit changes dt several times, which exercises
+  ``courant.py:80-155`` (``dt_courant``), the variable-step CNAB2 weights
+  (``time_scheme.py:56-73``) and the rebuild at ``step_time.py:1628-1629``;
+  energies and dt every step, fields at steps 10 and 30.
 
-Usage: ``python tests/golden/make_golden.py``
+Finite-difference operator cases (``fd4_ob2``, ``fd4_ob3``): the FD branches
+of ``build_s_matrices`` (``update_s.py:66-188``), ``build_z_matrices``
+(``update_z.py:89-240``) and ``build_w_matrices``
+(``update_w_doublecurl.py:67-307``) at fd_order 4 with boundary order 2 or 3,
+n_r_max=64, l_max=12, anelastic polytrope strat=3, polind=2, g2=1, called
+with wimp_lin0 = 6e-5: the radial grid, the background profiles, and for each
+system the LINPACK-factored bands, pivots and preconditioning factors.
+
+Output-file case (``files16``): l_max=16, n_r_max=17 (the config-1 physics,
+noise 1e-3 seed 3, dt 1e-4), 10 steps, then the reference's own writers:
+the Fortran MagIC v5 checkpoint (``checkpoint_io.write_checkpoint_fortran``,
+``checkpoint_io.py:336-490``) and the graph file (``graph_output.
+write_graph_file``, ``graph_output.py:53-242``), stored as written
+(``files16_checkpoint.bin``, ``files16_G_1.bin``), with the state they were
+written from (``files16.npz``: fields, explicit history slot, dt, time) and the
+energies of 10 further steps continued in memory.  A second process restarts
+from the checkpoint file the way ``main.run`` does for ``fortran_restart``
+(``main.py:318-374``) and steps 10 times: ``energies_restart`` (the reference's
+reader keeps the file's big-endian byte order, which its own
+``load_fortran_checkpoint`` cannot hand to ``torch.from_numpy``; the script
+gives it native-order copies of the same values).
+
+Usage: ``python tests/golden/make_golden.py [case ...]``
 """
 
 from __future__ import annotations
@@ -37,7 +67,21 @@ CASES = {
                     field_steps=[1, 20], expl_step1=True),
     "c1": dict(cfg=dict(l_max=32, n_r_max=33, ek=1e-3, ra=1e5, pr=1.0, prmag=5.0,
                         radratio=0.35, dtmax=1e-4), dt=1e-4, steps=100,
-               noise=dict(seed=20261202, amp=1e-5), field_steps=[100], expl_step1=False),
+               noise=dict(seed=20261202, amp=1e-5), field_steps=[100], expl_step1=False,
+               diag_steps=[100]),
+    "dtchange": dict(cfg=dict(l_max=16, n_r_max=33, ek=1e-3, ra=1e5, pr=1.0, prmag=5.0,
+                              radratio=0.35, dtmax=1e-3), dt=1e-3, steps=30,
+                     noise=dict(seed=7, amp=1.0), field_steps=[10, 30], expl_step1=False,
+                     diag_steps=[30]),
+}
+
+FD_CASES = {
+    "fd4_ob2": dict(cfg=dict(l_max=12, n_r_max=64, radial_scheme="FD", fd_order=4,
+                             fd_order_bound=2, strat=3.0, polind=2.0, g0=0.0, g1=0.0, g2=1.0,
+                             mode=1, ra=1e5, ek=1e-3, pr=1.0, radratio=0.35), wimp=6e-5),
+    "fd4_ob3": dict(cfg=dict(l_max=12, n_r_max=64, radial_scheme="FD", fd_order=4,
+                             fd_order_bound=3, strat=3.0, polind=2.0, g0=0.0, g1=0.0, g2=1.0,
+                             mode=1, ra=1e5, ek=1e-3, pr=1.0, radratio=0.35), wimp=6e-5),
 }
 
 CHILD = r"""
@@ -66,8 +110,9 @@ names = ["s", "z", "w", "p", "b", "aj"]
 res = {}
 en = []
 dts = []
+dt = case["dt"]
 for n in range(1, case["steps"] + 1):
-    dt = step_time.one_step(n, case["dt"])
+    dt = step_time.one_step(n, dt)
     dts.append(dt)
     ek = output.get_e_kin_full(fields.w_LMloc, fields.dw_LMloc, fields.z_LMloc)
     em = output.get_e_mag_oc_full(fields.b_LMloc, fields.db_LMloc, fields.aj_LMloc)
@@ -75,6 +120,17 @@ for n in range(1, case["steps"] + 1):
     if n in case["field_steps"]:
         for k in names:
             res[f"{k}_step{n}"] = getattr(fields, k + "_LMloc").numpy().copy()
+        for k in ("dw", "db"):
+            res[f"{k}_step{n}"] = getattr(fields, k + "_LMloc").numpy().copy()
+    if n in case.get("diag_steps", []):
+        res[f"ekin_full_step{n}"] = np.array(list(ek))
+        res[f"emag_full_step{n}"] = np.array(list(em))
+        for sw, (a, da, c) in (("V", (fields.w_LMloc, fields.dw_LMloc, fields.z_LMloc)),
+                               ("B", (fields.b_LMloc, fields.db_LMloc, fields.aj_LMloc))):
+            dl, dlR, dm, dlc, dlPolPeak, dlRc, dlPolPeakR = output.get_dlm(a, da, c, sw)
+            res[f"dlm{sw}_scalars_step{n}"] = np.array([dl, dm, dlc, dlPolPeak])
+            res[f"dlm{sw}_radial_step{n}"] = np.stack([dlR.numpy(), dlRc.numpy(),
+                                                      dlPolPeakR.numpy()])
     if n == 1 and case["expl_step1"]:
         for k, ta in (("s", dt_fields.dsdt), ("z", dt_fields.dzdt), ("w", dt_fields.dwdt),
                       ("p", dt_fields.dpdt), ("b", dt_fields.dbdt), ("j", dt_fields.djdt)):
@@ -85,19 +141,184 @@ res["dt"] = np.array(dts)
 np.savez(out, **res)
 """
 
+FD_CHILD = r"""
+import json, sys
+import numpy as np
+case = json.loads(sys.argv[1]); out = sys.argv[2]
+sys.path.insert(0, sys.argv[3])
+from magic_torch.config import configure
+configure(dict(case["cfg"], device="cpu", use_shtns=False))
+import torch
+torch.set_num_threads(8)
+from magic_torch import radial_functions as rf
+from magic_torch import update_s, update_z, update_w_doublecurl as uw
+res = {}
+res["r"] = rf.r.numpy().copy()
+for k in ("rho0", "temp0", "beta", "dbeta", "ddbeta", "dLtemp0", "orho1", "rgrav"):
+    res["prof_" + k] = getattr(rf, k).numpy().copy()
+w = case["wimp"]
+update_s.build_s_matrices(w)
+update_z.build_z_matrices(w)
+uw.build_w_matrices(w)
+res["S_abd"] = update_s._s_bands_by_l.numpy().copy()
+res["S_piv"] = update_s._s_piv_by_l.numpy().copy()
+res["S_fac"] = update_s._s_fac_by_l.numpy().copy()
+res["S_k"] = np.array([update_s._s_kl, update_s._s_ku])
+res["Z_abd"] = update_z._z_bands_by_l.numpy().copy()
+res["Z_piv"] = update_z._z_piv_by_l.numpy().copy()
+res["Z_fac"] = update_z._z_fac_by_l.numpy().copy()
+res["Z_k"] = np.array([update_z._z_kl, update_z._z_ku])
+res["W_abd"] = uw._w_bands_by_l.numpy().copy()
+res["W_piv"] = uw._w_piv_by_l.numpy().copy()
+res["W_fac_row"] = uw._w_fac_row_by_l.numpy().copy()
+res["W_fac_col"] = uw._w_fac_col_by_l.numpy().copy()
+res["W_k"] = np.array([uw._w_kl, uw._w_ku])
+np.savez(out, **res)
+"""
 
-def main():
+
+FILES_CASE = dict(cfg=dict(l_max=16, n_r_max=17, ek=1e-3, ra=1e5, pr=1.0, prmag=5.0,
+                           radratio=0.35, dtmax=1e-4), dt=1e-4, steps=10,
+                  noise=dict(seed=3, amp=1e-3), cont=10)
+
+FILES_CHILD = r"""
+import json, sys
+import numpy as np
+case = json.loads(sys.argv[1]); out = sys.argv[2]
+sys.path.insert(0, sys.argv[3]); sys.path.insert(0, sys.argv[4])
+from magic_torch.config import configure
+configure(dict(case["cfg"], device="cpu", use_shtns=False))
+import torch
+torch.set_num_threads(8)
+from magic_torch import fields, dt_fields, step_time, output
+from magic_torch.init_fields import initialize_fields
+from magic_torch.checkpoint_io import write_checkpoint_fortran
+from magic_torch.graph_output import write_graph_file
+from oracle.params import Params
+from oracle import horizontal, radial
+from oracle.dynamo import seeded_noise
+initialize_fields()
+p = Params(case["cfg"])
+nz = seeded_noise(p, horizontal.lm_maps(p), radial.radial_functions(p),
+                  case["noise"]["seed"], case["noise"]["amp"])
+fields.s_LMloc += torch.from_numpy(nz)
+step_time.setup_initial_state()
+step_time.initialize_dt(case["dt"])
+dt = case["dt"]; t = 0.0
+for n in range(1, case["steps"] + 1):
+    dt = step_time.one_step(n, dt)
+    t += dt
+write_checkpoint_fortran(out + "_checkpoint.bin", t, case["steps"] + 1)
+write_graph_file(out + "_G_1.bin", t, fields.w_LMloc, fields.dw_LMloc, fields.z_LMloc,
+                 fields.s_LMloc, fields.p_LMloc, fields.b_LMloc, fields.db_LMloc,
+                 fields.aj_LMloc)
+res = dict(time=np.array(t), dt=np.array([dt]))
+for k in ("w", "dw", "z", "p", "s", "b", "db", "aj"):
+    res[k] = getattr(fields, k + "_LMloc").numpy().copy()
+for k, ta in (("w", dt_fields.dwdt), ("z", dt_fields.dzdt), ("p", dt_fields.dpdt),
+              ("s", dt_fields.dsdt), ("b", dt_fields.dbdt), ("aj", dt_fields.djdt)):
+    res["expl1_" + k] = ta.expl[:, :, 1].numpy().copy()
+import magic_torch.time_scheme as ts
+res["tscheme_dt"] = ts.tscheme.dt.numpy().copy()
+en = []
+for n in range(case["steps"] + 1, case["steps"] + case["cont"] + 1):
+    dt = step_time.one_step(n, dt)
+    ek = output.get_e_kin_full(fields.w_LMloc, fields.dw_LMloc, fields.z_LMloc)
+    em = output.get_e_mag_oc_full(fields.b_LMloc, fields.db_LMloc, fields.aj_LMloc)
+    en.append([ek.e_p, ek.e_t, em.e_p, em.e_t])
+res["energies_cont"] = np.array(en)
+np.savez(out + ".npz", **res)
+"""
+
+RESTART_CHILD = r"""
+import json, sys
+import numpy as np
+case = json.loads(sys.argv[1]); out = sys.argv[2]
+sys.path.insert(0, sys.argv[3])
+from magic_torch.config import configure
+configure(dict(case["cfg"], device="cpu", use_shtns=False))
+import torch
+torch.set_num_threads(8)
+from magic_torch import fields, dt_fields, step_time, output
+import magic_torch.main as _m
+load_fortran_checkpoint = _m.load_fortran_checkpoint
+from magic_torch.time_scheme import tscheme
+from magic_torch.precision import CDTYPE, DEVICE
+from magic_torch import checkpoint_io
+# the reader keeps the file's (big-endian) byte order, which torch.from_numpy in
+# load_fortran_checkpoint refuses: hand it native-order copies of the arrays
+_read = checkpoint_io.read_checkpoint
+def _native(path):
+    ck = _read(path)
+    nat = lambda a: a.astype(a.dtype.newbyteorder("=")) if isinstance(a, np.ndarray) else a
+    for k in ("w", "z", "p", "s", "xi", "b", "aj", "b_ic", "aj_ic", "dt", "r"):
+        setattr(ck, k, nat(getattr(ck, k)))
+    ck.derivatives = {n: {part: [nat(a) for a in v] for part, v in d.items()}
+                      for n, d in ck.derivatives.items()}
+    return ck
+checkpoint_io.read_checkpoint = _native
+# main.run's fortran_restart branch (main.py:318-374), TensorBoard aside
+ck = load_fortran_checkpoint(out + "_checkpoint.bin")
+step_time.setup_initial_state()
+for i in range(min(len(ck.dt), tscheme.nexp)):
+    tscheme.dt[i] = ck.dt[i]
+tscheme.set_weights()
+dt = tscheme.dt[0].item()
+step_time.initialize_dt(dt)
+m = {"w": dt_fields.dwdt, "z": dt_fields.dzdt, "p": dt_fields.dpdt, "s": dt_fields.dsdt,
+     "b": dt_fields.dbdt, "aj": dt_fields.djdt}
+for name, d in ck.derivatives.items():
+    for i, arr in enumerate(d["expl"]):
+        m[name].expl[:, :, i + 1].copy_(torch.from_numpy(arr).to(dtype=CDTYPE, device=DEVICE))
+en = []
+for n in range(1, case["cont"] + 1):
+    dt = step_time.one_step(n, dt)
+    ek = output.get_e_kin_full(fields.w_LMloc, fields.dw_LMloc, fields.z_LMloc)
+    em = output.get_e_mag_oc_full(fields.b_LMloc, fields.db_LMloc, fields.aj_LMloc)
+    en.append([ek.e_p, ek.e_t, em.e_p, em.e_t])
+res = dict(np.load(out + ".npz"))
+res["energies_restart"] = np.array(en)
+res["w_restart"] = fields.w_LMloc.numpy().copy()
+np.savez(out + ".npz", **res)
+"""
+
+
+def main(names=None):
     env = dict(os.environ, MAGIC_DEVICE="cpu", PYTHONDONTWRITEBYTECODE="1",
                TORCH_COMPILE_DISABLE="1", TORCHDYNAMO_DISABLE="1")
+    src = "rvilim/torch-magic src/magic_torch (FP64 CPU)"
     for name, case in CASES.items():
+        if names and name not in names:
+            continue
         out = os.path.join(HERE, f"traj_{name}.npz")
         subprocess.run([sys.executable, "-c", CHILD, json.dumps(case), out, REF_SRC, REPO],
                        check=True, env=env)
-        meta = dict(case, source="rvilim/torch-magic src/magic_torch step_time.one_step (FP64 CPU)")
+        meta = dict(case, source=src + ": step_time.one_step, dt chained")
         with open(os.path.join(HERE, f"traj_{name}.json"), "w") as f:
+            json.dump(meta, f, indent=1)
+        print("wrote", out)
+    if not names or "files16" in names:
+        out = os.path.join(HERE, "files16")
+        subprocess.run([sys.executable, "-c", FILES_CHILD, json.dumps(FILES_CASE), out, REF_SRC,
+                        REPO], check=True, env=env)
+        subprocess.run([sys.executable, "-c", RESTART_CHILD, json.dumps(FILES_CASE), out,
+                        REF_SRC], check=True, env=env)
+        meta = dict(FILES_CASE, source=src + ": checkpoint_io.write_checkpoint_fortran, "
+                    "graph_output.write_graph_file, main.run fortran_restart sequence")
+        with open(out + ".json", "w") as f:
+            json.dump(meta, f, indent=1)
+        print("wrote", out)
+    for name, case in FD_CASES.items():
+        if names and name not in names:
+            continue
+        out = os.path.join(HERE, f"{name}.npz")
+        subprocess.run([sys.executable, "-c", FD_CHILD, json.dumps(case), out, REF_SRC],
+                       check=True, env=env)
+        meta = dict(case, source=src + ": build_s/z/w_matrices, FD branch")
+        with open(os.path.join(HERE, f"{name}.json"), "w") as f:
             json.dump(meta, f, indent=1)
         print("wrote", out)
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:] or None)

@@ -60,3 +60,31 @@ def test_operators_match_oracle(ob):
         A, fr, fc = ofd.w_operator(l, wimp, mats, prof)
         assert np.abs(_dense(ops.W[0][l], hw, n) - A).max() < 1e-12
         assert np.abs(ops.W[2][l] - fc).max() < 1e-12 * np.abs(fc).max()
+
+
+@pytest.mark.parametrize("ob", [2, 3])
+def test_host_bands_factor_to_the_reference_bands(ob):
+    """The band-native host build of every degree, factored in LINPACK order
+    on the CPU, against the reference's own factored bands, pivots and
+    preconditioning factors (tests/golden/fd4_ob*.npz: fd_order 4, n_r_max 64,
+    l_max 12, strat 3, polind 2, g2 1, wimp_lin0 6e-5).  The GPU factorisation
+    of the same bands is checked against this fixture in test_gpu_fd.py."""
+    import os
+    from oracle.algebra import prepare_band
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                             f"fd4_ob{ob}.npz"))
+    ops = pfd.FDOperators(64, 12, 6e-5, fd_order=4, fd_order_bound=ob)
+    assert np.abs(ops.r - g["r"]).max() <= 1e-15 * ops.r[0]
+    for name, (abd, fr, fc), k in (("S", ops.S, ops.kls), ("Z", ops.Z, ops.kls),
+                                   ("W", ops.W, ops.klw)):
+        assert tuple(g[f"{name}_k"]) == (k, k)
+        for l in range(0 if name == "S" else 1, 13):
+            lu, piv, info = prepare_band(abd[l].copy(), 64, k, k)
+            assert info == 0
+            assert np.array_equal(np.asarray(piv), g[f"{name}_piv"][l]), (name, l)
+            ref = g[f"{name}_abd"][l]
+            assert np.abs(lu - ref).max() <= 1e-13 * np.abs(ref).max(), (name, l)
+            rf = g["W_fac_row" if name == "W" else f"{name}_fac"][l]
+            assert np.abs(fr[l] - rf).max() <= 1e-14 * np.abs(rf).max(), (name, l)
+            if name == "W":
+                assert np.abs(fc[l] - g["W_fac_col"][l]).max() <= 1e-14 * np.abs(fc[l]).max()

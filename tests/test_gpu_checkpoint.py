@@ -91,3 +91,28 @@ def test_restart_refuses_queued_device_steps(tmp_path):
     run.ctl_end()
     run.save_checkpoint(tmp_path / "x.pt")
     run.close()
+
+
+def test_load_after_device_steps_resets_step_control(tmp_path):
+    """A run that stepped past the file with device step control, then loads
+    the earlier file, saves and continues from the restored state (not from the
+    pre-load device state): same steps as a run restored in a fresh object."""
+    import torch
+    path, path2 = tmp_path / "at12.pt", tmp_path / "again.pt"
+    src = _started()
+    src.run_steps(CASE["at"], CASE["dt"])
+    src.save_checkpoint(path)
+    src.use_graphs = True
+    src.run_steps(8, src.dt_hist[0])           # past the file
+    step, _ = src.load_checkpoint(path)
+    src.save_checkpoint(path2)                 # no stale queued steps
+    assert torch.load(path2, weights_only=True)["step"] == step == CASE["at"]
+    src.run_steps(CASE["steps"] - step, src.dt_hist[0])
+    ref = _fresh()
+    ref.load_checkpoint(path)
+    ref.run_steps(CASE["steps"] - step, ref.dt_hist[0])
+    assert src.n_step == ref.n_step == CASE["steps"] and src.time == ref.time
+    for k in ("s", "z", "w", "p", "b", "aj"):
+        assert torch.equal(src.f[k], ref.f[k]), k
+    for r in (src, ref):
+        r.close()

@@ -88,3 +88,41 @@ def test_fd_config3_full_size_residuals():
             rows = torch.as_tensor(lidx > 0).cuda()
             rel = float((Ay - b)[rows].abs().max() / b[rows].abs().max())
             assert rel < 1e-10, (name, rel)
+
+
+@pytest.mark.parametrize("ob", [2, 3])
+def test_fd_gpu_factors_match_reference_fixture(ob):
+    """GPU LINPACK factorisation of every degree against the reference's own
+    factored bands and pivots (build_s/z/w_matrices FD branch, fd_order 4,
+    n_r_max 64, l_max 12, strat 3, polind 2, g2 1, wimp_lin0 6e-5): pivots
+    equal, factors within 1e-13 of the largest entry; a solve with those
+    factors agrees with the oracle's dgbsl on the reference factors to 1e-12."""
+    import os
+    import torch
+    from magic_amd.fd import FDOperators
+    from oracle.algebra import solve_band
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                             f"fd4_ob{ob}.npz"))
+    n, l_max = 64, 12
+    ops = FDOperators(n, l_max, 6e-5, fd_order=4, fd_order_bound=ob).factor()
+    lidx = np.repeat(np.arange(l_max + 1), np.arange(1, l_max + 2)).astype(np.int32)
+    rhs = _rhs(len(lidx), n, 10 + ob)
+    d_l, d_rhs = torch.as_tensor(lidx).cuda(), torch.as_tensor(rhs).cuda()
+    for name in ("S", "Z", "W"):
+        s = ops.dev[name]
+        assert int(s["info"].abs().sum()) == 0
+        piv, lu = s["piv"].cpu().numpy(), s["abd"].cpu().numpy()
+        sol = ops.solve(name, d_rhs, d_l).cpu().numpy()
+        k = s["k"]
+        for l in range(0 if name == "S" else 1, l_max + 1):
+            assert np.array_equal(piv[l], g[f"{name}_piv"][l]), (name, l)
+            ref = g[f"{name}_abd"][l]
+            assert np.abs(lu[l] - ref).max() <= 1e-13 * np.abs(ref).max(), (name, l)
+            fr = g["W_fac_row" if name == "W" else f"{name}_fac"][l]
+            i = int(np.nonzero(lidx == l)[0][0])
+            b = rhs[i] * fr
+            x = (solve_band(ref, n, k, k, g[f"{name}_piv"][l], b.real)
+                 + 1j * solve_band(ref, n, k, k, g[f"{name}_piv"][l], b.imag))
+            if name == "W":
+                x = x * g["W_fac_col"][l]
+            assert np.abs(sol[i] - x).max() <= 1e-12 * np.abs(x).max(), (name, l)

@@ -69,3 +69,36 @@ def test_runs_of_one_resolution_overlapping_match_alone():
         assert r.energies == a.energies, r.rid
         for k, v in r.final_fields.items():
             assert torch.equal(v, a.final_fields[k]), (r.rid, k)
+
+
+def test_config5_resolutions_alone_and_in_the_mix():
+    """Requests of the ensemble workload itself (config5_requests) covering
+    l_max 64, 85 and 127 and n_r 49 and 65, 30 steps each, served with
+    overlapping admission: each run ends bit-identical to the same run alone."""
+    import torch
+    from magic_amd.serve import EnsembleServer, Request, config5_requests, serve_sequential
+    pool = config5_requests()
+    picked, seen = [], set()
+    for r in pool:
+        key = (r.cfg["l_max"], r.cfg["n_r_max"])
+        if r.cfg["l_max"] in (64, 85, 127) and r.cfg["n_r_max"] in (49, 65) and key not in seen:
+            seen.add(key)
+            picked.append(r)
+    assert {k[0] for k in seen} == {64, 85, 127} and {k[1] for k in seen} == {49, 65}
+    picked = picked[:6]
+
+    def reqs():
+        return [Request(i, dict(r.cfg), n_steps=30, arrival=0.02 * i, noise_amp=1e-5,
+                        noise_seed=r.noise_seed) for i, r in enumerate(picked)]
+
+    srv = EnsembleServer(max_slots=4)
+    mix, summary = srv.serve(reqs(), keep_fields=True)
+    alone, _ = serve_sequential(reqs(), keep_fields=True)
+    assert summary["runs"] == len(picked)
+    assert max(n for _, n in srv.occupancy) >= 2
+    by_id = {r.rid: r for r in alone}
+    for r in mix:
+        a = by_id[r.rid]
+        assert r.energies == a.energies, (r.rid, r.cfg)
+        for k, v in r.final_fields.items():
+            assert torch.equal(v, a.final_fields[k]), (r.rid, k)

@@ -40,13 +40,18 @@ def _rel(a, b):
     return np.abs(a - b).max() / np.abs(b).max()
 
 
-@pytest.mark.parametrize("name,fused", [("bench16", True), ("bench16", False), ("c1", True)])
+@pytest.mark.parametrize("name,fused", [("bench16", True), ("bench16", False), ("c1", True),
+                                        ("dtchange", True)])
 def test_step_matches_reference_trajectory(name, fused):
-    """fused: one grid-space kernel; otherwise separate FFT / products / FFT kernels."""
+    """fused: one grid-space kernel; otherwise separate FFT / products / FFT kernels.
+    The dt each step returns is passed to the next (dtchange: the reference
+    changes dt five times in 30 steps, each with a rebuild and variable-step
+    CNAB2 weights)."""
     case, g = _load(name)
     run = _run(case, fused)
+    dt = case["dt"]
     for n in range(1, case["steps"] + 1):
-        dt = run.one_step(n, case["dt"])
+        dt = run.one_step(n, dt)
         assert dt == pytest.approx(g["dt"][n - 1], rel=1e-12)
         e = np.array(run.energies())
         ge = g["energies"][n - 1]
@@ -64,8 +69,9 @@ def test_mixed_precision_step_tracks_reference(name):
     fields within 1e-5 of their largest coefficient at every recorded step."""
     case, g = _load(name)
     run = _run(case, precision="mixed")
+    dt = case["dt"]
     for n in range(1, case["steps"] + 1):
-        dt = run.one_step(n, case["dt"])
+        dt = run.one_step(n, dt)
         assert dt == pytest.approx(g["dt"][n - 1], rel=1e-6)
         e = np.array(run.energies())
         ge = g["energies"][n - 1]
@@ -231,6 +237,9 @@ def test_device_step_control_matches_host_steps():
     dev.run_steps(10, case["dt"])
     dev.run_steps(case["steps"] - 10, dev.dt_log[-1])
     assert dev.dt_log == host.dt_log
+    # the same case as the reference fixture traj_dtchange: the reference's dt sequence
+    _, g = _load("dtchange")
+    np.testing.assert_allclose(dev.dt_log, g["dt"], rtol=1e-12, atol=0)
     assert dev.n_rebuild == host.n_rebuild and host.n_rebuild >= 6, host.n_rebuild
     assert dev.time == host.time and dev.n_step == host.n_step == case["steps"]
     for k in ("s", "z", "w", "p", "b", "aj", "db", "ddw"):

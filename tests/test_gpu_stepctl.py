@@ -31,7 +31,7 @@ def _host_decision(v, delxr2, delxh2, st, alpha, BuoFac, sched=None):
         new = dt_new != st["dt"]
     dt1, dt2 = dt_new, st["dt1"]
     wl0 = alpha * dt_new
-    rebuild = n == 1 or new or st["wimp_lin0"] != wl0
+    rebuild = n == 1 or new or math.isnan(st["wimp_lin0"])
     w = [1.0, (1.0 - alpha) * dt1, (1.0 + 0.5 * dt1 / dt2) * dt1, -0.5 * dt1 * dt1 / dt2,
          alpha * dt1 * BuoFac]
     return dict(dt=dt_new, new_dt=int(new), rebuild=int(rebuild), dt1=dt1, dt2=dt2,

@@ -169,3 +169,43 @@ def test_grid_step_rejects_offsets_beyond_32_bits():
     assert call(T=1536, N=1366) == -1  # 1536 * 4 * 1366 * 2 = 16.79e6 >= 2^24
     assert call(rb=99) == -1
     assert call(fprec=2) == -1
+
+
+def test_solver_entry_points_check_shapes_before_launch():
+    """The drop-in solvers refuse shapes and degree indices that would make a
+    kernel read past its buffers (checked on the host, before any device work:
+    these raise on a machine without a GPU too)."""
+    import numpy as np
+    from magic_amd import algebra
+    n, kl, ku, nl = 16, 2, 2, 4
+    abd = np.zeros((nl, 2 * kl + ku + 1, n))
+    piv = np.ones((nl, n), np.int64)
+    rhs = np.zeros((6, n), np.complex128)
+    lidx = np.array([0, 1, 1, 2, 3, 3])
+    with pytest.raises(IndexError):
+        algebra.banded_solve_by_l(abd, piv, np.array([0, 1, 1, 2, 3, 4]), rhs, n, kl, ku)
+    with pytest.raises(IndexError):
+        algebra.banded_solve_by_l(abd, piv, np.array([0, 1, 1, 2, 3, -1]), rhs, n, kl, ku)
+    with pytest.raises(ValueError):
+        algebra.banded_solve_by_l(abd, piv, lidx[:5], rhs, n, kl, ku)
+    with pytest.raises(ValueError):
+        algebra.banded_solve_by_l(abd, piv, lidx, rhs[:, :n - 1], n, kl, ku)
+    with pytest.raises(ValueError):
+        algebra.banded_solve_by_l(abd[:, :6], piv, lidx, rhs, n, kl, ku)
+    with pytest.raises(ValueError):
+        algebra.banded_solve_by_l(abd, piv[:3], lidx, rhs, n, kl, ku)
+    with pytest.raises(ValueError):
+        algebra.banded_solve_by_l(abd, piv, lidx, rhs, n, kl, ku, fac_row_by_l=np.ones((nl, 3)))
+    with pytest.raises(ValueError):
+        algebra.banded_solve_by_l(np.zeros((nl, 16, n)), piv, lidx, rhs, n, 5, 5)
+    with pytest.raises(ValueError):
+        algebra.solve_band_real(abd[0], n, kl, ku, piv[0], np.zeros(n + 1))
+    with pytest.raises(ValueError):
+        algebra.prepare_band(np.zeros((6, n)), n, kl, ku)
+    inv = np.zeros((nl, n, n))
+    with pytest.raises(IndexError):
+        algebra.chunked_solve_complex(inv, np.array([0, 1, 1, 2, 3, 9]), rhs)
+    with pytest.raises(ValueError):
+        algebra.chunked_solve_complex(inv, lidx, rhs[:, :3])
+    with pytest.raises(ValueError):
+        algebra.chunked_solve_complex(inv[:, :, :3], lidx, rhs)

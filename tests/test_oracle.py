@@ -26,9 +26,12 @@ GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
 # ------------------------------------------------------------ golden trajectory
-@pytest.mark.parametrize("name", ["bench16", "c1"])
+@pytest.mark.parametrize("name", ["bench16", "c1", "dtchange"])
 def test_oracle_matches_reference_trajectory(name):
-    """Energies every step and fields at the saved steps, from the reference's one_step."""
+    """Energies every step and fields at the saved steps, from the reference's
+    one_step with the returned dt passed to the next step; the dt sequence
+    itself (dtchange: five Courant-driven changes in 30 steps, each with a
+    rebuild and variable-step CNAB2 weights) is matched step by step."""
     case = json.load(open(os.path.join(GOLD, f"traj_{name}.json")))
     g = np.load(os.path.join(GOLD, f"traj_{name}.npz"))
     run = OracleRun(case["cfg"])
@@ -37,14 +40,18 @@ def test_oracle_matches_reference_trajectory(name):
         run.add_noise(case["noise"]["seed"], case["noise"]["amp"])
     run.setup_initial_state()
     run.initialize_dt(case["dt"])
+    dt = case["dt"]
     for n in range(1, case["steps"] + 1):
-        run.one_step(n, case["dt"])
+        dt = run.one_step(n, dt)
+        assert dt == pytest.approx(g["dt"][n - 1], rel=1e-12), (n, dt, g["dt"][n - 1])
         e = np.array(run.energies())
         assert np.all(np.abs(e - g["energies"][n - 1]) <= 1e-11 * np.abs(g["energies"][n - 1]))
         if n in case["field_steps"]:
             for k in ("s", "z", "w", "p", "b", "aj"):
                 ref = g[f"{k}_step{n}"]
                 assert np.abs(getattr(run, k) - ref).max() <= 1e-11 * np.abs(ref).max(), (n, k)
+    if name == "dtchange":
+        assert len(set(g["dt"].tolist())) >= 5   # the fixture does take the Courant branch
 
 
 def test_oracle_step1_kat_from_reference_log():

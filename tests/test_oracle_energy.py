@@ -104,3 +104,62 @@ def test_log_files_append_one_line_per_call(tmp_path):
     assert (tmp_path / "signal.t1").read_text() == "NOT\n"
     rad = (tmp_path / "radius.t1").read_text().splitlines()
     assert len(rad) == 5 and rad[0] == "   1  1.50000000E+00"
+
+
+# ------------------------------------------------------------ reference fixtures
+def _fixture_state(name, step):
+    import json
+    import os
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    case = json.load(open(os.path.join(gold, f"traj_{name}.json")))
+    g = np.load(os.path.join(gold, f"traj_{name}.npz"))
+    orc = OracleRun(case["cfg"])
+    for k in ("w", "dw", "z", "b", "db", "aj"):
+        getattr(orc, k)[:] = g[f"{k}_step{step}"]
+    return orc, g
+
+
+@pytest.mark.parametrize("name,step", [("c1", 100), ("dtchange", 30)])
+def test_energy_components_match_reference_fixture(name, step):
+    """All 8 e_kin and 12 e_mag_oc columns of the reference's get_e_kin_full /
+    get_e_mag_oc_full on its own fields."""
+    orc, g = _fixture_state(name, step)
+    ekin, emag = orc.energy_components()
+    np.testing.assert_allclose(ekin, g[f"ekin_full_step{step}"], rtol=1e-12, atol=0)
+    np.testing.assert_allclose(emag, g[f"emag_full_step{step}"], rtol=1e-12, atol=1e-300)
+
+
+@pytest.mark.parametrize("name,step", [("c1", 100), ("dtchange", 30)])
+def test_get_dlm_matches_reference_fixture(name, step):
+    """Length scales of the reference's get_dlm (output.py:750-861) for the
+    flow and the field: the four scalars and the three radial profiles."""
+    orc, g = _fixture_state(name, step)
+    for sw in ("V", "B"):
+        dl, dlR, dm, dlc, dlPolPeak, dlRc, dlPolPeakR = orc.get_dlm(sw)
+        np.testing.assert_allclose([dl, dm, dlc, dlPolPeak], g[f"dlm{sw}_scalars_step{step}"],
+                                   rtol=1e-12, atol=0)
+        np.testing.assert_allclose(np.stack([dlR, dlRc, dlPolPeakR]),
+                                   g[f"dlm{sw}_radial_step{step}"], rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("name,step", [("c1", 100), ("dtchange", 30)])
+def test_spectra_against_reference_length_scales_and_energies(name, step):
+    """The per-degree and per-order spectra: each sums to the reference's
+    energy (Parseval over l and over m), and their first moments give the
+    reference's get_dlm scales, dl = pi sum_l E_l / sum_l l E_l over l >= 1
+    (flow) or l >= 2 (field), and dm likewise over m >= 1.  A wrong weight,
+    index or bin in the spectra fails the moments."""
+    orc, g = _fixture_state(name, step)
+    sp = orc.spectra()
+    ek, em = g[f"ekin_full_step{step}"], g[f"emag_full_step{step}"]
+    for key, tot in (("kin", ek[0] + ek[1]), ("mag", em[0] + em[1])):
+        assert sp[key + "_l"].sum() == pytest.approx(tot, rel=1e-12)
+        assert sp[key + "_m"].sum() == pytest.approx(tot, rel=1e-12)
+    for key, sw, l0 in (("kin", "V", 1), ("mag", "B", 2)):
+        el, emm = sp[key + "_l"], sp[key + "_m"]
+        ls, ms = np.arange(len(el)), np.arange(len(emm))
+        dl = np.pi * el[l0:].sum() / (ls[l0:] * el[l0:]).sum()
+        dm = np.pi * emm[1:].sum() / (ms[1:] * emm[1:]).sum()
+        ref = g[f"dlm{sw}_scalars_step{step}"]
+        assert dl == pytest.approx(ref[0], rel=1e-12)
+        assert dm == pytest.approx(ref[1], rel=1e-12)

@@ -97,3 +97,45 @@ def test_operators_fit_band_and_match_lapack_pivots(ob):
         _, ipiv, info2 = lapack.dgbtrf(ab, k, k)
         assert info2 == 0
         assert np.array_equal(np.asarray(piv), ipiv + 1)
+
+
+# ------------------------------------------------------------ reference fixtures
+def _fd_fixture(ob):
+    import os
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    return np.load(os.path.join(gold, f"fd4_ob{ob}.npz"))
+
+
+@pytest.mark.parametrize("ob", [2, 3])
+def test_operators_match_reference_factored_bands(ob):
+    """The reference's own FD build (build_s_matrices, build_z_matrices,
+    build_w_matrices at fd_order 4, n_r_max 64, l_max 12, strat 3, polind 2,
+    g2 1, wimp_lin0 6e-5; tests/golden/fd4_ob*.npz): grid, polytropic profiles,
+    and per degree the LINPACK-factored band, the pivots and the row (and for W
+    column) preconditioning factors."""
+    g = _fd_fixture(ob)
+    n, order = 64, 4
+    r, rcmb = _grid(n)
+    assert np.abs(r - g["r"]).max() <= 1e-15 * rcmb
+    prof = fd.polytropic(r, rcmb, rcmb - 1.0, 0.35, 3.0, 2.0, 0.0, 0.0, 1.0)
+    for k in ("rho0", "temp0", "beta", "dbeta", "ddbeta", "dLtemp0", "orho1", "rgrav"):
+        assert np.abs(prof[k] - g["prof_" + k]).max() <= 1e-14 * np.abs(g["prof_" + k]).max(), k
+    mats = fd.fd_der_mats(r, fd.fd_coeffs(r, order, ob), order, ob)
+    hs, hw = fd.band_widths(order, ob)
+    assert tuple(g["S_k"]) == tuple(g["Z_k"]) == (hs, hs) and tuple(g["W_k"]) == (hw, hw)
+    for l in range(13):
+        ops = [("S", fd.s_operator(l, 6e-5, 1.0, mats, prof), hs)]
+        if l >= 1:   # the reference keeps identity bands for Z and W at l = 0
+            ops += [("Z", fd.z_operator(l, 6e-5, mats, prof), hs),
+                    ("W", fd.w_operator(l, 6e-5, mats, prof), hw)]
+        for name, op, k in ops:
+            lu, piv, info = prepare_band(dense_to_band_storage(op[0], k, k).copy(), n, k, k)
+            assert info == 0
+            assert np.array_equal(np.asarray(piv), g[f"{name}_piv"][l]), (name, l)
+            ref = g[f"{name}_abd"][l]
+            assert np.abs(lu - ref).max() <= 1e-13 * np.abs(ref).max(), (name, l)
+            fr = g["W_fac_row" if name == "W" else f"{name}_fac"][l]
+            assert np.abs(op[1] - fr).max() <= 1e-14 * np.abs(fr).max(), (name, l)
+            if name == "W":
+                fc = g["W_fac_col"][l]
+                assert np.abs(op[2] - fc).max() <= 1e-14 * np.abs(fc).max(), l

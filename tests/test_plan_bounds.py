@@ -176,3 +176,27 @@ def test_run_is_freed_with_its_last_reference(cpu_standin):
     finally:
         if was:
             gc.enable()
+
+
+def test_mixed_precision_tables_are_converted_once_outside_the_plans(cpu_standin):
+    """The FP32 copies of the Legendre tables are made when a mixed-precision
+    run is built (four conversions), never inside a recorded step plan, where
+    they would run again with every replay while other runs read the tables."""
+    cfg = dict(l_max=16, n_r_max=17, ek=1e-3, ra=1e5, pr=1.0, prmag=5.0, radratio=0.35,
+               dtmax=1e-4, precision="mixed")
+    with _lib.record() as built:
+        run = dyn.DynamoRun(cfg, device="cpu")
+        run.initialize_dt(cfg["dtmax"])
+        run._ctl_push(cfg["dtmax"])
+    assert sum(name == "mh_tables_to_f32" for _, name, _ in built.calls) == 4
+    for kind in ("radial", "lm", "step"):
+        for par in (0, 1):
+            run._par = par
+            _, plan = run._plan(kind)
+            assert not any(name == "mh_tables_to_f32" for _, name, _ in plan), (kind, par)
+            if kind != "lm":
+                assert any(name == "mh_leg_synth" for _, name, _ in plan)
+    # a second run of the resolution reuses the same tables
+    with _lib.record() as again:
+        dyn.DynamoRun(cfg, device="cpu", res=run.res)
+    assert not any(name == "mh_tables_to_f32" for _, name, _ in again.calls)

@@ -11,7 +11,10 @@
   hand sides of all degrees in one launch.
 
 Inputs may live on the CPU; they are moved to the GPU and results are
-returned on the GPU.
+returned on the GPU.  Shapes and degree indices are checked on the host
+before any launch (``ValueError`` for a shape that does not fit,
+``IndexError`` for a degree index outside the factor sets, as the reference's
+indexing raises), so no kernel reads past a buffer.
 """
 
 from __future__ import annotations
@@ -27,14 +30,31 @@ def _dev():
     return require_gpu("cuda")
 
 
+def _check_index(lidx, L, nl, what):
+    if lidx.ndim != 1 or lidx.shape[0] != L:
+        raise ValueError(f"{what}: l_index must have one entry per right-hand-side row "
+                         f"({L}), got shape {tuple(lidx.shape)}")
+    if L and (lidx.min() < 0 or lidx.max() >= nl):
+        bad = int(lidx[(lidx < 0) | (lidx >= nl)][0])
+        raise IndexError(f"{what}: degree index {bad} out of range for {nl} factor sets")
+
+
 def chunked_solve_complex(inv_by_l, l_index, rhs_complex, chunk_size=None):
     """(L, n) complex solution of inv_by_l[l_index[lm]] @ rhs[lm] for every row lm."""
-    dev = _dev()
-    inv = torch.as_tensor(inv_by_l, dtype=F64).to(dev)
-    rhs = torch.as_tensor(rhs_complex).to(dev).to(C128).contiguous()
+    inv = torch.as_tensor(inv_by_l, dtype=F64)
+    rhs = torch.as_tensor(rhs_complex)
     lidx = torch.as_tensor(l_index).cpu().numpy().astype(np.int64)
+    if inv.dim() != 3 or inv.shape[1] != inv.shape[2]:
+        raise ValueError(f"chunked_solve_complex: inv_by_l must be (nl, n, n), got "
+                         f"{tuple(inv.shape)}")
     nl, n, _ = inv.shape
+    if rhs.dim() != 2 or rhs.shape[1] != n:
+        raise ValueError(f"chunked_solve_complex: rhs must be (L, {n}), got {tuple(rhs.shape)}")
     L = rhs.shape[0]
+    _check_index(lidx, L, nl, "chunked_solve_complex")
+    dev = _dev()
+    inv = inv.to(dev)
+    rhs = rhs.to(dev).to(C128).contiguous()
     K4 = -(-n // 4) * 4
     Rp = -(-n // 64) * 64
     MT = torch.zeros(nl, K4, Rp, dtype=F64, device=dev)
@@ -83,10 +103,14 @@ def dense_to_band_storage(A_dense, kl: int, ku: int):
 
 def prepare_band_batched(abd_by_l, n: int, kl: int, ku: int):
     """Factor a batch (B, 2kl+ku+1, n) in place on the GPU; returns (abd, pivot int32, info)."""
-    dev = _dev()
-    abd = torch.as_tensor(abd_by_l, dtype=F64).to(dev).contiguous().clone()
+    abd = torch.as_tensor(abd_by_l, dtype=F64)
     if abd.dim() == 2:
         abd = abd.unsqueeze(0)
+    if abd.dim() != 3 or tuple(abd.shape[1:]) != (2 * kl + ku + 1, n):
+        raise ValueError(f"prepare_band: abd must be ({2 * kl + ku + 1}, {n}) per matrix, got "
+                         f"{tuple(abd.shape)}")
+    dev = _dev()
+    abd = abd.to(dev).contiguous().clone()
     nb = abd.shape[0]
     piv = torch.zeros(nb, n, dtype=I32, device=dev)
     info = torch.zeros(nb, dtype=I32, device=dev)
@@ -103,11 +127,31 @@ def prepare_band(abd, n: int, kl: int, ku: int):
 def banded_solve_by_l(abd_by_l, pivot_by_l, l_index, rhs_complex, n, kl, ku,
                       fac_row_by_l=None, fac_col_by_l=None):
     """Every row lm solved with the factors of degree l_index[lm] (``algebra.py:672-723``)."""
+    if not (0 <= kl <= 4 and 0 <= ku <= 4):
+        raise ValueError(f"banded_solve_by_l: kl, ku <= 4 supported, got {kl}, {ku}")
+    abd = torch.as_tensor(abd_by_l, dtype=F64)
+    piv = torch.as_tensor(pivot_by_l)
+    if abd.dim() != 3 or tuple(abd.shape[1:]) != (2 * kl + ku + 1, n):
+        raise ValueError(f"banded_solve_by_l: abd_by_l must be (nl, {2 * kl + ku + 1}, {n}), "
+                         f"got {tuple(abd.shape)}")
+    nl = abd.shape[0]
+    if tuple(piv.shape) != (nl, n):
+        raise ValueError(f"banded_solve_by_l: pivot_by_l must be ({nl}, {n}), got "
+                         f"{tuple(piv.shape)}")
+    lidx_h = torch.as_tensor(l_index).cpu().numpy().astype(np.int64).reshape(-1)
+    rhs = torch.as_tensor(rhs_complex)
+    if rhs.dim() != 2 or rhs.shape[1] != n:
+        raise ValueError(f"banded_solve_by_l: rhs must be (L, {n}), got {tuple(rhs.shape)}")
+    _check_index(lidx_h, rhs.shape[0], nl, "banded_solve_by_l")
+    for name, fac in (("fac_row_by_l", fac_row_by_l), ("fac_col_by_l", fac_col_by_l)):
+        if fac is not None and tuple(torch.as_tensor(fac).shape) != (nl, n):
+            raise ValueError(f"banded_solve_by_l: {name} must be ({nl}, {n}), got "
+                             f"{tuple(torch.as_tensor(fac).shape)}")
     dev = _dev()
-    abd = torch.as_tensor(abd_by_l, dtype=F64).to(dev).contiguous()
-    piv = torch.as_tensor(pivot_by_l).to(dev).to(I32).contiguous()
-    lidx = torch.as_tensor(l_index).to(dev).to(I32).contiguous()
-    rhs = torch.as_tensor(rhs_complex).to(dev)
+    abd = abd.to(dev).contiguous()
+    piv = piv.to(dev).to(I32).contiguous()
+    rhs = rhs.to(dev)
+    lidx = torch.as_tensor(lidx_h.astype(np.int32)).to(dev).contiguous()
     real = not rhs.is_complex()
     rhs = rhs.to(C128).contiguous()
     fr = None if fac_row_by_l is None else torch.as_tensor(fac_row_by_l, dtype=F64).to(dev).contiguous()
@@ -121,8 +165,11 @@ def banded_solve_by_l(abd_by_l, pivot_by_l, l_index, rhs_complex, n, kl, ku,
 
 def solve_band_real(abd, n: int, kl: int, ku: int, pivot, rhs):
     """dgbsl (``algebra.py:460-506``): rhs (n,) or (n, k), real or complex."""
+    rhs = torch.as_tensor(rhs)
+    if rhs.dim() not in (1, 2) or rhs.shape[0] != n:
+        raise ValueError(f"solve_band_real: rhs must be ({n},) or ({n}, k), got {tuple(rhs.shape)}")
     dev = _dev()
-    rhs = torch.as_tensor(rhs).to(dev)
+    rhs = rhs.to(dev)
     vec = rhs.dim() == 1
     cols = rhs.unsqueeze(1) if vec else rhs
     rows = cols.transpose(0, 1).contiguous()             # (k, n) rows

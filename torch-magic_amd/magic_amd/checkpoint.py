@@ -137,6 +137,12 @@ def apply_checkpoint_state(run, state: dict) -> tuple[int, float]:
     run.dt_hist = [float(dt[0]), float(dt[1])]
     run.time = float(state["sim_time"])
     run.n_step = int(state["step"])
+    # device step control restarts from the restored state: the next ctl_begin
+    # pushes it (no queued or pulled steps of the state before the load remain)
+    if getattr(run, "_ctl", None) is not None:
+        run._ctl["pulled"] = run._ctl["queued"] = run.n_step
+    if hasattr(run, "dt_log"):
+        del run.dt_log[run.n_step:]
     # operators for the restored dt, as the reference rebuilds after a load
     run.build_all_matrices(run.p.alpha * run.dt_hist[0])
     return run.n_step, run.time

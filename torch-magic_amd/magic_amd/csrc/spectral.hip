@@ -281,10 +281,55 @@ __global__ __launch_bounds__(256) void k_energy(EnergyArgs a) {
   }
 }
 
+// by_m 3: radial profiles per degree, summed over the orders of each degree
+// and not integrated, for the length scales of get_dlm (output.py:750-861):
+// out[(q*(l_max+1) + l)*N + r] for q = kinetic poloidal, toroidal, the same
+// two over m > 0 only, then the four magnetic ones (times LFfac).  One
+// workgroup per degree, consecutive lanes on consecutive radial levels.
+__global__ __launch_bounds__(128) void k_energy_lr(EnergyArgs a) {
+  const int l = blockIdx.x;
+  const int N = a.N;
+  const double dLh = (double)l * (double)(l + 1);
+  const long plane = (long)(a.l_max + 1) * N;
+  for (int r = threadIdx.x; r < N; r += blockDim.x) {
+    const double or2 = a.or2[r];
+    double e[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) e[q] = 0.0;
+    for (int m = 0; m <= l; ++m) {
+      const long idx = (long)(a.lm_start[m] + (l - m)) * N + r;
+      const bool m0 = m == 0;
+      const double kp = 0.5 * dLh * (dLh * or2 * cc2real(C2(a.w)[idx], m0) + cc2real(C2(a.dw)[idx], m0));
+      const double kt = 0.5 * dLh * cc2real(C2(a.z)[idx], m0);
+      e[0] += kp;
+      e[1] += kt;
+      e[2] += m0 ? 0.0 : kp;
+      e[3] += m0 ? 0.0 : kt;
+      if (a.l_mag) {
+        const double mp = 0.5 * a.LFfac * dLh * (dLh * or2 * cc2real(C2(a.b)[idx], m0) + cc2real(C2(a.db)[idx], m0));
+        const double mt = 0.5 * a.LFfac * dLh * cc2real(C2(a.aj)[idx], m0);
+        e[4] += mp;
+        e[5] += mt;
+        e[6] += m0 ? 0.0 : mp;
+        e[7] += m0 ? 0.0 : mt;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) a.out[q * plane + (long)l * N + r] = e[q];
+  }
+}
+
 extern "C" int mh_energy(const MhEnergyArgs* args, void* stream) {
   const EnergyArgs& a = *args;
-  if (a.by_m < 0 || a.by_m > 2 || a.l_max < 0 || a.N < 1) return MH_EINVAL;
-  hipLaunchKernelGGL(k_energy, dim3(a.l_max + 1), dim3(256), 0, mh_stream(stream), a);
+  if (a.by_m < 0 || a.by_m > 3 || a.l_max < 0 || a.N < 1 || a.out == nullptr ||
+      a.w == nullptr || a.dw == nullptr || a.z == nullptr || a.lm_start == nullptr ||
+      a.or2 == nullptr || (a.by_m != 3 && a.qw == nullptr) ||
+      (a.l_mag && (a.b == nullptr || a.db == nullptr || a.aj == nullptr)))
+    return MH_EINVAL;
+  if (a.by_m == 3)
+    hipLaunchKernelGGL(k_energy_lr, dim3(a.l_max + 1), dim3(128), 0, mh_stream(stream), a);
+  else
+    hipLaunchKernelGGL(k_energy, dim3(a.l_max + 1), dim3(256), 0, mh_stream(stream), a);
   return mh_launch_status();
 }
 

@@ -9,7 +9,7 @@
 //    (courant.py:80-127); their minima; the hysteresis decision of
 //    dt_courant (courant.py:130-155); the optional replayed step sizes; the
 //    dt roll and rebuild decision of one_step (step_time.py:1574-1637,
-//    rebuild on the first step, on a new dt or a new implicit weight); the
+//    rebuild on the first step or a new dt, as step_time.py:1628-1629); the
 //    CNAB2 weights of set_weights (time_scheme.py:56-109); the first singular
 //    system of the previous step's rebuild, if any.  Every expression
 //    is evaluated in the order the host evaluates it, with no fused
@@ -91,7 +91,9 @@ __global__ __launch_bounds__(256) void k_step_control(MhStepCtlArgs a) {
   if (a.dt_log != nullptr && a.log_cap > 0) a.dt_log[(n - 1) % a.log_cap] = dt_new;
   const double dt1 = dt_new, dt2 = st->dt1;
   const double wl0 = a.alpha * dt_new;
-  const bool rebuild = (n == 1) || new_dt || !(st->wimp_lin0 == wl0);
+  // step_time.py:1628-1629: rebuild on the first step or a new dt (and once
+  // when no operators were built yet: wimp_lin0 is NaN)
+  const bool rebuild = (n == 1) || new_dt || __builtin_isnan(st->wimp_lin0);
   if (rebuild) {
     st->wimp_lin0 = wl0;
     st->n_rebuild += 1;

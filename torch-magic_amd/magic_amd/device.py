@@ -93,8 +93,14 @@ class Resolution:
         torch.cuda.current_stream(dev).synchronize()
 
     def f32_tables(self):
-        """FP32 copies of the four Legendre tables (built on first use)."""
+        """FP32 copies of the four Legendre tables, shared by every run of the
+        resolution.  Built eagerly (a run or transform in mixed precision asks
+        for them when it is created): never inside a recorded launch plan, where
+        the conversion would be replayed with every step instead of run once."""
         if getattr(self, "_t32", None) is None:
+            if getattr(_lib.recording(), "plan", False):
+                raise RuntimeError("f32_tables: the FP32 tables must be built before "
+                                   "a launch plan is recorded")
             t = {}
             for k, src in (("synP", self.synP), ("synD", self.synD), ("anaW", self.anaW),
                            ("anaWd", self.anaWd)):

@@ -277,6 +277,8 @@ class DynamoRun:
         # and (fused grid path) FP32 Fourier buffers, azimuthal FFTs and nonlinear
         # products; solves, radial derivatives, IMEX terms and diagnostics stay FP64
         self.leg_prec = 1 if p.precision == "mixed" else 0
+        if self.leg_prec:
+            r.f32_tables()   # shared FP32 tables, converted once, outside any plan
         self.grid_generic = False  # force the run-time FFT plan of the fused kernel
         fdt = torch.float32 if (self.leg_prec and fused_grid) else F64
         self.F_s = r.fourier_buffer(1, N, fdt)
@@ -336,6 +338,7 @@ class DynamoRun:
         self.dt_schedule = None  # optional step sizes to use instead of the Courant decision
         self.n_step = 0          # steps taken
         self._ctl = None         # device step control buffers (run_steps / step_async)
+        self.expl_pending = False  # a radial loop's explicit terms await the lm loop
 
     # ------------------------------------------------------------ timing
     def enable_timers(self, on: bool = True):
@@ -360,6 +363,7 @@ class DynamoRun:
         plan = self._plans.get(key)
         if plan is None:
             with _lib.record() as rec:
+                rec.plan = True   # a step's launch plan (replayed every step)
                 if kind == "radial":
                     self._radial_launches()
                 elif kind == "lm":
@@ -783,6 +787,7 @@ class DynamoRun:
     def radial_loop(self):
         """Explicit terms of the current state; returns (dtrkc_min, dthkc_min)."""
         self._exec("radial")
+        self.expl_pending = True
         return self._cfl_decision()
 
     def _radial_launches(self):
@@ -903,6 +908,7 @@ class DynamoRun:
         self._exec("lm")
         self.expl_cur, self.expl_prev = self.expl_prev, self.expl_cur
         self._par ^= 1
+        self.expl_pending = False
 
     def _lm_launches(self):
         p = self.p
@@ -937,6 +943,7 @@ class DynamoRun:
     def begin_step(self):
         """Queue the radial loop and the copy of the Courant maxima (no host wait)."""
         self._exec("radial")
+        self.expl_pending = True
         self._cfl_copy()
 
     def finish_step(self, n_time_step: int, dt: float) -> float:
@@ -951,9 +958,11 @@ class DynamoRun:
         self.dt_log.append(dt_new)
         self.n_step = n_time_step
         self.dt_hist = [dt_new, self.dt_hist[0]]
-        wl0 = self.p.alpha * dt_new
-        if n_time_step == 1 or l_new_dt or self.wimp_lin0 != wl0:
-            self.build_all_matrices(wl0)
+        # the reference's rule (step_time.py:1628-1629): rebuild on the first
+        # step or when the Courant decision changed dt; a run whose operators
+        # were never built (the reference would fail there) builds them once
+        if n_time_step == 1 or l_new_dt or self.wimp_lin0 is None:
+            self.build_all_matrices(self.p.alpha * dt_new)
         self.lm_loop()
         self.time += dt_new
         return dt_new
@@ -1101,30 +1110,53 @@ class DynamoRun:
         return self.ctl_end()
 
     # ------------------------------------------------------------ diagnostics
-    def _energy_out(self, by_m: int):
-        out = torch.zeros(16 if by_m == 2 else 4, self.p.l_max + 1, dtype=F64, device=self.device)
+    def field_set(self, **given):
+        """{w, dw, z, b, db, aj}: the run's own field tensors, with any given
+        (lm_max, n_r_max) arrays in their place (moved to the run's device as
+        complex128 when they are not there already).  The diagnostics below
+        compute on these, as the reference's do on the tensors passed to them
+        (``output.py:111-193,750-861``)."""
+        fs = {k: self.f[k] for k in ("w", "dw", "z", "b", "db", "aj")}
+        shape = tuple(self.f["w"].shape)
+        for k, v in given.items():
+            if v is None:
+                continue
+            if k not in fs:
+                raise KeyError(k)
+            t = torch.as_tensor(v)
+            if tuple(t.shape) != shape:
+                raise ValueError(f"{k}: expected shape {shape}, got {tuple(t.shape)}")
+            fs[k] = t.to(device=self.device, dtype=C128).contiguous()
+        return fs
+
+    def _energy_out(self, by_m: int, fields=None, LFfac=None):
+        lp1 = self.p.l_max + 1
+        shape = {0: (4, lp1), 1: (4, lp1), 2: (16, lp1), 3: (8, lp1, self.N)}[by_m]
+        out = torch.zeros(shape, dtype=F64, device=self.device)
         a = _lib.EnergyArgs()
-        f = self.f
+        f = fields if fields is not None else self.f
         a.w, a.dw, a.z, a.b, a.db, a.aj = (ptr(f[k]) for k in ("w", "dw", "z", "b", "db", "aj"))
         a.lm_start = ptr(self.res.d["lm_start"])
         a.or2, a.qw, a.out = ptr(self.res.d["or2"]), ptr(self.res.d["q"]), ptr(out)
-        a.l_max, a.N, a.by_m, a.l_mag, a.LFfac = self.p.l_max, self.N, int(by_m), int(self.p.l_mag), self.p.LFfac
+        a.l_max, a.N, a.by_m, a.l_mag = self.p.l_max, self.N, int(by_m), int(self.p.l_mag)
+        a.LFfac = self.p.LFfac if LFfac is None else LFfac
         _lib.call("mh_energy", _lib.ref(a), stream_handle())
         return out
 
-    def energies(self):
+    def energies(self, fields=None):
         """(e_kin_pol, e_kin_tor, e_mag_pol, e_mag_tor) (``output.py:111-193``)."""
-        e = self._energy_out(0).cpu().numpy()
+        e = self._energy_out(0, fields).cpu().numpy()
         return tuple(float(x) for x in e.sum(axis=1))
 
-    def energy_components(self):
+    def energy_components(self, fields=None):
         """All components of the reference's e_kin.TAG (8) and e_mag_oc.TAG (12)
         lines (``output.py:111-193``): totals, axisymmetric (m = 0), equatorially
         symmetric ((l+m) even for kinetic poloidal and magnetic toroidal, odd for
         the other two), their m = 0 parts, and the poloidal magnetic energy of the
         potential field outside the shell from the CMB values.  Returns
         (ekin, emag) as tuples in the reference's column order."""
-        e = self._energy_out(2).cpu().numpy().reshape(4, 4, -1).sum(axis=2)  # [q][bin]
+        f = fields if fields is not None else self.f
+        e = self._energy_out(2, f).cpu().numpy().reshape(4, 4, -1).sum(axis=2)  # [q][bin]
         # bins: 0 m>0 (l+m) even, 1 m>0 odd, 2 m=0 l even, 3 m=0 l odd
         tot = e.sum(axis=1)
         as_ = e[:, 2] + e[:, 3]
@@ -1134,7 +1166,7 @@ class DynamoRun:
             r, maps = self.res, self.res.maps
             l = np.asarray(maps["l"], dtype=np.float64)
             m0 = np.asarray(maps["m"]) == 0
-            bc = self.f["b"][:, 0].cpu().numpy()  # radial index 0 is the CMB
+            bc = f["b"][:, 0].cpu().numpy()  # radial index 0 is the CMB
             cc = np.where(m0, bc.real ** 2, 2.0 * (bc.real ** 2 + bc.imag ** 2))
             w = 0.5 * self.p.LFfac / r.rad["r_cmb"] * l * l * (l + 1.0) * cc
             e_os, e_as_os = float(w.sum()), float(w[m0].sum())
@@ -1144,12 +1176,68 @@ class DynamoRun:
             emag = (0.0,) * 12
         return tuple(float(x) for x in ekin), tuple(float(x) for x in emag)
 
-    def spectra(self):
-        """Per-degree and per-order kinetic and magnetic energy spectra."""
-        el = self._energy_out(0).cpu().numpy()
-        em = self._energy_out(1).cpu().numpy()
+    def spectra(self, fields=None):
+        """Per-degree and per-order kinetic and magnetic energy spectra: the
+        radially integrated energies of each degree l and of each order m (the
+        per-l and per-m aggregation of ``get_dlm``, ``output.py:768-780``, with
+        the 1/2 and LFfac of ``get_e_kin_full`` / ``get_e_mag_oc_full``); each
+        sums to the total energy."""
+        el = self._energy_out(0, fields).cpu().numpy()
+        em = self._energy_out(1, fields).cpu().numpy()
         return dict(kin_l=el[0] + el[1], mag_l=el[2] + el[3], kin_m=em[0] + em[1],
                     mag_m=em[2] + em[3])
+
+    def get_dlm(self, switch="V", fields=None):
+        """Integral length scales of the flow (switch 'V': w, dw, z) or of the
+        field ('B': b, db, aj), as ``get_dlm`` (``output.py:750-861``):
+        (dl, dlR, dm, dlc, dlPolPeak, dlRc, dlPolPeakR); dlR, dlRc and
+        dlPolPeakR are numpy arrays over the radial levels.  The per-(l, r)
+        profiles and the per-order integrals come from the device reduction
+        (``mh_energy`` modes 3 and 1, the field's LFfac left out as the
+        reference does); the few (l_max+1) x n_r_max sums are formed here."""
+        if switch not in ("V", "B"):
+            raise ValueError(f"switch must be 'V' or 'B', not {switch!r}")
+        f = fields if fields is not None else self.f
+        if switch == "B":
+            f = dict(f, w=f["b"], dw=f["db"], z=f["aj"])   # the kinetic slots carry b
+        prof = self._energy_out(3, f, LFfac=1.0)[:4].cpu().numpy()   # halved, per (l, r)
+        per_m = self._energy_out(1, f, LFfac=1.0)[:2].cpu().numpy().sum(axis=0)
+        q = self.res.rad["q"]
+        l_max, N = self.p.l_max, self.N
+        l_first = 1 if switch == "V" else 2
+        e_lr = prof[0] + prof[1]           # 0.5 e_lr of output.py:768-772
+        e_pol_lr = prof[0]
+        e_lr_c = prof[2] + prof[3]
+        ls = np.arange(l_first, l_max + 1, dtype=np.float64)
+        E_l = e_lr[l_first:] @ q
+        E, EL = float(E_l.sum()), float((ls * E_l).sum())
+        dl = math.pi * E / EL if EL != 0.0 else 0.0
+        E_pol = np.zeros(l_max + 1)
+        E_pol[l_first:] = e_pol_lr[l_first:] @ q
+        lpeak = int(np.argmax(E_pol[1:])) + 1
+        dlPolPeak = math.pi / lpeak
+        eps = np.finfo(np.float64).eps
+        if switch == "V":
+            Ec_l = e_lr_c[l_first:] @ q
+            Ec, ELc = float(Ec_l.sum()), float((ls * Ec_l).sum())
+            dlc = math.pi * Ec / ELc if ELc != 0.0 else 0.0
+        else:
+            dlc = 0.0
+        ms = np.arange(self.p.minc, l_max + 1, self.p.minc)
+        E_m, EM_m = float(per_m[ms].sum()), float((ms * per_m[ms]).sum())
+        dm = math.pi * E_m / EM_m if EM_m != 0.0 else 0.0
+        if switch == "V":
+            def scale(e):
+                er = e[l_first:].sum(axis=0)
+                elr = (ls[:, None] * e[l_first:]).sum(axis=0)
+                return np.where(elr > 10.0 * eps, math.pi * er / np.where(elr != 0, elr, 1.0), 0.0)
+            dlR, dlRc = scale(e_lr), scale(e_lr_c)
+            per_r = 2.0 * e_pol_lr[1:]      # the reference compares the unhalved values
+            mx, ix = per_r.max(axis=0), per_r.argmax(axis=0) + 1
+            dlPolPeakR = np.where(mx > 10.0 * eps, math.pi / ix, 0.0)
+        else:
+            dlR, dlRc, dlPolPeakR = np.zeros(N), np.zeros(N), np.zeros(N)
+        return dl, dlR, dm, dlc, dlPolPeak, dlRc, dlPolPeakR
 
 
 def dt_courant(dt, dtMax, dtrkc_min, dthkc_min):

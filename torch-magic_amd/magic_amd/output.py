@@ -1,5 +1,11 @@
 """Energy diagnostics and their log lines with the reference's names
-(``output.py:76-80,111-193,304-320,486-501``)."""
+(``output.py:76-80,101-193,304-320,486-501,750-861``).
+
+Every function computes on the tensors it is given, as the reference's do:
+(lm_max, n_r_max) complex arrays of the configured resolution, on any device
+(moved to the GPU for the reduction), reduced by the device kernel
+``mh_energy``.  The default run supplies the resolution (radial grid,
+quadrature weights, lm maps)."""
 
 from __future__ import annotations
 
@@ -12,34 +18,55 @@ EMagOC = namedtuple("EMagOC", "e_p e_t e_p_as e_t_as e_p_os e_p_as_os "
                               "e_p_es e_t_es e_p_eas e_t_eas e_p_e e_p_as_e")
 
 
-def get_e_kin_full(w=None, dw=None, z=None):
-    """The eight kinetic-energy components of the default run's current state
-    (total, axisymmetric, equatorially symmetric and its m = 0 part, each
-    poloidal and toroidal)."""
-    return EKin(*config.default_run().energy_components()[0])
+def get_e_kin_full(w, dw, z):
+    """The eight kinetic-energy components of (w, dw, z) (reference
+    ``output.py:111-139``): total, axisymmetric, equatorially symmetric and its
+    m = 0 part, each poloidal and toroidal."""
+    run = config.default_run()
+    return EKin(*run.energy_components(run.field_set(w=w, dw=dw, z=z))[0])
 
 
-def get_e_mag_oc_full(b=None, db=None, aj=None):
-    """The twelve outer-core magnetic-energy components of the default run
-    (the external-field pair is zero, as in the reference's benchmarks)."""
-    return EMagOC(*config.default_run().energy_components()[1])
+def get_e_mag_oc_full(b, db, aj):
+    """The twelve outer-core magnetic-energy components of (b, db, aj)
+    (reference ``output.py:152-193``; the external-field pair is zero there)."""
+    run = config.default_run()
+    if not run.p.l_mag:
+        # the reference's formulas need no flag: run them on the field given
+        raise ValueError("get_e_mag_oc_full: the configured run has no magnetic field "
+                         "(mode 1 or 7)")
+    return EMagOC(*run.energy_components(run.field_set(b=b, db=db, aj=aj))[1])
 
 
-def get_e_kin(w=None, dw=None, ddw=None, z=None, dz=None):
-    """(e_kin_pol, e_kin_tor)."""
-    ek = get_e_kin_full()
+def get_e_kin(w, dw, ddw, z, dz):
+    """(e_kin_pol, e_kin_tor) of the given fields (reference ``output.py:101-108``)."""
+    ek = get_e_kin_full(w, dw, z)
     return ek.e_p, ek.e_t
 
 
-def get_e_mag(b=None, db=None, ddb=None, aj=None, dj=None):
-    """(e_mag_pol, e_mag_tor)."""
-    em = get_e_mag_oc_full()
+def get_e_mag(b, db, ddb, aj, dj):
+    """(e_mag_pol, e_mag_tor) of the given fields (reference ``output.py:142-149``)."""
+    em = get_e_mag_oc_full(b, db, aj)
     return em.e_p, em.e_t
 
 
-def get_spectra():
-    """Per-degree and per-order kinetic/magnetic energy spectra."""
-    return config.default_run().spectra()
+def get_dlm(w, dw, z, switch="V"):
+    """Integral length scales (dl, dlR, dm, dlc, dlPolPeak, dlRc, dlPolPeakR)
+    of the flow (switch 'V', arguments w, dw, z) or of the field ('B', arguments
+    b, db, aj) (reference ``output.py:750-861``); dlR, dlRc and dlPolPeakR are
+    (n_r_max,) float64 tensors, the rest floats."""
+    import torch
+    run = config.default_run()
+    fs = run.field_set(w=w, dw=dw, z=z) if switch == "V" else run.field_set(b=w, db=dw, aj=z)
+    dl, dlR, dm, dlc, pk, dlRc, pkR = run.get_dlm(switch, fs)
+    t = lambda a: torch.as_tensor(a, dtype=torch.float64)
+    return dl, t(dlR), dm, dlc, pk, t(dlRc), t(pkR)
+
+
+def get_spectra(w=None, dw=None, z=None, b=None, db=None, aj=None):
+    """Per-degree and per-order kinetic/magnetic energy spectra of the given
+    fields (the default run's own where an argument is None)."""
+    run = config.default_run()
+    return run.spectra(run.field_set(w=w, dw=dw, z=z, b=b, db=db, aj=aj))
 
 
 # --- log lines in the reference's Fortran formats (a file format: kept alike) ---

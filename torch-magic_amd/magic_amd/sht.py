@@ -10,6 +10,11 @@ libmagic_hip: FP64 MFMA Legendre contractions and LDS Stockham FFTs.
 
 The module-level functions use the resolution of the configured run
 (:func:`magic_amd.config.configure`); :class:`SHT` binds any resolution.
+
+``lcut`` is accepted and has no effect, exactly as in the reference: its four
+transforms set ``lcut = l_max`` when it is None and never read it again
+(``sht.py:308-309,361-362,420-421,586-587``), so every degree up to l_max
+always takes part; a drop-in keeps that result.
 """
 
 from __future__ import annotations
@@ -51,6 +56,8 @@ class SHT:
             raise ValueError(f"precision must be 'fp64' or 'mixed', not {precision!r}")
         self.res = res
         self.prec = 1 if precision == "mixed" else 0
+        if self.prec:
+            res.f32_tables()
 
     def _on(self, x):
         return x.to(self.res.device) if x.device != self.res.device else x
