@@ -38,7 +38,8 @@ def test_module_level_facade_reproduces_reference_c1():
     init_fields.initialize_fields()
     run = config.default_run()
     coef, shape = noise_factors(run.p, run.res, case["noise"]["seed"], case["noise"]["amp"])
-    fields.s_LMloc += torch.from_numpy(coef[:, None] * shape[None, :])   # the reference's idiom
+    # the reference's idiom: an in-place update of the module-level field
+    fields.s_LMloc += torch.from_numpy(coef[:, None] * shape[None, :]).to(fields.s_LMloc.device)
     assert fields.s_LMloc.data_ptr() == run.f["s"].data_ptr()
     step_time.setup_initial_state()
     step_time.initialize_dt(case["dt"])

@@ -137,6 +137,12 @@ def apply_checkpoint_state(run, state: dict) -> tuple[int, float]:
     run.dt_hist = [float(dt[0]), float(dt[1])]
     run.time = float(state["sim_time"])
     run.n_step = int(state["step"])
+    # Courant cap: a run that never had initialize_dt (a fresh process) takes the
+    # configured dtmax, clamped as initialize_dt clamps it.  The reference keeps
+    # its cap at 0.0 there (step_time.py:1551), so its first decision after such
+    # a restart would be dt = 0.
+    if not run.dtMax:
+        run.dtMax = run.dt_cap(run.p.dtmax)
     # device step control restarts from the restored state: the next ctl_begin
     # pushes it (no queued or pulled steps of the state before the load remain)
     if getattr(run, "_ctl", None) is not None:

@@ -760,10 +760,15 @@ class DynamoRun:
             self.impl[k][:, 0] = 0.0
             self.impl[k][:, -1] = 0.0
 
-    def initialize_dt(self, dt: float):
-        """dt <= intfac * E for rotating runs (``step_time.py:1554-1571``)."""
+    def dt_cap(self, dt: float) -> float:
+        """dt clamped to intfac * E for rotating runs (``step_time.py:1564-1570``)."""
         if not self.p.ek < 0.0:
             dt = min(dt, self.p.int_fac * self.p.ek)
+        return dt
+
+    def initialize_dt(self, dt: float):
+        """dt <= intfac * E for rotating runs (``step_time.py:1554-1571``)."""
+        dt = self.dt_cap(dt)
         self.dt_hist = [dt, dt]
         self.dtMax = dt
 
