@@ -26,6 +26,7 @@ extern "C" {
 #define MH_OK 0
 #define MH_EINVAL -1
 #define MH_ELAUNCH -2
+#define MH_ETOOBIG -3 /* mh_gj_inverse_noalloc: sizes beyond its fixed work space */
 
 /* ------------------------------------------------------------ Legendre */
 
@@ -317,6 +318,23 @@ int mh_build_matrices(const MhBuildArgs* args, void* stream);
  * when: optional, as in MhBuildArgs. */
 int mh_gj_inverse(const double* A, double* out, double* work, double* scal, int n, int nbatch,
                   int precond, int* info, const int* when, void* stream);
+
+/* Capacity, in doubles, of the device work space of mh_gj_inverse_noalloc
+ * (32 MiB, held on every device the library runs on until the process ends). */
+#define MH_GJ_SCRATCH_DOUBLES (4L << 20)
+#define MH_MAX_DEVICES 64
+
+/* mh_gj_inverse without a work-space argument and without any allocation, for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * serving loops that forbid hipMalloc).  No counterpart in the reference.
+ * Systems of n > 208 use a fixed device array of MH_GJ_SCRATCH_DOUBLES doubles:
+ * nbatch * (2n^2 + 2n) must fit, else MH_ETOOBIG and nothing is launched
+ * (n <= 208 needs no work space, any nbatch).  All calls on one device share
+ * that array: calls whose launches overlap on one device (several streams)
+ * race; stream-ordered calls do not.  Results are bit-identical to
+ * mh_gj_inverse's. */
+int mh_gj_inverse_noalloc(const double* A, double* out, int n, int nbatch, int precond,
+                          int* info, const int* when, void* stream);
 
 typedef struct {
   const double* A;

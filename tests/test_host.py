@@ -148,6 +148,22 @@ def test_library_exports_header_symbols():
     _lib.load()   # struct sizes agree with the binding
 
 
+def test_noalloc_inverse_checks_sizes_before_touching_a_device():
+    """mh_gj_inverse_noalloc refuses a batch beyond its fixed work space (and bad
+    sizes) on the host, before any HIP call, so this runs without a GPU."""
+    from magic_amd import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("library not built")
+    lib = _lib.load()
+    n = 300
+    fit = _lib.GJ_SCRATCH_DOUBLES // (2 * n * n + 2 * n)
+    assert lib.mh_gj_inverse_noalloc(None, None, n, fit + 1, 1, None, None, None) == -3
+    assert lib.mh_gj_inverse_noalloc(None, None, 0, 1, 1, None, None, None) == -1
+    assert lib.mh_gj_inverse_noalloc(None, None, n, 0, 1, None, None, None) == -1
+    with pytest.raises(_lib.MagicHipError, match="work space"):
+        _lib.call("mh_gj_inverse_noalloc", None, None, n, fit + 1, 1, None, None, None)
+
+
 def test_grid_step_rejects_offsets_beyond_32_bits():
     """mh_grid_step forms buffer offsets in 32 bits with 24-bit multiplies and
     refuses, before any launch, shapes whose per-order stride (T*4N*2 elements)

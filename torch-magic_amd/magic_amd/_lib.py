@@ -160,6 +160,7 @@ _SIGS = {
     "mh_energy": [P, P],
     "mh_build_matrices": [P, P],
     "mh_gj_inverse": [P, P, P, P, I, I, I, P, P, P],
+    "mh_gj_inverse_noalloc": [P, P, I, I, I, P, P, P],
     "mh_gemm_f64": [P, P],
     "mh_step_control": [P, P],
     "mh_fill_f64": [P, L, I, L, D, P, P],
@@ -169,6 +170,11 @@ _SIGS = {
 }
 
 EXPORTED = tuple(_SIGS)
+
+GJ_SCRATCH_DOUBLES = 4 << 20   # MH_GJ_SCRATCH_DOUBLES
+_ERRORS = {-1: "invalid argument", -2: "launch failure",
+           -3: f"sizes exceed the fixed work space ({GJ_SCRATCH_DOUBLES} doubles: "
+               "nbatch * (2n^2 + 2n) must fit)"}
 
 _lib = None
 
@@ -234,8 +240,7 @@ def call(name: str, *args):
         return 0
     rc = getattr(load(), name)(*args)
     if rc != 0:
-        kind = {-1: "invalid argument", -2: "launch failure"}.get(rc, f"code {rc}")
-        raise MagicHipError(f"{name}: {kind}")
+        raise MagicHipError(f"{name}: {_ERRORS.get(rc, f'code {rc}')}")
     return rc
 
 
@@ -257,7 +262,6 @@ def run_plan(calls, stream, on_section=None):
             cur = sec
         rc = getattr(lib, name)(*args, stream)
         if rc != 0:
-            kind = {-1: "invalid argument", -2: "launch failure"}.get(rc, f"code {rc}")
-            raise MagicHipError(f"{name}: {kind}")
+            raise MagicHipError(f"{name}: {_ERRORS.get(rc, f'code {rc}')}")
     if on_section is not None and cur is not None:
         on_section(cur, False)

@@ -12,6 +12,8 @@
 //  * k_gemm_f64: batched C = alpha * A B with general strides on FP64 MFMA,
 //    used to fold the DCT-I and the radial derivative matrices into the
 //    inverses.
+#include <mutex>
+
 #include "common.h"
 
 typedef MhBuildArgs BuildArgs;
@@ -438,6 +440,53 @@ extern "C" int mh_gj_inverse(const double* A, double* out, double* work, double*
   else
     hipLaunchKernelGGL(k_gj_inverse, dim3(nbatch), dim3(256), 0, mh_stream(stream), A, out, work,
                        scal, n, precond, info, when);
+  return mh_launch_status();
+}
+
+// ------------------------------------------- Gauss-Jordan without allocation
+// The generic kernel's work space (n x 2n matrix and 2n scale factors per
+// system) taken from a file-scope device array instead of caller buffers.
+// Host code never passes `scratch` itself to a kernel: its name is a host-side
+// stand-in.  The device address comes from hipGetSymbolAddress, looked up once
+// per device with that device current and kept in g_scratch_dev.
+static __device__ __align__(256) double scratch[MH_GJ_SCRATCH_DOUBLES];
+
+static std::mutex g_scratch_mu;
+static double* g_scratch_dev[MH_MAX_DEVICES];
+
+static int gj_scratch_ptr(double** out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MH_MAX_DEVICES) {
+    (void)hipGetLastError();
+    return MH_ELAUNCH;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_dev[dev] == nullptr) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || p == nullptr) {
+      (void)hipGetLastError();
+      return MH_ELAUNCH;
+    }
+    g_scratch_dev[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_dev[dev];
+  return MH_OK;
+}
+
+extern "C" int mh_gj_inverse_noalloc(const double* A, double* out, int n, int nbatch,
+                                     int precond, int* info, const int* when, void* stream) {
+  if (n < 1 || nbatch < 1) return MH_EINVAL;
+  if (n <= 208) return mh_gj_inverse(A, out, nullptr, nullptr, n, nbatch, precond, info, when, stream);
+  // work: nbatch * 2 n^2 doubles, then the scale factors: nbatch * 2 n
+  const long per = 2L * n * n + 2L * n;
+  if ((long)nbatch * per > (long)MH_GJ_SCRATCH_DOUBLES) return MH_ETOOBIG;
+  double* base = nullptr;
+  const int rc = gj_scratch_ptr(&base);
+  if (rc != MH_OK) return rc;
+  double* work = base;
+  double* scal = base + (long)nbatch * 2 * n * n;
+  hipLaunchKernelGGL(k_gj_inverse, dim3(nbatch), dim3(256), 0, mh_stream(stream), A, out, work,
+                     scal, n, precond, info, when);
   return mh_launch_status();
 }
 
