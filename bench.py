@@ -25,7 +25,9 @@ the finite-difference banded solves (FD4, n_r=512, l_max=255) and the 64-run ens
 against a sequential sample.  The CPU baseline is the FP64 numpy oracle on a
 bounded sample of the same workload.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision both|mixed|fp64]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+                       [--precision both|all|fp64|mixed|mixed_bf16|mixed_bf16x3]
+                       [--mixed mixed|mixed_bf16|mixed_bf16x3]
                        [--no-cpu-baseline] [--no-fd] [--no-ensemble] [--no-sht] [--no-peaks]
 
 The measured peaks (a device copy, FP64 and FP32 GEMMs) are reported beside the
@@ -66,6 +68,19 @@ PMC_TRAFFIC_BYTES = {
 PEAK_FP64_TFLOPS = 78.6
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
+PEAK_BF16_TFLOPS = 2500.0   # dense BF16 matrix rate
+
+MIXED = ("mixed", "mixed_bf16", "mixed_bf16x3")
+HEAD_MIXED = "mixed"
+LEG_SPLIT = {
+    "mixed": "Legendre synthesis/analysis on FP32 MFMA with FP32 accumulation",
+    "mixed_bf16": "Legendre synthesis/analysis on BF16 MFMA, tables and inputs split into two "
+                  "bf16 planes (hi*hi + hi*lo + lo*hi), FP32 accumulation",
+    "mixed_bf16x3": "Legendre synthesis/analysis on BF16 MFMA, tables and inputs split into three "
+                    "bf16 planes (the six products of plane pairs i + j < 3), FP32 accumulation",
+}
+DTYPE = {"fp64": "f64", "mixed": "f32+f64", "mixed_bf16": "bf16x2+f32+f64",
+         "mixed_bf16x3": "bf16x3+f32+f64"}
 
 
 def section_work(p, N, fbytes=16.0):
@@ -226,7 +241,7 @@ def sht_case():
     ev = lambda: torch.cuda.Event(enable_timing=True)
     out, grids = {}, {}
     flops = 4.0 * p.lm_max * (p.n_theta // 2) * B
-    for prec in ("fp64", "mixed"):
+    for prec in ("fp64",) + MIXED:
         sht = SHT(res, precision=prec)
         g = sht.scal_to_spat(dS)
         back = sht.scal_to_SH(g)                       # warm-up (tables, module load)
@@ -248,10 +263,11 @@ def sht_case():
                          roundtrip_rms_err=float(np.sqrt((err ** 2).mean()) / smax),
                          legendre_tflops_synthesis_upper=flops / (np.min(t["syn"]) * 1e-3) / 1e12)
         grids[prec] = g
-    d = (grids["mixed"] - grids["fp64"]).abs()
     gmax = grids["fp64"].abs().max()
-    out["mixed_grid_vs_fp64"] = dict(max_err=float(d.max() / gmax),
-                                     rms_err=float(d.pow(2).mean().sqrt() / gmax))
+    for prec in MIXED:
+        d = (grids[prec] - grids["fp64"]).abs()
+        out[f"{prec}_grid_vs_fp64"] = dict(max_err=float(d.max() / gmax),
+                                           rms_err=float(d.pow(2).mean().sqrt() / gmax))
     out["workload"] = "l_max=383, 576 x 1152 grid, 291 fields, coefficients N(0,1) (l+1)^-2"
     return out
 
@@ -333,13 +349,15 @@ def measured_peaks():
 
 def roofline(res, precision):
     """Roofline of the dominant kernel section (largest total time)."""
-    secs, work = res["secs"], section_work(res["p"], res["N"], 8.0 if precision == "mixed" else 16.0)
+    secs, work = res["secs"], section_work(res["p"], res["N"], 16.0 if precision == "fp64" else 8.0)
     dom = max(secs, key=lambda k: secs[k][0] * secs[k][1])
     cnt, ms = secs[dom]
     tkey = (dom, precision, res["grid_rb"]) if dom == "grid_step" else (dom, precision)
     kind, amount = work.get(dom, ("hbm", float("nan")))
     if kind == "mfma":
-        peak = PEAK_FP32_TFLOPS if (precision == "mixed" and dom.startswith("leg_")) else PEAK_FP64_TFLOPS
+        peak = {"mixed": PEAK_FP32_TFLOPS, "mixed_bf16": PEAK_BF16_TFLOPS,
+                "mixed_bf16x3": PEAK_BF16_TFLOPS}.get(precision, PEAK_FP64_TFLOPS) \
+            if dom.startswith("leg_") else PEAK_FP64_TFLOPS
         ach = amount / (ms * 1e-3) / 1e12
         return dict(bound="mfma", achieved=ach, peak=peak, unit="TFLOP/s", frac=ach / peak,
                     traffic=PMC_TRAFFIC_BYTES.get(tkey), kernel=dom, launch_ms=ms,
@@ -355,7 +373,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--precision", choices=("both", "mixed", "fp64"), default="both")
+    ap.add_argument("--precision", choices=("both", "all") + MIXED + ("fp64",), default="both",
+                    help="both: FP64 and the --mixed variant; all: FP64 and every mixed variant")
+    ap.add_argument("--mixed", choices=MIXED, default=HEAD_MIXED,
+                    help="the mixed-precision variant of the headline value")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fd", action="store_true", help="skip the config-3 banded-solve timing")
     ap.add_argument("--cpu-levels", type=int, default=9)
@@ -371,17 +392,18 @@ def main():
         sys.exit(2)
 
     import numpy as np
-    precs = ("fp64", "mixed") if args.precision == "both" else (args.precision,)
+    precs = {"both": ("fp64", args.mixed), "all": ("fp64",) + MIXED}.get(args.precision,
+                                                                          (args.precision,))
     res = {}
     for pr in precs:
         # the mixed run replays the FP64 run's step sizes, so the two trajectories
         # differ by arithmetic precision only (the Courant dt = 0.75 dt_min moves
         # with the last bits of the velocity maxima)
-        sched = res["fp64"]["dt_log"] if (pr == "mixed" and "fp64" in res) else None
+        sched = res["fp64"]["dt_log"] if (pr != "fp64" and "fp64" in res) else None
         res[pr] = run_case(pr, args.steps, args.warmup, sched)
         print(f"bench: {pr} {args.steps} steps in {res[pr]['elapsed']:.2f} s", file=sys.stderr,
               flush=True)
-    head = "mixed" if "mixed" in res else "fp64"
+    head = args.mixed if args.mixed in res else next((m for m in MIXED if m in res), "fp64")
     h = res[head]
     sps = {pr: args.steps / r["elapsed"] for pr, r in res.items()}
     breakdown = {pr: {k: round(v[1], 4) for k, v in sorted(r["secs"].items(), key=lambda kv: -kv[1][1])}
@@ -389,11 +411,10 @@ def main():
     cfg = {"workload": "Boussinesq CNAB2 dynamo step, l_max=383, n_r_max=97, n_theta=576, "
                        "n_phi=1152, E=1e-5, Ra=2e8, Pr=1, Pm=1",
            "l_max": 383, "n_r_max": 97, "precision": head,
-           "precision_split": "Legendre synthesis/analysis on FP32 MFMA with FP32 accumulation; "
-                              "Fourier buffers, azimuthal FFTs, nonlinear products and Courant "
-                              "maxima FP32; implicit solves, radial derivatives, IMEX terms and "
-                              "energy/spectrum diagnostics FP64"
-                              if head == "mixed" else "all FP64",
+           "precision_split": (LEG_SPLIT[head] + "; Fourier buffers, azimuthal FFTs, nonlinear "
+                               "products and Courant maxima FP32; implicit solves, radial "
+                               "derivatives, IMEX terms and energy/spectrum diagnostics FP64")
+                              if head != "fp64" else "all FP64",
            "steps_per_s": {pr: v for pr, v in sps.items()},
            "ms_per_step": {pr: 1e3 * r["elapsed"] / args.steps for pr, r in res.items()},
            "dt": h["dt"], "rebuilds": {pr: r["rebuilds"] for pr, r in res.items()},
@@ -405,17 +426,24 @@ def main():
            "analysis_waves_per_workgroup": {pr: r["ana_waves"] or "default" for pr, r in res.items()},
            "energies": {pr: r["energies"] for pr, r in res.items()},
            "section_ms": breakdown}
-    if len(res) == 2:
-        e64, emx = np.array(res["fp64"]["energies"]), np.array(res["mixed"]["energies"])
-        cfg["mixed_over_fp64"] = sps["mixed"] / sps["fp64"]
-        cfg["energy_rel_diff_mixed_vs_fp64"] = float(np.max(np.abs(emx - e64) / np.abs(e64)))
+    if "fp64" in res and head != "fp64":
+        cfg["mixed_over_fp64"] = sps[head] / sps["fp64"]
+        e64 = np.array(res["fp64"]["energies"])
+        per = {}
+        for mv in [m for m in MIXED if m in res]:
+            emx = np.array(res[mv]["energies"])
+            sp = {}
+            for k in ("kin_l", "mag_l", "kin_m", "mag_m"):
+                a, b = res["fp64"]["spectra"][k], res[mv]["spectra"][k]
+                sp[k] = float(np.max(np.abs(b - a)) / np.max(np.abs(a)))
+            per[mv] = dict(energy_rel_diff=float(np.max(np.abs(emx - e64) / np.abs(e64))),
+                           spectrum_rel_diff=sp, over_fp64=sps[mv] / sps["fp64"])
+        cfg["energy_rel_diff_mixed_vs_fp64"] = per[head]["energy_rel_diff"]
+        cfg["spectrum_rel_diff_mixed_vs_fp64"] = per[head]["spectrum_rel_diff"]
         cfg["energy_rel_diff_note"] = (f"after {args.warmup + args.steps} steps, mixed run on the "
                                        "FP64 run's dt sequence; max over e_kin_pol/tor, e_mag_pol/tor")
-        sp = {}
-        for k in ("kin_l", "mag_l", "kin_m", "mag_m"):
-            a, b = res["fp64"]["spectra"][k], res["mixed"]["spectra"][k]
-            sp[k] = float(np.max(np.abs(b - a)) / np.max(np.abs(a)))
-        cfg["spectrum_rel_diff_mixed_vs_fp64"] = sp
+        if len(per) > 1:
+            cfg["mixed_variants"] = per
     if not args.no_sht:
         cfg["sht_roundtrip"] = sht_case()
         print("bench: SHT round trip done", file=sys.stderr, flush=True)
@@ -433,6 +461,9 @@ def main():
         meas = {"GB/s": pk["hbm_copy_GBs"],
                 "TFLOP/s": pk["fp32_gemm_TFLOPs" if head == "mixed" and rl["kernel"].startswith("leg_")
                               else "fp64_gemm_TFLOPs"]}[rl["unit"]]
+        if rl["unit"] == "TFLOP/s" and head in ("mixed_bf16", "mixed_bf16x3") and \
+                rl["kernel"].startswith("leg_"):
+            meas = float("nan")   # no BF16 GEMM is measured
         rl["frac_of_measured_peak"] = rl["achieved"] / meas
         print("bench: measured peaks done", file=sys.stderr, flush=True)
     cpu = None
@@ -450,7 +481,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64" if head == "mixed" else "f64",
+        "dtype": DTYPE[head],
         "data": "synthetic (seeded noise + dipole seed, generated on device)",
         "config": cfg,
         "roofline": rl,

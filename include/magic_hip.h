@@ -57,11 +57,16 @@ typedef struct {
   double* F;
   long c_stride, m_stride;
   int Bout;
-  int prec;  /* 0: FP64 tables and MFMA; 1: FP32 tables (synP/synD hold floats), FP32 MFMA */
-  int fprec; /* 0: F holds doubles; 1: F holds floats (FP32 Fourier buffers, needs prec = 1) */
-  int waves; /* waves per workgroup (16 theta pairs each): 0 = default (4), 4 or 6 */
-  int nct;   /* column tiles of 8 complex columns per workgroup: 0/1, or 2 (FP32 vector only) */
+  int prec;  /* 0: FP64 tables and MFMA; 1: FP32 tables (synP/synD hold floats), FP32 MFMA;
+                2, 3: BF16 split tables of prec planes (synP/synD hold the bf16 tables of
+                mh_tables_to_bf16), BF16 MFMA on the products of planes i, j with
+                i + j < prec (3 or 6 products), FP32 accumulation */
+  int fprec; /* 0: F holds doubles; 1: F holds floats (FP32 Fourier buffers, needs prec >= 1) */
+  int waves; /* waves per workgroup (16 theta pairs each): 0 = default (4), 4 or 6 (prec <= 1) */
+  int nct;   /* column tiles of 8 complex columns per workgroup: 0/1, or 2 (prec >= 1, vector) */
   int pad;
+  const int* mp16; /* prec >= 2: per (m, parity) {K32, synthesis offset, analysis offset, 0}
+                      of the bf16 tables (elements) */
 } MhSynArgs;
 
 /* Legendre synthesis on FP64 MFMA: scalar (vector=0) replaces the bmm part of
@@ -84,9 +89,12 @@ typedef struct {
   const double* anaWd;
   const int* tiles; /* (m, p, k0) triples, k0 in steps of 96 degrees (one workgroup each) */
   int ntiles;
-  int prec;  /* as MhSynArgs.prec (anaW/anaWd hold floats for 1) */
-  int fprec; /* as MhSynArgs.fprec: F holds floats for 1 (needs prec = 1) */
-  int waves; /* waves per workgroup: 0 = default (6), 6 or 4; tiles must hold 16 * waves degrees */
+  int prec;  /* as MhSynArgs.prec (anaW/anaWd hold floats for 1, the bf16 tables for 2, 3) */
+  int fprec; /* as MhSynArgs.fprec: F holds floats for 1 (needs prec >= 1) */
+  int waves; /* waves per workgroup: 0 = default (6), 6 or 4 (4 only for prec >= 2); tiles
+                must hold 16 * waves degrees */
+  int pad1;
+  const int* mp16; /* prec >= 2: as MhSynArgs.mp16 */
 } MhAnaArgs;
 
 /* Legendre analysis on FP64 MFMA: scalar replaces the bmm part of scal_to_SH
@@ -98,6 +106,20 @@ int mh_leg_analysis(int vector, const MhAnaArgs* args, void* stream);
  * the mixed-precision transforms (no reference counterpart: the reference's
  * transforms are FP64 only, sht.py:24-296). */
 int mh_tables_to_f32(const double* src, float* dst, long n, void* stream);
+
+/* The BF16 split tables of the split-precision transforms (no reference
+ * counterpart): each FP64 table value x is written as np (2 or 3) bf16 planes
+ * hi = bf16(x), then bf16 of each remainder, so that x = sum of the planes to
+ * about 2^-(8 np) relative.  Layouts per (m, parity) block (offsets mp16[.. + 1],
+ * mp16[.. + 2], in elements; planes consecutive within a block): synthesis
+ * [plane][j < Hp][k < K32] (K32 = mp16[..]: degrees padded to 32, zero beyond
+ * K), analysis [plane][k < K16][j < Hp] (zero for j >= H), so that a lane's 8
+ * consecutive contraction indices of a BF16 MFMA operand are 16 contiguous
+ * bytes.  Inputs: the FP64 tables of mh_plm_tables and their mp_info. */
+int mh_tables_to_bf16(const double* synP, const double* synD, const double* anaW,
+                      const double* anaWd, const int* mp_info, const int* mp16, int n_mp,
+                      int H, int Hp, int np, unsigned short* synP16, unsigned short* synD16,
+                      unsigned short* anaW16, unsigned short* anaWd16, void* stream);
 
 /* ------------------------------------------------------------ FFT */
 typedef struct {

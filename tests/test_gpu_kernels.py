@@ -181,6 +181,39 @@ def test_sht_mixed_precision_against_fp64(l_max):
     assert rel(s32.scal_to_SH(s32.scal_to_spat(S)), S) < 1e-5
 
 
+@pytest.mark.parametrize("l_max,precision,tol", [(63, "mixed_bf16", 1e-4), (63, "mixed_bf16x3", 1e-5),
+                                                  (383, "mixed_bf16", 1e-4), (383, "mixed_bf16x3", 1e-5)])
+def test_sht_bf16_split_against_fp64(l_max, precision, tol):
+    """BF16 split-precision Legendre transforms (tables and inputs as two or
+    three bf16 planes, the plane products i + j < planes on the BF16 MFMA, FP32
+    accumulation) against the FP64 transforms on (l+1)^-2 spectra: every
+    transform, both directions, scalar and vector, within `tol` of the largest
+    value.  Two planes keep about 16 significant bits per operand (dropped
+    products below 2^-16 relative), three about 24 (FP32 inputs)."""
+    torch = _torch()
+    from magic_amd.params import Params
+    from magic_amd.device import Resolution
+    from magic_amd.sht import SHT
+    p = Params({"l_max": l_max, "n_r_max": 9})
+    res = Resolution(p)
+    s64, sbf = SHT(res), SHT(res, precision=precision)
+    rel = lambda a, b: float((a - b).abs().max() / b.abs().max())
+    S = torch.from_numpy(_rand_spec(p.lm_max, 5, res.maps, 7)).cuda()
+    T = torch.from_numpy(_rand_spec(p.lm_max, 5, res.maps, 8)).cuda()
+    Q = torch.from_numpy(_rand_spec(p.lm_max, 5, res.maps, 9)).cuda()
+    g64 = s64.scal_to_spat(S)
+    errs = [rel(sbf.scal_to_spat(S), g64), rel(sbf.scal_to_SH(g64), s64.scal_to_SH(g64))]
+    v64 = s64.torpol_to_spat(Q, S, T)
+    errs += [rel(a, b) for a, b in zip(sbf.torpol_to_spat(Q, S, T), v64)]
+    errs += [rel(a, b) for a, b in zip(sbf.spat_to_sphertor(v64[1], v64[2]),
+                                       s64.spat_to_sphertor(v64[1], v64[2]))]
+    errs.append(rel(sbf.scal_to_SH(sbf.scal_to_spat(S)), S))
+    print(f"bf16 split {precision} l_max={l_max}: max errors {['%.1e' % e for e in errs]}")
+    assert max(errs) < tol, errs
+    # the split is doing its work: an error far below one bf16 rounding (2^-9)
+    assert max(errs) < 2.0 ** -12
+
+
 @pytest.mark.parametrize("n", [5, 97, 112, 113, 194, 208, 230])
 def test_gj_inverse_matches_numpy(n):
     """Batched equilibrated Gauss-Jordan inverse (register-resident for n <= 208,

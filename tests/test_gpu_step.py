@@ -62,24 +62,33 @@ def test_step_matches_reference_trajectory(name, fused):
                 assert _rel(got, g[f"{k}_step{n}"]) < 1e-10, (n, k, _rel(got, g[f"{k}_step{n}"]))
 
 
-@pytest.mark.parametrize("name", ["bench16", "c1"])
-def test_mixed_precision_step_tracks_reference(name):
-    """Mixed precision (FP32-MFMA Legendre transforms, everything else FP64)
-    against the FP64 reference trajectory: energies within 1e-6 relative and
-    fields within 1e-5 of their largest coefficient at every recorded step."""
+@pytest.mark.parametrize("name,precision,etol,ftol", [
+    ("bench16", "mixed", 1e-6, 1e-5), ("c1", "mixed", 1e-6, 1e-5),
+    ("c1", "mixed_bf16x3", 1e-6, 1e-5), ("c1", "mixed_bf16", 1e-4, 1e-4)])
+def test_mixed_precision_step_tracks_reference(name, precision, etol, ftol):
+    """Mixed precision (Legendre transforms on FP32 MFMA, or on BF16 MFMA with
+    operands split into three or two bf16 planes; everything else FP64) against
+    the FP64 reference trajectory: energies within `etol` relative and fields
+    within `ftol` of their largest coefficient at every recorded step.  1e-6 is
+    the mixed-precision target; two bf16 planes (about 16 significant bits)
+    miss it by design and are held to 1e-4."""
     case, g = _load(name)
-    run = _run(case, precision="mixed")
+    run = _run(case, precision=precision)
     dt = case["dt"]
+    worst = [0.0, 0.0]
     for n in range(1, case["steps"] + 1):
         dt = run.one_step(n, dt)
         assert dt == pytest.approx(g["dt"][n - 1], rel=1e-6)
         e = np.array(run.energies())
         ge = g["energies"][n - 1]
-        assert np.all(np.abs(e - ge) <= 1e-6 * np.abs(ge)), (n, e, ge)
+        worst[0] = max(worst[0], float(np.max(np.abs(e - ge) / np.abs(ge))))
+        assert np.all(np.abs(e - ge) <= etol * np.abs(ge)), (n, e, ge)
         if n in case["field_steps"]:
             for k in ("s", "z", "w", "p", "b", "aj"):
                 got = run.f[k].cpu().numpy()
-                assert _rel(got, g[f"{k}_step{n}"]) < 1e-5, (n, k, _rel(got, g[f"{k}_step{n}"]))
+                worst[1] = max(worst[1], _rel(got, g[f"{k}_step{n}"]))
+                assert _rel(got, g[f"{k}_step{n}"]) < ftol, (n, k, _rel(got, g[f"{k}_step{n}"]))
+    print(f"{name} {precision}: worst energy rel. error {worst[0]:.2e}, field {worst[1]:.2e}")
 
 
 def test_first_step_explicit_terms():

@@ -32,14 +32,16 @@ class SynIn(C.Structure):
 class SynArgs(C.Structure):
     _fields_ = [("in_", (SynIn * 3) * 4), ("ngroups", I), ("B", I), ("ldx", I), ("H", I),
                 ("Hp", I), ("mp_info", P), ("lm_start", P), ("synP", P), ("synD", P),
-                ("or2", P), ("F", P), ("c_stride", L), ("m_stride", L), ("Bout", I), ("prec", I), ("fprec", I), ("waves", I), ("nct", I), ("pad", I)]
+                ("or2", P), ("F", P), ("c_stride", L), ("m_stride", L), ("Bout", I), ("prec", I), ("fprec", I), ("waves", I), ("nct", I), ("pad", I),
+                ("mp16", P)]
 
 
 class AnaArgs(C.Structure):
     _fields_ = [("F", P), ("c_stride", L), ("m_stride", L), ("Bin", I), ("B", I), ("ncomp", I),
                 ("cf1", I * 3), ("cf2", I * 3), ("pad0", I), ("outS", P * 3), ("outT", P * 3),
                 ("ldo", I), ("H", I), ("Hp", I), ("n_m", I), ("mp_info", P), ("lm_start", P),
-                ("anaW", P), ("anaWd", P), ("tiles", P), ("ntiles", I), ("prec", I), ("fprec", I), ("waves", I)]
+                ("anaW", P), ("anaWd", P), ("tiles", P), ("ntiles", I), ("prec", I), ("fprec", I), ("waves", I),
+                ("pad1", I), ("mp16", P)]
 
 
 class FftArgs(C.Structure):
@@ -152,6 +154,7 @@ _SIGS = {
     "mh_grid_step": [P, P],
     "mh_grid_plan": [I, P],
     "mh_tables_to_f32": [P, P, L, P],
+    "mh_tables_to_bf16": [P, P, P, P, P, P, I, I, I, I, P, P, P, P, P],
     "mh_rowop": [P, P, I, I, P],
     "mh_td": [P, P],
     "mh_rhs": [P, P],

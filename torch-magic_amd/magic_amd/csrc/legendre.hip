@@ -10,7 +10,7 @@
 // Fourier buffers F[c][m][t][b] (complex) hold, per component c and order m,
 // the (theta, batch) matrix; t = 2j is the northern and t = 2j+1 the southern
 // node of pair j (interleaved rows, as in the grid layout).
-#include "common.h"
+#include "legendre_common.h"
 
 // ---------------------------------------------------------------- P_lm tables
 // Orthonormal P_lm without Condon-Shortley phase and sin(theta) dP_lm/dtheta by
@@ -79,8 +79,6 @@ extern "C" int mh_plm_tables(int l_max, int n_m, int H, int Hp, const double* co
 }
 
 // ---------------------------------------------------------------- synthesis
-typedef MhSynIn SynIn;
-typedef MhSynArgs SynArgs;
 
 __device__ __forceinline__ double syn_load(const SynIn& in, long lm, int l, int b, int reim,
                                            int ldx, const double* or2) {
@@ -89,17 +87,6 @@ __device__ __forceinline__ double syn_load(const SynIn& in, long lm, int l, int 
   if (in.mode == 0) return in.x1[off];
   if (in.mode == 1) return dl * in.x1[off];
   return -in.x1[off] + or2[b] * dl * in.x2[off];
-}
-
-// i*m*z where lane pairs (2q, 2q+1) hold (re, im) of z
-__device__ __forceinline__ double imul_pair(double v, double m, int lane) {
-  const double o = __shfl_xor(v, 1);
-  return (lane & 1) ? m * o : -m * o;
-}
-
-// Offset (in doubles) of the real part of F(c, m, t, b), layout F[c][m][t][b].
-__device__ __forceinline__ long fidx(long cs, long ms, int B, int c, int m, int t, int b) {
-  return c * cs + m * ms + (long)t * 2 * B + 2L * b;
 }
 
 // Element type of the Legendre contraction: FP64 (16x16x4 f64 MFMA) or FP32
@@ -118,15 +105,6 @@ template <> struct Mfma<float> {
   static __device__ __forceinline__ int row(int lane, int v) { return mfma_row_f32(lane, v); }
 };
 
-// Per-lane group select without dynamic indexing of the argument struct.
-__device__ __forceinline__ SynIn syn_in(const SynArgs& a, int g, int c) {
-  SynIn r = a.in[0][c];
-  if (g == 1) r = a.in[1][c];
-  if (g == 2) r = a.in[2][c];
-  if (g == 3) r = a.in[3][c];
-  return r;
-}
-
 // Synthesis, one workgroup per (order m, 96 theta pairs, 8 complex columns).
 // Columns of the groups are laid side by side (column bc is batch column
 // bc % B of group bc / B, which is also its column in F).  The input tile of
@@ -137,7 +115,7 @@ __device__ __forceinline__ SynIn syn_in(const SynArgs& a, int g, int c) {
 // table operands from global memory through a ring of kSynDepth steps.
 // Waves per synthesis workgroup: each wave takes 16 theta pairs.  Four (the
 // default) or six; both give identical results, the tuner may pick either.
-constexpr int kSynWavesDefault = 4, kChunk = 32, kSynDepth = 4;
+constexpr int kSynWavesDefault = 4, kSynDepth = 4;
 
 // NCT column tiles of 8 complex columns per workgroup: every table operand
 // feeds NCT MFMAs.  Two are instantiated for the FP32 vector synthesis (its
@@ -355,13 +333,14 @@ extern "C" int mh_leg_synth(int vector, const MhSynArgs* args, int n_m, void* st
   if (a.ngroups < 1 || a.ngroups > 4 || a.B < 1 || a.H < 1 || a.Hp % 32 != 0 || a.Hp < a.H ||
       a.Bout != a.ngroups * a.B)
     return MH_EINVAL;
-  if (a.prec != 0 && a.prec != 1) return MH_EINVAL;
-  if (a.fprec != 0 && (a.fprec != 1 || a.prec != 1)) return MH_EINVAL;
+  if (a.prec < 0 || a.prec > 3) return MH_EINVAL;
+  if (a.fprec != 0 && (a.fprec != 1 || a.prec < 1)) return MH_EINVAL;
   // mode 2 (two inputs) is supported for the T field of a vector group only
   for (int g = 0; g < a.ngroups; ++g)
     for (int f = 0; f < (vector ? 3 : 1); ++f)
       if (a.in[g][f].mode < 0 || a.in[g][f].mode > 2 || (a.in[g][f].mode == 2 && !(vector && f == 2)))
         return MH_EINVAL;
+  if (a.prec >= 2) return leg_synth_bf16(vector, a, n_m, mh_stream(stream));
   // column tiles per workgroup: one, or two for the FP32 vector synthesis
   const int nct = a.nct == 0 ? 1 : a.nct;
   if (nct != 1 && (nct != 2 || a.prec != 1 || !vector)) return MH_EINVAL;
@@ -398,7 +377,6 @@ extern "C" int mh_leg_synth(int vector, const MhSynArgs* args, int n_m, void* st
 
 
 // ---------------------------------------------------------------- analysis
-typedef MhAnaArgs AnaArgs;
 
 // Analysis, one workgroup per (tile (m, p, k0), component, 8 complex columns):
 // wave w takes the 16 degrees k0 + 16w (96 per tile).  The hemisphere folds
@@ -586,9 +564,10 @@ extern "C" int mh_leg_analysis(int vector, const MhAnaArgs* args, void* stream) 
   if (waves != 4 && waves != 6) return MH_EINVAL;
   dim3 grid((unsigned)(a.ncomp * mh_cdiv(a.B, 8) * a.ntiles));
   const dim3 blk(64 * waves);
-  if (a.prec != 0 && a.prec != 1) return MH_EINVAL;
-  if (a.fprec != 0 && (a.fprec != 1 || a.prec != 1)) return MH_EINVAL;
+  if (a.prec < 0 || a.prec > 3) return MH_EINVAL;
+  if (a.fprec != 0 && (a.fprec != 1 || a.prec < 1)) return MH_EINVAL;
   hipStream_t st = mh_stream(stream);
+  if (a.prec >= 2) return leg_analysis_bf16(vector, a, st);
 #define MH_ANA_LAUNCH(W)                                                                      \
   if (a.prec == 0) {                                                                          \
     if (vector) hipLaunchKernelGGL((k_ana<true, double, double, W>), grid, blk, 0, st, a, n_m); \

@@ -111,6 +111,48 @@ class Resolution:
             self._t32 = t
         return self._t32
 
+    def bf16_tables(self, np_: int):
+        """BF16 split copies (np_ = 2 or 3 planes) of the four Legendre tables for
+        the split-precision transforms, shared by every run of the resolution and
+        built eagerly, never inside a recorded launch plan (as f32_tables)."""
+        if np_ not in (2, 3):
+            raise ValueError(f"bf16_tables: 2 or 3 planes, not {np_}")
+        cache = self.__dict__.setdefault("_t16", {})
+        if np_ not in cache:
+            if getattr(_lib.recording(), "plan", False):
+                raise RuntimeError("bf16_tables: the BF16 tables must be built before "
+                                   "a launch plan is recorded")
+            info = self.lay["mp_info"].reshape(-1, 4)
+            mp16 = np.zeros_like(info)
+            so = ao = 0
+            for i, (K, _s, _a, K16) in enumerate(info):
+                K32 = -(-int(K) // 32) * 32
+                mp16[i] = (K32, so, ao, 0)
+                so += np_ * self.Hp * K32
+                ao += np_ * int(K16) * self.Hp
+            if so >= 2 ** 31 or ao >= 2 ** 31:
+                raise ValueError("BF16 Legendre tables exceed 32-bit offsets")
+            t = dict(mp16=torch.as_tensor(mp16.reshape(-1), dtype=I32).to(self.device))
+            for k, n in (("synP", so), ("synD", so), ("anaW", ao), ("anaWd", ao)):
+                t[k] = torch.zeros(max(n, 1), dtype=torch.int16, device=self.device)
+            _lib.call("mh_tables_to_bf16", ptr(self.synP), ptr(self.synD), ptr(self.anaW),
+                      ptr(self.anaWd), ptr(self.d["mp_info"]), ptr(t["mp16"]), len(info), self.H,
+                      self.Hp, np_, ptr(t["synP"]), ptr(t["synD"]), ptr(t["anaW"]),
+                      ptr(t["anaWd"]), stream_handle())
+            torch.cuda.current_stream(self.device).synchronize()  # shared across streams
+            cache[np_] = t
+        return cache[np_]
+
+    def _tables(self, prec):
+        """(table pointers, mp16 pointer) of a transform precision: 0 FP64, 1 FP32,
+        2 and 3 BF16 split with that many planes."""
+        if prec >= 2:
+            t = self.bf16_tables(prec)
+            return t, ptr(t["mp16"])
+        if prec == 1:
+            return self.f32_tables(), 0
+        return dict(synP=self.synP, synD=self.synD, anaW=self.anaW, anaWd=self.anaWd), 0
+
     # ------------------------------------------------------------ launchers
     def fstrides(self, Bout):
         """(component stride, m stride) in doubles of a Fourier buffer F[c][m][t][b]."""
@@ -120,9 +162,11 @@ class Resolution:
     def synth(self, groups, B, ldx, F, Bout, vector: bool, or2=None, prec=0, waves=0, nct=0):
         """Legendre synthesis.  groups: list of [(x1, x2, mode)] * (3 if vector else 1).
 
-        prec 0: FP64 MFMA; 1: FP32 tables and MFMA with FP32 accumulation.
-        waves: waves per workgroup (0: the library default; 4 or 6); nct: column
-        tiles per workgroup (0/1, or 2 for the FP32 vector synthesis); same results."""
+        prec 0: FP64 MFMA; 1: FP32 tables and MFMA with FP32 accumulation; 2, 3:
+        BF16 split tables and inputs of that many planes on the BF16 MFMA, FP32
+        accumulation.  waves: waves per workgroup (0: the library default; 4 or
+        6, 4 only for BF16); nct: column tiles per workgroup (0/1, or 2 for the
+        FP32 and BF16 vector synthesis); same results."""
         a = _lib.SynArgs()
         for g, grp in enumerate(groups):
             for c, (x1, x2, mode) in enumerate(grp):
@@ -131,11 +175,8 @@ class Resolution:
                 a.in_[g][c].mode = mode
         a.ngroups, a.B, a.ldx, a.H, a.Hp = len(groups), B, ldx, self.H, self.Hp
         a.mp_info, a.lm_start = ptr(self.d["mp_info"]), ptr(self.d["lm_start"])
-        if prec:
-            t32 = self.f32_tables()
-            a.synP, a.synD = ptr(t32["synP"]), ptr(t32["synD"])
-        else:
-            a.synP, a.synD = ptr(self.synP), ptr(self.synD)
+        tab, a.mp16 = self._tables(prec)
+        a.synP, a.synD = ptr(tab["synP"]), ptr(tab["synD"])
         a.prec = prec
         a.fprec = 1 if F.dtype == torch.float32 else 0  # FP32 Fourier buffer (needs prec 1)
         a.or2 = ptr(or2 if or2 is not None else self.d["or2"])
@@ -167,13 +208,10 @@ class Resolution:
                 a.outT[i] = outT[i] if isinstance(outT[i], int) else ptr(outT[i])
         a.ldo, a.H, a.Hp = ldo, self.H, self.Hp
         a.mp_info, a.lm_start = ptr(self.d["mp_info"]), ptr(self.d["lm_start"])
-        if prec:
-            t32 = self.f32_tables()
-            a.anaW, a.anaWd = ptr(t32["anaW"]), ptr(t32["anaWd"])
-        else:
-            a.anaW, a.anaWd = ptr(self.anaW), ptr(self.anaWd)
+        tab, a.mp16 = self._tables(prec)
+        a.anaW, a.anaWd = ptr(tab["anaW"]), ptr(tab["anaWd"])
         a.prec = prec
-        w = waves or 6
+        w = waves or (4 if prec >= 2 else 6)
         a.tiles, a.ntiles = ptr(self.d[f"ana_tiles_{w}"]), self.lay[f"n_ana_tiles_{w}"]
         a.waves = w
         a.n_m = self.M

@@ -43,7 +43,7 @@ import torch
 
 from . import _lib
 from .device import C128, F64, I32, Resolution, ptr, require_gpu, stream_handle
-from .params import Params
+from .params import LEGENDRE_PREC, Params
 
 _FIELDS = ("s", "ds", "z", "dz", "w", "dw", "ddw", "p", "dp", "b", "db", "ddb", "aj", "dj", "ddj")
 _SYS = ("s", "z", "w", "p", "b", "j")
@@ -273,12 +273,15 @@ class DynamoRun:
         # (default): one kernel for inverse FFT + products + forward FFT, no grid in
         # HBM.  Separate: grids G[c][b][t][phi] and three kernels.
         self.fused_grid = fused_grid
-        # mixed precision: Legendre transforms on FP32 MFMA with FP32 accumulation,
-        # and (fused grid path) FP32 Fourier buffers, azimuthal FFTs and nonlinear
-        # products; solves, radial derivatives, IMEX terms and diagnostics stay FP64
-        self.leg_prec = 1 if p.precision == "mixed" else 0
-        if self.leg_prec:
+        # mixed precision: Legendre transforms on FP32 MFMA with FP32 accumulation
+        # (or BF16 MFMA on split operands), and (fused grid path) FP32 Fourier
+        # buffers, azimuthal FFTs and nonlinear products; solves, radial
+        # derivatives, IMEX terms and diagnostics stay FP64
+        self.leg_prec = LEGENDRE_PREC[p.precision]
+        if self.leg_prec == 1:
             r.f32_tables()   # shared FP32 tables, converted once, outside any plan
+        elif self.leg_prec >= 2:
+            r.bf16_tables(self.leg_prec)
         self.grid_generic = False  # force the run-time FFT plan of the fused kernel
         fdt = torch.float32 if (self.leg_prec and fused_grid) else F64
         self.F_s = r.fourier_buffer(1, N, fdt)

@@ -45,6 +45,13 @@ _BOOL = {"profile", "polar_opt", "use_shtns", "l_correct_AMz", "l_correct_AMe",
          "l_cour_alf_damp", "l_power", "l_hel", "l_hemi"}
 _STR = {"time_scheme", "start_file", "radial_scheme", "device", "precision"}
 
+# Arithmetic of the step (the one key added to the reference's): "fp64" (the
+# reference's), or mixed precision with the Legendre transforms on FP32 MFMA
+# ("mixed"), on BF16 MFMA with operands split into two bf16 planes, three
+# products ("mixed_bf16"), or three planes, six products ("mixed_bf16x3").
+PRECISIONS = ("fp64", "mixed", "mixed_bf16", "mixed_bf16x3")
+LEGENDRE_PREC = {"fp64": 0, "mixed": 1, "mixed_bf16": 2, "mixed_bf16x3": 3}
+
 
 def _as_bool(v) -> bool:
     return v if isinstance(v, bool) else str(v).lower() in ("true", "1", "yes")
@@ -147,8 +154,8 @@ class Params:
             bad.append("angular-momentum correction")
         if self.ktops != 1 or self.kbots != 1:
             bad.append("fixed-flux entropy boundaries")
-        if self.precision not in ("fp64", "mixed"):
-            bad.append(f"precision {self.precision!r} (fp64 or mixed)")
+        if self.precision not in PRECISIONS:
+            bad.append(f"precision {self.precision!r} ({', '.join(PRECISIONS)})")
         if bad:
             raise NotImplementedError("device step does not support: " + ", ".join(bad))
 

@@ -47,17 +47,22 @@ def _grid(x, T, P):
 
 
 class SHT:
-    """Transforms of one resolution.  precision "fp64" (the reference's) or
-    "mixed": Legendre contractions on FP32 MFMA with FP32 accumulation and FP32
-    tables, FFTs in FP64; inputs and outputs keep the reference's dtypes."""
+    """Transforms of one resolution.  precision "fp64" (the reference's),
+    "mixed" (Legendre contractions on FP32 MFMA with FP32 tables), "mixed_bf16"
+    or "mixed_bf16x3" (on BF16 MFMA with tables and inputs split into two or
+    three bf16 planes); FP32 accumulation and FP64 FFTs in every mixed form;
+    inputs and outputs keep the reference's dtypes."""
 
     def __init__(self, res: Resolution, precision: str = "fp64"):
-        if precision not in ("fp64", "mixed"):
-            raise ValueError(f"precision must be 'fp64' or 'mixed', not {precision!r}")
+        from .params import LEGENDRE_PREC, PRECISIONS
+        if precision not in PRECISIONS:
+            raise ValueError(f"precision must be one of {PRECISIONS}, not {precision!r}")
         self.res = res
-        self.prec = 1 if precision == "mixed" else 0
-        if self.prec:
+        self.prec = LEGENDRE_PREC[precision]
+        if self.prec == 1:
             res.f32_tables()
+        elif self.prec >= 2:
+            res.bf16_tables(self.prec)
 
     def _on(self, x):
         return x.to(self.res.device) if x.device != self.res.device else x

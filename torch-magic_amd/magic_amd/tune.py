@@ -131,7 +131,10 @@ def synth_shape(res, N: int, prec: int, fdtype=torch.float64, bufs=None,
     ng = 4 if vector else 1
     key = (f"{_device_name(res)}|leg_synth_{'vec' if vector else 'scal'}|L={res.L}|H={res.H}"
            f"|M={res.M}|B={ng * N}|prec={prec}|fprec={fprec}")
-    cands = [(4, 1), (6, 1)] + ([(4, 2), (6, 2)] if (vector and prec == 1) else [])
+    if prec >= 2:  # BF16 split: four-wave workgroups only
+        cands = [(4, 1)] + ([(4, 2)] if vector else [])
+    else:
+        cands = [(4, 1), (6, 1)] + ([(4, 2), (6, 2)] if (vector and prec == 1) else [])
     buf = {}
 
     def launch(c):  # buffers made on first use (none on a cache hit)
@@ -173,6 +176,8 @@ def analysis_waves(res, N: int, prec: int, fdtype=torch.float64, bufs=None) -> i
         res.analysis(buf["F"], Nb, Nb, [(3, 4), (5, 6), (7, 8)], [y, y, y], [y, None, y], N,
                      vector=True, prec=prec, waves=w)
 
+    if prec >= 2:
+        return 4   # BF16 split: four-wave workgroups only
     return timed_choice(key, [6, 4], launch, 6)
 
 
