@@ -141,7 +141,7 @@ def apply_checkpoint_state(run, state: dict) -> tuple[int, float]:
     # configured dtmax, clamped as initialize_dt clamps it.  The reference keeps
     # its cap at 0.0 there (step_time.py:1551), so its first decision after such
     # a restart would be dt = 0.
-    if not run.dtMax:
+    if not getattr(run, "dtMax", None) and hasattr(run, "dt_cap"):
         run.dtMax = run.dt_cap(run.p.dtmax)
     # device step control restarts from the restored state: the next ctl_begin
     # pushes it (no queued or pulled steps of the state before the load remain)
