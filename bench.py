@@ -108,199 +108,65 @@ def section_work(p, N, fbytes=16.0):
     }
 
 
-def cpu_baseline(sample_levels, steps=3):
-    """Time the FP64 oracle on a bounded sample of the workload (see 'sample')."""
-    import numpy as np
+def cpu_baseline(sample_levels, steps=1, second_levels=17):
+    """Time the FP64 oracle on a bounded sample of the workload: the full
+    l_max=383 angular grid (576 x 1152) with n_r_max = sample_levels and with
+    n_r_max = second_levels radial levels.  The line through the two samples,
+    t(N) = a + b N, is reported as an extrapolation to n_r_max = 97; it is a
+    lower bound: the per-degree solves and radial derivatives grow as N^2 and
+    the larger arrays leave the caches (one full step at n_r_max = 97, 8 threads,
+    took 271 s where the line through 9 and 17 levels gives 148 s).  A full step
+    takes minutes on the host, too long for the default run."""
     from oracle.dynamo import OracleRun
-    cfg = dict(WORKLOAD, n_r_max=sample_levels)
-    run = OracleRun(cfg)
-    run.initialize_fields()
-    run.add_noise(NOISE["seed"], NOISE["amp"])
-    run.setup_initial_state()
-    run.initialize_dt(WORKLOAD["dtmax"])
-    dt_step = run.one_step(1, WORKLOAD["dtmax"])   # builds the matrices
-    t0 = time.perf_counter()
-    for n in range(2, 2 + steps):
-        dt_step = run.one_step(n, dt_step)
-    dt = (time.perf_counter() - t0) / steps
+
+    def per_step(levels, n):
+        cfg = dict(WORKLOAD, n_r_max=levels)
+        run = OracleRun(cfg)
+        run.initialize_fields()
+        run.add_noise(NOISE["seed"], NOISE["amp"])
+        run.setup_initial_state()
+        run.initialize_dt(WORKLOAD["dtmax"])
+        dt_step = run.one_step(1, WORKLOAD["dtmax"])   # builds the matrices
+        t0 = time.perf_counter()
+        for k in range(2, 2 + n):
+            dt_step = run.one_step(k, dt_step)
+        return (time.perf_counter() - t0) / n
+
+    t1 = per_step(sample_levels, steps)
+    t2 = per_step(second_levels, steps)
+    slope = (t2 - t1) / (second_levels - sample_levels)
+    t97 = t1 + slope * (WORKLOAD["n_r_max"] - sample_levels)
     try:
         import torch
         cores = torch.get_num_threads()
     except Exception:
         cores = os.cpu_count()
-    return dict(value=1.0 / dt, unit="steps/s", cores=cores, kind="port",
-                sample=f"{steps} CNAB2 steps of the numpy FP64 oracle at l_max=383 with n_r_max="
-                       f"{sample_levels} (the full 576x1152 angular grid); the GPU value "
-                       f"is at n_r_max=97, so scale by ~{sample_levels}/97 for equal work",
-                seconds=dt)
+    return dict(value=1.0 / t1, unit="steps/s", cores=cores, kind="port",
+                sample=f"{steps} CNAB2 steps of the numpy FP64 oracle at l_max=383 (the full "
+                       f"576x1152 angular grid) with n_r_max={sample_levels}; value is that "
+                       f"sample's steps/s",
+                seconds=t1,
+                seconds_at_second_sample={str(second_levels): t2},
+                extrapolated_seconds_per_step_at_97=t97,
+                extrapolated_steps_per_s_at_97=1.0 / t97,
+                extrapolation=f"linear in n_r_max through n_r_max={sample_levels} and "
+                              f"{second_levels} (same angular grid), evaluated at 97: a lower "
+                              f"bound (superlinear in n_r_max)")
 
 
-def run_case(precision, steps, warmup, dt_schedule=None, section_steps=20):
-    """Time `steps` CNAB2 steps (one HIP graph each, dt decided on the device)
-    after `warmup` untimed ones, then time the kernel sections over
-    `section_steps` further steps run eagerly with HIP events between sections.
-    dt_schedule: step sizes to replay instead of the run's own Courant decisions."""
+def copy_calibration(reps=3):
+    """The device copies that calibrate rocprofv3's FETCH_SIZE and WRITE_SIZE
+    counters: `reps` copies of a 256 MiB float64 tensor (one read and one write
+    of 256 MiB each), profiled on their own (``--copy-calibration``)."""
     import torch
-    from magic_amd.dynamo import DynamoRun
-    run = DynamoRun(dict(WORKLOAD, precision=precision), device="cuda:0")
-    run.dt_schedule = dt_schedule
-    run.use_graphs = True
-    run.initialize_fields(noise_amp=NOISE["amp"], noise_seed=NOISE["seed"])
-    run.setup_initial_state()
-    run.initialize_dt(WORKLOAD["dtmax"])
-    dt = run.run_steps(warmup, run.dtMax)           # includes the graph captures
-    run.ctl_begin(dt)
+    n = (256 << 20) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        run.step_async()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    dt = run.ctl_end()
-    out = dict(elapsed=t1 - t0, energies=run.energies(), spectra=run.spectra(), dt=dt,
-               rebuilds=run.n_rebuild, p=run.p, N=run.N, grid_rb=run.grid_rb,
-               syn_shape=(run.syn_vec_shape, run.syn_scal_shape), ana_waves=run.ana_waves,
-               rowop_wide=run.rowop_wide)
-    run.enable_timers(True)
-    run.run_steps(section_steps, dt)
-    out["secs"] = run.timer_summary()
-    out["dt_log"] = list(run.dt_log)
-    run.enable_timers(False)
-    run.close()
-    del run
-    torch.cuda.empty_cache()
-    return out
-
-
-def fd_banded_case():
-    """Config 3: FD4 n_r=512 l_max=255 anelastic (N_rho=3, polytropic index 2)
-    banded S, Z, W operators: batched band LU of every degree and the solve of
-    all 32 896 complex right-hand sides per system in one launch each."""
-    import numpy as np
-    import torch
-    from magic_amd.fd import FDOperators
-    n, l_max = 512, 255
-    t0 = time.perf_counter()
-    ops = FDOperators(n, l_max, 0.6 * 1e-4, fd_order=4, fd_order_bound=2)
-    host_s = time.perf_counter() - t0
-    lidx = np.repeat(np.arange(l_max + 1), np.arange(1, l_max + 2)).astype(np.int32)
-    L = len(lidx)
-    rng = np.random.default_rng(NOISE["seed"])
-    rhs = torch.as_tensor(rng.standard_normal((L, n)) + 1j * rng.standard_normal((L, n))).cuda()
-    d_l = torch.as_tensor(lidx).cuda()
-    rows = torch.argsort(d_l.to(torch.int64), stable=True).to(torch.int32)
-    out = torch.empty_like(rhs)
-    ev = lambda: torch.cuda.Event(enable_timing=True)
-    ops.factor()                                  # warm-up (module load, first launch)
-    ops.solve("W", rhs, d_l, out, rows)
-    torch.cuda.synchronize()
-    e0, e1 = ev(), ev()
-    e0.record()
-    ops.factor()
-    e1.record()
-    solve_ms = {}
-    for name in ("S", "Z", "W"):
-        a, b = ev(), ev()
-        a.record()
-        ops.solve(name, rhs, d_l, out, rows)
-        b.record()
-        solve_ms[name] = (a, b)
-    torch.cuda.synchronize()
-    fac_ms = e0.elapsed_time(e1)
-    sm = {k: a.elapsed_time(b) for k, (a, b) in solve_ms.items()}
-    bytes_solve = 2.0 * L * n * 16.0                # right-hand sides in, solutions out
-    return dict(workload="FD4 n_r=512 (stretched), l_max=255, polytropic N_rho=3 n=2; "
-                         "S, Z kl=ku=2, W kl=ku=3; 32896 complex RHS per system",
-                host_build_s=host_s, factor_ms_all_systems=fac_ms, solve_ms=sm,
-                solve_GBps={k: bytes_solve / (v * 1e-3) / 1e9 for k, v in sm.items()},
-                info_ok=all(int(ops.dev[k]["info"].abs().sum()) == 0 for k in ("S", "Z", "W")))
-
-
-def sht_case():
-    """Config 2: SHT round trip at l_max=383 (n_theta=576, n_phi=1152) on 3 x 97 = 291
-    fields with seeded Gaussian coefficients scaled by (l+1)^-2 (m=0 real), in FP64 and
-    in mixed precision: ms per direction (synthesis + inverse FFT; forward FFT +
-    analysis) and the max / RMS round-trip error relative to the largest coefficient
-    (analysis(synthesis(x)) = x exactly for band-limited x), plus the mixed grid
-    against the FP64 grid."""
-    import numpy as np
-    import torch
-    from magic_amd.device import Resolution
-    from magic_amd.params import Params
-    from magic_amd.sht import SHT
-    p = Params({"l_max": 383, "n_r_max": 97})
-    res = Resolution(p)
-    B = 3 * 97
-    rng = np.random.default_rng(NOISE["seed"])
-    l, m = res.maps["l"], res.maps["m"]
-    S = (rng.standard_normal((p.lm_max, B)) + 1j * rng.standard_normal((p.lm_max, B))) \
-        * ((l + 1.0) ** -2.0)[:, None]
-    S[m == 0] = S[m == 0].real
-    dS = torch.from_numpy(S).cuda()
-    smax = np.abs(S).max()
-    ev = lambda: torch.cuda.Event(enable_timing=True)
-    out, grids = {}, {}
-    flops = 4.0 * p.lm_max * (p.n_theta // 2) * B
-    for prec in ("fp64",) + MIXED:
-        sht = SHT(res, precision=prec)
-        g = sht.scal_to_spat(dS)
-        back = sht.scal_to_SH(g)                       # warm-up (tables, module load)
-        torch.cuda.synchronize()
-        t = {"syn": [], "ana": []}
-        for _ in range(3):
-            a, b, c = ev(), ev(), ev()
-            a.record()
-            g = sht.scal_to_spat(dS)
-            b.record()
-            back = sht.scal_to_SH(g)
-            c.record()
-            torch.cuda.synchronize()
-            t["syn"].append(a.elapsed_time(b))
-            t["ana"].append(b.elapsed_time(c))
-        err = (back - dS).abs().cpu().numpy()
-        out[prec] = dict(synthesis_ms=float(np.min(t["syn"])), analysis_ms=float(np.min(t["ana"])),
-                         roundtrip_max_err=float(err.max() / smax),
-                         roundtrip_rms_err=float(np.sqrt((err ** 2).mean()) / smax),
-                         legendre_tflops_synthesis_upper=flops / (np.min(t["syn"]) * 1e-3) / 1e12)
-        grids[prec] = g
-    gmax = grids["fp64"].abs().max()
-    for prec in MIXED:
-        d = (grids[prec] - grids["fp64"]).abs()
-        out[f"{prec}_grid_vs_fp64"] = dict(max_err=float(d.max() / gmax),
-                                           rms_err=float(d.pow(2).mean().sqrt() / gmax))
-    out["workload"] = "l_max=383, 576 x 1152 grid, 291 fields, coefficients N(0,1) (l+1)^-2"
-    return out
-
-
-def ensemble_case(n_seq):
-    """Config 5: 64 Boussinesq runs (l_max in {32, 64, 85, 127}, n_r in {33, 49, 65},
-    E log-uniform [1e-4, 1e-3], Ra_mod uniform [60, 200], 500 steps each) arriving as a
-    seeded Poisson process at 2/s, served with continuous admission (16 slots, HBM
-    budget); baseline: the first n_seq of the same requests run back to back, one at
-    a time, in plain FP64 (arrivals ignored)."""
-    import numpy as np
-    from magic_amd.serve import EnsembleServer, config5_requests, serve_sequential
-    cfgs = [r.cfg for r in config5_requests()]
-    srv = EnsembleServer(max_slots=16)
-    tw = time.perf_counter()
-    srv.warm(cfgs)                 # device tables of the served resolutions, before serving
-    warm_s = time.perf_counter() - tw
-    _, mix = srv.serve(config5_requests(), realtime=True)
-    # the same 64 runs all waiting at t=0: throughput of the server when saturated
-    sat_srv = EnsembleServer(max_slots=16)
-    sat_srv.warm(cfgs)
-    _, sat = sat_srv.serve(config5_requests(), realtime=False)
-    _, seq = serve_sequential(config5_requests()[:n_seq], warm=True)
-    return dict(workload="64 runs x 500 steps, Poisson arrivals 2/s, l_max {32,64,85,127}, "
-                         "n_r {33,49,65}, FP64; each step one HIP graph with the dt decision "
-                         "on the device, one step per run per round",
-                table_warm_s=warm_s,
-                server=dict(mix, max_active=max(n for _, n in srv.occupancy),
-                            admit_ms_mean=1e3 * float(np.mean(srv.admit_s)),
-                            admit_ms_max=1e3 * float(np.max(srv.admit_s)),
-                            admit_stage_ms_max=1e3 * float(np.max(srv.stage_s))),
-                saturated=dict(sat, max_active=max(n for _, n in sat_srv.occupancy)),
-                sequential=dict(seq, sample=f"first {n_seq} of the 64 requests back to back"))
+    print(json.dumps({"copy_calibration": {"bytes_read": 8 * n, "bytes_written": 8 * n,
+                                           "copies": reps}}))
 
 
 def measured_peaks():
@@ -384,6 +250,9 @@ def main():
     ap.add_argument("--no-sht", action="store_true", help="skip the config-2 SHT round trip")
     ap.add_argument("--no-peaks", action="store_true",
                     help="skip the measured HBM / GEMM peaks")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run the 256 MiB device copies that calibrate the PMC byte "
+                         "counters, then exit")
     ap.add_argument("--ensemble-seq", type=int, default=16,
                     help="requests in the sequential ensemble baseline sample")
     args = ap.parse_args()
@@ -391,6 +260,9 @@ def main():
         print("bench.py: this build runs on one GPU only; --gpus must be 1", file=sys.stderr)
         sys.exit(2)
 
+    if args.copy_calibration:
+        copy_calibration()
+        return
     import numpy as np
     precs = {"both": ("fp64", args.mixed), "all": ("fp64",) + MIXED}.get(args.precision,
                                                                           (args.precision,))
@@ -469,7 +341,9 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(args.cpu_levels)
-        print(f"bench: CPU baseline step {cpu['seconds']:.1f} s", file=sys.stderr, flush=True)
+        print(f"bench: CPU baseline step {cpu['seconds']:.1f} s (n_r_max={args.cpu_levels}), "
+              f"extrapolated {cpu['extrapolated_seconds_per_step_at_97']:.0f} s at 97",
+              file=sys.stderr, flush=True)
     line = {
         "metric": "steps/s at l_max=383 mixed vs FP64; ensemble p50/p99 step latency",
         "value": sps[head],
