@@ -48,6 +48,13 @@ reader keeps the file's big-endian byte order, which its own
 ``load_fortran_checkpoint`` cannot hand to ``torch.from_numpy``; the script
 gives it native-order copies of the same values).
 
+Movie case (``movies16``): the reference's movie writer
+(``movie_output.py:120-287``: header, two frames) on the grid fields of the
+``files16`` state, computed as its radial loop does (``torpol_to_spat`` of
+(l(l+1) w, dw, z) and (l(l+1) b, db, aj), ``scal_to_spat`` of s,
+``step_time.py:955-1059``), for CMB and equatorial movies of Br, Bt, Vr, Vp and T
+(``files16_<Field>_<CMB|EQU>_mov.bin``).
+
 Usage: ``python tests/golden/make_golden.py [case ...]``
 """
 
@@ -283,6 +290,45 @@ np.savez(out + ".npz", **res)
 """
 
 
+MOVIES = ["Br CMB", "Vr CMB", "Vp CMB", "T CMB", "Br EQ", "Vr EQ", "T EQ", "Bt EQ"]
+
+MOVIE_CHILD = r"""
+import json, sys
+import numpy as np
+case = json.loads(sys.argv[1]); out = sys.argv[2]; movies = json.loads(sys.argv[5])
+sys.path.insert(0, sys.argv[3])
+from magic_torch.config import configure
+configure(dict(case["cfg"], device="cpu", use_shtns=False))
+import torch
+torch.set_num_threads(8)
+from magic_torch import fields
+from magic_torch.sht import scal_to_spat, torpol_to_spat
+from magic_torch.horizontal_data import dLh
+from magic_torch.movie_output import (parse_movie_string, write_movie_header,
+                                      write_movie_frame, extract_surface_field,
+                                      extract_equat_field)
+st = dict(np.load(out + ".npz"))
+for k in ("w", "dw", "z", "s", "b", "db", "aj"):
+    getattr(fields, k + "_LMloc").copy_(torch.from_numpy(st[k]))
+d = dLh.reshape(-1, 1).to(torch.complex128)
+# the grid fields of the radial loop (step_time.py:955-1059): velocity, field, entropy
+vr, vt, vp = torpol_to_spat(d * fields.w_LMloc, fields.dw_LMloc, fields.z_LMloc)
+br, bt, bp = torpol_to_spat(d * fields.b_LMloc, fields.db_LMloc, fields.aj_LMloc)
+sc = scal_to_spat(fields.s_LMloc)
+t = float(st["time"])
+for ms in movies:
+    ft, ns, const, fname = parse_movie_string(ms)
+    with open(f"{out}_{fname}.bin", "wb") as f:
+        write_movie_header(f, ns, 1, const, [ft])
+        for k, tk in enumerate((t, 2.0 * t)):
+            if ns == 1:
+                frame = extract_surface_field(ft, 0, vr, vt, vp, br, bt, bp, sc)
+            else:
+                frame = extract_equat_field(ft, vr, vt, vp, br, bt, bp, sc)
+            write_movie_frame(f, tk, [frame])
+"""
+
+
 def main(names=None):
     env = dict(os.environ, MAGIC_DEVICE="cpu", PYTHONDONTWRITEBYTECODE="1",
                TORCH_COMPILE_DISABLE="1", TORCHDYNAMO_DISABLE="1")
@@ -308,6 +354,11 @@ def main(names=None):
         with open(out + ".json", "w") as f:
             json.dump(meta, f, indent=1)
         print("wrote", out)
+    if not names or "movies16" in names:
+        out = os.path.join(HERE, "files16")
+        subprocess.run([sys.executable, "-c", MOVIE_CHILD, json.dumps(FILES_CASE), out, REF_SRC,
+                        REPO, json.dumps(MOVIES)], check=True, env=env)
+        print("wrote movie files of", out)
     for name, case in FD_CASES.items():
         if names and name not in names:
             continue

@@ -239,3 +239,60 @@ def test_main_run_writes_the_reference_files(tmp_path):
     assert torch.equal(config.default_run().f["w"], ref.f["w"])
     ref.close()
     config.configure({})
+
+
+def test_movie_frames_match_reference():
+    """Movie frames of the files16 state against the frames the reference's
+    writer produced from the same state (its radial loop's grids): every float32
+    value within 2e-6 of the frame's largest value (FP64 transforms on both
+    sides, rounded to float32), for CMB and equatorial movies of Br, Bt, Vr, Vp
+    and T."""
+    from magic_amd.dynamo import DynamoRun
+    from magic_amd.movie_output import movie_frame, parse_movie_string
+    from test_movie_host import MOVIES, records
+    g = np.load(os.path.join(GOLD, "files16.npz"))
+    meta = json.load(open(os.path.join(GOLD, "files16.json")))
+    run = DynamoRun(dict(meta["cfg"]))
+    import torch
+    for k, key in (("w", "w"), ("dw", "dw"), ("z", "z"), ("s", "s"), ("b", "b"), ("db", "db"),
+                   ("aj", "aj")):
+        run.f[key].copy_(torch.from_numpy(g[k]).to(run.device))
+    for ms in MOVIES:
+        ftype, nsurf, _, stem = parse_movie_string(ms, run.p)
+        ref = records(open(os.path.join(GOLD, f"files16_{stem}.bin"), "rb").read())
+        want = np.frombuffer(ref[11], ">f4").astype(np.float64)
+        got = movie_frame(run, ftype, nsurf).astype(np.float64).reshape(-1)
+        assert got.shape == want.shape, ms
+        assert np.abs(got - want).max() <= 2e-6 * np.abs(want).max(), ms
+        assert np.frombuffer(ref[10], ">f4")[0] == np.float32(g["time"])
+    run.close()
+
+
+def test_main_run_movie_cadence(tmp_path):
+    """main.run with movies every 3 steps over 7 steps: two frames per file, and
+    the frame of step 3 is the state before step 3 (after step 2), as the
+    reference writes its last radial loop's grids."""
+    from magic_amd import config, main
+    from magic_amd.dynamo import DynamoRun
+    from magic_amd.movie_output import movie_frame
+    from test_movie_host import records
+    cfg = dict(l_max=16, n_r_max=17, ek=1e-3, ra=1e5, pr=1.0, prmag=5.0, radratio=0.35,
+               n_steps=7, dt=1e-4, log_every=7, output_dir=str(tmp_path), tag="t", n_graphs=0,
+               movies=["Br CMB", "Vr EQ"], n_movie_step=3)
+    config.configure(dict(cfg))
+    main.run(dict(cfg))
+    ref = DynamoRun(dict(cfg))
+    ref.initialize_fields()
+    ref.setup_initial_state()
+    ref.initialize_dt(1e-4)
+    dt = 1e-4
+    for n in range(1, 3):
+        dt = ref.one_step(n, dt)
+    for stem, ft, ns in (("Br_CMB_mov", 1, 1), ("Vr_EQU_mov", 4, 2)):
+        rec = records((tmp_path / f"{stem}.t").read_bytes())
+        assert len(rec) == 10 + 2 * 2, stem
+        want = movie_frame(ref, ft, ns).reshape(-1)
+        assert np.array_equal(np.frombuffer(rec[11], ">f4"), want.astype(">f4")), stem
+        assert np.frombuffer(rec[10], ">f4")[0] == np.float32(3e-4)
+    ref.close()
+    config.configure({})

@@ -17,8 +17,9 @@ The steps between two log or checkpoint points run with the dt decision on
 the device, one HIP graph per step (``DynamoRun.run_steps``): the same steps,
 bit for bit, as calling ``one_step`` in a loop.  Not written: TensorBoard
 scalars (the reference's ``SummaryWriter``), the par / dipole / heat / power /
-helicity / hemi files, radial profile files and the log.TAG narrative, and
-movies (requested movies are reported and skipped).
+helicity / hemi files, radial profile files and the log.TAG narrative.
+Movies (``movies``, ``n_movie_step``) are written as the reference writes them:
+the frame of step k holds the state before step k.
 """
 from __future__ import annotations
 
@@ -75,8 +76,12 @@ def run(cfg=None, on_log=None):
     fortran_restart = cfg.get("fortran_restart", None)
     n_graphs = int(cfg.get("n_graphs", 1))
     movies = cfg.get("movies", []) or []
-    if int(cfg.get("n_movie_step", 0)) > 0 and movies:
-        print(f"  movies {movies} are not written by this build; skipped")
+    n_movie_step = int(cfg.get("n_movie_step", 0))
+    if not (n_movie_step > 0 and movies):
+        n_movie_step, movies = 0, []
+    if movies and int(getattr(p, "radial_chunk_size", 0) or 0) > 0:
+        print("  Warning: movies disabled in chunked radial mode")   # as main.py:465-468
+        n_movie_step, movies = 0, []
     tag = cfg.get("tag", "torch")
     os.makedirs(output_dir, exist_ok=True)
 
@@ -100,6 +105,10 @@ def run(cfg=None, on_log=None):
     f_emag = open(path("e_mag_oc"), mode) if p.l_mag else None
     f_ts = open(path("timestep"), mode)
     files = [f for f in (f_csv, f_ekin, f_emag, f_ts) if f is not None]
+    mov = None
+    if movies:
+        from .movie_output import MovieWriter
+        mov = MovieWriter(r, movies, output_dir, tag)   # new files, also on restart
     try:
         writer = csv.writer(f_csv)
         if not appending:
@@ -137,6 +146,15 @@ def run(cfg=None, on_log=None):
                 stop = min(stop, (n // log_every + 1) * log_every)
             if checkpoint_every > 0:
                 stop = min(stop, (n // checkpoint_every + 1) * checkpoint_every)
+            movie_step = False
+            if mov is not None:
+                # a movie frame at step k holds the grids of step k's radial loop,
+                # i.e. the state before step k (main.py:852-868): take it, run the step
+                if (n + 1) % n_movie_step == 0:
+                    mov.capture()
+                    stop, movie_step = n + 1, True
+                else:
+                    stop = min(stop, (n // n_movie_step + 1) * n_movie_step - 1)
             k0 = len(r.dt_log)
             r.run_steps(stop - n, dt)
             for dt_actual in r.dt_log[k0:]:
@@ -146,6 +164,8 @@ def run(cfg=None, on_log=None):
                 dt = dt_actual
                 sim_time += dt
             n = stop
+            if movie_step:
+                mov.write(sim_time)
             elapsed = _time.time() - t0
             if (log_every > 0 and n % log_every == 0) or n == n_steps:
                 write_row(n, sim_time, elapsed, n - start_step)
@@ -167,4 +187,6 @@ def run(cfg=None, on_log=None):
     finally:
         for f in files:
             f.close()
+        if mov is not None:
+            mov.close()
     return _energies(r)
