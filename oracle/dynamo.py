@@ -47,8 +47,10 @@ class OracleRun:
         p = self.prm = Params(dict(cfg))
         if p.l_anel or p.l_finite_diff or p.l_cond_ic or p.raxi != 0.0 or p.nRotIC != 0:
             raise NotImplementedError("oracle step: Boussinesq Chebyshev insulating-IC only")
-        if p.time_scheme != "CNAB2":
-            raise NotImplementedError("oracle step: CNAB2 only")
+        # BPR353 (time_scheme.py:157-326): four-stage SDIRK, explicit terms at
+        # stages 1-3, implicit terms stored one stage ahead, old state at stage 4
+        self.dirk = p.time_scheme == "BPR353"
+        self.istage = 1
         self.maps = horizontal.lm_maps(p)
         self.grid = horizontal.grid_theta(p)
         self.lmarr = horizontal.lm_arrays(p, self.maps)
@@ -64,6 +66,10 @@ class OracleRun:
             setattr(self, name, z())
         self.td = {k: dict(old=z(), impl=z(), expl=[z(), z()])
                    for k in ("s", "z", "w", "p", "b", "j")}
+        if self.dirk:
+            for td in self.td.values():
+                td["implS"] = [z() for _ in range(4)]
+                td["explS"] = [z() for _ in range(4)]
         self.dt_hist = np.zeros(2)
         self.dtMax = 0.0
         self.inv = None
@@ -111,6 +117,10 @@ class OracleRun:
         dLh = self.lmarr["dLh"][:, None]
         or1, or2, or3 = rad["or1"][None], rad["or2"][None], rad["or3"][None]
         for td in self.td.values():
+            if self.dirk:   # the set-up fills implicit slot 0 (step_time.py:398-617)
+                for a in td["implS"] + td["explS"]:
+                    a[:] = 0
+                td["impl"] = td["implS"][0]
             td["old"][:] = 0; td["impl"][:] = 0
             td["expl"][0][:] = 0; td["expl"][1][:] = 0
         ds, d2s = self._dr(self.s, self.D1), self._dr(self.s, self.D2)
@@ -196,12 +206,31 @@ class OracleRun:
                               VStLM=VStLM, VxBrLM=VxBrLM, VxBtLM=VxBtLM, VxBpLM=VxBpLM)
         td = get_td(self, AdvrLM, AdvtLM, AdvpLM, dVSrLM, VStLM, VxBrLM, VxBtLM, VxBpLM)
         for k, v in td.items():
-            self.td[k]["expl"][0][:] = v
+            self._expl_slot(k)[:] = v
         dtrkc, dthkc = courant_check(p, rad, self.grid, vr, vt, vp, br, bt, bp)
         return float(dtrkc.min()), float(dthkc.min())
 
     # ------------------------------------------------------------ lm-loop
+    def _expl_slot(self, k):
+        """Explicit-term array the radial loop writes: CNAB2's newest slot, or
+        stage istage - 1 of BPR353 (step_time.py:954)."""
+        return self.td[k]["explS"][max(0, self.istage - 1)] if self.dirk else self.td[k]["expl"][0]
+
+    # BPR353 tableaux (time_scheme.py:186-200), unscaled
+    BIMP = [[0.0, 0.0, 0.0, 0.0, 0.0], [1 / 2, 1 / 2, 0.0, 0.0, 0.0],
+            [5 / 18, -1 / 9, 1 / 2, 0.0, 0.0], [1 / 2, 0.0, 0.0, 1 / 2, 0.0],
+            [1 / 4, 0.0, 3 / 4, -1 / 2, 1 / 2]]
+    BEXP = [[0.0, 0.0, 0.0, 0.0, 0.0], [1.0, 0.0, 0.0, 0.0, 0.0], [4 / 9, 2 / 9, 0.0, 0.0, 0.0],
+            [1 / 4, 0.0, 3 / 4, 0.0, 0.0], [1 / 4, 0.0, 3 / 4, 0.0, 0.0]]
+    EXP_CALC = (True, True, True, False)
+
     def _weights(self):
+        if self.dirk:   # BPR353.set_weights (time_scheme.py:230-238)
+            dt = self.dt_hist[0]
+            self.wimp_lin = [0.5 * dt]
+            self.bimp = [[v * dt for v in row] for row in self.BIMP]
+            self.bexp = [[v * dt for v in row] for row in self.BEXP]
+            return
         dt1, dt2 = self.dt_hist
         a = self.prm.alpha
         self.wimp = 1.0
@@ -210,8 +239,28 @@ class OracleRun:
 
     def _rhs(self, key):
         td = self.td[key]
+        if self.dirk:   # _assemble_rhs_mega / set_imex_rhs (step_time.py:192-207)
+            k = self.istage
+            rhs = self.bexp[k][0] * td["explS"][0] + td["old"]
+            for j in range(1, k):
+                if self.bexp[k][j] != 0.0:
+                    rhs = rhs + self.bexp[k][j] * td["explS"][j]
+            for j in range(k):
+                if self.bimp[k][j] != 0.0:
+                    rhs = rhs + self.bimp[k][j] * td["implS"][j]
+            return rhs
         return (self.wimp * td["old"] + self.wimp_lin[1] * td["impl"]
                 + self.wexp[0] * td["expl"][0] + self.wexp[1] * td["expl"][1])
+
+    def _store(self):
+        """After a stage's solve: BPR353 writes its implicit terms one slot ahead
+        and the old state only after the last stage (time_scheme.py:211-228);
+        CNAB2 shifts its explicit history and always writes both."""
+        if self.dirk:
+            for td in self.td.values():
+                td["impl"] = td["implS"][self.istage % 4]
+            return self.istage == 4
+        return True
 
     def build_all_matrices(self):
         """``step_time.py:1530-1548`` (Boussinesq Chebyshev insulating)."""
@@ -250,7 +299,7 @@ class OracleRun:
         rc[:, N + 1:2 * N - 1] = rhs_p[:, 1:N - 1]
         p0_rhs = np.zeros(N)
         l0 = self.lm_l0
-        p0_rhs[1:] = p.BuoFac * rgrav[1:] * self.s[l0, 1:].real + self.td["w"]["expl"][0][l0, 1:].real
+        p0_rhs[1:] = p.BuoFac * rgrav[1:] * self.s[l0, 1:].real + self._expl_slot("w")[l0, 1:].real
         sol = self._solve(self.inv["wp"], rc)
         sol[self.m0] = sol[self.m0].real
         w_cheb = sol[:, :N].copy()
@@ -267,12 +316,14 @@ class OracleRun:
         self.dw[:] = self._dr(self.w, D1); self.ddw[:] = self._dr(self.w, D2)
         dddw = self._dr(self.w, D3)
         self.dp[:] = self._dr(self.p, D1)
-        for k in ("s", "z", "w", "p"):
-            self.td[k]["expl"][1][:] = self.td[k]["expl"][0]
-        self.td["s"]["old"][:] = self.s
-        self.td["z"]["old"][:] = dLh * or2 * self.z
-        self.td["w"]["old"][:] = dLh * or2 * self.w
-        self.td["p"]["old"][:] = -dLh * or2 * self.dw
+        if not self.dirk:
+            for k in ("s", "z", "w", "p"):
+                self.td[k]["expl"][1][:] = self.td[k]["expl"][0]
+        if self._store():
+            self.td["s"]["old"][:] = self.s
+            self.td["z"]["old"][:] = dLh * or2 * self.z
+            self.td["w"]["old"][:] = dLh * or2 * self.w
+            self.td["p"]["old"][:] = -dLh * or2 * self.dw
         self.td["s"]["impl"][:] = p.opr * (d2s + 2.0 * or1 * self.ds - dLh * or2 * self.s)
         self.td["z"]["impl"][:] = dLh * or2 * d2z - (dLh * or2) ** 2 * self.z
         self._w_p_impl(self.w, self.dw, self.ddw, dddw, self.p, self.dp, self.s)
@@ -297,16 +348,34 @@ class OracleRun:
         self.aj[:] = radial.costf(jc)
         self.db[:] = self._dr(self.b, self.D1); self.ddb[:] = self._dr(self.b, self.D2)
         self.dj[:] = self._dr(self.aj, self.D1); self.ddj[:] = self._dr(self.aj, self.D2)
-        for k in ("b", "j"):
-            self.td[k]["expl"][1][:] = self.td[k]["expl"][0]
-        self.td["b"]["old"][:] = dLh * or2 * self.b
-        self.td["j"]["old"][:] = dLh * or2 * self.aj
+        if not self.dirk:
+            for k in ("b", "j"):
+                self.td[k]["expl"][1][:] = self.td[k]["expl"][0]
+        if self._store():
+            self.td["b"]["old"][:] = dLh * or2 * self.b
+            self.td["j"]["old"][:] = dLh * or2 * self.aj
         self.td["b"]["impl"][:] = p.opm * dLh * or2 * (self.ddb - dLh * or2 * self.b)
         self.td["j"]["impl"][:] = p.opm * dLh * or2 * (self.ddj - dLh * or2 * self.aj)
 
     # ------------------------------------------------------------ step
     def one_step(self, n_time_step: int, dt: float) -> float:
-        """``_one_step_cnab2`` (``step_time.py:1602-1637``)."""
+        """``_one_step_cnab2`` (``step_time.py:1602-1637``) or, for BPR353,
+        ``_one_step_dirk`` (``step_time.py:1640-1680``)."""
+        if self.dirk:
+            self.istage = 1
+            dt_new = dt
+            for k in range(1, 5):
+                if self.EXP_CALC[k - 1]:
+                    dtrkc, dthkc = self.radial_loop()
+                    if k == 1:
+                        l_new_dt, dt_new = dt_courant(dt, self.dtMax, dtrkc, dthkc)
+                        self.dt_hist[0] = dt_new
+                        self._weights()
+                        if n_time_step == 1 or l_new_dt:
+                            self.build_all_matrices()
+                self.lm_loop()
+                self.istage += 1
+            return dt_new
         dtrkc, dthkc = self.radial_loop()
         l_new_dt, dt_new = dt_courant(dt, self.dtMax, dtrkc, dthkc)
         dt_old = self.dt_hist[0]

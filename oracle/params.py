@@ -83,10 +83,15 @@ class Params:
         self.CorFac = 1.0 / self.ek
         self.LFfac = 1.0 / (self.ek * self.prmag)
         self.BuoFac = self.ra / self.pr
-        # time_scheme.py:13-19,33: CNAB2 defaults courfac 2.5, alffac 1.0, intfac 0.15
-        self.cnab2_courfac = 2.5 if abs(self.courfac) >= 1e3 else self.courfac
-        self.cnab2_alffac = 1.0 if abs(self.alffac) >= 1e3 else self.alffac
-        self.cnab2_intfac = 0.15 if abs(self.intfac) >= 1e3 else self.intfac
+        # time_scheme.py:13-19,33,184: scheme defaults of the CFL factors (CNAB2:
+        # courfac 2.5, alffac 1.0, intfac 0.15; BPR353: 0.8, 0.35, 0.46), used
+        # unless the configuration gives a value below 1e3
+        if self.time_scheme not in ("CNAB2", "BPR353"):
+            raise ValueError(f"Unknown time scheme: {self.time_scheme!r}")
+        dc, da, di = (2.5, 1.0, 0.15) if self.time_scheme == "CNAB2" else (0.8, 0.35, 0.46)
+        self.cnab2_courfac = dc if abs(self.courfac) >= 1e3 else self.courfac
+        self.cnab2_alffac = da if abs(self.alffac) >= 1e3 else self.alffac
+        self.cnab2_intfac = di if abs(self.intfac) >= 1e3 else self.intfac
         self.init_b1 = 3
         self.amp_b1 = 5.0
 

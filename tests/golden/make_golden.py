@@ -26,6 +26,12 @@ that each step returns passed to the next (as ``main.py`` does):
   (``time_scheme.py:56-73``) and the rebuild at ``step_time.py:1628-1629``;
   energies and dt every step, fields at steps 10 and 30.
 
+* ``bpr16`` and ``bprdt``: the BPR353 four-stage DIRK step
+  (``time_scheme.py:157-326``, ``step_time.py:1640-1680``) on the dynamo
+  benchmark (l_max=16, n_r_max=33; noise 1e-3 seed 11, dt 1e-4, 20 steps) and on
+  the dt-change case above (dtmax 1e-3, noise 1.0 seed 7, 30 steps), whose
+  Courant limit (BPR353's courfac 0.8, alffac 0.35) changes dt at stage 1.
+
 Finite-difference operator cases (``fd4_ob2``, ``fd4_ob3``): the FD branches
 of ``build_s_matrices`` (``update_s.py:66-188``), ``build_z_matrices``
 (``update_z.py:89-240``) and ``build_w_matrices``
@@ -80,6 +86,11 @@ CASES = {
                               radratio=0.35, dtmax=1e-3), dt=1e-3, steps=30,
                      noise=dict(seed=7, amp=1.0), field_steps=[10, 30], expl_step1=False,
                      diag_steps=[30]),
+    "bpr16": dict(cfg=dict(l_max=16, n_r_max=33, time_scheme="BPR353"), dt=1e-4, steps=20,
+                  noise=dict(seed=11, amp=1e-3), field_steps=[1, 20], expl_step1=False),
+    "bprdt": dict(cfg=dict(l_max=16, n_r_max=33, ek=1e-3, ra=1e5, pr=1.0, prmag=5.0,
+                           radratio=0.35, dtmax=1e-3, time_scheme="BPR353"), dt=1e-3, steps=30,
+                  noise=dict(seed=7, amp=1.0), field_steps=[10, 30], expl_step1=False),
 }
 
 FD_CASES = {

@@ -26,12 +26,14 @@ GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
 # ------------------------------------------------------------ golden trajectory
-@pytest.mark.parametrize("name", ["bench16", "c1", "dtchange"])
+@pytest.mark.parametrize("name", ["bench16", "c1", "dtchange", "bpr16", "bprdt"])
 def test_oracle_matches_reference_trajectory(name):
     """Energies every step and fields at the saved steps, from the reference's
     one_step with the returned dt passed to the next step; the dt sequence
     itself (dtchange: five Courant-driven changes in 30 steps, each with a
-    rebuild and variable-step CNAB2 weights) is matched step by step."""
+    rebuild and variable-step CNAB2 weights; bprdt: the same with the BPR353
+    four-stage DIRK, whose decision is taken at stage 1 with its own Courant
+    factors) is matched step by step.  bpr16: BPR353 at constant dt."""
     case = json.load(open(os.path.join(GOLD, f"traj_{name}.json")))
     g = np.load(os.path.join(GOLD, f"traj_{name}.npz"))
     run = OracleRun(case["cfg"])
@@ -50,8 +52,8 @@ def test_oracle_matches_reference_trajectory(name):
             for k in ("s", "z", "w", "p", "b", "aj"):
                 ref = g[f"{k}_step{n}"]
                 assert np.abs(getattr(run, k) - ref).max() <= 1e-11 * np.abs(ref).max(), (n, k)
-    if name == "dtchange":
-        assert len(set(g["dt"].tolist())) >= 5   # the fixture does take the Courant branch
+    if name in ("dtchange", "bprdt"):   # the fixture does take the Courant branch
+        assert len(set(g["dt"].tolist())) >= {"dtchange": 5, "bprdt": 4}[name]
 
 
 def test_oracle_step1_kat_from_reference_log():
