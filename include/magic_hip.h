@@ -244,6 +244,9 @@ int mh_td(const MhTdArgs* args, void* stream);
 
 typedef struct {
   const double *old, *impl, *expl0, *expl1;
+  /* optional further terms of a BPR353 stage (weights wdev[8..11]); any term
+   * pointer may be NULL (no term) */
+  const double *t4, *t5, *t6, *t7;
 } MhImexSys;
 
 typedef struct {
@@ -253,12 +256,14 @@ typedef struct {
   int L, N;
   double wimp, wl1, we0, we1, bots_l0;
   int l_mag, pad;
-  const double* wdev; /* optional: device weights {wimp, wl1, we0, we1, ...} replace the four above */
+  const double* wdev; /* optional: device weights {wimp, wl1, we0, we1, ...} replace the four above;
+                         wdev[8..11] weigh the terms t4..t7 */
 } MhRhsArgs;
 
 /* IMEX right-hand sides with boundary rows and m=0 realness: replaces
  * _assemble_rhs_mega and the BC/WP-rhs assembly of _fused_lm_loop_fast
- * (step_time.py:186-219,239-280) and set_imex_rhs (time_scheme.py:75-90). */
+ * (step_time.py:186-219,239-280) and set_imex_rhs (time_scheme.py:75-90,
+ * 240-262: CNAB2 and the BPR353 stages). */
 int mh_rhs(const MhRhsArgs* args, void* stream);
 
 typedef struct {

@@ -41,12 +41,13 @@ def _rel(a, b):
 
 
 @pytest.mark.parametrize("name,fused", [("bench16", True), ("bench16", False), ("c1", True),
-                                        ("dtchange", True)])
+                                        ("dtchange", True), ("bpr16", True), ("bprdt", True)])
 def test_step_matches_reference_trajectory(name, fused):
     """fused: one grid-space kernel; otherwise separate FFT / products / FFT kernels.
     The dt each step returns is passed to the next (dtchange: the reference
     changes dt five times in 30 steps, each with a rebuild and variable-step
-    CNAB2 weights)."""
+    CNAB2 weights).  bpr16 / bprdt: the BPR353 four-stage DIRK step of the
+    reference at constant dt and with Courant-driven dt changes."""
     case, g = _load(name)
     run = _run(case, fused)
     dt = case["dt"]
@@ -200,7 +201,7 @@ def test_fused_grid_step_matches_separate_kernels(l_max):
     assert np.allclose(cf, cg, rtol=1e-14, atol=0), (cf, cg)
 
 
-@pytest.mark.parametrize("name", ["bench16", "c1"])
+@pytest.mark.parametrize("name", ["bench16", "c1", "bprdt"])
 def test_graph_replay_matches_eager(name):
     """The step's radial and lm phases captured as HIP graphs (one per parity of
     the explicit-history buffers, weights read from device memory) and replayed
@@ -219,7 +220,8 @@ def test_graph_replay_matches_eager(name):
         assert graph.energies() == eager.energies(), n
         for k in ("s", "z", "w", "p", "b", "aj"):
             assert torch.equal(graph.f[k], eager.f[k]), (n, k)
-    assert len(graph._graphs) == 4
+    # two phases x two parities (CNAB2), or 3 radial + 4 lm stage graphs (BPR353)
+    assert len(graph._graphs) == (7 if graph.dirk else 4)
 
 
 # Courant-limited case: noise of amplitude 1 on s makes the flow strong enough

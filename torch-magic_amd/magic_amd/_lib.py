@@ -88,7 +88,8 @@ class TdArgs(C.Structure):
 
 
 class ImexSys(C.Structure):
-    _fields_ = [("old", P), ("impl", P), ("expl0", P), ("expl1", P)]
+    _fields_ = [("old", P), ("impl", P), ("expl0", P), ("expl1", P),
+                ("t4", P), ("t5", P), ("t6", P), ("t7", P)]
 
 
 class RhsArgs(C.Structure):

@@ -88,6 +88,9 @@ def checkpoint_state(run, step: int | None = None, sim_time: float | None = None
 
 def save_checkpoint(run, path, step: int | None = None, sim_time: float | None = None) -> None:
     """Write the restart file of `run` (reference ``main.save_checkpoint``)."""
+    if getattr(run, "dirk", False):
+        raise NotImplementedError("restart files hold CNAB2 time arrays; BPR353 runs are not "
+                                  "written to them")
     torch.save(checkpoint_state(run, step, sim_time), path)
 
 
@@ -102,6 +105,9 @@ def _check_zero(state: dict, key: str) -> None:
 
 def apply_checkpoint_state(run, state: dict) -> tuple[int, float]:
     """Restore `run` from a restart dict; returns (step, sim_time)."""
+    if getattr(run, "dirk", False):
+        raise NotImplementedError("restart files hold CNAB2 time arrays; BPR353 runs are not "
+                                  "restarted from them")
     ref = run.f["s"]
     shape = tuple(ref.shape)
     for name, key in FIELD_NAMES.items():

@@ -83,20 +83,32 @@ typedef MhImexSys ImexSys;
 
 typedef MhRhsArgs RhsArgs;
 
-__device__ __forceinline__ double2 imex(const ImexSys& s, long i, const RhsArgs& a) {
+// weights of the optional terms t4..t7 (BPR353 stages)
+struct ExtraW {
+  double w[4];
+};
+
+__device__ __forceinline__ double2 imex(const ImexSys& s, long i, const RhsArgs& a,
+                                        const ExtraW& x) {
   double2 v = cscale(C2(s.old)[i], a.wimp);
-  v = cadd(v, cscale(C2(s.impl)[i], a.wl1));
-  v = cadd(v, cscale(C2(s.expl0)[i], a.we0));
-  v = cadd(v, cscale(C2(s.expl1)[i], a.we1));
+  if (s.impl) v = cadd(v, cscale(C2(s.impl)[i], a.wl1));
+  if (s.expl0) v = cadd(v, cscale(C2(s.expl0)[i], a.we0));
+  if (s.expl1) v = cadd(v, cscale(C2(s.expl1)[i], a.we1));
+  if (s.t4) v = cadd(v, cscale(C2(s.t4)[i], x.w[0]));
+  if (s.t5) v = cadd(v, cscale(C2(s.t5)[i], x.w[1]));
+  if (s.t6) v = cadd(v, cscale(C2(s.t6)[i], x.w[2]));
+  if (s.t7) v = cadd(v, cscale(C2(s.t7)[i], x.w[3]));
   return v;
 }
 
 __global__ __launch_bounds__(256) void k_rhs(RhsArgs a) {
+  ExtraW x = {{0.0, 0.0, 0.0, 0.0}};
   if (a.wdev) {  // weights of this step from device memory (launch arguments fixed)
     a.wimp = a.wdev[0];
     a.wl1 = a.wdev[1];
     a.we0 = a.wdev[2];
     a.we1 = a.wdev[3];
+    for (int q = 0; q < 4; ++q) x.w[q] = a.wdev[8 + q];
   }
   const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const int N = a.N;
@@ -106,20 +118,22 @@ __global__ __launch_bounds__(256) void k_rhs(RhsArgs a) {
   const bool m0 = a.m[lm] == 0;
   auto fix = [&](double2 v) { return m0 ? make_double2(v.x, 0.0) : v; };
   double2 vs = bdry ? make_double2((r == N - 1 && lm == 0) ? a.bots_l0 : 0.0, 0.0)
-                    : imex(a.sys[0], idx, a);
+                    : imex(a.sys[0], idx, a, x);
   W2(a.rs)[idx] = fix(vs);
-  W2(a.rz)[idx] = bdry ? make_double2(0.0, 0.0) : fix(imex(a.sys[1], idx, a));
+  W2(a.rz)[idx] = bdry ? make_double2(0.0, 0.0) : fix(imex(a.sys[1], idx, a, x));
   const long iw = (long)lm * 2 * N + r;
-  W2(a.rwp)[iw] = bdry ? make_double2(0.0, 0.0) : fix(imex(a.sys[2], idx, a));
-  W2(a.rwp)[iw + N] = bdry ? make_double2(0.0, 0.0) : fix(imex(a.sys[3], idx, a));
+  W2(a.rwp)[iw] = bdry ? make_double2(0.0, 0.0) : fix(imex(a.sys[2], idx, a, x));
+  W2(a.rwp)[iw + N] = bdry ? make_double2(0.0, 0.0) : fix(imex(a.sys[3], idx, a, x));
   if (a.l_mag) {
-    W2(a.rb)[idx] = bdry ? make_double2(0.0, 0.0) : fix(imex(a.sys[4], idx, a));
-    W2(a.rj)[idx] = bdry ? make_double2(0.0, 0.0) : fix(imex(a.sys[5], idx, a));
+    W2(a.rb)[idx] = bdry ? make_double2(0.0, 0.0) : fix(imex(a.sys[4], idx, a, x));
+    W2(a.rj)[idx] = bdry ? make_double2(0.0, 0.0) : fix(imex(a.sys[5], idx, a, x));
   }
 }
 
 extern "C" int mh_rhs(const MhRhsArgs* args, void* stream) {
   const RhsArgs& a = *args;
+  for (int k = 0; k < 6; ++k)
+    if ((a.sys[k].t4 || a.sys[k].t5 || a.sys[k].t6 || a.sys[k].t7) && !a.wdev) return MH_EINVAL;
   const long n = (long)a.L * a.N;
   hipLaunchKernelGGL(k_rhs, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, mh_stream(stream), a);
   return mh_launch_status();

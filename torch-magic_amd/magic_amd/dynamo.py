@@ -329,8 +329,23 @@ class DynamoRun:
         # its HIP graph) stays valid for the whole run
         self._E = (self.expl_cur, self.expl_prev)
         self._par = 0
-        self.wdev = torch.zeros(8, dtype=F64, device=dev)
-        self.w_host = torch.zeros(8, dtype=F64).pin_memory()
+        # IMEX weights on the device: [0..3] the rhs weights of old, impl, expl0,
+        # expl1, [4] the buoyancy factor wimp_lin0 * BuoFac, [8..11] the weights
+        # of the further terms of a BPR353 stage
+        self.wdev = torch.zeros(16, dtype=F64, device=dev)
+        self.w_host = torch.zeros(16, dtype=F64).pin_memory()
+        # BPR353 (time_scheme.py:157-326): four stages per step, explicit terms in
+        # slots 0..2 (slot 3 is never written: stage 4's pressure rhs reads it as
+        # zeros, as the reference does), implicit terms one slot ahead, the old
+        # state written after stage 4; weights per stage in wdev_stage
+        self.dirk = p.time_scheme.upper() == "BPR353"
+        self.stage = 0
+        if self.dirk:
+            zl = lambda: {k: torch.zeros(L, N, dtype=C128, device=dev) for k in _SYS}
+            self.implS = [self.impl, zl(), zl(), zl()]
+            self.explS = [self.expl_cur, self.expl_prev, zl(), zl()]
+            self.wdev_stage = torch.zeros(4 * 16, dtype=F64, device=dev)
+            self.w_stage_host = torch.zeros(4 * 16, dtype=F64).pin_memory()
         self._plans = {}
         self._graphs = {}
         self._cap_stream = None
@@ -361,8 +376,9 @@ class DynamoRun:
             self.timers.setdefault(name, []).append((self._sec_ev, end))
 
     def _plan(self, kind):
-        """The recorded launch plan of a phase for the current parity."""
-        key = (kind, self._par)
+        """The recorded launch plan of a phase for the current parity (for
+        BPR353: for the current stage)."""
+        key = (kind, "stage", self.stage) if self.dirk else (kind, self._par)
         plan = self._plans.get(key)
         if plan is None:
             with _lib.record() as rec:
@@ -752,7 +768,8 @@ class DynamoRun:
     def setup_initial_state(self):
         """Derivatives, old states and implicit terms from the fields (``step_time.py:398-617``)."""
         self._par = 0 if self.expl_cur is self._E[0] else 1
-        for d in (self.old, self.impl, self.expl_cur, self.expl_prev):
+        extra = (self.explS[2:] + self.implS[1:]) if self.dirk else []
+        for d in [self.old, self.impl, self.expl_cur, self.expl_prev] + extra:
             for t in d.values():
                 t.zero_()
         self._rowop(self.jobs_setup)
@@ -776,7 +793,11 @@ class DynamoRun:
         self.dtMax = dt
 
     # ------------------------------------------------------------ kernels
-    def _impl(self):
+    def _impl(self, old=None, impl=None):
+        """Implicit terms and old states of the new fields into `impl` / `old`
+        (default: the run's own arrays)."""
+        old = self.old if old is None else old
+        impl = self.impl if impl is None else impl
         a = _lib.ImplArgs()
         a.lc, a.rc = self._lmcoef(), self._radcoef()
         f, t = self.f, self.tmp
@@ -786,8 +807,8 @@ class DynamoRun:
                      ("ddb", f["ddb"]), ("aj", f["aj"]), ("ddj", f["ddj"])):
             setattr(a, k, ptr(v))
         for k in _SYS:
-            setattr(a, "old_" + k, ptr(self.old[k]))
-            setattr(a, "impl_" + k, ptr(self.impl[k]))
+            setattr(a, "old_" + k, ptr(old[k]))
+            setattr(a, "impl_" + k, ptr(impl[k]))
         a.opr, a.opm, a.BuoFac = self.p.opr, self.p.opm, self.p.BuoFac
         a.l_mag, a.lm_l0 = int(self.p.l_mag), 0
         _lib.call("mh_impl", _lib.ref(a), stream_handle())
@@ -921,32 +942,118 @@ class DynamoRun:
     def _lm_launches(self):
         p = self.p
         a = _lib.RhsArgs()
-        for i, k in enumerate(_SYS):
-            a.sys[i].old, a.sys[i].impl = ptr(self.old[k]), ptr(self.impl[k])
-            a.sys[i].expl0, a.sys[i].expl1 = ptr(self.expl_cur[k]), ptr(self.expl_prev[k])
+        wdev = ptr(self.wdev)
+        ew0 = self.expl_cur["w"]
+        if self.dirk:
+            # stage k: old + sum_{j<k} bexp[k][j] expl_j + sum_{j<k} bimp[k][j] impl_j
+            # (time_scheme.py:240-262; weights per stage in wdev_stage)
+            k = self.stage
+            E, I = self.explS, self.implS
+            for i, key in enumerate(_SYS):
+                sy = a.sys[i]
+                sy.old, sy.impl, sy.expl0 = ptr(self.old[key]), ptr(I[0][key]), ptr(E[0][key])
+                if k >= 2:
+                    sy.expl1, sy.t4 = ptr(E[1][key]), ptr(I[1][key])
+                if k >= 3:
+                    sy.t5, sy.t6 = ptr(E[2][key]), ptr(I[2][key])
+                if k >= 4:
+                    sy.t7 = ptr(I[3][key])
+            wdev = ptr(self.wdev_stage) + 8 * 16 * (k - 1)
+            ew0 = E[max(0, k - 1)]["w"]   # stage 4 reads slot 3: zeros (step_time.py:284-287)
+        else:
+            for i, k in enumerate(_SYS):
+                a.sys[i].old, a.sys[i].impl = ptr(self.old[k]), ptr(self.impl[k])
+                a.sys[i].expl0, a.sys[i].expl1 = ptr(self.expl_cur[k]), ptr(self.expl_prev[k])
         a.rs, a.rz, a.rb, a.rj, a.rwp = (ptr(self.rhs[k]) for k in ("s", "z", "b", "j", "wp"))
         a.m, a.L, a.N = ptr(self.res.d["m"]), self.L, self.N
-        a.wdev = ptr(self.wdev)
+        a.wdev = wdev
         a.bots_l0, a.l_mag = math.sqrt(4.0 * math.pi), int(p.l_mag)
         with self._t("rhs"):
             _lib.call("mh_rhs", _lib.ref(a), stream_handle())
         with self._t("rowop_szbj"):
             self._rowop(self.jobs1)
         bq = _lib.BuoyArgs()
-        bq.rwp, bq.s, bq.ew0 = ptr(self.rhs["wp"]), ptr(self.f["s"]), ptr(self.expl_cur["w"])
+        bq.rwp, bq.s, bq.ew0 = ptr(self.rhs["wp"]), ptr(self.f["s"]), ptr(ew0)
         bq.rgrav, bq.m, bq.L, bq.N = ptr(self.res.d["rgrav"]), ptr(self.res.d["m"]), self.L, self.N
-        bq.BuoFac, bq.lm_l0, bq.wdev = p.BuoFac, 0, ptr(self.wdev)
+        bq.BuoFac, bq.lm_l0, bq.wdev = p.BuoFac, 0, wdev
         with self._t("buoy"):
             _lib.call("mh_buoy", _lib.ref(bq), stream_handle())
         with self._t("rowop_wp"):
             self._rowop(self.jobs2)
         with self._t("impl"):
-            self._impl()
+            if self.dirk:
+                # implicit terms one slot ahead; the old state only after the last
+                # stage (time_scheme.py:211-228): earlier stages write it into the
+                # right-hand-side arrays, which the solves have consumed
+                k = self.stage
+                r = self.rhs
+                junk = {"s": r["s"], "z": r["z"], "b": r["b"], "j": r["j"],
+                        "w": r["wp"][:, :self.N], "p": r["wp"][:, self.N:]}
+                self._impl(old=self.old if k == 4 else junk, impl=self.implS[k % 4])
+            else:
+                self._impl()
 
     def one_step(self, n_time_step: int, dt: float) -> float:
-        """One CNAB2 step (reference ``step_time.py:1574-1637``); returns the dt used."""
+        """One CNAB2 step (reference ``step_time.py:1574-1637``), or one BPR353
+        step (``step_time.py:1640-1680``); returns the dt used."""
+        if self.dirk:
+            return self._one_step_dirk(n_time_step, dt)
         self.begin_step()
         return self.finish_step(n_time_step, dt)
+
+    # BPR353 tableaux (time_scheme.py:186-200), unscaled; row k is stage k
+    BIMP = ((0.0, 0.0, 0.0, 0.0, 0.0), (1 / 2, 1 / 2, 0.0, 0.0, 0.0),
+            (5 / 18, -1 / 9, 1 / 2, 0.0, 0.0), (1 / 2, 0.0, 0.0, 1 / 2, 0.0),
+            (1 / 4, 0.0, 3 / 4, -1 / 2, 1 / 2))
+    BEXP = ((0.0, 0.0, 0.0, 0.0, 0.0), (1.0, 0.0, 0.0, 0.0, 0.0), (4 / 9, 2 / 9, 0.0, 0.0, 0.0),
+            (1 / 4, 0.0, 3 / 4, 0.0, 0.0), (1 / 4, 0.0, 3 / 4, 0.0, 0.0))
+
+    def _dirk_weights(self, dt: float):
+        """The right-hand-side weights of the four stages for step size dt, laid
+        out for the stage plans: per stage [1, bimp_0, bexp_0, bexp_1, wimp_lin0
+        BuoFac, -, -, -, bimp_1, bexp_2, bimp_2, bimp_3] (BPR353.set_weights,
+        time_scheme.py:230-238; wimp_lin0 = dt / 2)."""
+        w = self.w_stage_host
+        w.zero_()
+        for k in range(1, 5):
+            B = [v * dt for v in self.BIMP[k]]
+            E = [v * dt for v in self.BEXP[k]]
+            o = 16 * (k - 1)
+            w[o + 0], w[o + 1], w[o + 2] = 1.0, B[0], E[0]
+            w[o + 3], w[o + 8] = E[1], B[1]
+            w[o + 9], w[o + 10], w[o + 11] = E[2], B[2], B[3]
+            w[o + 4] = 0.5 * dt * self.p.BuoFac
+        self.wdev_stage.copy_(w, non_blocking=True)
+
+    def _one_step_dirk(self, n_time_step: int, dt: float) -> float:
+        """BPR353: stages 1-3 run the radial loop into explicit slot k-1; stage 1
+        also takes the dt decision (this scheme's Courant factors) and rebuilds
+        the operators for wimp_lin0 = dt/2 on the first step or a new dt; every
+        stage runs the lm loop (step_time.py:1640-1680)."""
+        self._check_info()
+        dt_new = dt
+        for k in range(1, 5):
+            self.stage = k
+            if k <= 3:
+                self.expl_cur = self.explS[k - 1]
+                self._exec("radial")
+                if k == 1:
+                    self._cfl_copy()
+                    dtrkc, dthkc = self._cfl_values()
+                    l_new_dt, dt_new = dt_courant(dt, self.dtMax, dtrkc, dthkc)
+                    if self.dt_schedule is not None and n_time_step <= len(self.dt_schedule):
+                        dt_new = float(self.dt_schedule[n_time_step - 1])
+                        l_new_dt = dt_new != dt
+                    self.dt_log.append(dt_new)
+                    self.n_step = n_time_step
+                    self.dt_hist = [dt_new, dt_new]
+                    self._dirk_weights(dt_new)
+                    if n_time_step == 1 or l_new_dt or self.wimp_lin0 is None:
+                        self.build_all_matrices(0.5 * dt_new)
+            self._exec("lm")
+        self.expl_cur = self.explS[0]
+        self.time += dt_new
+        return dt_new
 
     def begin_step(self):
         """Queue the radial loop and the copy of the Courant maxima (no host wait)."""
@@ -1072,10 +1179,15 @@ class DynamoRun:
             self._check_info()
         return st.dt
 
+    def _no_dirk(self, what):
+        if self.dirk:
+            raise NotImplementedError(f"{what}: BPR353 runs are stepped with one_step / run_steps")
+
     def step_async(self):
         """Queue one whole step with the decisions on the device (one HIP graph
         when use_graphs); no host wait.  Call ctl_begin(dt) once before and
         ctl_end() to bring the stepping state back to the host."""
+        self._no_dirk("step_async")
         self._exec("step")
         self.expl_cur, self.expl_prev = self.expl_prev, self.expl_cur
         self._par ^= 1
@@ -1111,7 +1223,11 @@ class DynamoRun:
         one launch sequence (one HIP graph replay when use_graphs) and the host
         never waits inside the loop.  Same decisions, weights and results as
         n_steps calls of one_step; dt as for one_step's first call.  Returns the
-        dt of the last step."""
+        dt of the last step.  BPR353 steps are host-controlled one_step calls."""
+        if self.dirk:
+            for _ in range(n_steps):
+                dt = self.one_step(self.n_step + 1, dt)
+            return dt
         self.ctl_begin(dt)
         for _ in range(n_steps):
             self.step_async()

@@ -129,14 +129,17 @@ class Params:
         self.CorFac = 1.0 / self.ek
         self.LFfac = 1.0 / (self.ek * self.prmag)
         self.BuoFac = self.ra / self.pr
-        # CNAB2 scheme defaults unless the namelist value is below 1000
-        self.cour_fac = 2.5 if abs(self.courfac) >= 1e3 else self.courfac
-        self.alf_fac = 1.0 if abs(self.alffac) >= 1e3 else self.alffac
-        self.int_fac = 0.15 if abs(self.intfac) >= 1e3 else self.intfac
+        # the time scheme's CFL defaults unless the namelist value is below 1000
+        # (time_scheme.py:13-19,33,184: CNAB2 2.5, 1.0, 0.15; BPR353 0.8, 0.35, 0.46)
+        dc, da, di = (0.8, 0.35, 0.46) if self.time_scheme.upper() == "BPR353" else (2.5, 1.0, 0.15)
+        self.cour_fac = dc if abs(self.courfac) >= 1e3 else self.courfac
+        self.alf_fac = da if abs(self.alffac) >= 1e3 else self.alffac
+        self.int_fac = di if abs(self.intfac) >= 1e3 else self.intfac
         self.amp_b1 = 5.0
 
     def check_step_supported(self):
-        """The device step covers the Boussinesq Chebyshev insulating-IC CNAB2 path."""
+        """The device step covers the Boussinesq Chebyshev insulating-IC path with
+        CNAB2 or BPR353."""
         bad = []
         if self.l_anel:
             bad.append("anelastic (strat > 0)")
@@ -148,7 +151,7 @@ class Params:
             bad.append("composition")
         if self.nRotIC != 0:
             bad.append("rotating inner core")
-        if self.time_scheme.upper() != "CNAB2":
+        if self.time_scheme.upper() not in ("CNAB2", "BPR353"):
             bad.append(f"time scheme {self.time_scheme}")
         if self.l_correct_AMz or self.l_correct_AMe:
             bad.append("angular-momentum correction")
