@@ -169,6 +169,174 @@ def copy_calibration(reps=3):
                                            "copies": reps}}))
 
 
+def run_case(precision, steps, warmup, dt_schedule=None, section_steps=20):
+    """Time `steps` CNAB2 steps (one HIP graph each, dt decided on the device)
+    after `warmup` untimed ones, then time the kernel sections over
+    `section_steps` further steps run eagerly with HIP events between sections.
+    dt_schedule: step sizes to replay instead of the run's own Courant decisions."""
+    import torch
+    from magic_amd.dynamo import DynamoRun
+    run = DynamoRun(dict(WORKLOAD, precision=precision), device="cuda:0")
+    run.dt_schedule = dt_schedule
+    run.use_graphs = True
+    run.initialize_fields(noise_amp=NOISE["amp"], noise_seed=NOISE["seed"])
+    run.setup_initial_state()
+    run.initialize_dt(WORKLOAD["dtmax"])
+    dt = run.run_steps(warmup, run.dtMax)           # includes the graph captures
+    run.ctl_begin(dt)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        run.step_async()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    dt = run.ctl_end()
+    out = dict(elapsed=t1 - t0, energies=run.energies(), spectra=run.spectra(), dt=dt,
+               rebuilds=run.n_rebuild, p=run.p, N=run.N, grid_rb=run.grid_rb,
+               syn_shape=(run.syn_vec_shape, run.syn_scal_shape), ana_waves=run.ana_waves,
+               rowop_wide=run.rowop_wide)
+    run.enable_timers(True)
+    run.run_steps(section_steps, dt)
+    out["secs"] = run.timer_summary()
+    out["dt_log"] = list(run.dt_log)
+    run.enable_timers(False)
+    run.close()
+    del run
+    torch.cuda.empty_cache()
+    return out
+
+
+def fd_banded_case():
+    """Config 3: FD4 n_r=512 l_max=255 anelastic (N_rho=3, polytropic index 2)
+    banded S, Z, W operators: batched band LU of every degree and the solve of
+    all 32 896 complex right-hand sides per system in one launch each."""
+    import numpy as np
+    import torch
+    from magic_amd.fd import FDOperators
+    n, l_max = 512, 255
+    t0 = time.perf_counter()
+    ops = FDOperators(n, l_max, 0.6 * 1e-4, fd_order=4, fd_order_bound=2)
+    host_s = time.perf_counter() - t0
+    lidx = np.repeat(np.arange(l_max + 1), np.arange(1, l_max + 2)).astype(np.int32)
+    L = len(lidx)
+    rng = np.random.default_rng(NOISE["seed"])
+    rhs = torch.as_tensor(rng.standard_normal((L, n)) + 1j * rng.standard_normal((L, n))).cuda()
+    d_l = torch.as_tensor(lidx).cuda()
+    rows = torch.argsort(d_l.to(torch.int64), stable=True).to(torch.int32)
+    out = torch.empty_like(rhs)
+    ev = lambda: torch.cuda.Event(enable_timing=True)
+    ops.factor()                                  # warm-up (module load, first launch)
+    ops.solve("W", rhs, d_l, out, rows)
+    torch.cuda.synchronize()
+    e0, e1 = ev(), ev()
+    e0.record()
+    ops.factor()
+    e1.record()
+    solve_ms = {}
+    for name in ("S", "Z", "W"):
+        a, b = ev(), ev()
+        a.record()
+        ops.solve(name, rhs, d_l, out, rows)
+        b.record()
+        solve_ms[name] = (a, b)
+    torch.cuda.synchronize()
+    fac_ms = e0.elapsed_time(e1)
+    sm = {k: a.elapsed_time(b) for k, (a, b) in solve_ms.items()}
+    bytes_solve = 2.0 * L * n * 16.0                # right-hand sides in, solutions out
+    return dict(workload="FD4 n_r=512 (stretched), l_max=255, polytropic N_rho=3 n=2; "
+                         "S, Z kl=ku=2, W kl=ku=3; 32896 complex RHS per system",
+                host_build_s=host_s, factor_ms_all_systems=fac_ms, solve_ms=sm,
+                solve_GBps={k: bytes_solve / (v * 1e-3) / 1e9 for k, v in sm.items()},
+                info_ok=all(int(ops.dev[k]["info"].abs().sum()) == 0 for k in ("S", "Z", "W")))
+
+
+def sht_case():
+    """Config 2: SHT round trip at l_max=383 (n_theta=576, n_phi=1152) on 3 x 97 = 291
+    fields with seeded Gaussian coefficients scaled by (l+1)^-2 (m=0 real), in FP64 and
+    in mixed precision: ms per direction (synthesis + inverse FFT; forward FFT +
+    analysis) and the max / RMS round-trip error relative to the largest coefficient
+    (analysis(synthesis(x)) = x exactly for band-limited x), plus the mixed grid
+    against the FP64 grid."""
+    import numpy as np
+    import torch
+    from magic_amd.device import Resolution
+    from magic_amd.params import Params
+    from magic_amd.sht import SHT
+    p = Params({"l_max": 383, "n_r_max": 97})
+    res = Resolution(p)
+    B = 3 * 97
+    rng = np.random.default_rng(NOISE["seed"])
+    l, m = res.maps["l"], res.maps["m"]
+    S = (rng.standard_normal((p.lm_max, B)) + 1j * rng.standard_normal((p.lm_max, B))) \
+        * ((l + 1.0) ** -2.0)[:, None]
+    S[m == 0] = S[m == 0].real
+    dS = torch.from_numpy(S).cuda()
+    smax = np.abs(S).max()
+    ev = lambda: torch.cuda.Event(enable_timing=True)
+    out, grids = {}, {}
+    flops = 4.0 * p.lm_max * (p.n_theta // 2) * B
+    for prec in ("fp64",) + MIXED:
+        sht = SHT(res, precision=prec)
+        g = sht.scal_to_spat(dS)
+        back = sht.scal_to_SH(g)                       # warm-up (tables, module load)
+        torch.cuda.synchronize()
+        t = {"syn": [], "ana": []}
+        for _ in range(3):
+            a, b, c = ev(), ev(), ev()
+            a.record()
+            g = sht.scal_to_spat(dS)
+            b.record()
+            back = sht.scal_to_SH(g)
+            c.record()
+            torch.cuda.synchronize()
+            t["syn"].append(a.elapsed_time(b))
+            t["ana"].append(b.elapsed_time(c))
+        err = (back - dS).abs().cpu().numpy()
+        out[prec] = dict(synthesis_ms=float(np.min(t["syn"])), analysis_ms=float(np.min(t["ana"])),
+                         roundtrip_max_err=float(err.max() / smax),
+                         roundtrip_rms_err=float(np.sqrt((err ** 2).mean()) / smax),
+                         legendre_tflops_synthesis_upper=flops / (np.min(t["syn"]) * 1e-3) / 1e12)
+        grids[prec] = g
+    gmax = grids["fp64"].abs().max()
+    for prec in MIXED:
+        d = (grids[prec] - grids["fp64"]).abs()
+        out[f"{prec}_grid_vs_fp64"] = dict(max_err=float(d.max() / gmax),
+                                           rms_err=float(d.pow(2).mean().sqrt() / gmax))
+    out["workload"] = "l_max=383, 576 x 1152 grid, 291 fields, coefficients N(0,1) (l+1)^-2"
+    return out
+
+
+def ensemble_case(n_seq):
+    """Config 5: 64 Boussinesq runs (l_max in {32, 64, 85, 127}, n_r in {33, 49, 65},
+    E log-uniform [1e-4, 1e-3], Ra_mod uniform [60, 200], 500 steps each) arriving as a
+    seeded Poisson process at 2/s, served with continuous admission (16 slots, HBM
+    budget); baseline: the first n_seq of the same requests run back to back, one at
+    a time, in plain FP64 (arrivals ignored)."""
+    import numpy as np
+    from magic_amd.serve import EnsembleServer, config5_requests, serve_sequential
+    cfgs = [r.cfg for r in config5_requests()]
+    srv = EnsembleServer(max_slots=16)
+    tw = time.perf_counter()
+    srv.warm(cfgs)                 # device tables of the served resolutions, before serving
+    warm_s = time.perf_counter() - tw
+    _, mix = srv.serve(config5_requests(), realtime=True)
+    # the same 64 runs all waiting at t=0: throughput of the server when saturated
+    sat_srv = EnsembleServer(max_slots=16)
+    sat_srv.warm(cfgs)
+    _, sat = sat_srv.serve(config5_requests(), realtime=False)
+    _, seq = serve_sequential(config5_requests()[:n_seq], warm=True)
+    return dict(workload="64 runs x 500 steps, Poisson arrivals 2/s, l_max {32,64,85,127}, "
+                         "n_r {33,49,65}, FP64; each step one HIP graph with the dt decision "
+                         "on the device, one step per run per round",
+                table_warm_s=warm_s,
+                server=dict(mix, max_active=max(n for _, n in srv.occupancy),
+                            admit_ms_mean=1e3 * float(np.mean(srv.admit_s)),
+                            admit_ms_max=1e3 * float(np.max(srv.admit_s)),
+                            admit_stage_ms_max=1e3 * float(np.max(srv.stage_s))),
+                saturated=dict(sat, max_active=max(n for _, n in sat_srv.occupancy)),
+                sequential=dict(seq, sample=f"first {n_seq} of the 64 requests back to back"))
+
+
 def measured_peaks():
     """Peaks measured on this device, beside the published ones: HBM from a
     2 GiB device-to-device copy (bytes read + written), FP64 and FP32 matrix

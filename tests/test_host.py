@@ -225,3 +225,15 @@ def test_solver_entry_points_check_shapes_before_launch():
         algebra.chunked_solve_complex(inv, lidx, rhs[:, :3])
     with pytest.raises(ValueError):
         algebra.chunked_solve_complex(inv[:, :, :3], lidx, rhs)
+
+
+def test_bench_defines_every_function_main_calls():
+    """bench.py's main() calls module functions by name; a missing one would
+    only fail on the GPU box at the end of a run."""
+    import builtins
+    import dis
+    import bench
+    called = {ins.argval for ins in dis.get_instructions(bench.main)
+              if ins.opname in ("LOAD_GLOBAL", "LOAD_NAME")}
+    missing = [n for n in called if not hasattr(bench, n) and not hasattr(builtins, n)]
+    assert not missing, missing
