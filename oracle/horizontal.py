@@ -200,4 +200,9 @@ def plm_tables(p: Params, grid=None):
     th = grid["theta_ord"][:nhs]
     w = grid["gauss"][:nhs]
     P, dP = plm_theta(th, p.l_max)
+    if p.minc > 1:   # the rows of the orders m = 0, minc, ... in the run's lm order
+        start = np.concatenate([[0], np.cumsum(p.l_max + 1 - np.arange(p.l_max + 1))])
+        rows = np.concatenate([np.arange(start[m], start[m + 1])
+                               for m in range(0, p.m_max + 1, p.minc)])
+        P, dP = P[rows], dP[rows]
     return dict(Plm=P, dPlm=dP, wPlm=2.0 * math.pi * w * P, wdPlm=2.0 * math.pi * w * dP)

@@ -51,15 +51,17 @@ class Params:
         c = dict(_DEFAULTS)
         c.update(self.cfg)
         self.l_max = int(c["l_max"])
-        self.minc = int(c["minc"])
-        if self.minc != 1:
-            raise NotImplementedError("oracle supports minc=1 only")
+        self.minc = int(c["minc"])   # azimuthal symmetry: orders m = 0, minc, 2 minc, ...
+        if self.minc < 1:
+            raise ValueError(f"minc must be >= 1, got {self.minc}")
         self.m_max = self.l_max
         self.n_r_max = int(c["n_r_max"] if c["n_r_max"] is not None else 2 * self.l_max + 1)
         self.n_cheb_max = int(c["n_cheb_max"] if c["n_cheb_max"] is not None else self.n_r_max)
         nalias = 20
-        self.n_phi_max = prime_decomposition(2 * ((30 * self.l_max) // nalias))
-        self.n_theta_max = self.n_phi_max // 2
+        # params.py:69-71: n_phi_max covers one 2 pi / minc sector
+        self.n_phi_tot = prime_decomposition(2 * ((30 * self.l_max) // nalias))
+        self.n_phi_max = self.n_phi_tot // self.minc
+        self.n_theta_max = self.n_phi_tot // 2
         self.n_m_max = self.m_max // self.minc + 1
         self.lm_max = sum(self.l_max - m + 1 for m in range(0, self.m_max + 1, self.minc))
         for k in ("ra", "ek", "pr", "prmag", "radratio", "raxi", "sc", "strat", "polind",

@@ -32,6 +32,11 @@ that each step returns passed to the next (as ``main.py`` does):
   the dt-change case above (dtmax 1e-3, noise 1.0 seed 7, 30 steps), whose
   Courant limit (BPR353's courfac 0.8, alffac 0.35) changes dt at stage 1.
 
+* ``minc2`` and ``bprminc4``: azimuthal symmetry (``params.py:60-76``: orders
+  m = 0, minc, 2 minc, ..., one 2 pi / minc sector of n_phi_tot / minc
+  longitudes) with CNAB2 (l_max=16, minc=2) and with BPR353 (l_max=32, minc=4),
+  n_r_max=33, 20 steps.
+
 Finite-difference operator cases (``fd4_ob2``, ``fd4_ob3``): the FD branches
 of ``build_s_matrices`` (``update_s.py:66-188``), ``build_z_matrices``
 (``update_z.py:89-240``) and ``build_w_matrices``
@@ -91,6 +96,11 @@ CASES = {
     "bprdt": dict(cfg=dict(l_max=16, n_r_max=33, ek=1e-3, ra=1e5, pr=1.0, prmag=5.0,
                            radratio=0.35, dtmax=1e-3, time_scheme="BPR353"), dt=1e-3, steps=30,
                   noise=dict(seed=7, amp=1.0), field_steps=[10, 30], expl_step1=False),
+    "minc2": dict(cfg=dict(l_max=16, n_r_max=33, minc=2), dt=1e-4, steps=20,
+                  noise=dict(seed=13, amp=1e-3), field_steps=[1, 20], expl_step1=False),
+    "bprminc4": dict(cfg=dict(l_max=32, n_r_max=33, minc=4, time_scheme="BPR353"), dt=1e-4,
+                     steps=20, noise=dict(seed=17, amp=1e-3), field_steps=[1, 20],
+                     expl_step1=False),
 }
 
 FD_CASES = {

@@ -26,14 +26,17 @@ GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
 # ------------------------------------------------------------ golden trajectory
-@pytest.mark.parametrize("name", ["bench16", "c1", "dtchange", "bpr16", "bprdt"])
+@pytest.mark.parametrize("name", ["bench16", "c1", "dtchange", "bpr16", "bprdt", "minc2",
+                                  "bprminc4"])
 def test_oracle_matches_reference_trajectory(name):
     """Energies every step and fields at the saved steps, from the reference's
     one_step with the returned dt passed to the next step; the dt sequence
     itself (dtchange: five Courant-driven changes in 30 steps, each with a
     rebuild and variable-step CNAB2 weights; bprdt: the same with the BPR353
     four-stage DIRK, whose decision is taken at stage 1 with its own Courant
-    factors) is matched step by step.  bpr16: BPR353 at constant dt."""
+    factors) is matched step by step.  bpr16: BPR353 at constant dt.  minc2 and
+    bprminc4: azimuthal symmetry (orders m = 0, minc, 2 minc, ... on one
+    2 pi / minc sector) with CNAB2 and with BPR353."""
     case = json.load(open(os.path.join(GOLD, f"traj_{name}.json")))
     g = np.load(os.path.join(GOLD, f"traj_{name}.npz"))
     run = OracleRun(case["cfg"])
@@ -197,6 +200,21 @@ def test_scal_to_spat_brute_force():
     theta = sht.grid["theta_grid"]
     phi = 2 * math.pi * np.arange(p.n_phi_max) / p.n_phi_max
     assert np.abs(g - _brute_synthesis(p, S, theta, phi)).max() < 1e-12
+
+
+def test_scal_to_spat_brute_force_minc():
+    """minc = 3: the sector's longitudes phi_k = 2 pi k / (minc n_phi_max) and
+    the orders m = 0, 3, 6, ... of the lm order; sector periodicity follows."""
+    p = Params({"l_max": 12, "minc": 3})
+    assert p.n_phi_tot == 36 and p.n_phi_max == 12 and p.n_theta_max == 18
+    assert p.lm_max == sum(13 - m for m in range(0, 13, 3))
+    sht = SHT(p)
+    S = _rand(p, 1, 5)[:, 0]
+    g = sht.scal_to_spat(S)
+    theta = sht.grid["theta_grid"]
+    phi = 2 * math.pi * np.arange(p.n_phi_max) / p.n_phi_tot
+    assert np.abs(g - _brute_synthesis(p, S, theta, phi)).max() < 1e-12
+    assert np.abs(sht.scal_to_SH(g) - S).max() < 1e-12
 
 
 def test_sht_round_trips():
