@@ -35,7 +35,7 @@ extern "C" {
  * Replaces plms.build_plm_matrices (plms.py:158-186) and the padded/bucketed
  * SHT matrices of sht.py:84-289.  mp_info[(m*2+p)*4 + {0..3}] =
  * {K, syn_off, ana_off, K16} for parity p = (l-m)%2. */
-int mh_plm_tables(int l_max, int n_m, int H, int Hp, const double* cos_t, const double* sin_t,
+int mh_plm_tables(int l_max, int n_m, int minc, int H, int Hp, const double* cos_t, const double* sin_t,
                   const double* wgt, const int* mp_info, double* synP, double* synD,
                   double* anaW, double* anaWd, void* stream);
 
@@ -64,7 +64,7 @@ typedef struct {
   int fprec; /* 0: F holds doubles; 1: F holds floats (FP32 Fourier buffers, needs prec >= 1) */
   int waves; /* waves per workgroup (16 theta pairs each): 0 = default (4), 4 or 6 (prec <= 1) */
   int nct;   /* column tiles of 8 complex columns per workgroup: 0/1, or 2 (prec >= 1, vector) */
-  int pad;
+  int minc;  /* azimuthal symmetry: mode index m holds the order m * minc (0 reads as 1) */
   const int* mp16; /* prec >= 2: per (m, parity) {K32, synthesis offset, analysis offset, 0}
                       of the bf16 tables (elements) */
 } MhSynArgs;
@@ -93,7 +93,7 @@ typedef struct {
   int fprec; /* as MhSynArgs.fprec: F holds floats for 1 (needs prec >= 1) */
   int waves; /* waves per workgroup: 0 = default (6), 6 or 4 (4 only for prec >= 2); tiles
                 must hold 16 * waves degrees */
-  int pad1;
+  int minc;  /* as MhSynArgs.minc */
   const int* mp16; /* prec >= 2: as MhSynArgs.mp16 */
 } MhAnaArgs;
 
@@ -305,6 +305,8 @@ typedef struct {
   double* out;
   int l_max, N, by_m, l_mag;
   double LFfac;
+  int minc; /* azimuthal symmetry: the orders are 0, minc, 2 minc, ... (0 reads as 1) */
+  int pad;
 } MhEnergyArgs;
 
 /* Kinetic / magnetic poloidal and toroidal energy per degree (by_m=0) or per

@@ -41,17 +41,18 @@ def test_gemm_f64_matches_numpy():
         assert err < 1e-14, (M, Nn, K, err)
 
 
-@pytest.mark.parametrize("l_max", [16, 32, 63])
-def test_plm_tables_match_oracle(l_max):
-    """Device P_lm / dP_lm tables against the oracle's recurrence (plms.py)."""
+@pytest.mark.parametrize("l_max,minc", [(16, 1), (32, 1), (63, 1), (16, 2), (32, 4)])
+def test_plm_tables_match_oracle(l_max, minc):
+    """Device P_lm / dP_lm tables against the oracle's recurrence (plms.py);
+    minc > 1: the tables of the orders m = 0, minc, ... in the run's lm order."""
     torch = _torch()
     from magic_amd.params import Params
     from magic_amd.device import Resolution
     from oracle.params import Params as OP
     from oracle import horizontal
-    p = Params({"l_max": l_max, "n_r_max": 9})
+    p = Params({"l_max": l_max, "n_r_max": 9, "minc": minc})
     res = Resolution(p)
-    op = OP({"l_max": l_max, "n_r_max": 9})
+    op = OP({"l_max": l_max, "n_r_max": 9, "minc": minc})
     tab = horizontal.plm_tables(op)
     lay = res.lay
     info = lay["mp_info"].reshape(p.n_m, 2, 4)
@@ -61,10 +62,11 @@ def test_plm_tables_match_oracle(l_max):
     H, Hp = lay["H"], lay["Hp"]
     lm = 0
     worst = 0.0
-    for m in range(p.n_m):
+    for mi in range(p.n_m):
+        m = mi * minc
         for l in range(m, l_max + 1):
             par, k = (l - m) % 2, (l - m) // 2
-            K, so, ao, K16 = info[m, par]
+            K, so, ao, K16 = info[mi, par]
             gP = synP[so + k * Hp: so + k * Hp + H]
             gD = synD[so + k * Hp: so + k * Hp + H]
             gW = anaW[ao + np.arange(H) * K16 + k]
@@ -73,13 +75,14 @@ def test_plm_tables_match_oracle(l_max):
                         np.abs(gD - tab["dPlm"][lm]).max() / max(np.abs(tab["dPlm"][lm]).max(), 1e-300),
                         np.abs(gW - tab["wPlm"][lm]).max() / max(np.abs(tab["wPlm"][lm]).max(), 1e-300))
             lm += 1
+    assert lm == p.lm_max
     assert worst < 1e-12, worst
 
 
-def _oracle_sht(l_max):
+def _oracle_sht(l_max, minc=1):
     from oracle.params import Params as OP
     from oracle.sht import SHT as OSHT
-    return OSHT(OP({"l_max": l_max, "n_r_max": 9}))
+    return OSHT(OP({"l_max": l_max, "n_r_max": 9, "minc": minc}))
 
 
 def _rand_spec(L, B, lmaps, seed):
@@ -90,17 +93,19 @@ def _rand_spec(L, B, lmaps, seed):
     return x
 
 
-@pytest.mark.parametrize("l_max,B", [(16, 1), (16, 7), (32, 33), (64, 20)])
-def test_sht_matches_oracle(l_max, B):
-    """scal_to_spat / scal_to_SH / torpol_to_spat / spat_to_sphertor vs the oracle."""
+@pytest.mark.parametrize("l_max,B,minc", [(16, 1, 1), (16, 7, 1), (32, 33, 1), (64, 20, 1),
+                                          (16, 7, 2), (12, 5, 3), (32, 9, 4)])
+def test_sht_matches_oracle(l_max, B, minc):
+    """scal_to_spat / scal_to_SH / torpol_to_spat / spat_to_sphertor vs the
+    oracle; minc > 1: the orders m = 0, minc, ... on one 2 pi / minc sector."""
     torch = _torch()
     from magic_amd.params import Params
     from magic_amd.device import Resolution
     from magic_amd.sht import SHT
-    p = Params({"l_max": l_max, "n_r_max": 9})
+    p = Params({"l_max": l_max, "n_r_max": 9, "minc": minc})
     res = Resolution(p)
     sht = SHT(res)
-    osht = _oracle_sht(l_max)
+    osht = _oracle_sht(l_max, minc)
     L = p.lm_max
     S = _rand_spec(L, B, res.maps, 1)
     Q = _rand_spec(L, B, res.maps, 2)

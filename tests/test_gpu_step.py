@@ -41,13 +41,17 @@ def _rel(a, b):
 
 
 @pytest.mark.parametrize("name,fused", [("bench16", True), ("bench16", False), ("c1", True),
-                                        ("dtchange", True), ("bpr16", True), ("bprdt", True)])
+                                        ("dtchange", True), ("bpr16", True), ("bprdt", True),
+                                        ("minc2", True), ("minc2", False),
+                                        ("bprminc4", True)])
 def test_step_matches_reference_trajectory(name, fused):
     """fused: one grid-space kernel; otherwise separate FFT / products / FFT kernels.
     The dt each step returns is passed to the next (dtchange: the reference
     changes dt five times in 30 steps, each with a rebuild and variable-step
     CNAB2 weights).  bpr16 / bprdt: the BPR353 four-stage DIRK step of the
-    reference at constant dt and with Courant-driven dt changes."""
+    reference at constant dt and with Courant-driven dt changes.  minc2 /
+    bprminc4: azimuthal symmetry minc = 2 (CNAB2) and 4 (BPR353), the orders
+    m = 0, minc, ... on one 2 pi / minc sector of longitudes."""
     case, g = _load(name)
     run = _run(case, fused)
     dt = case["dt"]
@@ -65,7 +69,8 @@ def test_step_matches_reference_trajectory(name, fused):
 
 @pytest.mark.parametrize("name,precision,etol,ftol", [
     ("bench16", "mixed", 1e-6, 1e-5), ("c1", "mixed", 1e-6, 1e-5),
-    ("c1", "mixed_bf16x3", 1e-6, 1e-5), ("c1", "mixed_bf16", 1e-4, 1e-4)])
+    ("c1", "mixed_bf16x3", 1e-6, 1e-5), ("c1", "mixed_bf16", 1e-4, 1e-4),
+    ("bprminc4", "mixed", 1e-6, 1e-5)])
 def test_mixed_precision_step_tracks_reference(name, precision, etol, ftol):
     """Mixed precision (Legendre transforms on FP32 MFMA, or on BF16 MFMA with
     operands split into three or two bf16 planes; everything else FP64) against

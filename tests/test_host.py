@@ -23,6 +23,33 @@ def test_truncation_sizes(l_max, n_phi, L):
     assert (p.n_phi, p.n_theta, p.lm_max) == (n_phi, n_phi // 2, L)
 
 
+@pytest.mark.parametrize("l_max,minc", [(16, 1), (16, 2), (12, 3), (32, 4), (63, 2)])
+def test_minc_truncation_and_lm_maps_match_oracle(l_max, minc):
+    """Sizes (params.py:60-76) and the m-major lm order of the orders m = 0,
+    minc, ...: the same as the oracle's maps, which the reference trajectories
+    with minc = 2 and 4 pin; each degree's rows are its orders m <= l."""
+    from magic_amd import grid
+    from oracle import horizontal
+    from oracle.params import Params as OParams
+    p, o = Params({"l_max": l_max, "minc": minc}), OParams({"l_max": l_max, "minc": minc})
+    assert (p.n_phi, p.n_theta, p.lm_max, p.n_m) == (o.n_phi_max, o.n_theta_max, o.lm_max,
+                                                    o.n_m_max)
+    assert p.n_phi * minc == p.n_phi_tot
+    maps, om = grid.lm_maps(p), horizontal.lm_maps(o)
+    assert np.array_equal(maps["l"], om["lm2l"]) and np.array_equal(maps["m"], om["lm2m"])
+    for l in range(l_max + 1):
+        rows = maps["l_rows"][maps["l_rows_off"][l]:maps["l_rows_off"][l + 1]]
+        assert np.array_equal(rows, [om["lm2"][l, m] for m in range(0, l + 1, minc)])
+    lay = grid.sht_layout(p, maps)
+    K = lay["mp_info"].reshape(p.n_m, 2, 4)[:, :, 0]
+    assert K.sum() == p.lm_max   # every (l, m) once, by parity
+
+
+def test_minc_rejects_sectors_of_odd_length():
+    with pytest.raises(ValueError):
+        Params({"l_max": 16, "minc": 5})   # n_phi_tot = 48
+
+
 def test_unknown_key_rejected():
     with pytest.raises(KeyError):
         Params({"l_maxx": 3})

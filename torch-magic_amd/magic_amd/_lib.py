@@ -32,7 +32,7 @@ class SynIn(C.Structure):
 class SynArgs(C.Structure):
     _fields_ = [("in_", (SynIn * 3) * 4), ("ngroups", I), ("B", I), ("ldx", I), ("H", I),
                 ("Hp", I), ("mp_info", P), ("lm_start", P), ("synP", P), ("synD", P),
-                ("or2", P), ("F", P), ("c_stride", L), ("m_stride", L), ("Bout", I), ("prec", I), ("fprec", I), ("waves", I), ("nct", I), ("pad", I),
+                ("or2", P), ("F", P), ("c_stride", L), ("m_stride", L), ("Bout", I), ("prec", I), ("fprec", I), ("waves", I), ("nct", I), ("minc", I),
                 ("mp16", P)]
 
 
@@ -41,7 +41,7 @@ class AnaArgs(C.Structure):
                 ("cf1", I * 3), ("cf2", I * 3), ("pad0", I), ("outS", P * 3), ("outT", P * 3),
                 ("ldo", I), ("H", I), ("Hp", I), ("n_m", I), ("mp_info", P), ("lm_start", P),
                 ("anaW", P), ("anaWd", P), ("tiles", P), ("ntiles", I), ("prec", I), ("fprec", I), ("waves", I),
-                ("pad1", I), ("mp16", P)]
+                ("minc", I), ("mp16", P)]
 
 
 class FftArgs(C.Structure):
@@ -114,7 +114,7 @@ class ImplArgs(C.Structure):
 class EnergyArgs(C.Structure):
     _fields_ = [("w", P), ("dw", P), ("z", P), ("b", P), ("db", P), ("aj", P), ("lm_start", P),
                 ("or2", P), ("qw", P), ("out", P), ("l_max", I), ("N", I), ("by_m", I),
-                ("l_mag", I), ("LFfac", D)]
+                ("l_mag", I), ("LFfac", D), ("minc", I), ("pad", I)]
 
 
 class BuildArgs(C.Structure):
@@ -147,7 +147,7 @@ _STRUCT_ORDER = [SynArgs, AnaArgs, FftArgs, NlArgs, RowJob, TdArgs, RhsArgs, Buo
                  ImplArgs, EnergyArgs, BuildArgs, GemmArgs, GridArgs, StepState, StepCtlArgs]
 
 _SIGS = {
-    "mh_plm_tables": [I, I, I, I, P, P, P, P, P, P, P, P, P],
+    "mh_plm_tables": [I, I, I, I, I, P, P, P, P, P, P, P, P, P],
     "mh_leg_synth": [I, P, I, P],
     "mh_leg_analysis": [I, P, P],
     "mh_fft": [I, P, P],

@@ -228,7 +228,7 @@ def write_checkpoint_fortran(path, run, sim_time: float, n_time_step: int,
     pk("2d", float(run.dt_hist[0]), float(run.dt_hist[1]))
     pk("i", int(n_time_step))
     pk("9d", p.ra, p.pr, p.raxi, p.sc, p.prmag, p.ek, p.stef, p.radratio, p.sigma_ratio)
-    pk("6i", p.n_r_max, p.n_theta, p.n_phi, p.minc, 20, (p.n_r_max + 1) // 2)
+    pk("6i", p.n_r_max, p.n_theta, p.n_phi_tot, p.minc, 20, (p.n_r_max + 1) // 2)
     pk("3i", p.l_max, 0, p.l_max)
     out.extend("cheb".ljust(72).encode("ascii"))
     pk("2i", p.n_cheb_max, 0)
@@ -271,6 +271,8 @@ def load_fortran_checkpoint(run, path) -> CheckpointData:
     """Fields of a MagIC checkpoint into `run` (reference
     ``main.load_fortran_checkpoint``, ``main.py:121-166``); returns the data."""
     ck = read_checkpoint(path)
+    if ck.minc != run.p.minc:
+        raise ValueError(f"checkpoint has minc={ck.minc}, the run has minc={run.p.minc}")
     shape = tuple(run.f["w"].shape)
     for name in ("w", "z", "p", "s", "b", "aj"):
         a = getattr(ck, name)

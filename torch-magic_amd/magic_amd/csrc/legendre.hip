@@ -15,13 +15,13 @@
 // ---------------------------------------------------------------- P_lm tables
 // Orthonormal P_lm without Condon-Shortley phase and sin(theta) dP_lm/dtheta by
 // the three-term recurrence (the same recurrences as the reference plms.py).
-__global__ void k_plm_tables(int l_max, int H, int Hp, const double* __restrict__ cos_t,
+__global__ void k_plm_tables(int l_max, int minc, int H, int Hp, const double* __restrict__ cos_t,
                              const double* __restrict__ sin_t, const double* __restrict__ wgt,
                              const int* __restrict__ mp_info, double* __restrict__ synP,
                              double* __restrict__ synD, double* __restrict__ anaW,
                              double* __restrict__ anaWd) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  const int m = blockIdx.y;
+  const int mi = blockIdx.y, m = mi * minc;  // mode index, order
   if (j >= H) return;
   const double c = cos_t[j], s = sin_t[j];
   const double w2pi = 2.0 * M_PI * wgt[j];
@@ -55,7 +55,7 @@ __global__ void k_plm_tables(int l_max, int H, int Hp, const double* __restrict_
     const double P = dnorm * p0;
     const double dP = dnorm * ((double)l * cl1 * p1 - (double)(l + 1) * cl * pm1);
     const int p = (l - m) & 1, k = (l - m) >> 1;
-    const int* info = mp_info + (m * 2 + p) * 4;
+    const int* info = mp_info + (mi * 2 + p) * 4;
     const long so = (long)info[1] + (long)k * Hp + j;
     synP[so] = P;
     synD[so] = dP;
@@ -67,14 +67,14 @@ __global__ void k_plm_tables(int l_max, int H, int Hp, const double* __restrict_
   }
 }
 
-extern "C" int mh_plm_tables(int l_max, int n_m, int H, int Hp, const double* cos_t,
+extern "C" int mh_plm_tables(int l_max, int n_m, int minc, int H, int Hp, const double* cos_t,
                              const double* sin_t, const double* wgt, const int* mp_info,
                              double* synP, double* synD, double* anaW, double* anaWd,
                              void* stream) {
-  if (l_max < 0 || n_m != l_max + 1 || H <= 0 || Hp < H) return MH_EINVAL;
+  if (l_max < 0 || minc < 1 || n_m != l_max / minc + 1 || H <= 0 || Hp < H) return MH_EINVAL;
   dim3 grid(mh_cdiv(H, 64), n_m);
-  hipLaunchKernelGGL(k_plm_tables, grid, dim3(64), 0, mh_stream(stream), l_max, H, Hp, cos_t, sin_t, wgt,
-                     mp_info, synP, synD, anaW, anaWd);
+  hipLaunchKernelGGL(k_plm_tables, grid, dim3(64), 0, mh_stream(stream), l_max, minc, H, Hp, cos_t,
+                     sin_t, wgt, mp_info, synP, synD, anaW, anaWd);
   return mh_launch_status();
 }
 
@@ -144,6 +144,7 @@ __global__ __launch_bounds__(64 * WAVES, 3) void k_syn(SynArgs a, int n_m) {
   const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
   const int m = (slot / (nx * ny)) * 8 + xcd;
   if (m >= n_m) return;  // uniform per workgroup
+  const int mo = m * (a.minc > 1 ? a.minc : 1);  // the order of mode index m
   const int rem = slot % (nx * ny), ty = rem / nx, tx = rem - ty * nx;
   const int bc0 = tx * 8 * NCT;
   const int j0 = ty * kSynJ + 16 * w;
@@ -208,7 +209,7 @@ __global__ __launch_bounds__(64 * WAVES, 3) void k_syn(SynArgs a, int n_m) {
         for (int q = 0; q < NPS; ++q) {
           const int kr = srow + RPT * q;
           if (kr < kChunk) {
-            const int l = m + p + 2 * (c0 + kr);
+            const int l = mo + p + 2 * (c0 + kr);
             const double dl = (double)l * (double)(l + 1);
             const double x1 = r1[f][q], x2 = (f == 2) ? r2[q] : 0.0;
             const double x = mode == 0 ? x1 : mode == 1 ? dl * x1 : -x1 + sor2 * dl * x2;
@@ -287,7 +288,7 @@ __global__ __launch_bounds__(64 * WAVES, 3) void k_syn(SynArgs a, int n_m) {
       if (more) __syncthreads();
     }
   }
-  const double dm = (double)m;
+  const double dm = (double)mo;
   const int reim = col & 1;
 #pragma unroll
   for (int ct = 0; ct < NCT; ++ct) {
@@ -405,6 +406,7 @@ __global__ __launch_bounds__(64 * WAVES, 2) void k_ana(AnaArgs a, int n_m) {
   const int* tl = a.tiles + 3 * (xcd * tpx + slot / gx);
   const int m = tl[0], p = tl[1];
   if (m >= n_m) return;  // uniform per workgroup
+  const int mo = m * (a.minc > 1 ? a.minc : 1);  // the order of mode index m
   const int kbase = tl[2] + 16 * w;
   const int cx = slot % gx;
   const int comp = cx / nbt;
@@ -528,7 +530,7 @@ __global__ __launch_bounds__(64 * WAVES, 2) void k_ana(AnaArgs a, int n_m) {
     if (more) __syncthreads();
   }
   if (!active) return;
-  const double dm = (double)m;
+  const double dm = (double)mo;
   const long lms = a.lm_start[m];
   const int b = bc0 + (col >> 1), reim = col & 1;
   const bool bval = b < a.B;
@@ -536,7 +538,7 @@ __global__ __launch_bounds__(64 * WAVES, 2) void k_ana(AnaArgs a, int n_m) {
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const int k = kbase + MF::row(lane, i);
-    const int l = m + p + 2 * k;
+    const int l = mo + p + 2 * k;
     const bool st = bval && (k < K);
     const long off = (lms + p + 2 * k) * 2L * a.ldo + 2 * bl + reim;
     if (!VEC) {

@@ -88,6 +88,7 @@ __global__ __launch_bounds__(256, 2) void k_syn_bf(SynArgs a, int n_m) {
   const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
   const int m = (slot / (nx * ny)) * 8 + xcd;
   if (m >= n_m) return;  // uniform per workgroup
+  const int mo = m * (a.minc > 1 ? a.minc : 1);  // the order of mode index m
   const int rem = slot % (nx * ny), ty = rem / nx, tx = rem - ty * nx;
   const int bc0 = tx * 8 * NCT;
   const int j0 = ty * kSynJ + 16 * w;
@@ -152,7 +153,7 @@ __global__ __launch_bounds__(256, 2) void k_syn_bf(SynArgs a, int n_m) {
         for (int q = 0; q < NPS; ++q) {
           const int kr = srow + RPT * q;
           if (kr < kChunk) {
-            const int l = m + p + 2 * (c0 + kr);
+            const int l = mo + p + 2 * (c0 + kr);
             const double dl = (double)l * (double)(l + 1);
             const double x1 = r1[f][q], x2 = (f == 2) ? r2[q] : 0.0;
             const double x = mode == 0 ? x1 : mode == 1 ? dl * x1 : -x1 + sor2 * dl * x2;
@@ -221,7 +222,7 @@ __global__ __launch_bounds__(256, 2) void k_syn_bf(SynArgs a, int n_m) {
     }
   }
   if (!wact) return;
-  const double dm = (double)m;
+  const double dm = (double)mo;
   const int reim = col & 1;
 #pragma unroll
   for (int ct = 0; ct < NCT; ++ct) {
@@ -306,6 +307,7 @@ __global__ __launch_bounds__(256, 2) void k_ana_bf(AnaArgs a, int n_m) {
   const int* tl = a.tiles + 3 * (xcd * tpx + slot / gx);
   const int m = tl[0], p = tl[1];
   if (m >= n_m) return;  // uniform per workgroup
+  const int mo = m * (a.minc > 1 ? a.minc : 1);  // the order of mode index m
   const int kbase = tl[2] + 16 * w;
   const int cx = slot % gx;
   const int comp = cx / nbt;
@@ -422,7 +424,7 @@ __global__ __launch_bounds__(256, 2) void k_ana_bf(AnaArgs a, int n_m) {
     }
   }
   if (!active) return;
-  const double dm = (double)m;
+  const double dm = (double)mo;
   const long lms = a.lm_start[m];
   const int b = bc0 + (col >> 1), reim = col & 1;
   const bool bval = b < a.B;
@@ -430,7 +432,7 @@ __global__ __launch_bounds__(256, 2) void k_ana_bf(AnaArgs a, int n_m) {
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const int k = kbase + mfma_row_f32(lane, i);
-    const int l = m + p + 2 * k;
+    const int l = mo + p + 2 * k;
     const bool st = bval && (k < K);
     const long off = (lms + p + 2 * k) * 2L * a.ldo + 2 * bl + reim;
     if (!VEC) {

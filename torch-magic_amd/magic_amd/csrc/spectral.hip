@@ -220,7 +220,8 @@ extern "C" int mh_impl(const MhImplArgs* args, void* stream) {
 
 // ------------------------------------------------------------ energies
 // out[q * nout + i] for q = kinetic poloidal, kinetic toroidal, magnetic
-// poloidal, magnetic toroidal; i = degree l (by_m = 0) or order m (by_m = 1).
+// poloidal, magnetic toroidal; i = degree l (by_m = 0) or order m (by_m = 1; zero
+// for the orders that are not multiples of minc).
 // Radial integration uses the Clenshaw-Curtis weight vector qw.
 typedef MhEnergyArgs EnergyArgs;
 
@@ -237,13 +238,15 @@ __global__ __launch_bounds__(256) void k_energy(EnergyArgs a) {
   double acc[16];
 #pragma unroll
   for (int q = 0; q < 16; ++q) acc[q] = 0.0;
-  // work items: (m, r) for a fixed l, or (l, r) for a fixed m
-  const int nitem = (a.by_m == 1 ? (a.l_max - i + 1) : (i + 1)) * N;
+  // work items: (m, r) for a fixed l, or (l, r) for a fixed m; the orders are
+  // the multiples of minc (mode index m / minc), other orders have no work
+  const int minc = a.minc > 1 ? a.minc : 1;
+  const int nitem = (a.by_m == 1 ? (i % minc ? 0 : a.l_max - i + 1) : (i / minc + 1)) * N;
   for (int it = threadIdx.x; it < nitem; it += blockDim.x) {
     const int q = it / N, r = it - q * N;
     int l, m;
-    if (a.by_m == 1) { m = i; l = i + q; } else { l = i; m = q; }
-    const long lm = a.lm_start[m] + (l - m);
+    if (a.by_m == 1) { m = i; l = i + q; } else { l = i; m = q * minc; }
+    const long lm = a.lm_start[m / minc] + (l - m);
     const long idx = lm * N + r;
     const bool m0 = m == 0;
     const double dLh = (double)l * (double)(l + 1);
@@ -305,13 +308,14 @@ __global__ __launch_bounds__(128) void k_energy_lr(EnergyArgs a) {
   const int N = a.N;
   const double dLh = (double)l * (double)(l + 1);
   const long plane = (long)(a.l_max + 1) * N;
+  const int minc = a.minc > 1 ? a.minc : 1;
   for (int r = threadIdx.x; r < N; r += blockDim.x) {
     const double or2 = a.or2[r];
     double e[8];
 #pragma unroll
     for (int q = 0; q < 8; ++q) e[q] = 0.0;
-    for (int m = 0; m <= l; ++m) {
-      const long idx = (long)(a.lm_start[m] + (l - m)) * N + r;
+    for (int m = 0; m <= l; m += minc) {
+      const long idx = (long)(a.lm_start[m / minc] + (l - m)) * N + r;
       const bool m0 = m == 0;
       const double kp = 0.5 * dLh * (dLh * or2 * cc2real(C2(a.w)[idx], m0) + cc2real(C2(a.dw)[idx], m0));
       const double kt = 0.5 * dLh * cc2real(C2(a.z)[idx], m0);
@@ -335,7 +339,7 @@ __global__ __launch_bounds__(128) void k_energy_lr(EnergyArgs a) {
 
 extern "C" int mh_energy(const MhEnergyArgs* args, void* stream) {
   const EnergyArgs& a = *args;
-  if (a.by_m < 0 || a.by_m > 3 || a.l_max < 0 || a.N < 1 || a.out == nullptr ||
+  if (a.by_m < 0 || a.by_m > 3 || a.l_max < 0 || a.N < 1 || a.minc < 0 || a.out == nullptr ||
       a.w == nullptr || a.dw == nullptr || a.z == nullptr || a.lm_start == nullptr ||
       a.or2 == nullptr || (a.by_m != 3 && a.qw == nullptr) ||
       (a.l_mag && (a.b == nullptr || a.db == nullptr || a.aj == nullptr)))

@@ -74,7 +74,7 @@ class Resolution:
         self.anaW = torch.zeros(self.lay["ana_size"], dtype=F64, device=dev)
         self.anaWd = torch.zeros_like(self.anaW)
         cos_n, sin_n, w_n = up(self.th["cos_n"]), up(self.th["sin_n"]), up(self.th["w_n"])
-        _lib.call("mh_plm_tables", p.l_max, p.n_m, self.H, self.Hp, ptr(cos_n), ptr(sin_n),
+        _lib.call("mh_plm_tables", p.l_max, p.n_m, p.minc, self.H, self.Hp, ptr(cos_n), ptr(sin_n),
                   ptr(w_n), ptr(self.d["mp_info"]), ptr(self.synP), ptr(self.synD),
                   ptr(self.anaW), ptr(self.anaWd), stream_handle())
         # FFT plan: complex length Mh = n_phi/2, twiddles exp(-2 pi i k/Mh), exp(-2 pi i k/n_phi)
@@ -184,6 +184,7 @@ class Resolution:
         a.c_stride, a.m_stride = self.fstrides(Bout)
         a.Bout = Bout
         a.waves, a.nct = int(waves), int(nct)
+        a.minc = self.p.minc
         _lib.call("mh_leg_synth", 1 if vector else 0, _lib.ref(a), self.M, stream_handle())
 
     def analysis(self, F, Bin, B, comps, outS, outT, ldo, vector: bool, prec=0, waves=0):
@@ -214,7 +215,7 @@ class Resolution:
         w = waves or (4 if prec >= 2 else 6)
         a.tiles, a.ntiles = ptr(self.d[f"ana_tiles_{w}"]), self.lay[f"n_ana_tiles_{w}"]
         a.waves = w
-        a.n_m = self.M
+        a.n_m, a.minc = self.M, self.p.minc
         _lib.call("mh_leg_analysis", 1 if vector else 0, _lib.ref(a), stream_handle())
 
     def fft(self, forward: bool, F, FB, G, ncomp, B, comp_scale=None):

@@ -677,14 +677,15 @@ class DynamoRun:
                         ("j", self.rhs["j"], (f["aj"], f["dj"], f["ddj"]))]
         ls = np.arange(lp1, dtype=np.int64)
         row_ptrs = ptr(rows) + 4 * np.asarray(roff[:lp1], np.int64)
+        nrows_l = np.diff(np.asarray(roff, np.int64))   # orders of each degree
         for key, X, Y in systems:
             MT = self.MT[key]
-            jobs.append(job(ptr(X), N, row_ptrs, ls + 1, ptr(MT) + 8 * ls * self.K4 * self.Rpad3,
+            jobs.append(job(ptr(X), N, row_ptrs, nrows_l, ptr(MT) + 8 * ls * self.K4 * self.Rpad3,
                             N, 3 * N, self.Rpad3, [ptr(y) for y in Y], N, N))
         self.jobs1 = upload(jobs)
         # lm loop, stage 2: coupled (w, p) per degree
         Y = [f["w"], f["dw"], f["ddw"], tmp["dddw"], f["p"], f["dp"]]
-        self.jobs2 = upload([job(ptr(self.rhs["wp"]), 2 * N, row_ptrs, ls + 1,
+        self.jobs2 = upload([job(ptr(self.rhs["wp"]), 2 * N, row_ptrs, nrows_l,
                                  ptr(self.MT["wp"]) + 8 * ls * self.K4wp * self.Rpad6, 2 * N,
                                  6 * N, self.Rpad6, [ptr(y) for y in Y], N, N)])
         # radial loop: derivatives of the flux terms
@@ -742,7 +743,11 @@ class DynamoRun:
             t.zero_()
         up = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=C128).to(self.device)
         rr, r_cmb, r_icb = r.rad["r"], r.rad["r_cmb"], r.rad["r_icb"]
-        lm = lambda l, m: int(r.maps["lm_start"][m] + (l - m))
+        def lm(l, m):
+            if m % p.minc or m > l or l > p.l_max:
+                raise ValueError(f"initial mode (l={l}, m={m}) is not in the truncation "
+                                 f"(l_max={p.l_max}, minc={p.minc})")
+            return int(r.maps["lm_start"][m // p.minc] + (l - m))
         # the fields are zero but for a few rows and the rank-one seeded noise,
         # so only those go to the device (no (L, N) host arrays): s = rows + noise
         amp = p.noise_amp if noise_amp is None else noise_amp
@@ -1263,6 +1268,7 @@ class DynamoRun:
         a.lm_start = ptr(self.res.d["lm_start"])
         a.or2, a.qw, a.out = ptr(self.res.d["or2"]), ptr(self.res.d["q"]), ptr(out)
         a.l_max, a.N, a.by_m, a.l_mag = self.p.l_max, self.N, int(by_m), int(self.p.l_mag)
+        a.minc = self.p.minc
         a.LFfac = self.p.LFfac if LFfac is None else LFfac
         _lib.call("mh_energy", _lib.ref(a), stream_handle())
         return out

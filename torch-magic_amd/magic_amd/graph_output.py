@@ -73,7 +73,7 @@ def write_graph_file(path, time: float, run, fields=None):
     be_f(time)
     for v in (p.ra, p.pr, p.raxi, p.sc, p.ek, 0.0, p.prmag, p.radratio, p.sigma_ratio):
         be_f(v)
-    for v in (N, T, P, p.minc, n_ic):
+    for v in (N, T, p.n_phi_tot, p.minc, n_ic):   # the full circle's count in the header
         be_i(v)
     for v in (p.l_heat, False, False, p.l_mag, True, False):
         be_i(1 if v else 0)

@@ -25,15 +25,19 @@ from .params import Params
 
 
 def lm_maps(p: Params) -> dict:
-    l_max = p.l_max
+    """m-major lm order over the orders m = mi * minc (mode index mi < n_m):
+    lm_start[mi] is the row of (l = m, m); l / m: degree and order of each row;
+    l_rows[l_rows_off[l]:l_rows_off[l + 1]]: the rows of degree l."""
+    l_max, minc = p.l_max, p.minc
+    orders = minc * np.arange(p.n_m)
     lm_start = np.zeros(p.n_m + 1, np.int64)
-    for m in range(p.n_m):
-        lm_start[m + 1] = lm_start[m] + (l_max - m + 1)
-    m_of = np.repeat(np.arange(p.n_m), l_max + 1 - np.arange(p.n_m))
-    l_of = np.concatenate([np.arange(m, l_max + 1) for m in range(p.n_m)])
-    # rows of each degree l: lm(l, m) for m = 0..l
-    l_rows = np.concatenate([lm_start[:l + 1] + (l - np.arange(l + 1)) for l in range(l_max + 1)])
-    l_rows_off = np.concatenate([[0], np.cumsum(np.arange(1, l_max + 2))])
+    lm_start[1:] = np.cumsum(l_max + 1 - orders)
+    m_of = np.repeat(orders, l_max + 1 - orders)
+    l_of = np.concatenate([np.arange(m, l_max + 1) for m in orders])
+    # rows of each degree l: lm(l, m) for the orders m <= l
+    l_rows = np.concatenate([lm_start[:l // minc + 1] + (l - orders[:l // minc + 1])
+                             for l in range(l_max + 1)])
+    l_rows_off = np.concatenate([[0], np.cumsum(np.arange(l_max + 1) // minc + 1)])
     return dict(lm_start=lm_start, l=l_of, m=m_of, l_rows=l_rows, l_rows_off=l_rows_off)
 
 
@@ -165,12 +169,13 @@ def sht_layout(p: Params, maps: dict) -> dict:
     Hp = -(-H // 32) * 32
     info = np.zeros((p.n_m, 2, 4), np.int32)
     syn_off = ana_off = 0
-    for m in range(p.n_m):
+    for mi in range(p.n_m):
+        m = mi * p.minc
         for par in range(2):
             K = (p.l_max - m - par) // 2 + 1 if p.l_max - m - par >= 0 else 0
             K4 = -(-K // 4) * 4
             K16 = -(-K // 16) * 16
-            info[m, par] = (K, syn_off, ana_off, K16)
+            info[mi, par] = (K, syn_off, ana_off, K16)
             syn_off += K4 * Hp
             ana_off += K16 * Hp
     if syn_off >= 2 ** 31 or ana_off >= 2 ** 31:

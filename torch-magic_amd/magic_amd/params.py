@@ -7,9 +7,10 @@ radratio, ..., radial_scheme, fd_order, ...) plus the run keys of
 ``KeyError`` so typos do not pass silently.  Values may come as strings (YAML
 or ``MAGIC_*`` environment variables), as in the reference.
 
-Derived sizes follow the reference: n_phi is the smallest 2^a 3^b 5^c >=
-2*floor(30 l_max / 20) (``params.py:12-24,69``), n_theta = n_phi / 2,
-lm_max = sum over m of (l_max - m + 1).
+Derived sizes follow the reference: n_phi_tot is the smallest 2^a 3^b 5^c >=
+2*floor(30 l_max / 20) (``params.py:12-24,69``), n_theta = n_phi_tot / 2, the
+grid holds one 2 pi / minc sector of n_phi = n_phi_tot / minc longitudes,
+lm_max = sum over the orders m = 0, minc, ... of (l_max - m + 1).
 """
 
 from __future__ import annotations
@@ -105,20 +106,27 @@ class Params:
             else:
                 setattr(self, k, float(v))
         self.radial_scheme = self.radial_scheme.upper()
-        if self.minc != 1:
-            raise NotImplementedError("minc != 1 is not supported")
+        if self.minc < 1:
+            raise ValueError(f"minc must be >= 1, got {self.minc}")
         if self.n_r_max is None:
             self.n_r_max = 2 * self.l_max + 1
         if self.n_cheb_max is None:
             self.n_cheb_max = self.n_r_max
+        # azimuthal symmetry minc (params.py:60-76): the orders m = 0, minc, ...
+        # (n_m of them), the grid one 2 pi / minc sector of n_phi = n_phi_tot /
+        # minc longitudes; n_theta = n_phi_tot / 2 whatever minc
         self.m_max = self.l_max
-        self.n_m = self.l_max + 1
-        self.n_phi = smallest_235(2 * ((30 * self.l_max) // 20))
-        self.n_theta = self.n_phi // 2
+        self.n_m = self.l_max // self.minc + 1
+        self.n_phi_tot = smallest_235(2 * ((30 * self.l_max) // 20))
+        self.n_theta = self.n_phi_tot // 2
+        if self.n_phi_tot % (2 * self.minc):
+            raise ValueError(f"minc={self.minc} does not divide n_phi_tot={self.n_phi_tot} into "
+                             "sectors of an even number of longitudes")
+        self.n_phi = self.n_phi_tot // self.minc
         if self.n_theta % 2:
             raise ValueError(f"l_max={self.l_max} gives an odd number of latitudes "
                              f"({self.n_theta}); the N/S paired grid needs an even count")
-        self.lm_max = (self.l_max + 1) * (self.l_max + 2) // 2
+        self.lm_max = sum(self.l_max - m + 1 for m in range(0, self.m_max + 1, self.minc))
         self.l_mag = self.mode not in (1, 7)
         self.l_heat = self.mode != 7
         self.l_anel = self.strat > 0.0
