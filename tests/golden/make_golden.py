@@ -97,10 +97,11 @@ CASES = {
                            radratio=0.35, dtmax=1e-3, time_scheme="BPR353"), dt=1e-3, steps=30,
                   noise=dict(seed=7, amp=1.0), field_steps=[10, 30], expl_step1=False),
     "minc2": dict(cfg=dict(l_max=16, n_r_max=33, minc=2), dt=1e-4, steps=20,
-                  noise=dict(seed=13, amp=1e-3), field_steps=[1, 20], expl_step1=False),
+                  noise=dict(seed=13, amp=1e-3), field_steps=[1, 20], expl_step1=False,
+                  diag_steps=[20]),
     "bprminc4": dict(cfg=dict(l_max=32, n_r_max=33, minc=4, time_scheme="BPR353"), dt=1e-4,
                      steps=20, noise=dict(seed=17, amp=1e-3), field_steps=[1, 20],
-                     expl_step1=False),
+                     expl_step1=False, diag_steps=[20]),
 }
 
 FD_CASES = {

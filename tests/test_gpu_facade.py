@@ -64,7 +64,8 @@ def test_module_level_facade_reproduces_reference_c1():
     config.configure({})
 
 
-@pytest.mark.parametrize("name,step", [("c1", 100), ("dtchange", 30)])
+@pytest.mark.parametrize("name,step", [("c1", 100), ("dtchange", 30), ("minc2", 20),
+                                       ("bprminc4", 20)])
 def test_diagnostics_compute_on_the_tensors_given(name, step):
     """get_e_kin_full / get_e_mag_oc_full / get_dlm / spectra on arrays that are
     not the run's state (the reference's saved fields, host numpy): the

@@ -429,12 +429,14 @@ def test_row_operator_tile_shapes_give_identical_results(l_max, n_r):
     run.close()
 
 
-def test_headline_configuration_mixed_tracks_fp64():
+@pytest.mark.parametrize("mixed", ["mixed", "mixed_bf16x3"])
+def test_headline_configuration_mixed_tracks_fp64(mixed):
     """The headline configuration (l_max=383, n_r=97, E=1e-5, Ra=2e8, Pm=1,
     seeded noise 1e-4 and the dipole seed): 10 device-controlled steps (one HIP
-    graph each) in FP64 and in mixed precision on the FP64 run's step sizes;
-    energies within 1e-6 relative (the mixed-precision target), every field
-    finite, and the same dt sequence and rebuild count."""
+    graph each) in FP64 and in mixed precision (FP32 or three-plane BF16
+    Legendre transforms) on the FP64 run's step sizes; energies within 1e-6
+    relative (the mixed-precision target), every field finite, and the same dt
+    sequence and rebuild count."""
     import math
     import torch
     from magic_amd.dynamo import DynamoRun
@@ -442,7 +444,7 @@ def test_headline_configuration_mixed_tracks_fp64():
                dtmax=1e-4)
     out = {}
     sched = None
-    for precision in ("fp64", "mixed"):
+    for precision in ("fp64", mixed):
         run = DynamoRun(dict(cfg, precision=precision))
         run.dt_schedule = sched
         run.use_graphs = True
@@ -455,7 +457,7 @@ def test_headline_configuration_mixed_tracks_fp64():
         out[precision] = (run.energies(), list(run.dt_log), run.n_rebuild)
         sched = list(run.dt_log)
         run.close()
-    (e64, dt64, nr64), (emx, dtmx, nrmx) = out["fp64"], out["mixed"]
+    (e64, dt64, nr64), (emx, dtmx, nrmx) = out["fp64"], out[mixed]
     assert dt64 == dtmx and nr64 == nrmx
     for a, b in zip(e64, emx):
         assert math.isfinite(b) and abs(b - a) <= 1e-6 * abs(a), (a, b)
