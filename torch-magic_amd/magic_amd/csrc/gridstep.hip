@@ -23,6 +23,10 @@
 // accumulators and the component being transformed for the force terms),
 // which is why B is transformed twice.
 //
+// FP32 with a compiled plan (the mixed-precision step) runs k_grid_step3, which
+// transforms the three components of each vector together (3*RB LDS rows per
+// pass, a third of the workgroup barriers), with the same arithmetic per value.
+//
 // Level block: by default a workgroup takes the most levels its plan fits
 // (threads <= 768, LDS); MhGridArgs.rb asks for fewer (1 or 2 levels run
 // compiled plans instantiated in gridstep_rb1.hip / gridstep_rb2.hip, other
@@ -55,6 +59,18 @@ extern "C" int mh_grid_plan(int Mh, int* out) {
   return MH_OK;
 }
 
+// Whether Mh has a compiled plan (the FP32 launch of a compiled plan
+// transforms three components together and needs three times the LDS rows).
+static bool grid_compiled(int Mh) {
+  switch (Mh) {
+#define MH_CCASE(V) case V:
+    MH_GRID_COMPILED(MH_CCASE)
+#undef MH_CCASE
+    return true;
+    default: return false;
+  }
+}
+
 extern "C" int mh_grid_step(const MhGridArgs* args, void* stream) {
   const GridArgs& a = *args;
   if (a.N < 3 || a.T < 1 || a.Mh < 2 || a.M < 1 || a.M > a.Mh) return MH_EINVAL;
@@ -77,7 +93,10 @@ extern "C" int mh_grid_step(const MhGridArgs* args, void* stream) {
   if ((long)a.T * 4 * a.N * 2 >= (1L << 24) || (long)a.M * a.T * 4 * a.N * 2 >= (1L << 31))
     return MH_EINVAL;
   const size_t esz = a.fprec ? sizeof(float2) : sizeof(double2);
-  const size_t lds = (2ull * P.RB * (a.Mh + 2) + 2ull * a.Mh) * esz;
+  const bool batched =
+      a.fprec && !a.generic && (rb == own_rb || rb == 1 || rb == 2) && grid_compiled(a.Mh);
+  const size_t lds = (2ull * (batched ? 3 : 1) * P.RB * (a.Mh + 2) + 2ull * a.Mh) * esz;
+  if (lds > (size_t)kLdsBytes) return MH_EINVAL;
   const int nRB = (a.N + P.RB - 1) / P.RB;
   const long nwg = (long)a.T * nRB;
   const long grid = ((nwg + 7) / 8) * 8;

@@ -64,9 +64,10 @@ extern __shared__ double2 g_lds_raw[];
 // Geometry of one launch; compile-time constants when MH > 0.
 struct Geo {
   int Mh, M, RB, LS, TP, nst, threads;
+  int RBL;  // rows per LDS buffer: RB levels times the components transformed together
 };
 
-template <int MH, int RBC>
+template <int MH, int RBC, int NB = 1>
 __device__ __forceinline__ Geo make_geo(const GridArgs& a, const GPlan& rp) {
   Geo g;
   g.M = a.M;
@@ -85,6 +86,7 @@ __device__ __forceinline__ Geo make_geo(const GridArgs& a, const GPlan& rp) {
     g.threads = rp.threads;
   }
   g.LS = g.Mh + 2;
+  g.RBL = g.RB * NB;
   return g;
 }
 
@@ -96,11 +98,11 @@ __device__ __forceinline__ typename vec2<T>::type* lds_base() {
 }
 template <typename T>
 __device__ __forceinline__ typename vec2<T>::type* lbuf(const Geo& g, int w) {
-  return lds_base<T>() + w * g.RB * g.LS;
+  return lds_base<T>() + w * g.RBL * g.LS;
 }
 template <typename T>
 __device__ __forceinline__ const typename vec2<T>::type* ltw(const Geo& g) {
-  return lds_base<T>() + 2 * g.RB * g.LS;
+  return lds_base<T>() + 2 * g.RBL * g.LS;
 }
 template <typename T>
 __device__ __forceinline__ const typename vec2<T>::type* ltw2(const Geo& g) {
@@ -116,13 +118,13 @@ struct Ctx {
 
 template <int R, typename T>
 __device__ __forceinline__ void stage_mid(const Geo& g, const GridArgs& a, int win, int wout, int Ns,
-                                          double dir) {
+                                          double dir, int nrows) {
   typedef typename vec2<T>::type V;
   const int nb = g.Mh / R, tstep = g.Mh / (Ns * R);
   const V* tw = ltw<T>(g);
   const V* in = lbuf<T>(g, win);
   V* out = lbuf<T>(g, wout);
-  for (int idx = threadIdx.x; idx < g.RB * nb; idx += g.threads) {
+  for (int idx = threadIdx.x; idx < nrows * nb; idx += g.threads) {
     const int row = idx / nb, bf = idx - row * nb;
     const int k = bf % Ns;
     const V* src = in + row * g.LS;
@@ -159,22 +161,22 @@ __device__ __forceinline__ void stage_mid(const Geo& g, const GridArgs& a, int w
 
 template <typename T>
 __device__ __forceinline__ void stage_mid_rt(const Geo& g, const GridArgs& a, int R, int win, int wout,
-                                             int Ns, double dir) {
-  if (R == 4) stage_mid<4, T>(g, a, win, wout, Ns, dir);
-  else if (R == 2) stage_mid<2, T>(g, a, win, wout, Ns, dir);
-  else if (R == 3) stage_mid<3, T>(g, a, win, wout, Ns, dir);
-  else stage_mid<5, T>(g, a, win, wout, Ns, dir);
+                                             int Ns, double dir, int nrows) {
+  if (R == 4) stage_mid<4, T>(g, a, win, wout, Ns, dir, nrows);
+  else if (R == 2) stage_mid<2, T>(g, a, win, wout, Ns, dir, nrows);
+  else if (R == 3) stage_mid<3, T>(g, a, win, wout, Ns, dir, nrows);
+  else stage_mid<5, T>(g, a, win, wout, Ns, dir, nrows);
 }
 
 // Inverse stages 0 .. nst-2 of a compiled plan (S = stage, NS = its Ns).
 template <int MH, int S, int NS, typename T>
-__device__ __forceinline__ int inv_mid_ct(const Geo& g, const GridArgs& a, int w) {
+__device__ __forceinline__ int inv_mid_ct(const Geo& g, const GridArgs& a, int w, int nrows) {
   constexpr GPlan P = make_plan(MH);
   if constexpr (S < P.nst - 1) {
     constexpr int R = P.radix[S];
-    stage_mid<R, T>(g, a, w, 1 - w, NS, 1.0);
+    stage_mid<R, T>(g, a, w, 1 - w, NS, 1.0, nrows);
     lds_barrier();
-    return inv_mid_ct<MH, S + 1, NS * R, T>(g, a, 1 - w);
+    return inv_mid_ct<MH, S + 1, NS * R, T>(g, a, 1 - w, nrows);
   } else {
     return w;
   }
@@ -182,13 +184,13 @@ __device__ __forceinline__ int inv_mid_ct(const Geo& g, const GridArgs& a, int w
 
 // Forward stages S = nst-2 .. 0 of a compiled plan.
 template <int MH, int S, int NS, typename T>
-__device__ __forceinline__ int fwd_mid_ct(const Geo& g, const GridArgs& a, int w) {
+__device__ __forceinline__ int fwd_mid_ct(const Geo& g, const GridArgs& a, int w, int nrows) {
   constexpr GPlan P = make_plan(MH);
   if constexpr (S >= 0) {
     constexpr int R = P.radix[S];
-    stage_mid<R, T>(g, a, w, 1 - w, NS, -1.0);
+    stage_mid<R, T>(g, a, w, 1 - w, NS, -1.0, nrows);
     lds_barrier();
-    return fwd_mid_ct<MH, S - 1, NS * R, T>(g, a, 1 - w);
+    return fwd_mid_ct<MH, S - 1, NS * R, T>(g, a, 1 - w, nrows);
   } else {
     return w;
   }
@@ -275,12 +277,12 @@ __device__ __forceinline__ void inverse(Ctx& c, const Geo& g, const GridArgs& a,
   lds_barrier();
   if (g.nst > 1) {
     if constexpr (MH > 0) {
-      w = inv_mid_ct<MH, 0, 1, T>(g, a, w);
+      w = inv_mid_ct<MH, 0, 1, T>(g, a, w, g.RB);
     } else {
       const GPlan& P = *c.rp;
       int Ns = 1;
       for (int s = 0; s < P.nst - 1; ++s) {
-        stage_mid_rt<T>(g, a, P.radix[s], w, 1 - w, Ns, 1.0);
+        stage_mid_rt<T>(g, a, P.radix[s], w, 1 - w, Ns, 1.0, g.RB);
         Ns *= P.radix[s];
         w = 1 - w;
         lds_barrier();
@@ -329,12 +331,12 @@ __device__ __forceinline__ void forward(Ctx& c, const Geo& g, const GridArgs& a,
   lds_barrier();
   if constexpr (MH > 0) {
     constexpr GPlan P = make_plan(MH);
-    w = fwd_mid_ct<MH, P.nst - 2, RL, T>(g, a, w);
+    w = fwd_mid_ct<MH, P.nst - 2, RL, T>(g, a, w, g.RB);
   } else {
     const GPlan& P = *c.rp;
     int Ns = RL;
     for (int s = P.nst - 2; s >= 0; --s) {
-      stage_mid_rt<T>(g, a, P.radix[s], w, 1 - w, Ns, -1.0);
+      stage_mid_rt<T>(g, a, P.radix[s], w, 1 - w, Ns, -1.0, g.RB);
       Ns *= P.radix[s];
       w = 1 - w;
       lds_barrier();
@@ -369,6 +371,167 @@ __device__ __forceinline__ void forward(Ctx& c, const Geo& g, const GridArgs& a,
     if (p > 0 && 2 * p < Mh && pc < g.M)
       *reinterpret_cast<V*>(Fo + ((unsigned)__umul24((unsigned)pc, So) + oj)) =
           cscale(cconj(csub(E, Tm)), scale);
+  }
+  c.busy = w;
+}
+
+// ---------------------------------------------------------------- batched transforms
+// The FP32 kernel of the compiled plans transforms the three components of a
+// vector (u, B, curl u, curl B; the three outputs of each product) together:
+// component k of level j is LDS row k*RB + j, and every LDS pass runs over the
+// 3*RB rows, so a vector costs the barriers of one transform instead of three.
+// Each row's arithmetic is that of the single-component transform.
+
+template <int RL, typename T, int NC>
+struct PrefetchN {
+  static constexpr int PP = (RL + 2) / 2;
+  typename vec2<T>::type x[NC][PP], xc[NC][PP];
+};
+
+// Loads of NC components at base + k*cs (k < NC), as fetch().
+template <int MH, int RL, typename T, int NC>
+__device__ __forceinline__ void fetchN(const Geo& g, const GridArgs& a, PrefetchN<RL, T, NC>& pf,
+                                       const T* base, long cs, int Bst, int b0, int nvalid, int t) {
+  typedef typename vec2<T>::type V;
+  const int RB = g.RB, n = (g.Mh / 2 + 1) * RB;
+  const unsigned S = (unsigned)(a.T * Bst * 2);
+  const unsigned tb = (unsigned)((t * Bst + b0) * 2);
+  auto ld = [&](const T* b, int m, int j) {
+    return (m < g.M && j < nvalid)
+               ? *reinterpret_cast<const V*>(b + ((unsigned)__umul24((unsigned)m, S) + tb +
+                                                  2u * (unsigned)j))
+               : mk2((T)0, (T)0);
+  };
+#pragma unroll
+  for (int i = 0; i < PrefetchN<RL, T, NC>::PP; ++i) {
+    const int e = threadIdx.x + i * g.threads;
+#pragma unroll
+    for (int k = 0; k < NC; ++k) pf.x[k][i] = pf.xc[k][i] = mk2((T)0, (T)0);
+    if (e < n) {
+      const int p = e / RB, j = e - p * RB;
+#pragma unroll
+      for (int k = 0; k < NC; ++k) {
+        pf.x[k][i] = ld(base + k * cs, p, j);
+        if (p > 0 && 2 * p < g.Mh) pf.xc[k][i] = ld(base + k * cs, g.Mh - p, j);
+      }
+    }
+  }
+}
+
+// Inverse transforms of NC prefetched components (as inverse()).
+template <int MH, int RL, typename T, int NC, typename F = NoFetch>
+__device__ __forceinline__ void inverseN(Ctx& c, const Geo& g, const GridArgs& a,
+                                         const PrefetchN<RL, T, NC>& pf,
+                                         typename vec2<T>::type (&z)[NC][RL], F next = F()) {
+  typedef typename vec2<T>::type V;
+  const int RB = g.RB, LS = g.LS;
+  int w = 1 - c.busy;
+  {
+    V* zb = lbuf<T>(g, w);
+    const V* tw2 = ltw2<T>(g);
+    const int n = (g.Mh / 2 + 1) * RB;
+#pragma unroll
+    for (int i = 0; i < PrefetchN<RL, T, NC>::PP; ++i) {
+      const int e = threadIdx.x + i * g.threads;
+      if (e < n) {
+        const int p = e / RB, j = e - p * RB;
+        const V tw2c = cconj(tw2[p]);
+#pragma unroll
+        for (int k = 0; k < NC; ++k) {
+          V x = pf.x[k][i];
+          if (p == 0) x.y = (T)0;
+          const V xc = (2 * p == g.Mh) ? cconj(x) : cconj(pf.xc[k][i]);
+          const V E = cadd(x, xc);
+          const V O = cmul(csub(x, xc), tw2c);
+          V* row = zb + (k * RB + j) * LS;
+          row[p] = mk2(E.x - O.y, E.y + O.x);
+          if (p > 0 && 2 * p < g.Mh) row[g.Mh - p] = mk2(E.x + O.y, -E.y + O.x);
+        }
+      }
+    }
+  }
+  next();
+  lds_barrier();
+  if (g.nst > 1) w = inv_mid_ct<MH, 0, 1, T>(g, a, w, NC * RB);
+  if (c.act) {
+    const V* tw = ltw<T>(g);
+    const V w1 = cconj(tw[c.q]);
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+      const V* src = lbuf<T>(g, w) + (k * RB + c.row) * LS;
+#pragma unroll
+      for (int i = 0; i < RL; ++i) z[k][i] = src[c.q + i * g.TP];
+      if (g.nst > 1) {
+        V wi = w1;
+#pragma unroll
+        for (int i = 1; i < RL; ++i) {
+          z[k][i] = cmul(z[k][i], wi);
+          if (i + 1 < RL) wi = cmul(wi, w1);
+        }
+      }
+      dft<RL>(z[k], 1.0);
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < NC; ++k)
+#pragma unroll
+      for (int i = 0; i < RL; ++i) z[k][i] = mk2((T)0, (T)0);
+  }
+  c.busy = w;
+}
+
+// Forward transforms of NC components (as forward()); output component
+// comp[k] scaled by scale[k].
+template <int MH, int RL, typename T, int NC>
+__device__ __forceinline__ void forwardN(Ctx& c, const Geo& g, const GridArgs& a,
+                                         typename vec2<T>::type (&z)[NC][RL], const int (&comp)[NC],
+                                         const T (&scale)[NC], int t, int r0) {
+  typedef typename vec2<T>::type V;
+  const int LS = g.LS, RB = g.RB;
+  int w = 1 - c.busy;
+  if (c.act) {
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+      dft<RL>(z[k], -1.0);
+      V* dst = lbuf<T>(g, w) + (k * RB + c.row) * LS + c.q * RL;
+#pragma unroll
+      for (int i = 0; i < RL; ++i) dst[i] = z[k][i];
+    }
+  }
+  lds_barrier();
+  {
+    constexpr GPlan P = make_plan(MH);
+    w = fwd_mid_ct<MH, P.nst - 2, RL, T>(g, a, w, NC * RB);
+  }
+  const int Mh = g.Mh, N = a.N, No = N - 2, n = (Mh / 2 + 1) * RB;
+  const V* tw2 = ltw2<T>(g);
+  const unsigned So = (unsigned)(a.T * No * 2);
+  const unsigned to = (unsigned)((t * No + r0 - 1) * 2);
+  const V* res = lbuf<T>(g, w);
+  for (int e = threadIdx.x; e < n; e += g.threads) {
+    const int p = e / RB, j = e - p * RB;
+    const int r = r0 + j;
+    if (r < 1 || r > N - 2) continue;
+    const int pc = (p == 0) ? 0 : Mh - p;
+    const unsigned oj = to + 2u * (unsigned)j;
+    const V tw = tw2[p];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+      const V* src = res + (k * RB + j) * LS;
+      T* Fo = reinterpret_cast<T*>(a.Fo) + (long)comp[k] * g.M * a.T * No * 2;
+      const V Z = src[p];
+      const V Zc = cconj(src[pc]);
+      const V E = cscale(cadd(Z, Zc), (T)0.5);
+      const V D = csub(Z, Zc);
+      const V O = mk2((T)0.5 * D.y, (T)-0.5 * D.x);
+      const V Tm = cmul(tw, O);
+      if (p < g.M)
+        *reinterpret_cast<V*>(Fo + ((unsigned)__umul24((unsigned)p, So) + oj)) =
+            cscale(cadd(E, Tm), scale[k]);
+      if (p > 0 && 2 * p < Mh && pc < g.M)
+        *reinterpret_cast<V*>(Fo + ((unsigned)__umul24((unsigned)pc, So) + oj)) =
+            cscale(cconj(csub(E, Tm)), scale[k]);
+    }
   }
   c.busy = w;
 }
@@ -592,6 +755,202 @@ __global__ __launch_bounds__(kMaxThreads, 3) void k_grid_step(GridArgs a, GPlan 
   }
 }
 
+// FP32 grid step of the compiled plans with the vector components transformed
+// together (inverseN / forwardN over 3*RB LDS rows).  Same arithmetic per
+// value as k_grid_step<MH, RL, float, RBC>, in the same order.
+template <int MH, int RL, int RBC>
+__global__ __launch_bounds__(kMaxThreads, 3) void k_grid_step3(GridArgs a, GPlan rp) {
+  typedef float T;
+  typedef typename vec2<T>::type V;
+  __shared__ unsigned long long red[2 * 16];
+  const Geo g = make_geo<MH, RBC, 3>(a, rp);
+  const int N = a.N, nth = a.T, RB = g.RB;
+  const int nRB = (N + RB - 1) / RB;
+  const int nwg = nth * nRB, per = (nwg + 7) / 8;
+  const int L = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  if (L >= nwg) return;  // uniform per workgroup
+  const int t = L / nRB, r0 = (L - t * nRB) * RB;
+  const int nvalid = min(RB, N - r0);
+
+  Ctx c;
+  c.rp = &rp;
+  c.row = threadIdx.x / g.TP;
+  c.q = threadIdx.x - c.row * g.TP;
+  c.act = c.row < nvalid;
+  c.busy = 0;
+  if (threadIdx.x < 2 * RB) red[threadIdx.x] = 0ull;
+  {  // twiddles into LDS
+    V* tl = lds_base<T>() + 2 * g.RBL * g.LS;
+    const double2* t1 = reinterpret_cast<const double2*>(a.tw);
+    const double2* t2 = reinterpret_cast<const double2*>(a.tw2);
+    for (int k = threadIdx.x; k < g.Mh; k += g.threads) {
+      tl[k] = mk2((T)t1[k].x, (T)t1[k].y);
+      tl[g.Mh + k] = mk2((T)t2[k].x, (T)t2[k].y);
+    }
+  }
+  __syncthreads();
+
+  const int r = r0 + (c.act ? c.row : 0);
+  const T ost2 = (T)a.ost2[t];
+  const T or2 = (T)a.or2[r], or4 = (T)a.or4[r];
+  const T LF = (T)a.LFfac;
+  const int Bv = 4 * N;
+  const long vcs = (long)g.M * a.T * Bv * 2;
+  const T* Fv = reinterpret_cast<const T*>(a.Fv);
+  const T* Fs = reinterpret_cast<const T*>(a.Fs);
+  const bool mag = a.l_mag != 0;
+  const T inv_n = (T)(1.0 / (2.0 * g.Mh));
+
+  PrefetchN<RL, T, 3> pf3;
+  PrefetchN<RL, T, 1> pf1;
+  auto next3 = [&](int b0) {
+    return [&, b0]() { fetchN<MH, RL, T, 3>(g, a, pf3, Fv, vcs, Bv, b0, nvalid, t); };
+  };
+  V u[3][RL];  // vr, vt, vp
+  // 1. u (group 2), then s
+  fetchN<MH, RL, T, 3>(g, a, pf3, Fv, vcs, Bv, 2 * N + r0, nvalid, t);
+  inverseN<MH, RL, T, 3>(c, g, a, pf3, u,
+                         [&]() { fetchN<MH, RL, T, 1>(g, a, pf1, Fs, 0, N, r0, nvalid, t); });
+  // 2. s -> VSr, VSt, VSp; B next, or curl u without a magnetic field
+  {
+    V sv[1][RL];
+    inverseN<MH, RL, T, 1>(c, g, a, pf1, sv, next3((mag ? 0 : 3 * N) + r0));
+    V o[3][RL];
+#pragma unroll
+    for (int i = 0; i < RL; ++i) {
+      o[0][i] = mk2(u[0][i].x * sv[0][i].x, u[0][i].y * sv[0][i].y);
+      o[1][i] = mk2(or2 * u[1][i].x * sv[0][i].x, or2 * u[1][i].y * sv[0][i].y);
+      o[2][i] = mk2(or2 * u[2][i].x * sv[0][i].x, or2 * u[2][i].y * sv[0][i].y);
+    }
+    const int comp[3] = {1, 5, 6};
+    const T sc[3] = {inv_n, inv_n * ost2, inv_n * ost2};
+    forwardN<MH, RL, T, 3>(c, g, a, o, comp, sc, t, r0);
+  }
+  // 3. B -> Courant maxima and VxBr, VxBt, VxBp (B stays in registers)
+  V b[3][RL];
+  if (mag) {
+    inverseN<MH, RL, T, 3>(c, g, a, pf3, b, next3(3 * N + r0));  // curl u next
+  } else {
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+      for (int i = 0; i < RL; ++i) b[k][i] = mk2((T)0, (T)0);
+  }
+  if (c.act) {
+    T cr = (T)0, ch = (T)0;
+    const T vri = (T)a.valri2[r], vhi = (T)a.valhi2[r];
+#pragma unroll
+    for (int i = 0; i < RL; ++i) {
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const T Vr = h ? u[0][i].y : u[0][i].x, Vt = h ? u[1][i].y : u[1][i].x;
+        const T Vp = h ? u[2][i].y : u[2][i].x, Br = h ? b[0][i].y : b[0][i].x;
+        const T Bt = h ? b[1][i].y : b[1][i].x, Bp = h ? b[2][i].y : b[2][i].x;
+        const T vflr2 = Vr * Vr;
+        const T vflh2 = (Vt * Vt + Vp * Vp) * ost2;
+        T c1, c2;
+        if (mag) {
+          const T valr = Br * Br * LF;
+          const T valh2 = (Bt * Bt + Bp * Bp) * LF * ost2;
+          T valr2 = valr, valh2m = valh2;
+          if (a.alf_damp) {
+            valr2 = valr * valr / (valr + vri);
+            valh2m = valh2 * valh2 / (valh2 + vhi);
+          }
+          c1 = or4 * ((T)a.cf2 * vflr2 + (T)a.af2 * valr2);
+          c2 = or2 * ((T)a.cf2 * vflh2 + (T)a.af2 * valh2m);
+        } else {
+          c1 = (T)a.cf2 * or4 * vflr2;
+          c2 = (T)a.cf2 * or2 * vflh2;
+        }
+        cr = fmax(cr, c1);
+        ch = fmax(ch, c2);
+      }
+    }
+    atomicMax(&red[c.row], (unsigned long long)__double_as_longlong((double)cr));
+    atomicMax(&red[RB + c.row], (unsigned long long)__double_as_longlong((double)ch));
+  }
+  {
+    V o[3][RL];
+#pragma unroll
+    for (int i = 0; i < RL; ++i) {
+      o[0][i] = mk2(ost2 * (u[1][i].x * b[2][i].x - u[2][i].x * b[1][i].x),
+                    ost2 * (u[1][i].y * b[2][i].y - u[2][i].y * b[1][i].y));
+      o[1][i] = mk2(or4 * (u[2][i].x * b[0][i].x - u[0][i].x * b[2][i].x),
+                    or4 * (u[2][i].y * b[0][i].y - u[0][i].y * b[2][i].y));
+      o[2][i] = mk2(or4 * (u[0][i].x * b[1][i].x - u[1][i].x * b[0][i].x),
+                    or4 * (u[0][i].y * b[1][i].y - u[1][i].y * b[0][i].y));
+    }
+    const int comp[3] = {2, 7, 8};
+    const T sc[3] = {inv_n, inv_n * ost2, inv_n * ost2};
+    forwardN<MH, RL, T, 3>(c, g, a, o, comp, sc, t, r0);
+  }
+  // 4-5. Advection + Lorentz force (formulas as k_grid_step)
+  V ad[3][RL];  // Advr, Advt, Advp
+  {
+    V cu[3][RL];  // curl u
+    inverseN<MH, RL, T, 3>(c, g, a, pf3, cu, [&]() {
+      if (mag) fetchN<MH, RL, T, 3>(g, a, pf3, Fv, vcs, Bv, N + r0, nvalid, t);
+    });
+#pragma unroll
+    for (int i = 0; i < RL; ++i) {
+      ad[0][i] = mk2((T)0, (T)0);
+      ad[1][i] = mk2(or4 * cu[0][i].x * u[2][i].x, or4 * cu[0][i].y * u[2][i].y);
+      ad[2][i] = mk2(-or4 * cu[0][i].x * u[1][i].x, -or4 * cu[0][i].y * u[1][i].y);
+    }
+#pragma unroll
+    for (int i = 0; i < RL; ++i) {
+      ad[0][i].x -= ost2 * cu[1][i].x * u[2][i].x;
+      ad[0][i].y -= ost2 * cu[1][i].y * u[2][i].y;
+      ad[2][i].x += or4 * cu[1][i].x * u[0][i].x;
+      ad[2][i].y += or4 * cu[1][i].y * u[0][i].y;
+    }
+#pragma unroll
+    for (int i = 0; i < RL; ++i) {
+      ad[0][i].x += ost2 * cu[2][i].x * u[1][i].x;
+      ad[0][i].y += ost2 * cu[2][i].y * u[1][i].y;
+      ad[1][i].x -= or4 * cu[2][i].x * u[0][i].x;
+      ad[1][i].y -= or4 * cu[2][i].y * u[0][i].y;
+    }
+  }
+  if (mag) {
+    const T lr = LF * ost2, lt = LF * or4;
+    V cb[3][RL];  // curl B
+    inverseN<MH, RL, T, 3>(c, g, a, pf3, cb);
+#pragma unroll
+    for (int i = 0; i < RL; ++i) {
+      ad[1][i].x -= lt * cb[0][i].x * b[2][i].x;
+      ad[1][i].y -= lt * cb[0][i].y * b[2][i].y;
+      ad[2][i].x += lt * cb[0][i].x * b[1][i].x;
+      ad[2][i].y += lt * cb[0][i].y * b[1][i].y;
+    }
+#pragma unroll
+    for (int i = 0; i < RL; ++i) {
+      ad[0][i].x += lr * cb[1][i].x * b[2][i].x;
+      ad[0][i].y += lr * cb[1][i].y * b[2][i].y;
+      ad[2][i].x -= lt * cb[1][i].x * b[0][i].x;
+      ad[2][i].y -= lt * cb[1][i].y * b[0][i].y;
+    }
+#pragma unroll
+    for (int i = 0; i < RL; ++i) {
+      ad[0][i].x -= lr * cb[2][i].x * b[1][i].x;
+      ad[0][i].y -= lr * cb[2][i].y * b[1][i].y;
+      ad[1][i].x += lt * cb[2][i].x * b[0][i].x;
+      ad[1][i].y += lt * cb[2][i].y * b[0][i].y;
+    }
+  }
+  {
+    const int comp[3] = {0, 3, 4};
+    const T sc[3] = {inv_n, inv_n * ost2, inv_n * ost2};
+    forwardN<MH, RL, T, 3>(c, g, a, ad, comp, sc, t, r0);
+  }
+  lds_barrier();
+  if (threadIdx.x < 2 * nvalid) {
+    const int j = threadIdx.x % nvalid, h = threadIdx.x / nvalid;
+    atomicMax(a.cfl + h * N + r0 + j, red[h * RB + j]);
+  }
+}
+
 // Compiled plan for the common half lengths, run-time plan otherwise.
 
 // Launch of a compiled plan with RBC levels per workgroup (0: the plan's own);
@@ -600,9 +959,12 @@ template <int RBC, typename T>
 bool launch_compiled(const GridArgs& a, const GPlan& P, dim3 gd, dim3 bd, size_t lds,
                      hipStream_t st) {
   switch (a.Mh) {
-#define MH_CASE(V)                                                                   \
-  case V:                                                                            \
-    hipLaunchKernelGGL((k_grid_step<V, make_plan(V).RL, T, RBC>), gd, bd, lds, st, a, P); \
+#define MH_CASE(V)                                                                       \
+  case V:                                                                                \
+    if constexpr (sizeof(T) == 4)                                                        \
+      hipLaunchKernelGGL((k_grid_step3<V, make_plan(V).RL, RBC>), gd, bd, lds, st, a, P); \
+    else                                                                                 \
+      hipLaunchKernelGGL((k_grid_step<V, make_plan(V).RL, T, RBC>), gd, bd, lds, st, a, P); \
     return true;
     MH_GRID_COMPILED(MH_CASE)
 #undef MH_CASE
