@@ -45,8 +45,11 @@ from .sht import SHT
 class OracleRun:
     def __init__(self, cfg: dict):
         p = self.prm = Params(dict(cfg))
-        if p.l_anel or p.l_finite_diff or p.l_cond_ic or p.raxi != 0.0 or p.nRotIC != 0:
-            raise NotImplementedError("oracle step: Boussinesq Chebyshev insulating-IC only")
+        if p.l_anel or p.l_finite_diff or p.raxi != 0.0 or p.nRotIC != 0:
+            raise NotImplementedError("oracle step: Boussinesq Chebyshev, non-rotating inner core only")
+        # conducting inner core (sigma_ratio > 0): IC fields on the even Chebyshev
+        # grid, coupled OC+IC B/J solves (update_b.py:301-513,634-801)
+        self.cond_ic = p.l_cond_ic and p.l_mag
         # BPR353 (time_scheme.py:157-326): four-stage SDIRK, explicit terms at
         # stages 1-3, implicit terms stored one stage ahead, old state at stage 4
         self.dirk = p.time_scheme == "BPR353"
@@ -66,10 +69,17 @@ class OracleRun:
             setattr(self, name, z())
         self.td = {k: dict(old=z(), impl=z(), expl=[z(), z()])
                    for k in ("s", "z", "w", "p", "b", "j")}
+        if self.cond_ic:
+            self.ic = radial.ic_grid(p, self.rad["r_icb"])
+            zi = lambda: np.zeros((L, p.n_r_ic_max), np.complex128)
+            for name in ("b_ic", "db_ic", "ddb_ic", "aj_ic", "dj_ic", "ddj_ic"):
+                setattr(self, name, zi())
+            for k in ("b_ic", "j_ic"):
+                self.td[k] = dict(old=zi(), impl=zi(), expl=[zi(), zi()])
         if self.dirk:
             for td in self.td.values():
-                td["implS"] = [z() for _ in range(4)]
-                td["explS"] = [z() for _ in range(4)]
+                td["implS"] = [np.zeros_like(td["old"]) for _ in range(4)]
+                td["explS"] = [np.zeros_like(td["old"]) for _ in range(4)]
         self.dt_hist = np.zeros(2)
         self.dtMax = 0.0
         self.inv = None
@@ -98,6 +108,22 @@ class OracleRun:
             l1m0 = self.maps["lm2"][1, 0]
             l2m0 = self.maps["lm2"][2, 0]
             b_tor = -4.0 / 3.0 * p.amp_b1 * math.sqrt(math.pi / 5.0)
+            if self.cond_ic:   # initB, conducting inner core (init_fields.py:426-456)
+                for name in ("b_ic", "db_ic", "ddb_ic", "aj_ic", "dj_ic", "ddj_ic"):
+                    getattr(self, name)[:] = 0.0
+                r_ic = self.ic["r_ic"]
+                b_pol = p.amp_b1 * math.sqrt(3.0 * math.pi) / (3.0 + r_cmb)
+                self.b[l1m0] += b_pol * (r ** 3 - 4.0 / 3.0 * r_cmb * r ** 2)
+                self.b_ic[l1m0] += b_pol * r_icb ** 2 * (
+                    0.5 * r_ic ** 2 / r_icb + 0.5 * r_icb - 4.0 / 3.0 * r_cmb)
+                self.aj[l2m0] += b_tor * r * np.sin(math.pi * (r / r_cmb))
+                arg = math.pi * r_icb / r_cmb
+                sa, ca = math.sin(arg), math.cos(arg)
+                aj1 = (arg - 2.0 * sa * ca) / (arg + sa * ca)
+                aj2 = (1.0 - aj1) * r_icb * sa / ca
+                self.aj_ic[l2m0] += b_tor * (aj1 * r_ic * np.sin(math.pi * r_ic / r_cmb)
+                                             + aj2 * np.cos(math.pi * r_ic / r_cmb))
+                return
             b_pol = p.amp_b1 * math.sqrt(3.0 * math.pi) / 4.0
             self.b[l1m0] += b_pol * (r ** 3 - 4.0 / 3.0 * r_cmb * r ** 2 + r_icb ** 4 / 3.0 / r)
             self.aj[l2m0] += b_tor * r * np.sin(math.pi * (r - r_icb))
@@ -147,6 +173,29 @@ class OracleRun:
                 impl[:] = 0
                 impl[:, 1:-1] = (p.opm * dLh * or2[:, 1:-1]
                                  * (ddf[:, 1:-1] - dLh * or2[:, 1:-1] * f[:, 1:-1]))
+            if self.cond_ic:   # get_mag_ic_rhs_imp (update_b.py:803-873)
+                for f, df, ddf, key in ((self.b_ic, self.db_ic, self.ddb_ic, "b_ic"),
+                                        (self.aj_ic, self.dj_ic, self.ddj_ic, "j_ic")):
+                    df[:] = f @ self.ic["D1"]
+                    ddf[:] = f @ self.ic["D2"]
+                    self.td[key]["old"][1:] = dLh[1:] * or2[0, -1] * f[1:]
+                    self._ic_impl(f, df, ddf, self.td[key]["impl"])
+
+    def _ic_impl(self, f, df, ddf, impl):
+        """Inner-core diffusion terms (update_b.py:775-801): opm / sigma_ratio
+        dLh or2(r_icb) (f'' + 2 (l+1) f' / r) in the bulk, (1 + 2 (l+1)) f'' at
+        the centre; lm = 0 and the ICB point left as they are."""
+        p = self.prm
+        Nic = p.n_r_ic_max
+        dLh = self.lmarr["dLh"][:, None]
+        or2_icb = self.rad["or2"][-1]
+        lp1 = (self.maps["lm2l"] + 1.0)[:, None]
+        bulk = slice(1, Nic - 1)
+        ob = self.ic["O_r_ic"][None, bulk]
+        impl[1:, bulk] = (p.opm * p.O_sr * dLh * or2_icb
+                          * (ddf[:, bulk] + 2.0 * lp1 * ob * df[:, bulk]))[1:]
+        impl[1:, Nic - 1] = (p.opm * p.O_sr * dLh[1:, 0] * or2_icb * (1.0 + 2.0 * lp1[1:, 0])
+                             * ddf[1:, Nic - 1])
 
     def _w_p_impl(self, w, dw, ddw, dddw, pf, dp, s):
         p, rad = self.prm, self.rad
@@ -268,7 +317,9 @@ class OracleRun:
         p, rad, mats = self.prm, self.rad, self.mats
         self.inv = dict(s=matrices.build_s(p, rad, mats, w0), z=matrices.build_z(p, rad, mats, w0),
                         wp=matrices.build_wp(p, rad, mats, w0), p0=matrices.build_p0(p, rad, mats))
-        if p.l_mag:
+        if p.l_mag and self.cond_ic:
+            self.inv["b"], self.inv["j"] = matrices.build_bj_coupled(p, rad, mats, self.ic, w0)
+        elif p.l_mag:
             self.inv["b"], self.inv["j"] = matrices.build_bj(p, rad, mats, w0)
 
     def _solve(self, inv_by_l, rhs):
@@ -330,8 +381,49 @@ class OracleRun:
         if p.l_mag:
             self._update_b()
 
+    def _update_b_coupled(self):
+        """``update_b.py:634-801``: one (NT, NT) solve per degree for the outer
+        Chebyshev and inner even-Chebyshev coefficients of b and of aj."""
+        p, rad = self.prm, self.rad
+        N, Nic, NT, L = p.n_r_max, p.n_r_ic_max, p.n_r_tot, p.lm_max
+        dLh = self.lmarr["dLh"][:, None]
+        or2 = rad["or2"][None]
+        sols = []
+        for oc, ic, key in (("b", "b_ic", "b"), ("j", "j_ic", "j")):
+            ro, ri = self._rhs(oc), self._rhs(ic)
+            rhs = np.zeros((L, NT), np.complex128)
+            rhs[:, 1:N - 1] = ro[:, 1:N - 1]
+            rhs[:, N + 1:NT] = ri[:, 1:Nic]
+            sol = self._solve(self.inv[key], rhs)
+            sol[self.m0] = sol[self.m0].real
+            oc_c = sol[:, :N].copy()
+            oc_c[:, p.n_cheb_max:] = 0.0
+            ic_c = sol[:, N:N + Nic].copy()
+            ic_c[:, p.n_cheb_ic_max:] = 0.0
+            sols.append((radial.costf(oc_c), radial.costf(ic_c)))
+        (self.b[:], self.b_ic[:]), (self.aj[:], self.aj_ic[:]) = sols
+        self.db[:] = self._dr(self.b, self.D1); self.ddb[:] = self._dr(self.b, self.D2)
+        self.dj[:] = self._dr(self.aj, self.D1); self.ddj[:] = self._dr(self.aj, self.D2)
+        for f, df, ddf in ((self.b_ic, self.db_ic, self.ddb_ic), (self.aj_ic, self.dj_ic, self.ddj_ic)):
+            df[:] = f @ self.ic["D1"]
+            ddf[:] = f @ self.ic["D2"]
+        if not self.dirk:
+            for k in ("b", "j", "b_ic", "j_ic"):
+                self.td[k]["expl"][1][:] = self.td[k]["expl"][0]
+        if self._store():
+            self.td["b"]["old"][:] = dLh * or2 * self.b
+            self.td["j"]["old"][:] = dLh * or2 * self.aj
+            self.td["b_ic"]["old"][1:] = dLh[1:] * or2[0, -1] * self.b_ic[1:]
+            self.td["j_ic"]["old"][1:] = dLh[1:] * or2[0, -1] * self.aj_ic[1:]
+        self.td["b"]["impl"][:] = p.opm * dLh * or2 * (self.ddb - dLh * or2 * self.b)
+        self.td["j"]["impl"][:] = p.opm * dLh * or2 * (self.ddj - dLh * or2 * self.aj)
+        self._ic_impl(self.b_ic, self.db_ic, self.ddb_ic, self.td["b_ic"]["impl"])
+        self._ic_impl(self.aj_ic, self.dj_ic, self.ddj_ic, self.td["j_ic"]["impl"])
+
     def _update_b(self):
         """``update_b.py:557-631`` (insulating)."""
+        if self.cond_ic:
+            return self._update_b_coupled()
         p, rad = self.prm, self.rad
         N = p.n_r_max
         dLh = self.lmarr["dLh"][:, None]

@@ -159,6 +159,52 @@ def build_bj(p: Params, rad, mats, wimp_lin0):
     return b_out, j_out
 
 
+def build_bj_coupled(p: Params, rad, mats, ic, wimp_lin0):
+    """Coupled outer-core + inner-core bMat, jMat of a conducting inner core
+    (``update_b.py:301-513``, Chebyshev): rows 0 CMB condition, 1..N-2 outer
+    diffusion, N-1 continuity at the ICB, N derivative match at the ICB
+    (sigma_ratio on the outer dj/dr for j), N+1..NT-2 inner diffusion, NT-1 the
+    centre limit; columns 0..N-1 outer Chebyshev, N..NT-1 inner even
+    Chebyshev coefficients.  Returns (b_inv, j_inv), (l_max+1, NT, NT)."""
+    R, D, D2, _ = mats
+    N, Nic, NT = p.n_r_max, p.n_r_ic_max, p.n_r_tot
+    nci = p.n_cheb_ic_max
+    rn, cn = rad["rnorm"], ic["cnorm"]
+    or1, or2 = rad["or1"], rad["or2"]
+    o2 = or2[1:N - 1, None]
+    or2_icb = or2[N - 1]
+    cheb, dcheb, d2cheb = ic["cheb"], ic["dcheb"], ic["d2cheb"]
+    bulk = slice(1, Nic - 1)
+    ic_eval, dic_eval, d2ic_eval = cheb[:, bulk].T, dcheb[:, bulk].T, d2cheb[:, bulk].T
+    or_ic = ic["O_r_ic"][bulk, None]
+    b_out = np.zeros((p.l_max + 1, NT, NT))
+    j_out = np.zeros((p.l_max + 1, NT, NT))
+    for l in range(1, p.l_max + 1):
+        dL = float(l * (l + 1))
+        for which in ("b", "j"):
+            dat = np.zeros((NT, NT))
+            dat[1:N - 1, :N] = rn * dL * o2 * (R[1:N - 1] - wimp_lin0 * p.opm
+                                              * (D2[1:N - 1] - dL * o2 * R[1:N - 1]))
+            dat[0, :N] = rn * (D[0] + float(l) * or1[0] * R[0]) if which == "b" else rn * R[0]
+            dat[N - 1, :N] = rn * R[N - 1]
+            dat[N - 1, N:N + nci] = -cn * cheb[:nci, 0]
+            dat[N, :N] = rn * (1.0 if which == "b" else p.sigma_ratio) * D[N - 1]
+            dat[N, N:N + nci] = -cn * (dcheb[:nci, 0] + float(l + 1) * or1[N - 1] * cheb[:nci, 0])
+            if p.n_cheb_max < N:
+                dat[(0, N - 1, N), p.n_cheb_max:N] = 0.0
+            dat[N + 1:NT - 1, N:] = cn * dL * or2_icb * (
+                ic_eval - wimp_lin0 * p.opm * p.O_sr * (d2ic_eval + 2.0 * float(l + 1) * or_ic * dic_eval))
+            dat[NT - 1, N:] = cn * dL * or2_icb * (
+                cheb[:, Nic - 1] - wimp_lin0 * p.opm * p.O_sr * (1.0 + 2.0 * float(l + 1))
+                * d2cheb[:, Nic - 1])
+            _bfac_cols(dat, (0, N - 1))
+            dat[:, N] *= 0.5
+            dat[:, NT - 1] *= 0.5
+            datp, fac = _precond_rows(dat)
+            (b_out if which == "b" else j_out)[l] = np.linalg.inv(datp) * fac[None, :]
+    return b_out, j_out
+
+
 def ps_cond(p: Params, rad, mats):
     """Boussinesq conduction state (s0, p0) on the grid (``init_fields.py:108-180``)."""
     from .algebra import prepare_mat, solve_mat

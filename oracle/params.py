@@ -29,7 +29,7 @@ def prime_decomposition(n: int) -> int:
 
 
 _DEFAULTS = dict(
-    l_max=16, minc=1, n_r_max=None, n_cheb_max=None,
+    l_max=16, minc=1, n_r_max=None, n_cheb_max=None, n_cheb_ic_max=None,
     ra=1.0e5, ek=1.0e-3, pr=1.0, prmag=5.0, radratio=0.35, raxi=0.0, sc=10.0,
     mode=0, strat=0.0, polind=2.0, g0=0.0, g1=0.0, g2=0.0,
     ktops=1, kbots=1, ktopv=2, kbotv=2, kbotb=1,
@@ -57,6 +57,10 @@ class Params:
         self.m_max = self.l_max
         self.n_r_max = int(c["n_r_max"] if c["n_r_max"] is not None else 2 * self.l_max + 1)
         self.n_cheb_max = int(c["n_cheb_max"] if c["n_cheb_max"] is not None else self.n_r_max)
+        # inner-core grid (params.py:65-66): (n_r_max + 1) / 2 points, even Chebyshev
+        self.n_r_ic_max = (self.n_r_max + 1) // 2
+        self.n_cheb_ic_max = int(c["n_cheb_ic_max"] if c["n_cheb_ic_max"] is not None
+                                 else self.n_r_ic_max)
         nalias = 20
         # params.py:69-71: n_phi_max covers one 2 pi / minc sector
         self.n_phi_tot = prime_decomposition(2 * ((30 * self.l_max) // nalias))
@@ -79,6 +83,9 @@ class Params:
         self.l_anel = self.strat > 0.0
         self.l_finite_diff = self.radial_scheme == "FD"
         self.l_cond_ic = self.sigma_ratio > 0.0
+        self.l_rot_ic = self.nRotIC != 0
+        self.n_r_tot = self.n_r_max + self.n_r_ic_max if self.l_cond_ic else self.n_r_max
+        self.O_sr = 1.0 / self.sigma_ratio if self.l_cond_ic else 0.0   # pre_calculations.py:52
         # pre_calculations.py:21-32 (tScale = lScale = 1)
         self.opr = 1.0 / self.pr
         self.opm = 1.0 / self.prmag

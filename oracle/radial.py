@@ -98,6 +98,39 @@ def rint_weights(p: Params, r_cmb: float, r_icb: float) -> np.ndarray:
     return 2.0 * rnorm * (C.T @ (w * bf)) / drx
 
 
+def ic_grid(p: Params, r_icb: float) -> dict:
+    """Inner-core grid and even Chebyshev polynomials (``radial_functions.py:209-306``,
+    ``radial_derivatives.py:182-195``): the upper half of the 2 n_r_ic_max - 1
+    point Gauss-Lobatto grid on [-r_icb, r_icb] (r_ic[0] = r_icb .. r_ic[-1] = 0),
+    cheb[n, k] = T_2n(y_k) with its first and second r-derivatives by the
+    recursion over T_2n and T_2n+1 (map factor 1/r_icb), the norm
+    sqrt(2/(n_r_ic_max - 1)) and the grid-to-grid derivative matrices
+    D_k = cheb^-1 d^k cheb (applied as f @ D_k)."""
+    N = p.n_r_ic_max
+    n_tot = 2 * N - 1
+    y = [math.cos(math.pi * k / (n_tot - 1)) for k in range(n_tot)]
+    r_ic = np.zeros(N)
+    r_ic[:N - 1] = [r_icb * y[k] for k in range(N - 1)]
+    O_r_ic = np.zeros(N)
+    O_r_ic[:N - 1] = 1.0 / r_ic[:N - 1]
+    mf = 1.0 / r_icb
+    cheb = np.zeros((N, N)); dcheb = np.zeros((N, N)); d2cheb = np.zeros((N, N))
+    for k in range(N):
+        yk = y[k]
+        cheb[0, k] = 1.0
+        lc, ldc, ld2c = yk, mf, 0.0   # T_1 and its derivatives
+        for n in range(1, N):
+            cheb[n, k] = 2.0 * yk * lc - cheb[n - 1, k]
+            dcheb[n, k] = 2.0 * mf * lc + 2.0 * yk * ldc - dcheb[n - 1, k]
+            d2cheb[n, k] = 4.0 * mf * ldc + 2.0 * yk * ld2c - d2cheb[n - 1, k]
+            lc, ldc, ld2c = (2.0 * yk * cheb[n, k] - lc,
+                             2.0 * mf * cheb[n, k] + 2.0 * yk * dcheb[n, k] - ldc,
+                             4.0 * mf * dcheb[n, k] + 2.0 * yk * d2cheb[n, k] - ld2c)
+    inv = np.linalg.inv(cheb)
+    return dict(r_ic=r_ic, O_r_ic=O_r_ic, cheb=cheb, dcheb=dcheb, d2cheb=d2cheb,
+                cnorm=math.sqrt(2.0 / (N - 1)), D1=inv @ dcheb, D2=inv @ d2cheb)
+
+
 def rint(f: np.ndarray, q: np.ndarray):
     """Clenshaw-Curtis integral over r of f(..., N) (``integration.py:45-72``)."""
     return f @ q

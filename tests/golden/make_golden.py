@@ -37,6 +37,13 @@ that each step returns passed to the next (as ``main.py`` does):
   longitudes) with CNAB2 (l_max=16, minc=2) and with BPR353 (l_max=32, minc=4),
   n_r_max=33, 20 steps.
 
+* ``condic`` and ``bprcondic``: a conducting inner core (sigma_ratio = 1,
+  kbotb = 3, not rotating; the coupled outer/inner B and J solves of
+  ``update_b.py:301-513,634-801`` on the even-Chebyshev inner grid of
+  ``radial_functions.py:209-306``), l_max=16, n_r_max=33 with the benchmark's
+  truncations n_cheb_max=31, n_cheb_ic_max=15, CNAB2 and BPR353, 20 steps;
+  the inner-core fields b_ic, aj_ic at the saved steps.
+
 Finite-difference operator cases (``fd4_ob2``, ``fd4_ob3``): the FD branches
 of ``build_s_matrices`` (``update_s.py:66-188``), ``build_z_matrices``
 (``update_z.py:89-240``) and ``build_w_matrices``
@@ -102,6 +109,14 @@ CASES = {
     "bprminc4": dict(cfg=dict(l_max=32, n_r_max=33, minc=4, time_scheme="BPR353"), dt=1e-4,
                      steps=20, noise=dict(seed=17, amp=1e-3), field_steps=[1, 20],
                      expl_step1=False, diag_steps=[20]),
+    "condic": dict(cfg=dict(l_max=16, n_r_max=33, n_cheb_max=31, n_cheb_ic_max=15,
+                            sigma_ratio=1.0, kbotb=3), dt=1e-4, steps=20,
+                   noise=dict(seed=19, amp=1e-3), field_steps=[1, 20], expl_step1=False,
+                   diag_steps=[20]),
+    "bprcondic": dict(cfg=dict(l_max=16, n_r_max=33, n_cheb_max=31, n_cheb_ic_max=15,
+                               sigma_ratio=1.0, kbotb=3, time_scheme="BPR353"), dt=1e-4,
+                      steps=20, noise=dict(seed=23, amp=1e-3), field_steps=[1, 20],
+                      expl_step1=False),
 }
 
 FD_CASES = {
@@ -149,6 +164,9 @@ for n in range(1, case["steps"] + 1):
     if n in case["field_steps"]:
         for k in names:
             res[f"{k}_step{n}"] = getattr(fields, k + "_LMloc").numpy().copy()
+        if float(case["cfg"].get("sigma_ratio", 0.0)) > 0.0:
+            for k in ("b_ic", "aj_ic"):
+                res[f"{k}_step{n}"] = getattr(fields, k).numpy().copy()
         for k in ("dw", "db"):
             res[f"{k}_step{n}"] = getattr(fields, k + "_LMloc").numpy().copy()
     if n in case.get("diag_steps", []):

@@ -27,7 +27,7 @@ GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 # ------------------------------------------------------------ golden trajectory
 @pytest.mark.parametrize("name", ["bench16", "c1", "dtchange", "bpr16", "bprdt", "minc2",
-                                  "bprminc4"])
+                                  "bprminc4", "condic", "bprcondic"])
 def test_oracle_matches_reference_trajectory(name):
     """Energies every step and fields at the saved steps, from the reference's
     one_step with the returned dt passed to the next step; the dt sequence
@@ -36,7 +36,9 @@ def test_oracle_matches_reference_trajectory(name):
     four-stage DIRK, whose decision is taken at stage 1 with its own Courant
     factors) is matched step by step.  bpr16: BPR353 at constant dt.  minc2 and
     bprminc4: azimuthal symmetry (orders m = 0, minc, 2 minc, ... on one
-    2 pi / minc sector) with CNAB2 and with BPR353."""
+    2 pi / minc sector) with CNAB2 and with BPR353.  condic and bprcondic: a
+    conducting inner core (coupled outer/inner B and J solves), whose fields
+    b_ic and aj_ic are matched too."""
     case = json.load(open(os.path.join(GOLD, f"traj_{name}.json")))
     g = np.load(os.path.join(GOLD, f"traj_{name}.npz"))
     run = OracleRun(case["cfg"])
@@ -52,7 +54,8 @@ def test_oracle_matches_reference_trajectory(name):
         e = np.array(run.energies())
         assert np.all(np.abs(e - g["energies"][n - 1]) <= 1e-11 * np.abs(g["energies"][n - 1]))
         if n in case["field_steps"]:
-            for k in ("s", "z", "w", "p", "b", "aj"):
+            keys = ("s", "z", "w", "p", "b", "aj") + (("b_ic", "aj_ic") if run.cond_ic else ())
+            for k in keys:
                 ref = g[f"{k}_step{n}"]
                 assert np.abs(getattr(run, k) - ref).max() <= 1e-11 * np.abs(ref).max(), (n, k)
     if name in ("dtchange", "bprdt"):   # the fixture does take the Courant branch
