@@ -297,6 +297,40 @@ typedef struct {
  * update_b.py:619-631, setup_initial_state step_time.py:398-617). */
 int mh_impl(const MhImplArgs* args, void* stream);
 
+/* ------------------------------------------------------------ conducting inner core */
+typedef struct {
+  MhImexSys sys[2];      /* inner-core b, j: (L, n_ic) old, impl, expl0, expl1, t4..t7 */
+  const double *rb, *rj; /* outer right-hand sides of mh_rhs, (L, N) */
+  double *out_b, *out_j; /* coupled right-hand sides, (L, N + n_ic) */
+  const int* m;          /* order of each mode (m = 0 rows are made real) */
+  int L, N, n_ic, pad;
+  double wimp, wl1, we0, we1;
+  const double* wdev;    /* optional: device weights laid out as MhRhsArgs.wdev */
+} MhIcRhsArgs;
+
+/* Coupled outer/inner right-hand sides of the B and J systems of a conducting
+ * inner core (update_b.py:655-668): 0, outer rows 1..N-2, 0 (ICB continuity),
+ * 0 (ICB derivative match), inner IMEX right-hand side of levels 1..n_ic-1. */
+int mh_ic_rhs(const MhIcRhsArgs* args, void* stream);
+
+typedef struct {
+  const double *b, *db, *ddb, *aj, *dj, *ddj; /* (L, n_ic) inner-core fields */
+  double *old_b, *old_j;   /* (L, n_ic), or both NULL (no old state: BPR353 stages 1-3) */
+  double *impl_b, *impl_j; /* (L, n_ic) */
+  const double* dLh;       /* (L,) l(l+1) */
+  const int* l;            /* (L,) degree */
+  const double* O_r_ic;    /* (n_ic,) 1/r on the inner grid, 0 at the centre */
+  int L, n_ic;
+  double fac;              /* opm / sigma_ratio */
+  double or2_icb;          /* 1/r_icb^2 */
+} MhIcImplArgs;
+
+/* Inner-core old states and implicit terms (get_mag_ic_rhs_imp,
+ * update_b.py:803-873, and update_b.py:756-801): dLh or2(r_icb) f, and
+ * fac dLh or2(r_icb) (f'' + 2(l+1) f'/r) on the bulk, (1 + 2(l+1)) f'' at the
+ * centre; mode 0 and the ICB level untouched. */
+int mh_ic_impl(const MhIcImplArgs* args, void* stream);
+
 typedef struct {
   const double *w, *dw, *z, *b, *db, *aj;
   const int* lm_start;

@@ -43,7 +43,8 @@ def _rel(a, b):
 @pytest.mark.parametrize("name,fused", [("bench16", True), ("bench16", False), ("c1", True),
                                         ("dtchange", True), ("bpr16", True), ("bprdt", True),
                                         ("minc2", True), ("minc2", False),
-                                        ("bprminc4", True)])
+                                        ("bprminc4", True), ("condic", True),
+                                        ("bprcondic", True)])
 def test_step_matches_reference_trajectory(name, fused):
     """fused: one grid-space kernel; otherwise separate FFT / products / FFT kernels.
     The dt each step returns is passed to the next (dtchange: the reference
@@ -51,7 +52,9 @@ def test_step_matches_reference_trajectory(name, fused):
     CNAB2 weights).  bpr16 / bprdt: the BPR353 four-stage DIRK step of the
     reference at constant dt and with Courant-driven dt changes.  minc2 /
     bprminc4: azimuthal symmetry minc = 2 (CNAB2) and 4 (BPR353), the orders
-    m = 0, minc, ... on one 2 pi / minc sector of longitudes."""
+    m = 0, minc, ... on one 2 pi / minc sector of longitudes.  condic /
+    bprcondic: a conducting inner core (coupled outer/inner B and J solves),
+    with the inner-core fields b_ic and aj_ic checked too."""
     case, g = _load(name)
     run = _run(case, fused)
     dt = case["dt"]
@@ -65,6 +68,10 @@ def test_step_matches_reference_trajectory(name, fused):
             for k in ("s", "z", "w", "p", "b", "aj"):
                 got = run.f[k].cpu().numpy()
                 assert _rel(got, g[f"{k}_step{n}"]) < 1e-10, (n, k, _rel(got, g[f"{k}_step{n}"]))
+            if run.cond_ic:
+                for k in ("b_ic", "aj_ic"):
+                    got = run.fic[k].cpu().numpy()
+                    assert _rel(got, g[f"{k}_step{n}"]) < 1e-10, (n, k, _rel(got, g[f"{k}_step{n}"]))
 
 
 @pytest.mark.parametrize("name,precision,etol,ftol", [

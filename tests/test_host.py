@@ -264,3 +264,49 @@ def test_bench_defines_every_function_main_calls():
               if ins.opname in ("LOAD_GLOBAL", "LOAD_NAME")}
     missing = [n for n in called if not hasattr(bench, n) and not hasattr(builtins, n)]
     assert not missing, missing
+
+
+def test_inner_core_grid_and_coupled_operators_match_oracle():
+    """Conducting inner core: the host inner grid, even-Chebyshev polynomials
+    and derivative operators, and the coupled B/J inverses and row operators,
+    against the oracle (pinned by the reference's conducting-core trajectories):
+    applying the row operators to a right-hand side gives the oracle's solve ->
+    DCT-I -> derivatives of both shells."""
+    from magic_amd import grid, inner_core
+    from oracle import matrices, radial as orad
+    from oracle.dynamo import OracleRun
+    cfg = dict(l_max=8, n_r_max=17, n_cheb_max=15, n_cheb_ic_max=7, sigma_ratio=0.7, kbotb=3)
+    p, o = Params(cfg), OracleRun(cfg)
+    op = o.prm
+    assert (p.n_r_ic_max, p.n_r_tot, p.n_cheb_ic_max) == (op.n_r_ic_max, op.n_r_tot, 7)
+    rad = grid.radial(p)
+    ic = grid.inner_core(p, rad["r_icb"])
+    oic = orad.ic_grid(op, o.rad["r_icb"])
+    for k in ("r_ic", "O_r_ic", "cheb", "dcheb", "d2cheb"):
+        np.testing.assert_allclose(ic[k], oic[k], rtol=1e-13, atol=1e-13)
+    np.testing.assert_allclose(ic["D1"], oic["D1"].T, rtol=1e-10, atol=1e-10)
+    w0 = 3e-4
+    bi, ji = inner_core.coupled_inverses(p, rad, ic, w0)
+    obi, oji = matrices.build_bj_coupled(op, o.rad, o.mats, oic, w0)
+    for a, b in ((bi, obi), (ji, oji)):
+        assert np.abs(a - b).max() <= 1e-10 * np.abs(b).max()
+    N, Nic, NT = p.n_r_max, p.n_r_ic_max, p.n_r_tot
+    k4 = -(-NT // 4) * 4
+    (mo, Ro, _), (mi, Ri, _) = inner_core.row_operators(p, rad, ic, bi, k4)
+    rng = np.random.default_rng(5)
+    l = 3
+    rhs = rng.standard_normal(NT)
+    sol = obi[l] @ rhs
+    c_oc, c_ic = sol[:N].copy(), sol[N:].copy()
+    c_oc[op.n_cheb_max:] = 0.0
+    c_ic[op.n_cheb_ic_max:] = 0.0
+    f_oc, f_ic = orad.costf(c_oc), orad.costf(c_ic)
+    ref_oc = np.concatenate([f_oc, o.D1 @ f_oc, o.D2 @ f_oc])
+    ref_ic = np.concatenate([f_ic, f_ic @ oic["D1"], f_ic @ oic["D2"]])
+    got_oc = mo[l, :NT, :Ro].T @ rhs
+    got_ic = mi[l, :NT, :Ri].T @ rhs
+    assert np.abs(got_oc - ref_oc).max() <= 1e-9 * np.abs(ref_oc).max()
+    assert np.abs(got_ic - ref_ic).max() <= 1e-9 * np.abs(ref_ic).max()
+    p.check_step_supported()   # conducting, not rotating: on the device path
+    with pytest.raises(NotImplementedError):
+        Params(dict(cfg, nRotIC=1)).check_step_supported()

@@ -200,3 +200,51 @@ def test_mixed_precision_tables_are_converted_once_outside_the_plans(cpu_standin
     with _lib.record() as again:
         dyn.DynamoRun(cfg, device="cpu", res=run.res)
     assert not any(name == "mh_tables_to_f32" for _, name, _ in again.calls)
+
+
+@pytest.mark.parametrize("scheme", ["CNAB2", "BPR353"])
+def test_conducting_inner_core_launches_stay_inside_their_tensors(cpu_standin, scheme):
+    """Conducting inner core: the coupled B/J row-operator jobs (N + n_ic long
+    right-hand sides, outer and inner outputs), the inner derivative jobs of the
+    set-up, and the arguments of mh_ic_rhs / mh_ic_impl in every recorded lm
+    plan (each BPR353 stage) lie inside the run's tensors."""
+    cfg = dict(l_max=16, n_r_max=17, n_cheb_ic_max=7, sigma_ratio=1.0, kbotb=3, ek=1e-3,
+               ra=1e5, pr=1.0, prmag=5.0, radratio=0.35, dtmax=1e-4, time_scheme=scheme)
+    with _lib.record():
+        run = dyn.DynamoRun(cfg, device="cpu")
+        run.initialize_dt(cfg["dtmax"])
+        run._rebuild_launches(0.5e-4)
+        run._build_coupled(0.5e-4)
+    assert run.cond_ic
+    ext = _extents(run)
+    _check_row_jobs(run, ext)
+    L, N, nic = run.L, run.N, run.p.n_r_ic_max
+    stages = (1, 2, 3, 4) if run.dirk else (0,)
+    n_ic_calls = 0
+    for k in stages:
+        run.stage = k
+        _, plan = run._plan("lm")
+        for _, name, args in plan:
+            if name == "mh_ic_rhs":
+                a = C.cast(args[0], C.POINTER(_lib.IcRhsArgs)).contents
+                assert (a.L, a.N, a.n_ic) == (L, N, nic)
+                for sy in a.sys:
+                    for f in ("old", "impl", "expl0", "expl1", "t4", "t5", "t6", "t7"):
+                        v = getattr(sy, f)
+                        if v:
+                            _inside(ext, v, v + 16 * L * nic, f"ic rhs {f}")
+                for v in (a.rb, a.rj):
+                    _inside(ext, v, v + 16 * L * N, "outer rhs")
+                for v in (a.out_b, a.out_j):
+                    _inside(ext, v, v + 16 * L * (N + nic), "coupled rhs")
+                n_ic_calls += 1
+            elif name == "mh_ic_impl":
+                a = C.cast(args[0], C.POINTER(_lib.IcImplArgs)).contents
+                for f in ("b", "db", "ddb", "aj", "dj", "ddj", "old_b", "old_j", "impl_b",
+                          "impl_j"):
+                    v = getattr(a, f)
+                    if v:
+                        _inside(ext, v, v + 16 * L * nic, f"ic impl {f}")
+                _inside(ext, a.O_r_ic, a.O_r_ic + 8 * nic, "1/r inner")
+                n_ic_calls += 1
+    assert n_ic_calls == 2 * len(stages)

@@ -98,6 +98,18 @@ class RhsArgs(C.Structure):
                 ("bots_l0", D), ("l_mag", I), ("pad", I), ("wdev", P)]
 
 
+class IcRhsArgs(C.Structure):
+    _fields_ = [("sys", ImexSys * 2), ("rb", P), ("rj", P), ("out_b", P), ("out_j", P),
+                ("m", P), ("L", I), ("N", I), ("n_ic", I), ("pad", I), ("wimp", D), ("wl1", D),
+                ("we0", D), ("we1", D), ("wdev", P)]
+
+
+class IcImplArgs(C.Structure):
+    _fields_ = [("b", P), ("db", P), ("ddb", P), ("aj", P), ("dj", P), ("ddj", P),
+                ("old_b", P), ("old_j", P), ("impl_b", P), ("impl_j", P), ("dLh", P), ("l", P),
+                ("O_r_ic", P), ("L", I), ("n_ic", I), ("fac", D), ("or2_icb", D)]
+
+
 class BuoyArgs(C.Structure):
     _fields_ = [("rwp", P), ("s", P), ("ew0", P), ("rgrav", P), ("m", P), ("L", I), ("N", I),
                 ("fac", D), ("BuoFac", D), ("lm_l0", I), ("pad", I), ("wdev", P)]
@@ -144,7 +156,8 @@ class StepCtlArgs(C.Structure):
 
 
 _STRUCT_ORDER = [SynArgs, AnaArgs, FftArgs, NlArgs, RowJob, TdArgs, RhsArgs, BuoyArgs,
-                 ImplArgs, EnergyArgs, BuildArgs, GemmArgs, GridArgs, StepState, StepCtlArgs]
+                 ImplArgs, EnergyArgs, BuildArgs, GemmArgs, GridArgs, StepState, StepCtlArgs,
+                 IcRhsArgs, IcImplArgs]
 
 _SIGS = {
     "mh_plm_tables": [I, I, I, I, I, P, P, P, P, P, P, P, P, P],
@@ -154,6 +167,8 @@ _SIGS = {
     "mh_get_nl": [P, P],
     "mh_grid_step": [P, P],
     "mh_grid_plan": [I, P],
+    "mh_ic_rhs": [P, P],
+    "mh_ic_impl": [P, P],
     "mh_tables_to_f32": [P, P, L, P],
     "mh_tables_to_bf16": [P, P, P, P, P, P, I, I, I, I, P, P, P, P, P],
     "mh_rowop": [P, P, I, I, P],

@@ -91,6 +91,9 @@ def save_checkpoint(run, path, step: int | None = None, sim_time: float | None =
     if getattr(run, "dirk", False):
         raise NotImplementedError("restart files hold CNAB2 time arrays; BPR353 runs are not "
                                   "written to them")
+    if getattr(run, "cond_ic", False):
+        raise NotImplementedError("restart files of runs with a conducting inner core are not "
+                                  "written (the inner-core arrays are not stored)")
     torch.save(checkpoint_state(run, step, sim_time), path)
 
 
@@ -108,6 +111,9 @@ def apply_checkpoint_state(run, state: dict) -> tuple[int, float]:
     if getattr(run, "dirk", False):
         raise NotImplementedError("restart files hold CNAB2 time arrays; BPR353 runs are not "
                                   "restarted from them")
+    if getattr(run, "cond_ic", False):
+        raise NotImplementedError("runs with a conducting inner core are not restarted from "
+                                  "restart files")
     ref = run.f["s"]
     shape = tuple(ref.shape)
     for name, key in FIELD_NAMES.items():

@@ -215,6 +215,9 @@ def write_checkpoint_fortran(path, run, sim_time: float, n_time_step: int,
     (``checkpoint_io.py:336-490``; time scale 1 as in the Boussinesq
     benchmarks, ``pre_calculations.py:13-14``)."""
     p = run.p
+    if getattr(run, "cond_ic", False):
+        raise NotImplementedError("write_checkpoint_fortran: the inner-core fields of a "
+                                  "conducting inner core are not written")
     if getattr(run, "expl_pending", False):
         raise RuntimeError("write_checkpoint_fortran: a radial loop awaits its lm loop; "
                            "write after a whole step")

@@ -41,7 +41,7 @@ import math
 import numpy as np
 import torch
 
-from . import _lib
+from . import _lib, grid
 from .device import C128, F64, I32, Resolution, ptr, require_gpu, stream_handle
 from .params import LEGENDRE_PREC, Params
 
@@ -297,6 +297,24 @@ class DynamoRun:
         assert used[0] == self._pool.numel()
         self.cfl = torch.zeros(2 * N, dtype=torch.int64, device=dev)
         self.cfl_host = torch.zeros(2 * N, dtype=torch.int64).pin_memory()
+        # conducting inner core (sigma_ratio > 0): the inner fields on the
+        # even-Chebyshev grid, their old states and implicit terms (BPR353: one
+        # implicit slot per stage; the inner explicit terms of a non-rotating
+        # core are zero, update_b.py:875-901), and the coupled right-hand sides
+        # (L, N + n_ic) of the B and J systems
+        self.cond_ic = bool(p.l_cond_ic and p.l_mag)
+        if self.cond_ic:
+            Nic, NT = p.n_r_ic_max, p.n_r_tot
+            zi = lambda cols=Nic: torch.zeros(L, cols, dtype=C128, device=dev)
+            self.fic = {k: zi() for k in ("b_ic", "db_ic", "ddb_ic", "aj_ic", "dj_ic", "ddj_ic")}
+            self.old_ic = {k: zi() for k in ("b", "j")}
+            self.impl_ic = {k: zi() for k in ("b", "j")}
+            dirk = p.time_scheme.upper() == "BPR353"
+            self.implS_ic = [self.impl_ic] + [{k: zi() for k in ("b", "j")} for _ in range(3)] \
+                if dirk else [self.impl_ic]
+            self.rhs_c = {k: zi(NT) for k in ("b", "j")}
+            self.ic = grid.inner_core(p, r.rad["r_icb"])
+            self.d_ic = {"O_r_ic": torch.as_tensor(self.ic["O_r_ic"], dtype=F64).to(dev)}
         # launch shapes (timed once per shape, same results whatever the choice):
         # radial levels per workgroup of the grid step, waves per workgroup of
         # the Legendre synthesis
@@ -489,6 +507,17 @@ class DynamoRun:
         self.MT["wp"] = torch.zeros(lp1, self.K4wp, self.Rpad6, dtype=F64, device=dev)
         ops = self._radial_operators()
         self.Op3, self.Opw, self.Opp, self.cheb = ops["Op3"], ops["Opw"], ops["Opp"], ops["cheb"]
+        if self.cond_ic:
+            Nic, NT = self.p.n_r_ic_max, self.p.n_r_tot
+            self.K4c = -(-NT // 4) * 4
+            rp = lambda R: -(-R // 64) * 64
+            self.MTc = {k: {part: (torch.zeros(lp1, self.K4c, rp(3 * n), dtype=F64, device=dev),
+                                   3 * n, rp(3 * n))
+                            for part, n in (("oc", N), ("ic", Nic))} for k in ("b", "j")}
+            k4i = -(-Nic // 4) * 4
+            mt = np.zeros((k4i, rp(2 * Nic)))
+            mt[:Nic, :2 * Nic] = np.concatenate([self.ic["D1"], self.ic["D2"]]).T
+            self.MT_ic12 = (torch.as_tensor(mt, dtype=F64).to(dev), 2 * Nic, rp(2 * Nic))
         self.MT_d1, self.MT_d1r2, self.MT_d12, self.MT_d123 = (
             ops[k] for k in ("MT_d1", "MT_d1r2", "MT_d12", "MT_d123"))
         self.mat_nn = torch.empty(self.nsys * lp1, N, N, dtype=F64, device=dev)
@@ -576,9 +605,23 @@ class DynamoRun:
         the radial derivatives into the inverses.
         """
         self._rebuild_launches(wimp_lin0)
+        if self.cond_ic:
+            self._build_coupled(wimp_lin0)
         self.wimp_lin0 = wimp_lin0
         self.n_rebuild += 1
         self._queue_info_check()
+
+    def _build_coupled(self, wimp_lin0: float):
+        """The coupled outer/inner B and J row operators of a conducting inner
+        core for ``wimp_lin0`` (host assembly and inversion, magic_amd.inner_core),
+        copied into the run's operator storage on its stream."""
+        from . import inner_core
+        p, rad = self.p, self.res.rad
+        inv = dict(zip(("b", "j"), inner_core.coupled_inverses(p, rad, self.ic, wimp_lin0)))
+        for k in ("b", "j"):
+            (mo, _, _), (mi, _, _) = inner_core.row_operators(p, rad, self.ic, inv[k], self.K4c)
+            self.MTc[k]["oc"][0].copy_(torch.as_tensor(mo), non_blocking=False)
+            self.MTc[k]["ic"][0].copy_(torch.as_tensor(mi), non_blocking=False)
 
     def _rebuild_launches(self, wimp_lin0, ctl=None):
         """The launches of a rebuild (ten, whatever the number of systems).
@@ -672,7 +715,7 @@ class DynamoRun:
         jobs = []
         systems = [("s", self.rhs["s"], (f["s"], f["ds"], tmp["d2s"])),
                    ("z", self.rhs["z"], (f["z"], f["dz"], tmp["d2z"]))]
-        if self.p.l_mag:
+        if self.p.l_mag and not self.cond_ic:
             systems += [("b", self.rhs["b"], (f["b"], f["db"], f["ddb"])),
                         ("j", self.rhs["j"], (f["aj"], f["dj"], f["ddj"]))]
         ls = np.arange(lp1, dtype=np.int64)
@@ -682,6 +725,20 @@ class DynamoRun:
             MT = self.MT[key]
             jobs.append(job(ptr(X), N, row_ptrs, nrows_l, ptr(MT) + 8 * ls * self.K4 * self.Rpad3,
                             N, 3 * N, self.Rpad3, [ptr(y) for y in Y], N, N))
+        if self.p.l_mag and self.cond_ic:
+            # coupled B and J: outer (f, f', f'') and inner (f, f', f'') outputs of
+            # one (N + n_ic)-long right-hand side per mode
+            Nic, NT = self.p.n_r_ic_max, self.p.n_r_tot
+            fi = self.fic
+            for key, Yo, Yi in (("b", (f["b"], f["db"], f["ddb"]),
+                                 (fi["b_ic"], fi["db_ic"], fi["ddb_ic"])),
+                                ("j", (f["aj"], f["dj"], f["ddj"]),
+                                 (fi["aj_ic"], fi["dj_ic"], fi["ddj_ic"]))):
+                for part, Y, ld in (("oc", Yo, N), ("ic", Yi, Nic)):
+                    MT, R, Rp = self.MTc[key][part]
+                    jobs.append(job(ptr(self.rhs_c[key]), NT, row_ptrs, nrows_l,
+                                    ptr(MT) + 8 * ls * self.K4c * Rp, NT, R, Rp,
+                                    [ptr(y) for y in Y], ld, ld))
         self.jobs1 = upload(jobs)
         # lm loop, stage 2: coupled (w, p) per degree
         Y = [f["w"], f["dw"], f["ddw"], tmp["dddw"], f["p"], f["dp"]]
@@ -710,6 +767,11 @@ class DynamoRun:
                          [ptr(f["db"]), ptr(f["ddb"])], N, N),
                      job(ptr(f["aj"]), N, 0, L, ptr(self.MT_d12[0]), N, 2 * N, self.MT_d12[2],
                          [ptr(f["dj"]), ptr(f["ddj"])], N, N)]
+        if self.p.l_mag and self.cond_ic:
+            Nic, fi, (MT, R, Rp) = self.p.n_r_ic_max, self.fic, self.MT_ic12
+            jobs += [job(ptr(fi[k]), Nic, 0, L, ptr(MT), Nic, R, Rp,
+                         [ptr(fi[d1]), ptr(fi[d2])], Nic, Nic)
+                     for k, d1, d2 in (("b_ic", "db_ic", "ddb_ic"), ("aj_ic", "dj_ic", "ddj_ic"))]
         self.jobs_setup = upload(jobs)
 
     def _rowop(self, jt, wide=None):
@@ -763,7 +825,24 @@ class DynamoRun:
                 l, m = p.init_s1 // 100, p.init_s1 % 100
                 x = 2.0 * rr - r_cmb - r_icb
                 f["s"][lm(l, m)].add_(up(p.amp_s1 * (1.0 - 3.0 * x ** 2 + 3.0 * x ** 4 - x ** 6)))
-        if p.l_mag:
+        if p.l_mag and self.cond_ic:
+            # initB with a conducting inner core (init_fields.py:426-456)
+            for t in self.fic.values():
+                t.zero_()
+            r_ic = self.ic["r_ic"]
+            b_pol = p.amp_b1 * math.sqrt(3.0 * math.pi) / (3.0 + r_cmb)
+            b_tor = -4.0 / 3.0 * p.amp_b1 * math.sqrt(math.pi / 5.0)
+            f["b"][lm(1, 0)].copy_(up(b_pol * (rr ** 3 - 4.0 / 3.0 * r_cmb * rr ** 2)))
+            self.fic["b_ic"][lm(1, 0)].copy_(up(b_pol * r_icb ** 2 * (
+                0.5 * r_ic ** 2 / r_icb + 0.5 * r_icb - 4.0 / 3.0 * r_cmb)))
+            f["aj"][lm(2, 0)].copy_(up(b_tor * rr * np.sin(math.pi * (rr / r_cmb))))
+            arg = math.pi * r_icb / r_cmb
+            sa, ca = math.sin(arg), math.cos(arg)
+            aj1 = (arg - 2.0 * sa * ca) / (arg + sa * ca)
+            aj2 = (1.0 - aj1) * r_icb * sa / ca
+            self.fic["aj_ic"][lm(2, 0)].copy_(up(b_tor * (
+                aj1 * r_ic * np.sin(math.pi * r_ic / r_cmb) + aj2 * np.cos(math.pi * r_ic / r_cmb))))
+        elif p.l_mag:
             b_pol = p.amp_b1 * math.sqrt(3.0 * math.pi) / 4.0
             b_tor = -4.0 / 3.0 * p.amp_b1 * math.sqrt(math.pi / 5.0)
             f["b"][lm(1, 0)].copy_(up(b_pol * (rr ** 3 - 4.0 / 3.0 * r_cmb * rr ** 2
@@ -784,6 +863,24 @@ class DynamoRun:
         for k in ("b", "j"):
             self.impl[k][:, 0] = 0.0
             self.impl[k][:, -1] = 0.0
+        if self.cond_ic:   # get_mag_ic_rhs_imp (update_b.py:803-873)
+            for d in [self.old_ic] + self.implS_ic:
+                for t in d.values():
+                    t.zero_()
+            self._ic_impl(self.old_ic, self.impl_ic)
+
+    def _ic_impl(self, old, impl):
+        """Inner-core old states (old: None for none) and implicit terms."""
+        p, fi = self.p, self.fic
+        a = _lib.IcImplArgs()
+        a.b, a.db, a.ddb = ptr(fi["b_ic"]), ptr(fi["db_ic"]), ptr(fi["ddb_ic"])
+        a.aj, a.dj, a.ddj = ptr(fi["aj_ic"]), ptr(fi["dj_ic"]), ptr(fi["ddj_ic"])
+        a.old_b, a.old_j = (None, None) if old is None else (ptr(old["b"]), ptr(old["j"]))
+        a.impl_b, a.impl_j = ptr(impl["b"]), ptr(impl["j"])
+        a.dLh, a.l, a.O_r_ic = ptr(self.res.d["dLh"]), ptr(self.res.d["l"]), ptr(self.d_ic["O_r_ic"])
+        a.L, a.n_ic = self.L, p.n_r_ic_max
+        a.fac, a.or2_icb = p.opm * p.O_sr, float(self.res.rad["or2"][-1])
+        _lib.call("mh_ic_impl", _lib.ref(a), stream_handle())
 
     def dt_cap(self, dt: float) -> float:
         """dt clamped to intfac * E for rotating runs (``step_time.py:1564-1570``)."""
@@ -975,6 +1072,8 @@ class DynamoRun:
         a.bots_l0, a.l_mag = math.sqrt(4.0 * math.pi), int(p.l_mag)
         with self._t("rhs"):
             _lib.call("mh_rhs", _lib.ref(a), stream_handle())
+            if self.cond_ic:
+                self._ic_rhs(wdev)
         with self._t("rowop_szbj"):
             self._rowop(self.jobs1)
         bq = _lib.BuoyArgs()
@@ -995,8 +1094,37 @@ class DynamoRun:
                 junk = {"s": r["s"], "z": r["z"], "b": r["b"], "j": r["j"],
                         "w": r["wp"][:, :self.N], "p": r["wp"][:, self.N:]}
                 self._impl(old=self.old if k == 4 else junk, impl=self.implS[k % 4])
+                if self.cond_ic:
+                    self._ic_impl(self.old_ic if k == 4 else None, self.implS_ic[k % 4])
             else:
                 self._impl()
+                if self.cond_ic:
+                    self._ic_impl(self.old_ic, self.impl_ic)
+
+    def _ic_rhs(self, wdev):
+        """Coupled right-hand sides of the B and J systems (mh_ic_rhs): the inner
+        IMEX terms of the scheme (CNAB2 old + impl; BPR353 stage k the implicit
+        slots 0..k-1 with the stage weights), the outer ones from mh_rhs."""
+        a = _lib.IcRhsArgs()
+        for i, key in enumerate(("b", "j")):
+            sy = a.sys[i]
+            sy.old = ptr(self.old_ic[key])
+            if self.dirk:
+                k, I = self.stage, self.implS_ic
+                sy.impl = ptr(I[0][key])
+                if k >= 2:
+                    sy.t4 = ptr(I[1][key])
+                if k >= 3:
+                    sy.t6 = ptr(I[2][key])
+                if k >= 4:
+                    sy.t7 = ptr(I[3][key])
+            else:
+                sy.impl = ptr(self.impl_ic[key])
+        a.rb, a.rj = ptr(self.rhs["b"]), ptr(self.rhs["j"])
+        a.out_b, a.out_j = ptr(self.rhs_c["b"]), ptr(self.rhs_c["j"])
+        a.m, a.L, a.N, a.n_ic = ptr(self.res.d["m"]), self.L, self.N, self.p.n_r_ic_max
+        a.wdev = wdev
+        _lib.call("mh_ic_rhs", _lib.ref(a), stream_handle())
 
     def one_step(self, n_time_step: int, dt: float) -> float:
         """One CNAB2 step (reference ``step_time.py:1574-1637``), or one BPR353
@@ -1185,8 +1313,9 @@ class DynamoRun:
         return st.dt
 
     def _no_dirk(self, what):
-        if self.dirk:
-            raise NotImplementedError(f"{what}: BPR353 runs are stepped with one_step / run_steps")
+        if self.dirk or self.cond_ic:
+            raise NotImplementedError(f"{what}: BPR353 runs and runs with a conducting inner "
+                                      "core are stepped with one_step / run_steps")
 
     def step_async(self):
         """Queue one whole step with the decisions on the device (one HIP graph
@@ -1228,8 +1357,10 @@ class DynamoRun:
         one launch sequence (one HIP graph replay when use_graphs) and the host
         never waits inside the loop.  Same decisions, weights and results as
         n_steps calls of one_step; dt as for one_step's first call.  Returns the
-        dt of the last step.  BPR353 steps are host-controlled one_step calls."""
-        if self.dirk:
+        dt of the last step.  BPR353 steps, and steps with a conducting inner core
+        (whose coupled operators are rebuilt on the host), are host-controlled
+        one_step calls."""
+        if self.dirk or self.cond_ic:
             for _ in range(n_steps):
                 dt = self.one_step(self.n_step + 1, dt)
             return dt

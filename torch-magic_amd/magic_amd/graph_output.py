@@ -56,6 +56,9 @@ def ic_radii(p, r_icb):
 def write_graph_file(path, time: float, run, fields=None):
     """Write the graph file of `run` (its current fields, or `fields`: a dict
     of w, dw, z, s, p, b, db, aj arrays) at `time` to `path`."""
+    if getattr(run, "cond_ic", False):
+        raise NotImplementedError("graph files of a conducting inner core (inner-core field "
+                                  "slabs from b_ic, aj_ic) are not written")
     p, res = run.p, run.res
     f = dict(run.f)
     if fields:

@@ -147,6 +147,46 @@ def radial(p: Params) -> dict:
                 delxr2=delxr2, delxh2=delxh2, valri2=fac / delxr2, valhi2=fac / delxh2)
 
 
+def inner_core(p: Params, r_icb: float) -> dict:
+    """Inner-core radial grid of a conducting inner core (reference
+    ``radial_functions.py:209-306``, ``radial_derivatives.py:182-195``): the
+    upper half r_ic[k] = r_icb y_k (k < n_r_ic_max - 1, then the centre r = 0)
+    of the 2 n_r_ic_max - 1 point Gauss-Lobatto grid y_k = cos(pi k / (2
+    n_r_ic_max - 2)) on [-r_icb, r_icb]; the even Chebyshev polynomials
+    cheb[n, k] = T_2n(y_k) and their r-derivatives (map factor 1/r_icb, the
+    three-term recursion run over T_2n and T_2n+1); the grid-to-grid derivative
+    operators as column operators (df = D1 f); the DCT-I of n_r_ic_max points
+    with the n_cheb_ic_max truncation (values = CT coefficients)."""
+    n = p.n_r_ic_max
+    n_tot = 2 * n - 1
+    y = np.cos(math.pi * np.arange(n_tot) / (n_tot - 1))
+    r_ic = np.zeros(n)
+    r_ic[:n - 1] = r_icb * y[:n - 1]
+    O_r_ic = np.zeros(n)
+    O_r_ic[:n - 1] = 1.0 / r_ic[:n - 1]
+    mf = 1.0 / r_icb
+    T = np.zeros((n, n)); T1 = np.zeros((n, n)); T2 = np.zeros((n, n))
+    yk = y[:n]
+    T[0] = 1.0
+    odd, odd1, odd2 = yk.copy(), np.full(n, mf), np.zeros(n)   # T_1 and its derivatives
+    for j in range(1, n):
+        T[j] = 2.0 * yk * odd - T[j - 1]
+        T1[j] = 2.0 * mf * odd + 2.0 * yk * odd1 - T1[j - 1]
+        T2[j] = 4.0 * mf * odd1 + 2.0 * yk * odd2 - T2[j - 1]
+        odd, odd1, odd2 = (2.0 * yk * T[j] - odd, 2.0 * mf * T[j] + 2.0 * yk * T1[j] - odd1,
+                           4.0 * mf * T1[j] + 2.0 * yk * T2[j] - odd2)
+    Tinv = np.linalg.inv(T)
+    kk = np.arange(n)
+    C = np.cos(math.pi * np.outer(kk, kk) / (n - 1))
+    C[:, 1:n - 1] *= 2.0
+    C /= math.sqrt(2.0 * (n - 1))
+    trunc = np.ones(n)
+    trunc[p.n_cheb_ic_max:] = 0.0
+    return dict(r_ic=r_ic, O_r_ic=O_r_ic, cheb=T, dcheb=T1, d2cheb=T2,
+                cnorm=math.sqrt(2.0 / (n - 1)), D1=(Tinv @ T1).T, D2=(Tinv @ T2).T,
+                CT=C * trunc[None, :])
+
+
 def fft_radices(n: int) -> list:
     """Factor n into radices 4, 2, 3, 5 (4s first)."""
     out = []

@@ -1,0 +1,82 @@
+"""Conducting inner core: the coupled outer-core + inner-core B and J operators.
+
+With ``sigma_ratio > 0`` the poloidal and toroidal field equations of the
+outer core and the inner core are one linear system per degree l over the
+n_r_tot = n_r_max + n_r_ic_max coefficients (outer Chebyshev, then inner even
+Chebyshev; reference ``update_b.py:301-513``, Chebyshev branch): rows 0 the
+CMB condition, 1..N-2 outer diffusion, N-1 continuity at the ICB, N the
+derivative match at the ICB (sigma_ratio on the outer dj/dr for J), N+1..NT-2
+inner diffusion, NT-1 the limit at the centre; boundary columns halved, rows
+scaled by their largest entry.
+
+The matrices change with the implicit weight only, and they are small
+((l_max + 1) systems of n_r_tot <= 3 n_r_max / 2 unknowns), so they are
+assembled and inverted on the host at each rebuild; the DCT-I and the radial
+derivatives of both shells are folded into the inverses here, giving the two
+row-operator matrices of each system: outer outputs (f, f', f'') on the
+n_r_max levels and inner outputs on the n_r_ic_max levels, both acting on the
+coupled right-hand side (reference ``update_b.py:634-748``).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def coupled_inverses(p, rad: dict, ic: dict, wimp_lin0: float):
+    """(b_inv, j_inv): (l_max + 1, NT, NT) inverses with the row scaling folded
+    in (inverse times diag(row factors)); degree 0 is zero."""
+    R, D, D2 = rad["T"], rad["T1"], rad["T2"]
+    N, Nic, NT = p.n_r_max, p.n_r_ic_max, p.n_r_tot
+    nci = p.n_cheb_ic_max
+    rn, cn = rad["rnorm"], ic["cnorm"]
+    or1, or2 = rad["or1"], rad["or2"]
+    o2 = or2[1:N - 1, None]
+    or2_icb = or2[N - 1]
+    cheb, dcheb, d2cheb = ic["cheb"], ic["dcheb"], ic["d2cheb"]
+    inner = slice(1, Nic - 1)
+    c_in, dc_in, d2c_in = cheb[:, inner].T, dcheb[:, inner].T, d2cheb[:, inner].T
+    orc = ic["O_r_ic"][inner, None]
+    out = {k: np.zeros((p.l_max + 1, NT, NT)) for k in ("b", "j")}
+    for l in range(1, p.l_max + 1):
+        dL = float(l * (l + 1))
+        for key in ("b", "j"):
+            a = np.zeros((NT, NT))
+            a[1:N - 1, :N] = rn * dL * o2 * (R[1:N - 1] - wimp_lin0 * p.opm
+                                            * (D2[1:N - 1] - dL * o2 * R[1:N - 1]))
+            a[0, :N] = rn * (D[0] + l * or1[0] * R[0]) if key == "b" else rn * R[0]
+            a[N - 1, :N] = rn * R[N - 1]
+            a[N - 1, N:N + nci] = -cn * cheb[:nci, 0]
+            a[N, :N] = rn * (1.0 if key == "b" else p.sigma_ratio) * D[N - 1]
+            a[N, N:N + nci] = -cn * (dcheb[:nci, 0] + (l + 1) * or1[N - 1] * cheb[:nci, 0])
+            if p.n_cheb_max < N:
+                a[(0, N - 1, N), p.n_cheb_max:N] = 0.0
+            a[N + 1:NT - 1, N:] = cn * dL * or2_icb * (
+                c_in - wimp_lin0 * p.opm * p.O_sr * (d2c_in + 2.0 * (l + 1) * orc * dc_in))
+            a[NT - 1, N:] = cn * dL * or2_icb * (
+                cheb[:, Nic - 1] - wimp_lin0 * p.opm * p.O_sr * (1.0 + 2.0 * (l + 1))
+                * d2cheb[:, Nic - 1])
+            for c in (0, N - 1, N, NT - 1):
+                a[:, c] *= 0.5
+            fac = 1.0 / np.abs(a).max(axis=1)
+            out[key][l] = np.linalg.inv(fac[:, None] * a) * fac[None, :]
+    return out["b"], out["j"]
+
+
+def row_operators(p, rad: dict, ic: dict, inv: np.ndarray, k4: int):
+    """Row-operator matrices MT[l][k][r] (k < k4 padded, r padded to 64) of one
+    coupled system: outer (values, d/dr, d2/dr2 on n_r_max levels) and inner
+    (the same on n_r_ic_max levels), each acting on the n_r_tot right-hand side."""
+    N, Nic = p.n_r_max, p.n_r_ic_max
+    CT, D1, D2 = rad["CT"], rad["D1"], rad["D2"]
+    Ci, D1i, D2i = ic["CT"], ic["D1"], ic["D2"]
+    op_oc = np.concatenate([CT, D1 @ CT, D2 @ CT])          # (3N, N) on outer coefficients
+    op_ic = np.concatenate([Ci, D1i @ Ci, D2i @ Ci])        # (3Nic, Nic) on inner ones
+    mats = []
+    for op, part in ((op_oc, inv[:, :N, :]), (op_ic, inv[:, N:N + Nic, :])):
+        full = np.einsum("rc,lck->lkr", op, part)           # (l, NT, R) = (op @ inv_part)^T
+        R = op.shape[0]
+        Rp = -(-R // 64) * 64
+        mt = np.zeros((p.l_max + 1, k4, Rp))
+        mt[:, :full.shape[1], :R] = full
+        mats.append((mt, R, Rp))
+    return mats

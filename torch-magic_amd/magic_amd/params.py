@@ -112,6 +112,11 @@ class Params:
             self.n_r_max = 2 * self.l_max + 1
         if self.n_cheb_max is None:
             self.n_cheb_max = self.n_r_max
+        # inner core (params.py:65-66,162-173): (n_r_max + 1) / 2 even-Chebyshev
+        # points; with a conducting inner core the B and J systems span both shells
+        self.n_r_ic_max = (self.n_r_max + 1) // 2
+        if self.n_cheb_ic_max is None:
+            self.n_cheb_ic_max = self.n_r_ic_max
         # azimuthal symmetry minc (params.py:60-76): the orders m = 0, minc, ...
         # (n_m of them), the grid one 2 pi / minc sector of n_phi = n_phi_tot /
         # minc longitudes; n_theta = n_phi_tot / 2 whatever minc
@@ -132,6 +137,9 @@ class Params:
         self.l_anel = self.strat > 0.0
         self.l_finite_diff = self.radial_scheme == "FD"
         self.l_cond_ic = self.sigma_ratio > 0.0
+        self.l_rot_ic = self.nRotIC != 0
+        self.n_r_tot = self.n_r_max + self.n_r_ic_max if self.l_cond_ic else self.n_r_max
+        self.O_sr = 1.0 / self.sigma_ratio if self.l_cond_ic else 0.0
         self.opr = 1.0 / self.pr
         self.opm = 1.0 / self.prmag
         self.CorFac = 1.0 / self.ek
@@ -146,18 +154,18 @@ class Params:
         self.amp_b1 = 5.0
 
     def check_step_supported(self):
-        """The device step covers the Boussinesq Chebyshev insulating-IC path with
-        CNAB2 or BPR353."""
+        """The device step covers the Boussinesq Chebyshev path with CNAB2 or
+        BPR353 and an insulating or conducting (not rotating) inner core."""
         bad = []
         if self.l_anel:
             bad.append("anelastic (strat > 0)")
         if self.l_finite_diff:
             bad.append("finite-difference radial scheme (use the banded solver API)")
-        if self.l_cond_ic:
-            bad.append("conducting inner core")
+        if self.l_cond_ic and self.nRotIC != 0:
+            bad.append("rotating inner core")
         if self.raxi != 0.0:
             bad.append("composition")
-        if self.nRotIC != 0:
+        if self.nRotIC != 0 and not self.l_cond_ic:
             bad.append("rotating inner core")
         if self.time_scheme.upper() not in ("CNAB2", "BPR353"):
             bad.append(f"time scheme {self.time_scheme}")
