@@ -45,11 +45,19 @@ from .sht import SHT
 class OracleRun:
     def __init__(self, cfg: dict):
         p = self.prm = Params(dict(cfg))
-        if p.l_anel or p.l_finite_diff or p.raxi != 0.0 or p.nRotIC != 0:
-            raise NotImplementedError("oracle step: Boussinesq Chebyshev, non-rotating inner core only")
+        if p.l_anel or p.l_finite_diff or p.raxi != 0.0:
+            raise NotImplementedError("oracle step: Boussinesq Chebyshev only")
         # conducting inner core (sigma_ratio > 0): IC fields on the even Chebyshev
         # grid, coupled OC+IC B/J solves (update_b.py:301-513,634-801)
         self.cond_ic = p.l_cond_ic and p.l_mag
+        # rotating conducting inner core (nRotIC = 1, no-slip ICB: l_z10mat,
+        # pre_calculations.py:82): omega_ic from z(l=1, m=0) at the ICB, its
+        # angular-momentum equation as an IMEX scalar, the Lorentz torque, the
+        # rigid-rotation induction at the ICB and the advection of the inner field
+        self.rot_ic = p.nRotIC != 0
+        if self.rot_ic and (p.nRotIC != 1 or not self.cond_ic or p.kbotv != 2):
+            raise NotImplementedError("oracle step: a rotating inner core needs nRotIC = 1, a "
+                                      "conducting inner core and a no-slip ICB")
         # BPR353 (time_scheme.py:157-326): four-stage SDIRK, explicit terms at
         # stages 1-3, implicit terms stored one stage ahead, old state at stage 4
         self.dirk = p.time_scheme == "BPR353"
@@ -69,6 +77,15 @@ class OracleRun:
             setattr(self, name, z())
         self.td = {k: dict(old=z(), impl=z(), expl=[z(), z()])
                    for k in ("s", "z", "w", "p", "b", "j")}
+        if self.rot_ic:   # pre_calculations.py:56-92 (Boussinesq: rho0 = 1)
+            y10 = 0.5 * math.sqrt(3.0 / math.pi)
+            or2_icb = (1.0 - p.radratio) ** 2 / p.radratio ** 2
+            self.c_z10_omega_ic = y10 * or2_icb
+            self.c_dt_z10_ic = 0.2 * p.radratio / (1.0 - p.radratio)
+            self.c_lorentz_ic = 0.25 * math.sqrt(3.0 / math.pi) * or2_icb
+            self.omega_ic = 0.0
+            self.om = dict(old=0.0, impl=[0.0], expl=[0.0, 0.0],
+                           implS=[0.0] * 4, explS=[0.0] * 4)
         if self.cond_ic:
             self.ic = radial.ic_grid(p, self.rad["r_icb"])
             zi = lambda: np.zeros((L, p.n_r_ic_max), np.complex128)
@@ -173,6 +190,15 @@ class OracleRun:
                 impl[:] = 0
                 impl[:, 1:-1] = (p.opm * dLh * or2[:, 1:-1]
                                  * (ddf[:, 1:-1] - dLh * or2[:, 1:-1] * f[:, 1:-1]))
+            if self.rot_ic:   # step_time.py:597-617
+                self.om.update(old=0.0, impl=[0.0], expl=[0.0, 0.0], implS=[0.0] * 4,
+                               explS=[0.0] * 4)
+                l1m0 = self.maps["lm2"][1, 0]
+                z10 = self.z[l1m0, -1].real
+                dz10 = (self.z @ self.D1.T)[l1m0, -1].real
+                self.om["old"] = self.c_dt_z10_ic * z10
+                self._om_impl_slot(0, -(2.0 * rad["or1"][-1] * z10 - dz10))
+                self.omega_ic = self.c_z10_omega_ic * z10
             if self.cond_ic:   # get_mag_ic_rhs_imp (update_b.py:803-873)
                 for f, df, ddf, key in ((self.b_ic, self.db_ic, self.ddb_ic, "b_ic"),
                                         (self.aj_ic, self.dj_ic, self.ddj_ic, "j_ic")):
@@ -180,6 +206,28 @@ class OracleRun:
                     ddf[:] = f @ self.ic["D2"]
                     self.td[key]["old"][1:] = dLh[1:] * or2[0, -1] * f[1:]
                     self._ic_impl(f, df, ddf, self.td[key]["impl"])
+
+    def _om_impl_slot(self, k, v):
+        if self.dirk:
+            self.om["implS"][k] = v
+        else:
+            self.om["impl"][0] = v
+
+    def _om_rhs(self):
+        """set_imex_rhs_scalar (time_scheme.py:98-108, 278-295)."""
+        om = self.om
+        if self.dirk:
+            k = self.istage
+            rhs = om["old"]
+            for j in range(k):
+                if self.bexp[k][j] != 0.0:
+                    rhs += self.bexp[k][j] * om["explS"][j]
+            for j in range(k):
+                if self.bimp[k][j] != 0.0:
+                    rhs += self.bimp[k][j] * om["implS"][j]
+            return rhs
+        return (self.wimp * om["old"] + self.wimp_lin[1] * om["impl"][0]
+                + self.wexp[0] * om["expl"][0] + self.wexp[1] * om["expl"][1])
 
     def _ic_impl(self, f, df, ddf, impl):
         """Inner-core diffusion terms (update_b.py:775-801): opm / sigma_ratio
@@ -253,9 +301,29 @@ class OracleRun:
         self.last_grid = dict(vr=vr, vt=vt, vp=vp, br=br, bt=bt, bp=bp, sc=sc)
         self.last_spec = dict(AdvrLM=AdvrLM, AdvtLM=AdvtLM, AdvpLM=AdvpLM, dVSrLM=dVSrLM,
                               VStLM=VStLM, VxBrLM=VxBrLM, VxBtLM=VxBtLM, VxBpLM=VxBpLM)
+        if self.cond_ic and self.rot_ic and self.omega_ic != 0.0:
+            # rigid rotation of the conducting core at the ICB (step_time.py:1126-1138)
+            sin2 = self.grid["sin_grid"][:, None] ** 2
+            vpc = rad["r"][-1] ** 2 * sin2 * self.omega_ic
+            vxbt = rad["or4"][-1] * vpc * br[N - 1]
+            S_icb, _ = sht.spat_to_sphertor(vxbt[None], np.zeros_like(vxbt)[None])
+            VxBtLM[:, N - 1] = S_icb[:, 0]
         td = get_td(self, AdvrLM, AdvtLM, AdvpLM, dVSrLM, VStLM, VxBrLM, VxBtLM, VxBpLM)
         for k, v in td.items():
             self._expl_slot(k)[:] = v
+        if self.rot_ic:   # Lorentz torque on the inner core (step_time.py:1181-1192)
+            gauss = self.grid["gauss"][self.grid["grid_idx"]][:, None]
+            torque = p.LFfac * 2.0 * math.pi / p.n_phi_max * float(
+                (gauss * br[N - 1] * bp[N - 1]).sum())
+            slot = max(0, self.istage - 1) if self.dirk else 0
+            self.om["explS" if self.dirk else "expl"][slot] = self.c_lorentz_ic * torque
+        if self.cond_ic:   # finish_exp_mag_ic (update_b.py:875-901)
+            for f, key in ((self.b_ic, "b_ic"), (self.aj_ic, "j_ic")):
+                e = self._expl_slot(key)
+                e[:] = 0.0
+                if self.rot_ic and self.omega_ic != 0.0:
+                    im_dLh = (1j * self.maps["lm2m"] * self.lmarr["dLh"])[:, None]
+                    e[:, 1:] = (-self.omega_ic * rad["or2"][-1]) * im_dLh * f[:, 1:]
         dtrkc, dthkc = courant_check(p, rad, self.grid, vr, vt, vp, br, bt, bp)
         return float(dtrkc.min()), float(dthkc.min())
 
@@ -317,6 +385,8 @@ class OracleRun:
         p, rad, mats = self.prm, self.rad, self.mats
         self.inv = dict(s=matrices.build_s(p, rad, mats, w0), z=matrices.build_z(p, rad, mats, w0),
                         wp=matrices.build_wp(p, rad, mats, w0), p0=matrices.build_p0(p, rad, mats))
+        if self.rot_ic:
+            self.inv["z10"] = matrices.build_z10(p, rad, mats, w0, self.c_dt_z10_ic)
         if p.l_mag and self.cond_ic:
             self.inv["b"], self.inv["j"] = matrices.build_bj_coupled(p, rad, mats, self.ic, w0)
         elif p.l_mag:
@@ -341,6 +411,12 @@ class OracleRun:
         rhs_z[:, 0] = 0.0; rhs_z[:, N - 1] = 0.0
         s_cheb = self._solve(self.inv["s"], rhs_s)
         z_cheb = self._solve(self.inv["z"], rhs_z)
+        if self.rot_ic:   # updateZ with the z10Mat (update_z.py:355-395)
+            l1m0 = self.maps["lm2"][1, 0]
+            r10 = rhs_z[l1m0].copy()
+            r10[N - 1] = self._om_rhs()
+            r10[0] = 0.0
+            z_cheb[l1m0] = self.inv["z10"] @ r10
         for c in (s_cheb, z_cheb):
             c[self.m0] = c[self.m0].real
             c[:, nc:] = 0.0
@@ -359,6 +435,9 @@ class OracleRun:
         w_cheb[:, nc:] = 0.0
         p_cheb[:, nc:] = 0.0
         self.z[:] = radial.costf(z_cheb)
+        if self.rot_ic:
+            z10_icb = self.z[self.maps["lm2"][1, 0], N - 1].real
+            self.omega_ic = self.c_z10_omega_ic * z10_icb
         self.w[:] = radial.costf(w_cheb)
         self.p[:] = radial.costf(p_cheb)
         D1, D2, D3 = self.D1, self.D2, self.D3
@@ -370,6 +449,16 @@ class OracleRun:
         if not self.dirk:
             for k in ("s", "z", "w", "p"):
                 self.td[k]["expl"][1][:] = self.td[k]["expl"][0]
+            if self.rot_ic:
+                self.om["expl"][1] = self.om["expl"][0]
+        if self.rot_ic:
+            l1m0 = self.maps["lm2"][1, 0]
+            z10, dz10 = self.z[l1m0, N - 1].real, self.dz[l1m0, N - 1].real
+            store = self.istage == 4 if self.dirk else True
+            if store:
+                self.om["old"] = self.c_dt_z10_ic * z10
+            self._om_impl_slot(self.istage % 4 if self.dirk else 0,
+                               -(2.0 * rad["or1"][N - 1] * z10 - dz10))
         if self._store():
             self.td["s"]["old"][:] = self.s
             self.td["z"]["old"][:] = dLh * or2 * self.z

@@ -75,6 +75,28 @@ def build_z(p: Params, rad, mats, wimp_lin0):
     return out
 
 
+def build_z10(p: Params, rad, mats, wimp_lin0, c_dt_z10_ic):
+    """z10Mat of the rotating inner core (``update_z.py:244-298``, Boussinesq,
+    time-integrated rotation): the l = 1 zMat bulk rows, z = 0 at the CMB, and
+    at the ICB the angular-momentum balance c_dt_z10_ic z + wimp_lin0 (2 z / r -
+    dz/dr); returns the (N, N) inverse with the row scaling folded in."""
+    R, D, D2, _ = mats
+    N = p.n_r_max
+    rn = rad["rnorm"]
+    or1 = rad["or1"]; or2c = rad["or2"][:, None]
+    dL = 2.0
+    dat = rn * dL * or2c * (R - wimp_lin0 * (D2 - dL * or2c * R))
+    dat[0] = rn * R[0]
+    dat[N - 1] = rn * (c_dt_z10_ic * R[N - 1]
+                       + wimp_lin0 * (2.0 * or1[N - 1] * R[N - 1] - D[N - 1]))
+    if p.n_cheb_max < N:
+        dat[0, p.n_cheb_max:] = 0.0
+        dat[N - 1, p.n_cheb_max:] = 0.0
+    _bfac_cols(dat, (0, N - 1))
+    datp, fac = _precond_rows(dat)
+    return np.linalg.inv(datp) * fac[None, :]
+
+
 def build_p0(p: Params, rad, mats):
     """l=0 pressure matrix inverse (``update_wp.py:80-170``, Chebyshev, Boussinesq)."""
     R, D, _, _ = mats

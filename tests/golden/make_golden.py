@@ -43,6 +43,12 @@ that each step returns passed to the next (as ``main.py`` does):
   ``radial_functions.py:209-306``), l_max=16, n_r_max=33 with the benchmark's
   truncations n_cheb_max=31, n_cheb_ic_max=15, CNAB2 and BPR353, 20 steps;
   the inner-core fields b_ic, aj_ic at the saved steps.
+* ``rotic`` and ``bprrotic4``: the same core rotating (nRotIC = 1, no-slip ICB:
+  the z10 system of ``update_z.py:244-298,355-501``, the Lorentz torque and
+  the rigid-rotation induction and advection of ``step_time.py:1126-1200``,
+  ``update_b.py:875-901``), CNAB2 at l_max=16 and BPR353 at l_max=32 with
+  minc=4 (the boussBenchSat combination), dt 2e-4, noise 1e-2, 30 steps;
+  omega_ic every step.
 
 Finite-difference operator cases (``fd4_ob2``, ``fd4_ob3``): the FD branches
 of ``build_s_matrices`` (``update_s.py:66-188``), ``build_z_matrices``
@@ -117,6 +123,13 @@ CASES = {
                                sigma_ratio=1.0, kbotb=3, time_scheme="BPR353"), dt=1e-4,
                       steps=20, noise=dict(seed=23, amp=1e-3), field_steps=[1, 20],
                       expl_step1=False),
+    "rotic": dict(cfg=dict(l_max=16, n_r_max=33, n_cheb_max=31, n_cheb_ic_max=15,
+                           sigma_ratio=1.0, kbotb=3, nRotIC=1), dt=2e-4, steps=30,
+                  noise=dict(seed=29, amp=1e-2), field_steps=[1, 30], expl_step1=False),
+    "bprrotic4": dict(cfg=dict(l_max=32, n_r_max=33, n_cheb_max=31, n_cheb_ic_max=15, minc=4,
+                               sigma_ratio=1.0, kbotb=3, nRotIC=1, time_scheme="BPR353"),
+                      dt=2e-4, steps=30, noise=dict(seed=31, amp=1e-2), field_steps=[1, 30],
+                      expl_step1=False),
 }
 
 FD_CASES = {
@@ -161,6 +174,8 @@ for n in range(1, case["steps"] + 1):
     ek = output.get_e_kin_full(fields.w_LMloc, fields.dw_LMloc, fields.z_LMloc)
     em = output.get_e_mag_oc_full(fields.b_LMloc, fields.db_LMloc, fields.aj_LMloc)
     en.append([ek.e_p, ek.e_t, em.e_p, em.e_t])
+    if int(case["cfg"].get("nRotIC", 0)) != 0:
+        res.setdefault("omega_ic", []).append(float(fields.omega_ic))
     if n in case["field_steps"]:
         for k in names:
             res[f"{k}_step{n}"] = getattr(fields, k + "_LMloc").numpy().copy()
@@ -185,6 +200,8 @@ for n in range(1, case["steps"] + 1):
             res[f"expl_{k}_step1"] = ta.expl[:, :, 1].numpy().copy()
 res["energies"] = np.array(en)
 res["dt"] = np.array(dts)
+if "omega_ic" in res:
+    res["omega_ic"] = np.array(res["omega_ic"])
 np.savez(out, **res)
 """
 

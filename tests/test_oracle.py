@@ -27,7 +27,7 @@ GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 # ------------------------------------------------------------ golden trajectory
 @pytest.mark.parametrize("name", ["bench16", "c1", "dtchange", "bpr16", "bprdt", "minc2",
-                                  "bprminc4", "condic", "bprcondic"])
+                                  "bprminc4", "condic", "bprcondic", "rotic", "bprrotic4"])
 def test_oracle_matches_reference_trajectory(name):
     """Energies every step and fields at the saved steps, from the reference's
     one_step with the returned dt passed to the next step; the dt sequence
@@ -38,7 +38,8 @@ def test_oracle_matches_reference_trajectory(name):
     bprminc4: azimuthal symmetry (orders m = 0, minc, 2 minc, ... on one
     2 pi / minc sector) with CNAB2 and with BPR353.  condic and bprcondic: a
     conducting inner core (coupled outer/inner B and J solves), whose fields
-    b_ic and aj_ic are matched too."""
+    b_ic and aj_ic are matched too.  rotic and bprrotic4: the core rotating,
+    with its rotation rate omega_ic matched every step."""
     case = json.load(open(os.path.join(GOLD, f"traj_{name}.json")))
     g = np.load(os.path.join(GOLD, f"traj_{name}.npz"))
     run = OracleRun(case["cfg"])
@@ -53,6 +54,8 @@ def test_oracle_matches_reference_trajectory(name):
         assert dt == pytest.approx(g["dt"][n - 1], rel=1e-12), (n, dt, g["dt"][n - 1])
         e = np.array(run.energies())
         assert np.all(np.abs(e - g["energies"][n - 1]) <= 1e-11 * np.abs(g["energies"][n - 1]))
+        if "omega_ic" in g.files:
+            assert run.omega_ic == pytest.approx(g["omega_ic"][n - 1], rel=1e-11), n
         if n in case["field_steps"]:
             keys = ("s", "z", "w", "p", "b", "aj") + (("b_ic", "aj_ic") if run.cond_ic else ())
             for k in keys:
