@@ -332,6 +332,48 @@ typedef struct {
 int mh_ic_impl(const MhIcImplArgs* args, void* stream);
 
 typedef struct {
+  double* rot;         /* rotation state: [0] omega_ic, [1] old state, [2..5] implicit
+                          terms, [6..9] explicit terms (Lorentz torque) of omega's equation */
+  const double* wdev;  /* IMEX weights laid out as MhRhsArgs.wdev */
+  int term[8];         /* rot indices of old, impl, expl0, expl1, t4, t5, t6, t7 in the
+                          right-hand side (-1: absent), weighted as in MhRhsArgs */
+  int expl_slot;       /* rot index the torque is written to */
+  int impl_slot;       /* rot index the implicit term is written to */
+  int store_old;       /* write rot[1] (CNAB2 always, BPR353 after the last stage) */
+  int lm10;            /* row of the mode (l=1, m=0) */
+  int L, N, n_ic, T, P;
+  double c_z10_omega;  /* omega_ic = c_z10_omega z10(r_icb) */
+  double c_dt_z10;     /* old state c_dt_z10 z10(r_icb) */
+  double c_lorentz;    /* torque term c_lorentz * torque */
+  double fac_lt;       /* LFfac 2 pi / n_phi */
+  double or1_icb, or2_icb, or4_icb, r_icb;
+  const double *br, *bp;      /* (T, P) ICB grid: radial, phi (sin(theta)-weighted) field */
+  double *vt, *vp;            /* (T, P) output grid: rigid-rotation induction, zero */
+  const double *gauss, *sin2; /* (T,) Gauss weight and sin^2(theta) of each grid row */
+  const double *b_ic, *aj_ic; /* (L, n_ic) inner fields */
+  double *e_b, *e_j;          /* (L, n_ic) inner explicit terms */
+  const int* m;               /* (L,) order of each mode */
+  const double* dLh;          /* (L,) l(l+1) */
+  double* rhs_z;              /* (L, N) toroidal right-hand side */
+  const double *z, *dz;       /* (L, N) toroidal velocity and its r-derivative */
+} MhIcRotArgs;
+
+/* Rotating conducting inner core (nRotIC = 1, no-slip ICB), reference
+ * step_time.py:1126-1200, update_b.py:875-901, update_z.py:355-501,
+ * step_time.py:597-617:
+ *   mh_ic_torque: Lorentz torque fac_lt sum gauss br bp over the ICB grid, its
+ *     explicit term c_lorentz torque into rot[expl_slot], and the rigid-rotation
+ *     induction vt = or4 r_icb^2 sin^2(theta) omega br (vp = 0) on the grid;
+ *   mh_ic_advect: inner explicit terms -omega or2(r_icb) i m l(l+1) f (0 at the ICB);
+ *   mh_ic_omega_rhs: omega's IMEX right-hand side into rhs_z(l=1, m=0) at the ICB;
+ *   mh_ic_omega_update: omega_ic = c_z10_omega z10, old state c_dt_z10 z10 and the
+ *     implicit term -(2 z10 / r - dz10/dr) at the ICB from the solved z. */
+int mh_ic_torque(const MhIcRotArgs* args, void* stream);
+int mh_ic_advect(const MhIcRotArgs* args, void* stream);
+int mh_ic_omega_rhs(const MhIcRotArgs* args, void* stream);
+int mh_ic_omega_update(const MhIcRotArgs* args, void* stream);
+
+typedef struct {
   const double *w, *dw, *z, *b, *db, *aj;
   const int* lm_start;
   const double* or2;

@@ -44,7 +44,8 @@ def _rel(a, b):
                                         ("dtchange", True), ("bpr16", True), ("bprdt", True),
                                         ("minc2", True), ("minc2", False),
                                         ("bprminc4", True), ("condic", True),
-                                        ("bprcondic", True)])
+                                        ("bprcondic", True), ("rotic", True), ("rotic", False),
+                                        ("bprrotic4", True)])
 def test_step_matches_reference_trajectory(name, fused):
     """fused: one grid-space kernel; otherwise separate FFT / products / FFT kernels.
     The dt each step returns is passed to the next (dtchange: the reference
@@ -54,7 +55,10 @@ def test_step_matches_reference_trajectory(name, fused):
     bprminc4: azimuthal symmetry minc = 2 (CNAB2) and 4 (BPR353), the orders
     m = 0, minc, ... on one 2 pi / minc sector of longitudes.  condic /
     bprcondic: a conducting inner core (coupled outer/inner B and J solves),
-    with the inner-core fields b_ic and aj_ic checked too."""
+    with the inner-core fields b_ic and aj_ic checked too.  rotic / bprrotic4:
+    the conducting core also rotates (omega_ic integrated in time from the
+    Lorentz torque, CNAB2 and BPR353 with minc = 4), its rotation rate checked
+    at every step."""
     case, g = _load(name)
     run = _run(case, fused)
     dt = case["dt"]
@@ -64,6 +68,8 @@ def test_step_matches_reference_trajectory(name, fused):
         e = np.array(run.energies())
         ge = g["energies"][n - 1]
         assert np.all(np.abs(e - ge) <= 1e-9 * np.abs(ge)), (n, e, ge)
+        if "omega_ic" in g.files:
+            assert run.omega_ic == pytest.approx(g["omega_ic"][n - 1], rel=1e-10), n
         if n in case["field_steps"]:
             for k in ("s", "z", "w", "p", "b", "aj"):
                 got = run.f[k].cpu().numpy()

@@ -307,6 +307,44 @@ def test_inner_core_grid_and_coupled_operators_match_oracle():
     got_ic = mi[l, :NT, :Ri].T @ rhs
     assert np.abs(got_oc - ref_oc).max() <= 1e-9 * np.abs(ref_oc).max()
     assert np.abs(got_ic - ref_ic).max() <= 1e-9 * np.abs(ref_ic).max()
-    p.check_step_supported()   # conducting, not rotating: on the device path
-    with pytest.raises(NotImplementedError):
-        Params(dict(cfg, nRotIC=1)).check_step_supported()
+    p.check_step_supported()   # conducting: on the device path
+    Params(dict(cfg, nRotIC=1)).check_step_supported()   # and rotating (no-slip ICB)
+    for bad in (dict(nRotIC=-1), dict(nRotIC=1, sigma_ratio=0.0), dict(nRotIC=1, kbotv=1),
+                dict(nRotIC=1, mode=1)):
+        with pytest.raises(NotImplementedError):
+            Params(dict(cfg, **bad)).check_step_supported()
+
+
+def test_rotating_inner_core_operator_matches_oracle():
+    """Rotating conducting inner core: the rotation constants and the z(l=1, m=0)
+    inverse against the oracle (pinned by the reference's rotating-core
+    trajectories, omega_ic included), and its row operator applied to a
+    right-hand side against the oracle's solve -> truncation -> DCT-I ->
+    derivatives."""
+    from magic_amd import inner_core
+    from oracle import matrices, radial as orad
+    from oracle.dynamo import OracleRun
+    cfg = dict(l_max=8, n_r_max=17, n_cheb_max=15, n_cheb_ic_max=7, sigma_ratio=0.7, kbotb=3,
+               nRotIC=1, minc=2)
+    p, o = Params(cfg), OracleRun(cfg)
+    c = inner_core.rotation_constants(p)
+    assert math.isclose(c["c_z10_omega"], o.c_z10_omega_ic, rel_tol=1e-15)
+    assert math.isclose(c["c_dt_z10"], o.c_dt_z10_ic, rel_tol=1e-15)
+    assert math.isclose(c["c_lorentz"], o.c_lorentz_ic, rel_tol=1e-15)
+    assert math.isclose(c["fac_lt"], o.prm.LFfac * 2.0 * math.pi / o.prm.n_phi_max, rel_tol=1e-15)
+    rad = grid.radial(p)
+    w0 = 2.5e-4
+    inv = inner_core.z10_inverse(p, rad, w0, c["c_dt_z10"])
+    oinv = matrices.build_z10(o.prm, o.rad, o.mats, w0, o.c_dt_z10_ic)
+    assert np.abs(inv - oinv).max() <= 1e-10 * np.abs(oinv).max()
+    N = p.n_r_max
+    k4, rpad = -(-N // 4) * 4, -(-(3 * N) // 64) * 64
+    mt = inner_core.z10_row_operator(p, rad, inv, k4, rpad)
+    assert mt.shape == (k4, rpad) and not mt[N:].any() and not mt[:, 3 * N:].any()
+    rhs = np.random.default_rng(7).standard_normal(N)
+    sol = oinv @ rhs
+    sol[p.n_cheb_max:] = 0.0
+    f = orad.costf(sol)
+    ref = np.concatenate([f, o.D1 @ f, o.D2 @ f])
+    got = mt[:N, :3 * N].T @ rhs
+    assert np.abs(got - ref).max() <= 1e-9 * np.abs(ref).max()

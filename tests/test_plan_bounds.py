@@ -248,3 +248,78 @@ def test_conducting_inner_core_launches_stay_inside_their_tensors(cpu_standin, s
                 _inside(ext, a.O_r_ic, a.O_r_ic + 8 * nic, "1/r inner")
                 n_ic_calls += 1
     assert n_ic_calls == 2 * len(stages)
+
+
+@pytest.mark.parametrize("scheme", ["CNAB2", "BPR353"])
+def test_rotating_inner_core_launches_stay_inside_their_tensors(cpu_standin, scheme):
+    """Rotating conducting inner core: the z(l=1, m=0) job and the z job without
+    that row, the ICB synthesis / FFTs / analysis of the radial plan, and the
+    arguments of mh_ic_torque, mh_ic_advect, mh_ic_omega_rhs and
+    mh_ic_omega_update (rotation-state slots included) in every recorded plan
+    lie inside the run's tensors."""
+    cfg = dict(l_max=16, n_r_max=17, n_cheb_ic_max=7, sigma_ratio=1.0, kbotb=3, nRotIC=1,
+               ek=1e-3, ra=1e5, pr=1.0, prmag=5.0, radratio=0.35, dtmax=1e-4, time_scheme=scheme)
+    with _lib.record():
+        run = dyn.DynamoRun(cfg, device="cpu")
+        run.initialize_dt(cfg["dtmax"])
+        run._rebuild_launches(0.5e-4)
+        run._build_coupled(0.5e-4)
+    assert run.rot_ic
+    ext = _extents(run)
+    _check_row_jobs(run, ext)
+    L, N, nic, T, P = run.L, run.N, run.p.n_r_ic_max, run.res.T, run.res.P
+    # the z job skips (l=1, m=0), which has a job of its own
+    jobs = np.frombuffer(run.jobs1[0].numpy().tobytes(), dtype=dyn._ROWJOB_DTYPE)
+    z10 = jobs[jobs["rows"] == run.d_ic["lm10"].data_ptr()]
+    assert len(z10) == 1 and z10["nrows"][0] == 1 and z10["MT"][0] == run.MT_z10.data_ptr()
+    rows = run.res.d["l_rows"].numpy()
+    zrows = [int(rows[(int(j["rows"]) - run.res.d["l_rows"].data_ptr()) // 4 + i])
+             for j in jobs[jobs["X"] == run.rhs["z"].data_ptr()] if j["MT"] != z10["MT"][0]
+             for i in range(int(j["nrows"]))]
+    assert run.lm10 not in zrows and sorted(zrows) == sorted(set(range(L)) - {run.lm10})
+    stages = (1, 2, 3, 4) if run.dirk else (0,)
+    seen = set()
+    for k in stages:
+        run.stage = k
+        kinds = ("radial", "lm") if k <= 3 else ("lm",)
+        for kind in kinds:
+            _, plan = run._plan(kind)
+            for _, name, args in plan:
+                if not name.startswith("mh_ic_") or name in ("mh_ic_rhs", "mh_ic_impl"):
+                    continue
+                a = C.cast(args[0], C.POINTER(_lib.IcRotArgs)).contents
+                seen.add(name)
+                _inside(ext, a.rot, a.rot + 8 * 10, "rotation state")
+                assert 6 <= a.expl_slot <= 9 and 2 <= a.impl_slot <= 5
+                assert all(-1 <= t <= 9 for t in a.term)
+                assert (a.L, a.N, a.n_ic, a.T, a.P, a.lm10) == (L, N, nic, T, P, run.lm10)
+                if name == "mh_ic_torque":
+                    assert a.expl_slot == 6 + (max(0, k - 1) if run.dirk else run._par)
+                    for v in (a.br, a.bp, a.vt, a.vp):
+                        _inside(ext, v, v + 8 * T * P, "ICB grid")
+                    for v in (a.gauss, a.sin2):
+                        _inside(ext, v, v + 8 * T, "theta weights")
+                elif name == "mh_ic_advect":
+                    for v in (a.b_ic, a.aj_ic, a.e_b, a.e_j):
+                        _inside(ext, v, v + 16 * L * nic, "inner field")
+                    _inside(ext, a.m, a.m + 4 * L, "orders")
+                    _inside(ext, a.dLh, a.dLh + 8 * L, "l(l+1)")
+                elif name == "mh_ic_omega_rhs":
+                    _inside(ext, a.rhs_z, a.rhs_z + 16 * L * N, "z rhs")
+                    used = [t for t in a.term if t >= 0]
+                    assert used[:2] == [1, 2]
+                    if run.dirk:
+                        assert len(used) == {1: 3, 2: 5, 3: 7, 4: 8}[k]
+                else:
+                    for v in (a.z, a.dz):
+                        _inside(ext, v, v + 16 * L * N, "z")
+                    assert a.store_old == (k == 4 if run.dirk else 1)
+                    assert a.impl_slot == 2 + (k % 4 if run.dirk else 0)
+            if kind == "radial":
+                ana = [args for _, name, args in plan if name == "mh_leg_analysis"]
+                a = C.cast(ana[-1][1], C.POINTER(_lib.AnaArgs)).contents
+                assert (a.B, a.ldo, a.prec) == (1, N, 0)
+                lo = a.outS[0]
+                assert lo == run.nl["VxBt"].data_ptr() + 16 * (N - 1)
+                _inside(ext, lo, lo + 16 * ((L - 1) * N + 1), "ICB VxBt")
+    assert seen == {"mh_ic_torque", "mh_ic_advect", "mh_ic_omega_rhs", "mh_ic_omega_update"}

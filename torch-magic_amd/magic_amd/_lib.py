@@ -110,6 +110,16 @@ class IcImplArgs(C.Structure):
                 ("O_r_ic", P), ("L", I), ("n_ic", I), ("fac", D), ("or2_icb", D)]
 
 
+class IcRotArgs(C.Structure):
+    _fields_ = [("rot", P), ("wdev", P), ("term", I * 8), ("expl_slot", I), ("impl_slot", I),
+                ("store_old", I), ("lm10", I), ("L", I), ("N", I), ("n_ic", I), ("T", I),
+                ("P", I), ("c_z10_omega", D), ("c_dt_z10", D), ("c_lorentz", D), ("fac_lt", D),
+                ("or1_icb", D), ("or2_icb", D), ("or4_icb", D), ("r_icb", D), ("br", P),
+                ("bp", P), ("vt", P), ("vp", P), ("gauss", P), ("sin2", P), ("b_ic", P),
+                ("aj_ic", P), ("e_b", P), ("e_j", P), ("m", P), ("dLh", P), ("rhs_z", P),
+                ("z", P), ("dz", P)]
+
+
 class BuoyArgs(C.Structure):
     _fields_ = [("rwp", P), ("s", P), ("ew0", P), ("rgrav", P), ("m", P), ("L", I), ("N", I),
                 ("fac", D), ("BuoFac", D), ("lm_l0", I), ("pad", I), ("wdev", P)]
@@ -157,7 +167,7 @@ class StepCtlArgs(C.Structure):
 
 _STRUCT_ORDER = [SynArgs, AnaArgs, FftArgs, NlArgs, RowJob, TdArgs, RhsArgs, BuoyArgs,
                  ImplArgs, EnergyArgs, BuildArgs, GemmArgs, GridArgs, StepState, StepCtlArgs,
-                 IcRhsArgs, IcImplArgs]
+                 IcRhsArgs, IcImplArgs, IcRotArgs]
 
 _SIGS = {
     "mh_plm_tables": [I, I, I, I, I, P, P, P, P, P, P, P, P, P],
@@ -169,6 +179,10 @@ _SIGS = {
     "mh_grid_plan": [I, P],
     "mh_ic_rhs": [P, P],
     "mh_ic_impl": [P, P],
+    "mh_ic_torque": [P, P],
+    "mh_ic_advect": [P, P],
+    "mh_ic_omega_rhs": [P, P],
+    "mh_ic_omega_update": [P, P],
     "mh_tables_to_f32": [P, P, L, P],
     "mh_tables_to_bf16": [P, P, P, P, P, P, I, I, I, I, P, P, P, P, P],
     "mh_rowop": [P, P, I, I, P],

@@ -9,7 +9,7 @@ extern "C" int mh_struct_sizes(int* out, int n) {
       (int)sizeof(MhRhsArgs),  (int)sizeof(MhBuoyArgs), (int)sizeof(MhImplArgs),
       (int)sizeof(MhEnergyArgs), (int)sizeof(MhBuildArgs), (int)sizeof(MhGemmArgs),
       (int)sizeof(MhGridArgs), (int)sizeof(MhStepState), (int)sizeof(MhStepCtlArgs),
-      (int)sizeof(MhIcRhsArgs), (int)sizeof(MhIcImplArgs),
+      (int)sizeof(MhIcRhsArgs), (int)sizeof(MhIcImplArgs), (int)sizeof(MhIcRotArgs),
   };
   const int ns = (int)(sizeof(sizes) / sizeof(sizes[0]));
   for (int i = 0; i < n && i < ns; ++i) out[i] = sizes[i];

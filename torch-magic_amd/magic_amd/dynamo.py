@@ -315,6 +315,31 @@ class DynamoRun:
             self.rhs_c = {k: zi(NT) for k in ("b", "j")}
             self.ic = grid.inner_core(p, r.rad["r_icb"])
             self.d_ic = {"O_r_ic": torch.as_tensor(self.ic["O_r_ic"], dtype=F64).to(dev)}
+        # rotating conducting inner core (nRotIC = 1, no-slip ICB): omega_ic is
+        # the ICB value of z(l=1, m=0) times c_z10_omega; its angular-momentum
+        # equation is an IMEX scalar whose terms live in `rot` ([0] omega_ic,
+        # [1] old state, [2..5] implicit terms, [6..9] explicit terms: the Lorentz
+        # torque).  The radial loop adds, at the ICB level only, the torque (a
+        # grid sum of br bphi), the induction of the rigidly rotating core (the
+        # theta component of v x B, analysed into VxBt at the ICB) and the
+        # advection of the inner field (the inner explicit terms, one array per
+        # explicit slot of the scheme); the z(l=1, m=0) solve uses its own
+        # operator (update_z.py:244-298)
+        self.rot_ic = bool(self.cond_ic and p.nRotIC == 1)
+        if self.rot_ic:
+            from . import inner_core
+            self.rot_c = inner_core.rotation_constants(p)
+            self.rot = torch.zeros(16, dtype=F64, device=dev)
+            self.expl_ic = [{k: zi() for k in ("b", "j")} for _ in range(4 if dirk else 2)]
+            self.F_icb = r.fourier_buffer(3, 1)
+            self.G_icb = r.grid_buffer(3, 1)
+            self.F2_icb = r.fourier_buffer(2, 1)
+            self.G2_icb = r.grid_buffer(2, 1)
+            upd = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=F64).to(dev)
+            self.d_ic["gauss"] = upd(r.th["gauss_grid"])
+            self.d_ic["sin2"] = upd(r.th["sin_grid"] ** 2)
+            self.lm10 = int(r.maps["lm_start"][0] + 1)
+            self.d_ic["lm10"] = torch.tensor([self.lm10], dtype=I32, device=dev)
         # launch shapes (timed once per shape, same results whatever the choice):
         # radial levels per workgroup of the grid step, waves per workgroup of
         # the Legendre synthesis
@@ -518,6 +543,8 @@ class DynamoRun:
             mt = np.zeros((k4i, rp(2 * Nic)))
             mt[:Nic, :2 * Nic] = np.concatenate([self.ic["D1"], self.ic["D2"]]).T
             self.MT_ic12 = (torch.as_tensor(mt, dtype=F64).to(dev), 2 * Nic, rp(2 * Nic))
+        if self.rot_ic:
+            self.MT_z10 = torch.zeros(self.K4, self.Rpad3, dtype=F64, device=dev)
         self.MT_d1, self.MT_d1r2, self.MT_d12, self.MT_d123 = (
             ops[k] for k in ("MT_d1", "MT_d1r2", "MT_d12", "MT_d123"))
         self.mat_nn = torch.empty(self.nsys * lp1, N, N, dtype=F64, device=dev)
@@ -613,8 +640,9 @@ class DynamoRun:
 
     def _build_coupled(self, wimp_lin0: float):
         """The coupled outer/inner B and J row operators of a conducting inner
-        core for ``wimp_lin0`` (host assembly and inversion, magic_amd.inner_core),
-        copied into the run's operator storage on its stream."""
+        core (and the z(l=1, m=0) operator of a rotating one) for ``wimp_lin0``
+        (host assembly and inversion, magic_amd.inner_core), copied into the
+        run's operator storage on its stream."""
         from . import inner_core
         p, rad = self.p, self.res.rad
         inv = dict(zip(("b", "j"), inner_core.coupled_inverses(p, rad, self.ic, wimp_lin0)))
@@ -622,6 +650,10 @@ class DynamoRun:
             (mo, _, _), (mi, _, _) = inner_core.row_operators(p, rad, self.ic, inv[k], self.K4c)
             self.MTc[k]["oc"][0].copy_(torch.as_tensor(mo), non_blocking=False)
             self.MTc[k]["ic"][0].copy_(torch.as_tensor(mi), non_blocking=False)
+        if self.rot_ic:
+            inv = inner_core.z10_inverse(p, rad, wimp_lin0, self.rot_c["c_dt_z10"])
+            mt = inner_core.z10_row_operator(p, rad, inv, self.K4, self.Rpad3)
+            self.MT_z10.copy_(torch.as_tensor(mt), non_blocking=False)
 
     def _rebuild_launches(self, wimp_lin0, ctl=None):
         """The launches of a rebuild (ten, whatever the number of systems).
@@ -723,7 +755,15 @@ class DynamoRun:
         nrows_l = np.diff(np.asarray(roff, np.int64))   # orders of each degree
         for key, X, Y in systems:
             MT = self.MT[key]
-            jobs.append(job(ptr(X), N, row_ptrs, nrows_l, ptr(MT) + 8 * ls * self.K4 * self.Rpad3,
+            rp_k, nr_k = row_ptrs, nrows_l
+            if key == "z" and self.rot_ic:
+                # (l=1, m=0), the first row of degree 1, has its own operator
+                rp_k, nr_k = row_ptrs.copy(), nrows_l.copy()
+                rp_k[1] += 4
+                nr_k[1] -= 1
+                jobs.append(job(ptr(X), N, ptr(self.d_ic["lm10"]), 1, ptr(self.MT_z10), N, 3 * N,
+                                self.Rpad3, [ptr(y) for y in Y], N, N))
+            jobs.append(job(ptr(X), N, rp_k, nr_k, ptr(MT) + 8 * ls * self.K4 * self.Rpad3,
                             N, 3 * N, self.Rpad3, [ptr(y) for y in Y], N, N))
         if self.p.l_mag and self.cond_ic:
             # coupled B and J: outer (f, f', f'') and inner (f, f', f'') outputs of
@@ -868,6 +908,12 @@ class DynamoRun:
                 for t in d.values():
                     t.zero_()
             self._ic_impl(self.old_ic, self.impl_ic)
+        if self.rot_ic:   # step_time.py:597-617
+            for d in self.expl_ic:
+                for t in d.values():
+                    t.zero_()
+            self.rot.zero_()
+            self._ic_omega_update(True, 2)
 
     def _ic_impl(self, old, impl):
         """Inner-core old states (old: None for none) and implicit terms."""
@@ -955,7 +1001,88 @@ class DynamoRun:
                        [ptr(nl["Advt"]) + off, ptr(nl["VSt"]) + off, ptr(nl["VxBt"]) + off],
                        [ptr(nl["Advp"]) + off, None, ptr(nl["VxBp"]) + off], N, vector=True,
                        prec=self.leg_prec, waves=self.ana_waves)
+        if self.rot_ic:
+            with self._t("ic_rot"):
+                self._icb_launches()
         self._td_launches()
+
+    def _rot_args(self):
+        """MhIcRotArgs with the run's rotation state and constants."""
+        p, rad, c = self.p, self.res.rad, self.rot_c
+        a = _lib.IcRotArgs()
+        a.rot, a.lm10, a.L, a.N, a.n_ic = ptr(self.rot), self.lm10, self.L, self.N, p.n_r_ic_max
+        a.T, a.P = self.res.T, self.res.P
+        a.c_z10_omega, a.c_dt_z10 = c["c_z10_omega"], c["c_dt_z10"]
+        a.c_lorentz, a.fac_lt = c["c_lorentz"], c["fac_lt"]
+        a.or1_icb, a.or2_icb = float(rad["or1"][-1]), float(rad["or2"][-1])
+        a.or4_icb, a.r_icb = float(rad["or4"][-1]), float(rad["r"][-1])
+        for q in range(8):
+            a.term[q] = -1
+        a.expl_slot, a.impl_slot = 6, 2
+        return a
+
+    def _ic_expl_slot(self):
+        """Explicit slot the radial loop writes: the parity's newest (CNAB2) or
+        stage - 1 (BPR353)."""
+        return max(0, self.stage - 1) if self.dirk else self._par
+
+    def _icb_launches(self):
+        """The rotating inner core's part of the radial loop (step_time.py:
+        1126-1138, 1181-1192; update_b.py:875-901): (br, btheta, bphi) on the ICB
+        grid (FP64 synthesis and inverse FFT of the ICB level), the Lorentz torque
+        and the grid of the rotation's induction, its spheroidal analysis into
+        VxBt at the ICB, and the advection of the inner field."""
+        r, f, N = self.res, self.f, self.N
+        icb = lambda t: t[:, N - 1:]
+        r.synth([[(icb(f["b"]), None, 1), (icb(f["db"]), None, 0), (icb(f["aj"]), None, 0)]],
+                1, N, self.F_icb, 1, vector=True, prec=0)
+        r.fft(False, self.F_icb, 1, self.G_icb, 3, 1)
+        a = self._rot_args()
+        n = r.T * r.P
+        a.br, a.bp = ptr(self.G_icb), ptr(self.G_icb) + 8 * 2 * n
+        a.vt, a.vp = ptr(self.G2_icb), ptr(self.G2_icb) + 8 * n
+        a.gauss, a.sin2 = ptr(self.d_ic["gauss"]), ptr(self.d_ic["sin2"])
+        k = self._ic_expl_slot()
+        a.expl_slot = 6 + k
+        _lib.call("mh_ic_torque", _lib.ref(a), stream_handle())
+        r.fft(True, self.F2_icb, 1, self.G2_icb, 2, 1, comp_scale=r.d["comp_scale_2"])
+        r.analysis(self.F2_icb, 1, 1, [(0, 1)], [ptr(self.nl["VxBt"]) + 16 * (N - 1)], [None], N,
+                   vector=True, prec=0)
+        e = self.expl_ic[k]
+        a.b_ic, a.aj_ic = ptr(self.fic["b_ic"]), ptr(self.fic["aj_ic"])
+        a.e_b, a.e_j = ptr(e["b"]), ptr(e["j"])
+        a.m, a.dLh = ptr(r.d["m"]), ptr(r.d["dLh"])
+        _lib.call("mh_ic_advect", _lib.ref(a), stream_handle())
+
+    def _ic_omega_rhs(self, wdev):
+        """omega's IMEX right-hand side into the ICB entry of the z(l=1, m=0)
+        right-hand side (update_z.py:355-371)."""
+        a = self._rot_args()
+        a.wdev, a.rhs_z = wdev, ptr(self.rhs["z"])
+        if self.dirk:
+            # old, I0, E0, E1, I1, E2, I2, I3 as the stage's weights (rhs layout)
+            k = self.stage
+            terms = [1, 2, 6] + ([7, 3] if k >= 2 else [-1, -1]) + \
+                    ([8, 4] if k >= 3 else [-1, -1]) + ([5] if k >= 4 else [-1])
+        else:
+            terms = [1, 2, 6 + self._par, 7 - self._par, -1, -1, -1, -1]
+        for q, t in enumerate(terms):
+            a.term[q] = t
+        _lib.call("mh_ic_omega_rhs", _lib.ref(a), stream_handle())
+
+    def _ic_omega_update(self, store_old, impl_slot):
+        """omega_ic, the old state (store_old) and the implicit term of omega's
+        equation from the solved z(l=1, m=0) (update_z.py:435-501)."""
+        a = self._rot_args()
+        a.store_old, a.impl_slot = int(store_old), impl_slot
+        a.z, a.dz = ptr(self.f["z"]), ptr(self.f["dz"])
+        _lib.call("mh_ic_omega_update", _lib.ref(a), stream_handle())
+
+    @property
+    def omega_ic(self) -> float:
+        """Rotation rate of the inner core (0.0 without a rotating core; reads
+        the device state: synchronises the run's stream)."""
+        return float(self.rot[0].item()) if self.rot_ic else 0.0
 
     def _grid_separate(self):
         r, N, Nb, d = self.res, self.N, self.Nb, self.res.d
@@ -1074,6 +1201,8 @@ class DynamoRun:
             _lib.call("mh_rhs", _lib.ref(a), stream_handle())
             if self.cond_ic:
                 self._ic_rhs(wdev)
+            if self.rot_ic:
+                self._ic_omega_rhs(wdev)
         with self._t("rowop_szbj"):
             self._rowop(self.jobs1)
         bq = _lib.BuoyArgs()
@@ -1096,15 +1225,21 @@ class DynamoRun:
                 self._impl(old=self.old if k == 4 else junk, impl=self.implS[k % 4])
                 if self.cond_ic:
                     self._ic_impl(self.old_ic if k == 4 else None, self.implS_ic[k % 4])
+                if self.rot_ic:
+                    self._ic_omega_update(k == 4, 2 + k % 4)
             else:
                 self._impl()
                 if self.cond_ic:
                     self._ic_impl(self.old_ic, self.impl_ic)
+                if self.rot_ic:
+                    self._ic_omega_update(True, 2)
 
     def _ic_rhs(self, wdev):
         """Coupled right-hand sides of the B and J systems (mh_ic_rhs): the inner
-        IMEX terms of the scheme (CNAB2 old + impl; BPR353 stage k the implicit
-        slots 0..k-1 with the stage weights), the outer ones from mh_rhs."""
+        IMEX terms of the scheme (CNAB2 old + impl, and the two explicit slots of
+        a rotating core; BPR353 stage k the implicit slots 0..k-1, and the
+        explicit slots 0..k-2 of a rotating core, with the stage weights), the
+        outer ones from mh_rhs."""
         a = _lib.IcRhsArgs()
         for i, key in enumerate(("b", "j")):
             sy = a.sys[i]
@@ -1118,8 +1253,18 @@ class DynamoRun:
                     sy.t6 = ptr(I[2][key])
                 if k >= 4:
                     sy.t7 = ptr(I[3][key])
+                if self.rot_ic:
+                    E = self.expl_ic
+                    sy.expl0 = ptr(E[0][key])
+                    if k >= 2:
+                        sy.expl1 = ptr(E[1][key])
+                    if k >= 3:
+                        sy.t5 = ptr(E[2][key])
             else:
                 sy.impl = ptr(self.impl_ic[key])
+                if self.rot_ic:
+                    sy.expl0 = ptr(self.expl_ic[self._par][key])
+                    sy.expl1 = ptr(self.expl_ic[1 - self._par][key])
         a.rb, a.rj = ptr(self.rhs["b"]), ptr(self.rhs["j"])
         a.out_b, a.out_j = ptr(self.rhs_c["b"]), ptr(self.rhs_c["j"])
         a.m, a.L, a.N, a.n_ic = ptr(self.res.d["m"]), self.L, self.N, self.p.n_r_ic_max

@@ -19,6 +19,8 @@ coupled right-hand side (reference ``update_b.py:634-748``).
 """
 from __future__ import annotations
 
+import math
+
 import numpy as np
 
 
@@ -80,3 +82,43 @@ def row_operators(p, rad: dict, ic: dict, inv: np.ndarray, k4: int):
         mt[:, :full.shape[1], :R] = full
         mats.append((mt, R, Rp))
     return mats
+
+
+def rotation_constants(p) -> dict:
+    """Coefficients of the inner core's rotation (reference
+    ``pre_calculations.py:56-92``, Boussinesq): omega_ic = c_z10_omega z10(r_icb),
+    old state c_dt_z10 z10(r_icb), torque term c_lorentz * torque, and the
+    torque's quadrature factor LFfac 2 pi / n_phi over one sector of the grid."""
+    or2_icb = (1.0 - p.radratio) ** 2 / p.radratio ** 2
+    return dict(c_z10_omega=0.5 * math.sqrt(3.0 / math.pi) * or2_icb,
+                c_dt_z10=0.2 * p.radratio / (1.0 - p.radratio),
+                c_lorentz=0.25 * math.sqrt(3.0 / math.pi) * or2_icb,
+                fac_lt=p.LFfac * 2.0 * math.pi / p.n_phi)
+
+
+def z10_inverse(p, rad: dict, wimp_lin0: float, c_dt_z10: float) -> np.ndarray:
+    """(N, N) inverse of the z(l=1, m=0) system of a rotating inner core with the
+    row scaling folded in (reference ``update_z.py:244-298``, time-integrated
+    rotation): the l = 1 toroidal diffusion rows, z = 0 at the CMB, and at the ICB
+    the angular-momentum balance c_dt_z10 z + wimp_lin0 (2 z / r - dz/dr)."""
+    R, D, D2 = rad["T"], rad["T1"], rad["T2"]
+    N, rn = p.n_r_max, rad["rnorm"]
+    or1, o2 = rad["or1"], rad["or2"][:, None]
+    a = rn * 2.0 * o2 * (R - wimp_lin0 * (D2 - 2.0 * o2 * R))
+    a[0] = rn * R[0]
+    a[N - 1] = rn * (c_dt_z10 * R[N - 1] + wimp_lin0 * (2.0 * or1[N - 1] * R[N - 1] - D[N - 1]))
+    if p.n_cheb_max < N:
+        a[(0, N - 1), p.n_cheb_max:] = 0.0
+    a[:, (0, N - 1)] *= 0.5
+    fac = 1.0 / np.abs(a).max(axis=1)
+    return np.linalg.inv(fac[:, None] * a) * fac[None, :]
+
+
+def z10_row_operator(p, rad: dict, inv: np.ndarray, k4: int, rpad: int) -> np.ndarray:
+    """MT[k][r] (k < k4, r < rpad) of the z(l=1, m=0) solve: values, d/dr and
+    d2/dr2 on the N levels from the right-hand side."""
+    N, CT = p.n_r_max, rad["CT"]
+    op = np.concatenate([CT, rad["D1"] @ CT, rad["D2"] @ CT]) @ inv     # (3N, N)
+    mt = np.zeros((k4, rpad))
+    mt[:N, :3 * N] = op.T
+    return mt

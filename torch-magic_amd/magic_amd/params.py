@@ -155,18 +155,20 @@ class Params:
 
     def check_step_supported(self):
         """The device step covers the Boussinesq Chebyshev path with CNAB2 or
-        BPR353 and an insulating or conducting (not rotating) inner core."""
+        BPR353 and an insulating, conducting or rotating conducting inner core
+        (nRotIC = 1: omega_ic integrated in time, no-slip ICB)."""
         bad = []
         if self.l_anel:
             bad.append("anelastic (strat > 0)")
         if self.l_finite_diff:
             bad.append("finite-difference radial scheme (use the banded solver API)")
-        if self.l_cond_ic and self.nRotIC != 0:
-            bad.append("rotating inner core")
+        if self.nRotIC not in (0, 1):
+            bad.append(f"nRotIC={self.nRotIC} (prescribed inner-core rotation)")
+        elif self.nRotIC == 1 and not (self.l_cond_ic and self.l_mag and self.kbotv == 2):
+            bad.append("rotating inner core without a conducting inner core, a magnetic "
+                       "field and a no-slip ICB")
         if self.raxi != 0.0:
             bad.append("composition")
-        if self.nRotIC != 0 and not self.l_cond_ic:
-            bad.append("rotating inner core")
         if self.time_scheme.upper() not in ("CNAB2", "BPR353"):
             bad.append(f"time scheme {self.time_scheme}")
         if self.l_correct_AMz or self.l_correct_AMe:
