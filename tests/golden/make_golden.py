@@ -347,6 +347,147 @@ np.savez(out + ".npz", **res)
 """
 
 
+# Restart files of a conducting, rotating inner core (CNAB2 and BPR353): the
+# reference's Fortran checkpoint (MULTISTEP with the inner-core and omega_ic
+# histories, DIRK with fields only), its G_1 graph file and its .pt restart file
+# after 10 steps, the state they were written from, 10 further steps in memory,
+# and 10 steps after each restart: from the Fortran file the way main.run's
+# fortran_restart does it (main.py:318-374) and from the .pt file through
+# main.load_checkpoint (main.py:226-267), with the Courant cap of
+# initialize_dt(dtmax) set by hand (a fresh process would keep it at 0)
+IC_FILES_CASES = {
+    "filesrot8": dict(cfg=dict(l_max=8, n_r_max=17, n_cheb_max=15, n_cheb_ic_max=7,
+                               sigma_ratio=1.0, kbotb=3, nRotIC=1, dtmax=2e-4),
+                      dt=2e-4, steps=10, noise=dict(seed=37, amp=1e-2), cont=10),
+    "filesbprrot8": dict(cfg=dict(l_max=8, n_r_max=17, n_cheb_max=15, n_cheb_ic_max=7,
+                                  sigma_ratio=1.0, kbotb=3, nRotIC=1, dtmax=2e-4,
+                                  time_scheme="BPR353"),
+                         dt=2e-4, steps=10, noise=dict(seed=41, amp=1e-2), cont=10),
+}
+
+IC_FILES_CHILD = r"""
+import json, sys
+import numpy as np
+case = json.loads(sys.argv[1]); out = sys.argv[2]
+sys.path.insert(0, sys.argv[3]); sys.path.insert(0, sys.argv[4])
+from magic_torch.config import configure
+configure(dict(case["cfg"], device="cpu", use_shtns=False))
+import torch
+torch.set_num_threads(8)
+from magic_torch import fields, dt_fields, step_time, output
+import magic_torch.main as _m
+from magic_torch.init_fields import initialize_fields
+from magic_torch.checkpoint_io import write_checkpoint_fortran
+from magic_torch.graph_output import write_graph_file
+from oracle.params import Params
+from oracle import horizontal, radial
+from oracle.dynamo import seeded_noise
+initialize_fields()
+p = Params(case["cfg"])
+nz = seeded_noise(p, horizontal.lm_maps(p), radial.radial_functions(p),
+                  case["noise"]["seed"], case["noise"]["amp"])
+fields.s_LMloc += torch.from_numpy(nz)
+step_time.setup_initial_state()
+step_time.initialize_dt(case["dt"])
+dt = case["dt"]; t = 0.0
+for n in range(1, case["steps"] + 1):
+    dt = step_time.one_step(n, dt)
+    t += dt
+write_checkpoint_fortran(out + "_checkpoint.bin", t, case["steps"] + 1)
+write_graph_file(out + "_G_1.bin", t, fields.w_LMloc, fields.dw_LMloc, fields.z_LMloc,
+                 fields.s_LMloc, fields.p_LMloc, fields.b_LMloc, fields.db_LMloc,
+                 fields.aj_LMloc)
+_m.save_checkpoint(out + "_restart.pt", case["steps"], t, case["cfg"])
+res = dict(time=np.array(t), dt=np.array([dt]), omega_ic=np.array(float(fields.omega_ic)))
+for k in ("w", "dw", "z", "p", "s", "b", "db", "aj"):
+    res[k] = getattr(fields, k + "_LMloc").numpy().copy()
+for k in ("b_ic", "aj_ic"):
+    res[k] = getattr(fields, k).numpy().copy()
+import magic_torch.time_scheme as ts
+res["tscheme_dt"] = ts.tscheme.dt.numpy().copy()
+en, om = [], []
+for n in range(case["steps"] + 1, case["steps"] + case["cont"] + 1):
+    dt = step_time.one_step(n, dt)
+    ek = output.get_e_kin_full(fields.w_LMloc, fields.dw_LMloc, fields.z_LMloc)
+    em = output.get_e_mag_oc_full(fields.b_LMloc, fields.db_LMloc, fields.aj_LMloc)
+    en.append([ek.e_p, ek.e_t, em.e_p, em.e_t])
+    om.append(float(fields.omega_ic))
+res["energies_cont"] = np.array(en)
+res["omega_cont"] = np.array(om)
+np.savez(out + ".npz", **res)
+"""
+
+IC_RESTART_CHILD = r"""
+import json, sys
+import numpy as np
+case = json.loads(sys.argv[1]); out = sys.argv[2]; kind = sys.argv[4]
+sys.path.insert(0, sys.argv[3])
+from magic_torch.config import configure
+configure(dict(case["cfg"], device="cpu", use_shtns=False))
+import torch
+torch.set_num_threads(8)
+from magic_torch import fields, dt_fields, step_time, output
+import magic_torch.main as _m
+from magic_torch.time_scheme import tscheme
+from magic_torch.precision import CDTYPE, DEVICE
+from magic_torch import checkpoint_io
+_read = checkpoint_io.read_checkpoint
+def _native(path):
+    # native-order copies of the big-endian arrays (torch.from_numpy refuses them)
+    ck = _read(path)
+    nat = lambda a: a.astype(a.dtype.newbyteorder("=")) if isinstance(a, np.ndarray) else a
+    for k in ("w", "z", "p", "s", "xi", "b", "aj", "b_ic", "aj_ic", "dt", "r"):
+        setattr(ck, k, nat(getattr(ck, k)))
+    ck.derivatives = {n: {part: [nat(a) for a in v] for part, v in d.items()}
+                      for n, d in ck.derivatives.items()}
+    return ck
+checkpoint_io.read_checkpoint = _native
+if kind == "fortran":
+    # main.run's fortran_restart branch (main.py:318-374), TensorBoard aside
+    ck = _m.load_fortran_checkpoint(out + "_checkpoint.bin")
+    step_time.setup_initial_state()
+    for i in range(min(len(ck.dt), tscheme.nexp)):
+        tscheme.dt[i] = ck.dt[i]
+    tscheme.set_weights()
+    dt = tscheme.dt[0].item()
+    step_time.initialize_dt(dt)
+    m = {"w": dt_fields.dwdt, "z": dt_fields.dzdt, "p": dt_fields.dpdt, "s": dt_fields.dsdt,
+         "b": dt_fields.dbdt, "aj": dt_fields.djdt, "b_ic": dt_fields.dbdt_ic,
+         "aj_ic": dt_fields.djdt_ic}
+    for name, d in ck.derivatives.items():
+        for part in ("expl", "impl", "old"):
+            for i, arr in enumerate(d[part]):
+                getattr(m[name], part)[:, :, i + 1].copy_(
+                    torch.from_numpy(arr).to(dtype=CDTYPE, device=DEVICE))
+    if ck.domega_ic_dt is not None:
+        for part in ("expl", "impl", "old"):
+            for i, v in enumerate(ck.domega_ic_dt[part]):
+                getattr(dt_fields.domega_ic_dt, part)[i + 1] = v
+    start = 0
+else:
+    start, _ = _m.load_checkpoint(out + "_restart.pt")
+    dt = tscheme.dt[0].item()
+    # a fresh process keeps the Courant cap at 0.0 (step_time.py:1551), so the
+    # reference's first step after this load would take dt = 0: give it the cap
+    # initialize_dt(dtmax) sets (this build does the same on a load)
+    from magic_torch.pre_calculations import ekScaled
+    step_time._dtMax = min(case["cfg"]["dtmax"], tscheme.intfac * ekScaled)
+en, om = [], []
+for n in range(start + 1, start + case["cont"] + 1):
+    dt = step_time.one_step(n, dt)
+    ek = output.get_e_kin_full(fields.w_LMloc, fields.dw_LMloc, fields.z_LMloc)
+    em = output.get_e_mag_oc_full(fields.b_LMloc, fields.db_LMloc, fields.aj_LMloc)
+    en.append([ek.e_p, ek.e_t, em.e_p, em.e_t])
+    om.append(float(fields.omega_ic))
+res = dict(np.load(out + ".npz"))
+res[f"energies_{kind}"] = np.array(en)
+res[f"omega_{kind}"] = np.array(om)
+res[f"w_{kind}"] = fields.w_LMloc.numpy().copy()
+res[f"b_ic_{kind}"] = fields.b_ic.numpy().copy()
+np.savez(out + ".npz", **res)
+"""
+
+
 MOVIES = ["Br CMB", "Vr CMB", "Vp CMB", "T CMB", "Br EQ", "Vr EQ", "T EQ", "Bt EQ"]
 
 MOVIE_CHILD = r"""
@@ -416,6 +557,21 @@ def main(names=None):
         subprocess.run([sys.executable, "-c", MOVIE_CHILD, json.dumps(FILES_CASE), out, REF_SRC,
                         REPO, json.dumps(MOVIES)], check=True, env=env)
         print("wrote movie files of", out)
+    for name, case in IC_FILES_CASES.items():
+        if names and name not in names:
+            continue
+        out = os.path.join(HERE, name)
+        subprocess.run([sys.executable, "-c", IC_FILES_CHILD, json.dumps(case), out, REF_SRC,
+                        REPO], check=True, env=env)
+        for kind in ("fortran", "pt"):
+            subprocess.run([sys.executable, "-c", IC_RESTART_CHILD, json.dumps(case), out,
+                            REF_SRC, kind], check=True, env=env)
+        meta = dict(case, source=src + ": checkpoint_io.write_checkpoint_fortran, "
+                    "graph_output.write_graph_file, main.save_checkpoint, main.run "
+                    "fortran_restart sequence, main.load_checkpoint")
+        with open(out + ".json", "w") as f:
+            json.dump(meta, f, indent=1)
+        print("wrote", out)
     for name, case in FD_CASES.items():
         if names and name not in names:
             continue

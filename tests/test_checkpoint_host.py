@@ -83,3 +83,56 @@ def test_restart_refuses_state_it_does_not_evolve(tmp_path):
     st = ck.checkpoint_state(run)
     with pytest.raises(ValueError, match="shape"):
         ck.apply_checkpoint_state(_stub(L=12), st)
+
+
+def _ic_stub(cfg):
+    """A run-shaped object with the arrays of a conducting, rotating inner-core
+    run of `cfg` (zeros)."""
+    p = Params(cfg, env=False)
+    L, N, nic = p.lm_max, p.n_r_max, p.n_r_ic_max
+    dirk = p.time_scheme.upper() == "BPR353"
+    z = lambda n=N: torch.zeros(L, n, dtype=torch.complex128)
+    sysk = ("s", "z", "w", "p", "b", "j")
+    d = lambda n=N: {k: z(n) for k in (sysk if n == N else ("b", "j"))}
+    run = types.SimpleNamespace(
+        p=p, dirk=dirk, cond_ic=True, rot_ic=True, _par=0,
+        f={k: z() for k in ("s", "ds", "z", "dz", "w", "dw", "ddw", "p", "dp", "b", "db",
+                            "ddb", "aj", "dj", "ddj")},
+        fic={k: z(nic) for k in ck.IC_FIELD_NAMES},
+        old=d(), impl=d(), expl_cur=d(), expl_prev=d(),
+        old_ic=d(nic), impl_ic=d(nic), rot=torch.zeros(16, dtype=torch.float64),
+        dt_hist=[0.0, 0.0], time=0.0, n_step=0, built=[])
+    run.implS = [run.impl, d(), d(), d()] if dirk else [run.impl]
+    run.explS = [run.expl_cur, run.expl_prev, d(), d()] if dirk else None
+    run.implS_ic = [run.impl_ic, d(nic), d(nic), d(nic)] if dirk else [run.impl_ic]
+    run.expl_ic = [d(nic) for _ in range(4 if dirk else 2)]
+    run.build_all_matrices = lambda w: run.built.append(w)
+    return run
+
+
+@pytest.mark.parametrize("name", ["filesrot8", "filesbprrot8"])
+def test_inner_core_restart_file_round_trips_the_reference_file(name):
+    """The reference's .pt restart file of a conducting, rotating inner core
+    (CNAB2 and BPR353, written after 10 steps): loaded into a run and written
+    again it gives the same keys, shapes and values (CNAB2: the reference keeps
+    the step's explicit term in both slots; BPR353: all four slots)."""
+    import json
+    import os
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    st = torch.load(os.path.join(gold, f"{name}_restart.pt"), weights_only=True)
+    cfg = json.load(open(os.path.join(gold, f"{name}.json")))["cfg"]
+    run = _ic_stub(cfg)
+    step, t = ck.apply_checkpoint_state(run, st)
+    assert (step, t) == (st["step"], st["sim_time"])
+    assert run.rot[0] == st["fields.omega_ic"] != 0.0
+    dirk = run.dirk
+    assert run.built == [(0.5 if dirk else run.p.alpha) * float(st["tscheme_dt"][0])]
+    out = ck.checkpoint_state(run)
+    assert set(out) == set(st)
+    for k, v in st.items():
+        if k == "cfg":
+            continue
+        if isinstance(v, torch.Tensor):
+            assert out[k].shape == v.shape and torch.equal(out[k], v), k
+        else:
+            assert out[k] == v, k

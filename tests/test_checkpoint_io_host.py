@@ -86,3 +86,58 @@ def test_writer_reproduces_the_reference_file(gold, tmp_path, endian):
     ck2 = cio.read_checkpoint(tmp_path / "ck2.bin")
     assert np.abs(ck2.r - r_ref).max() <= 1e-15 * r_ref[0]
     assert np.array_equal(ck2.w, gold["w"])
+
+
+IC_CASES = {"filesrot8": "MULTISTEP", "filesbprrot8": "DIRK"}
+
+
+def _ic_state_from_file(name, ck):
+    """A run-shaped state holding what the reference's file was written from:
+    fields and inner fields from the file, explicit histories (MULTISTEP) from
+    the file, the omega_ic history in the rotation buffer."""
+    import json
+    meta = json.load(open(os.path.join(GOLD, f"{name}.json")))
+    p = Params(meta["cfg"])
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a))
+    dirk = ck.family == "DIRK"
+    sysk = dict(w="w", z="z", p="p", s="s", b="b", aj="j")
+    zero = lambda a: torch.zeros(a.shape, dtype=torch.complex128)
+    hist = lambda k: t(ck.derivatives[k]["expl"][0]) if not dirk else zero(getattr(ck, k))
+    rot = torch.zeros(16, dtype=torch.float64)
+    if not dirk:
+        rot[7] = ck.domega_ic_dt["expl"][0]   # parity 0: slot 7 holds the history
+    return types.SimpleNamespace(
+        p=p, res=types.SimpleNamespace(rad={"r": ck.r}), dirk=dirk, cond_ic=True, _par=0,
+        f={k: t(getattr(ck, k)) for k in sysk},
+        fic={k: t(getattr(ck, k)) for k in ("b_ic", "aj_ic")},
+        expl_prev={sysk[k]: hist(k) for k in sysk},
+        expl_cur={sysk[k]: zero(getattr(ck, k)) for k in sysk},
+        expl_ic=[{"b": zero(ck.b_ic), "j": zero(ck.aj_ic)},
+                 {"b": hist("b_ic"), "j": hist("aj_ic")}],
+        rot=rot, expl_pending=False, dt_hist=[float(ck.dt[0]), float(ck.dt[-1])])
+
+
+@pytest.mark.parametrize("name", sorted(IC_CASES))
+def test_inner_core_checkpoint_reads_and_writes_like_the_reference(name, tmp_path):
+    """Conducting, rotating inner core (reference files after 10 steps of CNAB2
+    and of BPR353): the reader finds the family, slot counts, the inner fields
+    and (MULTISTEP) the inner-core and omega_ic histories; the writer, given the
+    same state, reproduces the file byte for byte."""
+    g = np.load(os.path.join(GOLD, f"{name}.npz"))
+    path = os.path.join(GOLD, f"{name}_checkpoint.bin")
+    ck = cio.read_checkpoint(path)
+    assert ck.family == IC_CASES[name] and ck.l_cond_ic and ck.l_mag
+    assert (ck.nexp, ck.nimp, ck.nold) == ((4, 4, 1) if ck.family == "DIRK" else (2, 1, 1))
+    assert ck.n_r_ic_max == 9 and ck.sigma_ratio == 1.0 and ck.time == float(g["time"])
+    for k in ("w", "z", "s", "b", "aj", "b_ic", "aj_ic"):
+        assert np.array_equal(getattr(ck, k), g[k]), k
+    if ck.family == "MULTISTEP":
+        assert set(ck.derivatives) == {"w", "z", "p", "s", "b", "aj", "b_ic", "aj_ic"}
+        assert len(ck.domega_ic_dt["expl"]) == 1 and ck.domega_ic_dt["expl"][0] != 0.0
+        assert np.any(ck.derivatives["b_ic"]["expl"][0])
+    else:
+        assert not ck.derivatives and ck.domega_ic_dt is None
+    run = _ic_state_from_file(name, ck)
+    out = tmp_path / "ck.bin"
+    cio.write_checkpoint_fortran(out, run, ck.time, ck.n_time_step)
+    assert out.read_bytes() == open(path, "rb").read()

@@ -116,3 +116,125 @@ def test_load_after_device_steps_resets_step_control(tmp_path):
         assert torch.equal(src.f[k], ref.f[k]), k
     for r in (src, ref):
         r.close()
+
+
+def _rel(a, b):
+    return float(np.abs(a - b).max() / np.abs(b).max())
+
+
+def _gold(name):
+    import json
+    import os
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    return (np.load(os.path.join(gold, f"{name}.npz")),
+            json.load(open(os.path.join(gold, f"{name}.json"))), os.path.join(gold, name))
+
+
+@pytest.mark.parametrize("name", ["filesrot8", "filesbprrot8"])
+@pytest.mark.parametrize("kind", ["fortran", "pt"])
+def test_inner_core_restart_from_reference_files_matches_reference(name, kind):
+    """Conducting, rotating inner core (CNAB2 and BPR353): a run restarted from
+    the reference's own Fortran checkpoint (main.run's fortran_restart sequence)
+    or its .pt restart file, both written after 10 steps, continues as the
+    reference did after the same restart: energies within 1e-9 and omega_ic
+    within 1e-10 every step, w and b_ic within 1e-10 after 10 steps."""
+    from magic_amd import checkpoint_io as cio
+    from magic_amd.dynamo import DynamoRun
+    g, meta, stem = _gold(name)
+    run = DynamoRun(dict(meta["cfg"]))
+    if kind == "fortran":
+        t, dt = cio.restart_from_fortran(run, stem + "_checkpoint.bin")
+        start = 0
+    else:
+        start, t = run.load_checkpoint(stem + "_restart.pt")
+        dt = run.dt_hist[0]
+    assert t == float(g["time"])
+    for i, n in enumerate(range(start + 1, start + meta["cont"] + 1)):
+        dt = run.one_step(n, dt)
+        e, ge = np.array(run.energies()), g[f"energies_{kind}"][i]
+        assert np.all(np.abs(e - ge) <= 1e-9 * np.abs(ge)), (n, e, ge)
+        assert run.omega_ic == pytest.approx(g[f"omega_{kind}"][i], rel=1e-10), n
+    assert _rel(run.f["w"].cpu().numpy(), g[f"w_{kind}"]) < 1e-10
+    assert _rel(run.fic["b_ic"].cpu().numpy(), g[f"b_ic_{kind}"]) < 1e-10
+    run.close()
+
+
+@pytest.mark.parametrize("scheme", ["CNAB2", "BPR353"])
+def test_inner_core_restart_continues_bit_for_bit(tmp_path, scheme):
+    """A rotating conducting inner core saved after 6 steps and restored into a
+    fresh run continues bit for bit (fields, inner fields, rotation state) like
+    the run that was never interrupted; the Fortran checkpoint it writes reads
+    back to the same fields and (CNAB2) histories."""
+    import torch
+    from magic_amd import checkpoint_io as cio
+    from magic_amd.dynamo import DynamoRun
+    cfg = dict(l_max=8, n_r_max=17, n_cheb_max=15, n_cheb_ic_max=7, sigma_ratio=1.0,
+               kbotb=3, nRotIC=1, dtmax=2e-4, time_scheme=scheme)
+    ref = DynamoRun(cfg)
+    ref.initialize_fields(noise_amp=1e-2, noise_seed=5)
+    ref.setup_initial_state()
+    ref.initialize_dt(2e-4)
+    dt = 2e-4
+    for n in range(1, 7):
+        dt = ref.one_step(n, dt)
+    path = tmp_path / "r.pt"
+    ref.save_checkpoint(path)
+    ckf = tmp_path / "ck.bin"
+    cio.write_checkpoint_fortran(ckf, ref, ref.time, 7)
+    ck = cio.read_checkpoint(ckf)
+    assert ck.family == ("DIRK" if ref.dirk else "MULTISTEP")
+    assert np.array_equal(ck.b_ic, ref.fic["b_ic"].cpu().numpy())
+    if not ref.dirk:
+        assert np.array_equal(ck.derivatives["aj_ic"]["expl"][0],
+                              ref.expl_ic[1 - ref._par]["j"].cpu().numpy())
+        assert ck.domega_ic_dt["expl"][0] == float(ref.rot[7 - ref._par])
+    for n in range(7, 13):
+        dt = ref.one_step(n, dt)
+    a = DynamoRun(cfg)
+    step, _ = a.load_checkpoint(path)
+    dt = a.dt_hist[0]
+    for n in range(step + 1, 13):
+        dt = a.one_step(n, dt)
+    assert a.time == ref.time and a.dt_log == ref.dt_log[6:]
+    for k in ("s", "z", "w", "p", "b", "aj"):
+        assert torch.equal(a.f[k], ref.f[k]), k
+    for k in ("b_ic", "aj_ic"):
+        assert torch.equal(a.fic[k], ref.fic[k]), k
+    assert torch.equal(a.rot[:10], ref.rot[:10])
+    for r in (ref, a):
+        r.close()
+
+
+def test_inner_core_graph_file_matches_reference():
+    """G_1 file of a conducting, rotating inner-core state: the reference's
+    header (l_cond_ic flag, sigma_ratio) byte for byte, every float32 value
+    within 2e-6 of the largest value of its field."""
+    import tempfile
+    import os
+    from magic_amd.dynamo import DynamoRun
+    from magic_amd.graph_output import write_graph_file
+    g, meta, stem = _gold("filesrot8")
+    run = DynamoRun(dict(meta["cfg"]))
+    ref = open(stem + "_G_1.bin", "rb").read()
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "G_1.t")
+        write_graph_file(path, float(g["time"]), run,
+                         {k: g[k] for k in ("w", "dw", "z", "s", "p", "b", "db", "aj")})
+        got = open(path, "rb").read()
+    assert len(got) == len(ref)
+    hdr = 4 + 64 + 4 * 10 + 4 * 11
+    assert got[:hdr] == ref[:hdr]
+    a = np.frombuffer(got[hdr:], ">f4").astype(np.float64)
+    b = np.frombuffer(ref[hdr:], ">f4").astype(np.float64)
+    T, P, N = run.p.n_theta, run.p.n_phi, run.p.n_r_max
+    n_ic = (N + 1) // 2
+    axes = T + N + n_ic
+    np.testing.assert_allclose(a[:axes], b[:axes], rtol=2e-7, atol=0)
+    slabs = (a[axes:].reshape(-1, T * P), b[axes:].reshape(-1, T * P))
+    oc = slabs[1][:8 * N].reshape(N, 8, T * P)
+    scale = np.abs(oc).max(axis=(0, 2))
+    got_oc = slabs[0][:8 * N].reshape(N, 8, T * P)
+    assert np.all(np.abs(got_oc - oc).max(axis=(0, 2)) <= 2e-6 * scale)
+    ic_ref, ic_got = slabs[1][8 * N:], slabs[0][8 * N:]
+    assert np.abs(ic_got - ic_ref).max() <= 2e-6 * np.abs(ic_ref).max()
+    run.close()

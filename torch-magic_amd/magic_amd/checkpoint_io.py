@@ -200,6 +200,7 @@ def read_checkpoint(path) -> CheckpointData:
 # CNAB2 slot counts and the order the fields are written in
 _NEXP, _NIMP, _NOLD = 2, 1, 1
 _SYS_OF = {"w": "w", "z": "z", "p": "p", "s": "s", "b": "b", "aj": "j"}
+_IC_SYS_OF = {"b_ic": "b", "aj_ic": "j"}
 
 
 def _history_slot(run, sys_key):
@@ -209,15 +210,21 @@ def _history_slot(run, sys_key):
         run.expl_prev[sys_key]
 
 
+def _ic_history_slot(run, sys_key):
+    """The inner core's explicit history (expl[:, :, 1] after a whole step)."""
+    return run.expl_ic[1 - run._par][sys_key]
+
+
 def write_checkpoint_fortran(path, run, sim_time: float, n_time_step: int,
                              endian: str = ">") -> None:
-    """Write `run`'s state as a MagIC v5 MULTISTEP checkpoint
-    (``checkpoint_io.py:336-490``; time scale 1 as in the Boussinesq
-    benchmarks, ``pre_calculations.py:13-14``)."""
+    """Write `run`'s state as a MagIC v5 checkpoint (``checkpoint_io.py:336-490``;
+    time scale 1 as in the Boussinesq benchmarks, ``pre_calculations.py:13-14``):
+    MULTISTEP (CNAB2) with the explicit histories of the fields and of the inner
+    core's rotation, or DIRK (BPR353) with the fields only; a conducting inner
+    core adds its fields b_ic, aj_ic (and their histories)."""
     p = run.p
-    if getattr(run, "cond_ic", False):
-        raise NotImplementedError("write_checkpoint_fortran: the inner-core fields of a "
-                                  "conducting inner core are not written")
+    dirk = getattr(run, "dirk", False)
+    cond_ic = getattr(run, "cond_ic", False)
     if getattr(run, "expl_pending", False):
         raise RuntimeError("write_checkpoint_fortran: a radial loop awaits its lm loop; "
                            "write after a whole step")
@@ -226,9 +233,14 @@ def write_checkpoint_fortran(path, run, sim_time: float, n_time_step: int,
     pk = lambda fmt, *v: out.extend(struct.pack(e + fmt, *v))
     pk("i", 5)
     pk("d", float(sim_time))
-    out.extend("MULTISTEP".ljust(10).encode("ascii"))
-    pk("3i", _NEXP, _NIMP, _NOLD)
-    pk("2d", float(run.dt_hist[0]), float(run.dt_hist[1]))
+    if dirk:
+        out.extend("DIRK".ljust(10).encode("ascii"))
+        pk("3i", 4, 4, 1)
+        pk("d", float(run.dt_hist[0]))
+    else:
+        out.extend("MULTISTEP".ljust(10).encode("ascii"))
+        pk("3i", _NEXP, _NIMP, _NOLD)
+        pk("2d", float(run.dt_hist[0]), float(run.dt_hist[1]))
     pk("i", int(n_time_step))
     pk("9d", p.ra, p.pr, p.raxi, p.sc, p.prmag, p.ek, p.stef, p.radratio, p.sigma_ratio)
     pk("6i", p.n_r_max, p.n_theta, p.n_phi_tot, p.minc, 20, (p.n_r_max + 1) // 2)
@@ -237,11 +249,14 @@ def write_checkpoint_fortran(path, run, sim_time: float, n_time_step: int,
     pk("2i", p.n_cheb_max, 0)
     pk("2d", p.alph1, p.alph2)
     out.extend(np.asarray(run.res.rad["r"], dtype=e + "f8").tobytes())
-    # rotation-rate histories (expl slot 1 of omega_ic, omega_ma): not evolved here
-    pk("d", 0.0)
-    pk("d", 0.0)
+    if not dirk:
+        # rotation-rate histories: expl slot 1 of omega_ic (the Lorentz-torque
+        # term of the last step), of omega_ma (not evolved here)
+        om_hist = float(run.rot[7 - run._par].item()) if cond_ic else 0.0
+        pk("d", om_hist)
+        pk("d", 0.0)
     pk("12d", p.omega_ic1, *([0.0] * 11))
-    for flag in (p.l_heat, False, False, p.l_mag, True, False):
+    for flag in (p.l_heat, False, False, p.l_mag, True, p.l_cond_ic):
         pk("i", 1 if flag else 0)
 
     def field_bytes(t):
@@ -251,7 +266,13 @@ def write_checkpoint_fortran(path, run, sim_time: float, n_time_step: int,
     names = ["w", "z", "p"] + (["s"] if p.l_heat else []) + (["b", "aj"] if p.l_mag else [])
     for name in names:
         out.extend(field_bytes(run.f[name]))
-        out.extend(field_bytes(_history_slot(run, _SYS_OF[name])))
+        if not dirk:
+            out.extend(field_bytes(_history_slot(run, _SYS_OF[name])))
+    if cond_ic:
+        for name, key in _IC_SYS_OF.items():
+            out.extend(field_bytes(run.fic[name]))
+            if not dirk:
+                out.extend(field_bytes(_ic_history_slot(run, key)))
     with open(path, "wb") as f:
         f.write(bytes(out))
 
@@ -290,9 +311,17 @@ def load_fortran_checkpoint(run, path) -> CheckpointData:
             _translate_s00(run)
     if ck.xi is not None and np.any(ck.xi):
         raise ValueError("checkpoint holds a composition field; this build does not evolve it")
-    if ck.l_cond_ic:
-        raise ValueError("checkpoint holds a conducting inner core; this build's step is "
-                         "insulating")
+    # inner-core fields when the run has a conducting core (kept otherwise, as the
+    # reference does); omega_ic is set from z(l=1, m=0) by setup_initial_state
+    if getattr(run, "cond_ic", False):
+        for name in ("b_ic", "aj_ic"):
+            a = getattr(ck, name)
+            if a is None:
+                continue
+            if a.shape != tuple(run.fic[name].shape):
+                raise ValueError(f"checkpoint field {name} is {a.shape}, the run is "
+                                 f"{tuple(run.fic[name].shape)}")
+            run.fic[name].copy_(torch.from_numpy(np.ascontiguousarray(a)))
     return ck
 
 
@@ -307,11 +336,21 @@ def restart_from_fortran(run, path) -> tuple[float, float]:
     run.setup_initial_state()
     dt = float(ck.dt[0]) if len(ck.dt) else run.p.dtmax
     run.initialize_dt(dt)
+    if getattr(run, "dirk", False):
+        # a DIRK file holds no histories, and a BPR353 step reads none
+        run.time = ck.time
+        return ck.time, run.dt_hist[0]
+    cond_ic = getattr(run, "cond_ic", False)
     for name, d in ck.derivatives.items():
         key = _SYS_OF.get(name)
-        if key is None or (key in ("b", "j") and not run.p.l_mag):
-            continue
-        for i, a in enumerate(d["expl"][:1]):
-            run.expl_prev[key].copy_(torch.from_numpy(np.ascontiguousarray(a)))
+        if key is not None and (key not in ("b", "j") or run.p.l_mag):
+            for i, a in enumerate(d["expl"][:1]):
+                run.expl_prev[key].copy_(torch.from_numpy(np.ascontiguousarray(a)))
+        key = _IC_SYS_OF.get(name)
+        if key is not None and cond_ic:
+            for i, a in enumerate(d["expl"][:1]):
+                _ic_history_slot(run, key).copy_(torch.from_numpy(np.ascontiguousarray(a)))
+    if getattr(run, "rot_ic", False) and ck.domega_ic_dt and ck.domega_ic_dt["expl"]:
+        run.rot[7 - run._par] = float(ck.domega_ic_dt["expl"][0])
     run.time = ck.time
     return ck.time, run.dt_hist[0]

@@ -323,14 +323,17 @@ class DynamoRun:
         # grid sum of br bphi), the induction of the rigidly rotating core (the
         # theta component of v x B, analysed into VxBt at the ICB) and the
         # advection of the inner field (the inner explicit terms, one array per
-        # explicit slot of the scheme); the z(l=1, m=0) solve uses its own
-        # operator (update_z.py:244-298)
+        # explicit slot of the scheme: zeros written every radial loop when the
+        # core does not rotate, as the reference writes them); the z(l=1, m=0)
+        # solve uses its own operator (update_z.py:244-298)
         self.rot_ic = bool(self.cond_ic and p.nRotIC == 1)
-        if self.rot_ic:
+        if self.cond_ic:
             from . import inner_core
             self.rot_c = inner_core.rotation_constants(p)
             self.rot = torch.zeros(16, dtype=F64, device=dev)
             self.expl_ic = [{k: zi() for k in ("b", "j")} for _ in range(4 if dirk else 2)]
+            self.lm10 = int(r.maps["lm_start"][0] + 1)
+        if self.rot_ic:
             self.F_icb = r.fourier_buffer(3, 1)
             self.G_icb = r.grid_buffer(3, 1)
             self.F2_icb = r.fourier_buffer(2, 1)
@@ -338,7 +341,6 @@ class DynamoRun:
             upd = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=F64).to(dev)
             self.d_ic["gauss"] = upd(r.th["gauss_grid"])
             self.d_ic["sin2"] = upd(r.th["sin_grid"] ** 2)
-            self.lm10 = int(r.maps["lm_start"][0] + 1)
             self.d_ic["lm10"] = torch.tensor([self.lm10], dtype=I32, device=dev)
         # launch shapes (timed once per shape, same results whatever the choice):
         # radial levels per workgroup of the grid step, waves per workgroup of
@@ -908,11 +910,12 @@ class DynamoRun:
                 for t in d.values():
                     t.zero_()
             self._ic_impl(self.old_ic, self.impl_ic)
-        if self.rot_ic:   # step_time.py:597-617
+        if self.cond_ic:
             for d in self.expl_ic:
                 for t in d.values():
                     t.zero_()
             self.rot.zero_()
+        if self.rot_ic:   # step_time.py:597-617
             self._ic_omega_update(True, 2)
 
     def _ic_impl(self, old, impl):
@@ -1001,8 +1004,8 @@ class DynamoRun:
                        [ptr(nl["Advt"]) + off, ptr(nl["VSt"]) + off, ptr(nl["VxBt"]) + off],
                        [ptr(nl["Advp"]) + off, None, ptr(nl["VxBp"]) + off], N, vector=True,
                        prec=self.leg_prec, waves=self.ana_waves)
-        if self.rot_ic:
-            with self._t("ic_rot"):
+        if self.cond_ic:
+            with self._t("ic_expl"):
                 self._icb_launches()
         self._td_launches()
 
@@ -1027,27 +1030,29 @@ class DynamoRun:
         return max(0, self.stage - 1) if self.dirk else self._par
 
     def _icb_launches(self):
-        """The rotating inner core's part of the radial loop (step_time.py:
-        1126-1138, 1181-1192; update_b.py:875-901): (br, btheta, bphi) on the ICB
-        grid (FP64 synthesis and inverse FFT of the ICB level), the Lorentz torque
-        and the grid of the rotation's induction, its spheroidal analysis into
-        VxBt at the ICB, and the advection of the inner field."""
+        """The inner core's part of the radial loop (step_time.py:1126-1138,
+        1181-1192; update_b.py:875-901).  Rotating core: (br, btheta, bphi) on the
+        ICB grid (FP64 synthesis and inverse FFT of the ICB level), the Lorentz
+        torque and the grid of the rotation's induction, its spheroidal analysis
+        into VxBt at the ICB.  Every conducting core: the inner explicit terms
+        (the advection of the inner field; zeros without rotation)."""
         r, f, N = self.res, self.f, self.N
-        icb = lambda t: t[:, N - 1:]
-        r.synth([[(icb(f["b"]), None, 1), (icb(f["db"]), None, 0), (icb(f["aj"]), None, 0)]],
-                1, N, self.F_icb, 1, vector=True, prec=0)
-        r.fft(False, self.F_icb, 1, self.G_icb, 3, 1)
         a = self._rot_args()
-        n = r.T * r.P
-        a.br, a.bp = ptr(self.G_icb), ptr(self.G_icb) + 8 * 2 * n
-        a.vt, a.vp = ptr(self.G2_icb), ptr(self.G2_icb) + 8 * n
-        a.gauss, a.sin2 = ptr(self.d_ic["gauss"]), ptr(self.d_ic["sin2"])
         k = self._ic_expl_slot()
         a.expl_slot = 6 + k
-        _lib.call("mh_ic_torque", _lib.ref(a), stream_handle())
-        r.fft(True, self.F2_icb, 1, self.G2_icb, 2, 1, comp_scale=r.d["comp_scale_2"])
-        r.analysis(self.F2_icb, 1, 1, [(0, 1)], [ptr(self.nl["VxBt"]) + 16 * (N - 1)], [None], N,
-                   vector=True, prec=0)
+        if self.rot_ic:
+            icb = lambda t: t[:, N - 1:]
+            r.synth([[(icb(f["b"]), None, 1), (icb(f["db"]), None, 0), (icb(f["aj"]), None, 0)]],
+                    1, N, self.F_icb, 1, vector=True, prec=0)
+            r.fft(False, self.F_icb, 1, self.G_icb, 3, 1)
+            n = r.T * r.P
+            a.br, a.bp = ptr(self.G_icb), ptr(self.G_icb) + 8 * 2 * n
+            a.vt, a.vp = ptr(self.G2_icb), ptr(self.G2_icb) + 8 * n
+            a.gauss, a.sin2 = ptr(self.d_ic["gauss"]), ptr(self.d_ic["sin2"])
+            _lib.call("mh_ic_torque", _lib.ref(a), stream_handle())
+            r.fft(True, self.F2_icb, 1, self.G2_icb, 2, 1, comp_scale=r.d["comp_scale_2"])
+            r.analysis(self.F2_icb, 1, 1, [(0, 1)], [ptr(self.nl["VxBt"]) + 16 * (N - 1)], [None],
+                       N, vector=True, prec=0)
         e = self.expl_ic[k]
         a.b_ic, a.aj_ic = ptr(self.fic["b_ic"]), ptr(self.fic["aj_ic"])
         a.e_b, a.e_j = ptr(e["b"]), ptr(e["j"])
@@ -1082,7 +1087,7 @@ class DynamoRun:
     def omega_ic(self) -> float:
         """Rotation rate of the inner core (0.0 without a rotating core; reads
         the device state: synchronises the run's stream)."""
-        return float(self.rot[0].item()) if self.rot_ic else 0.0
+        return float(self.rot[0].item()) if self.cond_ic else 0.0
 
     def _grid_separate(self):
         r, N, Nb, d = self.res, self.N, self.Nb, self.res.d
@@ -1236,10 +1241,9 @@ class DynamoRun:
 
     def _ic_rhs(self, wdev):
         """Coupled right-hand sides of the B and J systems (mh_ic_rhs): the inner
-        IMEX terms of the scheme (CNAB2 old + impl, and the two explicit slots of
-        a rotating core; BPR353 stage k the implicit slots 0..k-1, and the
-        explicit slots 0..k-2 of a rotating core, with the stage weights), the
-        outer ones from mh_rhs."""
+        IMEX terms of the scheme (CNAB2 old, impl and the two explicit slots;
+        BPR353 stage k the implicit slots 0..k-1 and the explicit slots 0..k-2,
+        with the stage weights), the outer ones from mh_rhs."""
         a = _lib.IcRhsArgs()
         for i, key in enumerate(("b", "j")):
             sy = a.sys[i]
@@ -1253,18 +1257,16 @@ class DynamoRun:
                     sy.t6 = ptr(I[2][key])
                 if k >= 4:
                     sy.t7 = ptr(I[3][key])
-                if self.rot_ic:
-                    E = self.expl_ic
-                    sy.expl0 = ptr(E[0][key])
-                    if k >= 2:
-                        sy.expl1 = ptr(E[1][key])
-                    if k >= 3:
-                        sy.t5 = ptr(E[2][key])
+                E = self.expl_ic
+                sy.expl0 = ptr(E[0][key])
+                if k >= 2:
+                    sy.expl1 = ptr(E[1][key])
+                if k >= 3:
+                    sy.t5 = ptr(E[2][key])
             else:
                 sy.impl = ptr(self.impl_ic[key])
-                if self.rot_ic:
-                    sy.expl0 = ptr(self.expl_ic[self._par][key])
-                    sy.expl1 = ptr(self.expl_ic[1 - self._par][key])
+                sy.expl0 = ptr(self.expl_ic[self._par][key])
+                sy.expl1 = ptr(self.expl_ic[1 - self._par][key])
         a.rb, a.rj = ptr(self.rhs["b"]), ptr(self.rhs["j"])
         a.out_b, a.out_j = ptr(self.rhs_c["b"]), ptr(self.rhs_c["j"])
         a.m, a.L, a.N, a.n_ic = ptr(self.res.d["m"]), self.L, self.N, self.p.n_r_ic_max

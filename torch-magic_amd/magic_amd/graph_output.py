@@ -55,10 +55,10 @@ def ic_radii(p, r_icb):
 
 def write_graph_file(path, time: float, run, fields=None):
     """Write the graph file of `run` (its current fields, or `fields`: a dict
-    of w, dw, z, s, p, b, db, aj arrays) at `time` to `path`."""
-    if getattr(run, "cond_ic", False):
-        raise NotImplementedError("graph files of a conducting inner core (inner-core field "
-                                  "slabs from b_ic, aj_ic) are not written")
+    of w, dw, z, s, p, b, db, aj arrays) at `time` to `path`.  The inner-core
+    slabs hold the potential field continued from the outer b at the ICB, also
+    for a conducting inner core, as the reference writes them
+    (``graph_output.py:174-242``)."""
     p, res = run.p, run.res
     f = dict(run.f)
     if fields:
@@ -78,7 +78,7 @@ def write_graph_file(path, time: float, run, fields=None):
         be_f(v)
     for v in (N, T, p.n_phi_tot, p.minc, n_ic):   # the full circle's count in the header
         be_i(v)
-    for v in (p.l_heat, False, False, p.l_mag, True, False):
+    for v in (p.l_heat, False, False, p.l_mag, True, p.l_cond_ic):
         be_i(1 if v else 0)
     be_arr(_theta_geo(p))
     be_arr(rad["r"])

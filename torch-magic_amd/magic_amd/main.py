@@ -176,14 +176,11 @@ def run(cfg=None, on_log=None):
                 r.save_checkpoint(cp, n, sim_time)
                 print(f"  Saved checkpoint: {cp}")
         if n_steps > start_step:
-            if getattr(r, "cond_ic", False):
-                print("  Note: no G_1 / checkpoint_end files for a conducting inner core")
-            else:
-                if n_graphs >= 1:
-                    from .graph_output import write_graph_file
-                    write_graph_file(path("G_1"), sim_time, r)
-                from .checkpoint_io import write_checkpoint_fortran
-                write_checkpoint_fortran(path("checkpoint_end"), r, sim_time, n_steps + 1)
+            if n_graphs >= 1:
+                from .graph_output import write_graph_file
+                write_graph_file(path("G_1"), sim_time, r)
+            from .checkpoint_io import write_checkpoint_fortran
+            write_checkpoint_fortran(path("checkpoint_end"), r, sim_time, n_steps + 1)
             done = n_steps - start_step
             elapsed = _time.time() - t0
             print(f"\nDone: {done} steps in {elapsed:.1f}s ({elapsed / done * 1000:.1f}ms/step)")
