@@ -1,4 +1,5 @@
-"""Boussinesq Chebyshev CNAB2 dynamo step (oracle restatement, FP64 numpy).
+"""Chebyshev dynamo step, Boussinesq or anelastic, CNAB2 or BPR353 (oracle
+restatement, FP64 numpy; test infrastructure only).
 
 One :class:`OracleRun` holds one configuration's state (the reference keeps
 it in module globals: ``fields.py:14-67``, ``dt_fields.py:55-123``).
@@ -17,6 +18,10 @@ it in module globals: ``fields.py:14-67``, ``dt_fields.py:55-123``).
 * ``one_step`` — ``_one_step_cnab2`` (``step_time.py:1602-1637``) with
   ``dt_courant`` (``courant.py:130-155``) and ``CNAB2.set_weights``
   (``time_scheme.py:56-73``).
+* anelastic runs (strat > 0) — ``radial_loop_anel`` (``step_time.py:1215-1475``),
+  the anelastic operators, implicit terms, conduction state and l=0 pressure
+  condition, 1/rho0-weighted kinetic energies and Courant limits, and the
+  angular-momentum corrections (``oracle.anelastic``).
 * energies — ``get_e_kin_full``/``get_e_mag_oc_full`` (``output.py:111-193``);
   ``energy_components`` adds the masked parts (pinned by single-mode fields
   whose energy the (l, m) parity places in known columns); ``get_dlm``
@@ -38,15 +43,20 @@ import math
 import numpy as np
 
 from .params import Params
-from . import horizontal, radial, matrices
+from . import anelastic, horizontal, radial, matrices
 from .sht import SHT
 
 
 class OracleRun:
     def __init__(self, cfg: dict):
         p = self.prm = Params(dict(cfg))
-        if p.l_anel or p.l_finite_diff or p.raxi != 0.0:
-            raise NotImplementedError("oracle step: Boussinesq Chebyshev only")
+        if p.l_finite_diff or p.raxi != 0.0:
+            raise NotImplementedError("oracle step: Chebyshev radial scheme, no composition")
+        if p.l_anel and p.sigma_ratio > 0.0:
+            raise NotImplementedError("oracle step: anelastic runs with an insulating inner core")
+        # anelastic reference state (strat > 0): step_time.py:1215-1475 and the
+        # anelastic branches of update_s/z/wp.py (oracle.anelastic)
+        self.anel = p.l_anel
         # conducting inner core (sigma_ratio > 0): IC fields on the even Chebyshev
         # grid, coupled OC+IC B/J solves (update_b.py:301-513,634-801)
         self.cond_ic = p.l_cond_ic and p.l_mag
@@ -102,6 +112,9 @@ class OracleRun:
         self.inv = None
         self.m0 = self.maps["lm2m"] == 0
         self.lm_l0 = int(self.maps["lm2"][0, 0])
+        self.prof = anelastic.background(p, self.rad)
+        self.am = (anelastic.am_constants(p, self.rad, self.prof, self.q)
+                   if p.l_correct_AMz or p.l_correct_AMe else None)
 
     # ------------------------------------------------------------ init
     def initialize_fields(self):
@@ -111,7 +124,8 @@ class OracleRun:
                      "b", "db", "ddb", "aj", "dj", "ddj"):
             getattr(self, name)[:] = 0.0
         if p.l_heat:
-            s0, p0 = matrices.ps_cond(p, rad, self.mats)
+            s0, p0 = (anelastic.ps_cond(p, rad, self.mats, self.prof) if self.anel
+                      else matrices.ps_cond(p, rad, self.mats))
             self.s[0] = s0
             self.p[0] = p0
             r, r_cmb, r_icb = rad["r"], rad["r_cmb"], rad["r_icb"]
@@ -169,11 +183,14 @@ class OracleRun:
         ds, d2s = self._dr(self.s, self.D1), self._dr(self.s, self.D2)
         self.ds[:] = ds
         self.td["s"]["old"][:] = self.s
-        self.td["s"]["impl"][:] = p.opr * (d2s + 2.0 * or1 * ds - dLh * or2 * self.s)
         dz, d2z = self._dr(self.z, self.D1), self._dr(self.z, self.D2)
         self.dz[:] = dz
         self.td["z"]["old"][:] = dLh * or2 * self.z
-        self.td["z"]["impl"][:] = dLh * or2 * (d2z - dLh * or2 * self.z)
+        if self.anel:
+            self._s_z_impl_anel(self.s, ds, d2s, self.z, dz, d2z)
+        else:
+            self.td["s"]["impl"][:] = p.opr * (d2s + 2.0 * or1 * ds - dLh * or2 * self.s)
+            self.td["z"]["impl"][:] = dLh * or2 * (d2z - dLh * or2 * self.z)
         dw, ddw, dddw = (self._dr(self.w, D) for D in (self.D1, self.D2, self.D3))
         self.dw[:] = dw; self.ddw[:] = ddw
         dp = self._dr(self.p, self.D1)
@@ -245,7 +262,44 @@ class OracleRun:
         impl[1:, Nic - 1] = (p.opm * p.O_sr * dLh[1:, 0] * or2_icb * (1.0 + 2.0 * lp1[1:, 0])
                              * ddf[1:, Nic - 1])
 
-    def _w_p_impl(self, w, dw, ddw, dddw, pf, dp, s):
+    def _s_z_impl_anel(self, s, ds, d2s, z, dz, d2z):
+        """Anelastic implicit terms of s and z (update_s.py:265-268,
+        update_z.py:495-499; constant kappa and visc)."""
+        p, pr = self.prm, self.prof
+        dLh = self.lmarr["dLh"][:, None]
+        c = lambda k: pr[k][None]
+        or1, or2, beta, dbeta, dLv = c("or1"), c("or2"), c("beta"), c("dbeta"), c("dLvisc")
+        d1s = c("beta") + c("dLtemp0") + 2.0 * or1 + c("dLkappa")
+        self.td["s"]["impl"][:] = p.opr * c("kappa") * (d2s + d1s * ds - dLh * or2 * s)
+        z0 = dLv * beta + 2.0 * dLv * or1 + dbeta + 2.0 * beta * or1
+        self.td["z"]["impl"][:] = dLh * or2 * c("visc") * (d2z + (dLv - beta) * dz
+                                                           - (z0 + dLh * or2) * z)
+
+    def _w_p_impl_anel(self, w, dw, ddw, dddw, pf, dp, s, zero_edges):
+        """Anelastic implicit terms of w and p (update_wp.py:500-531); the set-up
+        zeroes l = 0 and the boundary levels (step_time.py:528-535), the lm loop
+        does not."""
+        p, pr = self.prm, self.prof
+        dLh = self.lmarr["dLh"][:, None]
+        c = lambda k: pr[k][None]
+        or1, or2, beta, dbeta, dLv, visc = (c("or1"), c("or2"), c("beta"), c("dbeta"),
+                                            c("dLvisc"), c("visc"))
+        dif = dLh * or2 * visc * (ddw + (2.0 * dLv - beta / 3.0) * dw
+                                  - (dLh * or2 + 4.0 / 3.0 * (dbeta + dLv * beta
+                                                              + (3.0 * dLv + beta) * or1)) * w)
+        wi = self.td["w"]["impl"]; pi_ = self.td["p"]["impl"]
+        wi[:] = -dp + beta * pf + dif + p.BuoFac * c("rho0") * c("rgrav") * s
+        pi_[:] = dLh * or2 * pf + visc * dLh * or2 * (
+            -dddw + (beta - dLv) * ddw
+            + (dLh * or2 + dLv * beta + dbeta + 2.0 * (dLv + beta) * or1) * dw
+            - dLh * or2 * (2.0 * or1 + 2.0 / 3.0 * beta + dLv) * w)
+        if zero_edges:
+            wi[self.lm_l0] = 0; pi_[self.lm_l0] = 0
+            wi[:, 0] = 0; wi[:, -1] = 0; pi_[:, 0] = 0; pi_[:, -1] = 0
+
+    def _w_p_impl(self, w, dw, ddw, dddw, pf, dp, s, zero_edges=True):
+        if self.anel:
+            return self._w_p_impl_anel(w, dw, ddw, dddw, pf, dp, s, zero_edges)
         p, rad = self.prm, self.rad
         dLh = self.lmarr["dLh"][:, None]
         or2, or3 = rad["or2"][None], rad["or3"][None]
@@ -266,6 +320,8 @@ class OracleRun:
     # ------------------------------------------------------------ r-loop
     def radial_loop(self):
         """``step_time.py:931-1212``; returns (dtrkc_min, dthkc_min)."""
+        if self.anel:
+            return self._radial_loop_anel()
         p, rad, sht = self.prm, self.rad, self.sht
         N = p.n_r_max
         dLh = self.lmarr["dLh"][:, None]
@@ -327,6 +383,81 @@ class OracleRun:
         dtrkc, dthkc = courant_check(p, rad, self.grid, vr, vt, vp, br, bt, bp)
         return float(dtrkc.min()), float(dthkc.min())
 
+    def _radial_loop_anel(self):
+        """``radial_loop_anel`` (``step_time.py:1215-1475``): the magnetic field
+        and its curl on every level, the velocity, its radial derivative, the
+        radial vorticity, the gradient of the radial velocity and the azimuthal
+        derivatives of the horizontal velocity on the bulk levels (zero on the
+        boundaries), the anelastic products, forward transforms of the bulk, the
+        explicit terms with the viscous heating in the entropy equation, and the
+        Courant limits."""
+        p, rad, sht, pr = self.prm, self.rad, self.sht, self.prof
+        N = p.n_r_max
+        bulk = slice(1, N - 1)
+        dLh = self.lmarr["dLh"][:, None]
+        dPhi = self.lmarr["dPhi"][:, None]
+        or2 = rad["or2"][None]
+        shape = (N, p.n_theta_max, p.n_phi_max)
+        sc = sht.scal_to_spat(self.s)
+        if p.l_mag:
+            ar, at, ap = sht.torpol_to_spat(np.concatenate([dLh * self.b, dLh * self.aj], axis=1),
+                                            np.concatenate([self.db, self.dj], axis=1),
+                                            np.concatenate([self.aj, or2 * dLh * self.b - self.ddb],
+                                                           axis=1))
+            br, bt, bp, cbr, cbt, cbp = ar[:N], at[:N], ap[:N], ar[N:], at[N:], ap[N:]
+        else:
+            br = bt = bp = cbr = cbt = cbp = np.zeros(shape)
+
+        def onbulk(*grids):
+            out = []
+            for g in grids:
+                f = np.zeros(shape)
+                f[bulk] = g
+                out.append(f)
+            return out
+
+        zb = np.zeros_like(self.w[:, bulk])
+        vr, vt, vp = onbulk(*sht.torpol_to_spat(dLh * self.w[:, bulk], self.dw[:, bulk],
+                                                self.z[:, bulk]))
+        dvrdr, dvtdr, dvpdr = onbulk(*sht.torpol_to_spat(dLh * self.dw[:, bulk],
+                                                         self.ddw[:, bulk], self.dz[:, bulk]))
+        (cvr,) = onbulk(sht.scal_to_spat(dLh * self.z[:, bulk]))
+        _, gt, gp = sht.torpol_to_spat(zb, dLh * self.w[:, bulk], zb)
+        dvrdt, dvrdp = onbulk(gt, gp)
+        ost2 = self.grid["O_sin_theta_E2_grid"][None, :, None]
+        _, ht, hp = sht.torpol_to_spat(zb, dPhi * self.dw[:, bulk], dPhi * self.z[:, bulk])
+        dvtdp, dvpdp = onbulk(ht * ost2, hp * ost2)
+        nl = anelastic.get_nl(p, pr, self.grid, vr, vt, vp, dvrdr, dvtdr, dvpdr, dvrdt, dvrdp,
+                              dvtdp, dvpdp, cvr, sc, br, bt, bp, cbr, cbt, cbp)
+        Advr, Advt, Advp, VSr, VSt, VSp, VxBr, VxBt, VxBp, heat = nl
+        Nb, L = N - 2, p.lm_max
+        scal_out = sht.scal_to_SH(np.concatenate([Advr[bulk], VSr[bulk], VxBr[bulk],
+                                                  heat[bulk]], axis=0))
+        S_out, T_out = sht.spat_to_sphertor(np.concatenate([Advt[bulk], VSt[bulk], VxBt[bulk]]),
+                                            np.concatenate([Advp[bulk], VSp[bulk], VxBp[bulk]]))
+        z = lambda: np.zeros((L, N), np.complex128)
+        AdvrLM, AdvtLM, AdvpLM, dVSrLM, VStLM, VxBrLM, VxBtLM, VxBpLM, heatLM = (
+            z() for _ in range(9))
+        AdvrLM[:, bulk] = scal_out[:, :Nb]
+        dVSrLM[:, bulk] = scal_out[:, Nb:2 * Nb]
+        VxBrLM[:, bulk] = scal_out[:, 2 * Nb:3 * Nb]
+        heatLM[:, bulk] = scal_out[:, 3 * Nb:]
+        AdvtLM[:, bulk] = S_out[:, :Nb]
+        AdvpLM[:, bulk] = T_out[:, :Nb]
+        VStLM[:, bulk] = S_out[:, Nb:2 * Nb]
+        VxBtLM[:, bulk] = S_out[:, 2 * Nb:]
+        VxBpLM[:, bulk] = T_out[:, 2 * Nb:]
+        self.last_grid = dict(vr=vr, vt=vt, vp=vp, br=br, bt=bt, bp=bp, sc=sc, heat=heat)
+        self.last_spec = dict(AdvrLM=AdvrLM, AdvtLM=AdvtLM, AdvpLM=AdvpLM, dVSrLM=dVSrLM,
+                              VStLM=VStLM, VxBrLM=VxBrLM, VxBtLM=VxBtLM, VxBpLM=VxBpLM,
+                              heatLM=heatLM)
+        td = get_td(self, AdvrLM, AdvtLM, AdvpLM, dVSrLM, VStLM, VxBrLM, VxBtLM, VxBpLM,
+                    heatLM=heatLM)
+        for k, v in td.items():
+            self._expl_slot(k)[:] = v
+        dtrkc, dthkc = courant_check(p, rad, self.grid, vr, vt, vp, br, bt, bp, orho1=pr["orho1"])
+        return float(dtrkc.min()), float(dthkc.min())
+
     # ------------------------------------------------------------ lm-loop
     def _expl_slot(self, k):
         """Explicit-term array the radial loop writes: CNAB2's newest slot, or
@@ -383,8 +514,17 @@ class OracleRun:
         """``step_time.py:1530-1548`` (Boussinesq Chebyshev insulating)."""
         w0 = self.wimp_lin[0]
         p, rad, mats = self.prm, self.rad, self.mats
-        self.inv = dict(s=matrices.build_s(p, rad, mats, w0), z=matrices.build_z(p, rad, mats, w0),
-                        wp=matrices.build_wp(p, rad, mats, w0), p0=matrices.build_p0(p, rad, mats))
+        if self.anel:
+            pr = self.prof
+            self.inv = dict(s=anelastic.build_s(p, rad, mats, pr, w0),
+                            z=anelastic.build_z(p, rad, mats, pr, w0),
+                            wp=anelastic.build_wp(p, rad, mats, pr, w0),
+                            p0=anelastic.build_p0(p, rad, mats, pr))
+        else:
+            self.inv = dict(s=matrices.build_s(p, rad, mats, w0),
+                            z=matrices.build_z(p, rad, mats, w0),
+                            wp=matrices.build_wp(p, rad, mats, w0),
+                            p0=matrices.build_p0(p, rad, mats))
         if self.rot_ic:
             self.inv["z10"] = matrices.build_z10(p, rad, mats, w0, self.c_dt_z10_ic)
         if p.l_mag and self.cond_ic:
@@ -403,7 +543,7 @@ class OracleRun:
         nc = p.n_cheb_max
         dLh = self.lmarr["dLh"][:, None]
         or1, or2, or3 = rad["or1"][None], rad["or2"][None], rad["or3"][None]
-        rgrav = rad["rgrav"]
+        rgrav = self.prof["rho0"] * self.prof["rgrav"] if self.anel else rad["rgrav"]
         rhs_s, rhs_z, rhs_w, rhs_p = (self._rhs(k) for k in ("s", "z", "w", "p"))
         rhs_s[:, 0] = 0.0
         rhs_s[:, N - 1] = 0.0
@@ -427,6 +567,9 @@ class OracleRun:
         p0_rhs = np.zeros(N)
         l0 = self.lm_l0
         p0_rhs[1:] = p.BuoFac * rgrav[1:] * self.s[l0, 1:].real + self._expl_slot("w")[l0, 1:].real
+        if self.anel and anelastic.p0_integral_bc(p, self.prof):   # update_wp.py:431-436
+            p0_rhs[0] = float(anelastic.p0_rhs_weights(p, rad, self.prof, self.q)
+                              @ self.s[l0].real)
         sol = self._solve(self.inv["wp"], rc)
         sol[self.m0] = sol[self.m0].real
         w_cheb = sol[:, :N].copy()
@@ -443,6 +586,10 @@ class OracleRun:
         D1, D2, D3 = self.D1, self.D2, self.D3
         self.ds[:] = self._dr(self.s, D1); d2s = self._dr(self.s, D2)
         self.dz[:] = self._dr(self.z, D1); d2z = self._dr(self.z, D2)
+        if self.am is not None:   # update_z.py:450-475
+            l1m1 = int(self.maps["lm2"][1, 1]) if p.minc == 1 else -1
+            anelastic.am_correct(p, self.am, self.z, self.dz, d2z,
+                                 int(self.maps["lm2"][1, 0]), l1m1)
         self.dw[:] = self._dr(self.w, D1); self.ddw[:] = self._dr(self.w, D2)
         dddw = self._dr(self.w, D3)
         self.dp[:] = self._dr(self.p, D1)
@@ -464,9 +611,13 @@ class OracleRun:
             self.td["z"]["old"][:] = dLh * or2 * self.z
             self.td["w"]["old"][:] = dLh * or2 * self.w
             self.td["p"]["old"][:] = -dLh * or2 * self.dw
-        self.td["s"]["impl"][:] = p.opr * (d2s + 2.0 * or1 * self.ds - dLh * or2 * self.s)
-        self.td["z"]["impl"][:] = dLh * or2 * d2z - (dLh * or2) ** 2 * self.z
-        self._w_p_impl(self.w, self.dw, self.ddw, dddw, self.p, self.dp, self.s)
+        if self.anel:
+            self._s_z_impl_anel(self.s, self.ds, d2s, self.z, self.dz, d2z)
+        else:
+            self.td["s"]["impl"][:] = p.opr * (d2s + 2.0 * or1 * self.ds - dLh * or2 * self.s)
+            self.td["z"]["impl"][:] = dLh * or2 * d2z - (dLh * or2) ** 2 * self.z
+        self._w_p_impl(self.w, self.dw, self.ddw, dddw, self.p, self.dp, self.s,
+                       zero_edges=not self.anel)
         if p.l_mag:
             self._update_b()
 
@@ -578,8 +729,9 @@ class OracleRun:
         dLh = self.lmarr["dLh"][:, None]
         or2 = self.rad["or2"][None]
         q = self.q
-        ekp = 0.5 * dLh * (dLh * or2 * self._cc2real(self.w) + self._cc2real(self.dw))
-        ekt = 0.5 * dLh * self._cc2real(self.z)
+        ov = self.prof["orho1"][None]
+        ekp = 0.5 * ov * dLh * (dLh * or2 * self._cc2real(self.w) + self._cc2real(self.dw))
+        ekt = 0.5 * ov * dLh * self._cc2real(self.z)
         f = 0.5 * self.prm.LFfac
         emp = f * dLh * (dLh * or2 * self._cc2real(self.b) + self._cc2real(self.db))
         emt = f * dLh * self._cc2real(self.aj)
@@ -597,8 +749,9 @@ class OracleRun:
         m0 = m == 0
         lpm_even = (l + m) % 2 == 0
         f = 0.5 * self.prm.LFfac
-        ekp = (0.5 * dLh * (dLh * or2 * self._cc2real(self.w) + self._cc2real(self.dw))) @ q
-        ekt = (0.5 * dLh * self._cc2real(self.z)) @ q
+        ov = self.prof["orho1"][None]
+        ekp = (0.5 * ov * dLh * (dLh * or2 * self._cc2real(self.w) + self._cc2real(self.dw))) @ q
+        ekt = (0.5 * ov * dLh * self._cc2real(self.z)) @ q
         emp = (f * dLh * (dLh * or2 * self._cc2real(self.b) + self._cc2real(self.db))) @ q
         emt = (f * dLh * self._cc2real(self.aj)) @ q
         s = lambda e, mask=None: float(e.sum() if mask is None else e[mask].sum())
@@ -622,6 +775,9 @@ class OracleRun:
         a, da, c = (self.w, self.dw, self.z) if switch == "V" else (self.b, self.db, self.aj)
         e_p = dLh * (dLh * or2 * self._cc2real(a) + self._cc2real(da))
         e_t = dLh * self._cc2real(c)
+        if switch == "V":   # output.py:764-767
+            e_p = self.prof["orho1"][None] * e_p
+            e_t = self.prof["orho1"][None] * e_t
         l_first = 1 if switch == "V" else 2
         e = e_p + e_t
         l = self.maps["lm2l"]; m = self.maps["lm2m"]
@@ -678,8 +834,8 @@ class OracleRun:
         """Per-l and per-m kinetic/magnetic energy spectra (``output.py:750-861`` aggregation)."""
         dLh = self.lmarr["dLh"][:, None]
         or2 = self.rad["or2"][None]
-        ek = 0.5 * dLh * (dLh * or2 * self._cc2real(self.w) + self._cc2real(self.dw)
-                          + self._cc2real(self.z))
+        ek = 0.5 * self.prof["orho1"][None] * dLh * (
+            dLh * or2 * self._cc2real(self.w) + self._cc2real(self.dw) + self._cc2real(self.z))
         em = 0.5 * self.prm.LFfac * dLh * (dLh * or2 * self._cc2real(self.b) + self._cc2real(self.db)
                                          + self._cc2real(self.aj))
         l = self.maps["lm2l"]; m = self.maps["lm2m"]
@@ -730,8 +886,11 @@ def get_nl(p, rad, grid, vr, vt, vp, cvr, cvt, cvp, sc, br, bt, bp, cbr, cbt, cb
     return Advr, Advt, Advp, VSr, VSt, VSp, VxBr, VxBt, VxBp
 
 
-def get_td(run: OracleRun, AdvrLM, AdvtLM, AdvpLM, dVSrLM, VStLM, VxBrLM, VxBtLM, VxBpLM):
-    """Explicit terms (``get_td.py:58-308``) + finish_exp of S and B."""
+def get_td(run: OracleRun, AdvrLM, AdvtLM, AdvpLM, dVSrLM, VStLM, VxBrLM, VxBtLM, VxBpLM,
+           heatLM=None):
+    """Explicit terms (``get_td.py:58-308``) + finish_exp of S and B; with
+    heatLM (anelastic) the heating joins the entropy term and the whole term is
+    divided by rho0 (``step_time.py:1436-1440``, ``update_s.py:191-210``)."""
     p, rad, A = run.prm, run.rad, run.lmarr
     maps = run.maps
     CorFac = p.CorFac
@@ -763,23 +922,35 @@ def get_td(run: OracleRun, AdvrLM, AdvtLM, AdvpLM, dVSrLM, VStLM, VxBrLM, VxBtLM
         f[:, -1] = 0.0
     dVxBh[0, 0] = 0.0
     dVxBh[0, -1] = 0.0
-    dsdt = dsdt - or2 * (dVSr @ run.D1.T)
+    if heatLM is not None:
+        dsdt = run.prof["orho1"][None] * (dsdt + heatLM - or2 * (dVSr @ run.D1.T))
+    else:
+        dsdt = dsdt - or2 * (dVSr @ run.D1.T)
     djdt = djdt + or2 * (dVxBh @ run.D1.T)
     return dict(w=dwdt, z=dzdt, p=dpdt, s=dsdt, b=dbdt, j=djdt)
 
 
-def courant_check(p, rad, grid, vr, vt, vp, br, bt, bp):
-    """Per-level Courant limits (``courant.py:25-127``, damped Alfvén branch)."""
+def courant_check(p, rad, grid, vr, vt, vp, br, bt, bp, orho1=None):
+    """Per-level Courant limits (``courant.py:25-127``, damped Alfvén branch);
+    orho1: 1/rho0 of an anelastic run (velocities weighted by 1/rho0^2, the
+    Alfvén speeds by 1/rho0)."""
     or4 = rad["or4"][:, None, None]
     or2 = rad["or2"][:, None, None]
     ost2 = grid["O_sin_theta_E2_grid"][None, :, None]
     cf2 = p.cnab2_courfac ** 2
     vflr2 = vr * vr
     vflh2 = (vt * vt + vp * vp) * ost2
+    if orho1 is not None:
+        o1 = orho1[:, None, None]
+        vflr2 = o1 * o1 * vflr2
+        vflh2 = vflh2 * (o1 * o1)
     if p.l_mag:
         af2 = p.cnab2_alffac ** 2
         valr = br * br * p.LFfac
         valh2 = (bt * bt + bp * bp) * p.LFfac * ost2
+        if orho1 is not None:
+            valr = valr * o1
+            valh2 = valh2 * o1
         if p.l_cour_alf_damp:
             valri2 = ((0.5 * (1.0 + p.opm)) ** 2 / rad["delxr2"])[:, None, None]
             valhi2 = ((0.5 * (1.0 + p.opm)) ** 2 / rad["delxh2"])[:, None, None]

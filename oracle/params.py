@@ -38,6 +38,7 @@ _DEFAULTS = dict(
     radial_scheme="chebyshev", fd_order=2, fd_order_bound=2,
     fd_stretch=0.3, fd_ratio=0.1,
     l_cour_alf_damp=True, sigma_ratio=0.0, nRotIC=0,
+    l_correct_AMz=False, l_correct_AMe=False,
 )
 
 
@@ -78,6 +79,8 @@ class Params:
         self.time_scheme = str(c["time_scheme"])
         self.radial_scheme = str(c["radial_scheme"]).upper()
         self.l_cour_alf_damp = bool(c["l_cour_alf_damp"])
+        self.l_correct_AMz = bool(c["l_correct_AMz"])
+        self.l_correct_AMe = bool(c["l_correct_AMe"])
         self.l_mag = self.mode not in (1, 7)
         self.l_heat = self.mode not in (7,)
         self.l_anel = self.strat > 0.0

@@ -130,6 +130,24 @@ CASES = {
                                sigma_ratio=1.0, kbotb=3, nRotIC=1, time_scheme="BPR353"),
                       dt=2e-4, steps=30, noise=dict(seed=31, amp=1e-2), field_steps=[1, 30],
                       expl_step1=False),
+    # anelastic (polytropic) step: the hydrodynamic benchmark's physics at small
+    # size (stress-free walls, angular-momentum corrections, (19, 19) seed), a
+    # magnetic anelastic run (gravity ~ r, no-slip), and BPR353 with minc = 2
+    "anel21": dict(cfg=dict(l_max=21, n_r_max=33, n_cheb_max=31, mode=1, strat=5.0,
+                            polind=2.0, g0=0.0, g1=0.0, g2=1.0, ra=1.48638035e5, ek=1e-3,
+                            pr=1.0, ktopv=1, kbotv=1, l_correct_AMz=True, l_correct_AMe=True,
+                            init_s1=1919, amp_s1=0.01, alpha=0.6, dtmax=1e-4),
+                   dt=1e-4, steps=30, noise=dict(seed=43, amp=1e-3), field_steps=[1, 30],
+                   expl_step1=True, diag_steps=[30]),
+    "anelmag16": dict(cfg=dict(l_max=16, n_r_max=33, strat=1.0, polind=2.0, g1=1.0, ra=1e5,
+                               ek=1e-3, pr=1.0, prmag=5.0, dtmax=1e-4),
+                      dt=1e-4, steps=20, noise=dict(seed=47, amp=1e-3), field_steps=[1, 20],
+                      expl_step1=True, diag_steps=[20]),
+    "bpranel2": dict(cfg=dict(l_max=16, n_r_max=33, n_cheb_max=31, minc=2, mode=1, strat=3.0,
+                              polind=2.0, g2=1.0, ra=1e5, ek=1e-3, pr=1.0, ktopv=1, kbotv=1,
+                              l_correct_AMz=True, dtmax=1e-4, time_scheme="BPR353"),
+                     dt=1e-4, steps=20, noise=dict(seed=53, amp=1e-3), field_steps=[1, 20],
+                     expl_step1=False),
 }
 
 FD_CASES = {

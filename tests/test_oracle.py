@@ -26,8 +26,16 @@ GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
 # ------------------------------------------------------------ golden trajectory
+# l=0 pressure of an anelastic run: its system carries the integral condition
+# and has a condition number near 3e9 (strat = 5), and p(l=0) ~ 1e8 from the
+# hydrostatic balance; numpy's inverse and the reference's (torch/MKL) differ by
+# 3e-10 there (torch's own inverse reproduces the fixture to 2e-14)
+P_TOL_ANEL = 1e-9
+
+
 @pytest.mark.parametrize("name", ["bench16", "c1", "dtchange", "bpr16", "bprdt", "minc2",
-                                  "bprminc4", "condic", "bprcondic", "rotic", "bprrotic4"])
+                                  "bprminc4", "condic", "bprcondic", "rotic", "bprrotic4",
+                                  "anel21", "anelmag16", "bpranel2"])
 def test_oracle_matches_reference_trajectory(name):
     """Energies every step and fields at the saved steps, from the reference's
     one_step with the returned dt passed to the next step; the dt sequence
@@ -39,7 +47,11 @@ def test_oracle_matches_reference_trajectory(name):
     2 pi / minc sector) with CNAB2 and with BPR353.  condic and bprcondic: a
     conducting inner core (coupled outer/inner B and J solves), whose fields
     b_ic and aj_ic are matched too.  rotic and bprrotic4: the core rotating,
-    with its rotation rate omega_ic matched every step."""
+    with its rotation rate omega_ic matched every step.  anel21, anelmag16 and
+    bpranel2: the anelastic (polytropic) step: the hydrodynamic benchmark's
+    physics (strat 5, stress-free walls, angular-momentum corrections), a
+    magnetic run (strat 1, gravity ~ r) and BPR353 with minc = 2; the pressure
+    within P_TOL_ANEL."""
     case = json.load(open(os.path.join(GOLD, f"traj_{name}.json")))
     g = np.load(os.path.join(GOLD, f"traj_{name}.npz"))
     run = OracleRun(case["cfg"])
@@ -60,7 +72,8 @@ def test_oracle_matches_reference_trajectory(name):
             keys = ("s", "z", "w", "p", "b", "aj") + (("b_ic", "aj_ic") if run.cond_ic else ())
             for k in keys:
                 ref = g[f"{k}_step{n}"]
-                assert np.abs(getattr(run, k) - ref).max() <= 1e-11 * np.abs(ref).max(), (n, k)
+                tol = P_TOL_ANEL if (k == "p" and run.anel) else 1e-11
+                assert np.abs(getattr(run, k) - ref).max() <= tol * np.abs(ref).max(), (n, k)
     if name in ("dtchange", "bprdt"):   # the fixture does take the Courant branch
         assert len(set(g["dt"].tolist())) >= {"dtchange": 5, "bprdt": 4}[name]
 

@@ -120,7 +120,8 @@ def _fixture_state(name, step):
 
 
 @pytest.mark.parametrize("name,step", [("c1", 100), ("dtchange", 30), ("minc2", 20),
-                                       ("bprminc4", 20), ("condic", 20)])
+                                       ("bprminc4", 20), ("condic", 20), ("anel21", 30),
+                                       ("anelmag16", 20)])
 def test_energy_components_match_reference_fixture(name, step):
     """All 8 e_kin and 12 e_mag_oc columns of the reference's get_e_kin_full /
     get_e_mag_oc_full on its own fields."""
@@ -131,7 +132,8 @@ def test_energy_components_match_reference_fixture(name, step):
 
 
 @pytest.mark.parametrize("name,step", [("c1", 100), ("dtchange", 30), ("minc2", 20),
-                                       ("bprminc4", 20), ("condic", 20)])
+                                       ("bprminc4", 20), ("condic", 20), ("anel21", 30),
+                                       ("anelmag16", 20)])
 def test_get_dlm_matches_reference_fixture(name, step):
     """Length scales of the reference's get_dlm (output.py:750-861) for the
     flow and the field: the four scalars and the three radial profiles."""
@@ -145,7 +147,7 @@ def test_get_dlm_matches_reference_fixture(name, step):
 
 
 @pytest.mark.parametrize("name,step", [("c1", 100), ("dtchange", 30), ("minc2", 20),
-                                       ("bprminc4", 20), ("condic", 20)])
+                                       ("bprminc4", 20), ("condic", 20), ("anelmag16", 20)])
 def test_spectra_against_reference_length_scales_and_energies(name, step):
     """The per-degree and per-order spectra: each sums to the reference's
     energy (Parseval over l and over m), and their first moments give the
