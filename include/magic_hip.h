@@ -373,6 +373,109 @@ int mh_ic_advect(const MhIcRotArgs* args, void* stream);
 int mh_ic_omega_rhs(const MhIcRotArgs* args, void* stream);
 int mh_ic_omega_update(const MhIcRotArgs* args, void* stream);
 
+/* ------------------------------------------------------------ anelastic step */
+typedef struct {
+  /* radial profiles of the reference state, (N,) each (radial_functions.py:71-167) */
+  const double *beta, *dbeta, *rho0, *orho1, *otemp1, *dLtemp0;
+  const double *visc, *dLvisc, *kappa, *dLkappa;
+  const double* rgrav_rho; /* rho0 rgrav */
+  double ViscHeatFac, OhmLossFac;
+} MhAnelProf;
+
+typedef struct {
+  const double* Gs;     /* [b][t][phi]: s on levels 0..N-1, then l(l+1) z synthesised (cvr) */
+  const double* Gv;     /* [c][g*N + b][t][phi]: g = 0 u, 1 du/dr, 2 (theta, phi) gradient
+                           of the radial velocity; c = r, theta, phi as synthesised */
+  long Gv_c_stride;
+  const double* Gb;     /* [c][g*N + b][t][phi]: g = 0 B, 1 curl B (NULL: no field) */
+  long Gb_c_stride;
+  const double* Gd;     /* [c][b][t][phi]: azimuthal derivatives of u_theta, u_phi (raw) */
+  long Gd_c_stride;
+  double* Go;           /* [c][b-1][t][phi], bulk levels: 0 Advr 1 VSr 2 VxBr 3 heat 4 Advt
+                           5 Advp 6 VSt 7 VSp 8 VxBt 9 VxBp */
+  long Go_c_stride;
+  int N, T, P, l_mag;
+  const double *or1, *or2, *or4;
+  const double *ost2, *cosn2; /* (T,) 1/sin^2 and cos/sin^2 of each grid row */
+  const double *valri2, *valhi2;
+  MhAnelProf prof;
+  double LFfac, cf2, af2;
+  int alf_damp, pad;
+  unsigned long long* cfl; /* 2N: per-level Courant maxima (bit patterns of doubles) */
+} MhNlAnelArgs;
+
+/* Grid products of the anelastic step and per-level Courant maxima:
+ * get_nl_anel (get_nl_anel.py:41-136: non-curl advection, Lorentz force,
+ * v x B / rho0, v s, viscous and Ohmic heating) and courant_check
+ * (courant.py:25-127, velocities weighted by 1/rho0^2 and Alfven speeds by
+ * 1/rho0); the velocity and its derivatives are zero on the boundary levels
+ * (step_time.py:1264-1268). */
+int mh_nl_anel(const MhNlAnelArgs* args, void* stream);
+
+typedef struct {
+  const double* F;   /* Fourier buffer F[c][m][t][b] (complex as re, im) */
+  long Fc_stride, Fm_stride;
+  int FB;            /* columns of F */
+  int c0;            /* first component read: out[k] = i m F[c0 + k] for k = 0, 1 */
+  double* out;       /* F[k][m][t][b], k = 0, 1, with out's own strides */
+  long Oc_stride, Om_stride;
+  int OB;            /* columns of out */
+  int B, n_m, T, minc, pad;
+} MhDphiArgs;
+
+/* Azimuthal derivative in Fourier space, i m F (m = mode index times minc), of
+ * two components: with the inverse FFT it gives torpol_to_dphspat
+ * (sht.py:725-743) before its 1/sin^2 factor. */
+int mh_fourier_dphi(const MhDphiArgs* args, void* stream);
+
+typedef struct {
+  double* es;          /* (L, N) entropy explicit term of mh_td, updated in place */
+  const double* heat;  /* (L, N) heating terms in spectral space */
+  const double* orho1; /* (N,) */
+  int L, N;
+} MhEntropyAnelArgs;
+
+/* es = (es + heat) / rho0: the heating source and finish_exp_entropy of the
+ * anelastic step (step_time.py:1436-1440, update_s.py:191-210). */
+int mh_entropy_anel(const MhEntropyAnelArgs* args, void* stream);
+
+typedef struct {
+  MhImplArgs base;   /* fields, old/impl outputs, opr, opm, BuoFac, l_mag, lm_l0 */
+  MhAnelProf prof;
+  const double* dz;  /* (L, N) dz/dr (the anelastic z implicit term has a first derivative) */
+  int zero_edges;    /* zero the w, p implicit terms at l = 0 and the boundaries (set-up) */
+  int pad;
+} MhImplAnelArgs;
+
+/* Old states and implicit terms of the anelastic step (update_s.py:265-268,
+ * update_z.py:495-499, update_wp.py:500-531, step_time.py:398-617). */
+int mh_impl_anel(const MhImplAnelArgs* args, void* stream);
+
+typedef struct {
+  double *z, *dz, *d2z;     /* (L, N) toroidal velocity and its derivatives, corrected */
+  const double* r2q;        /* (N,) r^2 times the radial integration weights */
+  const double *pz, *pdz, *pd2z; /* (N,) correction profiles rho0 r^2 and its derivatives */
+  int l1m0, l1m1;           /* rows of (1, 0) and (1, 1) (-1: absent) */
+  int N, do_z, do_e, pad;   /* axial / equatorial corrections */
+  double cz;                /* 8/3 pi y10 / (c_moi_oc y10) */
+  double ce;                /* 8/3 pi / c_moi_oc */
+} MhAmArgs;
+
+/* Angular-momentum corrections of z(l=1, m=0) (axial) and z(l=1, m=1)
+ * (equatorial), update_z.py:301-331,450-475; one workgroup. */
+int mh_am_correct(const MhAmArgs* args, void* stream);
+
+typedef struct {
+  double* rwp;        /* (L, 2N) (w, p) right-hand side; row lm_l0, entry N is written */
+  const double* s;    /* (L, N) entropy */
+  const double* u;    /* (N,) integral weights */
+  int lm_l0, N;
+} MhP0IntArgs;
+
+/* The l=0 pressure's integral condition, rhs = sum u Re s(l=0)
+ * (update_wp.py:431-436); one workgroup. */
+int mh_p0_integral(const MhP0IntArgs* args, void* stream);
+
 typedef struct {
   const double *w, *dw, *z, *b, *db, *aj;
   const int* lm_start;
@@ -383,6 +486,7 @@ typedef struct {
   double LFfac;
   int minc; /* azimuthal symmetry: the orders are 0, minc, 2 minc, ... (0 reads as 1) */
   int pad;
+  const double* orho1; /* (N,) 1/rho0 weighting the kinetic energies (anelastic), or NULL */
 } MhEnergyArgs;
 
 /* Kinetic / magnetic poloidal and toroidal energy per degree (by_m=0) or per

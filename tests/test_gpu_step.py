@@ -40,12 +40,16 @@ def _rel(a, b):
     return np.abs(a - b).max() / np.abs(b).max()
 
 
+P_TOL_ANEL = 1e-9
+
+
 @pytest.mark.parametrize("name,fused", [("bench16", True), ("bench16", False), ("c1", True),
                                         ("dtchange", True), ("bpr16", True), ("bprdt", True),
                                         ("minc2", True), ("minc2", False),
                                         ("bprminc4", True), ("condic", True),
                                         ("bprcondic", True), ("rotic", True), ("rotic", False),
-                                        ("bprrotic4", True)])
+                                        ("bprrotic4", True), ("anel21", True), ("anelmag16", True),
+                                        ("bpranel2", True)])
 def test_step_matches_reference_trajectory(name, fused):
     """fused: one grid-space kernel; otherwise separate FFT / products / FFT kernels.
     The dt each step returns is passed to the next (dtchange: the reference
@@ -58,7 +62,11 @@ def test_step_matches_reference_trajectory(name, fused):
     with the inner-core fields b_ic and aj_ic checked too.  rotic / bprrotic4:
     the conducting core also rotates (omega_ic integrated in time from the
     Lorentz torque, CNAB2 and BPR353 with minc = 4), its rotation rate checked
-    at every step."""
+    at every step.  anel21 / anelmag16 / bpranel2: the anelastic step (the
+    hydrodynamic benchmark's physics with angular-momentum corrections, a
+    magnetic run, BPR353 with minc = 2); the pressure within P_TOL_ANEL (the
+    l = 0 pressure's system under the integral condition has a condition number
+    near 3e9: two LAPACK inverses differ by 3e-10 there)."""
     case, g = _load(name)
     run = _run(case, fused)
     dt = case["dt"]
@@ -72,8 +80,12 @@ def test_step_matches_reference_trajectory(name, fused):
             assert run.omega_ic == pytest.approx(g["omega_ic"][n - 1], rel=1e-10), n
         if n in case["field_steps"]:
             for k in ("s", "z", "w", "p", "b", "aj"):
-                got = run.f[k].cpu().numpy()
-                assert _rel(got, g[f"{k}_step{n}"]) < 1e-10, (n, k, _rel(got, g[f"{k}_step{n}"]))
+                got, ref = run.f[k].cpu().numpy(), g[f"{k}_step{n}"]
+                if not np.abs(ref).max():
+                    assert not np.abs(got).max(), (n, k)
+                    continue
+                tol = P_TOL_ANEL if (k == "p" and run.anel) else 1e-10
+                assert _rel(got, ref) < tol, (n, k, _rel(got, ref))
             if run.cond_ic:
                 for k in ("b_ic", "aj_ic"):
                     got = run.fic[k].cpu().numpy()

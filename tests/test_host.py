@@ -348,3 +348,65 @@ def test_rotating_inner_core_operator_matches_oracle():
     ref = np.concatenate([f, o.D1 @ f, o.D2 @ f])
     got = mt[:N, :3 * N].T @ rhs
     assert np.abs(got - ref).max() <= 1e-9 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("cfg", [
+    dict(l_max=12, n_r_max=17, n_cheb_max=15, mode=1, strat=5.0, polind=2.0, g2=1.0,
+         ktopv=1, kbotv=1, ra=1.48638035e5),
+    dict(l_max=12, n_r_max=17, strat=1.0, polind=2.0, g1=1.0, ra=1e5, prmag=5.0)])
+def test_anelastic_operators_match_oracle(cfg):
+    """Anelastic host side (magic_amd.anelastic) against the oracle (pinned by
+    the reference's anelastic trajectories): background profiles and factors,
+    the S, Z, (w, p) and l = 0 pressure inverses, the conduction state, the
+    angular-momentum factors, and the row operators applied to a right-hand
+    side against the oracle's solve -> truncation -> DCT-I -> derivatives."""
+    from magic_amd import anelastic
+    from oracle import anelastic as oan, radial as orad
+    from oracle.dynamo import OracleRun
+    p, o = Params(dict(cfg, l_correct_AMz=True)), OracleRun(dict(cfg, l_correct_AMz=True))
+    op = o.prm
+    rad = grid.radial(p)
+    prof = anelastic.background(p, rad)
+    for k in ("rgrav", "temp0", "rho0", "orho1", "otemp1", "beta", "dbeta", "dLtemp0"):
+        np.testing.assert_allclose(prof[k], o.prof[k], rtol=1e-13, atol=1e-13)
+    for k in ("ViscHeatFac", "OhmLossFac"):
+        assert prof[k] == pytest.approx(o.prof[k], rel=1e-14)
+    w0 = 3e-4
+    pairs = ((anelastic.s_inverse(p, rad, prof, w0), oan.build_s(op, o.rad, o.mats, o.prof, w0)),
+             (anelastic.z_inverse(p, rad, prof, w0), oan.build_z(op, o.rad, o.mats, o.prof, w0)),
+             (anelastic.wp_inverse(p, rad, prof, w0), oan.build_wp(op, o.rad, o.mats, o.prof, w0)),
+             (anelastic.p0_inverse(p, rad, prof)[None], oan.build_p0(op, o.rad, o.mats, o.prof)[None]))
+    for a, b in pairs:
+        for l in range(a.shape[0]):
+            assert np.abs(a[l] - b[l]).max() <= 1e-8 * max(np.abs(b[l]).max(), 1e-300), l
+    assert anelastic.p0_integral(p, prof) == oan.p0_integral_bc(op, o.prof)
+    np.testing.assert_allclose(anelastic.p0_weights(p, rad, prof),
+                               oan.p0_rhs_weights(op, o.rad, o.prof, o.q), rtol=1e-12, atol=1e-300)
+    s0, p0 = anelastic.conduction_state(p, rad, prof)
+    os0, op0 = oan.ps_cond(op, o.rad, o.mats, o.prof)
+    assert np.abs(s0 - os0).max() <= 1e-11 * np.abs(os0).max()
+    assert np.abs(p0 - op0).max() <= 1e-8 * np.abs(op0).max()
+    am, oam = anelastic.am_setup(p, rad, prof), o.am
+    for k, ok in (("r2q", "r2q"), ("pz", "z"), ("pdz", "dz"), ("pd2z", "d2z")):
+        np.testing.assert_allclose(am[k], oam[ok], rtol=1e-12, atol=1e-300)
+    assert am["cz"] == pytest.approx(8.0 / 3.0 * math.pi / oam["c_moi_oc"], rel=1e-13)
+    # row operators: outputs (f, f', f'') of the solve of a random right-hand side
+    N = p.n_r_max
+    k4, r3, k4w, r6 = -(-N // 4) * 4, -(-(3 * N) // 64) * 64, -(-(2 * N) // 4) * 4, -(-(6 * N) // 64) * 64
+    ms, mz, mwp = anelastic.row_operators(p, rad, prof, w0, k4, r3, k4w, r6)
+    rhs = np.random.default_rng(11).standard_normal(2 * N)
+    l = 3
+    sol = pairs[2][1][l] @ rhs
+    cw, cp = sol[:N].copy(), sol[N:].copy()
+    cw[p.n_cheb_max:] = 0.0
+    cp[p.n_cheb_max:] = 0.0
+    fw, fp = orad.costf(cw), orad.costf(cp)
+    ref = np.concatenate([fw, o.D1 @ fw, o.D2 @ fw, o.D3 @ fw, fp, o.D1 @ fp])
+    got = mwp[l, :2 * N, :6 * N].T @ rhs
+    assert np.abs(got - ref).max() <= 1e-8 * np.abs(ref).max()
+    sol = pairs[0][1][l] @ rhs[:N]
+    sol[p.n_cheb_max:] = 0.0
+    fs = orad.costf(sol)
+    got = ms[l, :N, :3 * N].T @ rhs[:N]
+    ref = np.concatenate([fs, o.D1 @ fs, o.D2 @ fs])
+    assert np.abs(got - ref).max() <= 1e-9 * np.abs(ref).max()

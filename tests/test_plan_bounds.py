@@ -323,3 +323,98 @@ def test_rotating_inner_core_launches_stay_inside_their_tensors(cpu_standin, sch
                 assert lo == run.nl["VxBt"].data_ptr() + 16 * (N - 1)
                 _inside(ext, lo, lo + 16 * ((L - 1) * N + 1), "ICB VxBt")
     assert seen == {"mh_ic_torque", "mh_ic_advect", "mh_ic_omega_rhs", "mh_ic_omega_update"}
+
+
+@pytest.mark.parametrize("cfg", [
+    dict(l_max=12, n_r_max=17, mode=1, strat=3.0, polind=2.0, g2=1.0, ktopv=1, kbotv=1,
+         l_correct_AMz=True, l_correct_AMe=True, dtmax=1e-4),
+    dict(l_max=12, n_r_max=17, strat=1.0, polind=2.0, g1=1.0, dtmax=1e-4, minc=2),
+    dict(l_max=12, n_r_max=17, strat=1.0, polind=2.0, g1=1.0, dtmax=1e-4,
+         time_scheme="BPR353")])
+def test_anelastic_launches_stay_inside_their_tensors(cpu_standin, cfg):
+    """Anelastic step: every synthesis, Fourier derivative, FFT, grid-product,
+    analysis, entropy-finish, angular-momentum, l = 0 pressure and implicit-term
+    launch of the recorded radial and lm plans (each BPR353 stage) reads and
+    writes inside the run's tensors."""
+    with _lib.record():
+        run = dyn.DynamoRun(cfg, device="cpu")
+        run.initialize_dt(cfg["dtmax"])
+        run._rebuild_launches(0.5e-4)
+        run._build_anel(0.5e-4)
+    assert run.anel and not run.fused_grid
+    ext = _extents(run)
+    _check_row_jobs(run, ext)
+    r, N, L = run.res, run.N, run.L
+    Nb, M, T, P = N - 2, r.M, r.T, r.P
+    npt = T * P
+    stages = (1, 2, 3, 4) if run.dirk else (0,)
+    seen = set()
+    for k in stages:
+        run.stage = k
+        for kind in (("radial", "lm") if k <= 3 else ("lm",)):
+            _, plan = run._plan(kind)
+            for _, name, args in plan:
+                seen.add(name)
+                if name == "mh_leg_synth":
+                    a = C.cast(args[1], C.POINTER(_lib.SynArgs)).contents
+                    nc = 3 if args[0] else 1
+                    _inside(ext, a.F, a.F + 8 * (nc * a.c_stride), "synth F")
+                    for g in range(a.ngroups):
+                        for c in range(nc):
+                            x = a.in_[g][c].x1
+                            _inside(ext, x, x + 16 * L * a.ldx, "synth input")
+                elif name == "mh_fft":
+                    a = C.cast(args[1], C.POINTER(_lib.FftArgs)).contents
+                    _inside(ext, a.F, a.F + 8 * a.ncomp * a.Fc_stride, "fft F")
+                    _inside(ext, a.G, a.G + 8 * a.ncomp * a.Gc_stride, "fft G")
+                elif name == "mh_fourier_dphi":
+                    a = C.cast(args[0], C.POINTER(_lib.DphiArgs)).contents
+                    _inside(ext, a.F, a.F + 8 * ((a.c0 + 2) * a.Fc_stride), "dphi F")
+                    _inside(ext, a.out, a.out + 8 * 2 * a.Oc_stride, "dphi out")
+                    assert a.B <= a.FB and a.B <= a.OB and a.n_m == M
+                elif name == "mh_nl_anel":
+                    a = C.cast(args[0], C.POINTER(_lib.NlAnelArgs)).contents
+                    _inside(ext, a.Gs, a.Gs + 8 * 2 * N * npt, "nl s")
+                    _inside(ext, a.Gv, a.Gv + 8 * 3 * a.Gv_c_stride, "nl u")
+                    assert a.Gv_c_stride == 3 * N * npt and a.Gd_c_stride == N * npt
+                    if a.l_mag:
+                        _inside(ext, a.Gb, a.Gb + 8 * 3 * a.Gb_c_stride, "nl B")
+                        assert a.Gb_c_stride == 2 * N * npt
+                    _inside(ext, a.Gd, a.Gd + 8 * 2 * a.Gd_c_stride, "nl dphi")
+                    _inside(ext, a.Go, a.Go + 8 * 10 * a.Go_c_stride, "nl out")
+                    assert a.Go_c_stride == Nb * npt
+                    _inside(ext, a.cfl, a.cfl + 8 * 2 * N, "cfl")
+                    for v in (a.prof.beta, a.prof.orho1, a.prof.otemp1, a.prof.visc):
+                        _inside(ext, v, v + 8 * N, "profile")
+                    _inside(ext, a.cosn2, a.cosn2 + 8 * T, "cos/sin^2")
+                elif name == "mh_leg_analysis":
+                    a = C.cast(args[1], C.POINTER(_lib.AnaArgs)).contents
+                    for i in range(a.ncomp):
+                        lo = a.outS[i]
+                        _inside(ext, lo, lo + 16 * ((L - 1) * a.ldo + a.B), "analysis out")
+                elif name == "mh_entropy_anel":
+                    a = C.cast(args[0], C.POINTER(_lib.EntropyAnelArgs)).contents
+                    for v in (a.es, a.heat):
+                        _inside(ext, v, v + 16 * L * N, "entropy")
+                elif name == "mh_am_correct":
+                    a = C.cast(args[0], C.POINTER(_lib.AmArgs)).contents
+                    for v in (a.z, a.dz, a.d2z):
+                        _inside(ext, v, v + 16 * L * N, "am z")
+                    assert 0 <= a.l1m0 < L and -1 <= a.l1m1 < L
+                elif name == "mh_p0_integral":
+                    a = C.cast(args[0], C.POINTER(_lib.P0IntArgs)).contents
+                    _inside(ext, a.rwp, a.rwp + 16 * L * 2 * N, "p0 rhs")
+                    _inside(ext, a.u, a.u + 8 * N, "p0 weights")
+                elif name == "mh_impl_anel":
+                    a = C.cast(args[0], C.POINTER(_lib.ImplAnelArgs)).contents
+                    for f in ("s", "z", "w", "dw", "ddw", "dddw", "p", "dp", "old_s", "impl_s",
+                              "impl_w", "impl_p", "impl_z"):
+                        v = getattr(a.base, f)
+                        _inside(ext, v, v + 16 * L * N, f"impl {f}")
+                    _inside(ext, a.dz, a.dz + 16 * L * N, "impl dz")
+                    assert a.zero_edges == 0
+    assert {"mh_nl_anel", "mh_fourier_dphi", "mh_entropy_anel", "mh_impl_anel"} <= seen
+    if run.p.l_correct_AMz:
+        assert "mh_am_correct" in seen
+    if run.p0_integral:
+        assert "mh_p0_integral" in seen

@@ -136,7 +136,45 @@ class ImplArgs(C.Structure):
 class EnergyArgs(C.Structure):
     _fields_ = [("w", P), ("dw", P), ("z", P), ("b", P), ("db", P), ("aj", P), ("lm_start", P),
                 ("or2", P), ("qw", P), ("out", P), ("l_max", I), ("N", I), ("by_m", I),
-                ("l_mag", I), ("LFfac", D), ("minc", I), ("pad", I)]
+                ("l_mag", I), ("LFfac", D), ("minc", I), ("pad", I), ("orho1", P)]
+
+
+class AnelProf(C.Structure):
+    _fields_ = [(n, P) for n in ("beta", "dbeta", "rho0", "orho1", "otemp1", "dLtemp0", "visc",
+                                 "dLvisc", "kappa", "dLkappa", "rgrav_rho")] + [
+        ("ViscHeatFac", D), ("OhmLossFac", D)]
+
+
+class NlAnelArgs(C.Structure):
+    _fields_ = [("Gs", P), ("Gv", P), ("Gv_c_stride", L), ("Gb", P), ("Gb_c_stride", L),
+                ("Gd", P), ("Gd_c_stride", L), ("Go", P), ("Go_c_stride", L), ("N", I), ("T", I),
+                ("P", I), ("l_mag", I), ("or1", P), ("or2", P), ("or4", P), ("ost2", P),
+                ("cosn2", P), ("valri2", P), ("valhi2", P), ("prof", AnelProf), ("LFfac", D),
+                ("cf2", D), ("af2", D), ("alf_damp", I), ("pad", I), ("cfl", P)]
+
+
+class DphiArgs(C.Structure):
+    _fields_ = [("F", P), ("Fc_stride", L), ("Fm_stride", L), ("FB", I), ("c0", I), ("out", P),
+                ("Oc_stride", L), ("Om_stride", L), ("OB", I), ("B", I), ("n_m", I), ("T", I),
+                ("minc", I), ("pad", I)]
+
+
+class EntropyAnelArgs(C.Structure):
+    _fields_ = [("es", P), ("heat", P), ("orho1", P), ("L", I), ("N", I)]
+
+
+class AmArgs(C.Structure):
+    _fields_ = [("z", P), ("dz", P), ("d2z", P), ("r2q", P), ("pz", P), ("pdz", P), ("pd2z", P),
+                ("l1m0", I), ("l1m1", I), ("N", I), ("do_z", I), ("do_e", I), ("pad", I),
+                ("cz", D), ("ce", D)]
+
+
+class P0IntArgs(C.Structure):
+    _fields_ = [("rwp", P), ("s", P), ("u", P), ("lm_l0", I), ("N", I)]
+
+
+class ImplAnelArgs(C.Structure):
+    _fields_ = [("base", ImplArgs), ("prof", AnelProf), ("dz", P), ("zero_edges", I), ("pad", I)]
 
 
 class BuildArgs(C.Structure):
@@ -167,7 +205,8 @@ class StepCtlArgs(C.Structure):
 
 _STRUCT_ORDER = [SynArgs, AnaArgs, FftArgs, NlArgs, RowJob, TdArgs, RhsArgs, BuoyArgs,
                  ImplArgs, EnergyArgs, BuildArgs, GemmArgs, GridArgs, StepState, StepCtlArgs,
-                 IcRhsArgs, IcImplArgs, IcRotArgs]
+                 IcRhsArgs, IcImplArgs, IcRotArgs, NlAnelArgs, DphiArgs, EntropyAnelArgs,
+                 ImplAnelArgs, AmArgs, P0IntArgs]
 
 _SIGS = {
     "mh_plm_tables": [I, I, I, I, I, P, P, P, P, P, P, P, P, P],
@@ -175,6 +214,12 @@ _SIGS = {
     "mh_leg_analysis": [I, P, P],
     "mh_fft": [I, P, P],
     "mh_get_nl": [P, P],
+    "mh_nl_anel": [P, P],
+    "mh_fourier_dphi": [P, P],
+    "mh_entropy_anel": [P, P],
+    "mh_impl_anel": [P, P],
+    "mh_am_correct": [P, P],
+    "mh_p0_integral": [P, P],
     "mh_grid_step": [P, P],
     "mh_grid_plan": [I, P],
     "mh_ic_rhs": [P, P],

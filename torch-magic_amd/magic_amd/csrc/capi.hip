@@ -10,6 +10,8 @@ extern "C" int mh_struct_sizes(int* out, int n) {
       (int)sizeof(MhEnergyArgs), (int)sizeof(MhBuildArgs), (int)sizeof(MhGemmArgs),
       (int)sizeof(MhGridArgs), (int)sizeof(MhStepState), (int)sizeof(MhStepCtlArgs),
       (int)sizeof(MhIcRhsArgs), (int)sizeof(MhIcImplArgs), (int)sizeof(MhIcRotArgs),
+      (int)sizeof(MhNlAnelArgs), (int)sizeof(MhDphiArgs), (int)sizeof(MhEntropyAnelArgs),
+      (int)sizeof(MhImplAnelArgs), (int)sizeof(MhAmArgs), (int)sizeof(MhP0IntArgs),
   };
   const int ns = (int)(sizeof(sizes) / sizeof(sizes[0]));
   for (int i = 0; i < n && i < ns; ++i) out[i] = sizes[i];

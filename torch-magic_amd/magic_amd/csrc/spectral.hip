@@ -251,17 +251,18 @@ __global__ __launch_bounds__(256) void k_energy(EnergyArgs a) {
     const bool m0 = m == 0;
     const double dLh = (double)l * (double)(l + 1);
     const double wq = a.qw[r], or2 = a.or2[r];
+    const double wk = a.orho1 ? wq * a.orho1[r] : wq;  // kinetic: 1/rho0 (anelastic)
     if (!parts) {
-      acc[0] += wq * 0.5 * dLh * (dLh * or2 * cc2real(C2(a.w)[idx], m0) + cc2real(C2(a.dw)[idx], m0));
-      acc[1] += wq * 0.5 * dLh * cc2real(C2(a.z)[idx], m0);
+      acc[0] += wk * 0.5 * dLh * (dLh * or2 * cc2real(C2(a.w)[idx], m0) + cc2real(C2(a.dw)[idx], m0));
+      acc[1] += wk * 0.5 * dLh * cc2real(C2(a.z)[idx], m0);
       if (a.l_mag) {
         acc[2] += wq * 0.5 * a.LFfac * dLh * (dLh * or2 * cc2real(C2(a.b)[idx], m0) + cc2real(C2(a.db)[idx], m0));
         acc[3] += wq * 0.5 * a.LFfac * dLh * cc2real(C2(a.aj)[idx], m0);
       }
     } else {
       double e[4];
-      e[0] = wq * 0.5 * dLh * (dLh * or2 * cc2real(C2(a.w)[idx], m0) + cc2real(C2(a.dw)[idx], m0));
-      e[1] = wq * 0.5 * dLh * cc2real(C2(a.z)[idx], m0);
+      e[0] = wk * 0.5 * dLh * (dLh * or2 * cc2real(C2(a.w)[idx], m0) + cc2real(C2(a.dw)[idx], m0));
+      e[1] = wk * 0.5 * dLh * cc2real(C2(a.z)[idx], m0);
       e[2] = e[3] = 0.0;
       if (a.l_mag) {
         e[2] = wq * 0.5 * a.LFfac * dLh * (dLh * or2 * cc2real(C2(a.b)[idx], m0) + cc2real(C2(a.db)[idx], m0));
@@ -311,14 +312,15 @@ __global__ __launch_bounds__(128) void k_energy_lr(EnergyArgs a) {
   const int minc = a.minc > 1 ? a.minc : 1;
   for (int r = threadIdx.x; r < N; r += blockDim.x) {
     const double or2 = a.or2[r];
+    const double ok = a.orho1 ? a.orho1[r] : 1.0;  // kinetic: 1/rho0 (anelastic)
     double e[8];
 #pragma unroll
     for (int q = 0; q < 8; ++q) e[q] = 0.0;
     for (int m = 0; m <= l; m += minc) {
       const long idx = (long)(a.lm_start[m / minc] + (l - m)) * N + r;
       const bool m0 = m == 0;
-      const double kp = 0.5 * dLh * (dLh * or2 * cc2real(C2(a.w)[idx], m0) + cc2real(C2(a.dw)[idx], m0));
-      const double kt = 0.5 * dLh * cc2real(C2(a.z)[idx], m0);
+      const double kp = ok * 0.5 * dLh * (dLh * or2 * cc2real(C2(a.w)[idx], m0) + cc2real(C2(a.dw)[idx], m0));
+      const double kt = ok * 0.5 * dLh * cc2real(C2(a.z)[idx], m0);
       e[0] += kp;
       e[1] += kt;
       e[2] += m0 ? 0.0 : kp;

@@ -239,6 +239,11 @@ class DynamoRun:
     def _construct(self, cfg, device, res, fused_grid):
         self.p = p = cfg if isinstance(cfg, Params) else Params(cfg)
         p.check_step_supported()
+        # anelastic runs (strat > 0) take the separate grid path (FP64 FFTs and a
+        # grid kernel of their own) with FP64 transforms
+        self.anel = bool(p.l_anel)
+        if self.anel:
+            fused_grid = False
         self.device = require_gpu(device)
         self.res = res if res is not None else Resolution(p, self.device)
         r = self.res
@@ -295,7 +300,20 @@ class DynamoRun:
             nFo = 9 * M * T * self.Nb * 2
             self.F_o = self.F_v[:nFo] if nFo <= self.F_v.numel() else r.fourier_buffer(9, self.Nb)
         assert used[0] == self._pool.numel()
+        if self.anel:
+            self._anel_buffers()
         self.cfl = torch.zeros(2 * N, dtype=torch.int64, device=dev)
+        self.am = None
+        if p.l_correct_AMz or p.l_correct_AMe:   # update_z.py:450-475
+            from . import anelastic
+            prof = self.prof if self.anel else dict(
+                rho0=np.ones(N), beta=np.zeros(N), dbeta=np.zeros(N))
+            am = anelastic.am_setup(p, r.rad, prof)
+            self.am = {k: (torch.as_tensor(v, dtype=F64).to(dev) if isinstance(v, np.ndarray)
+                           else v) for k, v in am.items()}
+            self.am["l1m0"] = int(r.maps["lm_start"][0] + 1)
+            # (l=1, m=1) is the first row of order 1 (absent when minc > 1)
+            self.am["l1m1"] = int(r.maps["lm_start"][1]) if (p.minc == 1 and p.n_m > 1) else -1
         self.cfl_host = torch.zeros(2 * N, dtype=torch.int64).pin_memory()
         # conducting inner core (sigma_ratio > 0): the inner fields on the
         # even-Chebyshev grid, their old states and implicit terms (BPR353: one
@@ -402,6 +420,39 @@ class DynamoRun:
         self.n_step = 0          # steps taken
         self._ctl = None         # device step control buffers (run_steps / step_async)
         self.expl_pending = False  # a radial loop's explicit terms await the lm loop
+
+    def _anel_buffers(self):
+        """Fourier and grid buffers of the anelastic radial loop, the reference
+        state's profiles on the device, and the angular-momentum and l = 0
+        pressure factors."""
+        from . import anelastic
+        r, p, N, L, dev = self.res, self.p, self.N, self.L, self.device
+        Nb = N - 2
+        self.prof = prof = anelastic.background(p, r.rad)
+        up = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=F64).to(dev)
+        self.d_an = {k: up(prof[k]) for k in ("beta", "dbeta", "rho0", "orho1", "otemp1",
+                                              "dLtemp0", "visc", "dLvisc", "kappa", "dLkappa")}
+        self.d_an["rgrav_rho"] = up(prof["rho0"] * prof["rgrav"])
+        self.d_an["cosn2"] = up(r.th["cos_grid"] * r.th["ost2_grid"])
+        self.d_an["p0u"] = up(anelastic.p0_weights(p, r.rad, prof))
+        self.p0_integral = anelastic.p0_integral(p, prof)
+        self.F_s2, self.G_s2 = r.fourier_buffer(1, 2 * N), r.grid_buffer(1, 2 * N)
+        self.F_va, self.G_va = r.fourier_buffer(3, 3 * N), r.grid_buffer(3, 3 * N)
+        if p.l_mag:
+            self.F_vb, self.G_vb = r.fourier_buffer(3, 2 * N), r.grid_buffer(3, 2 * N)
+        self.F_d, self.G_d = r.fourier_buffer(2, N), r.grid_buffer(2, N)
+        self.F_oa, self.G_oa = r.fourier_buffer(10, Nb), r.grid_buffer(10, Nb)
+        self.heatLM = torch.zeros(L, N, dtype=C128, device=dev)
+        self.zeroLN = torch.zeros(L, N, dtype=C128, device=dev)
+        self.comp_scale_10 = torch.tensor([0, 0, 0, 0, 1, 1, 1, 1, 1, 1], dtype=I32, device=dev)
+
+    def _anel_prof(self, a):
+        """Fill an MhAnelProf from the run's device profiles."""
+        d = self.d_an
+        for k in ("beta", "dbeta", "rho0", "orho1", "otemp1", "dLtemp0", "visc", "dLvisc",
+                  "kappa", "dLkappa", "rgrav_rho"):
+            setattr(a, k, ptr(d[k]))
+        a.ViscHeatFac, a.OhmLossFac = self.prof["ViscHeatFac"], self.prof["OhmLossFac"]
 
     # ------------------------------------------------------------ timing
     def enable_timers(self, on: bool = True):
@@ -636,6 +687,8 @@ class DynamoRun:
         self._rebuild_launches(wimp_lin0)
         if self.cond_ic:
             self._build_coupled(wimp_lin0)
+        if self.anel:
+            self._build_anel(wimp_lin0)
         self.wimp_lin0 = wimp_lin0
         self.n_rebuild += 1
         self._queue_info_check()
@@ -656,6 +709,18 @@ class DynamoRun:
             inv = inner_core.z10_inverse(p, rad, wimp_lin0, self.rot_c["c_dt_z10"])
             mt = inner_core.z10_row_operator(p, rad, inv, self.K4, self.Rpad3)
             self.MT_z10.copy_(torch.as_tensor(mt), non_blocking=False)
+
+    def _build_anel(self, wimp_lin0: float):
+        """The anelastic S, Z and (w, p) row operators (and the l = 0 pressure)
+        for ``wimp_lin0``, assembled and inverted on the host
+        (magic_amd.anelastic), copied over the Boussinesq ones of the device
+        rebuild on the run's stream (B and J are the same in both)."""
+        from . import anelastic
+        ms, mz, mwp = anelastic.row_operators(self.p, self.res.rad, self.prof, wimp_lin0,
+                                              self.K4, self.Rpad3, self.K4wp, self.Rpad6)
+        self.MT4[0].copy_(torch.as_tensor(ms), non_blocking=False)
+        self.MT4[1].copy_(torch.as_tensor(mz), non_blocking=False)
+        self.MT["wp"].copy_(torch.as_tensor(mwp), non_blocking=False)
 
     def _rebuild_launches(self, wimp_lin0, ctl=None):
         """The launches of a rebuild (ten, whatever the number of systems).
@@ -860,7 +925,11 @@ class DynamoRun:
             coef, shape = noise_factors(p, r, seed, amp)
             f["s"].add_(up(coef)[:, None] * up(shape)[None, :])
         if p.l_heat:
-            s0, p0 = conduction_state(p, r)
+            if self.anel:
+                from . import anelastic
+                s0, p0 = anelastic.conduction_state(p, r.rad, self.prof)
+            else:
+                s0, p0 = conduction_state(p, r)
             f["s"][0].add_(up(s0))
             f["p"][0].copy_(up(p0))
             if p.init_s1 >= 100:
@@ -899,7 +968,7 @@ class DynamoRun:
             for t in d.values():
                 t.zero_()
         self._rowop(self.jobs_setup)
-        self._impl()
+        self._impl(zero_edges=True)
         # the reference's set-up fills the magnetic implicit terms on the
         # interior levels only (step_time.py:582-595)
         for k in ("b", "j"):
@@ -944,7 +1013,7 @@ class DynamoRun:
         self.dtMax = dt
 
     # ------------------------------------------------------------ kernels
-    def _impl(self, old=None, impl=None):
+    def _impl(self, old=None, impl=None, zero_edges=False):
         """Implicit terms and old states of the new fields into `impl` / `old`
         (default: the run's own arrays)."""
         old = self.old if old is None else old
@@ -962,6 +1031,16 @@ class DynamoRun:
             setattr(a, "impl_" + k, ptr(impl[k]))
         a.opr, a.opm, a.BuoFac = self.p.opr, self.p.opm, self.p.BuoFac
         a.l_mag, a.lm_l0 = int(self.p.l_mag), 0
+        if self.anel:
+            # the set-up zeroes the w, p terms at l = 0 and the boundaries, the lm
+            # loop leaves them (step_time.py:528-535, update_wp.py:500-531)
+            A = _lib.ImplAnelArgs()
+            A.base = a
+            self._anel_prof(A.prof)
+            A.dz = ptr(f["dz"])
+            A.zero_edges = int(zero_edges)
+            _lib.call("mh_impl_anel", _lib.ref(A), stream_handle())
+            return
         _lib.call("mh_impl", _lib.ref(a), stream_handle())
 
     def radial_loop(self):
@@ -971,6 +1050,8 @@ class DynamoRun:
         return self._cfl_decision()
 
     def _radial_launches(self):
+        if self.anel:
+            return self._radial_launches_anel()
         r, f, N, Nb = self.res, self.f, self.N, self.Nb
         # 1. Legendre synthesis
         with self._t("leg_syn_scal"):
@@ -1088,6 +1169,77 @@ class DynamoRun:
         """Rotation rate of the inner core (0.0 without a rotating core; reads
         the device state: synchronises the run's stream)."""
         return float(self.rot[0].item()) if self.cond_ic else 0.0
+
+    def _radial_launches_anel(self):
+        """The anelastic radial loop (``radial_loop_anel``, step_time.py:1215-1475),
+        FP64: synthesis of s and l(l+1) z; of u, du/dr and the gradient of the
+        radial velocity; of B and curl B; the azimuthal derivatives of u_theta,
+        u_phi in Fourier space; inverse FFTs; the anelastic grid products with the
+        Courant maxima; forward FFTs and analysis of the bulk levels; the explicit
+        terms, with the heating and 1/rho0 in the entropy equation."""
+        r, f, N, Nb, p = self.res, self.f, self.N, self.Nb, self.p
+        M, T = r.M, r.T
+        with self._t("leg_syn_scal"):
+            r.synth([[(f["s"], None, 0)], [(f["z"], None, 1)]], N, N, self.F_s2, 2 * N,
+                    vector=False, prec=0)
+        groups = [[(f["w"], None, 1), (f["dw"], None, 0), (f["z"], None, 0)],
+                  [(f["dw"], None, 1), (f["ddw"], None, 0), (f["dz"], None, 0)],
+                  [(self.zeroLN, None, 0), (f["w"], None, 1), (self.zeroLN, None, 0)]]
+        with self._t("leg_syn_vec"):
+            r.synth(groups, N, N, self.F_va, 3 * N, vector=True, prec=0)
+            if p.l_mag:
+                r.synth([[(f["b"], None, 1), (f["db"], None, 0), (f["aj"], None, 0)],
+                         [(f["aj"], None, 1), (f["dj"], None, 0), (f["ddb"], f["b"], 2)]],
+                        N, N, self.F_vb, 2 * N, vector=True, prec=0)
+        d = _lib.DphiArgs()
+        d.F, (d.Fc_stride, d.Fm_stride), d.FB, d.c0 = ptr(self.F_va), r.fstrides(3 * N), 3 * N, 1
+        d.out, (d.Oc_stride, d.Om_stride), d.OB = ptr(self.F_d), r.fstrides(N), N
+        d.B, d.n_m, d.T, d.minc = N, M, T, p.minc
+        with self._t("fft_inv"):
+            _lib.call("mh_fourier_dphi", _lib.ref(d), stream_handle())
+            r.fft(False, self.F_s2, 2 * N, self.G_s2, 1, 2 * N)
+            r.fft(False, self.F_va, 3 * N, self.G_va, 3, 3 * N)
+            if p.l_mag:
+                r.fft(False, self.F_vb, 2 * N, self.G_vb, 3, 2 * N)
+            r.fft(False, self.F_d, N, self.G_d, 2, N)
+        a = _lib.NlAnelArgs()
+        npt = r.T * r.P
+        a.Gs, a.Gv, a.Gv_c_stride = ptr(self.G_s2), ptr(self.G_va), 3 * N * npt
+        if p.l_mag:
+            a.Gb, a.Gb_c_stride = ptr(self.G_vb), 2 * N * npt
+        a.Gd, a.Gd_c_stride = ptr(self.G_d), N * npt
+        a.Go, a.Go_c_stride = ptr(self.G_oa), Nb * npt
+        a.N, a.T, a.P, a.l_mag = N, r.T, r.P, int(p.l_mag)
+        dd = r.d
+        a.or1, a.or2, a.or4 = ptr(dd["or1"]), ptr(dd["or2"]), ptr(dd["or4"])
+        a.ost2, a.cosn2 = ptr(dd["ost2_grid"]), ptr(self.d_an["cosn2"])
+        a.valri2, a.valhi2 = ptr(dd["valri2"]), ptr(dd["valhi2"])
+        self._anel_prof(a.prof)
+        a.LFfac, a.cf2, a.af2 = p.LFfac, p.cour_fac ** 2, p.alf_fac ** 2
+        a.alf_damp, a.cfl = int(p.l_cour_alf_damp), ptr(self.cfl)
+        with self._t("get_nl"):
+            _lib.call("mh_nl_anel", _lib.ref(a), stream_handle())
+        with self._t("fft_fwd"):
+            r.fft(True, self.F_oa, Nb, self.G_oa, 10, Nb, comp_scale=self.comp_scale_10)
+        nl = self.nl
+        off = 16  # one complex column
+        with self._t("leg_ana_scal"):
+            r.analysis(self.F_oa, Nb, Nb, [0, 1, 2],
+                       [ptr(nl["Advr"]) + off, ptr(nl["dVSr"]) + off, ptr(nl["VxBr"]) + off],
+                       [None, None, None], N, vector=False, prec=0)
+            r.analysis(self.F_oa, Nb, Nb, [3], [ptr(self.heatLM) + off], [None], N,
+                       vector=False, prec=0)
+        with self._t("leg_ana_vec"):
+            r.analysis(self.F_oa, Nb, Nb, [(4, 5), (6, 7), (8, 9)],
+                       [ptr(nl["Advt"]) + off, ptr(nl["VSt"]) + off, ptr(nl["VxBt"]) + off],
+                       [ptr(nl["Advp"]) + off, None, ptr(nl["VxBp"]) + off], N, vector=True,
+                       prec=0)
+        self._td_launches()
+        e = _lib.EntropyAnelArgs()
+        e.es, e.heat, e.orho1 = ptr(self.expl_cur["s"]), ptr(self.heatLM), ptr(self.d_an["orho1"])
+        e.L, e.N = self.L, N
+        with self._t("td"):
+            _lib.call("mh_entropy_anel", _lib.ref(e), stream_handle())
 
     def _grid_separate(self):
         r, N, Nb, d = self.res, self.N, self.Nb, self.res.d
@@ -1210,12 +1362,20 @@ class DynamoRun:
                 self._ic_omega_rhs(wdev)
         with self._t("rowop_szbj"):
             self._rowop(self.jobs1)
+        if self.am is not None:
+            self._am_correct()
         bq = _lib.BuoyArgs()
         bq.rwp, bq.s, bq.ew0 = ptr(self.rhs["wp"]), ptr(self.f["s"]), ptr(ew0)
-        bq.rgrav, bq.m, bq.L, bq.N = ptr(self.res.d["rgrav"]), ptr(self.res.d["m"]), self.L, self.N
+        rg = self.d_an["rgrav_rho"] if self.anel else self.res.d["rgrav"]   # rho0 rgrav
+        bq.rgrav, bq.m, bq.L, bq.N = ptr(rg), ptr(self.res.d["m"]), self.L, self.N
         bq.BuoFac, bq.lm_l0, bq.wdev = p.BuoFac, 0, wdev
         with self._t("buoy"):
             _lib.call("mh_buoy", _lib.ref(bq), stream_handle())
+            if self.anel and self.p0_integral:
+                q0 = _lib.P0IntArgs()
+                q0.rwp, q0.s, q0.u = ptr(self.rhs["wp"]), ptr(self.f["s"]), ptr(self.d_an["p0u"])
+                q0.lm_l0, q0.N = 0, self.N
+                _lib.call("mh_p0_integral", _lib.ref(q0), stream_handle())
         with self._t("rowop_wp"):
             self._rowop(self.jobs2)
         with self._t("impl"):
@@ -1238,6 +1398,19 @@ class DynamoRun:
                     self._ic_impl(self.old_ic, self.impl_ic)
                 if self.rot_ic:
                     self._ic_omega_update(True, 2)
+
+    def _am_correct(self):
+        """Angular-momentum corrections of z(l=1, m=0) and z(l=1, m=1) and their
+        derivatives after the toroidal solve (update_z.py:450-475)."""
+        am, p = self.am, self.p
+        a = _lib.AmArgs()
+        a.z, a.dz, a.d2z = ptr(self.f["z"]), ptr(self.f["dz"]), ptr(self.tmp["d2z"])
+        a.r2q, a.pz, a.pdz, a.pd2z = (ptr(am[k]) for k in ("r2q", "pz", "pdz", "pd2z"))
+        a.l1m0, a.l1m1, a.N = am["l1m0"], am["l1m1"], self.N
+        a.do_z, a.do_e = int(p.l_correct_AMz), int(p.l_correct_AMe)
+        a.cz, a.ce = am["cz"], am["ce"]
+        with self._t("am"):
+            _lib.call("mh_am_correct", _lib.ref(a), stream_handle())
 
     def _ic_rhs(self, wdev):
         """Coupled right-hand sides of the B and J systems (mh_ic_rhs): the inner
@@ -1460,9 +1633,10 @@ class DynamoRun:
         return st.dt
 
     def _no_dirk(self, what):
-        if self.dirk or self.cond_ic:
-            raise NotImplementedError(f"{what}: BPR353 runs and runs with a conducting inner "
-                                      "core are stepped with one_step / run_steps")
+        if self.dirk or self.cond_ic or self.anel:
+            raise NotImplementedError(f"{what}: BPR353 runs, anelastic runs and runs with a "
+                                      "conducting inner core are stepped with one_step / "
+                                      "run_steps")
 
     def step_async(self):
         """Queue one whole step with the decisions on the device (one HIP graph
@@ -1504,10 +1678,10 @@ class DynamoRun:
         one launch sequence (one HIP graph replay when use_graphs) and the host
         never waits inside the loop.  Same decisions, weights and results as
         n_steps calls of one_step; dt as for one_step's first call.  Returns the
-        dt of the last step.  BPR353 steps, and steps with a conducting inner core
-        (whose coupled operators are rebuilt on the host), are host-controlled
-        one_step calls."""
-        if self.dirk or self.cond_ic:
+        dt of the last step.  BPR353 steps, anelastic steps and steps with a
+        conducting inner core (whose operators are rebuilt on the host) are
+        host-controlled one_step calls."""
+        if self.dirk or self.cond_ic or self.anel:
             for _ in range(n_steps):
                 dt = self.one_step(self.n_step + 1, dt)
             return dt
@@ -1548,6 +1722,8 @@ class DynamoRun:
         a.l_max, a.N, a.by_m, a.l_mag = self.p.l_max, self.N, int(by_m), int(self.p.l_mag)
         a.minc = self.p.minc
         a.LFfac = self.p.LFfac if LFfac is None else LFfac
+        if self.anel:
+            a.orho1 = ptr(self.d_an["orho1"])
         _lib.call("mh_energy", _lib.ref(a), stream_handle())
         return out
 

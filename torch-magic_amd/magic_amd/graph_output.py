@@ -103,13 +103,16 @@ def write_graph_file(path, time: float, run, fields=None):
         return a.astype(">f4").tobytes(order="F")
 
     zero = slab(torch.zeros(T, P, dtype=torch.float64, device=run.device))
+    # anelastic runs write the velocity, rho0 u / rho0 (graph_output.py:144-147)
+    orh = run.prof["orho1"] if getattr(run, "anel", False) else None
     for n in range(N):
         if n in (0, N - 1):
             out.extend(zero * 3)
         else:
-            out.extend(slab(or2[n] * vr[n]))
-            out.extend(slab(or1[n] * osn * vt[n]))
-            out.extend(slab(or1[n] * osn * vp[n]))
+            fr, ft = (or2[n], or1[n]) if orh is None else (or2[n] * orh[n], or1[n] * orh[n])
+            out.extend(slab(fr * vr[n]))
+            out.extend(slab(ft * osn * vt[n]))
+            out.extend(slab(ft * osn * vp[n]))
         if p.l_heat:
             out.extend(slab(s_g[n]))
         out.extend(slab(p_g[n]))

@@ -154,12 +154,18 @@ class Params:
         self.amp_b1 = 5.0
 
     def check_step_supported(self):
-        """The device step covers the Boussinesq Chebyshev path with CNAB2 or
-        BPR353 and an insulating, conducting or rotating conducting inner core
-        (nRotIC = 1: omega_ic integrated in time, no-slip ICB)."""
+        """The device step covers the Chebyshev path with CNAB2 or BPR353: the
+        Boussinesq step with an insulating, conducting or rotating conducting
+        inner core (nRotIC = 1: omega_ic integrated in time, no-slip ICB), and
+        the anelastic step (strat > 0, polytropic, FP64) with an insulating
+        inner core; angular-momentum corrections without a rotating core."""
         bad = []
-        if self.l_anel:
-            bad.append("anelastic (strat > 0)")
+        if self.l_anel and self.l_cond_ic:
+            bad.append("anelastic step with a conducting inner core")
+        if self.l_anel and self.precision != "fp64":
+            bad.append(f"anelastic step in precision {self.precision!r} (fp64 only)")
+        if self.l_anel and self.g0 == 0.0 and self.g1 == 0.0 and self.g2 == 0.0:
+            bad.append("anelastic step without gravity (g0 = g1 = g2 = 0)")
         if self.l_finite_diff:
             bad.append("finite-difference radial scheme (use the banded solver API)")
         if self.nRotIC not in (0, 1):
@@ -171,8 +177,8 @@ class Params:
             bad.append("composition")
         if self.time_scheme.upper() not in ("CNAB2", "BPR353"):
             bad.append(f"time scheme {self.time_scheme}")
-        if self.l_correct_AMz or self.l_correct_AMe:
-            bad.append("angular-momentum correction")
+        if (self.l_correct_AMz or self.l_correct_AMe) and self.nRotIC != 0:
+            bad.append("angular-momentum correction with a rotating inner core")
         if self.ktops != 1 or self.kbots != 1:
             bad.append("fixed-flux entropy boundaries")
         if self.precision not in PRECISIONS:
