@@ -154,10 +154,13 @@ class OracleRun:
                 aj2 = (1.0 - aj1) * r_icb * sa / ca
                 self.aj_ic[l2m0] += b_tor * (aj1 * r_ic * np.sin(math.pi * r_ic / r_cmb)
                                              + aj2 * np.cos(math.pi * r_ic / r_cmb))
-                return
-            b_pol = p.amp_b1 * math.sqrt(3.0 * math.pi) / 4.0
-            self.b[l1m0] += b_pol * (r ** 3 - 4.0 / 3.0 * r_cmb * r ** 2 + r_icb ** 4 / 3.0 / r)
-            self.aj[l2m0] += b_tor * r * np.sin(math.pi * (r - r_icb))
+            else:
+                b_pol = p.amp_b1 * math.sqrt(3.0 * math.pi) / 4.0
+                self.b[l1m0] += b_pol * (r ** 3 - 4.0 / 3.0 * r_cmb * r ** 2
+                                         + r_icb ** 4 / 3.0 / r)
+                self.aj[l2m0] += b_tor * r * np.sin(math.pi * (r - r_icb))
+        if self.rot_ic and p.omega_ic1 != 0.0:   # initV (init_fields.py:528-532)
+            self.z[self.maps["lm2"][1, 0], p.n_r_max - 1] = p.omega_ic1 / self.c_z10_omega_ic
 
     def add_noise(self, seed: int, amp: float):
         """Seeded Gaussian noise on s for l>=1 (build extension, see module doc)."""

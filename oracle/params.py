@@ -38,7 +38,7 @@ _DEFAULTS = dict(
     radial_scheme="chebyshev", fd_order=2, fd_order_bound=2,
     fd_stretch=0.3, fd_ratio=0.1,
     l_cour_alf_damp=True, sigma_ratio=0.0, nRotIC=0,
-    l_correct_AMz=False, l_correct_AMe=False,
+    l_correct_AMz=False, l_correct_AMe=False, omega_ic1=0.0,
 )
 
 
@@ -71,7 +71,7 @@ class Params:
         self.lm_max = sum(self.l_max - m + 1 for m in range(0, self.m_max + 1, self.minc))
         for k in ("ra", "ek", "pr", "prmag", "radratio", "raxi", "sc", "strat", "polind",
                   "g0", "g1", "g2", "dtmax", "alpha", "courfac", "alffac", "intfac",
-                  "amp_s1", "fd_stretch", "fd_ratio", "sigma_ratio"):
+                  "amp_s1", "fd_stretch", "fd_ratio", "sigma_ratio", "omega_ic1"):
             setattr(self, k, float(c[k]))
         for k in ("mode", "ktops", "kbots", "ktopv", "kbotv", "kbotb", "init_s1",
                   "init_v1", "fd_order", "fd_order_bound", "nRotIC"):

@@ -148,6 +148,16 @@ CASES = {
                               l_correct_AMz=True, dtmax=1e-4, time_scheme="BPR353"),
                      dt=1e-4, steps=20, noise=dict(seed=53, amp=1e-3), field_steps=[1, 20],
                      expl_step1=False),
+    # rotating inner core started at omega_ic1 = 2 (initV, init_fields.py:528-532)
+    "rotom8": dict(cfg=dict(l_max=8, n_r_max=17, n_cheb_max=15, n_cheb_ic_max=7,
+                            sigma_ratio=1.0, kbotb=3, nRotIC=1, omega_ic1=2.0, dtmax=2e-4),
+                   dt=2e-4, steps=15, noise=dict(seed=61, amp=1e-2), field_steps=[1, 15],
+                   expl_step1=False),
+    # Boussinesq, stress-free walls with both angular-momentum corrections
+    "amc16": dict(cfg=dict(l_max=16, n_r_max=33, ktopv=1, kbotv=1, l_correct_AMz=True,
+                           l_correct_AMe=True, dtmax=1e-4),
+                  dt=1e-4, steps=20, noise=dict(seed=59, amp=1e-2), field_steps=[1, 20],
+                  expl_step1=False),
 }
 
 FD_CASES = {

@@ -49,7 +49,8 @@ P_TOL_ANEL = 1e-9
                                         ("bprminc4", True), ("condic", True),
                                         ("bprcondic", True), ("rotic", True), ("rotic", False),
                                         ("bprrotic4", True), ("anel21", True), ("anelmag16", True),
-                                        ("bpranel2", True)])
+                                        ("bpranel2", True), ("amc16", True), ("amc16", False),
+                                        ("rotom8", True)])
 def test_step_matches_reference_trajectory(name, fused):
     """fused: one grid-space kernel; otherwise separate FFT / products / FFT kernels.
     The dt each step returns is passed to the next (dtchange: the reference
@@ -66,7 +67,9 @@ def test_step_matches_reference_trajectory(name, fused):
     hydrodynamic benchmark's physics with angular-momentum corrections, a
     magnetic run, BPR353 with minc = 2); the pressure within P_TOL_ANEL (the
     l = 0 pressure's system under the integral condition has a condition number
-    near 3e9: two LAPACK inverses differ by 3e-10 there)."""
+    near 3e9: two LAPACK inverses differ by 3e-10 there).  amc16: Boussinesq,
+    stress-free, with the axial and equatorial angular-momentum corrections.
+    rotom8: a rotating core started at omega_ic1 = 2."""
     case, g = _load(name)
     run = _run(case, fused)
     dt = case["dt"]

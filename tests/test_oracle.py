@@ -35,7 +35,7 @@ P_TOL_ANEL = 1e-9
 
 @pytest.mark.parametrize("name", ["bench16", "c1", "dtchange", "bpr16", "bprdt", "minc2",
                                   "bprminc4", "condic", "bprcondic", "rotic", "bprrotic4",
-                                  "anel21", "anelmag16", "bpranel2"])
+                                  "anel21", "anelmag16", "bpranel2", "amc16", "rotom8"])
 def test_oracle_matches_reference_trajectory(name):
     """Energies every step and fields at the saved steps, from the reference's
     one_step with the returned dt passed to the next step; the dt sequence
@@ -51,7 +51,9 @@ def test_oracle_matches_reference_trajectory(name):
     bpranel2: the anelastic (polytropic) step: the hydrodynamic benchmark's
     physics (strat 5, stress-free walls, angular-momentum corrections), a
     magnetic run (strat 1, gravity ~ r) and BPR353 with minc = 2; the pressure
-    within P_TOL_ANEL."""
+    within P_TOL_ANEL.  amc16: Boussinesq, stress-free, with both
+    angular-momentum corrections.  rotom8: a rotating core started at
+    omega_ic1 = 2."""
     case = json.load(open(os.path.join(GOLD, f"traj_{name}.json")))
     g = np.load(os.path.join(GOLD, f"traj_{name}.npz"))
     run = OracleRun(case["cfg"])

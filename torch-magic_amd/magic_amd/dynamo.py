@@ -959,6 +959,10 @@ class DynamoRun:
             f["b"][lm(1, 0)].copy_(up(b_pol * (rr ** 3 - 4.0 / 3.0 * r_cmb * rr ** 2
                                                + r_icb ** 4 / (3.0 * rr))))
             f["aj"][lm(2, 0)].copy_(up(b_tor * rr * np.sin(math.pi * (rr - r_icb))))
+        if self.rot_ic and p.omega_ic1 != 0.0:
+            # initV (init_fields.py:528-532): the core starts rotating at
+            # omega_ic1, z(l=1, m=0) at the ICB set to match
+            f["z"][lm(1, 0), self.N - 1] = p.omega_ic1 / self.rot_c["c_z10_omega"]
 
     def setup_initial_state(self):
         """Derivatives, old states and implicit terms from the fields (``step_time.py:398-617``)."""
