@@ -53,16 +53,19 @@ NOISE = dict(seed=20261202, amp=1e-4)
 # HBM bytes per launch (read + written) of the main kernels at this workload, from
 # rocprofv3 FETCH_SIZE / WRITE_SIZE passes scaled by a 256 MiB copy (FETCH x 2,
 # WRITE x 1); see profiles/hbm_traffic_l383_nr97_pmc.md.  Grid-step entries are
-# per radial levels per workgroup (the tuner's choice): the FP32 one-level figure
-# is from a pass with one level per workgroup (same loads and stores per level).
+# keyed by the tuner's radial levels per workgroup (0: the plan's own block, four
+# levels for the FP32 grid step at n_phi = 1152; the FP64 step chose one); other
+# level blocks have no measured figure (traffic null).  The FP32 Legendre entries
+# ("mixed") are from the same kernels profiled before the BF16 headline.
 PMC_TRAFFIC_BYTES = {
-    ("grid_step", "mixed", 1): 2.442e9 + 3.722e9,
-    ("grid_step", "mixed", 2): 2.372e9 + 2.013e9,
-    ("grid_step", "fp64", 1): 5.892e9 + 4.642e9,
+    ("grid_step", "mixed", 0): 2.540e9 + 1.791e9,
+    ("grid_step", "fp64", 1): 5.861e9 + 4.554e9,
+    ("leg_syn_vec", "mixed_bf16x3"): 3.444e9 + 2.171e9,
     ("leg_syn_vec", "mixed"): 4.196e9 + 2.192e9,
-    ("leg_syn_vec", "fp64"): 5.718e9 + 4.190e9,
+    ("leg_syn_vec", "fp64"): 5.721e9 + 4.187e9,
+    ("leg_ana_vec", "mixed_bf16x3"): 1.640e9 + 0.601e9,
     ("leg_ana_vec", "mixed"): 1.414e9 + 0.599e9,
-    ("leg_ana_vec", "fp64"): 7.759e9 + 0.620e9,
+    ("leg_ana_vec", "fp64"): 7.779e9 + 0.620e9,
 }
 
 PEAK_FP64_TFLOPS = 78.6
