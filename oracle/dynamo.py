@@ -1,5 +1,5 @@
-"""Chebyshev dynamo step, Boussinesq or anelastic, CNAB2 or BPR353 (oracle
-restatement, FP64 numpy; test infrastructure only).
+"""Dynamo step, Chebyshev or finite differences, Boussinesq or anelastic, CNAB2
+or BPR353 (oracle restatement, FP64 numpy; test infrastructure only).
 
 One :class:`OracleRun` holds one configuration's state (the reference keeps
 it in module globals: ``fields.py:14-67``, ``dt_fields.py:55-123``).
@@ -22,6 +22,10 @@ it in module globals: ``fields.py:14-67``, ``dt_fields.py:55-123``).
   the anelastic operators, implicit terms, conduction state and l=0 pressure
   condition, 1/rho0-weighted kinetic energies and Courant limits, and the
   angular-momentum corrections (``oracle.anelastic``).
+* finite-difference runs (radial_scheme = "FD") — the stretched grid and its
+  stencils, the double-curl poloidal equation, the l=0 pressure, the
+  conduction state and Simpson's rule in the energies (``oracle.fdstep``,
+  ``oracle.radial.simps_weights``).
 * energies — ``get_e_kin_full``/``get_e_mag_oc_full`` (``output.py:111-193``);
   ``energy_components`` adds the masked parts (pinned by single-mode fields
   whose energy the (l, m) parity places in known columns); ``get_dlm``
@@ -43,15 +47,21 @@ import math
 import numpy as np
 
 from .params import Params
-from . import anelastic, horizontal, radial, matrices
+from . import anelastic, fdstep, horizontal, radial, matrices
 from .sht import SHT
 
 
 class OracleRun:
     def __init__(self, cfg: dict):
         p = self.prm = Params(dict(cfg))
-        if p.l_finite_diff or p.raxi != 0.0:
-            raise NotImplementedError("oracle step: Chebyshev radial scheme, no composition")
+        if p.raxi != 0.0:
+            raise NotImplementedError("oracle step: no composition")
+        # finite-difference radial scheme: double-curl poloidal equation
+        # (params.py:143), oracle.fdstep
+        self.fd = p.l_finite_diff
+        if self.fd and p.l_cond_ic:
+            raise NotImplementedError("oracle step: finite differences with an insulating "
+                                      "inner core")
         if p.l_anel and p.sigma_ratio > 0.0:
             raise NotImplementedError("oracle step: anelastic runs with an insulating inner core")
         # anelastic reference state (strat > 0): step_time.py:1215-1475 and the
@@ -77,9 +87,15 @@ class OracleRun:
         self.lmarr = horizontal.lm_arrays(p, self.maps)
         self.sht = SHT(p, self.grid, maps=self.maps)
         self.rad = radial.radial_functions(p)
-        self.mats = radial.cheb_mats(p, self.rad["x"], self.rad["r_cmb"], self.rad["r_icb"])
-        self.D1, self.D2, self.D3 = radial.diff_mats(p, self.mats)
-        self.q = radial.rint_weights(p, self.rad["r_cmb"], self.rad["r_icb"])
+        if self.fd:
+            fm = radial.fd_mats(p, self.rad["r"])
+            self.mats, self.D4 = fm[:4], fm[4]
+            self.D1, self.D2, self.D3 = fm[1:4]
+            self.q = radial.simps_weights(self.rad["r"])
+        else:
+            self.mats = radial.cheb_mats(p, self.rad["x"], self.rad["r_cmb"], self.rad["r_icb"])
+            self.D1, self.D2, self.D3 = radial.diff_mats(p, self.mats)
+            self.q = radial.rint_weights(p, self.rad["r_cmb"], self.rad["r_icb"])
         L, N = p.lm_max, p.n_r_max
         z = lambda: np.zeros((L, N), np.complex128)
         for name in ("w", "dw", "ddw", "z", "dz", "p", "dp", "s", "ds",
@@ -113,6 +129,8 @@ class OracleRun:
         self.m0 = self.maps["lm2m"] == 0
         self.lm_l0 = int(self.maps["lm2"][0, 0])
         self.prof = anelastic.background(p, self.rad)
+        # coefficients -> grid values: DCT-I (Chebyshev) or none (finite differences)
+        self._grid = (lambda c: c) if self.fd else radial.costf
         self.am = (anelastic.am_constants(p, self.rad, self.prof, self.q)
                    if p.l_correct_AMz or p.l_correct_AMe else None)
 
@@ -124,8 +142,11 @@ class OracleRun:
                      "b", "db", "ddb", "aj", "dj", "ddj"):
             getattr(self, name)[:] = 0.0
         if p.l_heat:
-            s0, p0 = (anelastic.ps_cond(p, rad, self.mats, self.prof) if self.anel
-                      else matrices.ps_cond(p, rad, self.mats))
+            if self.fd:
+                s0, p0 = fdstep.ps_cond(p, rad, self.mats, self.prof)
+            else:
+                s0, p0 = (anelastic.ps_cond(p, rad, self.mats, self.prof) if self.anel
+                          else matrices.ps_cond(p, rad, self.mats))
             self.s[0] = s0
             self.p[0] = p0
             r, r_cmb, r_icb = rad["r"], rad["r_cmb"], rad["r_icb"]
@@ -194,13 +215,22 @@ class OracleRun:
         else:
             self.td["s"]["impl"][:] = p.opr * (d2s + 2.0 * or1 * ds - dLh * or2 * self.s)
             self.td["z"]["impl"][:] = dLh * or2 * (d2z - dLh * or2 * self.z)
-        dw, ddw, dddw = (self._dr(self.w, D) for D in (self.D1, self.D2, self.D3))
-        self.dw[:] = dw; self.ddw[:] = ddw
-        dp = self._dr(self.p, self.D1)
-        self.dp[:] = dp
-        self.td["w"]["old"][:] = dLh * or2 * self.w
-        self.td["p"]["old"][:] = -dLh * or2 * dw
-        self._w_p_impl(self.w, dw, ddw, dddw, self.p, dp, self.s)
+        if self.fd:   # double curl (step_time.py:471-488): dp is not recomputed
+            dw, ddw = self._dr(self.w, self.D1), self._dr(self.w, self.D2)
+            self.dw[:] = dw; self.ddw[:] = ddw
+            dddw, ddddw = self._dr(ddw, self.D1), self._dr(ddw, self.D2)
+            self.td["w"]["old"][:] = fdstep.w_old(self, self.w, dw, ddw)
+            wi = self.td["w"]["impl"]
+            wi[:] = fdstep.w_impl(self, self.w, dw, ddw, dddw, ddddw, self.s)
+            wi[self.lm_l0] = 0
+        else:
+            dw, ddw, dddw = (self._dr(self.w, D) for D in (self.D1, self.D2, self.D3))
+            self.dw[:] = dw; self.ddw[:] = ddw
+            dp = self._dr(self.p, self.D1)
+            self.dp[:] = dp
+            self.td["w"]["old"][:] = dLh * or2 * self.w
+            self.td["p"]["old"][:] = -dLh * or2 * dw
+            self._w_p_impl(self.w, dw, ddw, dddw, self.p, dp, self.s)
         if p.l_mag:
             for f, df, ddf, key in ((self.b, self.db, self.ddb, "b"), (self.aj, self.dj, self.ddj, "j")):
                 df[:] = self._dr(f, self.D1)
@@ -368,6 +398,9 @@ class OracleRun:
             S_icb, _ = sht.spat_to_sphertor(vxbt[None], np.zeros_like(vxbt)[None])
             VxBtLM[:, N - 1] = S_icb[:, 0]
         td = get_td(self, AdvrLM, AdvtLM, AdvpLM, dVSrLM, VStLM, VxBrLM, VxBtLM, VxBpLM)
+        if self.fd:   # double curl: no pressure equation (step_time.py:877-891)
+            td["w"] = fdstep.td_w(self, AdvrLM, AdvtLM)
+            td["p"] = np.zeros_like(td["p"])
         for k, v in td.items():
             self._expl_slot(k)[:] = v
         if self.rot_ic:   # Lorentz torque on the inner core (step_time.py:1181-1192)
@@ -456,6 +489,9 @@ class OracleRun:
                               heatLM=heatLM)
         td = get_td(self, AdvrLM, AdvtLM, AdvpLM, dVSrLM, VStLM, VxBrLM, VxBtLM, VxBpLM,
                     heatLM=heatLM)
+        if self.fd:   # step_time.py:1409-1425
+            td["w"] = fdstep.td_w(self, AdvrLM, AdvtLM)
+            td["p"] = np.zeros_like(td["p"])
         for k, v in td.items():
             self._expl_slot(k)[:] = v
         dtrkc, dthkc = courant_check(p, rad, self.grid, vr, vt, vp, br, bt, bp, orho1=pr["orho1"])
@@ -517,6 +553,9 @@ class OracleRun:
         """``step_time.py:1530-1548`` (Boussinesq Chebyshev insulating)."""
         w0 = self.wimp_lin[0]
         p, rad, mats = self.prm, self.rad, self.mats
+        if self.fd:
+            self.inv = fdstep.build(p, rad, tuple(mats) + (self.D4,), self.prof, w0)
+            return
         if self.anel:
             pr = self.prof
             self.inv = dict(s=anelastic.build_s(p, rad, mats, pr, w0),
@@ -563,7 +602,9 @@ class OracleRun:
         for c in (s_cheb, z_cheb):
             c[self.m0] = c[self.m0].real
             c[:, nc:] = 0.0
-        self.s[:] = radial.costf(s_cheb)
+        self.s[:] = self._grid(s_cheb)
+        if self.fd:
+            return self._lm_loop_fd(z_cheb)
         rc = np.zeros((L, 2 * N), np.complex128)
         rc[:, 1:N - 1] = rhs_w[:, 1:N - 1] + self.wimp_lin[0] * p.BuoFac * rgrav[None, 1:N - 1] * self.s[:, 1:N - 1]
         rc[:, N + 1:2 * N - 1] = rhs_p[:, 1:N - 1]
@@ -580,12 +621,12 @@ class OracleRun:
         p_cheb[l0] = self.inv["p0"] @ p0_rhs
         w_cheb[:, nc:] = 0.0
         p_cheb[:, nc:] = 0.0
-        self.z[:] = radial.costf(z_cheb)
+        self.z[:] = self._grid(z_cheb)
         if self.rot_ic:
             z10_icb = self.z[self.maps["lm2"][1, 0], N - 1].real
             self.omega_ic = self.c_z10_omega_ic * z10_icb
-        self.w[:] = radial.costf(w_cheb)
-        self.p[:] = radial.costf(p_cheb)
+        self.w[:] = self._grid(w_cheb)
+        self.p[:] = self._grid(p_cheb)
         D1, D2, D3 = self.D1, self.D2, self.D3
         self.ds[:] = self._dr(self.s, D1); d2s = self._dr(self.s, D2)
         self.dz[:] = self._dr(self.z, D1); d2z = self._dr(self.z, D2)
@@ -624,6 +665,60 @@ class OracleRun:
         if p.l_mag:
             self._update_b()
 
+    def _lm_loop_fd(self, z_sol):
+        """The rest of the lm loop with the double-curl poloidal equation
+        (``update_w_doublecurl.py:395-483``): w from the N x N fourth-order
+        system (rows 0, 1, N-2, N-1 and l = 0 zeroed, buoyancy on rows
+        2..N-3); first and second derivatives D1 w, D2 w, third and fourth
+        D1 and D2 of the second; the l = 0 pressure (Dirichlet row 0); no
+        other pressure and no dp update."""
+        p, rad, pr = self.prm, self.rad, self.prof
+        N = p.n_r_max
+        dLh = self.lmarr["dLh"][:, None]
+        or1, or2 = rad["or1"][None], rad["or2"][None]
+        self.z[:] = z_sol
+        rhs_w = self._rhs("w")
+        b = slice(2, N - 2)
+        rhs_w[:, b] += (self.wimp_lin[0] * p.BuoFac * dLh * or2[:, b] * pr["rgrav"][None, b]
+                        * self.s[:, b])
+        for i in (0, 1, N - 2, N - 1):
+            rhs_w[:, i] = 0.0
+        l0 = self.lm_l0
+        rhs_w[l0] = 0.0
+        w = self._solve(self.inv["w"], rhs_w)
+        w[l0] = 0.0
+        w[self.m0] = w[self.m0].real
+        self.w[:] = w
+        D1, D2 = self.D1, self.D2
+        self.dw[:] = self._dr(self.w, D1); self.ddw[:] = self._dr(self.w, D2)
+        dddw, ddddw = self._dr(self.ddw, D1), self._dr(self.ddw, D2)
+        p0_rhs = np.zeros(N)
+        p0_rhs[1:] = (pr["rho0"][1:] * p.BuoFac * pr["rgrav"][1:] * self.s[l0, 1:].real
+                      + self._expl_slot("w")[l0, 1:].real)
+        self.p[l0] = self.inv["p0"] @ p0_rhs
+        self.ds[:] = self._dr(self.s, D1); d2s = self._dr(self.s, D2)
+        self.dz[:] = self._dr(self.z, D1); d2z = self._dr(self.z, D2)
+        if self.am is not None:   # update_z.py:450-475
+            l1m1 = int(self.maps["lm2"][1, 1]) if p.minc == 1 else -1
+            anelastic.am_correct(p, self.am, self.z, self.dz, d2z,
+                                 int(self.maps["lm2"][1, 0]), l1m1)
+        if not self.dirk:
+            for k in ("s", "z", "w", "p"):
+                self.td[k]["expl"][1][:] = self.td[k]["expl"][0]
+        if self._store():
+            self.td["s"]["old"][:] = self.s
+            self.td["z"]["old"][:] = dLh * or2 * self.z
+            self.td["w"]["old"][:] = fdstep.w_old(self, self.w, self.dw, self.ddw)
+        if self.anel:
+            self._s_z_impl_anel(self.s, self.ds, d2s, self.z, self.dz, d2z)
+        else:
+            self.td["s"]["impl"][:] = p.opr * (d2s + 2.0 * or1 * self.ds - dLh * or2 * self.s)
+            self.td["z"]["impl"][:] = dLh * or2 * d2z - (dLh * or2) ** 2 * self.z
+        self.td["w"]["impl"][:] = fdstep.w_impl(self, self.w, self.dw, self.ddw, dddw, ddddw,
+                                                self.s)
+        if p.l_mag:
+            self._update_b()
+
     def _update_b_coupled(self):
         """``update_b.py:634-801``: one (NT, NT) solve per degree for the outer
         Chebyshev and inner even-Chebyshev coefficients of b and of aj."""
@@ -643,7 +738,7 @@ class OracleRun:
             oc_c[:, p.n_cheb_max:] = 0.0
             ic_c = sol[:, N:N + Nic].copy()
             ic_c[:, p.n_cheb_ic_max:] = 0.0
-            sols.append((radial.costf(oc_c), radial.costf(ic_c)))
+            sols.append((self._grid(oc_c), self._grid(ic_c)))
         (self.b[:], self.b_ic[:]), (self.aj[:], self.aj_ic[:]) = sols
         self.db[:] = self._dr(self.b, self.D1); self.ddb[:] = self._dr(self.b, self.D2)
         self.dj[:] = self._dr(self.aj, self.D1); self.ddj[:] = self._dr(self.aj, self.D2)
@@ -679,8 +774,8 @@ class OracleRun:
         for c in (bc, jc):
             c[self.m0] = c[self.m0].real
             c[:, p.n_cheb_max:] = 0.0
-        self.b[:] = radial.costf(bc)
-        self.aj[:] = radial.costf(jc)
+        self.b[:] = self._grid(bc)
+        self.aj[:] = self._grid(jc)
         self.db[:] = self._dr(self.b, self.D1); self.ddb[:] = self._dr(self.b, self.D2)
         self.dj[:] = self._dr(self.aj, self.D1); self.ddj[:] = self._dr(self.aj, self.D2)
         if not self.dirk:

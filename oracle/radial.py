@@ -136,9 +136,57 @@ def rint(f: np.ndarray, q: np.ndarray):
     return f @ q
 
 
+def simps_weights(r: np.ndarray) -> np.ndarray:
+    """Vector q with simps(f, r) = f @ q (``integration.py:75-122``, linear in f):
+    composite three-point panels on the non-uniform grid for odd N; for even N
+    the mean of (trapezoid on the first interval + panels centred on the even
+    points) and (panels centred on the odd points + trapezoid on the last
+    interval); negated because r decreases."""
+    n = r.shape[0]
+    q = np.zeros(n)
+
+    def panel(i, s):
+        h2, h1 = r[i + 1] - r[i], r[i] - r[i - 1]
+        c = s * (h1 + h2) / 6.0
+        q[i - 1] += c * (2.0 * h1 - h2) / h1
+        q[i] += c * (h1 + h2) ** 2 / (h1 * h2)
+        q[i + 1] += c * (2.0 * h2 - h1) / h2
+
+    if n % 2 == 1:
+        for i in range(1, n - 1, 2):
+            panel(i, -1.0)
+        return q
+    s = -0.5
+    q[0] += s * 0.5 * (r[1] - r[0]); q[1] += s * 0.5 * (r[1] - r[0])
+    for i in range(2, n - 1, 2):
+        panel(i, s)
+    q[n - 1] += s * 0.5 * (r[n - 1] - r[n - 2]); q[n - 2] += s * 0.5 * (r[n - 1] - r[n - 2])
+    for i in range(1, n - 1, 2):
+        panel(i, s)
+    return q
+
+
+def fd_mats(p: Params, r: np.ndarray):
+    """(rMat = I, drMat, d2rMat, d3rMat, d4rMat) of the finite-difference scheme
+    (``finite_differences.py:279-331``), dense."""
+    from . import fd
+    c = fd.fd_coeffs(r, p.fd_order, p.fd_order_bound)
+    return fd.fd_der_mats(r, c, p.fd_order, p.fd_order_bound)
+
+
 def radial_functions(p: Params):
-    """Boussinesq radial profiles and CFL spacings (``radial_functions.py:18-69,189-202``)."""
-    r, x, r_cmb, r_icb = cheb_grid(p)
+    """Boussinesq radial profiles and CFL spacings (``radial_functions.py:18-69,189-202``)
+    on the Chebyshev grid, or on the stretched finite-difference grid
+    (``finite_differences.py:62-127,342-351``: rnorm = boundary_fac = 1)."""
+    if p.l_finite_diff:
+        from . import fd
+        r_cmb = 1.0 / (1.0 - p.radratio)
+        r_icb = r_cmb - 1.0
+        r = fd.fd_grid(p.n_r_max, r_icb, r_cmb, p.fd_stretch, p.fd_ratio)
+        x, rnorm, bfac = None, 1.0, 1.0
+    else:
+        r, x, r_cmb, r_icb = cheb_grid(p)
+        rnorm, bfac = math.sqrt(2.0 / (p.n_r_max - 1)), 0.5
     or1 = 1.0 / r
     dr = r[:-1] - r[1:]
     delxr2 = np.empty(p.n_r_max)
@@ -148,4 +196,4 @@ def radial_functions(p: Params):
     return dict(r=r, x=x, r_cmb=r_cmb, r_icb=r_icb, or1=or1, or2=or1 ** 2, or3=or1 ** 3,
                 or4=or1 ** 4, rgrav=r / r_cmb,
                 delxh2=r ** 2 / float(p.l_max * (p.l_max + 1)), delxr2=delxr2,
-                rnorm=math.sqrt(2.0 / (p.n_r_max - 1)))
+                rnorm=rnorm, bfac=bfac)

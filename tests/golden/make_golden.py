@@ -158,6 +158,41 @@ CASES = {
                            l_correct_AMe=True, dtmax=1e-4),
                   dt=1e-4, steps=20, noise=dict(seed=59, amp=1e-2), field_steps=[1, 20],
                   expl_step1=False),
+    # finite-difference radial scheme (double-curl poloidal equation): the physics
+    # of the reference's FD step tests (test_fd_step.py, test_fd4_step.py,
+    # test_fd_anel.py, test_fd_anel_mhd.py) with seeded noise
+    "fd2mhd": dict(cfg=dict(l_max=16, n_r_max=33, radial_scheme="FD", fd_order=2,
+                            fd_order_bound=2, fd_stretch=0.3, fd_ratio=0.1, ra=1e5, ek=1e-3,
+                            pr=1.0, prmag=5.0, dtmax=1e-4, alpha=0.6, init_s1=404, amp_s1=0.1,
+                            ktopv=2, kbotv=2),
+                   dt=1e-4, steps=20, noise=dict(seed=67, amp=1e-3), field_steps=[1, 20],
+                   expl_step1=True, diag_steps=[20]),
+    "fd4mhd": dict(cfg=dict(l_max=16, n_r_max=40, radial_scheme="FD", fd_order=4,
+                            fd_order_bound=2, fd_stretch=0.3, fd_ratio=0.1, ra=1e5, ek=1e-3,
+                            pr=1.0, prmag=5.0, dtmax=1e-4, alpha=0.6, init_s1=404, amp_s1=0.1,
+                            ktopv=2, kbotv=2),
+                   dt=1e-4, steps=20, noise=dict(seed=71, amp=1e-3), field_steps=[1, 20],
+                   expl_step1=False, diag_steps=[20]),
+    "fdanel": dict(cfg=dict(l_max=16, n_r_max=33, radial_scheme="FD", fd_order=2,
+                            fd_order_bound=2, fd_stretch=0.3, fd_ratio=0.1, strat=0.1,
+                            polind=2.0, g0=0.0, g1=0.0, g2=1.0, mode=1, ktopv=1, kbotv=1,
+                            ra=1.48638035e5, ek=1e-3, pr=1.0, dtmax=1e-4, alpha=0.6,
+                            init_s1=1010, amp_s1=0.01),
+                   dt=1e-4, steps=20, noise=dict(seed=73, amp=1e-3), field_steps=[1, 20],
+                   expl_step1=True, diag_steps=[20]),
+    "fdanelmhd": dict(cfg=dict(l_max=16, n_r_max=33, radial_scheme="FD", fd_order=2,
+                               fd_order_bound=2, fd_stretch=0.3, fd_ratio=0.1, strat=0.1,
+                               polind=2.0, g0=0.0, g1=0.0, g2=1.0, mode=0, ktopv=2, kbotv=2,
+                               ra=1e5, ek=1e-3, pr=1.0, prmag=5.0, dtmax=1e-4, alpha=0.6,
+                               init_s1=404, amp_s1=0.1),
+                      dt=1e-4, steps=20, noise=dict(seed=79, amp=1e-3), field_steps=[1, 20],
+                      expl_step1=False),
+    "fdbpr": dict(cfg=dict(l_max=16, n_r_max=33, radial_scheme="FD", fd_order=2,
+                           fd_order_bound=2, fd_stretch=0.3, fd_ratio=0.1, ra=1e5, ek=1e-3,
+                           pr=1.0, prmag=5.0, dtmax=1e-4, init_s1=404, amp_s1=0.1,
+                           time_scheme="BPR353"),
+                  dt=1e-4, steps=10, noise=dict(seed=83, amp=1e-3), field_steps=[1, 10],
+                  expl_step1=False),
 }
 
 FD_CASES = {

@@ -35,7 +35,8 @@ P_TOL_ANEL = 1e-9
 
 @pytest.mark.parametrize("name", ["bench16", "c1", "dtchange", "bpr16", "bprdt", "minc2",
                                   "bprminc4", "condic", "bprcondic", "rotic", "bprrotic4",
-                                  "anel21", "anelmag16", "bpranel2", "amc16", "rotom8"])
+                                  "anel21", "anelmag16", "bpranel2", "amc16", "rotom8",
+                                  "fd2mhd", "fd4mhd", "fdanel", "fdanelmhd", "fdbpr"])
 def test_oracle_matches_reference_trajectory(name):
     """Energies every step and fields at the saved steps, from the reference's
     one_step with the returned dt passed to the next step; the dt sequence
@@ -53,7 +54,11 @@ def test_oracle_matches_reference_trajectory(name):
     magnetic run (strat 1, gravity ~ r) and BPR353 with minc = 2; the pressure
     within P_TOL_ANEL.  amc16: Boussinesq, stress-free, with both
     angular-momentum corrections.  rotom8: a rotating core started at
-    omega_ic1 = 2."""
+    omega_ic1 = 2.  fd2mhd, fd4mhd, fdanel, fdanelmhd, fdbpr: the
+    finite-difference radial scheme with the double-curl poloidal equation
+    (second- and fourth-order stencils on the stretched grid, an even n_r with
+    Simpson's even-N rule in the energies, anelastic hydrodynamic and magnetic
+    runs, BPR353)."""
     case = json.load(open(os.path.join(GOLD, f"traj_{name}.json")))
     g = np.load(os.path.join(GOLD, f"traj_{name}.npz"))
     run = OracleRun(case["cfg"])
@@ -175,6 +180,24 @@ def test_rint_exact_on_polynomials():
     r, a, b = rad["r"], rad["r_icb"], rad["r_cmb"]
     for k in range(0, 10):
         assert abs(radial.rint(r ** k, q) - (b ** (k + 1) - a ** (k + 1)) / (k + 1)) < 1e-12 * b ** (k + 1)
+
+
+@pytest.mark.parametrize("n", [9, 10, 33, 40])
+def test_simpson_weights_exact_on_low_degree(n):
+    """simps is exact for linear functions on any grid (both parities) and for
+    quadratics with an odd number of points (three-point panels); checked on
+    the stretched finite-difference grid against the closed-form integrals."""
+    from oracle import fd
+    r = fd.fd_grid(n, 0.5, 1.5, 0.3, 0.1)
+    q = radial.simps_weights(r)
+    a, b = r[-1], r[0]
+    assert q @ (2.0 + 3.0 * r) == pytest.approx(2.0 * (b - a) + 1.5 * (b * b - a * a), rel=1e-14)
+    quad = q @ (r * r)
+    exact = (b ** 3 - a ** 3) / 3.0
+    if n % 2:
+        assert quad == pytest.approx(exact, rel=1e-14)
+    else:
+        assert quad == pytest.approx(exact, rel=1e-3) and quad != pytest.approx(exact, rel=1e-13)
 
 
 def test_conduction_state_analytic():
