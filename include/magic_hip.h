@@ -626,6 +626,47 @@ int mh_band_solve(const double* abd, const int* pivot, const double* fac_row,
                   const double* fac_col, const int* lm2l, const int* rows, const double* rhs,
                   double* sol, int L, int n, int kl, int ku, void* stream);
 
+/* ------------------------------------------------------------ double-curl (FD) step */
+typedef struct {
+  MhLmCoef lc;
+  MhRadCoef rc;
+  /* reference-state profiles, (N,) each; NULL selects the Boussinesq value
+   * (beta, its derivatives and those of ln visc 0; orho1, rho0, visc 1) */
+  const double *beta, *dbeta, *ddbeta, *orho1, *rho0, *visc, *dLvisc, *ddLvisc;
+  const double* grav;      /* gravity profile rgrav (radial_functions.py) */
+  /* mh_td_dc */
+  const double *Advr;      /* (L, N) radial advection */
+  const double *dAdvt_dr;  /* (L, N) d/dr (orho1 r^2 Advt), Advt zeroed on the boundaries */
+  const double *w, *dw, *ddw, *dddw, *ddddw, *z, *dz, *s;
+  double *ew, *ep;         /* explicit terms of w (written) and p (zeroed) */
+  /* mh_buoy_dc */
+  double* rwp;             /* (L, 2N) right-hand side: w half, then the pressure half */
+  const double* ew0;       /* explicit w term of the step (l = 0 pressure) */
+  const double* wdev;      /* optional: wdev[4] = wimp_lin[0] BuoFac replaces fac */
+  /* mh_impl_dc (old_w, old_p, impl_p may be NULL) */
+  double *old_w, *impl_w, *old_p, *impl_p;
+  double CorFac, BuoFac, fac;
+  int lm_l0, pad;
+} MhDcArgs;
+
+/* Double-curl poloidal equation (params.py:143, radial_scheme = "FD"):
+ *   mh_td_dc: explicit w term get_dwdt_double_curl (get_td.py:91-147) with
+ *     finish_exp_pol (update_wp.py:371-384), l = 0 in the standard form; p term 0;
+ *   mh_buoy_dc: buoyancy wimp_lin[0] BuoFac dLh/r^2 rgrav s on rows 2..N-3 and
+ *     the l = 0 pressure right-hand side (update_w_doublecurl.py:413-459);
+ *   mh_impl_dc: get_w_old and get_w_impl (update_w_doublecurl.py:309-393), the
+ *     pressure's old state and implicit terms zeroed. */
+int mh_td_dc(const MhDcArgs* args, void* stream);
+int mh_buoy_dc(const MhDcArgs* args, void* stream);
+int mh_impl_dc(const MhDcArgs* args, void* stream);
+
+/* out[i] = A0[i] + w A1[i], i < n; w = *wdev when wdev is not NULL; nothing
+ * unless *when != 0 when `when` is not NULL.  Operators affine in the
+ * implicit weight (the finite-difference systems, assembled once on the host)
+ * take their new matrices from it on a dt change (build_s/z/w/b_matrices). */
+int mh_affine_f64(const double* A0, const double* A1, double* out, long n, double w,
+                  const double* wdev, const int* when, void* stream);
+
 /* Sizes of the argument structs as compiled (binding checks). */
 int mh_struct_sizes(int* out, int n);
 

@@ -50,7 +50,9 @@ P_TOL_ANEL = 1e-9
                                         ("bprcondic", True), ("rotic", True), ("rotic", False),
                                         ("bprrotic4", True), ("anel21", True), ("anelmag16", True),
                                         ("bpranel2", True), ("amc16", True), ("amc16", False),
-                                        ("rotom8", True)])
+                                        ("rotom8", True), ("fd2mhd", True), ("fd2mhd", False),
+                                        ("fd4mhd", True), ("fdanel", True), ("fdanelmhd", True),
+                                        ("fdbpr", True)])
 def test_step_matches_reference_trajectory(name, fused):
     """fused: one grid-space kernel; otherwise separate FFT / products / FFT kernels.
     The dt each step returns is passed to the next (dtchange: the reference
@@ -69,7 +71,11 @@ def test_step_matches_reference_trajectory(name, fused):
     l = 0 pressure's system under the integral condition has a condition number
     near 3e9: two LAPACK inverses differ by 3e-10 there).  amc16: Boussinesq,
     stress-free, with the axial and equatorial angular-momentum corrections.
-    rotom8: a rotating core started at omega_ic1 = 2."""
+    rotom8: a rotating core started at omega_ic1 = 2.  fd2mhd / fd4mhd / fdanel /
+    fdanelmhd / fdbpr: the finite-difference radial scheme with the
+    double-curl poloidal equation (second- and fourth-order stencils on the
+    stretched grid, n_r even in fd4mhd, anelastic hydrodynamic and magnetic
+    runs, BPR353)."""
     case, g = _load(name)
     run = _run(case, fused)
     dt = case["dt"]
@@ -125,9 +131,13 @@ def test_mixed_precision_step_tracks_reference(name, precision, etol, ftol):
     print(f"{name} {precision}: worst energy rel. error {worst[0]:.2e}, field {worst[1]:.2e}")
 
 
-def test_first_step_explicit_terms():
-    """Explicit terms after the first radial loop (get_nl + get_td in context)."""
-    case, g = _load("bench16")
+@pytest.mark.parametrize("name", ["bench16", "fd2mhd", "fdanel"])
+def test_first_step_explicit_terms(name):
+    """Explicit terms after the first radial loop (get_nl + get_td in context);
+    fd2mhd and fdanel: the double-curl poloidal term (Coriolis form with w'',
+    z' and the radial derivative of the horizontal advection) and a zero
+    pressure term."""
+    case, g = _load(name)
     run = _run(case)
     run.one_step(1, case["dt"])
     for k in ("s", "z", "w", "p", "b", "j"):

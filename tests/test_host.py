@@ -410,3 +410,44 @@ def test_anelastic_operators_match_oracle(cfg):
     got = ms[l, :N, :3 * N].T @ rhs[:N]
     ref = np.concatenate([fs, o.D1 @ fs, o.D2 @ fs])
     assert np.abs(got - ref).max() <= 1e-9 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("cfg", [
+    dict(l_max=8, n_r_max=33, radial_scheme="FD", fd_order=2, fd_order_bound=2, prmag=5.0),
+    dict(l_max=8, n_r_max=40, radial_scheme="FD", fd_order=4, fd_order_bound=2, prmag=5.0,
+         ktopv=1, kbotv=1),
+    dict(l_max=8, n_r_max=33, radial_scheme="FD", strat=0.5, polind=2.0, g0=0.0, g1=0.0,
+         g2=1.0, mode=1, ktopv=1, kbotv=1)])
+def test_finite_difference_host_operators_match_oracle(cfg):
+    """Finite-difference set-up of the device step against the oracle (itself
+    pinned by reference FD trajectories): grid, dense stencil matrices, Simpson
+    weights, inverses of A0 + w A1 for S, Z, W (double curl), B, J, the l = 0
+    pressure inverse and the conduction state."""
+    from magic_amd import anelastic, fdstep
+    from oracle import anelastic as oan, fdstep as ofd, radial as orad
+    from oracle.params import Params as OP
+    op, p = OP(cfg), Params(cfg)
+    orr = orad.radial_functions(op)
+    fm = orad.fd_mats(op, orr["r"])
+    oprof = oan.background(op, orr)
+    rad = fdstep.radial(p)
+    assert np.array_equal(rad["r"], orr["r"])
+    for a, b in zip((rad["D1"], rad["D2"], rad["D3"], rad["D4"]), fm[1:]):
+        assert np.abs(a - b).max() <= 1e-13 * np.abs(b).max()
+    assert np.abs(rad["q"] - orad.simps_weights(orr["r"])).max() < 1e-15
+    prof = anelastic.background(p, rad) if p.l_anel else fdstep.boussinesq_profile(rad)
+    w0 = 3e-5
+    inv = ofd.build(op, orr, fm, oprof, w0)
+    ops = fdstep.affine_operators(p, rad, prof)
+    (A0, A1), (W0, W1) = ops["szbj"], ops["w"]
+    for k, name in enumerate(("s", "z", "b", "j")[:A0.shape[0]]):
+        for l in range(0 if name == "s" else 1, p.l_max + 1):
+            M = np.linalg.inv(A0[k, l] + w0 * A1[k, l])
+            assert np.abs(M - inv[name][l]).max() <= 1e-10 * np.abs(inv[name][l]).max(), (name, l)
+    for l in range(1, p.l_max + 1):
+        M = np.linalg.inv(W0[l] + w0 * W1[l])
+        assert np.abs(M - inv["w"][l]).max() <= 1e-9 * np.abs(inv["w"][l]).max(), ("w", l)
+    assert np.abs(fdstep.p0_inverse(p, rad, prof) - inv["p0"]).max() < 1e-12
+    s0, p0 = fdstep.conduction_state(p, rad, prof)
+    os0, op0 = ofd.ps_cond(op, orr, fm, oprof)
+    assert np.abs(s0 - os0).max() < 1e-12 and np.abs(p0 - op0).max() < 1e-12 * np.abs(op0).max()

@@ -104,7 +104,9 @@ def _check_rebuild_calls(calls, ext):
 
 
 def _check_row_jobs(run, ext):
-    for key in ("jobs1", "jobs2", "jobs_td", "jobs_setup"):
+    for key in ("jobs1", "jobs2", "jobs_td", "jobs_setup", "jobs_setup2"):
+        if not hasattr(run, key):
+            continue
         dj, tile_lists = getattr(run, key)
         jobs = np.frombuffer(dj.numpy().tobytes(), dtype=dyn._ROWJOB_DTYPE)
         for tc, (tl, nt) in tile_lists.items():
@@ -418,3 +420,63 @@ def test_anelastic_launches_stay_inside_their_tensors(cpu_standin, cfg):
         assert "mh_am_correct" in seen
     if run.p0_integral:
         assert "mh_p0_integral" in seen
+
+
+@pytest.mark.parametrize("cfg", [
+    dict(l_max=12, n_r_max=33, radial_scheme="FD", fd_order=2, fd_order_bound=2, dtmax=1e-4),
+    dict(l_max=12, n_r_max=40, radial_scheme="FD", fd_order=4, fd_order_bound=2, dtmax=1e-4,
+         minc=2, time_scheme="BPR353"),
+    dict(l_max=12, n_r_max=33, radial_scheme="FD", strat=0.5, polind=2.0, g2=1.0, mode=1,
+         ktopv=1, kbotv=1, dtmax=1e-4)])
+def test_finite_difference_launches_stay_inside_their_tensors(cpu_standin, cfg):
+    """Finite-difference (double-curl) step: the rebuild (affine assembly,
+    inverses, folds, row fills), the row-operator jobs (including the second
+    set-up pass over w''), and the double-curl explicit, buoyancy and
+    implicit-term launches of every recorded plan read and write inside the
+    run's tensors; the W row operator keeps its fixed l = 0 pressure block."""
+    with _lib.record() as rec:
+        run = dyn.DynamoRun(cfg, device="cpu")
+        run.initialize_dt(cfg["dtmax"])
+        run._rebuild_launches(0.5e-4)
+    calls = rec.calls
+    assert run.dc
+    ext = _extents(run)
+    _check_row_jobs(run, ext)
+    dj, tile_lists = run.jobs_setup2
+    assert len(np.frombuffer(dj.numpy().tobytes(), dtype=dyn._ROWJOB_DTYPE)) == 1
+    N, L, lp1 = run.N, run.L, run.p.l_max + 1
+    n = _check_rebuild_calls(calls, ext)
+    aff = [a for _, name, a in calls if name == "mh_affine_f64"]
+    assert n >= 6 and len(aff) == 2
+    for A0, A1, out, cnt in (a[:4] for a in aff):
+        for v in (A0, A1, out):
+            _inside(ext, v, v + 8 * cnt, "affine operand")
+    mt0 = run.MT["wp"][0].cpu().numpy()
+    assert np.abs(mt0[N:2 * N, 4 * N:5 * N]).max() > 0 and np.abs(mt0[:N]).max() == 0
+    stages = (1, 2, 3, 4) if run.dirk else (0,)
+    seen = set()
+    for k in stages:
+        run.stage = k
+        for kind in (("radial", "lm") if k <= 3 else ("lm",)):
+            _, plan = run._plan(kind)
+            for _, name, args in plan:
+                if name not in ("mh_td_dc", "mh_buoy_dc", "mh_impl_dc"):
+                    continue
+                seen.add(name)
+                a = C.cast(args[0], C.POINTER(_lib.DcArgs)).contents
+                assert (a.rc.N, a.lc.L) == (N, L)
+                full = [a.w, a.dw, a.ddw, a.z, a.dz, a.s]
+                if name == "mh_td_dc":
+                    full += [a.Advr, a.dAdvt_dr, a.ew, a.ep]
+                elif name == "mh_buoy_dc":
+                    full += [a.ew0]
+                    _inside(ext, a.rwp, a.rwp + 16 * L * 2 * N, "buoyancy rhs")
+                else:
+                    full += [a.dddw, a.ddddw, a.impl_w, a.old_w, a.old_p, a.impl_p]
+                for v in full:
+                    _inside(ext, v, v + 16 * L * N, name)
+                _inside(ext, a.grav, a.grav + 8 * N, "gravity")
+                if run.anel:
+                    for v in (a.beta, a.ddbeta, a.orho1, a.rho0, a.visc, a.ddLvisc):
+                        _inside(ext, v, v + 8 * N, "profile")
+    assert seen == {"mh_td_dc", "mh_buoy_dc", "mh_impl_dc"}

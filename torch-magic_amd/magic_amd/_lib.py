@@ -203,10 +203,20 @@ class StepCtlArgs(C.Structure):
                 ("BuoFac", D)]
 
 
+class DcArgs(C.Structure):
+    _fields_ = [("lc", LmCoef), ("rc", RadCoef), ("beta", P), ("dbeta", P), ("ddbeta", P),
+                ("orho1", P), ("rho0", P), ("visc", P), ("dLvisc", P), ("ddLvisc", P),
+                ("grav", P), ("Advr", P), ("dAdvt_dr", P), ("w", P), ("dw", P), ("ddw", P),
+                ("dddw", P), ("ddddw", P), ("z", P), ("dz", P), ("s", P), ("ew", P), ("ep", P),
+                ("rwp", P), ("ew0", P), ("wdev", P), ("old_w", P), ("impl_w", P),
+                ("old_p", P), ("impl_p", P), ("CorFac", D), ("BuoFac", D), ("fac", D),
+                ("lm_l0", I), ("pad", I)]
+
+
 _STRUCT_ORDER = [SynArgs, AnaArgs, FftArgs, NlArgs, RowJob, TdArgs, RhsArgs, BuoyArgs,
                  ImplArgs, EnergyArgs, BuildArgs, GemmArgs, GridArgs, StepState, StepCtlArgs,
                  IcRhsArgs, IcImplArgs, IcRotArgs, NlAnelArgs, DphiArgs, EntropyAnelArgs,
-                 ImplAnelArgs, AmArgs, P0IntArgs]
+                 ImplAnelArgs, AmArgs, P0IntArgs, DcArgs]
 
 _SIGS = {
     "mh_plm_tables": [I, I, I, I, I, P, P, P, P, P, P, P, P, P],
@@ -244,6 +254,10 @@ _SIGS = {
     "mh_fill_f64": [P, L, I, L, D, P, P],
     "mh_band_factor": [P, P, P, I, I, I, I, P],
     "mh_band_solve": [P, P, P, P, P, P, P, P, I, I, I, I, P],
+    "mh_td_dc": [P, P],
+    "mh_buoy_dc": [P, P],
+    "mh_impl_dc": [P, P],
+    "mh_affine_f64": [P, P, P, L, D, P, P, P],
     "mh_struct_sizes": [P, I],
 }
 

@@ -40,11 +40,15 @@ def background(p, rad: dict) -> dict:
     dgrav = g1 / rc - 2.0 * g2 * rc * rc * or3
     beta = -pi * diss * rgrav / temp0
     dbeta = -pi * diss / temp0 ** 2 * (dgrav * temp0 + diss * rgrav ** 2)
+    ddbeta = -pi * diss / temp0 ** 3 * (
+        temp0 * (6.0 * g2 * rc * rc * or2 * or2 * temp0 + diss * rgrav * dgrav)
+        + 2.0 * diss * rgrav * (diss * rgrav ** 2 + dgrav * temp0))
     rho0 = temp0 ** pi
     vhf = diss * p.pr / p.ra
     return dict(rgrav=rgrav, temp0=temp0, rho0=rho0, orho1=1.0 / rho0, otemp1=1.0 / temp0,
                 alpha0=1.0 / temp0, ogrun=np.full(n, pi), beta=beta, dbeta=dbeta,
-                dLtemp0=-diss * rgrav / temp0, visc=np.ones(n), dLvisc=np.zeros(n),
+                ddbeta=ddbeta, dLtemp0=-diss * rgrav / temp0, visc=np.ones(n),
+                dLvisc=np.zeros(n), ddLvisc=np.zeros(n),
                 kappa=np.ones(n), dLkappa=np.zeros(n), DissNb=diss, ViscHeatFac=vhf,
                 OhmLossFac=vhf / (p.ek * p.prmag ** 2) if p.l_mag else 0.0, ThExpNb=1.0)
 

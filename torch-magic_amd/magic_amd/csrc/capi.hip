@@ -12,6 +12,7 @@ extern "C" int mh_struct_sizes(int* out, int n) {
       (int)sizeof(MhIcRhsArgs), (int)sizeof(MhIcImplArgs), (int)sizeof(MhIcRotArgs),
       (int)sizeof(MhNlAnelArgs), (int)sizeof(MhDphiArgs), (int)sizeof(MhEntropyAnelArgs),
       (int)sizeof(MhImplAnelArgs), (int)sizeof(MhAmArgs), (int)sizeof(MhP0IntArgs),
+      (int)sizeof(MhDcArgs),
   };
   const int ns = (int)(sizeof(sizes) / sizeof(sizes[0]));
   for (int i = 0; i < n && i < ns; ++i) out[i] = sizes[i];

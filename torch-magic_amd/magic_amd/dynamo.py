@@ -254,7 +254,7 @@ class DynamoRun:
         # every (L, cols) complex array of the run is a view of one zeroed pool,
         # each starting on a 256-byte boundary
         slot = lambda cols: -(-(L * cols) // 16) * 16
-        n_arr = len(_FIELDS) + 3 + 4 * len(_SYS) + 4 + 10
+        n_arr = len(_FIELDS) + 4 + 4 * len(_SYS) + 4 + 11
         self._pool = torch.zeros(n_arr * slot(N) + slot(2 * N), dtype=C128, device=dev)
         used = [0]
 
@@ -264,7 +264,7 @@ class DynamoRun:
             return a
 
         self.f = {k: z() for k in _FIELDS}
-        self.tmp = {k: z() for k in ("d2s", "d2z", "dddw")}
+        self.tmp = {k: z() for k in ("d2s", "d2z", "dddw", "ddddw")}
         self.old = {k: z() for k in _SYS}
         self.impl = {k: z() for k in _SYS}
         self.expl_cur = {k: z() for k in _SYS}
@@ -273,7 +273,7 @@ class DynamoRun:
         self.rhs["wp"] = z(2 * N)
         # radial-loop spectral outputs (boundary columns stay zero)
         self.nl = {k: z() for k in ("Advr", "Advt", "Advp", "dVSr", "VSt", "VxBr", "VxBt",
-                                    "VxBp", "dVSr_dr", "dVxBt_dr")}
+                                    "VxBp", "dVSr_dr", "dVxBt_dr", "dAdvt_dr")}
         # grid-space work (FP64) between Fourier buffers F[c][m][t][b].  Fused
         # (default): one kernel for inverse FFT + products + forward FFT, no grid in
         # HBM.  Separate: grids G[c][b][t][phi] and three kernels.
@@ -302,6 +302,12 @@ class DynamoRun:
         assert used[0] == self._pool.numel()
         if self.anel:
             self._anel_buffers()
+        # finite-difference radial scheme: the double-curl poloidal equation
+        # (params.py:143); operators affine in the implicit weight, assembled on
+        # the host once (magic_amd.fdstep), refreshed on the device per dt change
+        self.dc = bool(p.l_finite_diff)
+        if self.dc:
+            self._dc_setup()
         self.cfl = torch.zeros(2 * N, dtype=torch.int64, device=dev)
         self.am = None
         if p.l_correct_AMz or p.l_correct_AMe:   # update_z.py:450-475
@@ -420,6 +426,97 @@ class DynamoRun:
         self.n_step = 0          # steps taken
         self._ctl = None         # device step control buffers (run_steps / step_async)
         self.expl_pending = False  # a radial loop's explicit terms await the lm loop
+
+    def _dc_setup(self):
+        """Profiles of the double-curl kernels on the device and the derivative
+        operator of the flux d/dr (orho1 r^2 Advt) (boundary values dropped)."""
+        from . import fdstep
+        rad, N, dev = self.res.rad, self.N, self.device
+        self.dc_prof = self.prof if self.anel else fdstep.boussinesq_profile(rad)
+        pr = self.dc_prof
+        up = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=F64).to(dev)
+        self.d_dc = {k: up(pr[k]) for k in ("beta", "dbeta", "ddbeta", "orho1", "rho0", "visc",
+                                             "dLvisc", "ddLvisc", "rgrav")}
+        wt = pr["orho1"] * rad["r"] ** 2
+        wt[0] = wt[N - 1] = 0.0
+        Op = rad["D1"] * wt[None, :]
+        Rp = -(-N // 64) * 64
+        K4 = -(-N // 4) * 4
+        mt = np.zeros((K4, Rp))
+        mt[:N, :N] = Op.T
+        self.MT_dvxvh = (up(mt), N, Rp)
+
+    def _dc_args(self):
+        a = _lib.DcArgs()
+        a.lc, a.rc = self._lmcoef(), self._radcoef()
+        d, f, t = self.d_dc, self.f, self.tmp
+        if self.anel:
+            for k in ("beta", "dbeta", "ddbeta", "orho1", "rho0", "visc", "dLvisc", "ddLvisc"):
+                setattr(a, k, ptr(d[k]))
+        a.grav = ptr(d["rgrav"])
+        a.w, a.dw, a.ddw, a.dddw, a.ddddw = (ptr(f["w"]), ptr(f["dw"]), ptr(f["ddw"]),
+                                             ptr(t["dddw"]), ptr(t["ddddw"]))
+        a.z, a.dz, a.s = ptr(f["z"]), ptr(f["dz"]), ptr(f["s"])
+        a.CorFac, a.BuoFac, a.lm_l0 = self.p.CorFac, self.p.BuoFac, 0
+        return a
+
+    def _dc_storage(self):
+        """Host-assembled operators A0 + w A1 (S, Z (B, J) stacked like mat_nn;
+        the double-curl W in the first (l_max+1) N^2 of mat_wp) and the fixed
+        l = 0 pressure block of the W row operator."""
+        from . import fdstep
+        p, rad, N, dev = self.p, self.res.rad, self.N, self.device
+        lp1 = p.l_max + 1
+        ops = fdstep.affine_operators(p, rad, self.dc_prof)
+        up = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=F64).to(dev)
+        (a0, a1), (w0, w1) = ops["szbj"], ops["w"]
+        self.dc_ops = dict(a0=up(a0.reshape(-1)), a1=up(a1.reshape(-1)), w0=up(w0.reshape(-1)),
+                           w1=up(w1.reshape(-1)))
+        self.mat_w = self.mat_wp.view(-1)[:lp1 * N * N].view(lp1, N, N)
+        D1, D2 = rad["D1"], rad["D2"]
+        self.Op4dc = up(np.concatenate([np.eye(N), D1, D2, D1 @ D2]))   # (4N, N)
+        self.Op22 = up(D2 @ D2)
+        # MT[l][k][r] of degree 0: p(l=0) = inv_p0 @ (pressure half of the rhs)
+        mt0 = np.zeros((self.K4wp, self.Rpad6))
+        mt0[N:2 * N, 4 * N:5 * N] = fdstep.p0_inverse(p, rad, self.dc_prof).T
+        self.MT["wp"][0].copy_(up(mt0))
+
+    def _rebuild_launches_dc(self, wimp_lin0, ctl=None):
+        """Finite-difference rebuild: A0 + w A1 for every system, batched
+        Gauss-Jordan inverses (row equilibration; the double-curl W also by
+        columns, as the reference preconditions it), and the folds with the
+        identity and D1, D2 (S, Z, B, J) and with I, D1, D2, D1 D2 and D2 D2 (W)
+        into the row operators; rows 1 and N-2 of the W input (its boundary
+        conditions on w') are dropped, as the reference zeroes those rhs rows."""
+        N, lp1, ns = self.N, self.p.l_max + 1, self.nsys
+        st = stream_handle()
+        wdev, when = (None, None) if ctl is None else ctl
+        info = lambda key: ptr(self.gj_info[_INFO_ROWS.index(key)])
+        o = self.dc_ops
+        _lib.call("mh_affine_f64", ptr(o["a0"]), ptr(o["a1"]), ptr(self.mat_nn),
+                  ns * lp1 * N * N, wimp_lin0, wdev, when, st)
+        _lib.call("mh_gj_inverse", ptr(self.mat_nn), ptr(self.mat_nn), ptr(self.gj_work),
+                  ptr(self.gj_scal), N, ns * lp1, 1, info("s"), when, st)
+        self._gemm(ptr(self.mat_nn), 1, N, N * N, ptr(self.Op3), 1, N, 0,
+                   ptr(self.MT4), self.Rpad3, 1, self.K4 * self.Rpad3, N, 3 * N, N, ns * lp1, when)
+        sb = lp1 * self.K4 * self.Rpad3
+        self._zero(self.MT4[1:ns, 0], when, count=ns - 1, stride=sb)
+        # W, degrees 1..l_max (degree 0 holds the fixed l = 0 pressure block)
+        nn, nb = N * N, lp1 - 1
+        mw = ptr(self.mat_w) + 8 * nn
+        _lib.call("mh_affine_f64", ptr(o["w0"]) + 8 * nn, ptr(o["w1"]) + 8 * nn, mw, nb * nn,
+                  wimp_lin0, wdev, when, st)
+        _lib.call("mh_gj_inverse", mw, mw, ptr(self.gj_work), ptr(self.gj_scal), N, nb, 2,
+                  info("wp") + 4, when, st)
+        MT = self.MT["wp"]
+        sbw = self.K4wp * self.Rpad6
+        mt1 = ptr(MT[1])
+        self._gemm(mw, 1, N, nn, ptr(self.Op4dc), 1, N, 0, mt1, self.Rpad6, 1, sbw,
+                   N, 4 * N, N, nb, when)
+        self._gemm(mw, 1, N, nn, ptr(self.Op22), 1, N, 0, mt1 + 8 * 5 * N, self.Rpad6, 1, sbw,
+                   N, N, N, nb, when)
+        for k in (1, N - 2):
+            self._zero(MT[1:, k], when, count=nb, stride=sbw)
 
     def _anel_buffers(self):
         """Fourier and grid buffers of the anelastic radial loop, the reference
@@ -616,6 +713,8 @@ class DynamoRun:
         self._info_host = torch.zeros(len(_INFO_ROWS), lp1, dtype=I32).pin_memory()
         self._info_event = None
         self.defer_checks = False  # check the pivots of a rebuild at the next step (no host wait)
+        if self.dc:
+            self._dc_storage()
     def _radial_operators(self):
         """Radial operators of the resolution, uploaded once and shared by every
         run of it (read-only): the DCT-I / derivative stacks folded into the
@@ -687,7 +786,7 @@ class DynamoRun:
         self._rebuild_launches(wimp_lin0)
         if self.cond_ic:
             self._build_coupled(wimp_lin0)
-        if self.anel:
+        if self.anel and not self.dc:
             self._build_anel(wimp_lin0)
         self.wimp_lin0 = wimp_lin0
         self.n_rebuild += 1
@@ -727,6 +826,8 @@ class DynamoRun:
         ctl = (wimp_dev, when) device pointers: the weight is read on the device
         and every kernel returns at once unless *when != 0 (the rebuild inside
         a captured step)."""
+        if self.dc:
+            return self._rebuild_launches_dc(wimp_lin0, ctl)
         N, lp1, ns = self.N, self.p.l_max + 1, self.nsys
         st = stream_handle()
         when = None if ctl is None else ctl[1]
@@ -847,8 +948,9 @@ class DynamoRun:
                                     ptr(MT) + 8 * ls * self.K4c * Rp, NT, R, Rp,
                                     [ptr(y) for y in Y], ld, ld))
         self.jobs1 = upload(jobs)
-        # lm loop, stage 2: coupled (w, p) per degree
-        Y = [f["w"], f["dw"], f["ddw"], tmp["dddw"], f["p"], f["dp"]]
+        # lm loop, stage 2: coupled (w, p) per degree; double curl: w and its four
+        # derivatives from the w half, the l = 0 pressure from the pressure half
+        Y = [f["w"], f["dw"], f["ddw"], tmp["dddw"], f["p"], tmp["ddddw"] if self.dc else f["dp"]]
         self.jobs2 = upload([job(ptr(self.rhs["wp"]), 2 * N, row_ptrs, nrows_l,
                                  ptr(self.MT["wp"]) + 8 * ls * self.K4wp * self.Rpad6, 2 * N,
                                  6 * N, self.Rpad6, [ptr(y) for y in Y], N, N)])
@@ -859,6 +961,9 @@ class DynamoRun:
         if self.p.l_mag:
             jobs.append(job(ptr(nl["VxBt"]), N, 0, L, ptr(self.MT_d1r2[0]), N, N,
                             self.MT_d1r2[2], [ptr(nl["dVxBt_dr"])], N, N))
+        if self.dc:   # d/dr (orho1 r^2 Advt) of the double-curl explicit term
+            jobs.append(job(ptr(nl["Advt"]), N, 0, L, ptr(self.MT_dvxvh[0]), N, N,
+                            self.MT_dvxvh[2], [ptr(nl["dAdvt_dr"])], N, N))
         self.jobs_td = upload(jobs)
         # set-up: derivatives of the initial fields
         jobs = [job(ptr(f["s"]), N, 0, L, ptr(self.MT_d12[0]), N, 2 * N, self.MT_d12[2],
@@ -866,9 +971,14 @@ class DynamoRun:
                 job(ptr(f["z"]), N, 0, L, ptr(self.MT_d12[0]), N, 2 * N, self.MT_d12[2],
                     [ptr(f["dz"]), ptr(tmp["d2z"])], N, N),
                 job(ptr(f["w"]), N, 0, L, ptr(self.MT_d123[0]), N, 3 * N, self.MT_d123[2],
-                    [ptr(f["dw"]), ptr(f["ddw"]), ptr(tmp["dddw"])], N, N),
-                job(ptr(f["p"]), N, 0, L, ptr(self.MT_d1[0]), N, N, self.MT_d1[2],
-                    [ptr(f["dp"])], N, N)]
+                    [ptr(f["dw"]), ptr(f["ddw"]), ptr(tmp["dddw"])], N, N)]
+        if self.dc:   # the set-up leaves dp as it is (step_time.py:465-470)
+            self.jobs_setup2 = upload([job(ptr(f["ddw"]), N, 0, L, ptr(self.MT_d12[0]), N,
+                                           2 * N, self.MT_d12[2],
+                                           [ptr(tmp["dddw"]), ptr(tmp["ddddw"])], N, N)])
+        else:
+            jobs.append(job(ptr(f["p"]), N, 0, L, ptr(self.MT_d1[0]), N, N, self.MT_d1[2],
+                            [ptr(f["dp"])], N, N))
         if self.p.l_mag:
             jobs += [job(ptr(f["b"]), N, 0, L, ptr(self.MT_d12[0]), N, 2 * N, self.MT_d12[2],
                          [ptr(f["db"]), ptr(f["ddb"])], N, N),
@@ -925,7 +1035,10 @@ class DynamoRun:
             coef, shape = noise_factors(p, r, seed, amp)
             f["s"].add_(up(coef)[:, None] * up(shape)[None, :])
         if p.l_heat:
-            if self.anel:
+            if self.dc:
+                from . import fdstep
+                s0, p0 = fdstep.conduction_state(p, r.rad, self.dc_prof)
+            elif self.anel:
                 from . import anelastic
                 s0, p0 = anelastic.conduction_state(p, r.rad, self.prof)
             else:
@@ -972,6 +1085,8 @@ class DynamoRun:
             for t in d.values():
                 t.zero_()
         self._rowop(self.jobs_setup)
+        if self.dc:
+            self._rowop(self.jobs_setup2)
         self._impl(zero_edges=True)
         # the reference's set-up fills the magnetic implicit terms on the
         # interior levels only (step_time.py:582-595)
@@ -1044,8 +1159,13 @@ class DynamoRun:
             A.dz = ptr(f["dz"])
             A.zero_edges = int(zero_edges)
             _lib.call("mh_impl_anel", _lib.ref(A), stream_handle())
-            return
-        _lib.call("mh_impl", _lib.ref(a), stream_handle())
+        else:
+            _lib.call("mh_impl", _lib.ref(a), stream_handle())
+        if self.dc:   # w from the double-curl operator; no pressure terms
+            d = self._dc_args()
+            d.old_w, d.impl_w = ptr(old["w"]), ptr(impl["w"])
+            d.old_p, d.impl_p = ptr(old["p"]), ptr(impl["p"])
+            _lib.call("mh_impl_dc", _lib.ref(d), stream_handle())
 
     def radial_loop(self):
         """Explicit terms of the current state; returns (dtrkc_min, dthkc_min)."""
@@ -1284,6 +1404,11 @@ class DynamoRun:
         t.CorFac, t.l_mag = self.p.CorFac, int(self.p.l_mag)
         with self._t("td"):
             _lib.call("mh_td", _lib.ref(t), stream_handle())
+            if self.dc:
+                a = self._dc_args()
+                a.Advr, a.dAdvt_dr = ptr(nl["Advr"]), ptr(nl["dAdvt_dr"])
+                a.ew, a.ep = ptr(e["w"]), ptr(e["p"])
+                _lib.call("mh_td_dc", _lib.ref(a), stream_handle())
 
     def _cfl_decision(self):
         """Courant limits (host decision, like the reference's .item())."""
@@ -1374,8 +1499,13 @@ class DynamoRun:
         bq.rgrav, bq.m, bq.L, bq.N = ptr(rg), ptr(self.res.d["m"]), self.L, self.N
         bq.BuoFac, bq.lm_l0, bq.wdev = p.BuoFac, 0, wdev
         with self._t("buoy"):
-            _lib.call("mh_buoy", _lib.ref(bq), stream_handle())
-            if self.anel and self.p0_integral:
+            if self.dc:
+                a = self._dc_args()
+                a.rwp, a.ew0, a.wdev = ptr(self.rhs["wp"]), ptr(ew0), wdev
+                _lib.call("mh_buoy_dc", _lib.ref(a), stream_handle())
+            else:
+                _lib.call("mh_buoy", _lib.ref(bq), stream_handle())
+            if self.anel and self.p0_integral and not self.dc:
                 q0 = _lib.P0IntArgs()
                 q0.rwp, q0.s, q0.u = ptr(self.rhs["wp"]), ptr(self.f["s"]), ptr(self.d_an["p0u"])
                 q0.lm_l0, q0.N = 0, self.N

@@ -99,6 +99,9 @@ def lm_coefficients(p: Params, maps: dict) -> dict:
 
 
 def radial(p: Params) -> dict:
+    if p.l_finite_diff:   # stretched finite-difference grid (magic_amd.fdstep)
+        from . import fdstep
+        return fdstep.radial(p)
     N = p.n_r_max
     r_cmb = 1.0 / (1.0 - p.radratio)
     r_icb = r_cmb - 1.0
