@@ -260,7 +260,7 @@ def test_fused_grid_step_matches_separate_kernels(l_max):
     assert np.allclose(cf, cg, rtol=1e-14, atol=0), (cf, cg)
 
 
-@pytest.mark.parametrize("name", ["bench16", "c1", "bprdt"])
+@pytest.mark.parametrize("name", ["bench16", "c1", "bprdt", "fd2mhd", "fdbpr", "fdanel"])
 def test_graph_replay_matches_eager(name):
     """The step's radial and lm phases captured as HIP graphs (one per parity of
     the explicit-history buffers, weights read from device memory) and replayed
@@ -288,6 +288,43 @@ def test_graph_replay_matches_eager(name):
 _DT_CASE = dict(cfg=dict(l_max=16, n_r_max=33, ek=1e-3, ra=1e5, pr=1.0, prmag=5.0,
                          radratio=0.35, dtmax=1e-3),
                 dt=1e-3, noise=dict(amp=1.0, seed=7), steps=30)
+
+
+def test_device_step_control_matches_host_steps_finite_differences():
+    """The finite-difference (double-curl) step under device step control: the
+    Courant rule changes dt six times in 30 steps and every rebuild (affine
+    assembly, inverses, folds) runs inside the captured step; dt sequence,
+    rebuild count and fields equal the host-decided eager steps bit for bit,
+    and the dt sequence and energies follow the oracle."""
+    import torch
+    case = dict(_DT_CASE, cfg=dict(_DT_CASE["cfg"], radial_scheme="FD", fd_order=2,
+                                   fd_order_bound=2))
+    host = _run(case)
+    dev = _run(case)
+    dev.use_graphs = True
+    dt = case["dt"]
+    for n in range(1, case["steps"] + 1):
+        dt = host.one_step(n, dt)
+    dev.run_steps(10, case["dt"])
+    dev.run_steps(case["steps"] - 10, dev.dt_log[-1])
+    assert dev.dt_log == host.dt_log
+    assert dev.n_rebuild == host.n_rebuild and host.n_rebuild >= 6, host.n_rebuild
+    for k in ("s", "z", "w", "p", "b", "aj", "dw", "ddw"):
+        assert torch.equal(dev.f[k], host.f[k]), k
+    from oracle.dynamo import OracleRun
+    orc = OracleRun(dict(case["cfg"]))
+    orc.initialize_fields()
+    orc.add_noise(case["noise"]["seed"], case["noise"]["amp"])
+    orc.setup_initial_state()
+    orc.initialize_dt(case["dt"])
+    dt = case["dt"]
+    odts = []
+    for n in range(1, case["steps"] + 1):
+        dt = orc.one_step(n, dt)
+        odts.append(dt)
+    np.testing.assert_allclose(dev.dt_log, odts, rtol=1e-9)
+    e, oe = np.array(dev.energies()), np.array(orc.energies())
+    assert np.all(np.abs(e - oe) <= 1e-9 * np.abs(oe)), (e, oe)
 
 
 def test_device_step_control_matches_host_steps():
