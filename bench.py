@@ -211,6 +211,67 @@ def run_case(precision, steps, warmup, dt_schedule=None, section_steps=20):
     return out
 
 
+# The reference's own published FP64 step times on NVIDIA parts (dynamo
+# benchmark physics, eager PyTorch on the GPU): l_max=64/n_r=129 Chebyshev
+# (PROGRESS.md:2252) and finite differences FD2 (perf.md:249-257) on an H100,
+# l_max=128/n_r=257 Chebyshev on a B200 (perf.md:270-283), and the operator
+# rebuild on a dt change at l_max=64 on an H100 (PROGRESS.md:2254).
+PUBLISHED = [
+    dict(name="chebyshev_l64_n129", cfg=dict(l_max=64, n_r_max=129), ms=6.7, hw="H100"),
+    dict(name="fd2_l64_n129", cfg=dict(l_max=64, n_r_max=129, radial_scheme="FD", fd_order=2,
+                                       fd_order_bound=2), ms=9.9, hw="H100"),
+    dict(name="chebyshev_l128_n257", cfg=dict(l_max=128, n_r_max=257), ms=56.0, hw="B200"),
+]
+PUBLISHED_REBUILD_MS = 70.0   # l_max=64, n_r=129, H100
+
+
+def published_case(steps=100, warmup=10):
+    """The reference's published step-time points, run here in FP64 with the
+    same physics (E=1e-3, Ra=1e5, Pr=1, Pm=5, radius ratio 0.35, CNAB2, dt 1e-4,
+    benchmark initial fields): ms per step over `steps` graph-replayed steps
+    with the dt decision on the device, and (l_max=64 Chebyshev) the mean time
+    of a full operator rebuild."""
+    import torch
+    from magic_amd.dynamo import DynamoRun
+    phys = dict(ek=1e-3, ra=1e5, pr=1.0, prmag=5.0, radratio=0.35, dtmax=1e-4)
+    out = {}
+    for pt in PUBLISHED:
+        run = DynamoRun(dict(phys, precision="fp64", **pt["cfg"]), device="cuda:0")
+        run.use_graphs = True
+        run.initialize_fields()
+        run.setup_initial_state()
+        run.initialize_dt(phys["dtmax"])
+        dt = run.run_steps(warmup, run.dtMax)
+        run.ctl_begin(dt)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            run.step_async()
+        torch.cuda.synchronize()
+        ms = 1e3 * (time.perf_counter() - t0) / steps
+        run.ctl_end()
+        rec = dict(ms_per_step=ms, published_ms=pt["ms"], published_hw=pt["hw"],
+                   speedup=pt["ms"] / ms, energies=run.energies())
+        if pt["name"] == "chebyshev_l64_n129":
+            w = run.p.alpha * run.dtMax     # CNAB2 implicit weight at the run's dt
+            run.build_all_matrices(w)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(5):
+                run.build_all_matrices(w)
+            torch.cuda.synchronize()
+            rb = 1e3 * (time.perf_counter() - t0) / 5
+            rec.update(rebuild_ms=rb, published_rebuild_ms=PUBLISHED_REBUILD_MS,
+                       rebuild_speedup=PUBLISHED_REBUILD_MS / rb)
+        out[pt["name"]] = rec
+        print(f"bench: published point {pt['name']}: {ms:.3f} ms/step "
+              f"(reference {pt['ms']} ms on {pt['hw']})", file=sys.stderr, flush=True)
+        run.close()
+        del run
+        torch.cuda.empty_cache()
+    return out
+
+
 def fd_banded_case():
     """Config 3: FD4 n_r=512 l_max=255 anelastic (N_rho=3, polytropic index 2)
     banded S, Z, W operators: batched band LU of every degree and the solve of
@@ -420,6 +481,8 @@ def main():
                     help="the mixed-precision variant of the headline value")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fd", action="store_true", help="skip the config-3 banded-solve timing")
+    ap.add_argument("--no-published", action="store_true",
+                    help="skip the step times at the reference's published points")
     ap.add_argument("--cpu-levels", type=int, default=9)
     ap.add_argument("--no-ensemble", action="store_true", help="skip the config-5 ensemble")
     ap.add_argument("--no-sht", action="store_true", help="skip the config-2 SHT round trip")
@@ -497,6 +560,8 @@ def main():
     if not args.no_fd:
         cfg["fd_banded"] = fd_banded_case()
         print("bench: FD banded solves done", file=sys.stderr, flush=True)
+    if not args.no_published:
+        cfg["published_points"] = published_case()
     if not args.no_ensemble:
         cfg["ensemble"] = ensemble_case(args.ensemble_seq)
         print(f"bench: ensemble wall {cfg['ensemble']['server']['wall_s']:.1f} s", file=sys.stderr,
