@@ -272,6 +272,44 @@ def published_case(steps=100, warmup=10):
     return out
 
 
+def fd_step_case(steps=20, warmup=3):
+    """Config 3's physics as a full time step: finite differences FD4 (boundary
+    order 2) on the stretched n_r=512 grid, l_max=255, anelastic polytrope
+    N_rho=3 (strat 3, polind 2, gravity ~ 1/r^2), hydrodynamic, stress-free
+    walls, E=1e-3, Ra=1e5, Pr=1, FP64, CNAB2 with the double-curl poloidal
+    equation: ms per step (host dt decision, eager launches) and the time of
+    the first operator build (host assembly excluded)."""
+    import torch
+    from magic_amd.dynamo import DynamoRun
+    cfg = dict(l_max=255, n_r_max=512, radial_scheme="FD", fd_order=4, fd_order_bound=2,
+               strat=3.0, polind=2.0, g0=0.0, g1=0.0, g2=1.0, mode=1, ktopv=1, kbotv=1,
+               ra=1e5, ek=1e-3, pr=1.0, dtmax=1e-5, precision="fp64")
+    t0 = time.perf_counter()
+    run = DynamoRun(cfg, device="cuda:0")
+    setup_s = time.perf_counter() - t0
+    run.initialize_fields(noise_amp=NOISE["amp"], noise_seed=NOISE["seed"])
+    run.setup_initial_state()
+    run.initialize_dt(cfg["dtmax"])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    run.build_all_matrices(run.p.alpha * run.dtMax)
+    torch.cuda.synchronize()
+    build_ms = 1e3 * (time.perf_counter() - t0)
+    dt = run.run_steps(warmup, run.dtMax)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    dt = run.run_steps(steps, dt)
+    torch.cuda.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0) / steps
+    out = dict(ms_per_step=ms, steps=steps, operator_build_ms=build_ms, setup_s=setup_s,
+               rebuilds=run.n_rebuild, energies=run.energies(), dt=dt,
+               workload="FD4 n_r=512 l_max=255 anelastic N_rho=3 hydro, FP64")
+    run.close()
+    del run
+    torch.cuda.empty_cache()
+    return out
+
+
 def fd_banded_case():
     """Config 3: FD4 n_r=512 l_max=255 anelastic (N_rho=3, polytropic index 2)
     banded S, Z, W operators: batched band LU of every degree and the solve of
@@ -560,6 +598,9 @@ def main():
     if not args.no_fd:
         cfg["fd_banded"] = fd_banded_case()
         print("bench: FD banded solves done", file=sys.stderr, flush=True)
+        cfg["fd_step"] = fd_step_case()
+        print(f"bench: FD step {cfg['fd_step']['ms_per_step']:.2f} ms", file=sys.stderr,
+              flush=True)
     if not args.no_published:
         cfg["published_points"] = published_case()
     if not args.no_ensemble:

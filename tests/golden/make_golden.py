@@ -50,6 +50,16 @@ that each step returns passed to the next (as ``main.py`` does):
   minc=4 (the boussBenchSat combination), dt 2e-4, noise 1e-2, 30 steps;
   omega_ic every step.
 
+* ``fd2mhd``, ``fd4mhd``, ``fdanel``, ``fdanelmhd``, ``fd4anel``, ``fdbpr``: the
+  finite-difference radial scheme (stretched grid, fd_stretch 0.3, fd_ratio
+  0.1) with the double-curl poloidal equation (``update_w_doublecurl.py``,
+  ``get_td.py:91-147``): the physics of the reference's FD step tests
+  (``tests/test_fd_step.py``, ``test_fd4_step.py``, ``test_fd_anel.py``,
+  ``test_fd_anel_mhd.py``) with seeded noise, FD2 and FD4 (n_r even in
+  ``fd4mhd``), config 3's anelastic physics (FD4, strat 3, polind 2) at
+  l_max=16, n_r=48 in ``fd4anel``, and BPR353 in ``fdbpr``; 20 steps (10 for
+  BPR353).
+
 Finite-difference operator cases (``fd4_ob2``, ``fd4_ob3``): the FD branches
 of ``build_s_matrices`` (``update_s.py:66-188``), ``build_z_matrices``
 (``update_z.py:89-240``) and ``build_w_matrices``
@@ -187,6 +197,15 @@ CASES = {
                                init_s1=404, amp_s1=0.1),
                       dt=1e-4, steps=20, noise=dict(seed=79, amp=1e-3), field_steps=[1, 20],
                       expl_step1=False),
+    # config 3's physics (FD4, N_rho = 3, polytropic index 2, hydrodynamic) at a
+    # small size
+    "fd4anel": dict(cfg=dict(l_max=16, n_r_max=48, radial_scheme="FD", fd_order=4,
+                             fd_order_bound=2, fd_stretch=0.3, fd_ratio=0.1, strat=3.0,
+                             polind=2.0, g0=0.0, g1=0.0, g2=1.0, mode=1, ktopv=1, kbotv=1,
+                             ra=1e5, ek=1e-3, pr=1.0, dtmax=1e-4, alpha=0.6, init_s1=1010,
+                             amp_s1=0.01),
+                    dt=1e-4, steps=20, noise=dict(seed=89, amp=1e-3), field_steps=[1, 20],
+                    expl_step1=True, diag_steps=[20]),
     "fdbpr": dict(cfg=dict(l_max=16, n_r_max=33, radial_scheme="FD", fd_order=2,
                            fd_order_bound=2, fd_stretch=0.3, fd_ratio=0.1, ra=1e5, ek=1e-3,
                            pr=1.0, prmag=5.0, dtmax=1e-4, init_s1=404, amp_s1=0.1,
