@@ -105,12 +105,16 @@ def test_step_matches_reference_trajectory(name, fused):
 @pytest.mark.parametrize("name,precision,etol,ftol", [
     ("bench16", "mixed", 1e-6, 1e-5), ("c1", "mixed", 1e-6, 1e-5),
     ("c1", "mixed_bf16x3", 1e-6, 1e-5), ("c1", "mixed_bf16", 1e-4, 1e-4),
-    ("bprminc4", "mixed", 1e-6, 1e-5)])
+    ("bprminc4", "mixed", 1e-6, 1e-5), ("condic", "mixed_bf16x3", 1e-6, 1e-5),
+    ("rotic", "mixed", 1e-6, 1e-5), ("fd2mhd", "mixed_bf16x3", 1e-6, 1e-5),
+    ("fd4mhd", "mixed", 1e-6, 1e-5)])
 def test_mixed_precision_step_tracks_reference(name, precision, etol, ftol):
     """Mixed precision (Legendre transforms on FP32 MFMA, or on BF16 MFMA with
     operands split into three or two bf16 planes; everything else FP64) against
     the FP64 reference trajectory: energies within `etol` relative and fields
-    within `ftol` of their largest coefficient at every recorded step.  1e-6 is
+    within `ftol` of their largest coefficient at every recorded step (a
+    conducting and a rotating inner core, and the finite-difference step,
+    included).  1e-6 is
     the mixed-precision target; two bf16 planes (about 16 significant bits)
     miss it by design and are held to 1e-4."""
     case, g = _load(name)
