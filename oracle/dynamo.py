@@ -59,9 +59,6 @@ class OracleRun:
         # finite-difference radial scheme: double-curl poloidal equation
         # (params.py:143), oracle.fdstep
         self.fd = p.l_finite_diff
-        if self.fd and p.l_cond_ic:
-            raise NotImplementedError("oracle step: finite differences with an insulating "
-                                      "inner core")
         if p.l_anel and p.sigma_ratio > 0.0:
             raise NotImplementedError("oracle step: anelastic runs with an insulating inner core")
         # anelastic reference state (strat > 0): step_time.py:1215-1475 and the
@@ -555,6 +552,10 @@ class OracleRun:
         p, rad, mats = self.prm, self.rad, self.mats
         if self.fd:
             self.inv = fdstep.build(p, rad, tuple(mats) + (self.D4,), self.prof, w0)
+            if self.rot_ic:
+                self.inv["z10"] = matrices.build_z10(p, rad, mats, w0, self.c_dt_z10_ic)
+            if p.l_mag and self.cond_ic:   # coupled outer (FD) + inner (even Chebyshev)
+                self.inv["b"], self.inv["j"] = matrices.build_bj_coupled(p, rad, mats, self.ic, w0)
             return
         if self.anel:
             pr = self.prof
@@ -677,6 +678,9 @@ class OracleRun:
         dLh = self.lmarr["dLh"][:, None]
         or1, or2 = rad["or1"][None], rad["or2"][None]
         self.z[:] = z_sol
+        l1m0 = self.maps["lm2"][1, 0]
+        if self.rot_ic:
+            self.omega_ic = self.c_z10_omega_ic * self.z[l1m0, N - 1].real
         rhs_w = self._rhs("w")
         b = slice(2, N - 2)
         rhs_w[:, b] += (self.wimp_lin[0] * p.BuoFac * dLh * or2[:, b] * pr["rgrav"][None, b]
@@ -705,6 +709,14 @@ class OracleRun:
         if not self.dirk:
             for k in ("s", "z", "w", "p"):
                 self.td[k]["expl"][1][:] = self.td[k]["expl"][0]
+            if self.rot_ic:
+                self.om["expl"][1] = self.om["expl"][0]
+        if self.rot_ic:
+            z10, dz10 = self.z[l1m0, N - 1].real, self.dz[l1m0, N - 1].real
+            if (self.istage == 4) if self.dirk else True:
+                self.om["old"] = self.c_dt_z10_ic * z10
+            self._om_impl_slot(self.istage % 4 if self.dirk else 0,
+                               -(2.0 * rad["or1"][N - 1] * z10 - dz10))
         if self._store():
             self.td["s"]["old"][:] = self.s
             self.td["z"]["old"][:] = dLh * or2 * self.z
@@ -738,7 +750,7 @@ class OracleRun:
             oc_c[:, p.n_cheb_max:] = 0.0
             ic_c = sol[:, N:N + Nic].copy()
             ic_c[:, p.n_cheb_ic_max:] = 0.0
-            sols.append((self._grid(oc_c), self._grid(ic_c)))
+            sols.append((self._grid(oc_c), radial.costf(ic_c)))
         (self.b[:], self.b_ic[:]), (self.aj[:], self.aj_ic[:]) = sols
         self.db[:] = self._dr(self.b, self.D1); self.ddb[:] = self._dr(self.b, self.D2)
         self.dj[:] = self._dr(self.aj, self.D1); self.ddj[:] = self._dr(self.aj, self.D2)

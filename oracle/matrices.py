@@ -26,9 +26,11 @@ def _precond_rows(dat):
     return fac[:, None] * dat, fac
 
 
-def _bfac_cols(dat, cols):
+def _bfac_cols(dat, cols, f=0.5):
+    """Boundary factor on the first/last outer columns: 1/2 for Chebyshev
+    coefficients, 1 on the finite-difference grid (boundary_fac)."""
     for c in cols:
-        dat[:, c] *= 0.5
+        dat[:, c] *= f
     return dat
 
 
@@ -92,7 +94,7 @@ def build_z10(p: Params, rad, mats, wimp_lin0, c_dt_z10_ic):
     if p.n_cheb_max < N:
         dat[0, p.n_cheb_max:] = 0.0
         dat[N - 1, p.n_cheb_max:] = 0.0
-    _bfac_cols(dat, (0, N - 1))
+    _bfac_cols(dat, (0, N - 1), rad.get("bfac", 0.5))
     datp, fac = _precond_rows(dat)
     return np.linalg.inv(datp) * fac[None, :]
 
@@ -219,7 +221,7 @@ def build_bj_coupled(p: Params, rad, mats, ic, wimp_lin0):
             dat[NT - 1, N:] = cn * dL * or2_icb * (
                 cheb[:, Nic - 1] - wimp_lin0 * p.opm * p.O_sr * (1.0 + 2.0 * float(l + 1))
                 * d2cheb[:, Nic - 1])
-            _bfac_cols(dat, (0, N - 1))
+            _bfac_cols(dat, (0, N - 1), rad.get("bfac", 0.5))
             dat[:, N] *= 0.5
             dat[:, NT - 1] *= 0.5
             datp, fac = _precond_rows(dat)

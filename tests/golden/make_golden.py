@@ -57,8 +57,9 @@ that each step returns passed to the next (as ``main.py`` does):
   (``tests/test_fd_step.py``, ``test_fd4_step.py``, ``test_fd_anel.py``,
   ``test_fd_anel_mhd.py``) with seeded noise, FD2 and FD4 (n_r even in
   ``fd4mhd``), config 3's anelastic physics (FD4, strat 3, polind 2) at
-  l_max=16, n_r=48 in ``fd4anel``, and BPR353 in ``fdbpr``; 20 steps (10 for
-  BPR353).
+  l_max=16, n_r=48 in ``fd4anel``, a conducting and a rotating inner core
+  (``fdcondic``, ``fdrotic``: the reference's bordered band solves,
+  ``algebra.py:513-617``) and BPR353 in ``fdbpr``; 20 steps (10 for BPR353).
 
 Finite-difference operator cases (``fd4_ob2``, ``fd4_ob3``): the FD branches
 of ``build_s_matrices`` (``update_s.py:66-188``), ``build_z_matrices``
@@ -206,6 +207,22 @@ CASES = {
                              amp_s1=0.01),
                     dt=1e-4, steps=20, noise=dict(seed=89, amp=1e-3), field_steps=[1, 20],
                     expl_step1=True, diag_steps=[20]),
+    # conducting and rotating inner core on the finite-difference grid (the
+    # physics of test_fd_condic_step.py and test_fd_rotic_step.py)
+    "fdcondic": dict(cfg=dict(l_max=16, n_r_max=33, radial_scheme="FD", fd_order=2,
+                              fd_order_bound=2, fd_stretch=0.3, fd_ratio=0.1, ra=1e5, ek=1e-3,
+                              pr=1.0, prmag=5.0, dtmax=1e-4, alpha=0.6, init_s1=404,
+                              amp_s1=0.1, ktopv=2, kbotv=2, kbotb=3, sigma_ratio=1.0,
+                              n_cheb_ic_max=15),
+                     dt=1e-4, steps=20, noise=dict(seed=97, amp=1e-3), field_steps=[1, 20],
+                     expl_step1=False),
+    "fdrotic": dict(cfg=dict(l_max=16, n_r_max=33, radial_scheme="FD", fd_order=2,
+                             fd_order_bound=2, fd_stretch=0.3, fd_ratio=0.1, ra=1e5, ek=1e-3,
+                             pr=1.0, prmag=5.0, dtmax=2e-4, alpha=0.6, init_s1=404,
+                             amp_s1=0.1, ktopv=2, kbotv=2, kbotb=3, sigma_ratio=1.0,
+                             n_cheb_ic_max=15, nRotIC=1),
+                    dt=2e-4, steps=20, noise=dict(seed=101, amp=1e-2), field_steps=[1, 20],
+                    expl_step1=False),
     "fdbpr": dict(cfg=dict(l_max=16, n_r_max=33, radial_scheme="FD", fd_order=2,
                            fd_order_bound=2, fd_stretch=0.3, fd_ratio=0.1, ra=1e5, ek=1e-3,
                            pr=1.0, prmag=5.0, dtmax=1e-4, init_s1=404, amp_s1=0.1,

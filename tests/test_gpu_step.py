@@ -52,7 +52,7 @@ P_TOL_ANEL = 1e-9
                                         ("bpranel2", True), ("amc16", True), ("amc16", False),
                                         ("rotom8", True), ("fd2mhd", True), ("fd2mhd", False),
                                         ("fd4mhd", True), ("fdanel", True), ("fdanelmhd", True),
-                                        ("fd4anel", True),
+                                        ("fd4anel", True), ("fdcondic", True), ("fdrotic", True),
                                         ("fdbpr", True)])
 def test_step_matches_reference_trajectory(name, fused):
     """fused: one grid-space kernel; otherwise separate FFT / products / FFT kernels.

@@ -37,7 +37,7 @@ P_TOL_ANEL = 1e-9
                                   "bprminc4", "condic", "bprcondic", "rotic", "bprrotic4",
                                   "anel21", "anelmag16", "bpranel2", "amc16", "rotom8",
                                   "fd2mhd", "fd4mhd", "fdanel", "fdanelmhd", "fd4anel",
-                                  "fdbpr"])
+                                  "fdbpr", "fdcondic", "fdrotic"])
 def test_oracle_matches_reference_trajectory(name):
     """Energies every step and fields at the saved steps, from the reference's
     one_step with the returned dt passed to the next step; the dt sequence

@@ -427,7 +427,9 @@ def test_anelastic_launches_stay_inside_their_tensors(cpu_standin, cfg):
     dict(l_max=12, n_r_max=40, radial_scheme="FD", fd_order=4, fd_order_bound=2, dtmax=1e-4,
          minc=2, time_scheme="BPR353"),
     dict(l_max=12, n_r_max=33, radial_scheme="FD", strat=0.5, polind=2.0, g2=1.0, mode=1,
-         ktopv=1, kbotv=1, dtmax=1e-4)])
+         ktopv=1, kbotv=1, dtmax=1e-4),
+    dict(l_max=12, n_r_max=33, radial_scheme="FD", sigma_ratio=1.0, kbotb=3, nRotIC=1,
+         n_cheb_ic_max=15, dtmax=1e-4)])
 def test_finite_difference_launches_stay_inside_their_tensors(cpu_standin, cfg):
     """Finite-difference (double-curl) step: the rebuild (affine assembly,
     inverses, folds, row fills), the row-operator jobs (including the second

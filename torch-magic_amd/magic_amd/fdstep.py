@@ -97,7 +97,7 @@ def radial(p) -> dict:
                 D1=D1, D2=D2, D3=D3, D4=D4, C=eye, CT=eye, q=simpson_weights(r),
                 or1=1.0 / r, or2=r ** -2.0, or3=r ** -3.0, or4=r ** -4.0, r2=r * r,
                 rgrav=r / r_cmb, delxr2=delxr2, delxh2=delxh2, valri2=fac / delxr2,
-                valhi2=fac / delxh2, fd=True)
+                valhi2=fac / delxh2, fd=True, bfac=1.0)
 
 
 def boussinesq_profile(rad) -> dict:

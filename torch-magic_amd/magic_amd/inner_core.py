@@ -57,7 +57,9 @@ def coupled_inverses(p, rad: dict, ic: dict, wimp_lin0: float):
             a[NT - 1, N:] = cn * dL * or2_icb * (
                 cheb[:, Nic - 1] - wimp_lin0 * p.opm * p.O_sr * (1.0 + 2.0 * (l + 1))
                 * d2cheb[:, Nic - 1])
-            for c in (0, N - 1, N, NT - 1):
+            for c in (0, N - 1):      # outer boundary factor (1 on the FD grid)
+                a[:, c] *= rad.get("bfac", 0.5)
+            for c in (N, NT - 1):     # inner even-Chebyshev end coefficients
                 a[:, c] *= 0.5
             fac = 1.0 / np.abs(a).max(axis=1)
             out[key][l] = np.linalg.inv(fac[:, None] * a) * fac[None, :]
@@ -109,7 +111,7 @@ def z10_inverse(p, rad: dict, wimp_lin0: float, c_dt_z10: float) -> np.ndarray:
     a[N - 1] = rn * (c_dt_z10 * R[N - 1] + wimp_lin0 * (2.0 * or1[N - 1] * R[N - 1] - D[N - 1]))
     if p.n_cheb_max < N:
         a[(0, N - 1), p.n_cheb_max:] = 0.0
-    a[:, (0, N - 1)] *= 0.5
+    a[:, (0, N - 1)] *= rad.get("bfac", 0.5)
     fac = 1.0 / np.abs(a).max(axis=1)
     return np.linalg.inv(fac[:, None] * a) * fac[None, :]
 

@@ -159,8 +159,8 @@ class Params:
         inner core (nRotIC = 1: omega_ic integrated in time, no-slip ICB), and
         the anelastic step (strat > 0, polytropic, FP64) with an insulating
         inner core; angular-momentum corrections without a rotating core; the
-        finite-difference radial scheme (double-curl poloidal equation) with
-        an insulating inner core, Boussinesq or anelastic."""
+        finite-difference radial scheme (double-curl poloidal equation),
+        Boussinesq (any inner core) or anelastic (insulating inner core)."""
         bad = []
         if self.l_anel and self.l_cond_ic:
             bad.append("anelastic step with a conducting inner core")
@@ -168,8 +168,6 @@ class Params:
             bad.append(f"anelastic step in precision {self.precision!r} (fp64 only)")
         if self.l_anel and self.g0 == 0.0 and self.g1 == 0.0 and self.g2 == 0.0:
             bad.append("anelastic step without gravity (g0 = g1 = g2 = 0)")
-        if self.l_finite_diff and self.l_cond_ic:
-            bad.append("finite-difference radial scheme with a conducting inner core")
         if self.nRotIC not in (0, 1):
             bad.append(f"nRotIC={self.nRotIC} (prescribed inner-core rotation)")
         elif self.nRotIC == 1 and not (self.l_cond_ic and self.l_mag and self.kbotv == 2):
