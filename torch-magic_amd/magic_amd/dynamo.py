@@ -1300,21 +1300,23 @@ class DynamoRun:
         radial velocity; of B and curl B; the azimuthal derivatives of u_theta,
         u_phi in Fourier space; inverse FFTs; the anelastic grid products with the
         Courant maxima; forward FFTs and analysis of the bulk levels; the explicit
-        terms, with the heating and 1/rho0 in the entropy equation."""
+        terms, with the heating and 1/rho0 in the entropy equation.  Mixed
+        precision changes the Legendre transforms only (FP64 Fourier buffers,
+        FFTs and products)."""
         r, f, N, Nb, p = self.res, self.f, self.N, self.Nb, self.p
         M, T = r.M, r.T
         with self._t("leg_syn_scal"):
             r.synth([[(f["s"], None, 0)], [(f["z"], None, 1)]], N, N, self.F_s2, 2 * N,
-                    vector=False, prec=0)
+                    vector=False, prec=self.leg_prec)
         groups = [[(f["w"], None, 1), (f["dw"], None, 0), (f["z"], None, 0)],
                   [(f["dw"], None, 1), (f["ddw"], None, 0), (f["dz"], None, 0)],
                   [(self.zeroLN, None, 0), (f["w"], None, 1), (self.zeroLN, None, 0)]]
         with self._t("leg_syn_vec"):
-            r.synth(groups, N, N, self.F_va, 3 * N, vector=True, prec=0)
+            r.synth(groups, N, N, self.F_va, 3 * N, vector=True, prec=self.leg_prec)
             if p.l_mag:
                 r.synth([[(f["b"], None, 1), (f["db"], None, 0), (f["aj"], None, 0)],
                          [(f["aj"], None, 1), (f["dj"], None, 0), (f["ddb"], f["b"], 2)]],
-                        N, N, self.F_vb, 2 * N, vector=True, prec=0)
+                        N, N, self.F_vb, 2 * N, vector=True, prec=self.leg_prec)
         d = _lib.DphiArgs()
         d.F, (d.Fc_stride, d.Fm_stride), d.FB, d.c0 = ptr(self.F_va), r.fstrides(3 * N), 3 * N, 1
         d.out, (d.Oc_stride, d.Om_stride), d.OB = ptr(self.F_d), r.fstrides(N), N
@@ -1350,14 +1352,14 @@ class DynamoRun:
         with self._t("leg_ana_scal"):
             r.analysis(self.F_oa, Nb, Nb, [0, 1, 2],
                        [ptr(nl["Advr"]) + off, ptr(nl["dVSr"]) + off, ptr(nl["VxBr"]) + off],
-                       [None, None, None], N, vector=False, prec=0)
+                       [None, None, None], N, vector=False, prec=self.leg_prec)
             r.analysis(self.F_oa, Nb, Nb, [3], [ptr(self.heatLM) + off], [None], N,
-                       vector=False, prec=0)
+                       vector=False, prec=self.leg_prec)
         with self._t("leg_ana_vec"):
             r.analysis(self.F_oa, Nb, Nb, [(4, 5), (6, 7), (8, 9)],
                        [ptr(nl["Advt"]) + off, ptr(nl["VSt"]) + off, ptr(nl["VxBt"]) + off],
                        [ptr(nl["Advp"]) + off, None, ptr(nl["VxBp"]) + off], N, vector=True,
-                       prec=0)
+                       prec=self.leg_prec)
         self._td_launches()
         e = _lib.EntropyAnelArgs()
         e.es, e.heat, e.orho1 = ptr(self.expl_cur["s"]), ptr(self.heatLM), ptr(self.d_an["orho1"])

@@ -157,15 +157,13 @@ class Params:
         """The device step covers the Chebyshev path with CNAB2 or BPR353: the
         Boussinesq step with an insulating, conducting or rotating conducting
         inner core (nRotIC = 1: omega_ic integrated in time, no-slip ICB), and
-        the anelastic step (strat > 0, polytropic, FP64) with an insulating
-        inner core; angular-momentum corrections without a rotating core; the
+        the anelastic step (strat > 0, polytropic; mixed precision in the
+        Legendre transforms only) with an insulating inner core; angular-momentum corrections without a rotating core; the
         finite-difference radial scheme (double-curl poloidal equation),
         Boussinesq (any inner core) or anelastic (insulating inner core)."""
         bad = []
         if self.l_anel and self.l_cond_ic:
             bad.append("anelastic step with a conducting inner core")
-        if self.l_anel and self.precision != "fp64":
-            bad.append(f"anelastic step in precision {self.precision!r} (fp64 only)")
         if self.l_anel and self.g0 == 0.0 and self.g1 == 0.0 and self.g2 == 0.0:
             bad.append("anelastic step without gravity (g0 = g1 = g2 = 0)")
         if self.nRotIC not in (0, 1):
