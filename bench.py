@@ -521,6 +521,9 @@ def main():
     ap.add_argument("--no-fd", action="store_true", help="skip the config-3 banded-solve timing")
     ap.add_argument("--no-published", action="store_true",
                     help="skip the step times at the reference's published points")
+    ap.add_argument("--fd-step-only", type=int, default=0, metavar="STEPS",
+                    help="only time STEPS finite-difference steps at config 3's physics "
+                         "(FD4, n_r=512, l_max=255, N_rho=3) and print their JSON")
     ap.add_argument("--cpu-levels", type=int, default=9)
     ap.add_argument("--no-ensemble", action="store_true", help="skip the config-5 ensemble")
     ap.add_argument("--no-sht", action="store_true", help="skip the config-2 SHT round trip")
@@ -532,6 +535,9 @@ def main():
     ap.add_argument("--ensemble-seq", type=int, default=16,
                     help="requests in the sequential ensemble baseline sample")
     args = ap.parse_args()
+    if args.fd_step_only:
+        print(json.dumps(fd_step_case(steps=args.fd_step_only, warmup=2)))
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only; --gpus must be 1", file=sys.stderr)
         sys.exit(2)
