@@ -272,7 +272,8 @@ def test_fused_grid_step_matches_separate_kernels(l_max):
     assert np.allclose(cf, cg, rtol=1e-14, atol=0), (cf, cg)
 
 
-@pytest.mark.parametrize("name", ["bench16", "c1", "bprdt", "fd2mhd", "fdbpr", "fdanel"])
+@pytest.mark.parametrize("name", ["bench16", "c1", "bprdt", "fd2mhd", "fdbpr", "fdanel", "anel21",
+                                  "bpranel2"])
 def test_graph_replay_matches_eager(name):
     """The step's radial and lm phases captured as HIP graphs (one per parity of
     the explicit-history buffers, weights read from device memory) and replayed
@@ -300,6 +301,45 @@ def test_graph_replay_matches_eager(name):
 _DT_CASE = dict(cfg=dict(l_max=16, n_r_max=33, ek=1e-3, ra=1e5, pr=1.0, prmag=5.0,
                          radratio=0.35, dtmax=1e-3),
                 dt=1e-3, noise=dict(amp=1.0, seed=7), steps=30)
+
+
+def test_device_step_control_matches_host_steps_anelastic():
+    """The anelastic Chebyshev step under device step control: noise of
+    amplitude 1 makes the Courant rule change dt six times in 20 steps, and each
+    rebuild (A0 + w A1 of the anelastic S, Z and (w, p) matrices, inverses,
+    folds, the fixed l = 0 pressure block) runs inside the captured step; dt
+    sequence, rebuild count and fields equal the host-decided eager steps bit
+    for bit, and the dt sequence and energies follow the oracle."""
+    import torch
+    case = dict(cfg=dict(l_max=16, n_r_max=33, strat=1.0, polind=2.0, g1=1.0, ra=1e5,
+                         ek=1e-3, pr=1.0, prmag=5.0, dtmax=1e-3),
+                dt=1e-3, noise=dict(amp=1.0, seed=7), steps=20)
+    host = _run(case)
+    dev = _run(case)
+    dev.use_graphs = True
+    dt = case["dt"]
+    for n in range(1, case["steps"] + 1):
+        dt = host.one_step(n, dt)
+    dev.run_steps(8, case["dt"])
+    dev.run_steps(case["steps"] - 8, dev.dt_log[-1])
+    assert dev.dt_log == host.dt_log
+    assert dev.n_rebuild == host.n_rebuild and host.n_rebuild >= 5, host.n_rebuild
+    for k in ("s", "z", "w", "p", "b", "aj", "dw", "ddw"):
+        assert torch.equal(dev.f[k], host.f[k]), k
+    from oracle.dynamo import OracleRun
+    orc = OracleRun(dict(case["cfg"]))
+    orc.initialize_fields()
+    orc.add_noise(case["noise"]["seed"], case["noise"]["amp"])
+    orc.setup_initial_state()
+    orc.initialize_dt(case["dt"])
+    dt = case["dt"]
+    odts = []
+    for n in range(1, case["steps"] + 1):
+        dt = orc.one_step(n, dt)
+        odts.append(dt)
+    np.testing.assert_allclose(dev.dt_log, odts, rtol=1e-9)
+    e, oe = np.array(dev.energies()), np.array(orc.energies())
+    assert np.all(np.abs(e - oe) <= 1e-9 * np.abs(oe)), (e, oe)
 
 
 def test_device_step_control_matches_host_steps_finite_differences():

@@ -390,26 +390,31 @@ def test_anelastic_operators_match_oracle(cfg):
     for k, ok in (("r2q", "r2q"), ("pz", "z"), ("pdz", "dz"), ("pd2z", "d2z")):
         np.testing.assert_allclose(am[k], oam[ok], rtol=1e-12, atol=1e-300)
     assert am["cz"] == pytest.approx(8.0 / 3.0 * math.pi / oam["c_moi_oc"], rel=1e-13)
-    # row operators: outputs (f, f', f'') of the solve of a random right-hand side
+    # the device rebuild's operators: A0 + w0 A1 is the collocation matrix (its
+    # inverse is the oracle's), and the l = 0 pressure block maps the pressure half
+    # of a right-hand side to (p, dp/dr)
     N = p.n_r_max
-    k4, r3, k4w, r6 = -(-N // 4) * 4, -(-(3 * N) // 64) * 64, -(-(2 * N) // 4) * 4, -(-(6 * N) // 64) * 64
-    ms, mz, mwp = anelastic.row_operators(p, rad, prof, w0, k4, r3, k4w, r6)
-    rhs = np.random.default_rng(11).standard_normal(2 * N)
-    l = 3
-    sol = pairs[2][1][l] @ rhs
-    cw, cp = sol[:N].copy(), sol[N:].copy()
-    cw[p.n_cheb_max:] = 0.0
+    ops = anelastic.affine_operators(p, rad, prof)
+    (a0, a1), (w0m, w1m) = ops["sz"], ops["wp"]
+    for k, inv in ((0, pairs[0][1]), (1, pairs[1][1])):
+        for l in range(1, p.l_max + 1):
+            M = np.linalg.inv(a0[k, l] + w0 * a1[k, l])
+            assert np.abs(M - inv[l]).max() <= 1e-8 * np.abs(inv[l]).max(), (k, l)
+    for l in range(1, p.l_max + 1):
+        A = w0m[l] + w0 * w1m[l]
+        x = np.random.default_rng(l).standard_normal(2 * N)
+        y = pairs[2][1][l] @ x
+        assert np.abs(A @ y - x).max() <= 1e-7 * np.abs(x).max(), l
+    k4w, r6 = -(-(2 * N) // 4) * 4, -(-(6 * N) // 64) * 64
+    mt0 = anelastic.p0_block(p, rad, prof, k4w, r6)
+    rhs = np.random.default_rng(11).standard_normal(N)
+    cp = pairs[3][1][0] @ rhs
     cp[p.n_cheb_max:] = 0.0
-    fw, fp = orad.costf(cw), orad.costf(cp)
-    ref = np.concatenate([fw, o.D1 @ fw, o.D2 @ fw, o.D3 @ fw, fp, o.D1 @ fp])
-    got = mwp[l, :2 * N, :6 * N].T @ rhs
+    fp = orad.costf(cp)
+    got = mt0[N:2 * N, 4 * N:6 * N].T @ rhs
+    ref = np.concatenate([fp, o.D1 @ fp])
     assert np.abs(got - ref).max() <= 1e-8 * np.abs(ref).max()
-    sol = pairs[0][1][l] @ rhs[:N]
-    sol[p.n_cheb_max:] = 0.0
-    fs = orad.costf(sol)
-    got = ms[l, :N, :3 * N].T @ rhs[:N]
-    ref = np.concatenate([fs, o.D1 @ fs, o.D2 @ fs])
-    assert np.abs(got - ref).max() <= 1e-9 * np.abs(ref).max()
+    assert not np.abs(mt0[:N]).any()
 
 
 @pytest.mark.parametrize("cfg", [

@@ -342,7 +342,6 @@ def test_anelastic_launches_stay_inside_their_tensors(cpu_standin, cfg):
         run = dyn.DynamoRun(cfg, device="cpu")
         run.initialize_dt(cfg["dtmax"])
         run._rebuild_launches(0.5e-4)
-        run._build_anel(0.5e-4)
     assert run.anel and not run.fused_grid
     ext = _extents(run)
     _check_row_jobs(run, ext)

@@ -10,12 +10,17 @@ scale height beta = d ln rho0 / dr (``update_s.py:66-188``,
 constant (visc = kappa = lambda = 1), the entropy gradient of the background
 zero.
 
-The operators change with the implicit weight only and are assembled and
-inverted on the host at each rebuild, with the DCT-I and the radial
-derivatives folded in (row-operator layout MT[l][k][r]); the (w, p) systems
-use the reference's row and column scaling (the reference LU-solves them: the
-same solution), the l = 0 pressure its integral condition when
-ViscHeatFac ThExpNb != 0 (``update_wp.py:100-117``, ``init_fields.py:84-106``).
+The S, Z and (w, p) collocation matrices are affine in the implicit weight:
+``affine_operators`` assembles A0 and A1 once per run on the host, and every
+rebuild forms A0 + w A1 on the device before the batched inverses (the (w, p)
+systems with the reference's row and column scaling; the reference LU-solves
+them: the same solution) and the folds with the DCT-I and the radial
+derivatives.  The l = 0 pressure (its integral condition when
+ViscHeatFac ThExpNb != 0, ``update_wp.py:100-117``, ``init_fields.py:84-106``)
+does not depend on the weight: its inverse is folded into a fixed block of the
+(w, p) row operator once (``p0_block``).  ``s_inverse`` / ``z_inverse`` /
+``wp_inverse`` / ``p0_inverse`` are the host inverses (conduction state,
+checks).
 """
 from __future__ import annotations
 
@@ -66,6 +71,10 @@ def _halve(a, cols):
 
 
 def s_inverse(p, rad, prof, w0):
+    return _rows_inv(_s_matrix(p, rad, prof, w0))
+
+
+def _s_matrix(p, rad, prof, w0):
     R, D, D2 = rad["T"], rad["T1"], rad["T2"]
     N, rn, lp1 = p.n_r_max, rad["rnorm"], p.l_max + 1
     dL = (np.arange(lp1) * (np.arange(lp1) + 1.0))[:, None, None]
@@ -75,10 +84,16 @@ def s_inverse(p, rad, prof, w0):
     a[:, N - 1] = rn * R[N - 1]
     if p.n_cheb_max < N:
         a[:, (0, N - 1), p.n_cheb_max:] = 0.0
-    return _rows_inv(_halve(a, (0, N - 1)))
+    return _halve(a, (0, N - 1))
 
 
 def z_inverse(p, rad, prof, w0):
+    out = _rows_inv(_z_matrix(p, rad, prof, w0))
+    out[0] = 0.0
+    return out
+
+
+def _z_matrix(p, rad, prof, w0):
     R, D, D2 = rad["T"], rad["T1"], rad["T2"]
     N, rn, lp1 = p.n_r_max, rad["rnorm"], p.l_max + 1
     dL = (np.arange(lp1) * (np.arange(lp1) + 1.0))[:, None, None]
@@ -94,12 +109,25 @@ def z_inverse(p, rad, prof, w0):
     if p.n_cheb_max < N:
         a[:, (0, N - 1), p.n_cheb_max:] = 0.0
     a[0] = np.eye(N)   # no l = 0 equation
-    out = _rows_inv(_halve(a, (0, N - 1)))
+    return _halve(a, (0, N - 1))
+
+
+def wp_inverse(p, rad, prof, w0):
+    a = _wp_matrix(p, rad, prof, w0)
+    rm = np.abs(a).max(axis=2)
+    rm[rm == 0] = 1.0
+    fr = 1.0 / rm
+    a = fr[:, :, None] * a
+    cm = np.abs(a).max(axis=1)
+    cm[cm == 0] = 1.0
+    fc = 1.0 / cm
+    a = a * fc[:, None, :]
+    out = np.linalg.inv(a) * fc[:, :, None] * fr[:, None, :]
     out[0] = 0.0
     return out
 
 
-def wp_inverse(p, rad, prof, w0):
+def _wp_matrix(p, rad, prof, w0):
     R, D, D2, D3 = rad["T"], rad["T1"], rad["T2"], rad["T3"]
     N, rn, lp1 = p.n_r_max, rad["rnorm"], p.l_max + 1
     b = slice(1, N - 1)
@@ -128,17 +156,30 @@ def wp_inverse(p, rad, prof, w0):
             a[:, row, N + p.n_cheb_max:] = 0.0
     _halve(a, (0, N - 1, N, 2 * N - 1))
     a[0] = np.eye(2 * N)   # no l = 0 equation
-    rm = np.abs(a).max(axis=2)
-    rm[rm == 0] = 1.0
-    fr = 1.0 / rm
-    a = fr[:, :, None] * a
-    cm = np.abs(a).max(axis=1)
-    cm[cm == 0] = 1.0
-    fc = 1.0 / cm
-    a = a * fc[:, None, :]
-    out = np.linalg.inv(a) * fc[:, :, None] * fr[:, None, :]
-    out[0] = 0.0
+    return a
+
+
+def affine_operators(p, rad, prof):
+    """The S, Z and (w, p) collocation matrices as A0 + w A1 (each is affine in
+    the implicit weight w): {"sz": (A0, A1) of shape (2, l_max+1, N, N),
+    "wp": (A0, A1) of shape (l_max+1, 2N, 2N)}, for the device rebuild."""
+    out = {}
+    for key, f in (("sz", lambda w: np.stack([_s_matrix(p, rad, prof, w),
+                                               _z_matrix(p, rad, prof, w)])),
+                   ("wp", lambda w: _wp_matrix(p, rad, prof, w))):
+        a0 = f(0.0)
+        out[key] = (a0, f(1.0) - a0)
     return out
+
+
+def p0_block(p, rad, prof, k4wp, rpad6):
+    """Degree-0 block of the (w, p) row operator: p(l=0) and dp/dr(l=0) from the
+    pressure half of the right-hand side through the l = 0 pressure inverse."""
+    N = p.n_r_max
+    opp = np.concatenate([rad["CT"], rad["D1"] @ rad["CT"]])
+    mt = np.zeros((k4wp, rpad6))
+    mt[N:2 * N, 4 * N:6 * N] = (opp @ p0_inverse(p, rad, prof)).T
+    return mt
 
 
 def integral_row(rad, phys):
@@ -182,29 +223,6 @@ def p0_weights(p, rad, prof):
     """u (N,) with the l = 0 pressure's integral right-hand side u . Re s(l=0)."""
     r = rad["r"]
     return prof["ThExpNb"] * prof["alpha0"] * prof["temp0"] * prof["rho0"] * r * r * rad["q"]
-
-
-def row_operators(p, rad, prof, w0, k4, rpad3, k4wp, rpad6):
-    """(MT_s, MT_z, MT_wp): row-operator matrices of the S, Z and coupled (w, p)
-    solves (and the l = 0 pressure) for implicit weight w0."""
-    N, lp1 = p.n_r_max, p.l_max + 1
-    CT, D1, D2, D3 = rad["CT"], rad["D1"], rad["D2"], rad["D3"]
-    op3 = np.concatenate([CT, D1 @ CT, D2 @ CT])
-    out = []
-    for inv in (s_inverse(p, rad, prof, w0), z_inverse(p, rad, prof, w0)):
-        mt = np.zeros((lp1, k4, rpad3))
-        mt[:, :N, :3 * N] = np.einsum("rj,ljk->lkr", op3, inv)
-        out.append(mt)
-    inv = wp_inverse(p, rad, prof, w0)
-    opw = np.concatenate([CT, D1 @ CT, D2 @ CT, D3 @ CT])
-    opp = np.concatenate([CT, D1 @ CT])
-    mt = np.zeros((lp1, k4wp, rpad6))
-    mt[:, :2 * N, :4 * N] = np.einsum("rj,ljk->lkr", opw, inv[:, :N, :])
-    mt[:, :2 * N, 4 * N:6 * N] = np.einsum("rj,ljk->lkr", opp, inv[:, N:, :])
-    mt[0] = 0.0
-    mt[0, N:2 * N, 4 * N:6 * N] = (opp @ p0_inverse(p, rad, prof)).T
-    out.append(mt)
-    return tuple(out)
 
 
 def conduction_state(p, rad, prof):

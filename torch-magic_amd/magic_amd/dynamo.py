@@ -715,6 +715,17 @@ class DynamoRun:
         self.defer_checks = False  # check the pivots of a rebuild at the next step (no host wait)
         if self.dc:
             self._dc_storage()
+        elif self.anel:
+            # anelastic Chebyshev operators: S, Z and (w, p) affine in the implicit
+            # weight (assembled once on the host), the l = 0 pressure block fixed
+            from . import anelastic
+            ops = anelastic.affine_operators(self.p, self.res.rad, self.prof)
+            up = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=F64).to(dev)
+            (a0, a1), (w0, w1) = ops["sz"], ops["wp"]
+            self.anel_ops = dict(a0=up(a0.reshape(-1)), a1=up(a1.reshape(-1)),
+                                 w0=up(w0.reshape(-1)), w1=up(w1.reshape(-1)),
+                                 mt0=up(anelastic.p0_block(self.p, self.res.rad, self.prof,
+                                                           self.K4wp, self.Rpad6)))
     def _radial_operators(self):
         """Radial operators of the resolution, uploaded once and shared by every
         run of it (read-only): the DCT-I / derivative stacks folded into the
@@ -786,8 +797,6 @@ class DynamoRun:
         self._rebuild_launches(wimp_lin0)
         if self.cond_ic:
             self._build_coupled(wimp_lin0)
-        if self.anel and not self.dc:
-            self._build_anel(wimp_lin0)
         self.wimp_lin0 = wimp_lin0
         self.n_rebuild += 1
         self._queue_info_check()
@@ -809,18 +818,6 @@ class DynamoRun:
             mt = inner_core.z10_row_operator(p, rad, inv, self.K4, self.Rpad3)
             self.MT_z10.copy_(torch.as_tensor(mt), non_blocking=False)
 
-    def _build_anel(self, wimp_lin0: float):
-        """The anelastic S, Z and (w, p) row operators (and the l = 0 pressure)
-        for ``wimp_lin0``, assembled and inverted on the host
-        (magic_amd.anelastic), copied over the Boussinesq ones of the device
-        rebuild on the run's stream (B and J are the same in both)."""
-        from . import anelastic
-        ms, mz, mwp = anelastic.row_operators(self.p, self.res.rad, self.prof, wimp_lin0,
-                                              self.K4, self.Rpad3, self.K4wp, self.Rpad6)
-        self.MT4[0].copy_(torch.as_tensor(ms), non_blocking=False)
-        self.MT4[1].copy_(torch.as_tensor(mz), non_blocking=False)
-        self.MT["wp"].copy_(torch.as_tensor(mwp), non_blocking=False)
-
     def _rebuild_launches(self, wimp_lin0, ctl=None):
         """The launches of a rebuild (ten, whatever the number of systems).
         ctl = (wimp_dev, when) device pointers: the weight is read on the device
@@ -833,6 +830,12 @@ class DynamoRun:
         when = None if ctl is None else ctl[1]
         info = lambda key: ptr(self.gj_info[_INFO_ROWS.index(key)])
         self._build_all(wimp_lin0, ctl)
+        if self.anel:   # the anelastic S, Z and (w, p) matrices replace the Boussinesq ones
+            o, wdev = self.anel_ops, (None if ctl is None else ctl[0])
+            _lib.call("mh_affine_f64", ptr(o["a0"]), ptr(o["a1"]), ptr(self.mat_nn),
+                      2 * lp1 * N * N, wimp_lin0, wdev, when, st)
+            _lib.call("mh_affine_f64", ptr(o["w0"]), ptr(o["w1"]), ptr(self.mat_wp),
+                      lp1 * 4 * N * N, wimp_lin0, wdev, when, st)
         # S, Z (B, J): all degrees of all systems in one inverse and one fold;
         # their pivot-flag rows are consecutive
         _lib.call("mh_gj_inverse", ptr(self.mat_nn), ptr(self.mat_nn), ptr(self.gj_work),
@@ -859,6 +862,10 @@ class DynamoRun:
                   info("p0"), when, st)
         self._gemm(ptr(p0), 1, N, 0, ptr(self.Opp), 1, N, 0,
                    ptr(MT) + 8 * (N * self.Rpad6 + 4 * N), self.Rpad6, 1, 0, N, 2 * N, N, 1, when)
+        if self.anel:   # the anelastic l = 0 pressure block (host inverse, fixed)
+            o = self.anel_ops
+            _lib.call("mh_affine_f64", ptr(o["mt0"]), ptr(o["mt0"]), ptr(MT[0]),
+                      self.K4wp * self.Rpad6, 0.0, None, when, st)
 
     def _queue_info_check(self):
         """Queue the host copy of the pivot flags of all systems; they are checked
@@ -1769,10 +1776,9 @@ class DynamoRun:
         return st.dt
 
     def _no_dirk(self, what):
-        if self.dirk or self.cond_ic or self.anel:
-            raise NotImplementedError(f"{what}: BPR353 runs, anelastic runs and runs with a "
-                                      "conducting inner core are stepped with one_step / "
-                                      "run_steps")
+        if self.dirk or self.cond_ic:
+            raise NotImplementedError(f"{what}: BPR353 runs and runs with a conducting inner "
+                                      "core are stepped with one_step / run_steps")
 
     def step_async(self):
         """Queue one whole step with the decisions on the device (one HIP graph
@@ -1814,10 +1820,10 @@ class DynamoRun:
         one launch sequence (one HIP graph replay when use_graphs) and the host
         never waits inside the loop.  Same decisions, weights and results as
         n_steps calls of one_step; dt as for one_step's first call.  Returns the
-        dt of the last step.  BPR353 steps, anelastic steps and steps with a
-        conducting inner core (whose operators are rebuilt on the host) are
+        dt of the last step.  BPR353 steps and steps with a conducting inner
+        core (whose coupled operators are rebuilt on the host) are
         host-controlled one_step calls."""
-        if self.dirk or self.cond_ic or self.anel:
+        if self.dirk or self.cond_ic:
             for _ in range(n_steps):
                 dt = self.one_step(self.n_step + 1, dt)
             return dt
