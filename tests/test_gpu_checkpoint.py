@@ -14,15 +14,20 @@ CASE = dict(cfg=dict(l_max=16, n_r_max=33, ek=1e-3, ra=1e5, pr=1.0, prmag=5.0,
             dt=1e-3, noise=dict(amp=1.0, seed=7), steps=30, at=12)
 
 
-def _fresh(precision="fp64"):
+# the same with the finite-difference (double-curl) step, and with the anelastic step
+EXTRA = {"fd": dict(radial_scheme="FD", fd_order=2, fd_order_bound=2),
+         "anel": dict(strat=1.0, polind=2.0, g1=1.0)}
+
+
+def _fresh(precision="fp64", extra=None):
     from magic_amd.dynamo import DynamoRun
-    run = DynamoRun(dict(CASE["cfg"], precision=precision))
+    run = DynamoRun(dict(CASE["cfg"], precision=precision, **(extra or {})))
     run.initialize_dt(CASE["dt"])
     return run
 
 
-def _started(precision="fp64"):
-    run = _fresh(precision)
+def _started(precision="fp64", extra=None):
+    run = _fresh(precision, extra)
     run.initialize_fields(noise_amp=CASE["noise"]["amp"], noise_seed=CASE["noise"]["seed"])
     run.setup_initial_state()
     return run
@@ -76,6 +81,42 @@ def test_restart_continues_bit_for_bit(tmp_path, precision):
     b.load_checkpoint(path)
     b.run_steps(CASE["steps"] - CASE["at"], b.dt_hist[0])
     assert b.dt_log == ref.dt_log[CASE["at"]:] and b.time == ref.time
+    for k in ("s", "z", "w", "p", "b", "aj"):
+        assert torch.equal(b.f[k], ref.f[k]), k
+    for r in (ref, a, b):
+        r.close()
+
+
+@pytest.mark.parametrize("kind", ["fd", "anel"])
+def test_restart_continues_bit_for_bit_fd_and_anelastic(tmp_path, kind):
+    """A finite-difference (double-curl) run and an anelastic run restored from
+    a .pt file written after step 12 continue bit for bit across the later dt
+    changes, host-stepped and with the dt decision on the device."""
+    import torch
+    path = tmp_path / "restart.pt"
+    extra = EXTRA[kind]
+    ref = _started(extra=extra)
+    dt = CASE["dt"]
+    for n in range(1, CASE["at"] + 1):
+        dt = ref.one_step(n, dt)
+    ref.save_checkpoint(path)
+    rebuilds_at = ref.n_rebuild
+    for n in range(CASE["at"] + 1, CASE["steps"] + 1):
+        dt = ref.one_step(n, dt)
+    assert ref.n_rebuild > rebuilds_at, "the continuation should cross a dt change"
+    a = _fresh(extra=extra)
+    step, t = a.load_checkpoint(path)
+    dt = a.dt_hist[0]
+    for n in range(step + 1, CASE["steps"] + 1):
+        dt = a.one_step(n, dt)
+    assert a.dt_log == ref.dt_log[CASE["at"]:] and a.time == ref.time
+    for k in ("s", "z", "w", "dw", "ddw", "p", "b", "aj"):
+        assert torch.equal(a.f[k], ref.f[k]), k
+    b = _fresh(extra=extra)
+    b.use_graphs = True
+    b.load_checkpoint(path)
+    b.run_steps(CASE["steps"] - CASE["at"], b.dt_hist[0])
+    assert b.dt_log == ref.dt_log[CASE["at"]:]
     for k in ("s", "z", "w", "p", "b", "aj"):
         assert torch.equal(b.f[k], ref.f[k]), k
     for r in (ref, a, b):
