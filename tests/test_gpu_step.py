@@ -273,7 +273,7 @@ def test_fused_grid_step_matches_separate_kernels(l_max):
 
 
 @pytest.mark.parametrize("name", ["bench16", "c1", "bprdt", "fd2mhd", "fdbpr", "fdanel", "anel21",
-                                  "bpranel2"])
+                                  "bpranel2", "condic", "rotic", "bprrotic4"])
 def test_graph_replay_matches_eager(name):
     """The step's radial and lm phases captured as HIP graphs (one per parity of
     the explicit-history buffers, weights read from device memory) and replayed
@@ -340,6 +340,33 @@ def test_device_step_control_matches_host_steps_anelastic():
     np.testing.assert_allclose(dev.dt_log, odts, rtol=1e-9)
     e, oe = np.array(dev.energies()), np.array(orc.energies())
     assert np.all(np.abs(e - oe) <= 1e-9 * np.abs(oe)), (e, oe)
+
+
+def test_device_step_control_matches_host_steps_rotating_inner_core():
+    """A conducting, rotating inner core under device step control: the coupled
+    outer/inner B and J operators and the z(l=1, m=0) operator are rebuilt on the
+    device (A0 + w A1, inverses, folds) inside the captured step whenever the
+    Courant rule changes dt; dt sequence, rebuild count, omega_ic and fields
+    (outer and inner) equal the host-decided eager steps bit for bit."""
+    import torch
+    case = dict(cfg=dict(l_max=16, n_r_max=33, n_cheb_max=31, n_cheb_ic_max=15,
+                         sigma_ratio=1.0, kbotb=3, nRotIC=1, dtmax=1e-3),
+                dt=1e-3, noise=dict(amp=1.0, seed=7), steps=20)
+    host = _run(case)
+    dev = _run(case)
+    dev.use_graphs = True
+    dt = case["dt"]
+    for n in range(1, case["steps"] + 1):
+        dt = host.one_step(n, dt)
+    dev.run_steps(8, case["dt"])
+    dev.run_steps(case["steps"] - 8, dev.dt_log[-1])
+    assert dev.dt_log == host.dt_log
+    assert dev.n_rebuild == host.n_rebuild and host.n_rebuild >= 3, host.n_rebuild
+    assert dev.omega_ic == host.omega_ic
+    for k in ("s", "z", "w", "p", "b", "aj", "db", "ddw"):
+        assert torch.equal(dev.f[k], host.f[k]), k
+    for k in ("b_ic", "aj_ic"):
+        assert torch.equal(dev.fic[k], host.fic[k]), k
 
 
 def test_device_step_control_matches_host_steps_finite_differences():

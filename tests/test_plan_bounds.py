@@ -212,14 +212,19 @@ def test_conducting_inner_core_launches_stay_inside_their_tensors(cpu_standin, s
     plan (each BPR353 stage) lie inside the run's tensors."""
     cfg = dict(l_max=16, n_r_max=17, n_cheb_ic_max=7, sigma_ratio=1.0, kbotb=3, ek=1e-3,
                ra=1e5, pr=1.0, prmag=5.0, radratio=0.35, dtmax=1e-4, time_scheme=scheme)
-    with _lib.record():
+    with _lib.record() as rec:
         run = dyn.DynamoRun(cfg, device="cpu")
         run.initialize_dt(cfg["dtmax"])
         run._rebuild_launches(0.5e-4)
-        run._build_coupled(0.5e-4)
     assert run.cond_ic
     ext = _extents(run)
     _check_row_jobs(run, ext)
+    # the device rebuild of the coupled operators: affine assembly, inverses, folds
+    aff = [a for _, name, a in rec.calls if name == "mh_affine_f64"]
+    for A0, A1, out, cnt in (a[:4] for a in aff):
+        for v in (A0, A1, out):
+            _inside(ext, v, v + 8 * cnt, "affine operand")
+    assert _check_rebuild_calls(rec.calls, ext) == 10 + 1 + 4 + 4  # + inverse, 4 folds, 4 fills
     L, N, nic = run.L, run.N, run.p.n_r_ic_max
     stages = (1, 2, 3, 4) if run.dirk else (0,)
     n_ic_calls = 0
@@ -265,7 +270,6 @@ def test_rotating_inner_core_launches_stay_inside_their_tensors(cpu_standin, sch
         run = dyn.DynamoRun(cfg, device="cpu")
         run.initialize_dt(cfg["dtmax"])
         run._rebuild_launches(0.5e-4)
-        run._build_coupled(0.5e-4)
     assert run.rot_ic
     ext = _extents(run)
     _check_row_jobs(run, ext)
@@ -448,7 +452,7 @@ def test_finite_difference_launches_stay_inside_their_tensors(cpu_standin, cfg):
     N, L, lp1 = run.N, run.L, run.p.l_max + 1
     n = _check_rebuild_calls(calls, ext)
     aff = [a for _, name, a in calls if name == "mh_affine_f64"]
-    assert n >= 6 and len(aff) == 2
+    assert n >= 6 and len(aff) == 2 + int(run.cond_ic) + int(run.rot_ic)
     for A0, A1, out, cnt in (a[:4] for a in aff):
         for v in (A0, A1, out):
             _inside(ext, v, v + 8 * cnt, "affine operand")

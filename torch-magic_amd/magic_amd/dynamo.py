@@ -695,6 +695,24 @@ class DynamoRun:
             self.MT_ic12 = (torch.as_tensor(mt, dtype=F64).to(dev), 2 * Nic, rp(2 * Nic))
         if self.rot_ic:
             self.MT_z10 = torch.zeros(self.K4, self.Rpad3, dtype=F64, device=dev)
+        if self.cond_ic:
+            # coupled B, J (and z(l=1, m=0)) matrices as A0 + w A1, refreshed and
+            # inverted on the device at each rebuild
+            from . import inner_core
+            up = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=F64).to(dev)
+            NT = self.p.n_r_tot
+            (b0, b1), (j0, j1) = inner_core.coupled_affine(self.p, r.rad, self.ic)
+            self.ic_ops = dict(a0=up(np.stack([b0, j0]).reshape(-1)),
+                               a1=up(np.stack([b1, j1]).reshape(-1)),
+                               op_ic=up(inner_core.inner_fold(self.ic)))
+            self.mat_c = torch.empty(2 * lp1, NT, NT, dtype=F64, device=dev)
+            self.gj_work_c = torch.empty(2 * lp1 * 2 * NT * NT if NT > 208 else 1, dtype=F64,
+                                         device=dev)
+            self.gj_scal_c = torch.empty(2 * lp1 * 2 * NT, dtype=F64, device=dev)
+            if self.rot_ic:
+                z0, z1 = inner_core.z10_affine(self.p, r.rad, self.rot_c["c_dt_z10"])
+                self.ic_ops.update(z0=up(z0), z1=up(z1))
+                self.mat_z10 = torch.empty(N, N, dtype=F64, device=dev)
         self.MT_d1, self.MT_d1r2, self.MT_d12, self.MT_d123 = (
             ops[k] for k in ("MT_d1", "MT_d1r2", "MT_d12", "MT_d123"))
         self.mat_nn = torch.empty(self.nsys * lp1, N, N, dtype=F64, device=dev)
@@ -795,36 +813,56 @@ class DynamoRun:
         the radial derivatives into the inverses.
         """
         self._rebuild_launches(wimp_lin0)
-        if self.cond_ic:
-            self._build_coupled(wimp_lin0)
         self.wimp_lin0 = wimp_lin0
         self.n_rebuild += 1
         self._queue_info_check()
 
-    def _build_coupled(self, wimp_lin0: float):
-        """The coupled outer/inner B and J row operators of a conducting inner
-        core (and the z(l=1, m=0) operator of a rotating one) for ``wimp_lin0``
-        (host assembly and inversion, magic_amd.inner_core), copied into the
-        run's operator storage on its stream."""
-        from . import inner_core
-        p, rad = self.p, self.res.rad
-        inv = dict(zip(("b", "j"), inner_core.coupled_inverses(p, rad, self.ic, wimp_lin0)))
-        for k in ("b", "j"):
-            (mo, _, _), (mi, _, _) = inner_core.row_operators(p, rad, self.ic, inv[k], self.K4c)
-            self.MTc[k]["oc"][0].copy_(torch.as_tensor(mo), non_blocking=False)
-            self.MTc[k]["ic"][0].copy_(torch.as_tensor(mi), non_blocking=False)
+    def _coupled_launches(self, wimp_lin0, ctl=None):
+        """The coupled outer/inner B and J row operators of a conducting inner core
+        (and the z(l=1, m=0) operator of a rotating one) for ``wimp_lin0``:
+        A0 + w A1, batched inverses with row scaling (degree 0 dropped), and the
+        folds with the outer (I, D1, D2) and inner (I, D1, D2) operators."""
+        p, N, lp1, st = self.p, self.N, self.p.l_max + 1, stream_handle()
+        Nic, NT = p.n_r_ic_max, p.n_r_tot
+        wdev, when = (None, None) if ctl is None else ctl
+        o = self.ic_ops
+        nn = NT * NT
+        _lib.call("mh_affine_f64", ptr(o["a0"]), ptr(o["a1"]), ptr(self.mat_c), 2 * lp1 * nn,
+                  wimp_lin0, wdev, when, st)
+        _lib.call("mh_gj_inverse", ptr(self.mat_c), ptr(self.mat_c), ptr(self.gj_work_c),
+                  ptr(self.gj_scal_c), NT, 2 * lp1, 1,
+                  ptr(self.gj_info[_INFO_ROWS.index("b")]), when, st)
+        for kk, key in enumerate(("b", "j")):
+            A = ptr(self.mat_c) + 8 * kk * lp1 * nn
+            for part, op, n_in, off in (("oc", self.Op3, N, 0), ("ic", o["op_ic"], Nic, N)):
+                MT, R, Rp = self.MTc[key][part]
+                # MT[l][k][r] = sum_j op[r][j] inv_l[off + j][k]
+                self._gemm(A + 8 * off * NT, 1, NT, nn, ptr(op), 1, n_in, 0, ptr(MT), Rp, 1,
+                           self.K4c * Rp, NT, R, n_in, lp1, when)
+                self._zero(MT[0], when)
         if self.rot_ic:
-            inv = inner_core.z10_inverse(p, rad, wimp_lin0, self.rot_c["c_dt_z10"])
-            mt = inner_core.z10_row_operator(p, rad, inv, self.K4, self.Rpad3)
-            self.MT_z10.copy_(torch.as_tensor(mt), non_blocking=False)
+            _lib.call("mh_affine_f64", ptr(o["z0"]), ptr(o["z1"]), ptr(self.mat_z10), N * N,
+                      wimp_lin0, wdev, when, st)
+            _lib.call("mh_gj_inverse", ptr(self.mat_z10), ptr(self.mat_z10), ptr(self.gj_work),
+                      ptr(self.gj_scal), N, 1, 1,
+                      ptr(self.gj_info[_INFO_ROWS.index("p0")]) + 4, when, st)
+            self._gemm(ptr(self.mat_z10), 1, N, 0, ptr(self.Op3), 1, N, 0, ptr(self.MT_z10),
+                       self.Rpad3, 1, 0, N, 3 * N, N, 1, when)
 
     def _rebuild_launches(self, wimp_lin0, ctl=None):
-        """The launches of a rebuild (ten, whatever the number of systems).
-        ctl = (wimp_dev, when) device pointers: the weight is read on the device
+        """The launches of a rebuild (ten for the Boussinesq Chebyshev systems;
+        the anelastic, finite-difference and coupled inner-core operators add
+        their own).  ctl = (wimp_dev, when) device pointers: the weight is read on the device
         and every kernel returns at once unless *when != 0 (the rebuild inside
         a captured step)."""
         if self.dc:
-            return self._rebuild_launches_dc(wimp_lin0, ctl)
+            self._rebuild_launches_dc(wimp_lin0, ctl)
+        else:
+            self._rebuild_launches_cheb(wimp_lin0, ctl)
+        if self.cond_ic:   # after the others: its pivot flags fill the B and J rows
+            self._coupled_launches(wimp_lin0, ctl)
+
+    def _rebuild_launches_cheb(self, wimp_lin0, ctl=None):
         N, lp1, ns = self.N, self.p.l_max + 1, self.nsys
         st = stream_handle()
         when = None if ctl is None else ctl[1]
@@ -1776,9 +1814,9 @@ class DynamoRun:
         return st.dt
 
     def _no_dirk(self, what):
-        if self.dirk or self.cond_ic:
-            raise NotImplementedError(f"{what}: BPR353 runs and runs with a conducting inner "
-                                      "core are stepped with one_step / run_steps")
+        if self.dirk:
+            raise NotImplementedError(f"{what}: BPR353 runs are stepped with one_step / "
+                                      "run_steps")
 
     def step_async(self):
         """Queue one whole step with the decisions on the device (one HIP graph
@@ -1820,10 +1858,9 @@ class DynamoRun:
         one launch sequence (one HIP graph replay when use_graphs) and the host
         never waits inside the loop.  Same decisions, weights and results as
         n_steps calls of one_step; dt as for one_step's first call.  Returns the
-        dt of the last step.  BPR353 steps and steps with a conducting inner
-        core (whose coupled operators are rebuilt on the host) are
-        host-controlled one_step calls."""
-        if self.dirk or self.cond_ic:
+        dt of the last step.  BPR353 steps are host-controlled one_step
+        calls."""
+        if self.dirk:
             for _ in range(n_steps):
                 dt = self.one_step(self.n_step + 1, dt)
             return dt

@@ -9,13 +9,15 @@ derivative match at the ICB (sigma_ratio on the outer dj/dr for J), N+1..NT-2
 inner diffusion, NT-1 the limit at the centre; boundary columns halved, rows
 scaled by their largest entry.
 
-The matrices change with the implicit weight only, and they are small
-((l_max + 1) systems of n_r_tot <= 3 n_r_max / 2 unknowns), so they are
-assembled and inverted on the host at each rebuild; the DCT-I and the radial
-derivatives of both shells are folded into the inverses here, giving the two
-row-operator matrices of each system: outer outputs (f, f', f'') on the
-n_r_max levels and inner outputs on the n_r_ic_max levels, both acting on the
-coupled right-hand side (reference ``update_b.py:634-748``).
+The matrices (and the z(l=1, m=0) matrix of a rotating core) are affine in
+the implicit weight: ``coupled_affine`` / ``z10_affine`` assemble A0 and A1
+once on the host, and every rebuild forms A0 + w A1 on the device, inverts all
+degrees in one launch and folds the DCT-I and the radial derivatives of both
+shells into the two row-operator matrices of each system: outer outputs
+(f, f', f'') on the n_r_max levels and inner outputs on the n_r_ic_max levels,
+both acting on the coupled right-hand side (reference ``update_b.py:634-748``).
+``coupled_inverses`` / ``row_operators`` / ``z10_inverse`` are the host
+equivalents (checks).
 """
 from __future__ import annotations
 
@@ -27,6 +29,25 @@ import numpy as np
 def coupled_inverses(p, rad: dict, ic: dict, wimp_lin0: float):
     """(b_inv, j_inv): (l_max + 1, NT, NT) inverses with the row scaling folded
     in (inverse times diag(row factors)); degree 0 is zero."""
+    out = []
+    for a in coupled_matrices(p, rad, ic, wimp_lin0):
+        fac = 1.0 / np.abs(a).max(axis=2)
+        inv = np.linalg.inv(fac[:, :, None] * a) * fac[:, None, :]
+        inv[0] = 0.0
+        out.append(inv)
+    return tuple(out)
+
+
+def coupled_affine(p, rad: dict, ic: dict):
+    """The coupled B and J matrices as A0 + w A1 (affine in the implicit weight):
+    ((A0_b, A1_b), (A0_j, A1_j)), each (l_max + 1, NT, NT), degree 0 the identity."""
+    m0, m1 = coupled_matrices(p, rad, ic, 0.0), coupled_matrices(p, rad, ic, 1.0)
+    return tuple((a, b - a) for a, b in zip(m0, m1))
+
+
+def coupled_matrices(p, rad: dict, ic: dict, wimp_lin0: float):
+    """(b, j): (l_max + 1, NT, NT) collocation matrices of the coupled outer and
+    inner systems (degree 0: the identity, no equation)."""
     R, D, D2 = rad["T"], rad["T1"], rad["T2"]
     N, Nic, NT = p.n_r_max, p.n_r_ic_max, p.n_r_tot
     nci = p.n_cheb_ic_max
@@ -39,6 +60,8 @@ def coupled_inverses(p, rad: dict, ic: dict, wimp_lin0: float):
     c_in, dc_in, d2c_in = cheb[:, inner].T, dcheb[:, inner].T, d2cheb[:, inner].T
     orc = ic["O_r_ic"][inner, None]
     out = {k: np.zeros((p.l_max + 1, NT, NT)) for k in ("b", "j")}
+    for k in ("b", "j"):
+        out[k][0] = np.eye(NT)
     for l in range(1, p.l_max + 1):
         dL = float(l * (l + 1))
         for key in ("b", "j"):
@@ -61,9 +84,15 @@ def coupled_inverses(p, rad: dict, ic: dict, wimp_lin0: float):
                 a[:, c] *= rad.get("bfac", 0.5)
             for c in (N, NT - 1):     # inner even-Chebyshev end coefficients
                 a[:, c] *= 0.5
-            fac = 1.0 / np.abs(a).max(axis=1)
-            out[key][l] = np.linalg.inv(fac[:, None] * a) * fac[None, :]
+            out[key][l] = a
     return out["b"], out["j"]
+
+
+def inner_fold(ic: dict) -> np.ndarray:
+    """(3 n_r_ic_max, n_r_ic_max): inner values, d/dr and d2/dr2 from the inner
+    even-Chebyshev coefficients."""
+    Ci, D1i, D2i = ic["CT"], ic["D1"], ic["D2"]
+    return np.concatenate([Ci, D1i @ Ci, D2i @ Ci])
 
 
 def row_operators(p, rad: dict, ic: dict, inv: np.ndarray, k4: int):
@@ -103,6 +132,18 @@ def z10_inverse(p, rad: dict, wimp_lin0: float, c_dt_z10: float) -> np.ndarray:
     row scaling folded in (reference ``update_z.py:244-298``, time-integrated
     rotation): the l = 1 toroidal diffusion rows, z = 0 at the CMB, and at the ICB
     the angular-momentum balance c_dt_z10 z + wimp_lin0 (2 z / r - dz/dr)."""
+    a = z10_matrix(p, rad, wimp_lin0, c_dt_z10)
+    fac = 1.0 / np.abs(a).max(axis=1)
+    return np.linalg.inv(fac[:, None] * a) * fac[None, :]
+
+
+def z10_affine(p, rad: dict, c_dt_z10: float):
+    """(A0, A1) with the z(l=1, m=0) matrix A0 + w A1."""
+    a0 = z10_matrix(p, rad, 0.0, c_dt_z10)
+    return a0, z10_matrix(p, rad, 1.0, c_dt_z10) - a0
+
+
+def z10_matrix(p, rad: dict, wimp_lin0: float, c_dt_z10: float) -> np.ndarray:
     R, D, D2 = rad["T"], rad["T1"], rad["T2"]
     N, rn = p.n_r_max, rad["rnorm"]
     or1, o2 = rad["or1"], rad["or2"][:, None]
@@ -112,8 +153,7 @@ def z10_inverse(p, rad: dict, wimp_lin0: float, c_dt_z10: float) -> np.ndarray:
     if p.n_cheb_max < N:
         a[(0, N - 1), p.n_cheb_max:] = 0.0
     a[:, (0, N - 1)] *= rad.get("bfac", 0.5)
-    fac = 1.0 / np.abs(a).max(axis=1)
-    return np.linalg.inv(fac[:, None] * a) * fac[None, :]
+    return a
 
 
 def z10_row_operator(p, rad: dict, inv: np.ndarray, k4: int, rpad: int) -> np.ndarray:
