@@ -589,6 +589,12 @@ typedef struct {
   const int* info;               /* optional: pivot flags [info_rows][info_cols] of the last rebuild */
   int N, n_schedule, log_cap, info_rows, info_cols, pad;
   double alpha, BuoFac;
+  /* scheme 1 (BPR353, time_scheme.py:157-326): the stage weights go to w_stage
+   * (4 x 16 doubles: per stage [1, bimp_0, bexp_0, bexp_1, wimp_lin0 BuoFac, -, -, -,
+   * bimp_1, bexp_2, bimp_2, bimp_3], all scaled by dt), wimp_lin0 = dt / 2 and
+   * both dt history entries are dt; scheme 0 (CNAB2) writes w */
+  double* w_stage;
+  int scheme, pad2;
 } MhStepCtlArgs;
 
 /* The host decisions of one CNAB2 step taken on the device, so that a whole

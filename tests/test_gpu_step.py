@@ -303,6 +303,35 @@ _DT_CASE = dict(cfg=dict(l_max=16, n_r_max=33, ek=1e-3, ra=1e5, pr=1.0, prmag=5.
                 dt=1e-3, noise=dict(amp=1.0, seed=7), steps=30)
 
 
+def test_device_step_control_matches_host_steps_bpr353():
+    """BPR353 under device step control: each step one captured sequence of the
+    three radial loops, the dt decision at stage 1 (this scheme's Courant
+    factors) with the stage weights and wimp_lin0 = dt/2 written on the device,
+    the rebuild when dt changed, and the four lm loops; against host-decided
+    steps bit for bit (dt sequence, rebuilds, time, fields) and the reference's
+    dt sequence of traj_bprdt (the same case)."""
+    import torch
+    case = dict(_DT_CASE, cfg=dict(_DT_CASE["cfg"], time_scheme="BPR353"))
+    host = _run(case)
+    dev = _run(case)
+    dev.use_graphs = True
+    dt = case["dt"]
+    for n in range(1, case["steps"] + 1):
+        dt = host.one_step(n, dt)
+    dev.run_steps(10, case["dt"])
+    dev.run_steps(case["steps"] - 10, dev.dt_log[-1])
+    assert dev.dt_log == host.dt_log
+    assert dev.n_rebuild == host.n_rebuild and host.n_rebuild >= 4, host.n_rebuild
+    assert dev.time == host.time and dev.n_step == host.n_step == case["steps"]
+    for k in ("s", "z", "w", "p", "b", "aj", "db", "ddw"):
+        assert torch.equal(dev.f[k], host.f[k]), k
+    _, g = _load("bprdt")
+    np.testing.assert_allclose(dev.dt_log, g["dt"], rtol=1e-12, atol=0)
+    e, ge = np.array(dev.energies()), g["energies"][-1]
+    assert np.all(np.abs(e - ge) <= 1e-9 * np.abs(ge)), (e, ge)
+    assert [k for k in dev._graphs if k[0] == "step"] == [("step", "dirk")]
+
+
 def test_device_step_control_matches_host_steps_anelastic():
     """The anelastic Chebyshev step under device step control: noise of
     amplitude 1 makes the Courant rule change dt six times in 20 steps, and each

@@ -200,7 +200,7 @@ class StepCtlArgs(C.Structure):
     _fields_ = [("cfl", P), ("delxr2", P), ("delxh2", P), ("state", P), ("w", P),
                 ("dt_schedule", P), ("dt_log", P), ("info", P), ("N", I), ("n_schedule", I),
                 ("log_cap", I), ("info_rows", I), ("info_cols", I), ("pad", I), ("alpha", D),
-                ("BuoFac", D)]
+                ("BuoFac", D), ("w_stage", P), ("scheme", I), ("pad2", I)]
 
 
 class DcArgs(C.Structure):

@@ -89,8 +89,9 @@ __global__ __launch_bounds__(256) void k_step_control(MhStepCtlArgs a) {
     new_dt = dt_new != dt;
   }
   if (a.dt_log != nullptr && a.log_cap > 0) a.dt_log[(n - 1) % a.log_cap] = dt_new;
-  const double dt1 = dt_new, dt2 = st->dt1;
-  const double wl0 = a.alpha * dt_new;
+  const bool dirk = a.scheme == 1;
+  const double dt1 = dt_new, dt2 = dirk ? dt_new : st->dt1;
+  const double wl0 = dirk ? 0.5 * dt_new : a.alpha * dt_new;
   // step_time.py:1628-1629: rebuild on the first step or a new dt (and once
   // when no operators were built yet: wimp_lin0 is NaN)
   const bool rebuild = (n == 1) || new_dt || __builtin_isnan(st->wimp_lin0);
@@ -98,11 +99,38 @@ __global__ __launch_bounds__(256) void k_step_control(MhStepCtlArgs a) {
     st->wimp_lin0 = wl0;
     st->n_rebuild += 1;
   }
-  a.w[0] = 1.0;
-  a.w[1] = (1.0 - a.alpha) * dt1;
-  a.w[2] = (1.0 + 0.5 * dt1 / dt2) * dt1;
-  a.w[3] = -0.5 * dt1 * dt1 / dt2;
-  a.w[4] = a.alpha * dt1 * a.BuoFac;
+  if (dirk) {
+    // BPR353 tableaux (time_scheme.py:186-200), scaled by dt (set_weights, :230-238)
+    const double B[5][4] = {{0.0, 0.0, 0.0, 0.0},
+                            {0.5, 0.5, 0.0, 0.0},
+                            {5.0 / 18.0, -1.0 / 9.0, 0.5, 0.0},
+                            {0.5, 0.0, 0.0, 0.5},
+                            {0.25, 0.0, 0.75, -0.5}};
+    const double E[5][3] = {{0.0, 0.0, 0.0},
+                            {1.0, 0.0, 0.0},
+                            {4.0 / 9.0, 2.0 / 9.0, 0.0},
+                            {0.25, 0.0, 0.75},
+                            {0.25, 0.0, 0.75}};
+    for (int k = 1; k <= 4; ++k) {
+      double* w = a.w_stage + 16 * (k - 1);
+      for (int q = 0; q < 16; ++q) w[q] = 0.0;
+      w[0] = 1.0;
+      w[1] = B[k][0] * dt1;
+      w[2] = E[k][0] * dt1;
+      w[3] = E[k][1] * dt1;
+      w[4] = 0.5 * dt1 * a.BuoFac;
+      w[8] = B[k][1] * dt1;
+      w[9] = E[k][2] * dt1;
+      w[10] = B[k][2] * dt1;
+      w[11] = B[k][3] * dt1;
+    }
+  } else {
+    a.w[0] = 1.0;
+    a.w[1] = (1.0 - a.alpha) * dt1;
+    a.w[2] = (1.0 + 0.5 * dt1 / dt2) * dt1;
+    a.w[3] = -0.5 * dt1 * dt1 / dt2;
+    a.w[4] = a.alpha * dt1 * a.BuoFac;
+  }
   st->dt1 = dt1;
   st->dt2 = dt2;
   st->time = st->time + dt_new;
@@ -117,7 +145,8 @@ extern "C" int mh_step_control(const MhStepCtlArgs* args, void* stream) {
   if (a.N < 1 || a.cfl == nullptr || a.state == nullptr || a.w == nullptr || a.delxr2 == nullptr ||
       a.delxh2 == nullptr || (a.dt_schedule != nullptr && a.n_schedule < 0) ||
       (a.dt_log != nullptr && a.log_cap < 1) ||
-      (a.info != nullptr && (a.info_rows < 1 || a.info_cols < 1)))
+      (a.info != nullptr && (a.info_rows < 1 || a.info_cols < 1)) ||
+      (a.scheme != 0 && a.scheme != 1) || (a.scheme == 1 && a.w_stage == nullptr))
     return MH_EINVAL;
   hipLaunchKernelGGL(k_step_control, dim3(1), dim3(256), 0, mh_stream(stream), a);
   return mh_launch_status();

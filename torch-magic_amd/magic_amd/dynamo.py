@@ -571,7 +571,10 @@ class DynamoRun:
     def _plan(self, kind):
         """The recorded launch plan of a phase for the current parity (for
         BPR353: for the current stage)."""
-        key = (kind, "stage", self.stage) if self.dirk else (kind, self._par)
+        if kind == "step":   # a whole step: one plan per parity (CNAB2) or the four stages
+            key = (kind, "dirk") if self.dirk else (kind, self._par)
+        else:
+            key = (kind, "stage", self.stage) if self.dirk else (kind, self._par)
         plan = self._plans.get(key)
         if plan is None:
             with _lib.record() as rec:
@@ -1754,6 +1757,8 @@ class DynamoRun:
         a.dt_log, a.log_cap = ptr(c["log"]), self.LOG_CAP
         a.info, a.info_rows, a.info_cols = ptr(self.gj_info), self.gj_info.shape[0], self.gj_info.shape[1]
         a.N, a.alpha, a.BuoFac = self.N, self.p.alpha, self.p.BuoFac
+        if self.dirk:
+            a.w_stage, a.scheme = ptr(self.wdev_stage), 1
         with self._t("step_ctl"):
             _lib.call("mh_step_control", _lib.ref(a), stream_handle())
 
@@ -1761,12 +1766,27 @@ class DynamoRun:
         """A whole step as one launch sequence: radial loop, device step control,
         the rebuild launches (no-ops unless the step control asks for a
         rebuild) and the lm loop."""
+        base = ptr(self._ctl["state"])
+        ctl = (base + _lib.StepState.wimp_lin0.offset, base + _lib.StepState.rebuild.offset)
+        if self.dirk:
+            # BPR353: radial loops of stages 1-3 (explicit slots 0-2), the decision
+            # and the rebuild after the first, the lm loop of every stage
+            for k in range(1, 5):
+                self.stage = k
+                if k <= 3:
+                    self.expl_cur = self.explS[k - 1]
+                    self._radial_launches()
+                    if k == 1:
+                        self._ctl_launch()
+                        with self._t("rebuild"):
+                            self._rebuild_launches(0.0, ctl)
+                self._lm_launches()
+            self.expl_cur = self.explS[0]
+            return
         self._radial_launches()
         self._ctl_launch()
-        base = ptr(self._ctl["state"])
         with self._t("rebuild"):
-            self._rebuild_launches(0.0, (base + _lib.StepState.wimp_lin0.offset,
-                                         base + _lib.StepState.rebuild.offset))
+            self._rebuild_launches(0.0, ctl)
         self._lm_launches()
 
     def _ctl_push(self, dt):
@@ -1813,19 +1833,14 @@ class DynamoRun:
             self._check_info()
         return st.dt
 
-    def _no_dirk(self, what):
-        if self.dirk:
-            raise NotImplementedError(f"{what}: BPR353 runs are stepped with one_step / "
-                                      "run_steps")
-
     def step_async(self):
         """Queue one whole step with the decisions on the device (one HIP graph
         when use_graphs); no host wait.  Call ctl_begin(dt) once before and
         ctl_end() to bring the stepping state back to the host."""
-        self._no_dirk("step_async")
         self._exec("step")
-        self.expl_cur, self.expl_prev = self.expl_prev, self.expl_cur
-        self._par ^= 1
+        if not self.dirk:   # CNAB2: the explicit-history buffers swap roles
+            self.expl_cur, self.expl_prev = self.expl_prev, self.expl_cur
+            self._par ^= 1
         self._ctl["queued"] += 1
         if self._ctl["queued"] - self._ctl["pulled"] >= self.LOG_CAP:
             self._ctl_pull()
@@ -1858,12 +1873,7 @@ class DynamoRun:
         one launch sequence (one HIP graph replay when use_graphs) and the host
         never waits inside the loop.  Same decisions, weights and results as
         n_steps calls of one_step; dt as for one_step's first call.  Returns the
-        dt of the last step.  BPR353 steps are host-controlled one_step
-        calls."""
-        if self.dirk:
-            for _ in range(n_steps):
-                dt = self.one_step(self.n_step + 1, dt)
-            return dt
+        dt of the last step (CNAB2 or BPR353)."""
         self.ctl_begin(dt)
         for _ in range(n_steps):
             self.step_async()
