@@ -673,6 +673,15 @@ int mh_impl_dc(const MhDcArgs* args, void* stream);
 int mh_affine_f64(const double* A0, const double* A1, double* out, long n, double w,
                   const double* wdev, const int* when, void* stream);
 
+/* mh_band_factor / mh_band_solve that do nothing unless *when != 0 (`when`
+ * may be NULL: always run); the finite-difference rebuild inside a captured
+ * step uses them. */
+int mh_band_factor_gated(double* abd, int* pivot, int* info, int n, int kl, int ku, int nbatch,
+                         const int* when, void* stream);
+int mh_band_solve_gated(const double* abd, const int* pivot, const double* fac_row,
+                        const double* fac_col, const int* lm2l, const int* rows, const double* rhs,
+                        double* sol, int L, int n, int kl, int ku, const int* when, void* stream);
+
 /* Sizes of the argument structs as compiled (binding checks). */
 int mh_struct_sizes(int* out, int n);
 

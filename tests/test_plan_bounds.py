@@ -456,6 +456,24 @@ def test_finite_difference_launches_stay_inside_their_tensors(cpu_standin, cfg):
     for A0, A1, out, cnt in (a[:4] for a in aff):
         for v in (A0, A1, out):
             _inside(ext, v, v + 8 * cnt, "affine operand")
+    nband = 0
+    for _, name, a in calls:
+        if name == "mh_band_factor_gated":
+            abd, piv, inf, n, kl, ku, nb = a[:7]
+            _inside(ext, abd, abd + 8 * nb * (2 * kl + ku + 1) * n, "band factor abd")
+            _inside(ext, piv, piv + 4 * nb * n, "band pivots")
+            _inside(ext, inf, inf + 4 * nb, "band info")
+            nband += 1
+        elif name == "mh_band_solve_gated":
+            abd, piv, fr, fc, lm2l, rows, rhs, sol, nr, n, kl, ku = a[:12]
+            _inside(ext, rhs, rhs + 16 * nr * n, "band rhs")
+            _inside(ext, sol, sol + 16 * nr * n, "band solution")
+            _inside(ext, lm2l, lm2l + 4 * nr, "band degrees")
+            nl = int(np.frombuffer(C.string_at(lm2l, 4 * nr), np.int32).max()) + 1
+            _inside(ext, abd, abd + 8 * nl * (2 * kl + ku + 1) * n, "band solve abd")
+            _inside(ext, piv, piv + 4 * nl * n, "band solve pivots")
+            nband += 1
+    assert nband == 2 + run.nsys + 1
     mt0 = run.MT["wp"][0].cpu().numpy()
     assert np.abs(mt0[N:2 * N, 4 * N:5 * N]).max() > 0 and np.abs(mt0[:N]).max() == 0
     stages = (1, 2, 3, 4) if run.dirk else (0,)

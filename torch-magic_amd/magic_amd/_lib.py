@@ -254,6 +254,8 @@ _SIGS = {
     "mh_fill_f64": [P, L, I, L, D, P, P],
     "mh_band_factor": [P, P, P, I, I, I, I, P],
     "mh_band_solve": [P, P, P, P, P, P, P, P, I, I, I, I, P],
+    "mh_band_factor_gated": [P, P, P, I, I, I, I, P, P],
+    "mh_band_solve_gated": [P, P, P, P, P, P, P, P, I, I, I, I, P, P],
     "mh_td_dc": [P, P],
     "mh_buoy_dc": [P, P],
     "mh_impl_dc": [P, P],

@@ -14,7 +14,9 @@
 #define AB(i, j) abd[(long)(i) * n + (j)]  // 0-based band row i, column j
 
 __global__ void k_band_factor(double* __restrict__ abd_all, int* __restrict__ piv_all,
-                              int* __restrict__ info_all, int n, int kl, int ku, int nbatch) {
+                              int* __restrict__ info_all, int n, int kl, int ku, int nbatch,
+                              const int* when) {
+  if (when && *when == 0) return;
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nbatch) return;
   const int nrow = 2 * kl + ku + 1;
@@ -78,7 +80,8 @@ __global__ void k_band_factor(double* __restrict__ abd_all, int* __restrict__ pi
 __global__ __launch_bounds__(64) void k_band_factor_lds(double* __restrict__ abd_all,
                                                         int* __restrict__ piv_all,
                                                         int* __restrict__ info_all, int n, int kl,
-                                                        int ku) {
+                                                        int ku, const int* when) {
+  if (when && *when == 0) return;
   extern __shared__ double sb[];  // [nrow][n]
   const int b = blockIdx.x, lane = threadIdx.x;
   const int nrow = 2 * kl + ku + 1;
@@ -148,18 +151,23 @@ __global__ __launch_bounds__(64) void k_band_factor_lds(double* __restrict__ abd
 #undef SB
 }
 
-extern "C" int mh_band_factor(double* abd, int* pivot, int* info, int n, int kl, int ku,
-                              int nbatch, void* stream) {
+extern "C" int mh_band_factor_gated(double* abd, int* pivot, int* info, int n, int kl, int ku,
+                                    int nbatch, const int* when, void* stream) {
   if (n < 1 || kl < 0 || ku < 0 || nbatch < 1) return MH_EINVAL;
   const size_t lds = (size_t)(2 * kl + ku + 1) * n * sizeof(double);
   if (lds <= 64 * 1024) {
     hipLaunchKernelGGL(k_band_factor_lds, dim3(nbatch), dim3(64), lds, mh_stream(stream), abd,
-                       pivot, info, n, kl, ku);
+                       pivot, info, n, kl, ku, when);
   } else {
     hipLaunchKernelGGL(k_band_factor, dim3(mh_cdiv(nbatch, 64)), dim3(64), 0, mh_stream(stream),
-                       abd, pivot, info, n, kl, ku, nbatch);
+                       abd, pivot, info, n, kl, ku, nbatch, when);
   }
   return mh_launch_status();
+}
+
+extern "C" int mh_band_factor(double* abd, int* pivot, int* info, int n, int kl, int ku,
+                              int nbatch, void* stream) {
+  return mh_band_factor_gated(abd, pivot, info, n, kl, ku, nbatch, nullptr, stream);
 }
 
 // dgbsl over LDS tiles of 16 columns.  The factors are real, so the real and
@@ -187,7 +195,8 @@ __global__ __launch_bounds__(64) void k_band_solve(const double* __restrict__ ab
                                                    const int* __restrict__ lm2l,
                                                    const int* __restrict__ rows,
                                                    const double2* rhs, double2* sol, int L,
-                                                   int n) {
+                                                   int n, const int* when) {
+  if (when && *when == 0) return;
   constexpr int M = KL + KU + 1;
   constexpr int NR = 2 * KL + KU + 1;
   constexpr int TP = kBT + 1;                    // padded LDS row (conflict-free columns)
@@ -414,10 +423,10 @@ __global__ __launch_bounds__(64) void k_band_solve(const double* __restrict__ ab
   }
 }
 
-extern "C" int mh_band_solve(const double* abd, const int* pivot, const double* fac_row,
-                             const double* fac_col, const int* lm2l, const int* rows,
-                             const double* rhs, double* sol, int L, int n, int kl, int ku,
-                             void* stream) {
+extern "C" int mh_band_solve_gated(const double* abd, const int* pivot, const double* fac_row,
+                                   const double* fac_col, const int* lm2l, const int* rows,
+                                   const double* rhs, double* sol, int L, int n, int kl, int ku,
+                                   const int* when, void* stream) {
   if (L < 1 || n < 1 || kl < 0 || ku < 0 || kl > 4 || ku > 4) return MH_EINVAL;
   const dim3 g(mh_cdiv(L, kBR)), blk(64);  // one wave of 32 rows (two lanes each) per workgroup
   hipStream_t st = mh_stream(stream);
@@ -426,7 +435,7 @@ extern "C" int mh_band_solve(const double* abd, const int* pivot, const double* 
 #define MH_BAND_CASE(A, B)                                                                      \
   if (kl == A && ku == B) {                                                                     \
     hipLaunchKernelGGL((k_band_solve<A, B>), g, blk, 0, st, abd, pivot, fac_row, fac_col, lm2l, \
-                       rows, r, o, L, n);                                                       \
+                       rows, r, o, L, n, when);                                                 \
     return mh_launch_status();                                                                  \
   }
 #define MH_BAND_ROW(A) MH_BAND_CASE(A, 0) MH_BAND_CASE(A, 1) MH_BAND_CASE(A, 2) MH_BAND_CASE(A, 3) MH_BAND_CASE(A, 4)
@@ -434,4 +443,12 @@ extern "C" int mh_band_solve(const double* abd, const int* pivot, const double* 
 #undef MH_BAND_ROW
 #undef MH_BAND_CASE
   return MH_EINVAL;
+}
+
+extern "C" int mh_band_solve(const double* abd, const int* pivot, const double* fac_row,
+                             const double* fac_col, const int* lm2l, const int* rows,
+                             const double* rhs, double* sol, int L, int n, int kl, int ku,
+                             void* stream) {
+  return mh_band_solve_gated(abd, pivot, fac_row, fac_col, lm2l, rows, rhs, sol, L, n, kl, ku,
+                             nullptr, stream);
 }

@@ -467,12 +467,22 @@ class DynamoRun:
         from . import fdstep
         p, rad, N, dev = self.p, self.res.rad, self.N, self.device
         lp1 = p.l_max + 1
-        ops = fdstep.affine_operators(p, rad, self.dc_prof)
+        ops = fdstep.band_operators(p, rad, self.dc_prof)
         up = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=F64).to(dev)
-        (a0, a1), (w0, w1) = ops["szbj"], ops["w"]
+        (a0, a1, ks), (w0, w1, kw) = ops["szbj"], ops["w"]
+        # banded operators (LINPACK storage): A0 + w A1, factorised in place, then
+        # solved against the identity for the dense inverses the folds need
         self.dc_ops = dict(a0=up(a0.reshape(-1)), a1=up(a1.reshape(-1)), w0=up(w0.reshape(-1)),
-                           w1=up(w1.reshape(-1)))
-        self.mat_w = self.mat_wp.view(-1)[:lp1 * N * N].view(lp1, N, N)
+                           w1=up(w1.reshape(-1)), ks=ks, kw=kw)
+        self.dc_abd = torch.empty(a0.size, dtype=F64, device=dev)
+        self.dc_abd_w = torch.empty(w0.size, dtype=F64, device=dev)
+        self.dc_piv = torch.zeros(self.nsys * lp1, N, dtype=I32, device=dev)
+        self.dc_piv_w = torch.zeros(lp1, N, dtype=I32, device=dev)
+        eye = torch.zeros(lp1 * N, N, dtype=C128, device=dev)
+        eye.view(lp1, N, N).diagonal(dim1=1, dim2=2).fill_(1.0)
+        self.dc_eye = eye
+        self.dc_x = torch.empty(lp1 * N, N, dtype=C128, device=dev)
+        self.dc_l = torch.as_tensor(np.repeat(np.arange(lp1), N), dtype=I32).to(dev)
         D1, D2 = rad["D1"], rad["D2"]
         self.Op4dc = up(np.concatenate([np.eye(N), D1, D2, D1 @ D2]))   # (4N, N)
         self.Op22 = up(D2 @ D2)
@@ -482,39 +492,50 @@ class DynamoRun:
         self.MT["wp"][0].copy_(up(mt0))
 
     def _rebuild_launches_dc(self, wimp_lin0, ctl=None):
-        """Finite-difference rebuild: A0 + w A1 for every system, batched
-        Gauss-Jordan inverses (row equilibration; the double-curl W also by
-        columns, as the reference preconditions it), and the folds with the
-        identity and D1, D2 (S, Z, B, J) and with I, D1, D2, D1 D2 and D2 D2 (W)
-        into the row operators; rows 1 and N-2 of the W input (its boundary
+        """Finite-difference rebuild: A0 + w A1 for every system in band storage,
+        the batched LINPACK band LU of all degrees (the reference's banded
+        path, ``update_s.py:147-163``, ``update_w_doublecurl.py:224-242``), band
+        solves against the identity for the dense inverses, and the folds with
+        the identity and D1, D2 (S, Z, B, J) and with I, D1, D2, D1 D2 and D2 D2
+        (W) into the row operators; rows 1 and N-2 of the W input (its boundary
         conditions on w') are dropped, as the reference zeroes those rhs rows."""
         N, lp1, ns = self.N, self.p.l_max + 1, self.nsys
         st = stream_handle()
         wdev, when = (None, None) if ctl is None else ctl
         info = lambda key: ptr(self.gj_info[_INFO_ROWS.index(key)])
-        o = self.dc_ops
-        _lib.call("mh_affine_f64", ptr(o["a0"]), ptr(o["a1"]), ptr(self.mat_nn),
-                  ns * lp1 * N * N, wimp_lin0, wdev, when, st)
-        _lib.call("mh_gj_inverse", ptr(self.mat_nn), ptr(self.mat_nn), ptr(self.gj_work),
-                  ptr(self.gj_scal), N, ns * lp1, 1, info("s"), when, st)
-        self._gemm(ptr(self.mat_nn), 1, N, N * N, ptr(self.Op3), 1, N, 0,
-                   ptr(self.MT4), self.Rpad3, 1, self.K4 * self.Rpad3, N, 3 * N, N, ns * lp1, when)
+        o, ks, kw = self.dc_ops, self.dc_ops["ks"], self.dc_ops["kw"]
+        X, nx = ptr(self.dc_x), lp1 * N
+        band = lambda abd, piv, k, nbatch, inf: _lib.call(
+            "mh_band_factor_gated", abd, piv, inf, N, k, k, nbatch, when, st)
+        inverse = lambda abd, piv, k: _lib.call(   # X rows l N + c: column c of inv_l
+            "mh_band_solve_gated", abd, piv, None, None, ptr(self.dc_l), None, ptr(self.dc_eye),
+            X, nx, N, k, k, when, st)
+        _lib.call("mh_affine_f64", ptr(o["a0"]), ptr(o["a1"]), ptr(self.dc_abd),
+                  self.dc_abd.numel(), wimp_lin0, wdev, when, st)
+        band(ptr(self.dc_abd), ptr(self.dc_piv), ks, ns * lp1, info("s"))
+        per = (3 * ks + 1) * N
         sb = lp1 * self.K4 * self.Rpad3
+        for sy in range(ns):
+            inverse(ptr(self.dc_abd) + 8 * sy * lp1 * per, ptr(self.dc_piv) + 4 * sy * lp1 * N, ks)
+            # MT[l][c][r] = sum_j Op3[r][j] inv_l[j][c], inv_l[j][c] = Re X[l N + c][j]
+            self._gemm(X, 2 * N, 2, 2 * N * N, ptr(self.Op3), 1, N, 0,
+                       ptr(self.MT4) + 8 * sy * sb, self.Rpad3, 1, self.K4 * self.Rpad3,
+                       N, 3 * N, N, lp1, when)
         self._zero(self.MT4[1:ns, 0], when, count=ns - 1, stride=sb)
         # W, degrees 1..l_max (degree 0 holds the fixed l = 0 pressure block)
-        nn, nb = N * N, lp1 - 1
-        mw = ptr(self.mat_w) + 8 * nn
-        _lib.call("mh_affine_f64", ptr(o["w0"]) + 8 * nn, ptr(o["w1"]) + 8 * nn, mw, nb * nn,
-                  wimp_lin0, wdev, when, st)
-        _lib.call("mh_gj_inverse", mw, mw, ptr(self.gj_work), ptr(self.gj_scal), N, nb, 2,
-                  info("wp") + 4, when, st)
+        nb = lp1 - 1
+        _lib.call("mh_affine_f64", ptr(o["w0"]), ptr(o["w1"]), ptr(self.dc_abd_w),
+                  self.dc_abd_w.numel(), wimp_lin0, wdev, when, st)
+        band(ptr(self.dc_abd_w), ptr(self.dc_piv_w), kw, lp1, info("wp"))
+        inverse(ptr(self.dc_abd_w), ptr(self.dc_piv_w), kw)
         MT = self.MT["wp"]
         sbw = self.K4wp * self.Rpad6
         mt1 = ptr(MT[1])
-        self._gemm(mw, 1, N, nn, ptr(self.Op4dc), 1, N, 0, mt1, self.Rpad6, 1, sbw,
+        x1 = X + 8 * 2 * N * N     # degree 1
+        self._gemm(x1, 2 * N, 2, 2 * N * N, ptr(self.Op4dc), 1, N, 0, mt1, self.Rpad6, 1, sbw,
                    N, 4 * N, N, nb, when)
-        self._gemm(mw, 1, N, nn, ptr(self.Op22), 1, N, 0, mt1 + 8 * 5 * N, self.Rpad6, 1, sbw,
-                   N, N, N, nb, when)
+        self._gemm(x1, 2 * N, 2, 2 * N * N, ptr(self.Op22), 1, N, 0, mt1 + 8 * 5 * N, self.Rpad6,
+                   1, sbw, N, N, N, nb, when)
         for k in (1, N - 2):
             self._zero(MT[1:, k], when, count=nb, stride=sbw)
 
@@ -718,15 +739,18 @@ class DynamoRun:
                 self.mat_z10 = torch.empty(N, N, dtype=F64, device=dev)
         self.MT_d1, self.MT_d1r2, self.MT_d12, self.MT_d123 = (
             ops[k] for k in ("MT_d1", "MT_d1r2", "MT_d12", "MT_d123"))
-        self.mat_nn = torch.empty(self.nsys * lp1, N, N, dtype=F64, device=dev)
-        self.mat_wp = torch.empty(lp1, 2 * N, 2 * N, dtype=F64, device=dev)
+        # dense collocation stacks of the Chebyshev rebuild (the finite-difference
+        # one factorises band storage instead)
+        nd = 1 if self.dc else lp1
+        self.mat_nn = torch.empty(self.nsys * nd, N, N, dtype=F64, device=dev)
+        self.mat_wp = torch.empty(nd, 2 * N, 2 * N, dtype=F64, device=dev)
         self.mat_p0 = torch.empty(N, N, dtype=F64, device=dev)
         # Gauss-Jordan work space: only the kernel for n > 208 uses it
         nwork = 0
         if 2 * N > 208:
-            nwork = lp1 * 2 * (2 * N) ** 2
+            nwork = nd * 2 * (2 * N) ** 2
         if N > 208:
-            nwork = max(nwork, self.nsys * lp1 * 2 * N * N)
+            nwork = max(nwork, self.nsys * nd * 2 * N * N)
         self.gj_work = torch.empty(max(nwork, 1), dtype=F64, device=dev)
         self.gj_scal = torch.empty(max(self.nsys * lp1, 2 * lp1) * 2 * N * 2, dtype=F64, device=dev)
         # first zero pivot per degree, one row per system (s, z, b, j, wp, p0)

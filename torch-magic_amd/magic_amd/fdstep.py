@@ -222,3 +222,30 @@ def conduction_state(p, rad, prof):
     rhs[N - 1] = math.sqrt(4.0 * math.pi)
     x = np.linalg.solve(fac[:, None] * M, fac * rhs)
     return x[:N], x[N:]
+
+
+def to_band(A: np.ndarray):
+    """(abd, k): LINPACK storage abd[..., k + k + i - j, j] = A[..., i, j] with
+    2k + k + 1 rows (the first k left for fill-in) of a batch of matrices whose
+    nonzeros lie within k diagonals of the main one, k the smallest such."""
+    n = A.shape[-1]
+    nz = np.abs(A.reshape(-1, n, n)).max(axis=0) != 0
+    i, j = np.nonzero(nz)
+    k = int(np.abs(i - j).max()) if i.size else 0
+    abd = np.zeros(A.shape[:-2] + (3 * k + 1, n))
+    for d in range(-k, k + 1):       # d = j - i
+        rows = np.arange(max(0, -d), min(n, n - d))
+        abd[..., 2 * k - d, rows + d] = A[..., rows, rows + d]
+    return abd, k
+
+
+def band_operators(p, rad, prof) -> dict:
+    """The affine operators of ``affine_operators`` in LINPACK band storage for
+    the device's banded factorisation: {"szbj": (B0, B1, k), "w": (W0, W1, k)}."""
+    ops = affine_operators(p, rad, prof)
+    out = {}
+    for key in ("szbj", "w"):
+        a0, a1 = ops[key]
+        b, k = to_band(np.stack([a0, a1]))
+        out[key] = (b[0], b[1], k)
+    return out
